@@ -1,0 +1,113 @@
+"""Synthetic guiding-sample generator for the benchmark and test workloads.
+
+Records follow PGLSampleData (40 bytes: pos f32x3, dir f32x3, weight, pdf,
+distance f32, flags u32).  Positions: 80% from Gaussian blobs (sigma 0.05)
+with centres in U[0.1, 0.9]^3, 20% uniform in the unit cube.  Each blob owns
+a 4-lobe vMF mixture (equal weights, mean directions uniform on the sphere,
+kappa log-uniform in [5, 500]); uniform-position records draw from the
+nearest blob's lobes.  weight ~ lognormal(0, 1), pdf = 1/(4 pi),
+distance ~ Exp(1).  All draws come from numpy's PCG64 with the given seed.
+"""
+import numpy as np
+
+SAMPLE_DTYPE = np.dtype([("pos", "<f4", 3), ("dir", "<f4", 3), ("weight", "<f4"),
+                         ("pdf", "<f4"), ("distance", "<f4"), ("flags", "<u4")])
+assert SAMPLE_DTYPE.itemsize == 40
+
+
+def _unit_vectors(rng, n):
+    v = rng.normal(size=(n, 3))
+    return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+
+def sample_vmf(rng, mu, kappa):
+    """Draw one direction per row from vMF(mu[i], kappa[i]) (stable inverse CDF)."""
+    n = mu.shape[0]
+    u = rng.random(n)
+    # cos(theta) = 1 + log(u + (1 - u) exp(-2 kappa)) / kappa
+    w = 1.0 + np.log(u + (1.0 - u) * np.exp(-2.0 * kappa)) / kappa
+    w = np.clip(w, -1.0, 1.0)
+    phi = 2.0 * np.pi * rng.random(n)
+    s = np.sqrt(np.maximum(0.0, 1.0 - w * w))
+    local = np.stack([s * np.cos(phi), s * np.sin(phi), w], axis=1)
+    # orthonormal frame around mu
+    a = np.where(np.abs(mu[:, 0:1]) > 0.9, np.array([[0.0, 1.0, 0.0]]), np.array([[1.0, 0.0, 0.0]]))
+    t = np.cross(mu, a)
+    t /= np.linalg.norm(t, axis=1, keepdims=True)
+    b = np.cross(mu, t)
+    d = local[:, 0:1] * t + local[:, 1:2] * b + local[:, 2:3] * mu
+    return d / np.linalg.norm(d, axis=1, keepdims=True)
+
+
+class SceneModel:
+    """Blob centres, drift directions and per-blob lobes of one synthetic scene."""
+
+    def __init__(self, seed=0, num_blobs=16, lobes=4):
+        rng = np.random.Generator(np.random.PCG64(seed))
+        self.centres = rng.uniform(0.1, 0.9, size=(num_blobs, 3))
+        self.drift = _unit_vectors(rng, num_blobs)
+        self.lobe_mu = _unit_vectors(rng, num_blobs * lobes).reshape(num_blobs, lobes, 3)
+        self.lobe_kappa = np.exp(rng.uniform(np.log(5.0), np.log(500.0), size=(num_blobs, lobes)))
+        self.num_blobs = num_blobs
+        self.lobes = lobes
+
+    def centres_at(self, frame, drift_per_frame=0.01):
+        return self.centres + drift_per_frame * frame * self.drift
+
+
+def generate(n, seed=0, frame=0, num_blobs=16, flags=0, scene_seed=None, blob_fraction=0.8):
+    """Return a structured array of n PGLSampleData records."""
+    scene = SceneModel(seed if scene_seed is None else scene_seed, num_blobs)
+    rng = np.random.Generator(np.random.PCG64([seed, frame, flags, 7919]))
+    centres = scene.centres_at(frame)
+    nb = int(round(n * blob_fraction))
+    blob = rng.integers(0, num_blobs, size=n)
+    pos = np.empty((n, 3))
+    pos[:nb] = centres[blob[:nb]] + rng.normal(scale=0.05, size=(nb, 3))
+    pos[nb:] = rng.random((n - nb, 3))
+    if n - nb > 0:
+        d2 = ((pos[nb:, None, :] - centres[None, :, :]) ** 2).sum(-1)
+        blob[nb:] = np.argmin(d2, axis=1)
+    lobe = rng.integers(0, scene.lobes, size=n)
+    mu = scene.lobe_mu[blob, lobe]
+    kappa = scene.lobe_kappa[blob, lobe]
+    d = sample_vmf(rng, mu, kappa)
+    out = np.zeros(n, dtype=SAMPLE_DTYPE)
+    out["pos"] = pos.astype(np.float32)
+    out["dir"] = d.astype(np.float32)
+    out["weight"] = np.exp(rng.normal(size=n)).astype(np.float32)
+    out["pdf"] = np.float32(1.0 / (4.0 * np.pi))
+    out["distance"] = rng.exponential(1.0, size=n).astype(np.float32)
+    out["flags"] = flags
+    perm = rng.permutation(n)
+    return out[perm]
+
+
+def region_samples(n, seed=0, lobes=4):
+    """Samples of one synthetic region: positions in a 0.05-sigma blob at the
+    origin, directions from a 4-lobe vMF mixture (kappa log-U[5, 500])."""
+    rng = np.random.Generator(np.random.PCG64([seed, 4242]))
+    mu = _unit_vectors(rng, lobes)
+    kappa = np.exp(rng.uniform(np.log(5.0), np.log(500.0), size=lobes))
+    lobe = rng.integers(0, lobes, size=n)
+    out = np.zeros(n, dtype=SAMPLE_DTYPE)
+    out["pos"] = rng.normal(scale=0.05, size=(n, 3)).astype(np.float32)
+    out["dir"] = sample_vmf(rng, mu[lobe], kappa[lobe]).astype(np.float32)
+    out["weight"] = np.exp(rng.normal(size=n)).astype(np.float32)
+    out["pdf"] = np.float32(1.0 / (4.0 * np.pi))
+    out["distance"] = rng.exponential(1.0, size=n).astype(np.float32)
+    return out
+
+
+def queries(n, seed=0, scene_seed=0, num_blobs=16, volume_fraction=0.3):
+    """Query batch: position near a blob, normal uniform on the sphere, 2
+    uniforms; a volume_fraction of them carry an HG mean cosine g in [-0.9, 0.9]."""
+    scene = SceneModel(scene_seed, num_blobs)
+    rng = np.random.Generator(np.random.PCG64([seed, 31337]))
+    blob = rng.integers(0, num_blobs, size=n)
+    pos = (scene.centres[blob] + rng.normal(scale=0.05, size=(n, 3))).astype(np.float32)
+    nrm = _unit_vectors(rng, n).astype(np.float32)
+    u2 = rng.random((n, 2)).astype(np.float32)
+    is_vol = rng.random(n) < volume_fraction
+    g = rng.uniform(-0.9, 0.9, size=n).astype(np.float32)
+    return pos, nrm, u2, is_vol, g

@@ -1,0 +1,241 @@
+// ORACLE — test infrastructure only.
+//
+// Harness that compiles the reference's own directional code (Open PGL 0.6.0
+// headers under the reference tree, used where they lie) into
+// oracle/_ref/libpglref.so, so tests can produce golden vectors and pin the
+// restatement in pgl_oracle.cpp.  The reference's kd-tree builder, KNN and
+// Field need oneTBB headers, which this image does not have: those parts are
+// not built here.
+//
+// Mirrors, for a single region, the per-region body of Field::fitRegions
+// (field/Field.h:389-439) and Field::updateRegions (field/Field.h:441-511).
+#include <sstream>
+#include <vector>
+#include <algorithm>
+#include <cstring>
+#include <cstdlib>
+#include <new>
+
+#include "data/SampleStatistics.h"
+#include "directional/vmm/ParallaxAwareVonMisesFisherMixture.h"
+#include "directional/vmm/ParallaxAwareVonMisesFisherWeightedEMFactory.h"
+#include "directional/vmm/AdaptiveSplitandMergeFactory.h"
+#include "directional/vmm/VMMSurfaceSamplingDistribution.h"
+#include "directional/vmm/VMMVolumeSamplingDistribution.h"
+
+#include "pgl_oracle.h"
+
+using namespace openpgl;
+
+typedef ParallaxAwareVonMisesFisherMixture<8, 32, true> VMM8;
+typedef AdaptiveSplitAndMergeFactory<VMM8> ASM8;
+
+struct RefRegion {
+  VMM8 vmm;
+  ASM8::Statistics stats;
+  SampleStatistics ss;
+  ASM8::Configuration cfg;
+  bool first = true;
+  bool valid = false;
+  ASM8::FittingStatistics fit;
+  RefRegion() : vmm(), stats(), ss() {}
+};
+
+static void toCfg(const orc_config* c, ASM8::Configuration& a) {
+  a.weightedEMCfg.initK = c->initK;
+  a.weightedEMCfg.initKappa = c->initKappa;
+  a.weightedEMCfg.maxK = c->maxK;
+  a.weightedEMCfg.maxEMIterrations = c->maxEMIterations;
+  a.weightedEMCfg.maxKappa = c->maxKappa;
+  a.weightedEMCfg.maxMeanCosine = KappaToMeanCosine<float>(c->maxKappa);
+  a.weightedEMCfg.convergenceThreshold = c->convergenceThreshold;
+  a.weightedEMCfg.weightPrior = c->weightPrior;
+  a.weightedEMCfg.meanCosinePriorStrength = c->meanCosinePriorStrength;
+  a.weightedEMCfg.meanCosinePrior = c->meanCosinePrior;
+  a.splittingThreshold = c->splittingThreshold;
+  a.mergingThreshold = c->mergingThreshold;
+  a.partialReFit = c->partialReFit;
+  a.maxSplitItr = 1;
+  a.useSplitAndMerge = c->useSplitAndMerge;
+  a.minSamplesForSplitting = c->minSamplesForSplitting;
+  a.minSamplesForPartialRefitting = c->minSamplesForPartialRefitting;
+  a.minSamplesForMerging = c->minSamplesForMerging;
+}
+
+extern "C" {
+
+void* ref_region_new(const orc_config* c) {
+  static bool hgInit = false;
+  if (!hgInit) { VMMSingleLobeHenyeyGreensteinOracle::init(); hgInit = true; }
+  void* mem = nullptr;
+  if (posix_memalign(&mem, 64, sizeof(RefRegion)) != 0) return nullptr;
+  RefRegion* r = new (mem) RefRegion();  // vfloat<8> members need 32-byte alignment
+  toCfg(c, r->cfg);
+  return r;
+}
+void ref_region_release(void* p) {
+  ((RefRegion*)p)->~RefRegion();
+  free(p);
+}
+
+void ref_region_step(void* p, const orc_sample* s, size_t n, const float* m, const float* v, float ns, int splitFlag) {
+  RefRegion* r = (RefRegion*)p;
+  r->ss.mean = Point3(m[0], m[1], m[2]);
+  r->ss.sampleVariance = Vector3(v[0], v[1], v[2]);
+  r->ss.numSamples = ns;
+  std::vector<SampleData> data(n);
+  memcpy(data.data(), s, n * sizeof(SampleData));
+  std::sort(data.begin(), data.end(), SampleDataLess);
+  ASM8 factory;
+  r->fit = ASM8::FittingStatistics();
+  const Point3 sampleMean = r->ss.mean;
+  if (r->first) {
+    factory.prepareSamples(data.data(), n, r->ss, r->cfg);
+    factory.fit(r->vmm, r->stats, data.data(), n, r->cfg, r->fit);
+    r->vmm._pivotPosition = sampleMean;
+    r->valid = r->vmm.isValid();
+    r->first = false;
+  } else {
+    if (splitFlag) r->stats.decay(0.25f);
+    r->stats.sufficientStatistics.applyParallaxShift(r->vmm, r->vmm._pivotPosition - sampleMean);
+    r->vmm.performRelativeParallaxShift(r->vmm._pivotPosition - sampleMean);
+    factory.prepareSamples(data.data(), n, r->ss, r->cfg);
+    factory.update(r->vmm, r->stats, data.data(), n, r->cfg, r->fit);
+    r->valid = r->vmm.isValid() && r->stats.isValid();
+  }
+}
+
+void ref_region_export(void* p, orc_region_record* o) {
+  RefRegion* r = (RefRegion*)p;
+  memset(o, 0, sizeof(*o));
+  const VMM8& v = r->vmm;
+  o->K = v._numComponents;
+  const auto& su = r->stats.sufficientStatistics;
+  const auto& sp = r->stats.splittingStatistics;
+  for (int k = 0; k < 32; k++) {
+    const int q = k / 8, l = k % 8;
+    o->weights[k] = v._weights[q][l];
+    o->kappas[k] = v._kappas[q][l];
+    o->meanDirections[k][0] = v._meanDirections[q].x[l];
+    o->meanDirections[k][1] = v._meanDirections[q].y[l];
+    o->meanDirections[k][2] = v._meanDirections[q].z[l];
+    o->normalizations[k] = v._normalizations[q][l];
+    o->eMinus2Kappa[k] = v._eMinus2Kappa[q][l];
+    o->meanCosines[k] = v._meanCosines[q][l];
+    o->distances[k] = v._distances[q][l];
+    o->sumOfWeightedDirections[k][0] = su.sumOfWeightedDirections[q].x[l];
+    o->sumOfWeightedDirections[k][1] = su.sumOfWeightedDirections[q].y[l];
+    o->sumOfWeightedDirections[k][2] = su.sumOfWeightedDirections[q].z[l];
+    o->sumOfWeightedStats[k] = su.sumOfWeightedStats[q][l];
+    o->sumOfDistanceWeights[k] = su.sumOfDistanceWeightes[q][l];
+    o->chiSquareMCEstimates[k] = sp.chiSquareMCEstimates[q][l];
+    o->splitMeans[k][0] = sp.splitMeans[q].x[l];
+    o->splitMeans[k][1] = sp.splitMeans[q].y[l];
+    o->splitCovariances[k][0] = sp.splitWeightedSampleCovariances[q].x[l];
+    o->splitCovariances[k][1] = sp.splitWeightedSampleCovariances[q].y[l];
+    o->splitCovariances[k][2] = sp.splitWeightedSampleCovariances[q].z[l];
+    o->splitNumSamples[k] = sp.numSamples[q][l];
+    o->splitSumWeights[k] = sp.sumWeights[q][l];
+    o->splitSumAssignedSamples[k] = sp.sumAssignedSamples[q][l];
+  }
+  o->pivot[0] = v._pivotPosition.x; o->pivot[1] = v._pivotPosition.y; o->pivot[2] = v._pivotPosition.z;
+  o->sumWeights = su.sumWeights; o->numSamples = su.numSamples; o->overallNumSamples = su.overallNumSamples;
+  o->suffK = su.numComponents; o->normalized = su.normalized;
+  o->splitK = sp.numComponents;
+  o->numSamplesAfterLastSplit = r->stats.numSamplesAfterLastSplit;
+  o->numSamplesAfterLastMerge = r->stats.numSamplesAfterLastMerge;
+  for (int j = 0; j < 3; j++) {
+    o->ssMean[j] = r->ss.mean[j]; o->ssVariance[j] = r->ss.sampleVariance[j];
+  }
+  o->ssNumSamples = r->ss.numSamples;
+  o->valid = r->valid;
+  o->numSplits = r->fit.numSplits;
+  o->numMerges = r->fit.numMerges;
+  o->emIterations = r->fit.numUpdateWEMIterations;
+  o->partialEmIterations = r->fit.numPartialUpdateWEMIterations;
+}
+
+float ref_region_pdf(void* p, const float* d) {
+  RefRegion* r = (RefRegion*)p;
+  return r->vmm.pdf(Vector3(d[0], d[1], d[2]));
+}
+
+// init at pos (parallax shift) + cosine (surface) or HG (volume) product, then
+// pdf(dir) and sample(u) -> dirOut/pdfOut.  Returns the product integral.
+float ref_region_query(void* p, int volume, const float* pos, const float* nrm, float g, const float* u2, const float* pdfDir,
+                       float* pdfAtDir, float* dirOut, float* pdfOut) {
+  RefRegion* r = (RefRegion*)p;
+  float integral = 1.f;
+  if (volume) {
+    VMMVolumeSamplingDistribution<VMM8, true> d;
+    d.init(&r->vmm, Point3(pos[0], pos[1], pos[2]));
+    d.applySingleLobeHenyeyGreensteinProduct(Vector3(nrm[0], nrm[1], nrm[2]), g);
+    integral = d.m_productIntegral;
+    *pdfAtDir = d.pdf(Vector3(pdfDir[0], pdfDir[1], pdfDir[2]));
+    Vector3 o = d.sample(Point2(u2[0], u2[1]));
+    dirOut[0] = o.x; dirOut[1] = o.y; dirOut[2] = o.z;
+    *pdfOut = d.pdf(o);
+  } else {
+    VMMSurfaceSamplingDistribution<VMM8, true> d;
+    d.init(&r->vmm, Point3(pos[0], pos[1], pos[2]));
+    d.applyCosineProduct(Vector3(nrm[0], nrm[1], nrm[2]));
+    integral = d.m_productIntegral;
+    *pdfAtDir = d.pdf(Vector3(pdfDir[0], pdfDir[1], pdfDir[2]));
+    Vector3 o = d.sample(Point2(u2[0], u2[1]));
+    dirOut[0] = o.x; dirOut[1] = o.y; dirOut[2] = o.z;
+    *pdfOut = d.pdf(o);
+  }
+  return integral;
+}
+
+void ref_integer_stats(const float* pos, size_t n, const float* lo, const float* hi, float* out, int64_t* raw) {
+  BBox b(Vector3(lo[0], lo[1], lo[2]), Vector3(hi[0], hi[1], hi[2]));
+  IntegerSampleStatistics is(b);
+  for (size_t i = 0; i < n; i++) is.addSample(Point3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]));
+  SampleStatistics s = is.getSampleStatistics();
+  for (int j = 0; j < 3; j++) {
+    out[j] = s.mean[j]; out[3 + j] = s.sampleVariance[j];
+    out[7 + j] = s.sampleBounds.lower[j]; out[10 + j] = s.sampleBounds.upper[j];
+    raw[j] = is.mean[j]; raw[3 + j] = is.variance[j];
+    raw[7 + j] = is.sampleBounds.lower[j]; raw[10 + j] = is.sampleBounds.upper[j];
+  }
+  out[6] = s.numSamples;
+  raw[6] = is.numSamples;
+}
+
+// SampleStatistics::merge / split exercised on given inputs (KDTreeBuilder.h:254-263)
+void ref_stats_merge_split(const float* a13, const float* b13, int dim, float pos, float* merged13, float* left13, float* right13) {
+  auto load = [](const float* x) {
+    SampleStatistics s;
+    s.mean = Point3(x[0], x[1], x[2]); s.sampleVariance = Vector3(x[3], x[4], x[5]); s.numSamples = x[6];
+    s.sampleBounds.lower = Vector3(x[7], x[8], x[9]); s.sampleBounds.upper = Vector3(x[10], x[11], x[12]);
+    return s;
+  };
+  auto store = [](const SampleStatistics& s, float* x) {
+    for (int j = 0; j < 3; j++) { x[j] = s.mean[j]; x[3 + j] = s.sampleVariance[j]; x[7 + j] = s.sampleBounds.lower[j]; x[10 + j] = s.sampleBounds.upper[j]; }
+    x[6] = s.numSamples;
+  };
+  SampleStatistics a = load(a13), b = load(b13);
+  SampleStatistics m = a;
+  m.merge(b);
+  store(m, merged13);
+  SampleStatistics l = a, r = a;
+  l.split(dim, pos, 0.25f, false);
+  r.split(dim, pos, 0.25f, true);
+  store(l, left13);
+  store(r, right13);
+}
+
+float ref_rcp(float x) { return embree::rcp(x); }
+
+void ref_hg_table(float* t) {
+  VMMSingleLobeHenyeyGreensteinOracle::init();
+  for (int i = 0; i < 128; i++) {
+    const VMMPhaseFunctionRepresentation& rep = VMMSingleLobeHenyeyGreensteinOracle::representations[i];
+    float* e = t + 18 * i;
+    e[0] = float(rep.K); e[1] = rep.g;
+    for (int k = 0; k < 4; k++) { e[2 + k] = rep.weights[k]; e[6 + k] = rep.meanCosines[k]; e[10 + k] = rep.kappas[k]; e[14 + k] = rep.normalizations[k]; }
+  }
+}
+
+}  // extern "C"
