@@ -1,0 +1,276 @@
+/*
+ * Open PGL-compatible C ABI of the MI355X guiding-field library
+ * (libopenpgl_amd.so).  Every function keeps the name, argument meaning and
+ * error behaviour of the Open PGL 0.6.0 C API it replaces; each declaration
+ * cites the reference header line (paths relative to the reference tree,
+ * openpgl/include/openpgl/).  Functions marked EXTENSION have no counterpart
+ * in the reference.
+ *
+ * As in the reference, a few signatures take C++ references; from C they are
+ * the same pointer at the ABI level.
+ */
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+#include <stdbool.h>
+
+#ifdef __cplusplus
+#define PGL_REF(T) T&
+extern "C" {
+#else
+#define PGL_REF(T) T*
+#endif
+
+#define PGL_VMM_MAX_COMPONENTS 32      /* common.h:26 */
+#define PGL_VMM_MAX_KAPPA 320000.f     /* common.h:27 */
+#define PGL_TREE_MAX_SAMPLE_PER_LEAF 32000 /* config.h:25 */
+
+/* common.h:29-47 */
+typedef struct { float x, y, z; } pgl_vec3f;
+typedef pgl_vec3f pgl_point3f;
+typedef struct { float x, y; } pgl_vec2f;
+typedef pgl_vec2f pgl_point2f;
+typedef struct { pgl_vec3f lower, upper; } pgl_box3f;
+
+/* data.h:21-49: 40-byte training record; flags bit 1 (EInsideVolume) routes to the volume field */
+typedef struct PGLSampleData {
+  pgl_point3f position;
+  pgl_vec3f direction;
+  float weight;
+  float pdf;
+  float distance;
+  uint32_t flags;
+} PGLSampleData;
+#define PGL_SAMPLE_FLAG_SPLATTED 1u
+#define PGL_SAMPLE_FLAG_INSIDE_VOLUME 2u
+
+/* data.h:106-109 */
+typedef struct PGLString { char* m_str; size_t m_size; } PGLString;
+
+/* types.h */
+enum PGL_SPATIAL_STRUCTURE_TYPE { PGL_SPATIAL_STRUCTURE_KDTREE = 0 };
+enum PGL_DIRECTIONAL_DISTRIBUTION_TYPE {
+  PGL_DIRECTIONAL_DISTRIBUTION_PARALLAX_AWARE_VMM = 0,
+  PGL_DIRECTIONAL_DISTRIBUTION_QUADTREE,
+  PGL_DIRECTIONAL_DISTRIBUTION_VMM
+};
+
+/* config.h:25-31.  The CPU_* types select the reference device whose SIMD
+ * width (4/8/16) defines the merge floor; GPU_MI355X behaves as CPU_8.
+ * EXTENSION: PGL_DEVICE_TYPE_GPU_MI355X is appended without renumbering. */
+enum PGL_DEVICE_TYPE {
+  PGL_DEVICE_TYPE_CPU_4,
+  PGL_DEVICE_TYPE_CPU_8,
+  PGL_DEVICE_TYPE_CPU_16,
+  PGL_DEVICE_TYPE_NONE,
+  PGL_DEVICE_TYPE_GPU_MI355X
+};
+
+/* config.h:33-39 */
+typedef struct PGLKDTreeArguments {
+  bool knnLookup;
+  size_t minSamples;
+  size_t maxSamples;
+  size_t maxDepth;
+} PGLKDTreeArguments;
+
+/* config.h:41-81 */
+typedef struct PGLVMMFactoryArguments {
+  size_t initK;
+  float initKappa;
+  size_t maxK;
+  size_t maxEMIterrations;
+  float maxKappa;
+  float convergenceThreshold;
+  float weightPrior;
+  float meanCosinePriorStrength;
+  float meanCosinePrior;
+  bool useSplitAndMerge;
+  float splittingThreshold;
+  float mergingThreshold;
+  bool partialReFit;
+  int maxSplitItr;
+  int minSamplesForSplitting;
+  int minSamplesForPartialRefitting;
+  int minSamplesForMerging;
+} PGLVMMFactoryArguments;
+
+/* config.h:99-118 */
+typedef struct PGLDebugArguments { bool fitRegions; } PGLDebugArguments;
+typedef struct PGLFieldArguments {
+  enum PGL_SPATIAL_STRUCTURE_TYPE spatialStructureType;
+  void* spatialSturctureArguments;
+  enum PGL_DIRECTIONAL_DISTRIBUTION_TYPE directionalDistributionType;
+  void* directionalDistributionArguments;
+  bool deterministic;
+  PGLDebugArguments debugArguments;
+} PGLFieldArguments;
+
+typedef struct PGLDeviceImpl* PGLDevice;
+typedef struct PGLFieldImpl* PGLField;
+typedef struct PGLSampleStorageImpl* PGLSampleStorage;
+typedef struct PGLSurfaceSamplingDistributionImpl* PGLSurfaceSamplingDistribution;
+typedef struct PGLVolumeSamplingDistributionImpl* PGLVolumeSamplingDistribution;
+typedef struct PGLRegionImpl* PGLRegion;
+typedef struct PGLFieldStatisticsImpl* PGLFieldStatistics;
+
+/* config.h:120, api.cpp:755-787: heap-allocates the two argument structs;
+ * pglDeviceNewField takes ownership and deletes them (Device.h:109,136) */
+void pglFieldArgumentsSetDefaults(PGL_REF(PGLFieldArguments) fieldArguments,
+                                  enum PGL_SPATIAL_STRUCTURE_TYPE spatialType,
+                                  enum PGL_DIRECTIONAL_DISTRIBUTION_TYPE directionalType, bool deterministic,
+                                  size_t maxSamplesPerLeaf);
+void pglReleaseString(PGLString str);                                            /* data.h:115 */
+
+/* device.h:22-31 */
+PGLDevice pglNewDevice(enum PGL_DEVICE_TYPE deviceType, size_t numThreads);
+PGLField pglDeviceNewField(PGLDevice device, PGLFieldArguments args);
+PGLField pglDeviceNewFieldFromFile(PGLDevice device, const char* fieldFileName);
+void pglReleaseDevice(PGLDevice device);
+
+/* field.h:36-70 */
+void pglReleaseField(PGLField field);
+bool pglFieldStoreToFile(PGLField field, const char* fieldFileName);
+size_t pglFieldGetIteration(PGLField field);
+void pglFieldSetSceneBounds(PGLField field, pgl_box3f bounds);
+pgl_box3f pglFieldGetSceneBounds(PGLField field);
+void pglFieldUpdate(PGLField field, PGLSampleStorage sampleStorage);
+void pglFieldUpdateSurface(PGLField field, PGLSampleStorage sampleStorage);
+void pglFieldUpdateVolume(PGLField field, PGLSampleStorage sampleStorage);
+void pglFieldReset(PGLField field);
+PGLSurfaceSamplingDistribution pglFieldNewSurfaceSamplingDistribution(PGLField field);
+bool pglFieldInitSurfaceSamplingDistribution(PGLField field, PGLSurfaceSamplingDistribution surfaceSamplingDistribution,
+                                             pgl_point3f position, float* sample1D);
+PGLVolumeSamplingDistribution pglFieldNewVolumeSamplingDistribution(PGLField field);
+bool pglFieldInitVolumeSamplingDistribution(PGLField field, PGLVolumeSamplingDistribution volumeSamplingDistribution,
+                                            pgl_point3f position, float* sample1D);
+bool pglFieldValidate(PGLField field);
+bool pglFieldCompare(PGLField fieldA, PGLField fieldB);
+PGLFieldStatistics pglFieldGetSurfaceStatistics(PGLField field);
+PGLFieldStatistics pglFieldGetVolumeStatistics(PGLField field);
+
+/* fieldstatistics.h:21-27 */
+void pglReleaseFieldStatistics(PGLFieldStatistics fieldStatistics);
+PGLString pglFieldStatisticsToString(PGLFieldStatistics fieldStatistics);
+PGLString pglFieldStatisticsHeaderCSVString(PGLFieldStatistics fieldStatistics);
+PGLString pglFieldStatisticsToCSVString(PGLFieldStatistics fieldStatistics);
+
+/* samplestorage.h:35-65 */
+PGLSampleStorage pglNewSampleStorage(void);
+PGLSampleStorage pglNewSampleStorageFromFile(const char* sampleStorageFileName);
+void pglReleaseSampleStorage(PGLSampleStorage sampleStorage);
+bool pglSampleStorageStoreToFile(PGLSampleStorage sampleStorage, const char* sampleStorageFileName);
+void pglSampleStorageAddSample(PGLSampleStorage sampleStorage, PGL_REF(PGLSampleData) sample);
+void pglSampleStorageAddSamples(PGLSampleStorage sampleStorage, const PGLSampleData* samples, size_t numSamples);
+void pglSampleStorageReserve(PGLSampleStorage sampleStorage, const size_t sizeSurface, const size_t sizeVolume);
+void pglSampleStorageClear(PGLSampleStorage sampleStorage);
+void pglSampleStorageClearSurface(PGLSampleStorage sampleStorage);
+void pglSampleStorageClearVolume(PGLSampleStorage sampleStorage);
+size_t pglSampleStorageGetSizeSurface(PGLSampleStorage sampleStorage);
+size_t pglSampleStorageGetSizeVolume(PGLSampleStorage sampleStorage);
+PGLSampleData pglSampleStorageGetSampleSurface(PGLSampleStorage sampleStorage, const int idx);
+PGLSampleData pglSampleStorageGetSampleVolume(PGLSampleStorage sampleStorage, const int idx);
+bool pglSampleStorageValidate(PGLSampleStorage sampleStorage);
+bool pglSampleStorageCompare(PGLSampleStorage sampleStorageA, PGLSampleStorage sampleStorageB);
+
+/* surfacesamplingdistribution.h:24-46 */
+void pglReleaseSurfaceSamplingDistribution(PGLSurfaceSamplingDistribution surfaceSamplingDistribution);
+void pglSurfaceSamplingDistributionApplyCosineProduct(PGLSurfaceSamplingDistribution surfaceSamplingDistribution,
+                                                      pgl_vec3f normal);
+bool pglSurfaceSamplingDistributionSupportsApplyCosineProduct(PGLSurfaceSamplingDistribution surfaceSamplingDistribution);
+pgl_vec3f pglSurfaceSamplingDistributionSample(PGLSurfaceSamplingDistribution surfaceSamplingDistribution,
+                                               pgl_point2f sample);
+float pglSurfaceSamplingDistributionPDF(PGLSurfaceSamplingDistribution surfaceSamplingDistribution, pgl_vec3f direction);
+float pglSurfaceSamplingDistributionSamplePDF(PGLSurfaceSamplingDistribution surfaceSamplingDistribution,
+                                              pgl_point2f sample, PGL_REF(pgl_vec3f) direction);
+float pglSurfaceSamplingDistributionIncomingRadiancePDF(PGLSurfaceSamplingDistribution surfaceSamplingDistribution,
+                                                        pgl_vec3f direction);
+uint32_t pglSurfaceSamplingDistributionGetId(PGLSurfaceSamplingDistribution surfaceSamplingDistribution);
+bool pglSurfaceSamplingDistributionValidate(PGLSurfaceSamplingDistribution surfaceSamplingDistribution);
+void pglSurfaceSamplingDistributionClear(PGLSurfaceSamplingDistribution surfaceSamplingDistribution);
+PGLRegion pglSurfaceSamplingGetRegion(PGLSurfaceSamplingDistribution surfaceSamplingDistribution);
+
+/* volumesamplingdistribution.h:24-46 */
+void pglReleaseVolumeSamplingDistribution(PGLVolumeSamplingDistribution volumeSamplingDistribution);
+pgl_vec3f pglVolumeSamplingDistributionSample(PGLVolumeSamplingDistribution volumeSamplingDistribution,
+                                              pgl_point2f sample);
+float pglVolumeSamplingDistributionPDF(PGLVolumeSamplingDistribution volumeSamplingDistribution, pgl_vec3f direction);
+float pglVolumeSamplingDistributionSamplePDF(PGLVolumeSamplingDistribution volumeSamplingDistribution,
+                                             pgl_point2f sample, PGL_REF(pgl_vec3f) direction);
+float pglVolumeSamplingDistributionIncomingRadiancePDF(PGLVolumeSamplingDistribution volumeSamplingDistribution,
+                                                       pgl_vec3f direction);
+uint32_t pglVolumeSamplingDistributionGetId(PGLVolumeSamplingDistribution volumeSamplingDistribution);
+bool pglVolumeSamplingDistributionValidate(PGLVolumeSamplingDistribution volumeSamplingDistribution);
+void pglVolumeSamplingDistributionClear(PGLVolumeSamplingDistribution volumeSamplingDistribution);
+void pglVolumeSamplingDistributionApplySingleLobeHenyeyGreensteinProduct(
+    PGLVolumeSamplingDistribution volumeSamplingDistribution, pgl_vec3f dir, const float meanCosine);
+bool pglVolumeSamplingDistributionSupportsApplySingleLobeHenyeyGreensteinProduct(
+    PGLVolumeSamplingDistribution volumeSamplingDistribution);
+PGLRegion pglVolumeSamplingGetRegion(PGLVolumeSamplingDistribution volumeSamplingDistribution);
+
+/* ------------------------------------------------------------------------
+ * EXTENSIONS (no counterpart in the reference)
+ * ---------------------------------------------------------------------- */
+
+/* EXTENSION: Field::Update of one field kind from records already resident in
+ * device memory (same semantics as pglFieldUpdateSurface/Volume; no flag
+ * routing).  Used by GPU-side producers and by the benchmark. */
+void pglFieldUpdateSurfaceDevice(PGLField field, const PGLSampleData* deviceSamples, size_t numSamples);
+void pglFieldUpdateVolumeDevice(PGLField field, const PGLSampleData* deviceSamples, size_t numSamples);
+
+/* EXTENSION: batched Init + product + Sample + PDF for n queries on the GPU.
+ * All pointers are device pointers: pos/normalOrDir (3n floats), g (n, HG mean
+ * cosine, volume queries only; may be NULL), u2 (2n), u1 (n; < 0 disables the
+ * KNN pick; may be NULL = all disabled), isVolume (n bytes; NULL = all
+ * surface).  Outputs dirOut (3n), pdfOut (n; 0 for a failed Init), idOut (n,
+ * kd leaf id or 0xffffffff; may be NULL), integralOut (n, product integral; may
+ * be NULL).  stream = hipStream_t (NULL = default stream); asynchronous. */
+void pglFieldSampleBatchDevice(PGLField field, size_t n, const float* pos, const float* normalOrDir, const float* g,
+                               const float* u2, const float* u1, const uint8_t* isVolume, float* dirOut,
+                               float* pdfOut, uint32_t* idOut, float* integralOut, void* stream);
+/* EXTENSION: host-pointer variant of pglFieldSampleBatchDevice (synchronous). */
+void pglFieldSampleBatch(PGLField field, size_t n, const float* pos, const float* normalOrDir, const float* g,
+                         const float* u2, const float* u1, const uint8_t* isVolume, float* dirOut, float* pdfOut,
+                         uint32_t* idOut, float* integralOut);
+
+/* EXTENSION: flat region record used to compare fields against a checker
+ * (canonical depth-first region order of KDTree::rearrangeNodesForCompare). */
+typedef struct PGLRegionRecord {
+  uint32_t K;
+  float weights[32], kappas[32], meanDirections[32][3], normalizations[32], eMinus2Kappa[32], meanCosines[32],
+      distances[32];
+  float pivot[3];
+  float sumOfWeightedDirections[32][3], sumOfWeightedStats[32], sumOfDistanceWeights[32];
+  float sumWeights, numSamples, overallNumSamples;
+  uint32_t suffK, normalized;
+  float chiSquareMCEstimates[32], splitMeans[32][2], splitCovariances[32][3], splitNumSamples[32],
+      splitSumWeights[32], splitSumAssignedSamples[32];
+  uint32_t splitK;
+  uint64_t numSamplesAfterLastSplit, numSamplesAfterLastMerge;
+  float ssMean[3], ssVariance[3], ssNumSamples, ssLower[3], ssUpper[3];
+  float regionLower[3], regionUpper[3];
+  uint32_t splitFlag, valid;
+  uint32_t numSamplesThisUpdate;
+  uint32_t emIterations, partialEmIterations, numSplits, numMerges;
+} PGLRegionRecord;
+
+/* EXTENSION: introspection (kind: 0 = surface field, 1 = volume field) */
+uint32_t pglFieldGetNumNodes(PGLField field, int kind);
+uint32_t pglFieldGetNumRegions(PGLField field, int kind);
+void pglFieldExportNodes(PGLField field, int kind, float* pivots, uint32_t* dims, uint32_t* childOrRegion);
+void pglFieldExportRegion(PGLField field, int kind, uint32_t canonicalIdx, PGLRegionRecord* out);
+void pglFieldSampleLeafIds(PGLField field, int kind, const float* positions, size_t n, uint32_t* canonicalIds);
+void pglFieldGetKNN(PGLField field, int kind, uint32_t* neighbours /* 8 per region, canonical ids */);
+/* EXTENSION: timings of the last update of one kind, milliseconds:
+ * [0] total wall, [1] spatial (GPU), [2] directional region kernel (GPU),
+ * [3] KNN (GPU), [4] gather+reproject (GPU), [5] sort (GPU), [6] kd rounds,
+ * [7] region kernel launches, [8] E-step sample x component evaluations,
+ * [9] split/distance-pass evaluations, [10] EM iterations (sum), [11] samples */
+void pglFieldGetLastUpdateInfo(PGLField field, int kind, double* out12);
+/* EXTENSION: directional scheduling: 0 = one launch for all regions
+ * (default), 1 = one launch per region (the plain per-region path). */
+void pglFieldSetRegionSchedule(PGLField field, int schedule);
+
+#ifdef __cplusplus
+}  /* extern "C" */
+#endif

@@ -1,0 +1,381 @@
+"""Python bindings of libopenpgl_amd.so, mirroring Open PGL's C++ wrapper
+classes (openpgl/include/openpgl/cpp/*.h: Device, Field, SampleStorage,
+SurfaceSamplingDistribution, VolumeSamplingDistribution) over the C ABI.
+
+The library is the MI355X implementation; importing this module never falls
+back to a CPU path: if the shared library is missing, loading raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .synth import SAMPLE_DTYPE
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libopenpgl_amd.so")
+
+PGL_DEVICE_TYPE_CPU_4, PGL_DEVICE_TYPE_CPU_8, PGL_DEVICE_TYPE_CPU_16, PGL_DEVICE_TYPE_NONE, PGL_DEVICE_TYPE_GPU_MI355X = range(5)
+PGL_SPATIAL_STRUCTURE_KDTREE = 0
+PGL_DIRECTIONAL_DISTRIBUTION_PARALLAX_AWARE_VMM = 0
+
+_vp = ctypes.c_void_p
+_f = ctypes.c_float
+_u32 = ctypes.c_uint32
+
+
+class Vec3(ctypes.Structure):
+    _fields_ = [("x", _f), ("y", _f), ("z", _f)]
+
+
+class Vec2(ctypes.Structure):
+    _fields_ = [("x", _f), ("y", _f)]
+
+
+class Box3(ctypes.Structure):
+    _fields_ = [("lower", Vec3), ("upper", Vec3)]
+
+
+class PGLSampleData(ctypes.Structure):
+    _fields_ = [("position", Vec3), ("direction", Vec3), ("weight", _f), ("pdf", _f), ("distance", _f),
+                ("flags", _u32)]
+
+
+class PGLDebugArguments(ctypes.Structure):
+    _fields_ = [("fitRegions", ctypes.c_bool)]
+
+
+class PGLFieldArguments(ctypes.Structure):
+    _fields_ = [("spatialStructureType", ctypes.c_int), ("spatialSturctureArguments", _vp),
+                ("directionalDistributionType", ctypes.c_int), ("directionalDistributionArguments", _vp),
+                ("deterministic", ctypes.c_bool), ("debugArguments", PGLDebugArguments)]
+
+
+class PGLKDTreeArguments(ctypes.Structure):
+    _fields_ = [("knnLookup", ctypes.c_bool), ("minSamples", ctypes.c_size_t), ("maxSamples", ctypes.c_size_t),
+                ("maxDepth", ctypes.c_size_t)]
+
+
+class PGLVMMFactoryArguments(ctypes.Structure):
+    _fields_ = [("initK", ctypes.c_size_t), ("initKappa", _f), ("maxK", ctypes.c_size_t),
+                ("maxEMIterrations", ctypes.c_size_t), ("maxKappa", _f), ("convergenceThreshold", _f),
+                ("weightPrior", _f), ("meanCosinePriorStrength", _f), ("meanCosinePrior", _f),
+                ("useSplitAndMerge", ctypes.c_bool), ("splittingThreshold", _f), ("mergingThreshold", _f),
+                ("partialReFit", ctypes.c_bool), ("maxSplitItr", ctypes.c_int),
+                ("minSamplesForSplitting", ctypes.c_int), ("minSamplesForPartialRefitting", ctypes.c_int),
+                ("minSamplesForMerging", ctypes.c_int)]
+
+
+_A32 = _f * 32
+_A32x3 = (_f * 3) * 32
+_A32x2 = (_f * 2) * 32
+
+
+class PGLRegionRecord(ctypes.Structure):
+    _fields_ = [
+        ("K", _u32), ("weights", _A32), ("kappas", _A32), ("meanDirections", _A32x3), ("normalizations", _A32),
+        ("eMinus2Kappa", _A32), ("meanCosines", _A32), ("distances", _A32), ("pivot", _f * 3),
+        ("sumOfWeightedDirections", _A32x3), ("sumOfWeightedStats", _A32), ("sumOfDistanceWeights", _A32),
+        ("sumWeights", _f), ("numSamples", _f), ("overallNumSamples", _f), ("suffK", _u32), ("normalized", _u32),
+        ("chiSquareMCEstimates", _A32), ("splitMeans", _A32x2), ("splitCovariances", _A32x3),
+        ("splitNumSamples", _A32), ("splitSumWeights", _A32), ("splitSumAssignedSamples", _A32), ("splitK", _u32),
+        ("numSamplesAfterLastSplit", ctypes.c_uint64), ("numSamplesAfterLastMerge", ctypes.c_uint64),
+        ("ssMean", _f * 3), ("ssVariance", _f * 3), ("ssNumSamples", _f), ("ssLower", _f * 3), ("ssUpper", _f * 3),
+        ("regionLower", _f * 3), ("regionUpper", _f * 3), ("splitFlag", _u32), ("valid", _u32),
+        ("numSamplesThisUpdate", _u32), ("emIterations", _u32), ("partialEmIterations", _u32),
+        ("numSplits", _u32), ("numMerges", _u32)]
+
+    def to_dict(self):
+        out = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            out[name] = np.array(np.ctypeslib.as_array(v)) if hasattr(v, "_length_") else v
+        return out
+
+
+# name: (restype, argtypes)
+_SIGS = {
+    "pglFieldArgumentsSetDefaults": (None, [ctypes.POINTER(PGLFieldArguments), ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_bool, ctypes.c_size_t]),
+    "pglNewDevice": (_vp, [ctypes.c_int, ctypes.c_size_t]),
+    "pglReleaseDevice": (None, [_vp]),
+    "pglDeviceNewField": (_vp, [_vp, PGLFieldArguments]),
+    "pglReleaseField": (None, [_vp]),
+    "pglFieldGetIteration": (ctypes.c_size_t, [_vp]),
+    "pglFieldSetSceneBounds": (None, [_vp, Box3]),
+    "pglFieldGetSceneBounds": (Box3, [_vp]),
+    "pglFieldUpdate": (None, [_vp, _vp]),
+    "pglFieldUpdateSurface": (None, [_vp, _vp]),
+    "pglFieldUpdateVolume": (None, [_vp, _vp]),
+    "pglFieldUpdateSurfaceDevice": (None, [_vp, _vp, ctypes.c_size_t]),
+    "pglFieldUpdateVolumeDevice": (None, [_vp, _vp, ctypes.c_size_t]),
+    "pglFieldReset": (None, [_vp]),
+    "pglFieldValidate": (ctypes.c_bool, [_vp]),
+    "pglFieldCompare": (ctypes.c_bool, [_vp, _vp]),
+    "pglFieldNewSurfaceSamplingDistribution": (_vp, [_vp]),
+    "pglFieldInitSurfaceSamplingDistribution": (ctypes.c_bool, [_vp, _vp, Vec3, ctypes.POINTER(_f)]),
+    "pglFieldNewVolumeSamplingDistribution": (_vp, [_vp]),
+    "pglFieldInitVolumeSamplingDistribution": (ctypes.c_bool, [_vp, _vp, Vec3, ctypes.POINTER(_f)]),
+    "pglReleaseSurfaceSamplingDistribution": (None, [_vp]),
+    "pglSurfaceSamplingDistributionApplyCosineProduct": (None, [_vp, Vec3]),
+    "pglSurfaceSamplingDistributionSample": (Vec3, [_vp, Vec2]),
+    "pglSurfaceSamplingDistributionPDF": (_f, [_vp, Vec3]),
+    "pglSurfaceSamplingDistributionSamplePDF": (_f, [_vp, Vec2, ctypes.POINTER(Vec3)]),
+    "pglSurfaceSamplingDistributionIncomingRadiancePDF": (_f, [_vp, Vec3]),
+    "pglSurfaceSamplingDistributionGetId": (_u32, [_vp]),
+    "pglReleaseVolumeSamplingDistribution": (None, [_vp]),
+    "pglVolumeSamplingDistributionApplySingleLobeHenyeyGreensteinProduct": (None, [_vp, Vec3, _f]),
+    "pglVolumeSamplingDistributionSample": (Vec3, [_vp, Vec2]),
+    "pglVolumeSamplingDistributionPDF": (_f, [_vp, Vec3]),
+    "pglVolumeSamplingDistributionSamplePDF": (_f, [_vp, Vec2, ctypes.POINTER(Vec3)]),
+    "pglVolumeSamplingDistributionGetId": (_u32, [_vp]),
+    "pglNewSampleStorage": (_vp, []),
+    "pglReleaseSampleStorage": (None, [_vp]),
+    "pglSampleStorageAddSamples": (None, [_vp, _vp, ctypes.c_size_t]),
+    "pglSampleStorageGetSizeSurface": (ctypes.c_size_t, [_vp]),
+    "pglSampleStorageGetSizeVolume": (ctypes.c_size_t, [_vp]),
+    "pglSampleStorageClear": (None, [_vp]),
+    "pglSampleStorageStoreToFile": (ctypes.c_bool, [_vp, ctypes.c_char_p]),
+    "pglNewSampleStorageFromFile": (_vp, [ctypes.c_char_p]),
+    "pglSampleStorageCompare": (ctypes.c_bool, [_vp, _vp]),
+    "pglSampleStorageValidate": (ctypes.c_bool, [_vp]),
+    "pglFieldSampleBatch": (None, [_vp, ctypes.c_size_t] + [_vp] * 10),
+    "pglFieldSampleBatchDevice": (None, [_vp, ctypes.c_size_t] + [_vp] * 11),
+    "pglFieldGetNumNodes": (_u32, [_vp, ctypes.c_int]),
+    "pglFieldGetNumRegions": (_u32, [_vp, ctypes.c_int]),
+    "pglFieldExportNodes": (None, [_vp, ctypes.c_int, _vp, _vp, _vp]),
+    "pglFieldExportRegion": (None, [_vp, ctypes.c_int, _u32, ctypes.POINTER(PGLRegionRecord)]),
+    "pglFieldSampleLeafIds": (None, [_vp, ctypes.c_int, _vp, ctypes.c_size_t, _vp]),
+    "pglFieldGetKNN": (None, [_vp, ctypes.c_int, _vp]),
+    "pglFieldGetLastUpdateInfo": (None, [_vp, ctypes.c_int, _vp]),
+    "pglFieldSetRegionSchedule": (None, [_vp, ctypes.c_int]),
+}
+
+_lib = None
+
+
+def load_library():
+    """Load libopenpgl_amd.so from the package directory (raises if absent)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libopenpgl_amd.so is not built: run `python -m openpgl_amd.build`")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+class Device:
+    def __init__(self, device_type=PGL_DEVICE_TYPE_GPU_MI355X, num_threads=0):
+        self.lib = load_library()
+        self.h = self.lib.pglNewDevice(device_type, num_threads)
+        if not self.h:
+            raise RuntimeError("pglNewDevice failed (no GPU?)")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.pglReleaseDevice(self.h)
+            self.h = None
+
+
+class SampleStorage:
+    def __init__(self, samples=None):
+        self.lib = load_library()
+        self.h = self.lib.pglNewSampleStorage()
+        if samples is not None:
+            self.add_samples(samples)
+
+    def add_samples(self, samples):
+        samples = np.ascontiguousarray(samples, dtype=SAMPLE_DTYPE)
+        self.lib.pglSampleStorageAddSamples(self.h, _ptr(samples), len(samples))
+
+    def size_surface(self):
+        return self.lib.pglSampleStorageGetSizeSurface(self.h)
+
+    def size_volume(self):
+        return self.lib.pglSampleStorageGetSizeVolume(self.h)
+
+    def clear(self):
+        self.lib.pglSampleStorageClear(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.pglReleaseSampleStorage(self.h)
+            self.h = None
+
+
+class Field:
+    """openpgl::cpp::Field over the MI355X backend."""
+
+    def __init__(self, device, max_samples_per_leaf=32000, deterministic=True, knn=True, overrides=None):
+        self.lib = load_library()
+        self.device = device
+        args = PGLFieldArguments()
+        self.lib.pglFieldArgumentsSetDefaults(ctypes.byref(args), PGL_SPATIAL_STRUCTURE_KDTREE,
+                                              PGL_DIRECTIONAL_DISTRIBUTION_PARALLAX_AWARE_VMM, deterministic,
+                                              max_samples_per_leaf)
+        kd = ctypes.cast(args.spatialSturctureArguments, ctypes.POINTER(PGLKDTreeArguments)).contents
+        kd.knnLookup = knn
+        if overrides:
+            v = ctypes.cast(args.directionalDistributionArguments, ctypes.POINTER(PGLVMMFactoryArguments)).contents
+            for k, val in overrides.items():
+                if hasattr(kd, k):
+                    setattr(kd, k, val)
+                else:
+                    setattr(v, k, val)
+        self.h = self.lib.pglDeviceNewField(device.h, args)
+        if not self.h:
+            raise RuntimeError("pglDeviceNewField failed")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.pglReleaseField(self.h)
+            self.h = None
+
+    def set_scene_bounds(self, lower, upper):
+        b = Box3(Vec3(*lower), Vec3(*upper))
+        self.lib.pglFieldSetSceneBounds(self.h, b)
+
+    def scene_bounds(self):
+        b = self.lib.pglFieldGetSceneBounds(self.h)
+        return (b.lower.x, b.lower.y, b.lower.z), (b.upper.x, b.upper.y, b.upper.z)
+
+    def update(self, storage):
+        self.lib.pglFieldUpdate(self.h, storage.h)
+
+    def update_surface_device(self, device_ptr, n):
+        self.lib.pglFieldUpdateSurfaceDevice(self.h, ctypes.c_void_p(device_ptr), n)
+
+    def update_volume_device(self, device_ptr, n):
+        self.lib.pglFieldUpdateVolumeDevice(self.h, ctypes.c_void_p(device_ptr), n)
+
+    def iteration(self):
+        return self.lib.pglFieldGetIteration(self.h)
+
+    def validate(self):
+        return self.lib.pglFieldValidate(self.h)
+
+    def reset(self):
+        self.lib.pglFieldReset(self.h)
+
+    def set_region_schedule(self, schedule):
+        self.lib.pglFieldSetRegionSchedule(self.h, schedule)
+
+    # --- introspection
+    def num_regions(self, kind=0):
+        return self.lib.pglFieldGetNumRegions(self.h, kind)
+
+    def nodes(self, kind=0):
+        n = self.lib.pglFieldGetNumNodes(self.h, kind)
+        piv = np.zeros(n, np.float32)
+        dims = np.zeros(n, np.uint32)
+        cod = np.zeros(n, np.uint32)
+        self.lib.pglFieldExportNodes(self.h, kind, _ptr(piv), _ptr(dims), _ptr(cod))
+        return piv, dims, cod
+
+    def region(self, idx, kind=0):
+        r = PGLRegionRecord()
+        self.lib.pglFieldExportRegion(self.h, kind, idx, ctypes.byref(r))
+        return r.to_dict()
+
+    def leaf_ids(self, positions, kind=0):
+        p = np.ascontiguousarray(positions, np.float32)
+        out = np.zeros(len(p), np.uint32)
+        self.lib.pglFieldSampleLeafIds(self.h, kind, _ptr(p), len(p), _ptr(out))
+        return out
+
+    def knn(self, kind=0):
+        out = np.zeros(self.num_regions(kind) * 8, np.uint32)
+        self.lib.pglFieldGetKNN(self.h, kind, _ptr(out))
+        return out.reshape(-1, 8)
+
+    def last_update_info(self, kind=0):
+        o = np.zeros(12, np.float64)
+        self.lib.pglFieldGetLastUpdateInfo(self.h, kind, _ptr(o))
+        keys = ["wall_ms", "spatial_ms", "directional_ms", "knn_ms", "gather_ms", "sort_ms", "kd_rounds",
+                "region_launches", "estep_evals", "aux_evals", "em_iterations", "samples"]
+        return dict(zip(keys, o.tolist()))
+
+    # --- queries
+    def sample_batch(self, pos, nrm, u2, u1=None, g=None, is_volume=None):
+        n = len(pos)
+        pos = np.ascontiguousarray(pos, np.float32)
+        nrm = np.ascontiguousarray(nrm, np.float32)
+        u2 = np.ascontiguousarray(u2, np.float32)
+        u1 = None if u1 is None else np.ascontiguousarray(u1, np.float32)
+        g = None if g is None else np.ascontiguousarray(g, np.float32)
+        vol = None if is_volume is None else np.ascontiguousarray(is_volume, np.uint8)
+        d = np.zeros((n, 3), np.float32)
+        pdf = np.zeros(n, np.float32)
+        ids = np.zeros(n, np.uint32)
+        integ = np.zeros(n, np.float32)
+        self.lib.pglFieldSampleBatch(self.h, n, _ptr(pos), _ptr(nrm), _ptr(g), _ptr(u2), _ptr(u1), _ptr(vol),
+                                     _ptr(d), _ptr(pdf), _ptr(ids), _ptr(integ))
+        return d, pdf, ids, integ
+
+    def new_surface_distribution(self):
+        return SurfaceSamplingDistribution(self)
+
+    def new_volume_distribution(self):
+        return VolumeSamplingDistribution(self)
+
+
+class _Dist:
+    _kind = "Surface"
+
+    def __init__(self, field):
+        self.lib = field.lib
+        self.field = field
+        self.h = getattr(self.lib, "pglFieldNew%sSamplingDistribution" % self._kind)(field.h)
+
+    def init(self, pos, sample1d=-1.0):
+        u = ctypes.c_float(sample1d)
+        ok = getattr(self.lib, "pglFieldInit%sSamplingDistribution" % self._kind)(self.field.h, self.h, Vec3(*pos),
+                                                                                 ctypes.byref(u))
+        return ok, u.value
+
+    def sample(self, u):
+        v = getattr(self.lib, "pgl%sSamplingDistributionSample" % self._kind)(self.h, Vec2(*u))
+        return np.array([v.x, v.y, v.z], np.float32)
+
+    def pdf(self, d):
+        return getattr(self.lib, "pgl%sSamplingDistributionPDF" % self._kind)(self.h, Vec3(*[float(x) for x in d]))
+
+    def sample_pdf(self, u):
+        out = Vec3()
+        p = getattr(self.lib, "pgl%sSamplingDistributionSamplePDF" % self._kind)(self.h, Vec2(*u), ctypes.byref(out))
+        return np.array([out.x, out.y, out.z], np.float32), p
+
+    def id(self):
+        return getattr(self.lib, "pgl%sSamplingDistributionGetId" % self._kind)(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            getattr(self.lib, "pglRelease%sSamplingDistribution" % self._kind)(self.h)
+            self.h = None
+
+
+class SurfaceSamplingDistribution(_Dist):
+    _kind = "Surface"
+
+    def apply_cosine_product(self, n):
+        self.lib.pglSurfaceSamplingDistributionApplyCosineProduct(self.h, Vec3(*[float(x) for x in n]))
+
+    def incoming_radiance_pdf(self, d):
+        return self.lib.pglSurfaceSamplingDistributionIncomingRadiancePDF(self.h, Vec3(*[float(x) for x in d]))
+
+
+class VolumeSamplingDistribution(_Dist):
+    _kind = "Volume"
+
+    def apply_hg_product(self, d, g):
+        self.lib.pglVolumeSamplingDistributionApplySingleLobeHenyeyGreensteinProduct(
+            self.h, Vec3(*[float(x) for x in d]), float(g))

@@ -1,0 +1,746 @@
+// C ABI of libopenpgl_amd.so: the Open PGL 0.6.0 pgl* entry points
+// (openpgl/api/api.cpp in the reference) backed by the MI355X field.
+//
+// Error behaviour follows the reference: creation, update, reset, release and
+// file functions catch exceptions, print them to stdout and return
+// nullptr/false/0 (api.cpp:33-72); query functions are unguarded.
+// Scalar Init/Sample/PDF calls are served from a host mirror of the device
+// field (refreshed lazily after each update); batched queries run on the GPU.
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/openpgl_amd/openpgl.h"
+#include "field.h"
+
+using namespace pglgpu;
+
+#define PGL_CATCH_BEGIN try {
+#define PGL_CATCH_END(ret)                                            \
+  }                                                                   \
+  catch (const std::bad_alloc&) {                                     \
+    std::cout << "Open PGL was unable to allocate memory" << std::endl; \
+    return ret;                                                       \
+  }                                                                   \
+  catch (const std::exception& e) {                                   \
+    std::cout << e.what() << std::endl;                               \
+    return ret;                                                       \
+  }                                                                   \
+  catch (...) {                                                       \
+    std::cout << "an unrecognized exception was caught" << std::endl; \
+    return ret;                                                       \
+  }
+#define PGL_CATCH_END_VOID                                            \
+  }                                                                   \
+  catch (const std::bad_alloc&) {                                     \
+    std::cout << "Open PGL was unable to allocate memory" << std::endl; \
+    return;                                                           \
+  }                                                                   \
+  catch (const std::exception& e) {                                   \
+    std::cout << e.what() << std::endl;                               \
+    return;                                                           \
+  }                                                                   \
+  catch (...) {                                                       \
+    std::cout << "an unrecognized exception was caught" << std::endl; \
+    return;                                                           \
+  }
+#define THROW_IF_NULL(obj, name) \
+  if (obj == nullptr) throw std::runtime_error(std::string("null ") + name + " provided to " + __FUNCTION__)
+
+struct PGLDeviceImpl {
+  int type;
+  int vectorWidth;
+};
+
+struct PGLFieldImpl {
+  GuidingField* gf;
+  int deviceType;
+  std::vector<PGLSampleData>* staging = nullptr;
+  DevBuf dStage;
+};
+
+struct PGLSampleStorageImpl {
+  std::vector<PGLSampleData> surface, volume;
+  std::mutex mutex;
+};
+
+struct HostDist {
+  int kind = 0;
+  const KindField* field = nullptr;
+  const RegionState* region = nullptr;
+  uint32_t id = 0xffffffffu;
+  int num = 0;
+  int K = 0;
+  float mw[4] = {0, 0, 0, 0};
+  float w[4][32], kappa[4][32], mx[4][32], my[4][32], mz[4][32], norm[4][32], em2k[4][32];
+  float lw[32], lkappa[32], lmx[32], lmy[32], lmz[32], lnorm[32], lem2k[32];
+  float integral = 1.f;
+};
+struct PGLSurfaceSamplingDistributionImpl : HostDist {};
+struct PGLVolumeSamplingDistributionImpl : HostDist {};
+
+struct PGLFieldStatisticsImpl {
+  UpdateInfo info;
+  uint32_t regions, nodes;
+};
+
+static PGLString make_string(const std::string& s) {
+  PGLString r;
+  r.m_size = s.size();
+  r.m_str = new char[s.size() + 1];
+  memcpy(r.m_str, s.c_str(), s.size() + 1);
+  return r;
+}
+
+// ---- host distribution (VMMSurfaceSamplingDistribution / VMMVolumeSamplingDistribution)
+static bool dist_init(HostDist* d, const KindField& f, hipStream_t s, const float pos[3], float* sample1D) {
+  f.syncMirror(s);
+  uint32_t id = 0xffffffffu;
+  float u = sample1D ? *sample1D : -1.f;
+  const int r = f.lookupRegion(pos, &u, id);
+  if (sample1D) *sample1D = u;
+  if (r < 0) return false;
+  const RegionState& S = f.mirrorRegion((uint32_t)r);
+  if (!S.valid) return false;
+  d->field = &f;
+  d->region = &S;
+  d->id = id;
+  const VMMParams& V = S.vmm;
+  d->K = V.K;
+  const float sx = V.pivot[0] - pos[0], sy = V.pivot[1] - pos[1], sz = V.pivot[2] - pos[2];
+  for (int k = 0; k < V.K; k++) {
+    float dist;
+    shift_comp(V, k, sx, sy, sz, d->lmx[k], d->lmy[k], d->lmz[k], dist);
+    d->lw[k] = V.w[k];
+    d->lkappa[k] = V.kappa[k];
+    d->lnorm[k] = V.norm[k];
+    d->lem2k[k] = V.em2k[k];
+    d->w[0][k] = d->lw[k]; d->kappa[0][k] = d->lkappa[k]; d->mx[0][k] = d->lmx[k]; d->my[0][k] = d->lmy[k];
+    d->mz[0][k] = d->lmz[k]; d->norm[0][k] = d->lnorm[k]; d->em2k[0][k] = d->lem2k[k];
+  }
+  d->num = 1;
+  d->mw[0] = 1.f;
+  d->integral = 1.f;
+  return true;
+}
+
+static float dist_product(HostDist* d, int m, float weight, float nx, float ny, float nz, float kn, float nn) {
+  float integral = 0.f;
+  for (int k = 0; k < d->K; k++) {
+    const ProdComp p = product_comp(d->lw[k], d->lkappa[k], d->lmx[k], d->lmy[k], d->lmz[k], d->lnorm[k], weight, nx,
+                                    ny, nz, kn, nn);
+    d->w[m][k] = p.scale; d->kappa[m][k] = p.kappa; d->mx[m][k] = p.mx; d->my[m][k] = p.my; d->mz[m][k] = p.mz;
+    d->norm[m][k] = p.norm; d->em2k[m][k] = p.em2k;
+    integral += p.scale;
+  }
+  for (int k = 0; k < d->K; k++) d->w[m][k] /= integral;
+  return integral;
+}
+
+static void dist_sample(const HostDist* d, float ux, float uy, float out[3]) {
+  float acc = 0.f;
+  int m = 0;
+  for (; m < d->num - 1; ++m) {
+    const float nw = acc + d->mw[m];
+    if (nw > ux) break;
+    acc = nw;
+  }
+  const float u0 = (ux - acc) / d->mw[m];
+  float sum = 0.f, cdf = 0.f;
+  int k = 0;
+  for (;; k++) {
+    cdf = d->w[m][k];
+    if (sum + cdf >= uy || k + 1 >= d->K) break;
+    sum += cdf;
+  }
+  const float us = std::min(1.f - 1.19209290e-7f, (uy - sum) / cdf);
+  sample_comp(d->kappa[m][k], d->em2k[m][k], d->mx[m][k], d->my[m][k], d->mz[m][k], u0, us, out);
+}
+
+static float dist_pdf(const HostDist* d, const float dir[3]) {
+  float pdf = 0.f;
+  for (int m = 0; m < d->num; m++) {
+    float p = 0.f;
+    for (int k = 0; k < d->K; k++)
+      p += eval_comp(d->w[m][k], d->norm[m][k], d->kappa[m][k], d->mx[m][k], d->my[m][k], d->mz[m][k], dir[0], dir[1], dir[2]);
+    pdf += d->mw[m] * p;
+  }
+  return pdf;
+}
+
+static float dist_pdf_li(const HostDist* d, const float dir[3]) {
+  float p = 0.f;
+  for (int k = 0; k < d->K; k++)
+    p += eval_comp(d->lw[k], d->lnorm[k], d->lkappa[k], d->lmx[k], d->lmy[k], d->lmz[k], dir[0], dir[1], dir[2]);
+  return p;
+}
+
+static void field_update_host(PGLFieldImpl* F, KindField& kf, const std::vector<PGLSampleData>& samples) {
+  if (samples.empty()) return;
+  F->dStage.ensure(samples.size() * sizeof(PGLSampleData));
+  PGL_HIP_CHECK(hipMemcpyAsync(F->dStage.p, samples.data(), samples.size() * sizeof(PGLSampleData), hipMemcpyHostToDevice,
+                               F->gf->stream));
+  kf.update(F->dStage.p, (uint32_t)samples.size(), F->gf->stream);
+}
+
+extern "C" {
+
+void pglFieldArgumentsSetDefaults(PGLFieldArguments& a, enum PGL_SPATIAL_STRUCTURE_TYPE spatialType,
+                                  enum PGL_DIRECTIONAL_DISTRIBUTION_TYPE directionalType, bool deterministic,
+                                  size_t maxSamplesPerLeaf) {
+  (void)spatialType;
+  a.spatialStructureType = PGL_SPATIAL_STRUCTURE_KDTREE;
+  PGLKDTreeArguments* kd = new PGLKDTreeArguments();
+  kd->knnLookup = true; kd->minSamples = 100; kd->maxSamples = maxSamplesPerLeaf; kd->maxDepth = 32;
+  a.spatialSturctureArguments = kd;
+  a.deterministic = deterministic;
+  a.debugArguments.fitRegions = true;
+  a.directionalDistributionType = directionalType == PGL_DIRECTIONAL_DISTRIBUTION_QUADTREE
+                                      ? PGL_DIRECTIONAL_DISTRIBUTION_QUADTREE
+                                      : (directionalType == PGL_DIRECTIONAL_DISTRIBUTION_VMM ? PGL_DIRECTIONAL_DISTRIBUTION_VMM
+                                                                                              : PGL_DIRECTIONAL_DISTRIBUTION_PARALLAX_AWARE_VMM);
+  PGLVMMFactoryArguments* v = new PGLVMMFactoryArguments();
+  v->initK = PGL_VMM_MAX_COMPONENTS / 2; v->initKappa = 0.5f; v->maxK = PGL_VMM_MAX_COMPONENTS;
+  v->maxEMIterrations = 100; v->maxKappa = PGL_VMM_MAX_KAPPA; v->convergenceThreshold = 0.005f;
+  v->weightPrior = 0.01f; v->meanCosinePriorStrength = 0.2f; v->meanCosinePrior = 0.0f;
+  v->useSplitAndMerge = true; v->splittingThreshold = 0.5f; v->mergingThreshold = 0.025f; v->partialReFit = true;
+  v->maxSplitItr = 1;
+  v->minSamplesForSplitting = (int)(maxSamplesPerLeaf / 8);
+  v->minSamplesForPartialRefitting = (int)(maxSamplesPerLeaf / 8);
+  v->minSamplesForMerging = (int)(maxSamplesPerLeaf / 4);
+  a.directionalDistributionArguments = v;
+}
+
+void pglReleaseString(PGLString str) { delete[] str.m_str; }
+
+PGLDevice pglNewDevice(enum PGL_DEVICE_TYPE deviceType, size_t numThreads) PGL_CATCH_BEGIN {
+  (void)numThreads;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+    throw std::runtime_error("openpgl_amd: no HIP device available (this library runs on the GPU only)");
+  PGL_HIP_CHECK(hipSetDevice(0));
+  PGLDeviceImpl* d = new PGLDeviceImpl();
+  d->type = deviceType;
+  d->vectorWidth = deviceType == PGL_DEVICE_TYPE_CPU_4 ? 4 : (deviceType == PGL_DEVICE_TYPE_CPU_16 ? 16 : 8);
+  return d;
+}
+PGL_CATCH_END(nullptr)
+
+void pglReleaseDevice(PGLDevice device) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(device, "handle");
+  delete device;
+}
+PGL_CATCH_END_VOID
+
+PGLField pglDeviceNewField(PGLDevice device, PGLFieldArguments args) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(device, "handle");
+  if (args.directionalDistributionType != PGL_DIRECTIONAL_DISTRIBUTION_PARALLAX_AWARE_VMM)
+    throw std::runtime_error("openpgl_amd: only the parallax-aware VMM field is implemented");
+  PGLKDTreeArguments* kd = (PGLKDTreeArguments*)args.spatialSturctureArguments;
+  PGLVMMFactoryArguments* v = (PGLVMMFactoryArguments*)args.directionalDistributionArguments;
+  THROW_IF_NULL(kd, "spatial structure arguments");
+  THROW_IF_NULL(v, "directional distribution arguments");
+  KdSettings ks;
+  ks.knnLookup = kd->knnLookup;
+  ks.minSamples = (uint32_t)kd->minSamples;
+  ks.maxSamples = (uint32_t)kd->maxSamples;
+  ks.maxDepth = (uint32_t)kd->maxDepth;
+  FactoryConfig fc{};
+  fc.initK = (int)v->initK;
+  fc.initKappa = v->initKappa;
+  fc.maxEMIterations = (int)v->maxEMIterrations;
+  fc.maxMeanCosine = 1.f / std::tanh(v->maxKappa) - 1.f / v->maxKappa;  // KappaToMeanCosine<float>
+  fc.convergenceThreshold = v->convergenceThreshold;
+  fc.weightPrior = v->weightPrior;
+  fc.meanCosinePriorStrength = v->meanCosinePriorStrength;
+  fc.meanCosinePrior = v->meanCosinePrior;
+  fc.useSplitAndMerge = v->useSplitAndMerge;
+  fc.splittingThreshold = v->splittingThreshold;
+  fc.mergingThreshold = v->mergingThreshold;
+  fc.partialReFit = v->partialReFit;
+  fc.minSamplesForSplitting = v->minSamplesForSplitting;
+  fc.minSamplesForPartialRefitting = v->minSamplesForPartialRefitting;
+  fc.minSamplesForMerging = v->minSamplesForMerging;
+  fc.decayOnSpatialSplit = 0.25f;
+  fc.maxMeanCosineSplit = 1.f / std::tanh(32000.0f) - 1.f / 32000.0f;
+  fc.vectorWidth = device->vectorWidth;
+  if (fc.initK < 1 || fc.initK > 32) throw std::runtime_error("openpgl_amd: initK must be in [1, 32]");
+  // the reference deletes the argument structs it was handed (Device.h:109, 136)
+  delete kd;
+  delete v;
+  PGLFieldImpl* F = new PGLFieldImpl();
+  F->gf = new GuidingField(ks, fc, args.deterministic, device->vectorWidth);
+  F->deviceType = device->type;
+  return F;
+}
+PGL_CATCH_END(nullptr)
+
+PGLField pglDeviceNewFieldFromFile(PGLDevice device, const char* fieldFileName) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(device, "handle");
+  THROW_IF_NULL(fieldFileName, "string");
+  throw std::runtime_error("openpgl_amd: field files are not supported yet");
+}
+PGL_CATCH_END(nullptr)
+
+void pglReleaseField(PGLField field) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(field, "handle");
+  if (field->gf->stream) (void)hipStreamDestroy(field->gf->stream);
+  delete field->gf;
+  delete field;
+}
+PGL_CATCH_END_VOID
+
+bool pglFieldStoreToFile(PGLField field, const char* fieldFileName) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(field, "handle");
+  THROW_IF_NULL(fieldFileName, "string");
+  throw std::runtime_error("openpgl_amd: field files are not supported yet");
+}
+PGL_CATCH_END(false)
+
+size_t pglFieldGetIteration(PGLField field) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(field, "handle");
+  return field->gf->iteration;
+}
+PGL_CATCH_END(0)
+
+void pglFieldSetSceneBounds(PGLField field, pgl_box3f bounds) {
+  // SurfaceVolumeField::setSceneBounds: enlarge_by(1.01f) returns a new box and is discarded
+  const float lo[3] = {bounds.lower.x, bounds.lower.y, bounds.lower.z};
+  const float hi[3] = {bounds.upper.x, bounds.upper.y, bounds.upper.z};
+  field->gf->surface.setSceneBounds(lo, hi);
+  field->gf->volume.setSceneBounds(lo, hi);
+}
+
+pgl_box3f pglFieldGetSceneBounds(PGLField field) {
+  float a[3], b[3], c[3], d[3];
+  field->gf->surface.sceneBounds(a, b);
+  field->gf->volume.sceneBounds(c, d);
+  pgl_box3f r;
+  r.lower.x = std::min(a[0], c[0]); r.lower.y = std::min(a[1], c[1]); r.lower.z = std::min(a[2], c[2]);
+  r.upper.x = std::max(b[0], d[0]); r.upper.y = std::max(b[1], d[1]); r.upper.z = std::max(b[2], d[2]);
+  return r;
+}
+
+void pglFieldUpdate(PGLField field, PGLSampleStorage storage) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(field, "handle");
+  THROW_IF_NULL(storage, "handle");
+  field_update_host(field, field->gf->surface, storage->surface);
+  field_update_host(field, field->gf->volume, storage->volume);
+  field->gf->iteration++;
+}
+PGL_CATCH_END_VOID
+
+void pglFieldUpdateSurface(PGLField field, PGLSampleStorage storage) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(field, "handle");
+  THROW_IF_NULL(storage, "handle");
+  field_update_host(field, field->gf->surface, storage->surface);
+  field->gf->iteration++;
+}
+PGL_CATCH_END_VOID
+
+void pglFieldUpdateVolume(PGLField field, PGLSampleStorage storage) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(field, "handle");
+  THROW_IF_NULL(storage, "handle");
+  field_update_host(field, field->gf->volume, storage->volume);
+  field->gf->iteration++;
+}
+PGL_CATCH_END_VOID
+
+void pglFieldUpdateSurfaceDevice(PGLField field, const PGLSampleData* samples, size_t n) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(field, "handle");
+  if (n > 0xffffffffu) throw std::runtime_error("openpgl_amd: at most 2^32-1 samples per update");
+  field->gf->surface.update(samples, (uint32_t)n, field->gf->stream);
+  field->gf->iteration++;
+}
+PGL_CATCH_END_VOID
+
+void pglFieldUpdateVolumeDevice(PGLField field, const PGLSampleData* samples, size_t n) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(field, "handle");
+  if (n > 0xffffffffu) throw std::runtime_error("openpgl_amd: at most 2^32-1 samples per update");
+  field->gf->volume.update(samples, (uint32_t)n, field->gf->stream);
+  field->gf->iteration++;
+}
+PGL_CATCH_END_VOID
+
+void pglFieldReset(PGLField field) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(field, "handle");
+  field->gf->iteration = 0;
+  field->gf->surface.reset();
+  field->gf->volume.reset();
+}
+PGL_CATCH_END_VOID
+
+PGLSurfaceSamplingDistribution pglFieldNewSurfaceSamplingDistribution(PGLField field) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(field, "handle");
+  PGLSurfaceSamplingDistribution d = new PGLSurfaceSamplingDistributionImpl();
+  d->kind = 0;
+  return d;
+}
+PGL_CATCH_END(nullptr)
+
+bool pglFieldInitSurfaceSamplingDistribution(PGLField field, PGLSurfaceSamplingDistribution d, pgl_point3f position,
+                                             float* sample1D) {
+  const float p[3] = {position.x, position.y, position.z};
+  return dist_init(d, field->gf->surface, field->gf->stream, p, sample1D);
+}
+
+PGLVolumeSamplingDistribution pglFieldNewVolumeSamplingDistribution(PGLField field) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(field, "handle");
+  PGLVolumeSamplingDistribution d = new PGLVolumeSamplingDistributionImpl();
+  d->kind = 1;
+  return d;
+}
+PGL_CATCH_END(nullptr)
+
+bool pglFieldInitVolumeSamplingDistribution(PGLField field, PGLVolumeSamplingDistribution d, pgl_point3f position,
+                                            float* sample1D) {
+  const float p[3] = {position.x, position.y, position.z};
+  return dist_init(d, field->gf->volume, field->gf->stream, p, sample1D);
+}
+
+bool pglFieldValidate(PGLField field) {
+  bool ok = true;
+  if (field->gf->surface.initialized()) { field->gf->surface.syncMirror(field->gf->stream); ok = ok && field->gf->surface.validate(); }
+  if (field->gf->volume.initialized()) { field->gf->volume.syncMirror(field->gf->stream); ok = ok && field->gf->volume.validate(); }
+  return ok;
+}
+
+static bool kind_equal(const KindField& a, const KindField& b, hipStream_t sa, hipStream_t sb) {
+  if (a.initialized() != b.initialized() || a.iteration() != b.iteration()) return false;
+  if (!a.initialized()) return true;
+  a.syncMirror(sa);
+  b.syncMirror(sb);
+  std::vector<KDNode> na, nb;
+  std::vector<uint32_t> oa, ob;
+  a.canonicalOrder(na, oa);
+  b.canonicalOrder(nb, ob);
+  if (na.size() != nb.size()) return false;
+  for (size_t i = 0; i < na.size(); i++)
+    if (na[i].pivot != nb[i].pivot || na[i].bits != nb[i].bits) return false;
+  for (size_t c = 0; c < oa.size(); c++) {
+    PGLRegionRecordHost ra, rb;
+    a.exportRegion(oa[c], &ra);
+    b.exportRegion(ob[c], &rb);
+    if (memcmp(&ra, &rb, sizeof(ra)) != 0) return false;
+  }
+  return true;
+}
+
+bool pglFieldCompare(PGLField fieldA, PGLField fieldB) {  // Field::operator== after the canonical reorder
+  return kind_equal(fieldA->gf->surface, fieldB->gf->surface, fieldA->gf->stream, fieldB->gf->stream) &&
+         kind_equal(fieldA->gf->volume, fieldB->gf->volume, fieldA->gf->stream, fieldB->gf->stream) &&
+         fieldA->gf->iteration == fieldB->gf->iteration;
+}
+
+static PGLFieldStatistics make_stats(const KindField& k) {
+  PGLFieldStatistics s = new PGLFieldStatisticsImpl();
+  s->info = k.lastInfo();
+  s->regions = k.numRegions();
+  s->nodes = (uint32_t)k.nodes().size();
+  return s;
+}
+PGLFieldStatistics pglFieldGetSurfaceStatistics(PGLField field) { return make_stats(field->gf->surface); }
+PGLFieldStatistics pglFieldGetVolumeStatistics(PGLField field) { return make_stats(field->gf->volume); }
+void pglReleaseFieldStatistics(PGLFieldStatistics s) { delete s; }
+PGLString pglFieldStatisticsToString(PGLFieldStatistics s) {
+  std::ostringstream o;
+  o << "FieldStatistics:\n  numRegions: " << s->regions << "\n  numNodes: " << s->nodes
+    << "\n  timeLastUpdate[ms]: " << s->info.wallMs << "\n  timeSpatialStructureUpdate[ms]: " << s->info.spatialMs
+    << "\n  timeDirectionalDistriubtionUpdate[ms]: " << s->info.directionalMs << "\n";
+  return make_string(o.str());
+}
+PGLString pglFieldStatisticsHeaderCSVString(PGLFieldStatistics) {
+  return make_string("numRegions,numNodes,timeLastUpdate,timeSpatialStructureUpdate,timeDirectionalDistriubtionUpdate");
+}
+PGLString pglFieldStatisticsToCSVString(PGLFieldStatistics s) {
+  std::ostringstream o;
+  o << s->regions << "," << s->nodes << "," << s->info.wallMs << "," << s->info.spatialMs << "," << s->info.directionalMs;
+  return make_string(o.str());
+}
+
+// ---- sample storage
+PGLSampleStorage pglNewSampleStorage(void) PGL_CATCH_BEGIN { return new PGLSampleStorageImpl(); }
+PGL_CATCH_END(nullptr)
+
+static const char kStorageHeader[] = "OPENPGL_0.6.0_SAMPLE_STORAGE";
+
+PGLSampleStorage pglNewSampleStorageFromFile(const char* name) PGL_CATCH_BEGIN {  // SampleDataStorage.h:30-70, 196-216
+  THROW_IF_NULL(name, "string");
+  std::ifstream is(name, std::ios::binary);
+  if (!is.is_open()) throw std::runtime_error("error: couldn't open file");
+  char buf[256];
+  is.read(buf, sizeof(kStorageHeader));
+  if (!is) throw std::runtime_error("error: invalid file header");
+  PGLSampleStorageImpl* s = new PGLSampleStorageImpl();
+  for (int kind = 0; kind < 2; kind++) {
+    size_t n = 0;
+    is.read((char*)&n, sizeof(size_t));
+    auto& v = kind == 0 ? s->surface : s->volume;
+    v.resize(n);
+    if (n) is.read((char*)v.data(), n * sizeof(PGLSampleData));
+  }
+  return s;
+}
+PGL_CATCH_END(nullptr)
+
+void pglReleaseSampleStorage(PGLSampleStorage s) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(s, "handle");
+  delete s;
+}
+PGL_CATCH_END_VOID
+
+bool pglSampleStorageStoreToFile(PGLSampleStorage s, const char* name) PGL_CATCH_BEGIN {  // SampleDataStorage.h:176-194
+  THROW_IF_NULL(s, "handle");
+  THROW_IF_NULL(name, "string");
+  std::ofstream os(name, std::ios::binary);
+  if (!os.is_open()) throw std::runtime_error("error: couldn't open file!");
+  os.write(kStorageHeader, sizeof(kStorageHeader));
+  for (int kind = 0; kind < 2; kind++) {
+    const auto& v = kind == 0 ? s->surface : s->volume;
+    const size_t n = v.size();
+    os.write((const char*)&n, sizeof(size_t));
+    if (n) os.write((const char*)v.data(), n * sizeof(PGLSampleData));
+  }
+  return true;
+}
+PGL_CATCH_END(false)
+
+void pglSampleStorageAddSample(PGLSampleStorage s, PGLSampleData& sample) {  // SampleDataStorage.h:72-80
+  std::lock_guard<std::mutex> l(s->mutex);
+  if (sample.flags & PGL_SAMPLE_FLAG_INSIDE_VOLUME) s->volume.push_back(sample);
+  else s->surface.push_back(sample);
+}
+
+void pglSampleStorageAddSamples(PGLSampleStorage s, const PGLSampleData* samples, size_t n) {  // :82-90
+  std::lock_guard<std::mutex> l(s->mutex);
+  for (size_t i = 0; i < n; i++) {
+    if (samples[i].flags & PGL_SAMPLE_FLAG_INSIDE_VOLUME) s->volume.push_back(samples[i]);
+    else s->surface.push_back(samples[i]);
+  }
+}
+
+void pglSampleStorageReserve(PGLSampleStorage s, const size_t a, const size_t b) {
+  std::lock_guard<std::mutex> l(s->mutex);
+  s->surface.reserve(a);
+  s->volume.reserve(b);
+}
+void pglSampleStorageClear(PGLSampleStorage s) { std::lock_guard<std::mutex> l(s->mutex); s->surface.clear(); s->volume.clear(); }
+void pglSampleStorageClearSurface(PGLSampleStorage s) { std::lock_guard<std::mutex> l(s->mutex); s->surface.clear(); }
+void pglSampleStorageClearVolume(PGLSampleStorage s) { std::lock_guard<std::mutex> l(s->mutex); s->volume.clear(); }
+size_t pglSampleStorageGetSizeSurface(PGLSampleStorage s) { return s->surface.size(); }
+size_t pglSampleStorageGetSizeVolume(PGLSampleStorage s) { return s->volume.size(); }
+PGLSampleData pglSampleStorageGetSampleSurface(PGLSampleStorage s, const int idx) { return s->surface[idx]; }
+PGLSampleData pglSampleStorageGetSampleVolume(PGLSampleStorage s, const int idx) { return s->volume[idx]; }
+
+static bool sample_valid(const PGLSampleData& d) {  // SampleData.h:25-46
+  const float v[9] = {d.position.x, d.position.y, d.position.z, d.direction.x, d.direction.y, d.direction.z, d.weight, d.pdf, d.distance};
+  for (float x : v) if (!std::isfinite(x)) return false;
+  return d.weight >= 0.f && d.pdf > 0.f && d.distance > 0.f;
+}
+bool pglSampleStorageValidate(PGLSampleStorage s) {
+  for (const auto& d : s->surface) if (!sample_valid(d)) return false;
+  for (const auto& d : s->volume) if (!sample_valid(d)) return false;
+  return true;
+}
+bool pglSampleStorageCompare(PGLSampleStorage a, PGLSampleStorage b) {
+  if (a->surface.size() != b->surface.size() || a->volume.size() != b->volume.size()) return false;
+  return memcmp(a->surface.data(), b->surface.data(), a->surface.size() * sizeof(PGLSampleData)) == 0 &&
+         memcmp(a->volume.data(), b->volume.data(), a->volume.size() * sizeof(PGLSampleData)) == 0;
+}
+
+// ---- surface distribution
+void pglReleaseSurfaceSamplingDistribution(PGLSurfaceSamplingDistribution d) { delete d; }
+void pglSurfaceSamplingDistributionApplyCosineProduct(PGLSurfaceSamplingDistribution d, pgl_vec3f n) {
+  if (d->num > 0) d->integral = dist_product(d, 0, 1.f, n.x, n.y, n.z, kCosineKappa, cosine_normalization());
+}
+bool pglSurfaceSamplingDistributionSupportsApplyCosineProduct(PGLSurfaceSamplingDistribution) { return true; }
+pgl_vec3f pglSurfaceSamplingDistributionSample(PGLSurfaceSamplingDistribution d, pgl_point2f u) {
+  float o[3];
+  dist_sample(d, u.x, u.y, o);
+  return pgl_vec3f{o[0], o[1], o[2]};
+}
+float pglSurfaceSamplingDistributionPDF(PGLSurfaceSamplingDistribution d, pgl_vec3f dir) {
+  const float v[3] = {dir.x, dir.y, dir.z};
+  return dist_pdf(d, v);
+}
+float pglSurfaceSamplingDistributionSamplePDF(PGLSurfaceSamplingDistribution d, pgl_point2f u, pgl_vec3f& dir) {
+  float o[3];
+  dist_sample(d, u.x, u.y, o);
+  dir = pgl_vec3f{o[0], o[1], o[2]};
+  return dist_pdf(d, o);
+}
+float pglSurfaceSamplingDistributionIncomingRadiancePDF(PGLSurfaceSamplingDistribution d, pgl_vec3f dir) {
+  const float v[3] = {dir.x, dir.y, dir.z};
+  return dist_pdf_li(d, v);
+}
+uint32_t pglSurfaceSamplingDistributionGetId(PGLSurfaceSamplingDistribution d) { return d->id; }
+bool pglSurfaceSamplingDistributionValidate(PGLSurfaceSamplingDistribution d) { return d->num > 0; }
+void pglSurfaceSamplingDistributionClear(PGLSurfaceSamplingDistribution d) { d->num = 0; }
+PGLRegion pglSurfaceSamplingGetRegion(PGLSurfaceSamplingDistribution d) { return (PGLRegion)d->region; }
+
+// ---- volume distribution
+void pglReleaseVolumeSamplingDistribution(PGLVolumeSamplingDistribution d) { delete d; }
+pgl_vec3f pglVolumeSamplingDistributionSample(PGLVolumeSamplingDistribution d, pgl_point2f u) {
+  float o[3];
+  dist_sample(d, u.x, u.y, o);
+  return pgl_vec3f{o[0], o[1], o[2]};
+}
+float pglVolumeSamplingDistributionPDF(PGLVolumeSamplingDistribution d, pgl_vec3f dir) {
+  const float v[3] = {dir.x, dir.y, dir.z};
+  return dist_pdf(d, v);
+}
+float pglVolumeSamplingDistributionSamplePDF(PGLVolumeSamplingDistribution d, pgl_point2f u, pgl_vec3f& dir) {
+  float o[3];
+  dist_sample(d, u.x, u.y, o);
+  dir = pgl_vec3f{o[0], o[1], o[2]};
+  return dist_pdf(d, o);
+}
+float pglVolumeSamplingDistributionIncomingRadiancePDF(PGLVolumeSamplingDistribution d, pgl_vec3f dir) {
+  const float v[3] = {dir.x, dir.y, dir.z};
+  return dist_pdf_li(d, v);
+}
+uint32_t pglVolumeSamplingDistributionGetId(PGLVolumeSamplingDistribution d) { return d->id; }
+bool pglVolumeSamplingDistributionValidate(PGLVolumeSamplingDistribution d) { return d->num > 0; }
+void pglVolumeSamplingDistributionClear(PGLVolumeSamplingDistribution d) { d->num = 0; }
+void pglVolumeSamplingDistributionApplySingleLobeHenyeyGreensteinProduct(PGLVolumeSamplingDistribution d, pgl_vec3f dir,
+                                                                         const float meanCosine) {
+  // VMMVolumeSamplingDistribution.h:54-80
+  const HGRep& rep = host_hg_table()[hg_index(meanCosine)];
+  float sum = 0.f;
+  const int KP = (int)rep.K;
+  for (int i = 0; i < KP; i++) {
+    const float sg = meanCosine * rep.mc[i] > 0.f ? 1.f : -1.f;
+    d->mw[i] = dist_product(d, i, rep.w[i], dir.x * sg, dir.y * sg, dir.z * sg, rep.kappa[i], rep.norm[i]);
+    sum += d->mw[i];
+  }
+  for (int i = 0; i < KP; i++) d->mw[i] /= sum;
+  d->num = KP;
+  d->integral = sum;
+}
+bool pglVolumeSamplingDistributionSupportsApplySingleLobeHenyeyGreensteinProduct(PGLVolumeSamplingDistribution) {
+  return true;
+}
+PGLRegion pglVolumeSamplingGetRegion(PGLVolumeSamplingDistribution d) { return (PGLRegion)d->region; }
+
+// ---- batched queries
+void pglFieldSampleBatchDevice(PGLField field, size_t n, const float* pos, const float* nrm, const float* g,
+                               const float* u2, const float* u1, const uint8_t* isVolume, float* dirOut, float* pdfOut,
+                               uint32_t* idOut, float* integralOut, void* stream) {
+  const QueryField sf = field->gf->surface.queryView();
+  const QueryField vf = field->gf->volume.queryView();
+  launch_query(sf, vf, field->gf->dHG.as<HGRep>(), (uint32_t)n, pos, nrm, g, u2, u1, isVolume, dirOut, pdfOut, idOut,
+               integralOut, (hipStream_t)stream);
+}
+
+void pglFieldSampleBatch(PGLField field, size_t n, const float* pos, const float* nrm, const float* g, const float* u2,
+                         const float* u1, const uint8_t* isVolume, float* dirOut, float* pdfOut, uint32_t* idOut,
+                         float* integralOut) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(field, "handle");
+  if (n == 0) return;
+  // staging layout: pos 3n, nrm 3n, g n, u2 2n, u1 n, dir 3n, pdf n, id n, integral n (floats), vol n bytes
+  const size_t nf = 16 * n;
+  DevBuf& b = field->gf->dQuery;
+  b.ensure(nf * 4 + n + 64);
+  float* base = b.as<float>();
+  float *dPos = base, *dNrm = base + 3 * n, *dG = base + 6 * n, *dU2 = base + 7 * n, *dU1 = base + 9 * n;
+  float *dDir = base + 10 * n, *dPdf = base + 13 * n, *dInt = base + 15 * n;
+  uint32_t* dId = (uint32_t*)(base + 14 * n);
+  uint8_t* dVol = (uint8_t*)(base + nf);
+  hipStream_t s = field->gf->stream;
+  PGL_HIP_CHECK(hipMemcpyAsync(dPos, pos, 12 * n, hipMemcpyHostToDevice, s));
+  PGL_HIP_CHECK(hipMemcpyAsync(dNrm, nrm, 12 * n, hipMemcpyHostToDevice, s));
+  if (g) PGL_HIP_CHECK(hipMemcpyAsync(dG, g, 4 * n, hipMemcpyHostToDevice, s));
+  PGL_HIP_CHECK(hipMemcpyAsync(dU2, u2, 8 * n, hipMemcpyHostToDevice, s));
+  if (u1) PGL_HIP_CHECK(hipMemcpyAsync(dU1, u1, 4 * n, hipMemcpyHostToDevice, s));
+  if (isVolume) PGL_HIP_CHECK(hipMemcpyAsync(dVol, isVolume, n, hipMemcpyHostToDevice, s));
+  pglFieldSampleBatchDevice(field, n, dPos, dNrm, g ? dG : nullptr, dU2, u1 ? dU1 : nullptr, isVolume ? dVol : nullptr,
+                            dDir, dPdf, dId, dInt, s);
+  PGL_HIP_CHECK(hipGetLastError());
+  PGL_HIP_CHECK(hipMemcpyAsync(dirOut, dDir, 12 * n, hipMemcpyDeviceToHost, s));
+  PGL_HIP_CHECK(hipMemcpyAsync(pdfOut, dPdf, 4 * n, hipMemcpyDeviceToHost, s));
+  if (idOut) PGL_HIP_CHECK(hipMemcpyAsync(idOut, dId, 4 * n, hipMemcpyDeviceToHost, s));
+  if (integralOut) PGL_HIP_CHECK(hipMemcpyAsync(integralOut, dInt, 4 * n, hipMemcpyDeviceToHost, s));
+  PGL_HIP_CHECK(hipStreamSynchronize(s));
+}
+PGL_CATCH_END_VOID
+
+// ---- introspection
+static KindField& kind_of(PGLField f, int kind) { return kind == 0 ? f->gf->surface : f->gf->volume; }
+
+uint32_t pglFieldGetNumNodes(PGLField field, int kind) { return (uint32_t)kind_of(field, kind).nodes().size(); }
+uint32_t pglFieldGetNumRegions(PGLField field, int kind) { return kind_of(field, kind).numRegions(); }
+
+void pglFieldExportNodes(PGLField field, int kind, float* pivots, uint32_t* dims, uint32_t* childOrRegion) {
+  std::vector<KDNode> out;
+  std::vector<uint32_t> order;
+  kind_of(field, kind).canonicalOrder(out, order);
+  for (size_t i = 0; i < out.size(); i++) { pivots[i] = out[i].pivot; dims[i] = out[i].dim(); childOrRegion[i] = out[i].idx(); }
+}
+
+void pglFieldExportRegion(PGLField field, int kind, uint32_t canonicalIdx, PGLRegionRecord* out) {
+  KindField& k = kind_of(field, kind);
+  k.syncMirror(field->gf->stream);
+  std::vector<KDNode> nodes;
+  std::vector<uint32_t> order;
+  k.canonicalOrder(nodes, order);
+  static_assert(sizeof(PGLRegionRecord) == sizeof(PGLRegionRecordHost), "record layouts agree");
+  k.exportRegion(order[canonicalIdx], (PGLRegionRecordHost*)out);
+}
+
+void pglFieldSampleLeafIds(PGLField field, int kind, const float* positions, size_t n, uint32_t* ids) {
+  KindField& k = kind_of(field, kind);
+  std::vector<KDNode> nodes;
+  std::vector<uint32_t> order;
+  k.canonicalOrder(nodes, order);
+  for (size_t i = 0; i < n; i++) {
+    uint32_t j = 0;
+    const float* p = positions + 3 * i;
+    while (!nodes[j].isLeaf()) j = nodes[j].idx() + (p[nodes[j].dim()] >= nodes[j].pivot ? 1u : 0u);
+    ids[i] = nodes[j].idx();
+  }
+}
+
+void pglFieldGetKNN(PGLField field, int kind, uint32_t* nb) {
+  KindField& k = kind_of(field, kind);
+  k.syncMirror(field->gf->stream);
+  std::vector<KDNode> nodes;
+  std::vector<uint32_t> order;
+  k.canonicalOrder(nodes, order);
+  std::vector<uint32_t> s2c(k.numRegions(), 0xffffffffu);
+  for (size_t c = 0; c < order.size(); c++) s2c[order[c]] = (uint32_t)c;
+  std::vector<uint32_t> ids;
+  k.knnHost(ids);
+  for (size_t c = 0; c < order.size(); c++)
+    for (int i = 0; i < 8; i++) {
+      const uint32_t id = ids.empty() ? 0xffffffffu : ids[order[c] * 8 + i];
+      nb[c * 8 + i] = id == 0xffffffffu ? 0xffffffffu : s2c[id];
+    }
+}
+
+void pglFieldGetLastUpdateInfo(PGLField field, int kind, double* o) {
+  KindField& k = kind_of(field, kind);
+  const UpdateInfo& i = k.lastInfo();
+  o[0] = i.wallMs; o[1] = i.spatialMs; o[2] = i.directionalMs; o[3] = i.knnMs; o[4] = i.gatherMs; o[5] = i.sortMs;
+  o[6] = i.kdRounds; o[7] = i.regionLaunches;
+  k.syncMirror(field->gf->stream);
+  double ee = 0, ae = 0, it = 0;
+  for (uint32_t r = 0; r < k.numRegions(); r++) {
+    const RegionState& S = k.mirrorRegion(r);
+    ee += S.estepEvals;
+    ae += S.auxEvals;
+    it += S.emIterations + S.partialEmIterations;
+  }
+  o[8] = ee; o[9] = ae; o[10] = it; o[11] = i.samples;
+}
+
+void pglFieldSetRegionSchedule(PGLField field, int schedule) {
+  field->gf->surface.setSchedule(schedule);
+  field->gf->volume.setSchedule(schedule);
+}
+
+}  // extern "C"

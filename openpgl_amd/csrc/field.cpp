@@ -1,0 +1,689 @@
+// Host orchestration of Field::Update on the MI355X.
+//
+// The spatial decisions (split test, split dimension and position, region
+// statistics merge/split/decay) are taken on the host in exact IEEE float
+// arithmetic from integer statistics that the GPU accumulates; everything
+// proportional to the number of samples runs in HIP kernels.
+//
+// Reference behaviour:
+//   KDTreePartitionBuilder::updateTree/updateTreeNode  spatial/kdtree/KDTreeBuilder.h:76-118, 233-350
+//   SampleStatistics / IntegerSampleStatistics          data/SampleStatistics.h:12-342
+//   Field::buildField/updateField/fitRegions/updateRegions  field/Field.h:147-220, 389-511
+//   Field::estimateSceneBounds                           field/Field.h:326-344
+//   Field::getRegion                                     field/Field.h:109-145
+#pragma clang fp contract(off)
+#include "field.h"
+
+#include <immintrin.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+namespace pglgpu {
+
+static const float kFltMax = 3.402823466e38f;
+
+// ---------------------------------------------------------------- statistics
+void HostSampleStats::merge(const HostSampleStats& b) {  // SampleStatistics.h:101-118
+  float mA[3], mB[3], vA[3], vB[3];
+  for (int i = 0; i < 3; i++) { mA[i] = mean[i]; mB[i] = b.mean[i]; vA[i] = var[i]; vB[i] = b.var[i]; }
+  const float nA = n, nB = b.n;
+  for (int i = 0; i < 3; i++) mean[i] = mA[i] * nA + mB[i] * nB;
+  n += nB;
+  for (int i = 0; i < 3; i++) mean[i] /= n;
+  for (int i = 0; i < 3; i++) {
+    const float a = (vA[i] + (nA * mA[i]) * mA[i]) + vB[i];
+    const float s = a + (nB * mB[i]) * mB[i];
+    var[i] = s - (n * mean[i]) * mean[i];
+  }
+  for (int i = 0; i < 3; i++) { lo[i] = std::min(lo[i], b.lo[i]); hi[i] = std::max(hi[i], b.hi[i]); }
+}
+
+void HostSampleStats::split(int dim, float pos, float decay, bool splitLower) {  // SampleStatistics.h:68-99
+  if (n > 0.f) {
+    const float variance = var[dim] / n;
+    const float sd = std::sqrt(variance);
+    const float nv = variance - variance / 4.0f;
+    var[dim] = nv * n;
+    if (splitLower) {
+      lo[dim] = std::max(pos, lo[dim]);
+      mean[dim] = std::min(hi[dim], mean[dim] + sd / 2.0f);
+    } else {
+      hi[dim] = std::min(pos, hi[dim]);
+      mean[dim] = std::max(lo[dim], mean[dim] - sd / 2.0f);
+    }
+    n *= decay;
+    for (int i = 0; i < 3; i++) var[i] *= decay;
+  }
+}
+
+void HostSampleStats::variance(float v[3]) const {
+  for (int i = 0; i < 3; i++) v[i] = var[i] / n;
+}
+
+__attribute__((target("avx2,fma"))) float host_rcp(float x) {
+  const __m128 a = _mm_set_ss(x);
+  const __m128 r = _mm_rcp_ss(a);
+  return _mm_cvtss_f32(_mm_mul_ss(r, _mm_fnmadd_ss(r, a, _mm_set_ss(2.0f))));
+}
+
+// IntegerSampleStatistics::getSampleStatistics (SampleStatistics.h:297-340)
+HostSampleStats int_stats_to_float(const IntStatsAcc& a, const float center[3], const float half[3]) {
+  HostSampleStats s;
+  if (a.n > 0) {
+    const uint32_t n = (uint32_t)a.n;
+    const float invN = 1.f / float(n);
+    for (int i = 0; i < 3; i++) {
+      float m = (float(a.sum[i]) / 4096.f) * invN;
+      float v = (float(a.sq[i]) / 4096.f) * invN;
+      v = v - m * m;
+      v = std::fabs(v);
+      m = m * half[i];
+      m = m + center[i];
+      v = v * (half[i] * half[i]);
+      s.mean[i] = m;
+      s.var[i] = v * float(n);
+      float l = float(a.lo[i]) - 0.0f;
+      l = (l / 4096.f) * half[i] + center[i];
+      float u = float(a.hi[i]) + 0.0f;
+      u = (u / 4096.f) * half[i] + center[i];
+      s.lo[i] = l;
+      s.hi[i] = u;
+    }
+    s.n = float(n);
+  }
+  return s;
+}
+
+static void bounds_params(const float lo[3], const float hi[3], float center[3], float half[3], float inv[3]) {
+  for (int i = 0; i < 3; i++) {
+    const float ext = hi[i] - lo[i];
+    half[i] = ext * 0.5f;
+    inv[i] = host_rcp(half[i]);
+    center[i] = lo[i] + half[i];
+  }
+}
+
+// ------------------------------------------------------------------ DevBuf
+void DevBuf::ensure(size_t b, bool keep) {
+  if (b <= bytes) return;
+  size_t nb = std::max(b, bytes + bytes / 2);
+  void* np = nullptr;
+  PGL_HIP_CHECK(hipMalloc(&np, nb));
+  if (keep && p && bytes) PGL_HIP_CHECK(hipMemcpy(np, p, bytes, hipMemcpyDeviceToDevice));
+  if (p) (void)hipFree(p);
+  p = np;
+  bytes = nb;
+}
+void DevBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+// ------------------------------------------------------------ KindField
+static void uniform_directions(std::vector<float>& out) {  // WEM:668-700 spherical Fibonacci
+  out.assign(32 * 32 * 3, 0.f);
+  const float gr = 1.618033988749895f;
+  for (int l = 0; l < 32; l++)
+    for (int n = 0; n < 32; n++) {
+      float* d = &out[(l * 32 + n) * 3];
+      if (n < l + 1) {
+        const float phi = 2.0f * float(M_PI) * ((float)n / gr);
+        const float z = 1.0f - ((2.0f * n + 1.0f) / float(l + 1));
+        const float theta = std::acos(z);
+        d[0] = std::sin(theta) * std::cos(phi);
+        d[1] = std::sin(theta) * std::sin(phi);
+        d[2] = std::cos(theta);
+      } else {
+        d[0] = 0.f; d[1] = 0.f; d[2] = 1.f;
+      }
+    }
+}
+
+KindField::KindField(const KdSettings& kd, const FactoryConfig& fc, bool deterministic, int vectorWidth)
+    : kd_(kd), fc_(fc), deterministic_(deterministic), vectorWidth_(vectorWidth) {
+  for (int i = 0; i < 3; i++) { sceneLo_[i] = sceneHi_[i] = treeLo_[i] = treeHi_[i] = 0.f; }
+  for (auto& e : ev_) PGL_HIP_CHECK(hipEventCreate(&e));
+  std::vector<float> ud;
+  uniform_directions(ud);
+  dUniformDirs_.ensure(ud.size() * 4);
+  PGL_HIP_CHECK(hipMemcpy(dUniformDirs_.p, ud.data(), ud.size() * 4, hipMemcpyHostToDevice));
+}
+
+KindField::~KindField() {
+  for (auto& e : ev_) (void)hipEventDestroy(e);
+}
+
+void KindField::setSceneBounds(const float lo[3], const float hi[3]) {
+  for (int i = 0; i < 3; i++) { sceneLo_[i] = lo[i]; sceneHi_[i] = hi[i]; }
+  boundsSet_ = true;
+}
+
+void KindField::sceneBounds(float lo[3], float hi[3]) const {
+  for (int i = 0; i < 3; i++) { lo[i] = sceneLo_[i]; hi[i] = sceneHi_[i]; }
+}
+
+void KindField::reset() {  // Field::resetField (Field.h:222-232)
+  iteration_ = 0;
+  boundsSet_ = false;
+  initialized_ = false;
+  nodes_.clear();
+  regions_.clear();
+  hostStates_.clear();
+  mirrorDirty_ = true;
+}
+
+// Field::estimateSceneBounds: AABB over samples[0, n-2] (the container's end()
+// points at the last element), upper starting at FLT_MIN, scaled 3x about the centre.
+void KindField::estimateSceneBounds(const float* px, const float* py, const float* pz, uint32_t n, hipStream_t s) {
+  for (int i = 0; i < 3; i++) { sceneLo_[i] = kFltMax; sceneHi_[i] = std::numeric_limits<float>::min(); }
+  boundsSet_ = false;
+  if (n == 0) return;
+  const uint32_t m = n - 1;
+  if (m > 0) {
+    dPartial_.ensure(1024 * 6 * sizeof(float));
+    const uint32_t blocks = launch_bounds(px, py, pz, m, dPartial_.as<float>(), s);
+    std::vector<float> part(blocks * 6);
+    PGL_HIP_CHECK(hipMemcpyAsync(part.data(), dPartial_.p, blocks * 6 * sizeof(float), hipMemcpyDeviceToHost, s));
+    PGL_HIP_CHECK(hipStreamSynchronize(s));
+    for (uint32_t b = 0; b < blocks; b++)
+      for (int i = 0; i < 3; i++) {
+        sceneLo_[i] = std::min(sceneLo_[i], part[b * 6 + i]);
+        sceneHi_[i] = std::max(sceneHi_[i], part[b * 6 + 3 + i]);
+      }
+  }
+  for (int i = 0; i < 3; i++) {
+    const float c = (sceneLo_[i] + sceneHi_[i]) * 0.5f;
+    const float lo = c + 3.0f * (sceneLo_[i] - c);
+    const float hi = c + 3.0f * (sceneHi_[i] - c);
+    sceneLo_[i] = lo;
+    sceneHi_[i] = hi;
+  }
+  boundsSet_ = true;
+}
+
+struct FrontierItem {
+  uint32_t node, depth;
+  float lo[3], hi[3];
+};
+struct FrontierItemList {
+  std::vector<FrontierItem> items;
+};
+
+static uint8_t max_dim(const float v[3]) {  // KDTreeBuilder.h:223-226
+  const int a = v[1] > v[0];
+  return v[a] > v[2] ? uint8_t(a) : uint8_t(2);
+}
+
+static KDNode make_leaf(uint32_t data) { return KDNode{0.f, (3u << 30) | data}; }
+static KDNode make_inner(uint32_t dim, float pivot, uint32_t left) { return KDNode{pivot, (dim << 30) | left}; }
+
+// leaf visited with `count` new samples whose statistics are `st` (KDTreeBuilder.h:248-291)
+void KindField::visitLeaf(uint32_t node, uint32_t depth, const float lo[3], const float hi[3], uint32_t count,
+                          const HostSampleStats& st, FrontierItemList& next) {
+  const uint32_t data = nodes_[node].idx();
+  RegionHost& reg = regions_[data];
+  if (depth < kd_.maxDepth && reg.ss.n + float(count) > float(kd_.maxSamples)) {
+    HostSampleStats merged = reg.ss;
+    merged.merge(st);
+    float var[3];
+    merged.variance(var);
+    const uint8_t dim = max_dim(var);
+    const float pos = merged.mean[dim];
+    RegionHost right = reg;
+    reg.ss.split(dim, pos, 0.25f, false);
+    right.ss.split(dim, pos, 0.25f, true);
+    reg.splitFlag = true;
+    right.splitFlag = true;
+    reg.hi[dim] = pos;
+    right.lo[dim] = pos;
+    right.copyFrom = reg.copyFrom >= 0 ? reg.copyFrom : (int32_t)data;
+    right.count = 0;
+    regions_.push_back(right);
+    const uint32_t rightIdx = (uint32_t)regions_.size() - 1;
+    if (splitRegion_.size() < nodes_.size() + 2) splitRegion_.resize(nodes_.size() + 2, -1);
+    splitRegion_[node] = (int32_t)rightIdx;
+    const uint32_t left = (uint32_t)nodes_.size();
+    nodes_.push_back(make_leaf(data));
+    nodes_.push_back(make_leaf(rightIdx));
+    nodes_[node] = make_inner(dim, pos, left);
+    FrontierItem it;
+    it.node = node;
+    it.depth = depth;
+    for (int i = 0; i < 3; i++) { it.lo[i] = lo[i]; it.hi[i] = hi[i]; }
+    next.items.push_back(it);
+  } else {
+    reg.ss.merge(st);
+    reg.count = count;
+  }
+}
+
+void KindField::kdUpdate(uint32_t n, hipStream_t s) {
+  FrontierItemList frontier, next;
+  uint32_t rounds = 0;
+  dAcc_.ensure(2 * sizeof(IntStatsAcc));
+  if (nodes_[0].isLeaf()) {
+    FrontierNode root{};
+    float half[3];
+    bounds_params(treeLo_, treeHi_, root.center, half, root.invHalf);
+    launch_acc_init(dAcc_.as<IntStatsAcc>(), 1, s);
+    launch_root_stats(dPx_.as<float>(), dPy_.as<float>(), dPz_.as<float>(), n, root, dAcc_.as<IntStatsAcc>(), s);
+    IntStatsAcc acc;
+    PGL_HIP_CHECK(hipMemcpyAsync(&acc, dAcc_.p, sizeof(acc), hipMemcpyDeviceToHost, s));
+    PGL_HIP_CHECK(hipStreamSynchronize(s));
+    const HostSampleStats st = int_stats_to_float(acc, root.center, half);
+    visitLeaf(0, 1, treeLo_, treeHi_, n, st, frontier);
+  } else {
+    FrontierItem it;
+    it.node = 0;
+    it.depth = 1;
+    for (int i = 0; i < 3; i++) { it.lo[i] = treeLo_[i]; it.hi[i] = treeHi_[i]; }
+    frontier.items.push_back(it);
+  }
+  std::vector<FrontierNode> fn;
+  std::vector<int32_t> slot;
+  std::vector<IntStatsAcc> acc;
+  std::vector<float> halves;
+  while (!frontier.items.empty()) {
+    rounds++;
+    const uint32_t F = (uint32_t)frontier.items.size();
+    fn.resize(F);
+    halves.resize(3 * F);
+    slot.assign(nodes_.size(), -1);
+    for (uint32_t f = 0; f < F; f++) {
+      const FrontierItem& it = frontier.items[f];
+      const KDNode nd = nodes_[it.node];
+      fn[f].node = it.node;
+      fn[f].left = nd.idx();
+      fn[f].dim = nd.dim();
+      fn[f].pivot = nd.pivot;
+      bounds_params(it.lo, it.hi, fn[f].center, &halves[3 * f], fn[f].invHalf);
+      slot[it.node] = (int32_t)f;
+    }
+    dFrontier_.ensure(F * sizeof(FrontierNode));
+    dFrontierSlot_.ensure(slot.size() * sizeof(int32_t));
+    dAcc_.ensure(2 * F * sizeof(IntStatsAcc));
+    PGL_HIP_CHECK(hipMemcpyAsync(dFrontier_.p, fn.data(), F * sizeof(FrontierNode), hipMemcpyHostToDevice, s));
+    PGL_HIP_CHECK(hipMemcpyAsync(dFrontierSlot_.p, slot.data(), slot.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    launch_acc_init(dAcc_.as<IntStatsAcc>(), 2 * F, s);
+    launch_kd_round(dPx_.as<float>(), dPy_.as<float>(), dPz_.as<float>(), n, dNodeOf_.as<uint32_t>(),
+                    dFrontierSlot_.as<int32_t>(), dFrontier_.as<FrontierNode>(), F, dAcc_.as<IntStatsAcc>(), s);
+    acc.resize(2 * F);
+    PGL_HIP_CHECK(hipMemcpyAsync(acc.data(), dAcc_.p, 2 * F * sizeof(IntStatsAcc), hipMemcpyDeviceToHost, s));
+    PGL_HIP_CHECK(hipStreamSynchronize(s));
+    next.items.clear();
+    for (uint32_t f = 0; f < F; f++) {
+      const FrontierItem& it = frontier.items[f];
+      const FrontierNode& p = fn[f];
+      for (uint32_t side = 0; side < 2; side++) {
+        const IntStatsAcc& a = acc[2 * f + side];
+        if (a.n == 0) continue;  // empty range: updateTreeNode returns (KDTreeBuilder.h:235-238)
+        const uint32_t child = p.left + side;
+        float lo[3], hi[3];
+        for (int i = 0; i < 3; i++) { lo[i] = it.lo[i]; hi[i] = it.hi[i]; }
+        if (side == 0) hi[p.dim] = p.pivot; else lo[p.dim] = p.pivot;
+        if (nodes_[child].isLeaf()) {
+          const HostSampleStats st = int_stats_to_float(a, p.center, &halves[3 * f]);
+          visitLeaf(child, it.depth + 1, lo, hi, (uint32_t)a.n, st, next);
+        } else {
+          FrontierItem c;
+          c.node = child;
+          c.depth = it.depth + 1;
+          for (int i = 0; i < 3; i++) { c.lo[i] = lo[i]; c.hi[i] = hi[i]; }
+          next.items.push_back(c);
+        }
+      }
+    }
+    std::swap(frontier, next);
+  }
+  info_.kdRounds = rounds;
+}
+
+void KindField::ensureRegionCapacity(uint32_t r, hipStream_t s) {
+  if (r <= stateCapacity_) return;
+  uint32_t cap = std::max<uint32_t>(r, stateCapacity_ * 2);
+  cap = std::max<uint32_t>(cap, 64);
+  DevBuf nb;
+  nb.ensure(size_t(cap) * sizeof(RegionState));
+  PGL_HIP_CHECK(hipMemsetAsync(nb.p, 0, size_t(cap) * sizeof(RegionState), s));
+  if (stateCapacity_ > 0)
+    PGL_HIP_CHECK(hipMemcpyAsync(nb.p, dStates_.p, size_t(stateCapacity_) * sizeof(RegionState),
+                                 hipMemcpyDeviceToDevice, s));
+  PGL_HIP_CHECK(hipStreamSynchronize(s));
+  std::swap(dStates_.p, nb.p);
+  std::swap(dStates_.bytes, nb.bytes);
+  stateCapacity_ = cap;
+}
+
+// The serial reference recursion (KDTreeBuilder.h:346-349 with one thread)
+// creates child node pairs and right-hand regions in depth-first pre-order of
+// the split events.  The level-synchronous push-down creates them level by
+// level; renumber the ones created in this update into the reference order so
+// node and region indices (and therefore GetId) match the reference.
+void KindField::renumberDepthFirst(uint32_t nodesBefore, uint32_t regionsBefore) {
+  const uint32_t N = (uint32_t)nodes_.size(), R = (uint32_t)regions_.size();
+  if (N == nodesBefore) { keyNodes_ = nodes_; return; }
+  std::vector<uint32_t> nodeMap(N), regionMap(R);
+  for (uint32_t i = 0; i < N; i++) nodeMap[i] = i;
+  for (uint32_t i = 0; i < R; i++) regionMap[i] = i;
+  uint32_t nextNode = nodesBefore, nextRegion = regionsBefore;
+  splitRegion_.resize(N, -1);
+  std::vector<uint32_t> stack(1, 0u);
+  while (!stack.empty()) {
+    const uint32_t nd = stack.back();
+    stack.pop_back();
+    const KDNode& x = nodes_[nd];
+    if (x.isLeaf()) continue;
+    const uint32_t l = x.idx();
+    if (l >= nodesBefore) {
+      nodeMap[l] = nextNode;
+      nodeMap[l + 1] = nextNode + 1;
+      nextNode += 2;
+    }
+    if (splitRegion_[nd] >= 0) regionMap[(uint32_t)splitRegion_[nd]] = nextRegion++;
+    stack.push_back(l + 1);
+    stack.push_back(l);
+  }
+  // leaf-key view: push-down node numbering, final region ids
+  keyNodes_ = nodes_;
+  for (auto& k : keyNodes_)
+    if (k.isLeaf()) k.bits = (3u << 30) | regionMap[k.idx()];
+  std::vector<KDNode> nn(N);
+  for (uint32_t i = 0; i < N; i++) {
+    KDNode k = nodes_[i];
+    k.bits = k.isLeaf() ? ((3u << 30) | regionMap[k.idx()]) : ((k.bits & 0xC0000000u) | nodeMap[k.idx()]);
+    nn[nodeMap[i]] = k;
+  }
+  nodes_.swap(nn);
+  std::vector<RegionHost> nr(R);
+  for (uint32_t i = 0; i < R; i++) nr[regionMap[i]] = regions_[i];
+  regions_.swap(nr);
+}
+
+void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
+  if (n == 0) return;
+  const auto t0 = std::chrono::steady_clock::now();
+  info_ = UpdateInfo();
+  info_.samples = n;
+  dPx_.ensure(n * 4); dPy_.ensure(n * 4); dPz_.ensure(n * 4); dNodeOf_.ensure(n * 4);
+  dKeys_.ensure(n * 4); dVals_.ensure(n * 4); dKeys2_.ensure(n * 4); dVals2_.ensure(n * 4);
+  dDx_.ensure(n * 4); dDy_.ensure(n * 4); dDz_.ensure(n * 4); dW_.ensure(n * 4); dPdf_.ensure(n * 4); dDist_.ensure(n * 4);
+  const size_t sortBytes = sort_pairs_temp_bytes(n);
+  dSortTmp_.ensure(sortBytes);
+
+  PGL_HIP_CHECK(hipEventRecord(ev_[0], s));
+  launch_ingest(deviceSamples, n, dPx_.as<float>(), dPy_.as<float>(), dPz_.as<float>(), dNodeOf_.as<uint32_t>(), s);
+  const bool first = !initialized_;
+  if (first) {
+    if (!boundsSet_) estimateSceneBounds(dPx_.as<float>(), dPy_.as<float>(), dPz_.as<float>(), n, s);
+    for (int i = 0; i < 3; i++) { treeLo_[i] = sceneLo_[i]; treeHi_[i] = sceneHi_[i]; }
+    nodes_.assign(1, make_leaf(0));
+    regions_.assign(1, RegionHost());
+    for (int i = 0; i < 3; i++) { regions_[0].lo[i] = sceneLo_[i]; regions_[0].hi[i] = sceneHi_[i]; }
+    ensureRegionCapacity(1, s);
+    PGL_HIP_CHECK(hipMemsetAsync(dStates_.p, 0, sizeof(RegionState), s));
+  }
+  for (auto& r : regions_) { r.count = 0; r.copyFrom = -1; }
+  const uint32_t regionsBefore = (uint32_t)regions_.size();
+  const uint32_t nodesBefore = first ? 1u : (uint32_t)nodes_.size();
+  splitRegion_.assign(nodes_.size(), -1);
+  kdUpdate(n, s);
+  renumberDepthFirst(nodesBefore, regionsBefore);
+  const uint32_t R = (uint32_t)regions_.size();
+  PGL_HIP_CHECK(hipEventRecord(ev_[1], s));
+
+  // region-contiguous order: stable radix sort of (region, sample index)
+  dNodes_.ensure(nodes_.size() * sizeof(KDNode));
+  PGL_HIP_CHECK(hipMemcpyAsync(dNodes_.p, keyNodes_.data(), keyNodes_.size() * sizeof(KDNode), hipMemcpyHostToDevice, s));
+  launch_leaf_keys(dNodeOf_.as<uint32_t>(), dNodes_.as<KDNode>(), n, dKeys_.as<uint32_t>(), dVals_.as<uint32_t>(), s);
+  // the query kernels walk the tree in the final numbering
+  PGL_HIP_CHECK(hipStreamSynchronize(s));
+  PGL_HIP_CHECK(hipMemcpyAsync(dNodes_.p, nodes_.data(), nodes_.size() * sizeof(KDNode), hipMemcpyHostToDevice, s));
+  int endBit = 1;
+  while ((1u << endBit) < R && endBit < 32) endBit++;
+  sort_pairs(dSortTmp_.p, sortBytes, dKeys_.as<uint32_t>(), dKeys2_.as<uint32_t>(), dVals_.as<uint32_t>(),
+             dVals2_.as<uint32_t>(), n, endBit, s);
+  PGL_HIP_CHECK(hipEventRecord(ev_[2], s));
+
+  // directional state of regions created by spatial splits: copies of their ancestor
+  ensureRegionCapacity(R, s);
+  std::vector<uint32_t> copies;
+  for (uint32_t r = regionsBefore; r < R; r++)
+    if (regions_[r].copyFrom >= 0) { copies.push_back(r); copies.push_back((uint32_t)regions_[r].copyFrom); }
+  // per-region inputs
+  typedef RegionInput Input;
+  std::vector<Input> inputs(R);
+  std::vector<float4> pivots(R);
+  std::vector<std::pair<uint32_t, uint32_t>> work;
+  uint32_t begin = 0;
+  for (uint32_t r = 0; r < R; r++) {
+    const RegionHost& rg = regions_[r];
+    Input& in = inputs[r];
+    in.region = r;
+    for (int i = 0; i < 3; i++) { in.ssMean[i] = rg.ss.mean[i]; in.ssVar[i] = rg.ss.var[i]; }
+    in.ssN = rg.ss.n;
+    in.begin = begin;
+    in.count = rg.count;
+    in.mode = first ? 1 : 2;
+    in.splitFlag = rg.splitFlag ? 1 : 0;
+    begin += rg.count;
+    // prepareSamples minimum distance (WEM:1184-1196): 9 sqrt(|variance|), |.| with fused dot
+    float var[3];
+    rg.ss.variance(var);
+    float md = std::sqrt(std::fmaf(var[0], var[0], std::fmaf(var[1], var[1], var[2] * var[2])));
+    md = 3.f * 3.f * std::sqrt(md);
+    pivots[r] = make_float4(rg.ss.mean[0], rg.ss.mean[1], rg.ss.mean[2], md);
+    if (first || rg.count > 0 || rg.splitFlag) work.push_back({rg.count, r});
+  }
+  if (begin != n) throw std::runtime_error("kd update lost samples");
+  dCopyPairs_.ensure(std::max<size_t>(copies.size(), 2) * 4);
+  dInputs_.ensure(std::max<size_t>(R, 1) * sizeof(Input));
+  dPivots_.ensure(std::max<size_t>(R, 1) * sizeof(float4));
+  if (!copies.empty()) {
+    PGL_HIP_CHECK(hipMemcpyAsync(dCopyPairs_.p, copies.data(), copies.size() * 4, hipMemcpyHostToDevice, s));
+    launch_copy_states(dStates_.as<RegionState>(), dCopyPairs_.as<uint32_t>(), (uint32_t)(copies.size() / 2), s);
+  }
+  PGL_HIP_CHECK(hipMemcpyAsync(dInputs_.p, inputs.data(), R * sizeof(Input), hipMemcpyHostToDevice, s));
+  launch_set_inputs(dStates_.as<RegionState>(), dInputs_.as<RegionInput>(), R, s);
+  PGL_HIP_CHECK(hipMemcpyAsync(dPivots_.p, pivots.data(), R * sizeof(float4), hipMemcpyHostToDevice, s));
+  launch_gather_reproject(deviceSamples, dKeys2_.as<uint32_t>(), dVals2_.as<uint32_t>(), n, dPivots_.as<float4>(),
+                          dDx_.as<float>(), dDy_.as<float>(), dDz_.as<float>(), dW_.as<float>(), dPdf_.as<float>(),
+                          dDist_.as<float>(), s);
+  PGL_HIP_CHECK(hipEventRecord(ev_[3], s));
+
+  // directional update: largest regions first
+  std::stable_sort(work.begin(), work.end(), [](const std::pair<uint32_t, uint32_t>& a, const std::pair<uint32_t, uint32_t>& b) {
+    return a.first > b.first;
+  });
+  std::vector<uint32_t> list(work.size());
+  for (size_t i = 0; i < work.size(); i++) list[i] = work[i].second;
+  dRegionList_.ensure(std::max<size_t>(list.size(), 1) * 4);
+  PGL_HIP_CHECK(hipMemcpyAsync(dRegionList_.p, list.data(), list.size() * 4, hipMemcpyHostToDevice, s));
+  if (schedule_ == 1) {
+    for (size_t i = 0; i < list.size(); i++)
+      launch_region_update(dStates_.as<RegionState>(), dRegionList_.as<uint32_t>() + i, 1, dDx_.as<float>(),
+                           dDy_.as<float>(), dDz_.as<float>(), dW_.as<float>(), dPdf_.as<float>(), dDist_.as<float>(),
+                           fc_, dUniformDirs_.as<float>(), s);
+    info_.regionLaunches = (double)list.size();
+  } else {
+    launch_region_update(dStates_.as<RegionState>(), dRegionList_.as<uint32_t>(), (uint32_t)list.size(),
+                         dDx_.as<float>(), dDy_.as<float>(), dDz_.as<float>(), dW_.as<float>(), dPdf_.as<float>(),
+                         dDist_.as<float>(), fc_, dUniformDirs_.as<float>(), s);
+    info_.regionLaunches = list.empty() ? 0 : 1;
+  }
+  PGL_HIP_CHECK(hipEventRecord(ev_[4], s));
+
+  // KNN neighbourhoods of the region pivots (sample-statistics means)
+  if (kd_.knnLookup) {
+    std::vector<float4> piv(R);
+    for (uint32_t r = 0; r < R; r++) piv[r] = make_float4(regions_[r].ss.mean[0], regions_[r].ss.mean[1], regions_[r].ss.mean[2], 0.f);
+    dNbIds_.ensure(size_t(R) * 8 * 4);
+    dNbPts_.ensure(size_t(R) * 8 * sizeof(float4));
+    DevBuf& tmp = dPartial_;
+    tmp.ensure(size_t(R) * sizeof(float4));
+    PGL_HIP_CHECK(hipMemcpyAsync(tmp.p, piv.data(), R * sizeof(float4), hipMemcpyHostToDevice, s));
+    launch_knn8(tmp.as<float4>(), R, dNbIds_.as<uint32_t>(), dNbPts_.as<float4>(), s);
+  }
+  PGL_HIP_CHECK(hipEventRecord(ev_[5], s));
+  PGL_HIP_CHECK(hipStreamSynchronize(s));
+  for (auto& r : regions_) { r.splitFlag = false; r.copyFrom = -1; }
+  initialized_ = true;
+  iteration_++;
+  mirrorDirty_ = true;
+
+  float ms = 0.f;
+  PGL_HIP_CHECK(hipEventElapsedTime(&ms, ev_[0], ev_[1])); info_.spatialMs = ms;
+  PGL_HIP_CHECK(hipEventElapsedTime(&ms, ev_[1], ev_[2])); info_.sortMs = ms;
+  PGL_HIP_CHECK(hipEventElapsedTime(&ms, ev_[2], ev_[3])); info_.gatherMs = ms;
+  PGL_HIP_CHECK(hipEventElapsedTime(&ms, ev_[3], ev_[4])); info_.directionalMs = ms;
+  PGL_HIP_CHECK(hipEventElapsedTime(&ms, ev_[4], ev_[5])); info_.knnMs = ms;
+  info_.wallMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void KindField::syncMirror(hipStream_t s) const {
+  std::lock_guard<std::mutex> lock(mirrorMutex_);
+  if (!mirrorDirty_) return;
+  const uint32_t R = (uint32_t)regions_.size();
+  hostStates_.resize(R);
+  if (R) PGL_HIP_CHECK(hipMemcpyAsync(hostStates_.data(), dStates_.p, size_t(R) * sizeof(RegionState), hipMemcpyDeviceToHost, s));
+  if (kd_.knnLookup && R) {
+    hostNbIds_.resize(size_t(R) * 8);
+    hostNbPts_.resize(size_t(R) * 8);
+    PGL_HIP_CHECK(hipMemcpyAsync(hostNbIds_.data(), dNbIds_.p, size_t(R) * 8 * 4, hipMemcpyDeviceToHost, s));
+    PGL_HIP_CHECK(hipMemcpyAsync(hostNbPts_.data(), dNbPts_.p, size_t(R) * 8 * sizeof(float4), hipMemcpyDeviceToHost, s));
+  }
+  PGL_HIP_CHECK(hipStreamSynchronize(s));
+  mirrorDirty_ = false;
+}
+
+int KindField::lookupRegion(const float p[3], float* u, uint32_t& id) const {
+  if (iteration_ == 0 || !initialized_) return -1;
+  for (int j = 0; j < 3; j++)
+    if (!(p[j] >= treeLo_[j] && p[j] <= treeHi_[j])) return -1;
+  uint32_t i = 0;
+  while (!nodes_[i].isLeaf()) {
+    const uint32_t d = nodes_[i].dim();
+    i = nodes_[i].idx() + (p[d] >= nodes_[i].pivot ? 1u : 0u);
+  }
+  const uint32_t leaf = nodes_[i].idx();
+  id = leaf;
+  if (kd_.knnLookup && *u >= 0.f) {
+    uint32_t ids[8];
+    float px[8], py[8], pz[8];
+    for (int k = 0; k < 8; k++) {
+      ids[k] = hostNbIds_[leaf * 8 + k];
+      px[k] = hostNbPts_[leaf * 8 + k].x;
+      py[k] = hostNbPts_[leaf * 8 + k].y;
+      pz[k] = hostNbPts_[leaf * 8 + k].z;
+    }
+    const uint32_t nbSize = std::min<uint32_t>(8, (uint32_t)regions_.size());
+    return (int)knn_pick(ids, px, py, pz, nbSize, p[0], p[1], p[2], u);
+  }
+  return (int)leaf;
+}
+
+QueryField KindField::queryView() const {
+  QueryField q;
+  q.nodes = dNodes_.as<KDNode>();
+  q.regions = dStates_.as<RegionState>();
+  q.nbIds = dNbIds_.as<uint32_t>();
+  q.nbPts = dNbPts_.as<float4>();
+  q.nbSize = std::min<uint32_t>(8, (uint32_t)regions_.size());
+  q.knn = kd_.knnLookup ? 1 : 0;
+  q.trained = (initialized_ && iteration_ > 0) ? 1 : 0;
+  for (int i = 0; i < 3; i++) { q.lo[i] = treeLo_[i]; q.hi[i] = treeHi_[i]; }
+  return q;
+}
+
+void KindField::canonicalOrder(std::vector<KDNode>& out, std::vector<uint32_t>& order) const {
+  out.clear();
+  order.clear();
+  if (nodes_.empty()) return;
+  // iterative depth-first traversal with the reference's numbering (KDTree.h:256-283)
+  out.push_back(nodes_[0]);
+  struct Item { uint32_t src, dst; };
+  std::vector<Item> stack;
+  stack.push_back({0, 0});
+  while (!stack.empty()) {
+    const Item it = stack.back();
+    stack.pop_back();
+    const KDNode n = nodes_[it.src];
+    if (!n.isLeaf()) {
+      const uint32_t l = n.idx();
+      const uint32_t ni = (uint32_t)out.size();
+      out[it.dst].bits = (out[it.dst].bits & 0xC0000000u) | ni;
+      out.push_back(nodes_[l]);
+      out.push_back(nodes_[l + 1]);
+      stack.push_back({l + 1, ni + 1});
+      stack.push_back({l, ni});
+    } else {
+      out[it.dst] = make_leaf((uint32_t)order.size());
+      order.push_back(n.idx());
+    }
+  }
+}
+
+void KindField::exportRegion(uint32_t r, PGLRegionRecordHost* o) const {
+  const RegionState& S = hostStates_[r];
+  const RegionHost& h = regions_[r];
+  memset(o, 0, sizeof(*o));
+  o->K = S.vmm.K;
+  for (int k = 0; k < 32; k++) {
+    o->weights[k] = S.vmm.w[k]; o->kappas[k] = S.vmm.kappa[k];
+    o->meanDirections[k][0] = S.vmm.mux[k]; o->meanDirections[k][1] = S.vmm.muy[k]; o->meanDirections[k][2] = S.vmm.muz[k];
+    o->normalizations[k] = S.vmm.norm[k]; o->eMinus2Kappa[k] = S.vmm.em2k[k]; o->meanCosines[k] = S.vmm.mc[k];
+    o->distances[k] = S.vmm.dist[k];
+    o->sumOfWeightedDirections[k][0] = S.suff.swdx[k]; o->sumOfWeightedDirections[k][1] = S.suff.swdy[k];
+    o->sumOfWeightedDirections[k][2] = S.suff.swdz[k];
+    o->sumOfWeightedStats[k] = S.suff.sw[k]; o->sumOfDistanceWeights[k] = S.suff.sdw[k];
+    o->chiSquareMCEstimates[k] = S.split.chi[k];
+    o->splitCovariances[k][0] = S.split.covx[k]; o->splitCovariances[k][1] = S.split.covy[k];
+    o->splitCovariances[k][2] = S.split.covz[k];
+    o->splitNumSamples[k] = S.split.sn[k]; o->splitSumWeights[k] = S.split.ssw[k]; o->splitSumAssignedSamples[k] = S.split.ssa[k];
+  }
+  for (int j = 0; j < 3; j++) o->pivot[j] = S.vmm.pivot[j];
+  o->sumWeights = S.suff.sumWeights; o->numSamples = S.suff.numSamples; o->overallNumSamples = S.suff.overall;
+  o->suffK = S.suff.K; o->normalized = S.suff.normalized; o->splitK = S.split.K;
+  o->numSamplesAfterLastSplit = ((uint64_t)S.afterSplitHi << 32) | S.afterSplitLo;
+  o->numSamplesAfterLastMerge = ((uint64_t)S.afterMergeHi << 32) | S.afterMergeLo;
+  for (int j = 0; j < 3; j++) {
+    o->ssMean[j] = h.ss.mean[j]; o->ssVariance[j] = h.ss.var[j]; o->ssLower[j] = h.ss.lo[j]; o->ssUpper[j] = h.ss.hi[j];
+    o->regionLower[j] = h.lo[j]; o->regionUpper[j] = h.hi[j];
+  }
+  o->ssNumSamples = h.ss.n;
+  o->splitFlag = h.splitFlag; o->valid = S.valid;
+  o->numSamplesThisUpdate = S.sampleCount;
+  o->emIterations = S.emIterations; o->partialEmIterations = S.partialEmIterations;
+  o->numSplits = S.numSplits; o->numMerges = S.numMerges;
+}
+
+void KindField::knnHost(std::vector<uint32_t>& ids) const { ids = hostNbIds_; }
+
+bool KindField::validate() const {  // Field::isValid (Field.h:307-317)
+  for (uint32_t r = 0; r < (uint32_t)regions_.size(); r++) {
+    if (!hostStates_[r].valid) return false;
+  }
+  return true;
+}
+
+GuidingField::GuidingField(const KdSettings& kd, const FactoryConfig& fc, bool deterministic, int vectorWidth)
+    : surface(kd, fc, deterministic, vectorWidth), volume(kd, fc, deterministic, vectorWidth) {
+  PGL_HIP_CHECK(hipStreamCreate(&stream));
+  dHG.ensure(128 * sizeof(HGRep));
+  PGL_HIP_CHECK(hipMemcpy(dHG.p, host_hg_table(), 128 * sizeof(HGRep), hipMemcpyHostToDevice));
+}
+
+static const float kHGTable[128 * 18] = {
+#include "hg_table.inc"
+};
+
+const HGRep* host_hg_table() {
+  static_assert(sizeof(HGRep) == 18 * sizeof(float), "HGRep row is 18 floats");
+  return reinterpret_cast<const HGRep*>(kHGTable);
+}
+
+}  // namespace pglgpu

@@ -1,0 +1,174 @@
+// Host side of the MI355X guiding field: configuration, the host copy of the
+// kd tree and of the per-region float sample statistics (the bit-exact part of
+// the spatial decisions), device buffers and the update pipeline.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "kernels.h"
+#include "query_math.h"
+
+namespace pglgpu {
+
+#define PGL_HIP_CHECK(expr)                                                                          \
+  do {                                                                                               \
+    hipError_t e_ = (expr);                                                                          \
+    if (e_ != hipSuccess)                                                                            \
+      throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(e_) + " at " + __FILE__ + \
+                               ":" + std::to_string(__LINE__));                                      \
+  } while (0)
+
+// SampleStatistics (data/SampleStatistics.h:12-193), float, evaluated without contraction.
+struct HostSampleStats {
+  float mean[3] = {0.f, 0.f, 0.f};
+  float var[3] = {0.f, 0.f, 0.f};
+  float n = 0.f;
+  float lo[3] = {3.402823466e38f, 3.402823466e38f, 3.402823466e38f};
+  float hi[3] = {-3.402823466e38f, -3.402823466e38f, -3.402823466e38f};
+  void merge(const HostSampleStats& b);
+  void split(int dim, float pos, float decay, bool splitLower);
+  void variance(float v[3]) const;
+};
+
+float host_rcp(float x);  // embree::rcp under AVX2: rcpss + fused Newton step (emath.h:61-88)
+HostSampleStats int_stats_to_float(const IntStatsAcc& a, const float center[3], const float half[3]);
+
+struct KdSettings {
+  uint32_t maxSamples = 32000;
+  uint32_t maxDepth = 32;
+  uint32_t minSamples = 100;
+  bool knnLookup = true;
+};
+
+// Growable device allocation.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void ensure(size_t b, bool keep = false);
+  void release();
+  ~DevBuf() { release(); }
+  template <class T> T* as() const { return (T*)p; }
+};
+
+struct UpdateInfo {
+  double wallMs = 0, spatialMs = 0, directionalMs = 0, knnMs = 0, gatherMs = 0, sortMs = 0;
+  double kdRounds = 0, regionLaunches = 0, estepEvals = 0, auxEvals = 0, emIterations = 0, samples = 0;
+};
+
+// One guiding field (surface or volume): Field<> of field/Field.h.
+class KindField {
+ public:
+  KindField(const KdSettings& kd, const FactoryConfig& fc, bool deterministic, int vectorWidth);
+  ~KindField();
+
+  void setSceneBounds(const float lo[3], const float hi[3]);
+  void update(const void* deviceSamples, uint32_t n, hipStream_t s);
+  void reset();
+
+  // host mirror of the device state for scalar queries and exports
+  void syncMirror(hipStream_t s) const;
+  const RegionState& mirrorRegion(uint32_t r) const { return hostStates_[r]; }
+  uint32_t numRegions() const { return (uint32_t)regions_.size(); }
+  const std::vector<KDNode>& nodes() const { return nodes_; }
+  bool initialized() const { return initialized_; }
+  uint32_t iteration() const { return iteration_; }
+  bool sceneBoundsSet() const { return boundsSet_; }
+  void sceneBounds(float lo[3], float hi[3]) const;
+  // Field::getRegion (field/Field.h:109-145) on the host mirror; -1 = none
+  int lookupRegion(const float p[3], float* u, uint32_t& id) const;
+  QueryField queryView() const;
+  void canonicalOrder(std::vector<KDNode>& nodesOut, std::vector<uint32_t>& regionOrder) const;
+  void exportRegion(uint32_t storageIdx, struct PGLRegionRecordHost* out) const;
+  void knnHost(std::vector<uint32_t>& ids) const;
+  const UpdateInfo& lastInfo() const { return info_; }
+  void setSchedule(int s) { schedule_ = s; }
+  bool validate() const;
+
+  struct RegionHost {
+    HostSampleStats ss;
+    float lo[3], hi[3];
+    bool splitFlag = false;
+    uint32_t count = 0;       // samples of this update
+    int32_t copyFrom = -1;    // device state to clone (new regions of this update)
+  };
+
+ private:
+  void estimateSceneBounds(const float* px, const float* py, const float* pz, uint32_t n, hipStream_t s);
+  void kdUpdate(uint32_t n, hipStream_t s);
+  void visitLeaf(uint32_t node, uint32_t depth, const float lo[3], const float hi[3], uint32_t count,
+                 const HostSampleStats& st, struct FrontierItemList& next);
+  void ensureRegionCapacity(uint32_t r, hipStream_t s);
+  void renumberDepthFirst(uint32_t nodesBefore, uint32_t regionsBefore);
+
+  KdSettings kd_;
+  FactoryConfig fc_;
+  bool deterministic_;
+  int vectorWidth_;
+  int schedule_ = 0;
+
+  std::vector<KDNode> nodes_;
+  std::vector<RegionHost> regions_;
+  std::vector<int32_t> splitRegion_;  // per node: region created when it split in this update (-1 = none)
+  std::vector<KDNode> keyNodes_;      // node array of the push-down numbering with final region ids
+  float sceneLo_[3], sceneHi_[3];
+  bool boundsSet_ = false;
+  float treeLo_[3], treeHi_[3];
+  bool initialized_ = false;
+  uint32_t iteration_ = 0;
+
+  // device buffers
+  DevBuf dPx_, dPy_, dPz_, dNodeOf_, dKeys_, dVals_, dKeys2_, dVals2_, dSortTmp_;
+  DevBuf dDx_, dDy_, dDz_, dW_, dPdf_, dDist_;
+  DevBuf dAos_;
+  DevBuf dNodes_, dFrontier_, dFrontierSlot_, dAcc_, dPartial_;
+  DevBuf dStates_, dRegionList_, dPivots_, dNbIds_, dNbPts_, dUniformDirs_, dCopyPairs_, dInputs_;
+  uint32_t stateCapacity_ = 0;
+
+  mutable std::vector<RegionState> hostStates_;
+  mutable std::vector<uint32_t> hostNbIds_;
+  mutable std::vector<float4> hostNbPts_;
+  mutable bool mirrorDirty_ = true;
+  mutable std::mutex mirrorMutex_;
+  UpdateInfo info_;
+  hipEvent_t ev_[8];
+};
+
+// Region record layout of the EXTENSION export (mirrors PGLRegionRecord).
+struct PGLRegionRecordHost {
+  uint32_t K;
+  float weights[32], kappas[32], meanDirections[32][3], normalizations[32], eMinus2Kappa[32], meanCosines[32],
+      distances[32];
+  float pivot[3];
+  float sumOfWeightedDirections[32][3], sumOfWeightedStats[32], sumOfDistanceWeights[32];
+  float sumWeights, numSamples, overallNumSamples;
+  uint32_t suffK, normalized;
+  float chiSquareMCEstimates[32], splitMeans[32][2], splitCovariances[32][3], splitNumSamples[32],
+      splitSumWeights[32], splitSumAssignedSamples[32];
+  uint32_t splitK;
+  uint64_t numSamplesAfterLastSplit, numSamplesAfterLastMerge;
+  float ssMean[3], ssVariance[3], ssNumSamples, ssLower[3], ssUpper[3];
+  float regionLower[3], regionUpper[3];
+  uint32_t splitFlag, valid;
+  uint32_t numSamplesThisUpdate;
+  uint32_t emIterations, partialEmIterations, numSplits, numMerges;
+};
+
+// SurfaceVolumeField (field/SurfaceVolumeField.h): the surface and the volume field.
+class GuidingField {
+ public:
+  GuidingField(const KdSettings& kd, const FactoryConfig& fc, bool deterministic, int vectorWidth);
+  KindField surface, volume;
+  uint32_t iteration = 0;
+  hipStream_t stream = nullptr;
+  DevBuf dHG;
+  DevBuf dQuery;  // staging for the host-pointer batch API
+};
+
+const HGRep* host_hg_table();
+
+}  // namespace pglgpu

@@ -1,0 +1,58 @@
+// Launchers of the gfx950 kernels (implemented in the *.hip files).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "common.h"
+
+namespace pglgpu {
+
+struct HGRep;
+
+// Device view of one trained field used by the query kernel.
+struct QueryField {
+  const KDNode* nodes;
+  const RegionState* regions;
+  const uint32_t* nbIds;
+  const float4* nbPts;
+  uint32_t nbSize;
+  int32_t knn;
+  int32_t trained;
+  float lo[3], hi[3];
+};
+
+// spatial
+void launch_ingest(const void* aos, uint32_t n, float* px, float* py, float* pz, uint32_t* nodeOf, hipStream_t s);
+uint32_t launch_bounds(const float* px, const float* py, const float* pz, uint32_t n, float* partial, hipStream_t s);
+void launch_acc_init(IntStatsAcc* a, uint32_t count, hipStream_t s);
+void launch_root_stats(const float* px, const float* py, const float* pz, uint32_t n, const FrontierNode& root,
+                       IntStatsAcc* out, hipStream_t s);
+void launch_kd_round(const float* px, const float* py, const float* pz, uint32_t n, uint32_t* nodeOf,
+                     const int32_t* frontierSlot, const FrontierNode* frontier, uint32_t numFrontier,
+                     IntStatsAcc* childAcc, hipStream_t s);
+void launch_leaf_keys(const uint32_t* nodeOf, const KDNode* nodes, uint32_t n, uint32_t* keys, uint32_t* vals,
+                      hipStream_t s);
+size_t sort_pairs_temp_bytes(uint32_t n);
+void sort_pairs(void* temp, size_t tempBytes, const uint32_t* keysIn, uint32_t* keysOut, const uint32_t* valsIn,
+                uint32_t* valsOut, uint32_t n, int endBit, hipStream_t s);
+void launch_region_ranges(const uint32_t* sortedKeys, uint32_t n, uint32_t* begin, uint32_t* end, hipStream_t s);
+void launch_gather_reproject(const void* aos, const uint32_t* sortedKeys, const uint32_t* sortedIdx, uint32_t n,
+                             const float4* regionPivot, float* dx, float* dy, float* dz, float* w, float* pdf,
+                             float* dist, hipStream_t s);
+void launch_knn8(const float4* pivots, uint32_t numRegions, uint32_t* nbIds, float4* nbPts, hipStream_t s);
+
+// directional
+void launch_region_update(RegionState* states, const uint32_t* regionList, uint32_t numRegions, const float* dx,
+                          const float* dy, const float* dz, const float* w, const float* pdf, const float* dist,
+                          const FactoryConfig& cfg, const float* uniformDirs, hipStream_t s);
+
+void launch_copy_states(RegionState* states, const uint32_t* pairs /* dst, src */, uint32_t numPairs, hipStream_t s);
+void launch_set_inputs(RegionState* states, const RegionInput* inputs, uint32_t count, hipStream_t s);
+
+// query
+void launch_query(const QueryField& surf, const QueryField& vol, const HGRep* hg, uint32_t n, const float* pos,
+                  const float* nrm, const float* g, const float* u2, const float* u1, const uint8_t* isVolume,
+                  float* dirOut, float* pdfOut, uint32_t* idOut, float* integralOut, hipStream_t s);
+
+}  // namespace pglgpu

@@ -1,0 +1,1253 @@
+// Directional training of Field::Update on gfx950: one 256-thread workgroup
+// owns one region and runs the whole adaptive split-and-merge update for it
+// on the device — weighted EM (E-step over the region's sample stream, MAP
+// M-step), chi-square split statistics, recursive/partial splitting with
+// partial re-fit, pairwise chi-square merging and the harmonic-mean parallax
+// distances — with no host round trip.  All regions of an update run in one
+// launch (grid = regions, largest first).
+//
+// Work layout: a quad of 4 lanes evaluates one sample; lane q of the quad owns
+// mixture components k = 4j + q (j < 8), so a wave evaluates 16 samples x 32
+// components per step with the component parameters held in registers.  The
+// per-sample pdf is a 2-step quad reduction; the weighted sufficient
+// statistics are accumulated per lane in FP32 blocks of 8 samples folded into
+// Kahan-compensated totals, then reduced over quads (xor-shuffles) and waves
+// (fixed order in LDS).  Per-component M-step / split / merge logic runs in
+// wave 0 with one lane per component, serial decisions in lane 0.
+// The mean-cosine -> kappa inversion and the log-likelihood convergence test
+// are evaluated in FP64; everything else is FP32.
+//
+// Reference behaviour followed (directional/vmm/, Open PGL 0.6.0):
+//   AdaptiveSplitandMergeFactory.h          fit 303-347, update 352-462
+//   ParallaxAwareVonMisesFisherWeightedEMFactory.h
+//     SufficientStatistics 93-593, InitUniformVMM 632-700, updateMixture 751-795,
+//     partialUpdateMixture 798-844, weightedExpectationStep 847-892, MAP 895-999,
+//     partial MAP 1015-1148, component distances 1199-1326
+//   ParallaxAwareVonMisesFisherMixture.h    softAssignment 697-733, split/merge 165-289,
+//     performRelativeParallaxShift 809-831, normalisation 852-876, kappa maps 910-928
+//   VMMChiSquareComponentSplitter.h         PerformRecursiveSplitting 293-356,
+//     UpdateSplitStatistics 439-523, SplitComponent 527-665, mergeComponentStats 994-1100
+//   VMMChiSquareComponentMerger.h           PerformMerging 67-89, CalculateMergeCost 93-177,
+//     ThresholdedMergeNext 218-279, _IntegratedDivision/_Product 338-396
+//   field/Field.h                           updateRegions 441-511 (decay, parallax shift)
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+#include "kernels.h"
+
+namespace pglgpu {
+
+#define RT 256
+#define RWAVES 4
+static const float kTwoPi = 6.283185307179586f;
+
+struct RegionSamples {
+  const float* __restrict__ dx;
+  const float* __restrict__ dy;
+  const float* __restrict__ dz;
+  const float* __restrict__ w;
+  const float* __restrict__ pdf;
+  const float* __restrict__ dist;
+};
+
+struct Shared {
+  RegionState S;
+  SuffStats cur;
+  SuffStats tmp;
+  SplitStatsState lsplit;
+  float red[RWAVES][32][8];
+  float fxx[32], fxy[32], fxz[32], fyx[32], fyy[32], fyz[32];
+  float cost[32 * 32];
+  float scal[RWAVES][8];
+  double ll;
+  double prevLL, invPrevLL;
+  float un[4];
+  float nAssigned;
+  uint32_t mask;
+  int32_t converged;
+  int32_t numSplits;
+  int32_t found, ci, cj;
+  int32_t order[32];
+  int32_t W;  // SIMD width of the emulated reference device
+};
+
+// ----------------------------------------------------------------- helpers
+__device__ __forceinline__ float dot3f(float ax, float ay, float az, float bx, float by, float bz) {
+  return fmaf(ax, bx, fmaf(ay, by, az * bz));
+}
+
+// full-wave sum with the result of lane 0 broadcast (identical on all lanes)
+__device__ __forceinline__ float wave_sum(float v) {
+  v += __shfl_xor(v, 32);
+  v += __shfl_xor(v, 16);
+  v += __shfl_xor(v, 8);
+  v += __shfl_xor(v, 4);
+  v += __shfl_xor(v, 2);
+  v += __shfl_xor(v, 1);
+  return __shfl(v, 0);
+}
+
+__device__ __forceinline__ float mc_to_kappa(float r) {  // (3r - r^3) / (1 - r^2) in FP64
+  const double rd = r;
+  const double r2 = rd * rd;
+  return (float)((rd * 3.0 - rd * r2) / (1.0 - r2));
+}
+
+__device__ __forceinline__ float kappa_to_mc(float kappa) {  // coth(k) - 1/k
+  if (!(kappa > 0.f)) return 0.f;
+  return 1.f / tanhf(kappa) - 1.f / kappa;
+}
+
+// embree::frame(N): x axis from the larger of (0,Nz,-Ny) / (-Nz,0,Nx), y = N x x
+__device__ __forceinline__ void frame_of(float nx, float ny, float nz, float& xx, float& xy, float& xz, float& yx,
+                                         float& yy, float& yz) {
+  const float a0x = 0.f, a0y = nz, a0z = -ny;
+  const float a1x = -nz, a1y = 0.f, a1z = nx;
+  const bool first = dot3f(a0x, a0y, a0z, a0x, a0y, a0z) > dot3f(a1x, a1y, a1z, a1x, a1y, a1z);
+  float dx = first ? a0x : a1x, dy = first ? a0y : a1y, dz = first ? a0z : a1z;
+  float l = 1.f / sqrtf(dot3f(dx, dy, dz, dx, dy, dz));
+  xx = dx * l;
+  xy = dy * l;
+  xz = dz * l;
+  const float cx = ny * xz - nz * xy, cy = nz * xx - nx * xz, cz = nx * xy - ny * xx;
+  l = 1.f / sqrtf(dot3f(cx, cy, cz, cx, cy, cz));
+  yx = cx * l;
+  yy = cy * l;
+  yz = cz * l;
+}
+
+__device__ void calc_normalization_lane(VMMParams& V, int k) {  // PAVMM:852-863
+  const float e = expf(-2.f * V.kappa[k]);
+  V.em2k[k] = e;
+  V.norm[k] = V.kappa[k] > 0.f ? V.kappa[k] / (kTwoPi * (1.f - e)) : kOneOverFourPi;
+}
+
+__device__ void clear_comp(VMMParams& V, int k) {  // PAVMM:276-289
+  V.w[k] = 0.f; V.kappa[k] = 0.f; V.em2k[k] = 1.f; V.mc[k] = 0.f; V.norm[k] = kOneOverFourPi;
+  V.mux[k] = 0.f; V.muy[k] = 0.f; V.muz[k] = 1.f; V.dist[k] = 0.f;
+}
+
+__device__ __forceinline__ void suff_zero_slot(SuffStats& s, int k) {
+  s.swdx[k] = 0.f; s.swdy[k] = 0.f; s.swdz[k] = 0.f; s.sw[k] = 0.f; s.sdw[k] = 0.f;
+}
+__device__ __forceinline__ void split_zero_slot(SplitStatsState& s, int k) {
+  s.chi[k] = 0.f; s.covx[k] = 0.f; s.covy[k] = 0.f; s.covz[k] = 0.f; s.sn[k] = 0.f; s.ssw[k] = 0.f; s.ssa[k] = 0.f;
+}
+
+__device__ __forceinline__ int vec_lim(int W, int K) { int l = ((K + W - 1) / W) * W; return l > 32 ? 32 : l; }
+
+// ------------------------------------------------------------ sample passes
+// Loads this lane's component parameters (k = 4j + q) into registers.
+struct CompRegs {
+  float x[8], y[8], z[8], kap[8], wn[8];
+};
+
+__device__ __forceinline__ void load_comps(const VMMParams& V, int q, CompRegs& c) {
+  const int K = V.K;
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    const int k = 4 * j + q;
+    const bool a = k < K;
+    c.x[j] = a ? V.mux[k] : 0.f;
+    c.y[j] = a ? V.muy[k] : 0.f;
+    c.z[j] = a ? V.muz[k] : 1.f;
+    c.kap[j] = a ? V.kappa[k] : 0.f;
+    c.wn[j] = a ? V.w[k] * V.norm[k] : 0.f;
+  }
+}
+
+// e_k = w_k norm_k exp(kappa_k min(mu_k.d - 1, 0)); returns the quad-wide pdf
+__device__ __forceinline__ float soft_eval(const CompRegs& c, int J, float dx, float dy, float dz, float e[8]) {
+  float part = 0.f;
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    if (j < J) {
+      const float ct = dot3f(dx, dy, dz, c.x[j], c.y[j], c.z[j]);
+      e[j] = c.wn[j] * __expf(c.kap[j] * fminf(ct - 1.f, 0.f));
+      part += e[j];
+    } else {
+      e[j] = 0.f;
+    }
+  }
+  float pdf = part + __shfl_xor(part, 1);
+  pdf = pdf + __shfl_xor(pdf, 2);
+  return pdf;
+}
+
+__device__ __forceinline__ float quad_group_sum(float v) {  // over the 16 quads of a wave (same q)
+  v += __shfl_xor(v, 4);
+  v += __shfl_xor(v, 8);
+  v += __shfl_xor(v, 16);
+  v += __shfl_xor(v, 32);
+  return v;
+}
+
+__device__ __forceinline__ void kahan_fold(float& sum, float& comp, float& blk) {
+  const float y = blk - comp;
+  const float t = sum + y;
+  comp = (t - sum) - y;
+  sum = t;
+  blk = 0.f;
+}
+
+// E-step (WEM:847-892): fills sh.cur (sums for k < K), sh.ll, sh.un.
+__device__ void pass_estep(Shared& sh, const RegionSamples& rs, uint32_t base, uint32_t n) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, q = tid & 3, quad = tid >> 2;
+  const VMMParams& V = sh.S.vmm;
+  const int K = V.K;
+  const int J = (K + 3) >> 2;
+  CompRegs c;
+  load_comps(V, q, c);
+  float bx[8], by[8], bz[8], bw[8], sx[8], sy[8], sz[8], sw[8], cx[8], cy[8], cz[8], cw[8];
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    bx[j] = by[j] = bz[j] = bw[j] = 0.f;
+    sx[j] = sy[j] = sz[j] = sw[j] = 0.f;
+    cx[j] = cy[j] = cz[j] = cw[j] = 0.f;
+  }
+  float llb = 0.f, lls = 0.f, llc = 0.f;
+  float u0 = 0.f, u1 = 0.f, u2 = 0.f, u3 = 0.f;
+  int blk = 0;
+  for (uint32_t s = quad; s < n; s += RT / 4) {
+    const uint32_t i = base + s;
+    const float dx = rs.dx[i], dy = rs.dy[i], dz = rs.dz[i], w = rs.w[i];
+    float e[8];
+    const float pdf = soft_eval(c, J, dx, dy, dz, e);
+    if (pdf <= 1e-16f) {
+      if (q == 0) { u0 += w; u1 += dx * w; u2 += dy * w; u3 += dz * w; }
+    } else {
+      if (q == 0) llb += w * logf(pdf);
+      const float r = w / pdf;
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const float g = e[j] * r;
+        bx[j] = fmaf(g, dx, bx[j]);
+        by[j] = fmaf(g, dy, by[j]);
+        bz[j] = fmaf(g, dz, bz[j]);
+        bw[j] += g;
+      }
+    }
+    if (++blk == 8) {
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        kahan_fold(sx[j], cx[j], bx[j]);
+        kahan_fold(sy[j], cy[j], by[j]);
+        kahan_fold(sz[j], cz[j], bz[j]);
+        kahan_fold(sw[j], cw[j], bw[j]);
+      }
+      kahan_fold(lls, llc, llb);
+      blk = 0;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    kahan_fold(sx[j], cx[j], bx[j]);
+    kahan_fold(sy[j], cy[j], by[j]);
+    kahan_fold(sz[j], cz[j], bz[j]);
+    kahan_fold(sw[j], cw[j], bw[j]);
+  }
+  kahan_fold(lls, llc, llb);
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    const float vx = quad_group_sum(sx[j] - cx[j]);
+    const float vy = quad_group_sum(sy[j] - cy[j]);
+    const float vz = quad_group_sum(sz[j] - cz[j]);
+    const float vw = quad_group_sum(sw[j] - cw[j]);
+    if (lane < 4) {
+      const int k = 4 * j + lane;
+      sh.red[wave][k][0] = vx;
+      sh.red[wave][k][1] = vy;
+      sh.red[wave][k][2] = vz;
+      sh.red[wave][k][3] = vw;
+    }
+  }
+  // lanes q != 0 hold zeros for the per-sample scalars
+  const float vll = wave_sum(lls - llc);
+  const float v0 = wave_sum(u0), v1 = wave_sum(u1), v2 = wave_sum(u2), v3 = wave_sum(u3);
+  if (lane == 0) {
+    sh.scal[wave][0] = vll;
+    sh.scal[wave][1] = v0;
+    sh.scal[wave][2] = v1;
+    sh.scal[wave][3] = v2;
+    sh.scal[wave][4] = v3;
+  }
+  __syncthreads();
+  if (tid < 128) {
+    const int k = tid >> 2, cc = tid & 3;
+    if (k < K) {
+      const float v = (sh.red[0][k][cc] + sh.red[1][k][cc]) + (sh.red[2][k][cc] + sh.red[3][k][cc]);
+      if (cc == 0) sh.cur.swdx[k] = v;
+      else if (cc == 1) sh.cur.swdy[k] = v;
+      else if (cc == 2) sh.cur.swdz[k] = v;
+      else sh.cur.sw[k] = v;
+    }
+  }
+  if (tid == 0) {
+    sh.ll = ((double)sh.scal[0][0] + (double)sh.scal[1][0]) + ((double)sh.scal[2][0] + (double)sh.scal[3][0]);
+    for (int c2 = 0; c2 < 4; c2++)
+      sh.un[c2] = (sh.scal[0][1 + c2] + sh.scal[1][1 + c2]) + (sh.scal[2][1 + c2] + sh.scal[3][1 + c2]);
+    sh.S.estepEvals += (float)n * (float)K;
+  }
+  __syncthreads();
+}
+
+// Distances pass (WEM:1199-1326): per component sum of w g_k (g_k pdf / w_k) / dist and of the weights.
+__device__ void pass_distances(Shared& sh, const RegionSamples& rs, uint32_t base, uint32_t n) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, q = tid & 3, quad = tid >> 2;
+  const VMMParams& V = sh.S.vmm;
+  const int K = V.K;
+  const int J = (K + 3) >> 2;
+  CompRegs c;
+  load_comps(V, q, c);
+  float invw[8];
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    const int k = 4 * j + q;
+    invw[j] = k < K ? 1.f / V.w[k] : 0.f;
+  }
+  float ad[8], aw[8];
+#pragma unroll
+  for (int j = 0; j < 8; j++) ad[j] = aw[j] = 0.f;
+  for (uint32_t s = quad; s < n; s += RT / 4) {
+    const uint32_t i = base + s;
+    const float dx = rs.dx[i], dy = rs.dy[i], dz = rs.dz[i], w = rs.w[i];
+    const float invd = 1.f / rs.dist[i];
+    float e[8];
+    const float pdf = soft_eval(c, J, dx, dy, dz, e);
+    if (pdf > 1e-16f) {
+      const float ip = 1.f / pdf;
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const float g = e[j] * ip;
+        const float wt = w * g * (e[j] * invw[j]);
+        ad[j] = fmaf(wt, invd, ad[j]);
+        aw[j] += wt;
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    const float vd = quad_group_sum(ad[j]);
+    const float vw = quad_group_sum(aw[j]);
+    if (lane < 4) {
+      sh.red[wave][4 * j + lane][0] = vd;
+      sh.red[wave][4 * j + lane][1] = vw;
+    }
+  }
+  __syncthreads();
+  if (tid == 0) sh.S.auxEvals += (float)n * (float)K;
+}
+
+// Split-statistics pass (Splitter:439-523): chi^2 terms, assigned mass and the
+// zero-mean tangent-plane covariance in each component's frame.
+__device__ void pass_split(Shared& sh, const RegionSamples& rs, uint32_t base, uint32_t n, float mcEst) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, q = tid & 3, quad = tid >> 2;
+  const VMMParams& V = sh.S.vmm;
+  const int K = V.K;
+  const int J = (K + 3) >> 2;
+  CompRegs c;
+  load_comps(V, q, c);
+  float achi[8], asa[8], asw[8], acx[8], acy[8], acz[8];
+#pragma unroll
+  for (int j = 0; j < 8; j++) achi[j] = asa[j] = asw[j] = acx[j] = acy[j] = acz[j] = 0.f;
+  float cnt = 0.f;
+  const float inv_mc = 1.f / mcEst;
+  for (uint32_t s = quad; s < n; s += RT / 4) {
+    const uint32_t i = base + s;
+    const float dx = rs.dx[i], dy = rs.dy[i], dz = rs.dz[i], w = rs.w[i], spdf = rs.pdf[i];
+    float e[8];
+    const float pdf = soft_eval(c, J, dx, dy, dz, e);
+    if (pdf > 1e-16f) {
+      cnt += 1.f;
+      const float ip = 1.f / pdf;
+      const float value = w * spdf;
+      const float a = value * inv_mc * ip;  // value / (mc pdf)
+      const float ispdf = 1.f / spdf;
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        if (j < J) {
+          const int k = 4 * j + q;
+          const float g = e[j] * ip;
+          const float vmf = e[j];
+          // (value^2 vmf / (mc pdf)^2 - 2 vmf value / (mc pdf) + vmf) / spdf
+          float chi = vmf * (a * a - 2.f * a + 1.f) * ispdf;
+          chi = g > 0.f ? chi : 0.f;
+          achi[j] += chi;
+          asa[j] += g;
+          const float lx = dot3f(sh.fxx[k], sh.fxy[k], sh.fxz[k], dx, dy, dz);
+          const float ly = dot3f(sh.fyx[k], sh.fyy[k], sh.fyz[k], dx, dy, dz);
+          const float aw = g * w;
+          asw[j] += aw;
+          acx[j] = fmaf(aw, lx * lx, acx[j]);
+          acy[j] = fmaf(aw, ly * ly, acy[j]);
+          acz[j] = fmaf(aw, lx * ly, acz[j]);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    const float v0 = quad_group_sum(achi[j]);
+    const float v1 = quad_group_sum(asa[j]);
+    const float v2 = quad_group_sum(asw[j]);
+    const float v3 = quad_group_sum(acx[j]);
+    const float v4 = quad_group_sum(acy[j]);
+    const float v5 = quad_group_sum(acz[j]);
+    if (lane < 4) {
+      const int k = 4 * j + lane;
+      sh.red[wave][k][0] = v0;
+      sh.red[wave][k][1] = v1;
+      sh.red[wave][k][2] = v2;
+      sh.red[wave][k][3] = v3;
+      sh.red[wave][k][4] = v4;
+      sh.red[wave][k][5] = v5;
+    }
+  }
+  // every lane of a quad counted the sample: divide the wave sum by 4
+  const float vc = wave_sum(cnt) * 0.25f;
+  if (lane == 0) sh.scal[wave][5] = vc;
+  __syncthreads();
+  if (tid == 0) {
+    sh.nAssigned = (sh.scal[0][5] + sh.scal[1][5]) + (sh.scal[2][5] + sh.scal[3][5]);
+    sh.S.auxEvals += (float)n * (float)K;
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ float red4(const Shared& sh, int k, int c) {
+  return (sh.red[0][k][c] + sh.red[1][k][c]) + (sh.red[2][k][c] + sh.red[3][k][c]);
+}
+
+// ------------------------------------------------------ region-level steps
+// UpdateSplitStatistics into SS (S.split or the local recursive-split copy).
+__device__ void update_split_stats(Shared& sh, SplitStatsState& SS, const RegionSamples& rs, uint32_t base,
+                                   uint32_t n, float mcEst) {
+  const int tid = threadIdx.x;
+  const VMMParams& V = sh.S.vmm;
+  if (tid < 32) {
+    const int k = tid;
+    float xx, xy, xz, yx, yy, yz;
+    frame_of(V.mux[k], V.muy[k], V.muz[k], xx, xy, xz, yx, yy, yz);
+    sh.fxx[k] = xx; sh.fxy[k] = xy; sh.fxz[k] = xz;
+    sh.fyx[k] = yx; sh.fyy[k] = yy; sh.fyz[k] = yz;
+  }
+  __syncthreads();
+  pass_split(sh, rs, base, n, mcEst);
+  if (tid < 32) {
+    const int k = tid;
+    const float nA = sh.nAssigned;
+    const int lim = vec_lim(sh.W, SS.K);
+    if (k < SS.K) {
+      const float nOld = SS.sn[k];
+      const float nNew = nOld + nA;
+      SS.ssa[k] += red4(sh, k, 1);
+      SS.ssw[k] += red4(sh, k, 2);
+      SS.covx[k] += red4(sh, k, 3);
+      SS.covy[k] += red4(sh, k, 4);
+      SS.covz[k] += red4(sh, k, 5);
+      // running mean chi += (x - chi)/count over the assigned samples, in closed form
+      if (nA > 0.f) SS.chi[k] = (SS.chi[k] * nOld + red4(sh, k, 0)) / nNew;
+      SS.sn[k] = nNew;
+    } else if (k < lim) {
+      SS.sn[k] += nA;
+    }
+  }
+  __syncthreads();
+}
+
+__device__ void distances(Shared& sh, SuffStats& ST, const RegionSamples& rs, uint32_t base, uint32_t n, bool init) {
+  pass_distances(sh, rs, base, n);
+  const int tid = threadIdx.x;
+  VMMParams& V = sh.S.vmm;
+  if (tid < 32) {
+    const int k = tid;
+    const int lim = vec_lim(sh.W, V.K);
+    if (k < V.K) {
+      const float bd = red4(sh, k, 0), bw = red4(sh, k, 1);
+      if (init) {
+        ST.sdw[k] = bw;
+        V.dist[k] = ST.sdw[k] / bd;
+      } else {
+        float sumInv = bd;
+        sumInv += V.dist[k] > 0.f ? (ST.sdw[k] / V.dist[k]) : 0.f;
+        ST.sdw[k] += bw;
+        V.dist[k] = ST.sdw[k] / sumInv;
+      }
+    } else if (k < lim) {
+      V.dist[k] = 0.f;
+      ST.sdw[k] = 0.f;
+    }
+  }
+  __syncthreads();
+}
+
+// EM loop (updateMixture / partialUpdateMixture). prev = region statistics or
+// the partial-refit temporary; masked = partial fit over sh.mask.
+__device__ int em(Shared& sh, SuffStats& prev, const RegionSamples& rs, uint32_t base, uint32_t n,
+                  const FactoryConfig& cfg, bool masked) {
+  const int tid = threadIdx.x;
+  VMMParams& V = sh.S.vmm;
+  SuffStats& cur = sh.cur;
+  if (tid < 32) suff_zero_slot(cur, tid);
+  if (tid == 0) {
+    cur.K = 32; cur.sumWeights = 0.f; cur.numSamples = 0.f; cur.overall = 0.f; cur.normalized = 0;
+    sh.converged = 0; sh.prevLL = 0.0; sh.invPrevLL = 1.0;
+  }
+  __syncthreads();
+  int it = 0;
+  while (!sh.converged && it < cfg.maxEMIterations) {
+    if (tid < 32) suff_zero_slot(cur, tid);
+    if (tid == 0) { cur.K = V.K; cur.numSamples = (float)n; cur.sumWeights = 0.f; cur.normalized = 0; }
+    __syncthreads();
+    pass_estep(sh, rs, base, n);
+    // unassigned mass becomes a new component (WEM:732-748)
+    if (tid == 0 && sh.un[0] > 0.f && cur.K < kMaxComponents) {
+      const int k = cur.K;
+      cur.K++;
+      cur.sw[k] = sh.un[0]; cur.swdx[k] = sh.un[1]; cur.swdy[k] = sh.un[2]; cur.swdz[k] = sh.un[3];
+      prev.K++;
+      prev.sw[k] = 0.f; prev.swdx[k] = 0.f; prev.swdy[k] = 0.f; prev.swdz[k] = 0.f;
+      V.K++;
+      if (masked) sh.mask |= 1u << (V.K - 1);
+    }
+    __syncthreads();
+    if (tid < 64) {
+      const int k = tid;
+      const int K = V.K;
+      const bool act = k < K;
+      // normalize(numSamples) (WEM:553-572)
+      const float tot = wave_sum(act ? cur.sw[k] : 0.f);
+      const float ns = cur.numSamples;
+      const float nrm = ns / tot;
+      if (act) { cur.swdx[k] *= nrm; cur.swdy[k] *= nrm; cur.swdz[k] *= nrm; cur.sw[k] *= nrm; }
+      // MAP weights (WEM:895-927 / 1030-1078)
+      const float wp = cfg.weightPrior;
+      const float nsum = cur.numSamples + prev.numSamples;
+      float wt = 0.f;
+      if (act) wt = (wp + (cur.sw[k] + prev.sw[k])) / ((wp * (float)K) + nsum);
+      const bool m = masked ? (act && ((sh.mask >> k) & 1u)) : act;
+      if (masked) {
+        const float sumP = wave_sum(m ? wt : 0.f);
+        const float sumW = wave_sum((act && !m) ? V.w[k] : 0.f);
+        float inv = 1.f / sumP;
+        inv *= 1.f - sumW;
+        if (k < 32) V.w[k] = m ? wt * inv : (act ? V.w[k] : 0.f);
+      } else if (k < 32) {
+        V.w[k] = wt;
+      }
+      // MAP mean direction and concentration (WEM:930-999 / 1081-1148)
+      if (k < 32) {
+        const float cn = cur.numSamples, pn = prev.numSamples;
+        const float nss = cn + pn;
+        const float overall = cur.numSamples + prev.overall;
+        const float ce = cn / nss;
+        const float pe = 1.f - ce;
+        if (act) {
+          const float partialN = V.w[k] * overall;
+          const float cmx = cur.sw[k] > 0.f ? cur.swdx[k] / cur.sw[k] : 0.f;
+          const float cmy = cur.sw[k] > 0.f ? cur.swdy[k] / cur.sw[k] : 0.f;
+          const float cmz = cur.sw[k] > 0.f ? cur.swdz[k] / cur.sw[k] : 0.f;
+          const float pmx = prev.sw[k] > 0.f ? prev.swdx[k] / prev.sw[k] : 0.f;
+          const float pmy = prev.sw[k] > 0.f ? prev.swdy[k] / prev.sw[k] : 0.f;
+          const float pmz = prev.sw[k] > 0.f ? prev.swdz[k] / prev.sw[k] : 0.f;
+          const float mdx = cmx * ce + pmx * pe, mdy = cmy * ce + pmy * pe, mdz = cmz * ce + pmz * pe;
+          float mcos = sqrtf(dot3f(mdx, mdy, mdz, mdx, mdy, mdz));
+          if (m) {
+            if (mcos > 0.f) { V.mux[k] = mdx / mcos; V.muy[k] = mdy / mcos; V.muz[k] = mdz / mcos; }
+            mcos = (cfg.meanCosinePrior * cfg.meanCosinePriorStrength + mcos * partialN) /
+                   (cfg.meanCosinePriorStrength + partialN);
+            mcos = fminf(cfg.maxMeanCosine, mcos);
+            V.mc[k] = mcos;
+            V.kappa[k] = mc_to_kappa(mcos);
+          }
+        } else {
+          V.mux[k] = 0.f; V.muy[k] = 0.f; V.muz[k] = 1.f; V.mc[k] = 0.f; V.kappa[k] = 0.f;
+        }
+        calc_normalization_lane(V, k);
+      }
+      if (tid == 0) { cur.numSamples = ns; cur.sumWeights = tot; cur.normalized = 1; }
+    }
+    __syncthreads();
+    it++;
+    if (tid == 0 && it > 1) {
+      const double rel = fabs(sh.ll - sh.prevLL) * sh.invPrevLL;
+      if (rel < (double)cfg.convergenceThreshold) sh.converged = 1;
+      sh.prevLL = sh.ll;
+      sh.invPrevLL = 1.0 / fabs(sh.ll);
+    }
+    __syncthreads();
+  }
+  // previousStats += currentStats (WEM:575-593)
+  if (tid < 32) {
+    const int k = tid;
+    prev.swdx[k] += cur.swdx[k]; prev.swdy[k] += cur.swdy[k]; prev.swdz[k] += cur.swdz[k];
+    prev.sw[k] += cur.sw[k]; prev.sdw[k] += cur.sdw[k];
+  }
+  if (tid == 0) {
+    prev.sumWeights += cur.sumWeights;
+    prev.numSamples += cur.numSamples;
+    prev.overall += cur.numSamples;
+    sh.S.lastLogLikelihood = (float)sh.prevLL;
+  }
+  __syncthreads();
+  return it;
+}
+
+// ---- splitting (serial, lane 0)
+__device__ __forceinline__ void exp_map(float vx, float vy, float& ox, float& oy, float& oz) {  // Splitter:175-190
+  const float len = sqrtf(vx * vx + vy * vy);
+  const float sinc = sinf(len) / len;
+  ox = len > 0.f ? vx * sinc : 0.f;
+  oy = len > 0.f ? vy * sinc : 0.f;
+  oz = cosf(len);
+}
+__device__ __forceinline__ void log_map(float x, float y, float z, float& ox, float& oy) {  // Splitter:161-172
+  const float zc = fminf(1.f, fmaxf(-1.f, z));
+  const float alpha = acosf(zc);
+  const float invSinc = alpha / sinf(alpha);
+  ox = alpha > 0.f ? x * invSinc : 0.f;
+  oy = alpha > 0.f ? y * invSinc : 0.f;
+}
+
+__device__ void vmm_normalize_all(VMMParams& V) {
+  for (int k = 0; k < 32; k++) calc_normalization_lane(V, k);
+}
+
+__device__ bool split_component(VMMParams& V, SplitStatsState& SS, SuffStats& ST, int idx, const FactoryConfig& cfg) {
+  const float numAssigned = SS.ssa[idx];
+  const float invSw = 1.f / SS.ssw[idx];
+  const float cx = SS.covx[idx] * invSw, cy = SS.covy[idx] * invSw, cz = SS.covz[idx] * invSw;
+  const float D = sqrtf((cx - cy) * (cx - cy) + (cz * cz * 4.f)) * 0.5f;
+  float ev0 = (float)((double)(cx + cy) * 0.5);
+  ev0 += D;
+  float e0x = -cz, e0y = cx - ev0;
+  const float l0 = sqrtf(e0x * e0x + e0y * e0y);
+  e0x /= l0;
+  e0y /= l0;
+  const float weight = V.w[idx], meanCosine = V.mc[idx], kappa = V.kappa[idx];
+  if ((double)kappa >= (double)kMaxKappaSplit * 0.9) return false;
+  const float mx = V.mux[idx], my = V.muy[idx], mz = V.muz[idx];
+  const float nw = weight * 0.5f;
+  float m0x = mx, m0y = my, m0z = mz, m1x = mx, m1y = my, m1z = mz;
+  float nc0 = meanCosine, nc1 = meanCosine * meanCosine;
+  if (D > 1e-8f) {
+    float xx, xy, xz, yx, yy, yz;
+    frame_of(mx, my, mz, xx, xy, xz, yx, yy, yz);
+    const float ax = e0x * ev0 * 0.5f, ay = e0y * ev0 * 0.5f;
+    float lx, ly, lz;
+    exp_map(ax, ay, lx, ly, lz);
+    m0x = xx * lx + yx * ly + mx * lz;
+    m0y = xy * lx + yy * ly + my * lz;
+    m0z = xz * lx + yz * ly + mz * lz;
+    nc0 = meanCosine / fabsf(dot3f(mx, my, mz, m0x, m0y, m0z));
+    nc0 = fminf(nc0, cfg.maxMeanCosineSplit);
+    nc1 = nc0;
+    exp_map(-ax, -ay, lx, ly, lz);
+    m1x = xx * lx + yx * ly + mx * lz;
+    m1y = xy * lx + yy * ly + my * lz;
+    m1z = xz * lx + yz * ly + mz * lz;
+  } else if (numAssigned < 2.f) {
+    return false;
+  }
+  const int K = V.K;
+  // PAVMM::splitComponent (PAVMM:165-190)
+  V.dist[K] = V.dist[idx];
+  V.w[idx] = nw; V.mc[idx] = nc0; V.kappa[idx] = mc_to_kappa(nc0);
+  V.mux[idx] = m0x; V.muy[idx] = m0y; V.muz[idx] = m0z;
+  V.w[K] = nw; V.mc[K] = nc1; V.kappa[K] = mc_to_kappa(nc1);
+  V.mux[K] = m1x; V.muy[K] = m1y; V.muz[K] = m1z;
+  V.K = K + 1;
+  vmm_normalize_all(V);
+  // SufficientStatistics::splitComponentsStats (WEM:520-550)
+  float s = ST.sw[idx];
+  s /= 2.f;
+  ST.sw[idx] = s;
+  ST.swdx[idx] = m0x * nc0 * s; ST.swdy[idx] = m0y * nc0 * s; ST.swdz[idx] = m0z * nc0 * s;
+  ST.sw[K] = s;
+  ST.swdx[K] = m1x * nc1 * s; ST.swdy[K] = m1y * nc1 * s; ST.swdz[K] = m1z * nc1 * s;
+  const float t = ST.sdw[idx] * 0.5f;
+  ST.sdw[idx] = t;
+  ST.sdw[K] = t;
+  ST.K += 1;
+  split_zero_slot(SS, idx);
+  split_zero_slot(SS, K);
+  SS.K = K + 1;
+  return true;
+}
+
+// candidates in descending chi^2 (stable for equal values)
+__device__ void sort_candidates(const SplitStatsState& SS, int* order) {
+  const int K = SS.K;
+  for (int i = 0; i < K; i++) order[i] = i;
+  for (int i = 1; i < K; i++) {
+    const int v = order[i];
+    const float c = SS.chi[v];
+    int j = i - 1;
+    while (j >= 0 && SS.chi[order[j]] < c) { order[j + 1] = order[j]; j--; }
+    order[j + 1] = v;
+  }
+}
+
+// maskedReplace (WEM:448-468)
+__device__ void masked_replace(Shared& sh, SuffStats& dst, const SuffStats& src) {
+  const int tid = threadIdx.x;
+  if (tid < 64) {
+    const int k = tid;
+    float v = 0.f;
+    if (k < 32) {
+      if ((sh.mask >> k) & 1u) {
+        dst.swdx[k] = src.swdx[k]; dst.swdy[k] = src.swdy[k]; dst.swdz[k] = src.swdz[k];
+        dst.sw[k] = src.sw[k]; dst.sdw[k] = src.sdw[k];
+      }
+      v = dst.sw[k];
+    }
+    const float sum = wave_sum(v);
+    if (tid == 0) { if (dst.normalized) dst.numSamples = sum; else dst.sumWeights = sum; }
+  }
+  __syncthreads();
+}
+
+__device__ void copy_suff(Shared& sh, SuffStats& dst, const SuffStats& src) {
+  const int tid = threadIdx.x;
+  if (tid < 32) {
+    dst.swdx[tid] = src.swdx[tid]; dst.swdy[tid] = src.swdy[tid]; dst.swdz[tid] = src.swdz[tid];
+    dst.sw[tid] = src.sw[tid]; dst.sdw[tid] = src.sdw[tid];
+  }
+  if (tid == 0) {
+    dst.sumWeights = src.sumWeights; dst.numSamples = src.numSamples; dst.overall = src.overall;
+    dst.K = src.K; dst.normalized = src.normalized;
+  }
+  __syncthreads();
+}
+
+// SufficientStatistics::clear(K) (WEM:372-389)
+__device__ void clear_suff(Shared& sh, SuffStats& s, int K) {
+  const int tid = threadIdx.x;
+  if (tid < vec_lim(sh.W, K)) suff_zero_slot(s, tid);
+  if (tid == 0) { s.K = K; s.sumWeights = 0.f; s.numSamples = 0.f; s.normalized = 0; }
+  __syncthreads();
+}
+__device__ void clear_split(Shared& sh, SplitStatsState& s, int K) {
+  const int tid = threadIdx.x;
+  if (tid < vec_lim(sh.W, K)) split_zero_slot(s, tid);
+  if (tid == 0) s.K = K;
+  __syncthreads();
+}
+
+// PerformRecursiveSplitting (Splitter:293-356)
+__device__ int recursive_splitting(Shared& sh, const RegionSamples& rs, uint32_t base, uint32_t n, float mcEst,
+                                   const FactoryConfig& cfg, int& pits) {
+  const int tid = threadIdx.x;
+  VMMParams& V = sh.S.vmm;
+  int total = 0;
+  copy_suff(sh, sh.tmp, sh.S.suff);
+  if (tid == 0) sh.numSplits = -1;
+  __syncthreads();
+  while (V.K < kMaxComponents && sh.numSplits != 0) {
+    clear_split(sh, sh.lsplit, 32);
+    clear_split(sh, sh.lsplit, V.K);
+    update_split_stats(sh, sh.lsplit, rs, base, n, mcEst);
+    if (tid == 0) {
+      sort_candidates(sh.lsplit, sh.order);
+      sh.mask = 0u;
+      int ns = 0;
+      const int numComp = V.K;
+      for (int c = 0; c < numComp; c++) {
+        const int idx = sh.order[c];
+        if (sh.lsplit.chi[idx] > cfg.splittingThreshold && V.K < kMaxComponents) {
+          const bool ok = split_component(V, sh.lsplit, sh.tmp, idx, cfg);
+          sh.mask |= 1u << idx;
+          sh.mask |= 1u << (V.K - 1);
+          if (ok) ns++;
+        }
+      }
+      sh.numSplits = ns;
+    }
+    __syncthreads();
+    if (sh.numSplits > 0) {
+      clear_suff(sh, sh.tmp, V.K);
+      pits += em(sh, sh.tmp, rs, base, n, cfg, true);
+      if (tid == 0) sh.S.suff.K = V.K;
+      __syncthreads();
+      masked_replace(sh, sh.S.suff, sh.tmp);
+    }
+    total += sh.numSplits;
+    __syncthreads();
+  }
+  return total;
+}
+
+// ---- merging
+__device__ float product_scale(float m0x, float m0y, float m0z, float k0, float n0, float m1x, float m1y, float m1z,
+                               float k1, float n1, float& pmx, float& pmy, float& pmz, float& pk, float& pn) {
+  pmx = k0 * m0x + k1 * m1x;
+  pmy = k0 * m0y + k1 * m1y;
+  pmz = k0 * m0z + k1 * m1z;
+  pk = sqrtf(dot3f(pmx, pmy, pmz, pmx, pmy, pmz));
+  pn = 1.f / (4.f * 3.14159265358979f);
+  if (pk > 1e-3f) {
+    const float e = expf(-2.f * pk);
+    pn = pk / (kTwoPi * (1.f - e));
+    pmx /= pk; pmy /= pk; pmz /= pk;
+  } else {
+    pk = 0.f;
+    pmx = m0x; pmy = m0y; pmz = m0z;
+  }
+  float scale = (n0 * n1) / pn;
+  const float c0 = dot3f(m0x, m0y, m0z, pmx, pmy, pmz), c1 = dot3f(m1x, m1y, m1z, pmx, pmy, pmz);
+  scale *= expf(k0 * (c0 - 1.f) + k1 * (c1 - 1.f));
+  return scale;
+}
+
+__device__ float integrated_division(float m0x, float m0y, float m0z, float k0, float n0, float m1x, float m1y,
+                                     float m1z, float k1, float n1, float em2k1) {
+  float pmx = k0 * m0x + k1 * m1x, pmy = k0 * m0y + k1 * m1y, pmz = k0 * m0z + k1 * m1z;
+  float pk = sqrtf(dot3f(pmx, pmy, pmz, pmx, pmy, pmz));
+  float pn = 1.f / (4.f * 3.14159265358979f);
+  if (pk > 1e-3f) {
+    const float e = expf(-2.f * pk);
+    pn = pk / (kTwoPi * (1.f - e));
+    pmx /= pk; pmy /= pk; pmz /= pk;
+  } else {
+    pk = 0.f;
+    pmx = m0x; pmy = m0y; pmz = m0z;
+  }
+  float scale = (n0 * n1) / pn;
+  const float c0 = dot3f(m0x, m0y, m0z, pmx, pmy, pmz), c1 = dot3f(m1x, m1y, m1z, pmx, pmy, pmz);
+  scale *= (4.f * 3.14159265358979f * 3.14159265358979f * (1.f - em2k1)) / (k1 * k1);
+  scale *= expf((k0 * (c0 - 1.f) + k1 * (c1 - 1.f)) + (2.f * k1));
+  return scale;
+}
+
+// CalculateMergeCost (Merger:93-177)
+__device__ float merge_cost(const VMMParams& V, int i0, int i1) {
+  const float w0 = V.w[i0], w1 = V.w[i1];
+  const float k0 = V.kappa[i0], k1 = V.kappa[i1];
+  const float m0x = V.mux[i0], m0y = V.muy[i0], m0z = V.muz[i0];
+  const float m1x = V.mux[i1], m1y = V.muy[i1], m1z = V.muz[i1];
+  const float c0 = V.mc[i0], c1 = V.mc[i1], n0 = V.norm[i0], n1 = V.norm[i1];
+  const float w = w0 + w1;
+  float kappa = 0.f, nrm = kOneOverFourPi, em2k = 1.f;
+  float mdx = (w0 * c0 * m0x + w1 * c1 * m1x) / w;
+  float mdy = (w0 * c0 * m0y + w1 * c1 * m1y) / w;
+  float mdz = (w0 * c0 * m0z + w1 * c1 * m1z) / w;
+  float mcos = mdx * mdx + mdy * mdy + mdz * mdz;
+  if (mcos > 0.f) {
+    mcos = sqrtf(mcos);
+    kappa = mc_to_kappa(mcos);
+    kappa = kappa < 1e-3f ? 0.f : kappa;
+    em2k = expf(-2.f * kappa);
+    nrm = kappa / (kTwoPi * (1.f - em2k));
+    mdx /= mcos; mdy /= mcos; mdz /= mcos;
+  } else {
+    mdx = m0x; mdy = m0y; mdz = m0z;
+  }
+  float a0x, a0y, a0z, ak, an, b0x, b0y, b0z, bk, bn, c0x, c0y, c0z, ck, cn;
+  const float s00 = product_scale(m0x, m0y, m0z, k0, n0, m0x, m0y, m0z, k0, n0, a0x, a0y, a0z, ak, an);
+  const float s11 = product_scale(m1x, m1y, m1z, k1, n1, m1x, m1y, m1z, k1, n1, b0x, b0y, b0z, bk, bn);
+  const float s01 = product_scale(m0x, m0y, m0z, k0, n0, m1x, m1y, m1z, k1, n1, c0x, c0y, c0z, ck, cn);
+  float chi = 0.f;
+  chi += ((w0 * w0) / w) * (s00 * integrated_division(a0x, a0y, a0z, ak, an, -mdx, -mdy, -mdz, kappa, nrm, em2k));
+  chi += ((w1 * w1) / w) * (s11 * integrated_division(b0x, b0y, b0z, bk, bn, -mdx, -mdy, -mdz, kappa, nrm, em2k));
+  chi += 2.f * ((w0 * w1) / w) * (s01 * integrated_division(c0x, c0y, c0z, ck, cn, -mdx, -mdy, -mdz, kappa, nrm, em2k));
+  chi -= w;
+  return chi;
+}
+
+__device__ void vmm_swap(VMMParams& V, int a, int b) {
+  if (a == b) return;
+  float t;
+#define SWP(arr) t = V.arr[a]; V.arr[a] = V.arr[b]; V.arr[b] = t;
+  SWP(w) SWP(kappa) SWP(em2k) SWP(mc) SWP(norm) SWP(mux) SWP(muy) SWP(muz) SWP(dist)
+#undef SWP
+}
+
+// PAVMM::mergeComponents (PAVMM:191-257)
+__device__ void vmm_merge(VMMParams& V, int i0, int i1) {
+  const float w0 = V.w[i0], w1 = V.w[i1], c0 = V.mc[i0], c1 = V.mc[i1];
+  float kappa = 0.f, nrm = kOneOverFourPi, e = 1.f;
+  const float w = w0 + w1;
+  float mdx = w0 * c0 * V.mux[i0] + w1 * c1 * V.mux[i1];
+  float mdy = w0 * c0 * V.muy[i0] + w1 * c1 * V.muy[i1];
+  float mdz = w0 * c0 * V.muz[i0] + w1 * c1 * V.muz[i1];
+  mdx /= w; mdy /= w; mdz /= w;
+  float mcos = mdx * mdx + mdy * mdy + mdz * mdz;
+  if (mcos > 0.f) {
+    mcos = sqrtf(mcos);
+    kappa = mc_to_kappa(mcos);
+    kappa = kappa < 1e-3f ? 0.f : kappa;
+    e = expf(-2.f * kappa);
+    nrm = kappa / (kTwoPi * (1.f - e));
+    mdx /= mcos; mdy /= mcos; mdz /= mcos;
+  } else {
+    mdx = V.mux[i0]; mdy = V.muy[i0]; mdz = V.muz[i0];
+  }
+  const float d0 = V.dist[i0], d1 = V.dist[i1];
+  V.w[i0] = w; V.kappa[i0] = kappa; V.mc[i0] = mcos; V.norm[i0] = nrm; V.em2k[i0] = e;
+  V.mux[i0] = mdx; V.muy[i0] = mdy; V.muz[i0] = mdz;
+  float nd = w0 * d0 + w1 * d1;
+  nd /= (w0 + w1);
+  V.dist[i0] = nd;
+  vmm_swap(V, i1, V.K - 1);
+  clear_comp(V, V.K - 1);
+  V.K -= 1;
+}
+
+// ComponentSplitStatistics::mergeComponentStats, zero-mean variant (Splitter:994-1100)
+__device__ void split_stats_merge(SplitStatsState& SS, int i, int j, float wI, float mIx, float mIy, float mIz,
+                                  float wJ, float mJx, float mJy, float mJz, float wK, float mKx, float mKy,
+                                  float mKz) {
+  const int L = SS.K - 1;
+  float xx, xy, xz, yx, yy, yz;
+  frame_of(mKx, mKy, mKz, xx, xy, xz, yx, yy, yz);
+  float aix, aiy, ajx, ajy;
+  log_map(dot3f(xx, xy, xz, mIx, mIy, mIz), dot3f(yx, yy, yz, mIx, mIy, mIz), dot3f(mKx, mKy, mKz, mIx, mIy, mIz),
+          aix, aiy);
+  log_map(dot3f(xx, xy, xz, mJx, mJy, mJz), dot3f(yx, yy, yz, mJx, mJy, mJz), dot3f(mKx, mKy, mKz, mJx, mJy, mJz),
+          ajx, ajy);
+  const float invWK = wK > 0.f ? 1.f / wK : 1.f;
+  const float swI = SS.ssw[i], swJ = SS.ssw[j], swK = swI + swJ;
+  float cI[3] = {0.f, 0.f, 0.f}, cJ[3] = {0.f, 0.f, 0.f};
+  if (swI > 0.f) { const float r = 1.f / swI; cI[0] = SS.covx[i] * r; cI[1] = SS.covy[i] * r; cI[2] = SS.covz[i] * r; }
+  if (swJ > 0.f) { const float r = 1.f / swJ; cJ[0] = SS.covx[j] * r; cJ[1] = SS.covy[j] * r; cJ[2] = SS.covz[j] * r; }
+  const float mII[3] = {aix * aix, aiy * aiy, aix * aiy};
+  const float mJJ[3] = {ajx * ajx, ajy * ajy, ajx * ajy};
+  float cK[3];
+  for (int d = 0; d < 3; d++) cK[d] = (wI * cI[d] + wI * mII[d] + wJ * cJ[d] + wJ * mJJ[d]) * invWK;
+  const float saK = SS.ssa[i] + SS.ssa[j];
+  const float nK = invWK * (wI * SS.sn[i] + wJ * SS.sn[j]);
+  const float chiK = SS.chi[i] + SS.chi[j];
+  SS.covx[i] = cK[0] * swK; SS.covy[i] = cK[1] * swK; SS.covz[i] = cK[2] * swK;
+  SS.ssw[i] = swK; SS.sn[i] = nK; SS.ssa[i] = saK; SS.chi[i] = chiK;
+  SS.covx[j] = SS.covx[L]; SS.covy[j] = SS.covy[L]; SS.covz[j] = SS.covz[L];
+  SS.ssw[j] = SS.ssw[L]; SS.sn[j] = SS.sn[L]; SS.ssa[j] = SS.ssa[L]; SS.chi[j] = SS.chi[L];
+  split_zero_slot(SS, L);
+  SS.K--;
+}
+
+__device__ void suff_merge(SuffStats& ST, int i, int j) {  // WEM:485-518
+  const int L = ST.K - 1;
+  ST.swdx[i] += ST.swdx[j]; ST.swdy[i] += ST.swdy[j]; ST.swdz[i] += ST.swdz[j];
+  ST.sw[i] += ST.sw[j]; ST.sdw[i] += ST.sdw[j];
+  ST.swdx[j] = ST.swdx[L]; ST.swdy[j] = ST.swdy[L]; ST.swdz[j] = ST.swdz[L];
+  ST.sw[j] = ST.sw[L]; ST.sdw[j] = ST.sdw[L];
+  suff_zero_slot(ST, L);
+  ST.K--;
+}
+
+// PerformMerging with ThresholdedMergeNext (Merger:45-89, 218-279)
+__device__ int perform_merging(Shared& sh, SuffStats& ST, SplitStatsState& SS, const FactoryConfig& cfg) {
+  const int tid = threadIdx.x;
+  VMMParams& V = sh.S.vmm;
+  int total = 0;
+  while (V.K > sh.W) {
+    const int K = V.K;
+    for (int p = tid; p < 32 * 32; p += RT) {
+      const int i = p >> 5, j = p & 31;
+      sh.cost[p] = (i < j && j < K) ? merge_cost(V, i, j) : 3.402823466e38f;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      float minCost = 3.402823466e38f;
+      int ci = 0, cj = 0, found = 0;
+      for (int i = 0; i < K - 1; i++)
+        for (int j = i + 1; j < K; j++) {
+          const float cst = sh.cost[i * 32 + j];
+          if (cst < cfg.mergingThreshold && cst < minCost && SS.sn[i] > 0.f && SS.sn[j] > 0.f) {
+            ci = i; cj = j; minCost = cst; found = 1;
+          }
+        }
+      sh.found = found;
+      if (found) {
+        const float mIx = V.mux[ci], mIy = V.muy[ci], mIz = V.muz[ci], wI = V.w[ci];
+        const float mJx = V.mux[cj], mJy = V.muy[cj], mJz = V.muz[cj], wJ = V.w[cj];
+        vmm_merge(V, ci, cj);
+        split_stats_merge(SS, ci, cj, wI, mIx, mIy, mIz, wJ, mJx, mJy, mJz, V.w[ci], V.mux[ci], V.muy[ci], V.muz[ci]);
+        suff_merge(ST, ci, cj);
+      }
+    }
+    __syncthreads();
+    if (!sh.found) break;
+    total++;
+  }
+  return total;
+}
+
+// ---- validity (PAVMM:421-513, WEM:285-343)
+__device__ bool region_valid(Shared& sh, bool checkSuff) {
+  const int tid = threadIdx.x;
+  int ok = 1;
+  if (tid < 32) {
+    const VMMParams& V = sh.S.vmm;
+    const int k = tid;
+    const float vals[9] = {V.w[k], V.kappa[k], V.mux[k], V.muy[k], V.muz[k], V.norm[k], V.em2k[k], V.mc[k], V.dist[k]};
+    for (int i = 0; i < 9; i++) if (!isfinite(vals[i])) ok = 0;
+    if (k < V.K) {
+      if (V.w[k] < 0.f || V.w[k] > 1.f + 1e-6f) ok = 0;
+      if (V.kappa[k] < 0.f || V.mc[k] < 0.f || V.mc[k] > 1.f || V.norm[k] < 0.f || V.dist[k] < 0.f) ok = 0;
+      if (V.mux[k] < -1.f || V.mux[k] > 1.f || V.muy[k] < -1.f || V.muy[k] > 1.f || V.muz[k] < -1.f || V.muz[k] > 1.f) ok = 0;
+    } else {
+      if (V.w[k] != 0.f || V.kappa[k] != 0.f || V.mux[k] != 0.f || V.muy[k] != 0.f || V.muz[k] != 1.f || V.mc[k] != 0.f ||
+          fabsf(V.norm[k] - kOneOverFourPi) >= 1e-6f || V.em2k[k] != 1.f || V.dist[k] != 0.f)
+        ok = 0;
+    }
+    if (checkSuff) {
+      const SuffStats& s = sh.S.suff;
+      if (!isfinite(s.swdx[k]) || !isfinite(s.swdy[k]) || !isfinite(s.swdz[k]) || !isfinite(s.sw[k]) || !isfinite(s.sdw[k])) ok = 0;
+      if (k < s.K) { if (s.sw[k] < 0.f || s.sdw[k] < 0.f) ok = 0; }
+      else if (s.swdx[k] != 0.f || s.swdy[k] != 0.f || s.swdz[k] != 0.f || s.sw[k] != 0.f || s.sdw[k] != 0.f) ok = 0;
+      if (k == 0 && (!isfinite(s.numSamples) || s.numSamples < 0.f || !isfinite(s.sumWeights) || s.sumWeights < 0.f)) ok = 0;
+    }
+  }
+  return __syncthreads_and(ok) != 0;
+}
+
+// ---- ASM::fit (ASM:303-347)
+__device__ void region_fit(Shared& sh, const RegionSamples& rs, uint32_t base, uint32_t n, const FactoryConfig& cfg,
+                           const float* __restrict__ uniformDirs) {
+  const int tid = threadIdx.x;
+  VMMParams& V = sh.S.vmm;
+  const int K0 = cfg.initK;
+  clear_suff(sh, sh.S.suff, K0);
+  clear_split(sh, sh.S.split, K0);
+  // InitUniformVMM (WEM:632-666): spherical Fibonacci means for K0 components
+  if (tid < 32) {
+    const int k = tid;
+    const float* d = uniformDirs + ((K0 - 1) * 32 + k) * 3;
+    V.mux[k] = d[0]; V.muy[k] = d[1]; V.muz[k] = d[2];
+    if (k < K0) {
+      V.kappa[k] = cfg.initKappa;
+      V.w[k] = 1.f / (float)K0;
+    } else {
+      V.kappa[k] = 0.f; V.w[k] = 0.f; V.norm[k] = kOneOverFourPi; V.em2k[k] = 1.f; V.mc[k] = 0.f;
+    }
+    calc_normalization_lane(V, k);
+    const float t = 1.f - 2.f / (expf(2.f * V.kappa[k]) - 1.f);
+    const float m = 1.f / t - 1.f / V.kappa[k];
+    V.mc[k] = V.kappa[k] > 0.f ? m : 0.f;
+  }
+  if (tid == 0) {
+    V.K = K0;
+    sh.S.afterSplitLo = sh.S.afterSplitHi = sh.S.afterMergeLo = sh.S.afterMergeHi = 0;
+  }
+  __syncthreads();
+  clear_suff(sh, sh.S.suff, K0);
+  if (tid == 0) sh.S.suff.normalized = 1;
+  __syncthreads();
+  const int its = em(sh, sh.S.suff, rs, base, n, cfg, false);
+  distances(sh, sh.S.suff, rs, base, n, true);
+  int splits = 0, merges = 0, pits = 0;
+  if (cfg.useSplitAndMerge) {
+    const float mcEst = sh.S.suff.sumWeights / sh.S.suff.numSamples;
+    splits = recursive_splitting(sh, rs, base, n, mcEst, cfg, pits);
+    clear_split(sh, sh.S.split, V.K);
+    update_split_stats(sh, sh.S.split, rs, base, n, mcEst);
+    merges = perform_merging(sh, sh.S.suff, sh.S.split, cfg);
+  }
+  distances(sh, sh.S.suff, rs, base, n, true);
+  if (tid == 0) {
+    sh.S.afterSplitLo = sh.S.afterSplitHi = sh.S.afterMergeLo = sh.S.afterMergeHi = 0;
+    sh.S.emIterations = its; sh.S.partialEmIterations = pits; sh.S.numSplits = splits; sh.S.numMerges = merges;
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ unsigned long long get64(uint32_t lo, uint32_t hi) {
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+// ---- ASM::update (ASM:352-462)
+__device__ void region_update_body(Shared& sh, const RegionSamples& rs, uint32_t base, uint32_t n,
+                                   const FactoryConfig& cfg) {
+  const int tid = threadIdx.x;
+  VMMParams& V = sh.S.vmm;
+  RegionState& S = sh.S;
+  const int prevK = V.K;
+  const int its = em(sh, S.suff, rs, base, n, cfg, false);
+  if (tid == 0 && prevK < V.K) S.split.K = V.K;
+  __syncthreads();
+  int splits = 0, merges = 0, pits = 0;
+  if (cfg.useSplitAndMerge) {
+    const float mcEst = S.suff.sumWeights / S.suff.numSamples;
+    unsigned long long afterSplit = get64(S.afterSplitLo, S.afterSplitHi) + n;
+    unsigned long long afterMerge = get64(S.afterMergeLo, S.afterMergeHi) + n;
+    update_split_stats(sh, S.split, rs, base, n, mcEst);
+    if (afterSplit >= (unsigned long long)(cfg.minSamplesForSplitting < 0 ? 0 : cfg.minSamplesForSplitting)) {
+      if (tid == 0) {
+        sort_candidates(S.split, sh.order);
+        sh.mask = 0u;
+        int ns = 0;
+        const int numCand = S.split.K;
+        for (int c = 0; c < numCand; c++) {
+          const int idx = sh.order[c];
+          if (S.split.chi[idx] > cfg.splittingThreshold && V.K < kMaxComponents) {
+            split_component(V, S.split, S.suff, idx, cfg);
+            sh.mask |= 1u << idx;
+            sh.mask |= 1u << (V.K - 1);
+            ns++;
+          }
+        }
+        sh.numSplits = ns;
+      }
+      __syncthreads();
+      splits = sh.numSplits;
+      if (splits > 0 && cfg.partialReFit && n >= (uint32_t)(cfg.minSamplesForPartialRefitting < 0 ? 0 : cfg.minSamplesForPartialRefitting)) {
+        copy_suff(sh, sh.tmp, S.suff);
+        clear_suff(sh, sh.tmp, V.K);
+        pits = em(sh, sh.tmp, rs, base, n, cfg, true);
+        if (tid == 0) S.suff.K = V.K;
+        __syncthreads();
+        masked_replace(sh, S.suff, sh.tmp);
+        if (tid == 0) S.split.K = V.K;
+        __syncthreads();
+      }
+      afterSplit = 0;
+    }
+    if (afterMerge >= (unsigned long long)(cfg.minSamplesForMerging < 0 ? 0 : cfg.minSamplesForMerging)) {
+      merges = perform_merging(sh, S.suff, S.split, cfg);
+      afterMerge = 0;
+    }
+    if (tid == 0) {
+      S.afterSplitLo = (uint32_t)afterSplit; S.afterSplitHi = (uint32_t)(afterSplit >> 32);
+      S.afterMergeLo = (uint32_t)afterMerge; S.afterMergeHi = (uint32_t)(afterMerge >> 32);
+    }
+    __syncthreads();
+  }
+  distances(sh, S.suff, rs, base, n, false);
+  if (tid == 0) { S.emIterations = its; S.partialEmIterations = pits; S.numSplits = splits; S.numMerges = merges; }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(RT) k_region_update(RegionState* __restrict__ states,
+                                                      const uint32_t* __restrict__ regionList, RegionSamples rs,
+                                                      FactoryConfig cfg, const float* __restrict__ uniformDirs) {
+  __shared__ Shared sh;
+  const int tid = threadIdx.x;
+  const uint32_t r = regionList[blockIdx.x];
+  {
+    const uint32_t* src = (const uint32_t*)(states + r);
+    uint32_t* dst = (uint32_t*)&sh.S;
+    for (int i = tid; i < (int)(sizeof(RegionState) / 4); i += RT) dst[i] = src[i];
+  }
+  __syncthreads();
+  RegionState& S = sh.S;
+  const int mode = S.mode;
+  const uint32_t n = S.sampleCount, base = S.sampleBegin;
+  if (tid == 0) {
+    S.emIterations = 0; S.partialEmIterations = 0; S.numSplits = 0; S.numMerges = 0;
+    S.estepEvals = 0.f; S.auxEvals = 0.f;
+    sh.W = cfg.vectorWidth;
+  }
+  __syncthreads();
+  if (mode == 2 && S.splitFlag) {  // Field.h:457-461: decay the training statistics of split regions
+    const float a = cfg.decayOnSpatialSplit;
+    if (tid < 32) {
+      const int k = tid;
+      S.suff.swdx[k] *= a; S.suff.swdy[k] *= a; S.suff.swdz[k] *= a; S.suff.sw[k] *= a; S.suff.sdw[k] *= a;
+      if (k < vec_lim(sh.W, S.split.K)) {
+        S.split.covx[k] *= a; S.split.covy[k] *= a; S.split.covz[k] *= a;
+        S.split.sn[k] *= a; S.split.ssw[k] *= a; S.split.ssa[k] *= a;
+      }
+    }
+    if (tid == 0) { S.suff.numSamples *= a; S.suff.sumWeights *= a; S.splitFlag = 0; }
+    __syncthreads();
+  }
+  if (n > 0) {
+    if (mode == 2) {
+      // parallax shift of the statistics and of the mixture to the new pivot
+      // (WEM:265-283, PAVMM:809-831)
+      const float shx = S.vmm.pivot[0] - S.ssMean[0], shy = S.vmm.pivot[1] - S.ssMean[1],
+                  shz = S.vmm.pivot[2] - S.ssMean[2];
+      if (tid < 32) {
+        const int k = tid;
+        VMMParams& V = S.vmm;
+        if (k < V.K) {
+          const float dk = V.dist[k];
+          if (dk > 0.f) {
+            float sx = S.suff.swdx[k], sy = S.suff.swdy[k], sz = S.suff.swdz[k];
+            const float mcos = sqrtf(dot3f(sx, sy, sz, sx, sy, sz));
+            if (mcos > 0.f) {
+              sx = (sx / mcos) * dk + shx;
+              sy = (sy / mcos) * dk + shy;
+              sz = (sz / mcos) * dk + shz;
+              const float l = sqrtf(dot3f(sx, sy, sz, sx, sy, sz));
+              S.suff.swdx[k] = (sx / l) * mcos;
+              S.suff.swdy[k] = (sy / l) * mcos;
+              S.suff.swdz[k] = (sz / l) * mcos;
+            }
+          }
+          const float px = V.mux[k] * dk + shx, py = V.muy[k] * dk + shy, pz = V.muz[k] * dk + shz;
+          const float l = sqrtf(dot3f(px, py, pz, px, py, pz));
+          if (dk > 0.f && isfinite(dk)) { V.mux[k] = px / l; V.muy[k] = py / l; V.muz[k] = pz / l; }
+          V.dist[k] = dk > 0.f ? l : 0.f;
+        }
+      }
+      __syncthreads();
+      if (tid == 0) { S.vmm.pivot[0] -= shx; S.vmm.pivot[1] -= shy; S.vmm.pivot[2] -= shz; }
+      __syncthreads();
+      region_update_body(sh, rs, base, n, cfg);
+      const bool ok = region_valid(sh, true);
+      if (tid == 0) S.valid = ok ? 1 : 0;
+    } else if (mode == 1) {
+      region_fit(sh, rs, base, n, cfg, uniformDirs);
+      if (tid == 0) { S.vmm.pivot[0] = S.ssMean[0]; S.vmm.pivot[1] = S.ssMean[1]; S.vmm.pivot[2] = S.ssMean[2]; }
+      const bool ok = region_valid(sh, false);
+      if (tid == 0) { S.valid = ok ? 1 : 0; S.splitFlag = 0; S.fitted = 1; }
+    }
+  } else if (mode == 1) {
+    if (tid == 0) { S.valid = 0; S.splitFlag = 0; }
+  }
+  __syncthreads();
+  {
+    uint32_t* dst = (uint32_t*)(states + r);
+    const uint32_t* src = (const uint32_t*)&sh.S;
+    for (int i = tid; i < (int)(sizeof(RegionState) / 4); i += RT) dst[i] = src[i];
+  }
+}
+
+// clone the directional state of the ancestor into regions created by spatial splits
+__global__ void __launch_bounds__(256) k_copy_states(RegionState* __restrict__ states, const uint32_t* __restrict__ pairs,
+                                                     uint32_t numPairs) {
+  const uint32_t p = blockIdx.x;
+  if (p >= numPairs) return;
+  const uint32_t dst = pairs[2 * p], src = pairs[2 * p + 1];
+  const uint4* s = (const uint4*)(states + src);
+  uint4* d = (uint4*)(states + dst);
+  for (uint32_t i = threadIdx.x; i < sizeof(RegionState) / 16; i += blockDim.x) d[i] = s[i];
+}
+
+__global__ void __launch_bounds__(256) k_set_inputs(RegionState* __restrict__ states, const RegionInput* __restrict__ in,
+                                                    uint32_t count) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const RegionInput v = in[i];
+  RegionState& S = states[v.region];
+  for (int j = 0; j < 3; j++) { S.ssMean[j] = v.ssMean[j]; S.ssVar[j] = v.ssVar[j]; }
+  S.ssN = v.ssN;
+  S.sampleBegin = v.begin;
+  S.sampleCount = v.count;
+  S.mode = v.mode;
+  S.splitFlag = v.splitFlag;
+  S.emIterations = 0; S.partialEmIterations = 0; S.numSplits = 0; S.numMerges = 0;
+  S.estepEvals = 0.f; S.auxEvals = 0.f;
+}
+
+void launch_copy_states(RegionState* states, const uint32_t* pairs, uint32_t numPairs, hipStream_t s) {
+  if (numPairs == 0) return;
+  hipLaunchKernelGGL(k_copy_states, dim3(numPairs), dim3(256), 0, s, states, pairs, numPairs);
+}
+
+void launch_set_inputs(RegionState* states, const RegionInput* inputs, uint32_t count, hipStream_t s) {
+  if (count == 0) return;
+  hipLaunchKernelGGL(k_set_inputs, dim3((count + 255) / 256), dim3(256), 0, s, states, inputs, count);
+}
+
+void launch_region_update(RegionState* states, const uint32_t* regionList, uint32_t numRegions, const float* dx,
+                          const float* dy, const float* dz, const float* w, const float* pdf, const float* dist,
+                          const FactoryConfig& cfg, const float* uniformDirs, hipStream_t s) {
+  if (numRegions == 0) return;
+  RegionSamples rs{dx, dy, dz, w, pdf, dist};
+  hipLaunchKernelGGL(k_region_update, dim3(numRegions), dim3(RT), 0, s, states, regionList, rs, cfg, uniformDirs);
+}
+
+}  // namespace pglgpu

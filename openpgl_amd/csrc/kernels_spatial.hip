@@ -1,0 +1,341 @@
+// Spatial-side kernels of Field::Update on gfx950: AoS->SoA ingest, scene
+// bounds, the level-synchronous kd push-down with integer statistics, leaf
+// assignment, region ranges and the gather + parallax reprojection into
+// region-contiguous SoA streams.
+//
+// Reference behaviour:
+//   IntegerSampleStatistics::addSample  data/SampleStatistics.h:261-277
+//   partition predicate p[dim] < pivot  spatial/kdtree/KDTreeBuilder.h:193
+//   estimateSceneBounds (skips the last sample)  field/Field.h:326-344
+//   reprojectSample                      directional/vmm/ParallaxAwareVonMisesFisherWeightedEMFactory.h:1151-1181
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include "common.h"
+#include "kernels.h"
+
+namespace pglgpu {
+
+struct SampleAoS {
+  float px, py, pz, dx, dy, dz, w, pdf, dist;
+  uint32_t flags;
+};
+static_assert(sizeof(SampleAoS) == 40, "PGLSampleData is 40 B");
+
+// One coalesced pass over the 40-byte records: positions to SoA for the kd
+// rounds, every sample starts at the root.
+__global__ void __launch_bounds__(256) k_ingest(const SampleAoS* __restrict__ in, uint32_t n,
+                                                float* __restrict__ px, float* __restrict__ py,
+                                                float* __restrict__ pz, uint32_t* __restrict__ nodeOf) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const SampleAoS s = in[i];
+    px[i] = s.px;
+    py[i] = s.py;
+    pz[i] = s.pz;
+    nodeOf[i] = 0u;
+  }
+}
+
+// min/max of positions over [0, n) (the caller passes n-1 to reproduce the
+// reference's skipped last sample); one partial per block.
+__global__ void __launch_bounds__(256) k_bounds(const float* __restrict__ px, const float* __restrict__ py,
+                                                const float* __restrict__ pz, uint32_t n,
+                                                float* __restrict__ partial /* [blocks][6] */) {
+  __shared__ float red[6][256];
+  float lo[3] = {3.402823466e38f, 3.402823466e38f, 3.402823466e38f};
+  float hi[3] = {-3.402823466e38f, -3.402823466e38f, -3.402823466e38f};
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const float p[3] = {px[i], py[i], pz[i]};
+    for (int j = 0; j < 3; j++) {
+      lo[j] = fminf(lo[j], p[j]);
+      hi[j] = fmaxf(hi[j], p[j]);
+    }
+  }
+  for (int j = 0; j < 3; j++) {
+    red[j][threadIdx.x] = lo[j];
+    red[3 + j][threadIdx.x] = hi[j];
+  }
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      for (int j = 0; j < 3; j++) {
+        red[j][threadIdx.x] = fminf(red[j][threadIdx.x], red[j][threadIdx.x + s]);
+        red[3 + j][threadIdx.x] = fmaxf(red[3 + j][threadIdx.x], red[3 + j][threadIdx.x + s]);
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) partial[blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// Integer statistics of one point w.r.t. a node's (centre, 1/half-extent):
+// t = (p - c) * inv; mean term trunc(t * 4096), variance term trunc((t * t) * 4096).
+// Explicit round-to-nearest ops keep the sequence free of contraction.
+__device__ __forceinline__ void int_terms(float p, float c, float inv, long long& m, long long& v) {
+  const float t = __fmul_rn(__fsub_rn(p, c), inv);
+  const float tv = __fmul_rn(__fmul_rn(t, t), 4096.f);
+  const float ts = __fmul_rn(t, 4096.f);
+  m = (long long)ts;  // C++ conversion truncates toward zero, as the reference's Vec3<int64> ctor
+  v = (long long)tv;
+}
+
+__device__ __forceinline__ void acc_add(IntStatsAcc* a, const long long m[3], const long long v[3]) {
+  atomicAdd(&a->n, 1ull);
+  for (int j = 0; j < 3; j++) {
+    atomicAdd((unsigned long long*)&a->sum[j], (unsigned long long)m[j]);
+    atomicAdd((unsigned long long*)&a->sq[j], (unsigned long long)v[j]);
+    atomicMin(&a->lo[j], m[j]);
+    atomicMax(&a->hi[j], m[j]);
+  }
+}
+
+__device__ __forceinline__ void acc_merge(IntStatsAcc* dst, const IntStatsAcc& s) {
+  if (s.n == 0) return;
+  atomicAdd(&dst->n, s.n);
+  for (int j = 0; j < 3; j++) {
+    atomicAdd((unsigned long long*)&dst->sum[j], (unsigned long long)s.sum[j]);
+    atomicAdd((unsigned long long*)&dst->sq[j], (unsigned long long)s.sq[j]);
+    atomicMin(&dst->lo[j], s.lo[j]);
+    atomicMax(&dst->hi[j], s.hi[j]);
+  }
+}
+
+__device__ __forceinline__ void acc_init(IntStatsAcc* a) {
+  a->n = 0;
+  for (int j = 0; j < 3; j++) {
+    a->sum[j] = 0;
+    a->sq[j] = 0;
+    a->lo[j] = 2147483647ll;    //  std::numeric_limits<int>::max()
+    a->hi[j] = -2147483647ll;   // -std::numeric_limits<int>::max()
+  }
+}
+
+__global__ void k_acc_init(IntStatsAcc* a, uint32_t count) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) acc_init(a + i);
+}
+
+// Root statistics (root is a leaf): all samples w.r.t. the tree bounds.
+// Block-private LDS accumulator, one global merge per block.
+__global__ void __launch_bounds__(256) k_root_stats(const float* __restrict__ px, const float* __restrict__ py,
+                                                    const float* __restrict__ pz, uint32_t n, FrontierNode root,
+                                                    IntStatsAcc* __restrict__ out) {
+  __shared__ IntStatsAcc acc;
+  if (threadIdx.x == 0) acc_init(&acc);
+  __syncthreads();
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const float p[3] = {px[i], py[i], pz[i]};
+    long long m[3], v[3];
+    for (int j = 0; j < 3; j++) int_terms(p[j], root.center[j], root.invHalf[j], m[j], v[j]);
+    acc_add(&acc, m, v);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) acc_merge(out, acc);
+}
+
+// One level of the kd push-down.  Samples sitting at a frontier node move to
+// the left (p[dim] < pivot) or right child; the child's integer statistics are
+// accumulated w.r.t. the frontier node's bounds.  For small frontiers the
+// accumulators are privatised in LDS per block.
+template <bool kLds>
+__global__ void __launch_bounds__(256) k_kd_round(const float* __restrict__ px, const float* __restrict__ py,
+                                                  const float* __restrict__ pz, uint32_t n,
+                                                  uint32_t* __restrict__ nodeOf,
+                                                  const int32_t* __restrict__ frontierSlot,
+                                                  const FrontierNode* __restrict__ frontier, uint32_t numFrontier,
+                                                  IntStatsAcc* __restrict__ childAcc) {
+  extern __shared__ IntStatsAcc lacc[];
+  const uint32_t numChildren = 2 * numFrontier;
+  if (kLds) {
+    for (uint32_t c = threadIdx.x; c < numChildren; c += blockDim.x) acc_init(&lacc[c]);
+    __syncthreads();
+  }
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint32_t node = nodeOf[i];
+    const int32_t slot = frontierSlot[node];
+    if (slot < 0) continue;
+    const FrontierNode fn = frontier[slot];
+    const float p[3] = {px[i], py[i], pz[i]};
+    const float pd = fn.dim == 0 ? p[0] : (fn.dim == 1 ? p[1] : p[2]);
+    const uint32_t side = (pd < fn.pivot) ? 0u : 1u;
+    nodeOf[i] = fn.left + side;
+    long long m[3], v[3];
+    for (int j = 0; j < 3; j++) int_terms(p[j], fn.center[j], fn.invHalf[j], m[j], v[j]);
+    if (kLds) acc_add(&lacc[2 * slot + side], m, v);
+    else acc_add(&childAcc[2 * slot + side], m, v);
+  }
+  if (kLds) {
+    __syncthreads();
+    for (uint32_t c = threadIdx.x; c < numChildren; c += blockDim.x) acc_merge(&childAcc[c], lacc[c]);
+  }
+}
+
+// sample -> region key (leaf data index) and identity index for the sort
+__global__ void __launch_bounds__(256) k_leaf_keys(const uint32_t* __restrict__ nodeOf, const KDNode* __restrict__ nodes,
+                                                   uint32_t n, uint32_t* __restrict__ keys,
+                                                   uint32_t* __restrict__ vals) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    keys[i] = nodes[nodeOf[i]].idx();
+    vals[i] = i;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_region_ranges(const uint32_t* __restrict__ sortedKeys, uint32_t n,
+                                                       uint32_t* __restrict__ begin, uint32_t* __restrict__ end) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+    const uint32_t k = sortedKeys[j];
+    if (j == 0 || sortedKeys[j - 1] != k) begin[k] = j;
+    if (j + 1 == n || sortedKeys[j + 1] != k) end[k] = j + 1;
+  }
+}
+
+// Gather into region-contiguous SoA and reproject each sample to its region's
+// pivot: origin = p + d * max(minDist, dist); d' = normalise(origin - pivot).
+// distance == inf keeps the direction (position moves to the pivot);
+// distance <= 0 keeps the sample unchanged.
+__global__ void __launch_bounds__(256) k_gather_reproject(const SampleAoS* __restrict__ in,
+                                                          const uint32_t* __restrict__ sortedKeys,
+                                                          const uint32_t* __restrict__ sortedIdx, uint32_t n,
+                                                          const float4* __restrict__ regionPivot /* xyz, minDist */,
+                                                          float* __restrict__ odx, float* __restrict__ ody,
+                                                          float* __restrict__ odz, float* __restrict__ ow,
+                                                          float* __restrict__ opdf, float* __restrict__ odist) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+    const SampleAoS s = in[sortedIdx[j]];
+    const float4 pv = regionPivot[sortedKeys[j]];
+    float dx = s.dx, dy = s.dy, dz = s.dz, dist = s.dist;
+    if (!isinf(s.dist) && s.dist > 0.f) {
+      const float dd = fmaxf(pv.w, s.dist);
+      const float ox = s.px + s.dx * dd - pv.x;
+      const float oy = s.py + s.dy * dd - pv.y;
+      const float oz = s.pz + s.dz * dd - pv.z;
+      const float len = sqrtf(ox * ox + oy * oy + oz * oz);
+      dx = ox / len;
+      dy = oy / len;
+      dz = oz / len;
+      dist = len;
+    }
+    odx[j] = dx;
+    ody[j] = dy;
+    odz[j] = dz;
+    ow[j] = s.w;
+    opdf[j] = s.pdf;
+    odist[j] = dist;
+  }
+}
+
+// Brute-force 8 nearest region pivots per region (exact, ties by index), the
+// precomputed neighbourhood of KNearestRegionsSearchTree::buildRegionNeighbours
+// (spatial/KNN.h:212-266).
+__global__ void __launch_bounds__(256) k_knn8(const float4* __restrict__ pivots, uint32_t numRegions,
+                                              uint32_t* __restrict__ nbIds, float4* __restrict__ nbPts) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= numRegions) return;
+  const float4 q = pivots[r];
+  float bd[8];
+  uint32_t bi[8];
+  for (int i = 0; i < 8; i++) { bd[i] = 3.402823466e38f; bi[i] = 0xffffffffu; }
+  for (uint32_t i = 0; i < numRegions; i++) {
+    const float4 p = pivots[i];
+    const float dx = __fsub_rn(q.x, p.x), dy = __fsub_rn(q.y, p.y), dz = __fsub_rn(q.z, p.z);
+    const float d = __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
+    if (d < bd[7] || (d == bd[7] && i < bi[7])) {
+      int k = 7;
+      while (k > 0 && (d < bd[k - 1] || (d == bd[k - 1] && i < bi[k - 1]))) {
+        bd[k] = bd[k - 1];
+        bi[k] = bi[k - 1];
+        k--;
+      }
+      bd[k] = d;
+      bi[k] = i;
+    }
+  }
+  for (int i = 0; i < 8; i++) {
+    nbIds[r * 8 + i] = bi[i];
+    const float4 p = bi[i] == 0xffffffffu ? make_float4(0.f, 0.f, 0.f, 0.f) : pivots[bi[i]];
+    nbPts[r * 8 + i] = p;
+  }
+}
+
+// ------------------------------------------------------------------ launchers
+static inline uint32_t grid_for(uint32_t n, uint32_t block, uint32_t maxBlocks = 4096) {
+  uint32_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  return g > maxBlocks ? maxBlocks : g;
+}
+
+void launch_ingest(const void* aos, uint32_t n, float* px, float* py, float* pz, uint32_t* nodeOf, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_ingest, dim3(grid_for(n, 256)), dim3(256), 0, s, (const SampleAoS*)aos, n, px, py, pz, nodeOf);
+}
+
+uint32_t launch_bounds(const float* px, const float* py, const float* pz, uint32_t n, float* partial, hipStream_t s) {
+  const uint32_t g = grid_for(n, 256, 1024);
+  hipLaunchKernelGGL(k_bounds, dim3(g), dim3(256), 0, s, px, py, pz, n, partial);
+  return g;
+}
+
+void launch_acc_init(IntStatsAcc* a, uint32_t count, hipStream_t s) {
+  if (count == 0) return;
+  hipLaunchKernelGGL(k_acc_init, dim3((count + 255) / 256), dim3(256), 0, s, a, count);
+}
+
+void launch_root_stats(const float* px, const float* py, const float* pz, uint32_t n, const FrontierNode& root,
+                       IntStatsAcc* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_root_stats, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, px, py, pz, n, root, out);
+}
+
+void launch_kd_round(const float* px, const float* py, const float* pz, uint32_t n, uint32_t* nodeOf,
+                     const int32_t* frontierSlot, const FrontierNode* frontier, uint32_t numFrontier,
+                     IntStatsAcc* childAcc, hipStream_t s) {
+  const uint32_t numChildren = 2 * numFrontier;
+  const size_t lds = numChildren * sizeof(IntStatsAcc);
+  if (lds <= 48 * 1024) {
+    hipLaunchKernelGGL(k_kd_round<true>, dim3(grid_for(n, 256, 2048)), dim3(256), lds, s, px, py, pz, n, nodeOf,
+                       frontierSlot, frontier, numFrontier, childAcc);
+  } else {
+    hipLaunchKernelGGL(k_kd_round<false>, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, px, py, pz, n, nodeOf,
+                       frontierSlot, frontier, numFrontier, childAcc);
+  }
+}
+
+void launch_leaf_keys(const uint32_t* nodeOf, const KDNode* nodes, uint32_t n, uint32_t* keys, uint32_t* vals,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(k_leaf_keys, dim3(grid_for(n, 256)), dim3(256), 0, s, nodeOf, nodes, n, keys, vals);
+}
+
+size_t sort_pairs_temp_bytes(uint32_t n) {
+  size_t bytes = 0;
+  hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                     (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0, 32);
+  return bytes;
+}
+
+void sort_pairs(void* temp, size_t tempBytes, const uint32_t* keysIn, uint32_t* keysOut, const uint32_t* valsIn,
+                uint32_t* valsOut, uint32_t n, int endBit, hipStream_t s) {
+  hipcub::DeviceRadixSort::SortPairs(temp, tempBytes, keysIn, keysOut, valsIn, valsOut, (int)n, 0, endBit, s);
+}
+
+void launch_region_ranges(const uint32_t* sortedKeys, uint32_t n, uint32_t* begin, uint32_t* end, hipStream_t s) {
+  hipLaunchKernelGGL(k_region_ranges, dim3(grid_for(n, 256)), dim3(256), 0, s, sortedKeys, n, begin, end);
+}
+
+void launch_gather_reproject(const void* aos, const uint32_t* sortedKeys, const uint32_t* sortedIdx, uint32_t n,
+                             const float4* regionPivot, float* dx, float* dy, float* dz, float* w, float* pdf,
+                             float* dist, hipStream_t s) {
+  hipLaunchKernelGGL(k_gather_reproject, dim3(grid_for(n, 256)), dim3(256), 0, s, (const SampleAoS*)aos, sortedKeys,
+                     sortedIdx, n, regionPivot, dx, dy, dz, w, pdf, dist);
+}
+
+void launch_knn8(const float4* pivots, uint32_t numRegions, uint32_t* nbIds, float4* nbPts, hipStream_t s) {
+  hipLaunchKernelGGL(k_knn8, dim3((numRegions + 255) / 256), dim3(256), 0, s, pivots, numRegions, nbIds, nbPts);
+}
+
+}  // namespace pglgpu

@@ -44,6 +44,8 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <cstdio>
+#include <cstdlib>
 #include <limits>
 #include <atomic>
 #include <thread>
@@ -323,10 +325,11 @@ template <class R> struct PreparedSample { R d[3]; R w, pdf, dist; };
 
 // ----------------------------------------------------------- the factory
 template <class R> struct Factory {
-  const orc_config& c;
+  const orc_config c;  // by value: callers may free their config after construction
   R uniformDirs[MAXK][MAXK][3];
   R maxMeanCosine;
   int W;
+  int stopPhase = 0;  // debugging aid: stop fit() after a phase (0 = run to the end)
   explicit Factory(const orc_config& cfg) : c(cfg) {
     W = cfg.vectorWidth;
     maxMeanCosine = KappaToMeanCosine<R>(R(cfg.maxKappa));
@@ -887,6 +890,7 @@ template <class R> struct Factory {
           }
         }
       if (!found) break;
+      if (getenv("ORC_DEBUG_MERGE")) fprintf(stderr, "merge %d %d cost %g K %d\n", ci, cj, double(minCost), v.K);
       R mI[3] = {v.mu[ci][0], v.mu[ci][1], v.mu[ci][2]};
       const R wI = v.w[ci];
       R mJ[3] = {v.mu[cj][0], v.mu[cj][1], v.mu[cj][2]};
@@ -912,13 +916,17 @@ template <class R> struct Factory {
     t.suff.normalized = true;
     int its = em(v, t.suff, s, n, nullptr);
     distances(v, t.suff, s, n, true);
+    if (stopPhase == 1) return;
     int splits = 0, merges = 0, pits = 0;
     if (c.useSplitAndMerge) {
       const R mc = t.suff.sumWeights / t.suff.numSamples;
       splits = recursiveSplitting(v, t.suff, mc, s, n, pits);
+      if (stopPhase == 2) return;
       t.split.clear(v.K, W);
       updateSplitStats(v, t.split, mc, s, n);
+      if (stopPhase == 3) return;
       merges = performMerging(v, t.suff, t.split);
+      if (stopPhase == 4) return;
     }
     t.afterSplit = 0; t.afterMerge = 0;
     distances(v, t.suff, s, n, true);
@@ -1738,6 +1746,12 @@ void orc_region_step(orc_region r, const orc_sample* s, size_t n, const float* m
   if (r->precision == 0) regionStep(r->r32, *r->fac32, r->first, s, n, m, v, ns, splitFlag);
   else regionStep(r->r64, *r->fac64, r->first, s, n, m, v, ns, splitFlag);
 }
+void orc_region_fit_phase(orc_region r, const orc_sample* s, size_t n, const float* m, const float* v, float ns, int phase) {
+  if (r->precision == 0) r->fac32->stopPhase = phase; else r->fac64->stopPhase = phase;
+  orc_region_step(r, s, n, m, v, ns, 0);
+  if (r->precision == 0) r->fac32->stopPhase = 0; else r->fac64->stopPhase = 0;
+}
+
 void orc_region_export(orc_region r, orc_region_record* o) {
   if (r->precision == 0) exportRegion(r->r32, o); else exportRegion(r->r64, o);
 }
@@ -1745,6 +1759,56 @@ float orc_region_pdf(orc_region r, const float* d) {
   if (r->precision == 0) { const float dd[3] = {d[0], d[1], d[2]}; return vmmPdf(r->r32.vmm, dd); }
   const double dd[3] = {d[0], d[1], d[2]};
   return float(vmmPdf(r->r64.vmm, dd));
+}
+
+}  // extern "C"
+
+template <class R>
+static float regionQuery(const Region<R>& reg, int volume, const float* pos, const float* nrm, float g, const float* u2,
+                         const float* pdfDir, float* pdfAtDir, float* dirOut, float* pdfOut) {
+  QueryDist<R> q;
+  q.li = reg.vmm;
+  const R shift[3] = {q.li.pivot[0] - R(pos[0]), q.li.pivot[1] - R(pos[1]), q.li.pivot[2] - R(pos[2])};
+  vmmShift(q.li, shift);
+  q.d[0] = q.li; q.weights[0] = 1; q.num = 1; q.integral = 1;
+  if (volume) applyHG(q, nrm, g); else applyCosine(q, nrm);
+  const R pd[3] = {R(pdfDir[0]), R(pdfDir[1]), R(pdfDir[2])};
+  *pdfAtDir = float(queryPdf(q, pd));
+  R d[3];
+  querySample(q, R(u2[0]), R(u2[1]), d);
+  for (int j = 0; j < 3; j++) dirOut[j] = float(d[j]);
+  *pdfOut = float(queryPdf(q, d));
+  return float(q.integral);
+}
+
+extern "C" {
+
+float orc_region_query(orc_region r, int volume, const float* pos, const float* nrm, float g, const float* u2,
+                       const float* pdfDir, float* pdfAtDir, float* dirOut, float* pdfOut) {
+  if (r->precision == 0) return regionQuery(r->r32, volume, pos, nrm, g, u2, pdfDir, pdfAtDir, dirOut, pdfOut);
+  return regionQuery(r->r64, volume, pos, nrm, g, u2, pdfDir, pdfAtDir, dirOut, pdfOut);
+}
+
+void orc_stats_merge_split(const float* a, const float* b, int dim, float pos, float* m, float* l, float* r) {
+  auto load = [](const float* x) {
+    SampleStats s;
+    for (int j = 0; j < 3; j++) { s.mean[j] = x[j]; s.var[j] = x[3 + j]; s.lo[j] = x[7 + j]; s.hi[j] = x[10 + j]; }
+    s.n = x[6];
+    return s;
+  };
+  auto store = [](const SampleStats& s, float* x) {
+    for (int j = 0; j < 3; j++) { x[j] = s.mean[j]; x[3 + j] = s.var[j]; x[7 + j] = s.lo[j]; x[10 + j] = s.hi[j]; }
+    x[6] = s.n;
+  };
+  SampleStats sa = load(a), sb = load(b);
+  SampleStats sm = sa;
+  sm.merge(sb);
+  store(sm, m);
+  SampleStats sl = sa, sr = sa;
+  sl.split(dim, pos, 0.25f, false);
+  sr.split(dim, pos, 0.25f, true);
+  store(sl, l);
+  store(sr, r);
 }
 
 void orc_integer_stats(const float* pos, size_t n, const float* lo, const float* hi, float* out, int64_t* raw) {

@@ -125,6 +125,14 @@ void orc_region_step(orc_region r, const orc_sample* samples, size_t n, const fl
 void orc_region_export(orc_region r, orc_region_record* out);
 float orc_region_pdf(orc_region r, const float* dir);
 
+// init at pos + cosine (volume=0) or HG (volume=1) product on a single region,
+// pdf at pdfDir, one sample (dirOut, pdfOut); returns the product integral
+float orc_region_query(orc_region r, int volume, const float* pos, const float* nrm, float g, const float* u2,
+                       const float* pdfDir, float* pdfAtDir, float* dirOut, float* pdfOut);
+// SampleStatistics::merge(a, b) and split of a at (dim, pos) into lower/upper halves (13 floats each)
+void orc_stats_merge_split(const float* a13, const float* b13, int dim, float pos, float* merged13, float* left13,
+                           float* right13);
+
 // integer sample statistics of a point set with given bounds, converted to
 // SampleStatistics (KDTreeBuilder semantics).  out: mean3 var3 n lower3 upper3
 void orc_integer_stats(const float* pos, size_t n, const float* lower3, const float* upper3,

@@ -105,6 +105,53 @@ void ref_region_step(void* p, const orc_sample* s, size_t n, const float* m, con
   }
 }
 
+// ASM::fit stopped after a phase: 1 EM + distances, 2 recursive splitting,
+// 3 split statistics, 4 merging (AdaptiveSplitandMergeFactory.h:303-347)
+void ref_region_fit_phase(void* p, const orc_sample* s, size_t n, const float* m, const float* v, float ns, int phase) {
+  RefRegion* r = (RefRegion*)p;
+  r->ss.mean = Point3(m[0], m[1], m[2]);
+  r->ss.sampleVariance = Vector3(v[0], v[1], v[2]);
+  r->ss.numSamples = ns;
+  std::vector<SampleData> data(n);
+  memcpy(data.data(), s, n * sizeof(SampleData));
+  std::sort(data.begin(), data.end(), SampleDataLess);
+  ASM8 factory;
+  factory.prepareSamples(data.data(), n, r->ss, r->cfg);
+  typedef ASM8::WeightedEMFactory WEM;
+  const auto& cfg = r->cfg;
+  r->stats.clear(cfg.weightedEMCfg.initK);
+  WEM wem;
+  typename WEM::FittingStatistics fs;
+  wem.fitMixture(r->vmm, r->stats.sufficientStatistics, data.data(), n, cfg.weightedEMCfg, fs);
+  wem.initComponentDistances(r->vmm, r->stats.sufficientStatistics, data.data(), n);
+  if (phase >= 2) {
+    float mc = r->stats.sufficientStatistics.getSumWeights() / r->stats.sufficientStatistics.getNumSamples();
+    ASM8::Splitter splitter;
+    splitter.PerformRecursiveSplitting(r->vmm, r->stats.sufficientStatistics, cfg.splittingThreshold, mc, data.data(), n, cfg.weightedEMCfg);
+    if (phase >= 3) splitter.CalculateSplitStatistics(r->vmm, r->stats.splittingStatistics, mc, data.data(), n);
+    if (phase >= 4) {
+      ASM8::Merger merger;
+      merger.PerformMerging(r->vmm, cfg.mergingThreshold, r->stats.sufficientStatistics, r->stats.splittingStatistics);
+    }
+  }
+  r->first = false;
+}
+
+// one ThresholdedMergeNext step (VMMChiSquareComponentMerger.h:218-279); returns the merge cost or -1
+float ref_region_merge_step(void* p) {
+  RefRegion* r = (RefRegion*)p;
+  ASM8::Merger merger;
+  float cost = -1.f;
+  const bool found = merger.ThresholdedMergeNext(r->vmm, r->cfg.mergingThreshold, cost, r->stats.sufficientStatistics,
+                                                 r->stats.splittingStatistics);
+  return found ? cost : 1e30f;
+}
+float ref_region_merge_cost(void* p, int i, int j) {
+  RefRegion* r = (RefRegion*)p;
+  ASM8::Merger merger;
+  return merger.CalculateMergeCost(r->vmm, i, j);
+}
+
 void ref_region_export(void* p, orc_region_record* o) {
   RefRegion* r = (RefRegion*)p;
   memset(o, 0, sizeof(*o));

@@ -1,0 +1,89 @@
+"""Helpers comparing a GPU field (through the C ABI) with the oracle field.
+Test infrastructure only."""
+import ctypes
+
+import numpy as np
+
+from oracle_ctypes import RegionRecord, oracle, ptr
+
+
+class OracleField:
+    def __init__(self, cfg, precision=0):
+        self.lib = oracle()
+        self.h = self.lib.orc_field_new(ctypes.byref(cfg), precision)
+
+    def update(self, samples, threads=8):
+        s = np.ascontiguousarray(samples)
+        self.lib.orc_field_update(self.h, ptr(s), len(s), threads)
+
+    def set_scene_bounds(self, lo, hi):
+        lo = np.asarray(lo, np.float32); hi = np.asarray(hi, np.float32)
+        self.lib.orc_field_set_scene_bounds(self.h, ptr(lo), ptr(hi))
+
+    def nodes(self):
+        n = self.lib.orc_field_num_nodes(self.h)
+        piv = np.zeros(n, np.float32); dims = np.zeros(n, np.uint32); cod = np.zeros(n, np.uint32)
+        self.lib.orc_field_export_nodes(self.h, ptr(piv), ptr(dims), ptr(cod))
+        return piv, dims, cod
+
+    def num_regions(self):
+        return self.lib.orc_field_num_regions(self.h)
+
+    def region(self, i):
+        r = RegionRecord()
+        self.lib.orc_field_export_region(self.h, i, ctypes.byref(r))
+        return r.to_dict()
+
+    def leaf_ids(self, pos):
+        p = np.ascontiguousarray(pos, np.float32)
+        out = np.zeros(len(p), np.uint32)
+        self.lib.orc_field_sample_leaf_ids(self.h, ptr(p), len(p), ptr(out))
+        return out
+
+    def knn(self):
+        out = np.zeros(self.num_regions() * 8, np.uint32)
+        self.lib.orc_field_knn(self.h, ptr(out))
+        return out.reshape(-1, 8)
+
+    def query_batch(self, volume, pos, nrm, u2, u1=None, g=None, threads=8):
+        n = len(pos)
+        pos = np.ascontiguousarray(pos, np.float32); nrm = np.ascontiguousarray(nrm, np.float32)
+        u2 = np.ascontiguousarray(u2, np.float32)
+        u1 = None if u1 is None else np.ascontiguousarray(u1, np.float32)
+        g = None if g is None else np.ascontiguousarray(g, np.float32)
+        d = np.zeros((n, 3), np.float32); pdf = np.zeros(n, np.float32); ids = np.zeros(n, np.uint32)
+        integ = np.zeros(n, np.float32)
+        self.lib.orc_field_query_batch(self.h, int(volume), n, ptr(pos), ptr(nrm), ptr(g) if g is not None else None,
+                                       ptr(u2), ptr(u1) if u1 is not None else None, ptr(d), ptr(pdf), ptr(ids),
+                                       ptr(integ), threads)
+        return d, pdf, ids, integ
+
+    def query_pdf(self, volume, pos, nrm, g, u1, d):
+        out = np.zeros(1, np.float32)
+        p = np.ascontiguousarray(pos, np.float32); nn = np.ascontiguousarray(nrm, np.float32)
+        dd = np.ascontiguousarray(d, np.float32)
+        ok = self.lib.orc_field_query_pdf(self.h, int(volume), ptr(p), ptr(nn), float(g), float(u1), ptr(dd), ptr(out))
+        return ok, float(out[0])
+
+    def __del__(self):
+        try:
+            self.lib.orc_field_release(self.h)
+        except Exception:
+            pass
+
+
+def vmm_pdf(rec, dirs):
+    """pdf of an exported region record's mixture at unit directions (FP64 numpy)."""
+    K = int(rec["K"])
+    mu = rec["meanDirections"][:K].astype(np.float64)
+    kap = rec["kappas"][:K].astype(np.float64)
+    w = rec["weights"][:K].astype(np.float64)
+    nrm = rec["normalizations"][:K].astype(np.float64)
+    c = dirs.astype(np.float64) @ mu.T
+    return (w * nrm * np.exp(kap * np.minimum(c - 1.0, 0.0))).sum(axis=1)
+
+
+def unit_dirs(n, seed=7):
+    rng = np.random.default_rng(seed)
+    d = rng.normal(size=(n, 3))
+    return d / np.linalg.norm(d, axis=1, keepdims=True)

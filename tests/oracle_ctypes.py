@@ -212,3 +212,34 @@ def region_sample_stats(samples):
     mean = p.mean(axis=0)
     var = ((p - mean) ** 2).sum(axis=0)
     return mean.astype(np.float32), var.astype(np.float32), float(len(samples))
+
+
+def _extra_sigs(lib):
+    vp = ctypes.c_void_p
+    lib.orc_region_query.restype = ctypes.c_float
+    lib.orc_region_query.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_float, vp, vp, vp, vp, vp]
+    lib.orc_stats_merge_split.argtypes = [vp, vp, ctypes.c_int, ctypes.c_float, vp, vp, vp]
+
+
+def oracle_full():
+    lib = oracle()
+    _extra_sigs(lib)
+    return lib
+
+
+def load_hg_table_into_oracle():
+    """The HG phase-function table comes from the golden fixture (reference data)."""
+    t = np.load(os.path.join(ROOT, "tests", "golden", "hg_table.npy"))
+    t = np.ascontiguousarray(t, np.float32)
+    oracle().orc_set_hg_table(ptr(t))
+    return t
+
+
+def region_query(region, volume, pos, nrm, g, u2, pdf_dir):
+    lib = oracle_full()
+    pos = np.ascontiguousarray(pos, np.float32); nrm = np.ascontiguousarray(nrm, np.float32)
+    u2 = np.ascontiguousarray(u2, np.float32); pdf_dir = np.ascontiguousarray(pdf_dir, np.float32)
+    pa = np.zeros(1, np.float32); d = np.zeros(3, np.float32); p = np.zeros(1, np.float32)
+    integ = lib.orc_region_query(region.h, int(volume), ptr(pos), ptr(nrm), float(g), ptr(u2), ptr(pdf_dir),
+                                 ptr(pa), ptr(d), ptr(p))
+    return integ, float(pa[0]), d, float(p[0])
