@@ -1,0 +1,188 @@
+"""Benchmark of the MI355X guiding field.
+
+Workload (headline line): 8 successive frames of incremental Field::Update,
+1M surface samples per frame from the synthetic generator (16 Gaussian blobs,
+sigma 0.05, centres drifting 0.01 per frame, 4-lobe vMF directions), default
+Open PGL settings (maxSamplesPerLeaf 32000, max depth 32, initK 16, maxK 32,
+split-and-merge, deterministic).  Frames 0..W-1 are warm-up (frame 0 builds
+the field), frames W..W+K-1 are timed.  All frames' records are resident in
+HBM before timing (pglFieldUpdateSurfaceDevice).
+
+value = timed samples / timed seconds.  The roofline object reports the
+region kernel (the dominant kernel: weighted EM + split/merge for all regions
+in one launch) against the FP32 vector peak; its time comes from HIP events
+recorded inside the library on the kernel's stream.  cpu_baseline times the
+oracle (a CPU restatement of the reference, std::thread over regions) on a
+bounded sample of the same workload.  A closed-loop serving run (batches of
+log-uniform size in [1k, 256k] queries) reports guided samples/s and per-batch
+p50/p99 latency.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector (packed) peak
+PEAK_HBM_GBS = 8000.0
+N_PER_FRAME = 1_000_000
+# algorithmic FP32 flops per (sample, component) evaluation of each pass
+FLOPS_ESTEP = 20.0
+FLOPS_SPLIT = 47.0
+FLOPS_DIST = 20.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_frames(n, frames, seed=0):
+    from openpgl_amd.synth import generate
+    return [generate(n, seed=seed, frame=f) for f in range(frames)]
+
+
+def run_gpu(frames_host, warmup, steps):
+    import openpgl_amd as pgl
+    from openpgl_amd.hip_runtime import DeviceArray, synchronize
+    dev = pgl.Device()
+    field = pgl.Field(dev, max_samples_per_leaf=32000, deterministic=True)
+    dframes = [DeviceArray.from_numpy(f) for f in frames_host]
+    synchronize()
+    for f in range(warmup):
+        field.update_surface_device(dframes[f].value, len(frames_host[f]))
+        log("warmup frame", f, "regions", field.num_regions(), field.last_update_info())
+    synchronize()
+    infos = []
+    t0 = time.perf_counter()
+    for s in range(steps):
+        f = warmup + s
+        field.update_surface_device(dframes[f].value, len(frames_host[f]))
+        infos.append(field.last_update_info())
+    synchronize()
+    t1 = time.perf_counter()
+    for s, i in enumerate(infos):
+        log("timed frame", warmup + s, i)
+    return field, dev, (t1 - t0), infos
+
+
+def serving(field, n_batches=40, seed=0):
+    """Closed loop: each batch is a log-uniform [1k, 256k] set of queries (30%
+    volume is not trained here -> surface only), inputs resident on device."""
+    from openpgl_amd.hip_runtime import DeviceArray, hip, synchronize
+    from openpgl_amd.synth import queries
+    import ctypes
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(1000), np.log(256000), n_batches)).astype(int)
+    nmax = int(sizes.max())
+    pos, nrm, u2, _, _ = queries(nmax, seed=seed)
+    u1 = rng.random(nmax).astype(np.float32)
+    dpos, dnrm, du2, du1 = (DeviceArray.from_numpy(a) for a in (pos, nrm, u2, u1))
+    ddir = DeviceArray(nmax * 12)
+    dpdf = DeviceArray(nmax * 4)
+    lib = field.lib
+    synchronize()
+    # warm-up
+    lib.pglFieldSampleBatchDevice(field.h, 1000, dpos.ptr, dnrm.ptr, None, du2.ptr, du1.ptr, None, ddir.ptr,
+                                  dpdf.ptr, None, None, None)
+    synchronize()
+    lat = []
+    total = 0
+    t0 = time.perf_counter()
+    for n in sizes:
+        a = time.perf_counter()
+        lib.pglFieldSampleBatchDevice(field.h, int(n), dpos.ptr, dnrm.ptr, None, du2.ptr, du1.ptr, None, ddir.ptr,
+                                      dpdf.ptr, None, None, None)
+        synchronize()
+        lat.append((time.perf_counter() - a) * 1e3)
+        total += int(n)
+    t1 = time.perf_counter()
+    pdf = dpdf.to_numpy(np.float32, nmax)
+    return {"guided_samples_per_s": total / (t1 - t0), "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "batches": int(n_batches), "queries": int(total),
+            "valid_fraction": float(np.mean(pdf[:sizes[-1]] > 0)), "mode": "closed-loop, device-resident inputs"}
+
+
+def cpu_baseline(frames_host, threads):
+    """Oracle on a bounded sample: frame 0 (build, untimed) then frame 1 (timed update)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from field_compare import OracleField
+    from oracle_ctypes import default_config
+    o = OracleField(default_config(32000))
+    o.update(frames_host[0], threads)
+    t0 = time.perf_counter()
+    o.update(frames_host[1], threads)
+    dt = time.perf_counter() - t0
+    return {"value": len(frames_host[1]) / dt, "unit": "samples/s", "cores": threads, "kind": "port",
+            "sample": "oracle (CPU restatement of Open PGL 0.6.0), one incremental Update of frame 1 "
+                      "(1M samples) after the frame-0 build; %.1f s" % dt}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=6)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--samples", type=int, default=N_PER_FRAME)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md for how it would "
+              "shard); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    nframes = args.warmup + args.steps
+    log("generating", nframes, "frames of", args.samples, "samples")
+    frames = make_frames(args.samples, nframes)
+    field, dev, dt, infos = run_gpu(frames, args.warmup, args.steps)
+    timed = sum(len(frames[args.warmup + s]) for s in range(args.steps))
+    value = timed / dt
+    # roofline of the region kernel over the timed frames
+    kern_ms = sum(i["directional_ms"] for i in infos)
+    flops = sum(FLOPS_ESTEP * i["estep_evals"] + FLOPS_SPLIT * i["split_evals"] + FLOPS_DIST * i["dist_evals"]
+                for i in infos)
+    achieved = flops / (kern_ms * 1e-3) / 1e12
+    per_launch = {
+        "avg_region_kernel_ms": kern_ms / len(infos),
+        "algorithmic_flops_per_launch": flops / len(infos),
+        "spatial_ms_avg": float(np.mean([i["spatial_ms"] for i in infos])),
+        "sort_ms_avg": float(np.mean([i["sort_ms"] for i in infos])),
+        "gather_ms_avg": float(np.mean([i["gather_ms"] for i in infos])),
+        "knn_ms_avg": float(np.mean([i["knn_ms"] for i in infos])),
+        "wall_ms_avg": float(np.mean([i["wall_ms"] for i in infos])),
+        "regions_last": int(infos[-1]["regions"]),
+    }
+    out = {
+        "metric": "Field::Update samples/s; guided samples/s + p50/p99 at fixed PDF error",
+        "value": value,
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (16 drifting Gaussian blobs, 4-lobe vMF directions, seed 0)",
+        "config": {"workload": "8 frames x 1M surface samples, incremental Field::Update, blobs drifting 0.01/frame, "
+                               "split-and-merge on, maxK 32, kd max depth 32, maxSamplesPerLeaf 32000",
+                   "samples_per_frame": args.samples, "frames_timed": args.steps, "parallelism": "single GPU"},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None, "kernel": "k_region_update",
+                     **per_launch},
+    }
+    if not args.no_serving:
+        out["serving"] = serving(field)
+    if not args.no_cpu:
+        threads = min(os.cpu_count() or 1, 16)
+        out["cpu_baseline"] = cpu_baseline(frames, threads)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -265,8 +265,9 @@ void pglFieldGetKNN(PGLField field, int kind, uint32_t* neighbours /* 8 per regi
  * [0] total wall, [1] spatial (GPU), [2] directional region kernel (GPU),
  * [3] KNN (GPU), [4] gather+reproject (GPU), [5] sort (GPU), [6] kd rounds,
  * [7] region kernel launches, [8] E-step sample x component evaluations,
- * [9] split/distance-pass evaluations, [10] EM iterations (sum), [11] samples */
-void pglFieldGetLastUpdateInfo(PGLField field, int kind, double* out12);
+ * [9] distance-pass evaluations, [10] EM iterations (sum), [11] samples,
+ * [12] split-statistics evaluations, [13] regions */
+void pglFieldGetLastUpdateInfo(PGLField field, int kind, double* out14);
 /* EXTENSION: directional scheduling: 0 = one launch for all regions
  * (default), 1 = one launch per region (the plain per-region path). */
 void pglFieldSetRegionSchedule(PGLField field, int schedule);

@@ -298,10 +298,10 @@ class Field:
         return out.reshape(-1, 8)
 
     def last_update_info(self, kind=0):
-        o = np.zeros(12, np.float64)
+        o = np.zeros(14, np.float64)
         self.lib.pglFieldGetLastUpdateInfo(self.h, kind, _ptr(o))
         keys = ["wall_ms", "spatial_ms", "directional_ms", "knn_ms", "gather_ms", "sort_ms", "kd_rounds",
-                "region_launches", "estep_evals", "aux_evals", "em_iterations", "samples"]
+                "region_launches", "estep_evals", "dist_evals", "em_iterations", "samples", "split_evals", "regions"]
         return dict(zip(keys, o.tolist()))
 
     # --- queries

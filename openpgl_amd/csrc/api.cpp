@@ -728,14 +728,15 @@ void pglFieldGetLastUpdateInfo(PGLField field, int kind, double* o) {
   o[0] = i.wallMs; o[1] = i.spatialMs; o[2] = i.directionalMs; o[3] = i.knnMs; o[4] = i.gatherMs; o[5] = i.sortMs;
   o[6] = i.kdRounds; o[7] = i.regionLaunches;
   k.syncMirror(field->gf->stream);
-  double ee = 0, ae = 0, it = 0;
+  double ee = 0, de = 0, se = 0, it = 0;
   for (uint32_t r = 0; r < k.numRegions(); r++) {
     const RegionState& S = k.mirrorRegion(r);
     ee += S.estepEvals;
-    ae += S.auxEvals;
+    de += S.auxEvals;
+    se += S.splitEvals;
     it += S.emIterations + S.partialEmIterations;
   }
-  o[8] = ee; o[9] = ae; o[10] = it; o[11] = i.samples;
+  o[8] = ee; o[9] = de; o[10] = it; o[11] = i.samples; o[12] = se; o[13] = k.numRegions();
 }
 
 void pglFieldSetRegionSchedule(PGLField field, int schedule) {

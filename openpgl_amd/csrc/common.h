@@ -93,8 +93,9 @@ struct RegionState {
   int32_t fitted;
   float lastLogLikelihood;
   float estepEvals;     // sample x component evaluations in E-steps of the last update
-  float auxEvals;       // sample x component evaluations in split-statistics / distance passes
-  uint32_t pad[318];
+  float auxEvals;       // sample x component evaluations in distance passes
+  float splitEvals;     // sample x component evaluations in split-statistics passes
+  uint32_t pad[317];
 };
 static_assert(sizeof(RegionState) == 4096, "RegionState must be 4 KiB");
 

@@ -264,25 +264,78 @@ void KindField::visitLeaf(uint32_t node, uint32_t depth, const float lo[3], cons
 
 void KindField::kdUpdate(uint32_t n, hipStream_t s) {
   FrontierItemList frontier, next;
-  uint32_t rounds = 0;
-  dAcc_.ensure(2 * sizeof(IntStatsAcc));
-  if (nodes_[0].isLeaf()) {
-    FrontierNode root{};
-    float half[3];
-    bounds_params(treeLo_, treeHi_, root.center, half, root.invHalf);
-    launch_acc_init(dAcc_.as<IntStatsAcc>(), 1, s);
-    launch_root_stats(dPx_.as<float>(), dPy_.as<float>(), dPz_.as<float>(), n, root, dAcc_.as<IntStatsAcc>(), s);
-    IntStatsAcc acc;
-    PGL_HIP_CHECK(hipMemcpyAsync(&acc, dAcc_.p, sizeof(acc), hipMemcpyDeviceToHost, s));
+  uint32_t rounds = 1;
+  // Leaf table of the current tree: every sample first descends through the
+  // existing inner nodes to its leaf (those levels take no decisions), with
+  // the leaf's statistics taken w.r.t. its parent's bounds.
+  struct LeafInfo {
+    uint32_t node, depth;
+    float lo[3], hi[3];
+    float center[3], half[3], invHalf[3];
+  };
+  std::vector<LeafInfo> leaves;
+  std::vector<int32_t> leafSlot(nodes_.size(), -1);
+  {
+    struct Item { uint32_t node, depth; float lo[3], hi[3], plo[3], phi[3]; };
+    std::vector<Item> stack;
+    Item root;
+    root.node = 0;
+    root.depth = 1;
+    for (int i = 0; i < 3; i++) { root.lo[i] = root.plo[i] = treeLo_[i]; root.hi[i] = root.phi[i] = treeHi_[i]; }
+    stack.push_back(root);
+    while (!stack.empty()) {
+      const Item it = stack.back();
+      stack.pop_back();
+      const KDNode nd = nodes_[it.node];
+      if (nd.isLeaf()) {
+        LeafInfo li;
+        li.node = it.node;
+        li.depth = it.depth;
+        for (int i = 0; i < 3; i++) { li.lo[i] = it.lo[i]; li.hi[i] = it.hi[i]; }
+        bounds_params(it.plo, it.phi, li.center, li.half, li.invHalf);
+        leafSlot[it.node] = (int32_t)leaves.size();
+        leaves.push_back(li);
+        continue;
+      }
+      const uint32_t d = nd.dim(), l = nd.idx();
+      Item a, b;
+      a.node = l; b.node = l + 1;
+      a.depth = b.depth = it.depth + 1;
+      for (int i = 0; i < 3; i++) {
+        a.lo[i] = b.lo[i] = a.plo[i] = b.plo[i] = it.lo[i];
+        a.hi[i] = b.hi[i] = a.phi[i] = b.phi[i] = it.hi[i];
+      }
+      a.hi[d] = nd.pivot;
+      b.lo[d] = nd.pivot;
+      stack.push_back(b);
+      stack.push_back(a);
+    }
+  }
+  const uint32_t L = (uint32_t)leaves.size();
+  std::vector<float> params(6 * L);
+  for (uint32_t i = 0; i < L; i++)
+    for (int j = 0; j < 3; j++) { params[6 * i + j] = leaves[i].center[j]; params[6 * i + 3 + j] = leaves[i].invHalf[j]; }
+  dNodes_.ensure(nodes_.size() * sizeof(KDNode));
+  dFrontierSlot_.ensure(std::max<size_t>(leafSlot.size(), 1) * sizeof(int32_t));
+  dFrontier_.ensure(std::max<size_t>(params.size() * 4, sizeof(FrontierNode)));
+  dAcc_.ensure(std::max<size_t>(L, 2) * sizeof(IntStatsAcc));
+  PGL_HIP_CHECK(hipMemcpyAsync(dNodes_.p, nodes_.data(), nodes_.size() * sizeof(KDNode), hipMemcpyHostToDevice, s));
+  PGL_HIP_CHECK(hipMemcpyAsync(dFrontierSlot_.p, leafSlot.data(), leafSlot.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  PGL_HIP_CHECK(hipMemcpyAsync(dFrontier_.p, params.data(), params.size() * 4, hipMemcpyHostToDevice, s));
+  dStatPartial_.ensure(spatial_partial_bytes(n));
+  launch_descend_stats(dPx_.as<float>(), dPy_.as<float>(), dPz_.as<float>(), n, dNodes_.as<KDNode>(),
+                       dFrontierSlot_.as<int32_t>(), dFrontier_.p, L, dNodeOf_.as<uint32_t>(),
+                       dStatPartial_.as<IntStatsAcc>(), dAcc_.as<IntStatsAcc>(), s);
+  {
+    std::vector<IntStatsAcc> acc(L);
+    PGL_HIP_CHECK(hipMemcpyAsync(acc.data(), dAcc_.p, L * sizeof(IntStatsAcc), hipMemcpyDeviceToHost, s));
     PGL_HIP_CHECK(hipStreamSynchronize(s));
-    const HostSampleStats st = int_stats_to_float(acc, root.center, half);
-    visitLeaf(0, 1, treeLo_, treeHi_, n, st, frontier);
-  } else {
-    FrontierItem it;
-    it.node = 0;
-    it.depth = 1;
-    for (int i = 0; i < 3; i++) { it.lo[i] = treeLo_[i]; it.hi[i] = treeHi_[i]; }
-    frontier.items.push_back(it);
+    for (uint32_t i = 0; i < L; i++) {
+      if (acc[i].n == 0) continue;
+      const LeafInfo& li = leaves[i];
+      const HostSampleStats st = int_stats_to_float(acc[i], li.center, li.half);
+      visitLeaf(li.node, li.depth, li.lo, li.hi, (uint32_t)acc[i].n, st, frontier);
+    }
   }
   std::vector<FrontierNode> fn;
   std::vector<int32_t> slot;
@@ -309,9 +362,9 @@ void KindField::kdUpdate(uint32_t n, hipStream_t s) {
     dAcc_.ensure(2 * F * sizeof(IntStatsAcc));
     PGL_HIP_CHECK(hipMemcpyAsync(dFrontier_.p, fn.data(), F * sizeof(FrontierNode), hipMemcpyHostToDevice, s));
     PGL_HIP_CHECK(hipMemcpyAsync(dFrontierSlot_.p, slot.data(), slot.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
-    launch_acc_init(dAcc_.as<IntStatsAcc>(), 2 * F, s);
     launch_kd_round(dPx_.as<float>(), dPy_.as<float>(), dPz_.as<float>(), n, dNodeOf_.as<uint32_t>(),
-                    dFrontierSlot_.as<int32_t>(), dFrontier_.as<FrontierNode>(), F, dAcc_.as<IntStatsAcc>(), s);
+                    dFrontierSlot_.as<int32_t>(), dFrontier_.as<FrontierNode>(), F, dStatPartial_.as<IntStatsAcc>(),
+                    dAcc_.as<IntStatsAcc>(), s);
     acc.resize(2 * F);
     PGL_HIP_CHECK(hipMemcpyAsync(acc.data(), dAcc_.p, 2 * F * sizeof(IntStatsAcc), hipMemcpyDeviceToHost, s));
     PGL_HIP_CHECK(hipStreamSynchronize(s));

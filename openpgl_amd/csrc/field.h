@@ -125,7 +125,7 @@ class KindField {
   DevBuf dPx_, dPy_, dPz_, dNodeOf_, dKeys_, dVals_, dKeys2_, dVals2_, dSortTmp_;
   DevBuf dDx_, dDy_, dDz_, dW_, dPdf_, dDist_;
   DevBuf dAos_;
-  DevBuf dNodes_, dFrontier_, dFrontierSlot_, dAcc_, dPartial_;
+  DevBuf dNodes_, dFrontier_, dFrontierSlot_, dAcc_, dPartial_, dStatPartial_;
   DevBuf dStates_, dRegionList_, dPivots_, dNbIds_, dNbPts_, dUniformDirs_, dCopyPairs_, dInputs_;
   uint32_t stateCapacity_ = 0;
 

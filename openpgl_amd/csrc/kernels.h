@@ -25,12 +25,15 @@ struct QueryField {
 // spatial
 void launch_ingest(const void* aos, uint32_t n, float* px, float* py, float* pz, uint32_t* nodeOf, hipStream_t s);
 uint32_t launch_bounds(const float* px, const float* py, const float* pz, uint32_t n, float* partial, hipStream_t s);
-void launch_acc_init(IntStatsAcc* a, uint32_t count, hipStream_t s);
-void launch_root_stats(const float* px, const float* py, const float* pz, uint32_t n, const FrontierNode& root,
-                       IntStatsAcc* out, hipStream_t s);
+// leafParams: per leaf slot {center[3], invHalf[3]} of the leaf's parent bounds.
+// partial: scratch of spatial_partial_bytes(n) bytes for the per-block tables.
+size_t spatial_partial_bytes(uint32_t n);
+void launch_descend_stats(const float* px, const float* py, const float* pz, uint32_t n, const KDNode* nodes,
+                          const int32_t* leafSlot, const void* leafParams, uint32_t numLeaves, uint32_t* nodeOf,
+                          IntStatsAcc* partial, IntStatsAcc* acc, hipStream_t s);
 void launch_kd_round(const float* px, const float* py, const float* pz, uint32_t n, uint32_t* nodeOf,
                      const int32_t* frontierSlot, const FrontierNode* frontier, uint32_t numFrontier,
-                     IntStatsAcc* childAcc, hipStream_t s);
+                     IntStatsAcc* partial, IntStatsAcc* childAcc, hipStream_t s);
 void launch_leaf_keys(const uint32_t* nodeOf, const KDNode* nodes, uint32_t n, uint32_t* keys, uint32_t* vals,
                       hipStream_t s);
 size_t sort_pairs_temp_bytes(uint32_t n);

@@ -409,7 +409,7 @@ __device__ void pass_split(Shared& sh, const RegionSamples& rs, uint32_t base, u
   __syncthreads();
   if (tid == 0) {
     sh.nAssigned = (sh.scal[0][5] + sh.scal[1][5]) + (sh.scal[2][5] + sh.scal[3][5]);
-    sh.S.auxEvals += (float)n * (float)K;
+    sh.S.splitEvals += (float)n * (float)K;
   }
   __syncthreads();
 }
@@ -1135,7 +1135,7 @@ __global__ void __launch_bounds__(RT) k_region_update(RegionState* __restrict__ 
   const uint32_t n = S.sampleCount, base = S.sampleBegin;
   if (tid == 0) {
     S.emIterations = 0; S.partialEmIterations = 0; S.numSplits = 0; S.numMerges = 0;
-    S.estepEvals = 0.f; S.auxEvals = 0.f;
+    S.estepEvals = 0.f; S.auxEvals = 0.f; S.splitEvals = 0.f;
     sh.W = cfg.vectorWidth;
   }
   __syncthreads();
@@ -1229,7 +1229,7 @@ __global__ void __launch_bounds__(256) k_set_inputs(RegionState* __restrict__ st
   S.mode = v.mode;
   S.splitFlag = v.splitFlag;
   S.emIterations = 0; S.partialEmIterations = 0; S.numSplits = 0; S.numMerges = 0;
-  S.estepEvals = 0.f; S.auxEvals = 0.f;
+  S.estepEvals = 0.f; S.auxEvals = 0.f; S.splitEvals = 0.f;
 }
 
 void launch_copy_states(RegionState* states, const uint32_t* pairs, uint32_t numPairs, hipStream_t s) {

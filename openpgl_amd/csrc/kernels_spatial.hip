@@ -11,6 +11,8 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
+
 #include "common.h"
 #include "kernels.h"
 
@@ -33,7 +35,6 @@ __global__ void __launch_bounds__(256) k_ingest(const SampleAoS* __restrict__ in
     px[i] = s.px;
     py[i] = s.py;
     pz[i] = s.pz;
-    nodeOf[i] = 0u;
   }
 }
 
@@ -112,66 +113,172 @@ __device__ __forceinline__ void acc_init(IntStatsAcc* a) {
   }
 }
 
-__global__ void k_acc_init(IntStatsAcc* a, uint32_t count) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < count) acc_init(a + i);
+// Accumulation into a block-private LDS table.  When every active lane of
+// the wave targets the same slot (samples arrive grouped by region after the
+// first frames) the wave reduces with xor-shuffles and one lane issues the
+// LDS atomics; otherwise each lane issues its own LDS atomics (random slots
+// spread over the banks).  Every lane of the wave must call this; inactive
+// lanes pass slot < 0.
+__device__ void wave_accumulate(IntStatsAcc* __restrict__ base, int slot, const long long m[3], const long long v[3]) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long active = __ballot(slot >= 0);
+  if (!active) return;
+  const int leader = __ffsll((long long)active) - 1;
+  const int t = __shfl(slot, leader);
+  if (__ballot(slot >= 0 && slot != t) != 0) {
+    if (slot >= 0) {
+      IntStatsAcc* a = base + slot;
+      atomicAdd(&a->n, 1ull);
+      for (int j = 0; j < 3; j++) {
+        atomicAdd((unsigned long long*)&a->sum[j], (unsigned long long)m[j]);
+        atomicAdd((unsigned long long*)&a->sq[j], (unsigned long long)v[j]);
+        atomicMin(&a->lo[j], m[j]);
+        atomicMax(&a->hi[j], m[j]);
+      }
+    }
+    return;
+  }
+  const bool mine = slot >= 0;
+  long long sm[3], sq[3], lo[3], hi[3];
+  for (int j = 0; j < 3; j++) {
+    sm[j] = mine ? m[j] : 0ll;
+    sq[j] = mine ? v[j] : 0ll;
+    lo[j] = mine ? m[j] : 0x7fffffffffffffffll;
+    hi[j] = mine ? m[j] : (long long)0x8000000000000000ull;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    for (int j = 0; j < 3; j++) {
+      sm[j] += __shfl_xor(sm[j], off);
+      sq[j] += __shfl_xor(sq[j], off);
+      const long long ol = __shfl_xor(lo[j], off), oh = __shfl_xor(hi[j], off);
+      lo[j] = ol < lo[j] ? ol : lo[j];
+      hi[j] = oh > hi[j] ? oh : hi[j];
+    }
+  }
+  if (lane == leader) {
+    IntStatsAcc* a = base + t;
+    atomicAdd(&a->n, (unsigned long long)__popcll(active));
+    for (int j = 0; j < 3; j++) {
+      atomicAdd((unsigned long long*)&a->sum[j], (unsigned long long)sm[j]);
+      atomicAdd((unsigned long long*)&a->sq[j], (unsigned long long)sq[j]);
+      atomicMin(&a->lo[j], lo[j]);
+      atomicMax(&a->hi[j], hi[j]);
+    }
+  }
 }
 
-// Root statistics (root is a leaf): all samples w.r.t. the tree bounds.
-// Block-private LDS accumulator, one global merge per block.
-__global__ void __launch_bounds__(256) k_root_stats(const float* __restrict__ px, const float* __restrict__ py,
-                                                    const float* __restrict__ pz, uint32_t n, FrontierNode root,
-                                                    IntStatsAcc* __restrict__ out) {
-  __shared__ IntStatsAcc acc;
-  if (threadIdx.x == 0) acc_init(&acc);
+// Leaf parameters for the push-down to existing leaves: integer statistics of
+// a leaf are taken w.r.t. its parent's bounds (the root leaf: the tree bounds).
+struct LeafParams {
+  float center[3];
+  float invHalf[3];
+};
+
+static const int kSlotTile = 448;  // LDS accumulator slots per pass (56 KiB)
+
+// Statistics pass over all samples with a block-private accumulator table for
+// slots [slotBase, slotBase + kSlotTile); each block writes its table to
+// partial[block][slot] and k_reduce_partials combines the blocks (no global
+// atomics: device-scope atomics serialise across the XCDs).
+//   kDescend: every sample walks the existing inner nodes to its leaf
+//             (KDTreeBuilder.h:292-349 for the levels that already exist);
+//             statistics of the leaf w.r.t. its parent's bounds.
+//   else:     one level of the push-down below the frontier nodes: left iff
+//             p[dim] < pivot; child statistics w.r.t. the frontier node's bounds.
+template <bool kDescend>
+__global__ void __launch_bounds__(256) k_spatial_stats(const float* __restrict__ px, const float* __restrict__ py,
+                                                       const float* __restrict__ pz, uint32_t n,
+                                                       const KDNode* __restrict__ nodes,
+                                                       const int32_t* __restrict__ slotOfNode,
+                                                       const LeafParams* __restrict__ leafParams,
+                                                       const FrontierNode* __restrict__ frontier,
+                                                       uint32_t* __restrict__ nodeOf, int writeNodes, int slotBase,
+                                                       int numSlots, IntStatsAcc* __restrict__ partial) {
+  __shared__ IntStatsAcc acc[kSlotTile];
+  for (int c = threadIdx.x; c < numSlots; c += blockDim.x) acc_init(&acc[c]);
   __syncthreads();
   const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const float p[3] = {px[i], py[i], pz[i]};
-    long long m[3], v[3];
-    for (int j = 0; j < 3; j++) int_terms(p[j], root.center[j], root.invHalf[j], m[j], v[j]);
-    acc_add(&acc, m, v);
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+    const uint32_t i = base + threadIdx.x;
+    int local = -1;
+    long long m[3] = {0, 0, 0}, v[3] = {0, 0, 0};
+    if (i < n) {
+      const float p[3] = {px[i], py[i], pz[i]};
+      int slot = -1;
+      float c[3], inv[3];
+      if (kDescend) {
+        uint32_t nd = 0;
+        KDNode k = nodes[0];
+        while (!k.isLeaf()) {
+          const uint32_t d = k.dim();
+          const float pd = d == 0 ? p[0] : (d == 1 ? p[1] : p[2]);
+          nd = k.idx() + ((pd < k.pivot) ? 0u : 1u);
+          k = nodes[nd];
+        }
+        if (writeNodes) nodeOf[i] = nd;
+        slot = slotOfNode[nd];
+        const LeafParams lp = leafParams[slot];
+        for (int j = 0; j < 3; j++) { c[j] = lp.center[j]; inv[j] = lp.invHalf[j]; }
+      } else {
+        const uint32_t node = nodeOf[i];
+        const int32_t f = slotOfNode[node];
+        if (f >= 0) {
+          const FrontierNode fn = frontier[f];
+          const float pd = fn.dim == 0 ? p[0] : (fn.dim == 1 ? p[1] : p[2]);
+          const uint32_t side = (pd < fn.pivot) ? 0u : 1u;
+          if (writeNodes) nodeOf[i] = fn.left + side;
+          slot = 2 * f + (int)side;
+          for (int j = 0; j < 3; j++) { c[j] = fn.center[j]; inv[j] = fn.invHalf[j]; }
+        }
+      }
+      if (slot >= slotBase && slot < slotBase + numSlots) {
+        local = slot - slotBase;
+        for (int j = 0; j < 3; j++) int_terms(p[j], c[j], inv[j], m[j], v[j]);
+      }
+    }
+    wave_accumulate(acc, local, m, v);
   }
   __syncthreads();
-  if (threadIdx.x == 0) acc_merge(out, acc);
+  IntStatsAcc* out = partial + (size_t)blockIdx.x * numSlots;
+  for (int c = threadIdx.x; c < numSlots; c += blockDim.x) out[c] = acc[c];
 }
 
-// One level of the kd push-down.  Samples sitting at a frontier node move to
-// the left (p[dim] < pivot) or right child; the child's integer statistics are
-// accumulated w.r.t. the frontier node's bounds.  For small frontiers the
-// accumulators are privatised in LDS per block.
-template <bool kLds>
-__global__ void __launch_bounds__(256) k_kd_round(const float* __restrict__ px, const float* __restrict__ py,
-                                                  const float* __restrict__ pz, uint32_t n,
-                                                  uint32_t* __restrict__ nodeOf,
-                                                  const int32_t* __restrict__ frontierSlot,
-                                                  const FrontierNode* __restrict__ frontier, uint32_t numFrontier,
-                                                  IntStatsAcc* __restrict__ childAcc) {
-  extern __shared__ IntStatsAcc lacc[];
-  const uint32_t numChildren = 2 * numFrontier;
-  if (kLds) {
-    for (uint32_t c = threadIdx.x; c < numChildren; c += blockDim.x) acc_init(&lacc[c]);
+// out[slot] = combination of the per-block tables: one workgroup per slot,
+// threads stride over the blocks, then a tree reduction in LDS (integer sums
+// and min/max: exact and independent of the order).
+__global__ void __launch_bounds__(256) k_reduce_partials(const IntStatsAcc* __restrict__ partial, uint32_t blocks,
+                                                         int numSlots, IntStatsAcc* __restrict__ out) {
+  __shared__ IntStatsAcc red[256];
+  const int c = blockIdx.x;
+  IntStatsAcc r;
+  acc_init(&r);
+  for (uint32_t b = threadIdx.x; b < blocks; b += blockDim.x) {
+    const IntStatsAcc& p = partial[(size_t)b * numSlots + c];
+    r.n += p.n;
+    for (int j = 0; j < 3; j++) {
+      r.sum[j] += p.sum[j];
+      r.sq[j] += p.sq[j];
+      r.lo[j] = p.lo[j] < r.lo[j] ? p.lo[j] : r.lo[j];
+      r.hi[j] = p.hi[j] > r.hi[j] ? p.hi[j] : r.hi[j];
+    }
+  }
+  red[threadIdx.x] = r;
+  __syncthreads();
+  for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      IntStatsAcc& a = red[threadIdx.x];
+      const IntStatsAcc& b = red[threadIdx.x + h];
+      a.n += b.n;
+      for (int j = 0; j < 3; j++) {
+        a.sum[j] += b.sum[j];
+        a.sq[j] += b.sq[j];
+        a.lo[j] = b.lo[j] < a.lo[j] ? b.lo[j] : a.lo[j];
+        a.hi[j] = b.hi[j] > a.hi[j] ? b.hi[j] : a.hi[j];
+      }
+    }
     __syncthreads();
   }
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const uint32_t node = nodeOf[i];
-    const int32_t slot = frontierSlot[node];
-    if (slot < 0) continue;
-    const FrontierNode fn = frontier[slot];
-    const float p[3] = {px[i], py[i], pz[i]};
-    const float pd = fn.dim == 0 ? p[0] : (fn.dim == 1 ? p[1] : p[2]);
-    const uint32_t side = (pd < fn.pivot) ? 0u : 1u;
-    nodeOf[i] = fn.left + side;
-    long long m[3], v[3];
-    for (int j = 0; j < 3; j++) int_terms(p[j], fn.center[j], fn.invHalf[j], m[j], v[j]);
-    if (kLds) acc_add(&lacc[2 * slot + side], m, v);
-    else acc_add(&childAcc[2 * slot + side], m, v);
-  }
-  if (kLds) {
-    __syncthreads();
-    for (uint32_t c = threadIdx.x; c < numChildren; c += blockDim.x) acc_merge(&childAcc[c], lacc[c]);
-  }
+  if (threadIdx.x == 0) out[c] = red[0];
 }
 
 // sample -> region key (leaf data index) and identity index for the sort
@@ -282,28 +389,42 @@ uint32_t launch_bounds(const float* px, const float* py, const float* pz, uint32
   return g;
 }
 
-void launch_acc_init(IntStatsAcc* a, uint32_t count, hipStream_t s) {
-  if (count == 0) return;
-  hipLaunchKernelGGL(k_acc_init, dim3((count + 255) / 256), dim3(256), 0, s, a, count);
+static uint32_t stats_blocks(uint32_t n) { return grid_for(n, 256 * 4, 512); }
+
+size_t spatial_partial_bytes(uint32_t n) { return (size_t)stats_blocks(n) * kSlotTile * sizeof(IntStatsAcc); }
+
+static void run_stats_passes(bool descend, const float* px, const float* py, const float* pz, uint32_t n,
+                             const KDNode* nodes, const int32_t* slotOfNode, const void* leafParams,
+                             const FrontierNode* frontier, uint32_t* nodeOf, uint32_t numSlots, IntStatsAcc* partial,
+                             IntStatsAcc* out, hipStream_t s) {
+  const uint32_t blocks = stats_blocks(n);
+  for (uint32_t base = 0; base < numSlots || base == 0; base += kSlotTile) {
+    const int tile = (int)std::min<uint32_t>(kSlotTile, numSlots - base);
+    // the round kernel reads nodeOf to find its frontier node: write it in the last pass only
+    const int writeNodes = base + kSlotTile >= numSlots ? 1 : 0;
+    if (descend)
+      hipLaunchKernelGGL(k_spatial_stats<true>, dim3(blocks), dim3(256), 0, s, px, py, pz, n, nodes, slotOfNode,
+                         (const LeafParams*)leafParams, frontier, nodeOf, writeNodes, (int)base, tile, partial);
+    else
+      hipLaunchKernelGGL(k_spatial_stats<false>, dim3(blocks), dim3(256), 0, s, px, py, pz, n, nodes, slotOfNode,
+                         (const LeafParams*)leafParams, frontier, nodeOf, writeNodes, (int)base, tile, partial);
+    if (tile > 0)
+      hipLaunchKernelGGL(k_reduce_partials, dim3(tile), dim3(256), 0, s, partial, blocks, tile, out + base);
+    if (numSlots == 0) break;
+  }
 }
 
-void launch_root_stats(const float* px, const float* py, const float* pz, uint32_t n, const FrontierNode& root,
-                       IntStatsAcc* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_root_stats, dim3(grid_for(n, 256, 2048)), dim3(256), 0, s, px, py, pz, n, root, out);
+void launch_descend_stats(const float* px, const float* py, const float* pz, uint32_t n, const KDNode* nodes,
+                          const int32_t* leafSlot, const void* params, uint32_t numLeaves, uint32_t* nodeOf,
+                          IntStatsAcc* partial, IntStatsAcc* acc, hipStream_t s) {
+  run_stats_passes(true, px, py, pz, n, nodes, leafSlot, params, nullptr, nodeOf, numLeaves, partial, acc, s);
 }
 
 void launch_kd_round(const float* px, const float* py, const float* pz, uint32_t n, uint32_t* nodeOf,
                      const int32_t* frontierSlot, const FrontierNode* frontier, uint32_t numFrontier,
-                     IntStatsAcc* childAcc, hipStream_t s) {
-  const uint32_t numChildren = 2 * numFrontier;
-  const size_t lds = numChildren * sizeof(IntStatsAcc);
-  if (lds <= 48 * 1024) {
-    hipLaunchKernelGGL(k_kd_round<true>, dim3(grid_for(n, 256, 2048)), dim3(256), lds, s, px, py, pz, n, nodeOf,
-                       frontierSlot, frontier, numFrontier, childAcc);
-  } else {
-    hipLaunchKernelGGL(k_kd_round<false>, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, px, py, pz, n, nodeOf,
-                       frontierSlot, frontier, numFrontier, childAcc);
-  }
+                     IntStatsAcc* partial, IntStatsAcc* childAcc, hipStream_t s) {
+  run_stats_passes(false, px, py, pz, n, nullptr, frontierSlot, nullptr, frontier, nodeOf, 2 * numFrontier, partial,
+                   childAcc, s);
 }
 
 void launch_leaf_keys(const uint32_t* nodeOf, const KDNode* nodes, uint32_t n, uint32_t* keys, uint32_t* vals,
