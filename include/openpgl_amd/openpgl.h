@@ -261,16 +261,39 @@ void pglFieldExportNodes(PGLField field, int kind, float* pivots, uint32_t* dims
 void pglFieldExportRegion(PGLField field, int kind, uint32_t canonicalIdx, PGLRegionRecord* out);
 void pglFieldSampleLeafIds(PGLField field, int kind, const float* positions, size_t n, uint32_t* canonicalIds);
 void pglFieldGetKNN(PGLField field, int kind, uint32_t* neighbours /* 8 per region, canonical ids */);
-/* EXTENSION: timings of the last update of one kind, milliseconds:
+/* EXTENSION: figures of the last update of one kind (times in milliseconds):
  * [0] total wall, [1] spatial (GPU), [2] directional region kernel (GPU),
  * [3] KNN (GPU), [4] gather+reproject (GPU), [5] sort (GPU), [6] kd rounds,
  * [7] region kernel launches, [8] E-step sample x component evaluations,
  * [9] distance-pass evaluations, [10] EM iterations (sum), [11] samples,
- * [12] split-statistics evaluations, [13] regions */
-void pglFieldGetLastUpdateInfo(PGLField field, int kind, double* out14);
-/* EXTENSION: directional scheduling: 0 = one launch for all regions
- * (default), 1 = one launch per region (the plain per-region path). */
-void pglFieldSetRegionSchedule(PGLField field, int schedule);
+ * [12] split-statistics evaluations, [13] regions, [14] region jobs,
+ * [15] queue items (sum of team sizes), [16] largest team, [17] grid.
+ * Writes min(count, 18) values. */
+void pglFieldGetLastUpdateInfo(PGLField field, int kind, double* out, size_t count);
+
+/* EXTENSION: scheduling of the directional update.  None of the fields
+ * changes the result (bit-identical for every schedule).
+ *   perRegionLaunches  1: one launch of one workgroup per region (the plain
+ *                      per-region GPU path); 0: one persistent launch
+ *   grid               workgroups of the persistent launch (0: all co-resident)
+ *   balance            a team gets one workgroup per
+ *                      total samples / (grid * balance) of its region
+ *   minChunksPerRank   at least this many 1024-sample chunks per workgroup
+ *   maxTeam            largest team */
+typedef struct {
+  int perRegionLaunches;
+  uint32_t grid;
+  float balance;
+  uint32_t minChunksPerRank;
+  uint32_t maxTeam;
+} PGLRegionSchedule;
+void pglFieldSetRegionSchedule(PGLField field, const PGLRegionSchedule* schedule);
+/* EXTENSION: per-region profile of the last update, 8 floats per region in
+ * storage order: microseconds of [0] the whole region, [1] E-step passes,
+ * [2] split-statistics passes, [3] distance passes, [4] team barrier waits,
+ * [5] merging, [6] splitting logic; [7] team size. */
+void pglFieldGetRegionProfile(PGLField field, int kind, float* out);
+void pglFieldGetRegionSchedule(PGLField field, PGLRegionSchedule* schedule);
 
 #ifdef __cplusplus
 }  /* extern "C" */

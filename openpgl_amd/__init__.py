@@ -147,9 +147,19 @@ _SIGS = {
     "pglFieldExportRegion": (None, [_vp, ctypes.c_int, _u32, ctypes.POINTER(PGLRegionRecord)]),
     "pglFieldSampleLeafIds": (None, [_vp, ctypes.c_int, _vp, ctypes.c_size_t, _vp]),
     "pglFieldGetKNN": (None, [_vp, ctypes.c_int, _vp]),
-    "pglFieldGetLastUpdateInfo": (None, [_vp, ctypes.c_int, _vp]),
-    "pglFieldSetRegionSchedule": (None, [_vp, ctypes.c_int]),
+    "pglFieldGetLastUpdateInfo": (None, [_vp, ctypes.c_int, _vp, ctypes.c_size_t]),
+    "pglFieldSetRegionSchedule": (None, [_vp, _vp]),
+    "pglFieldGetRegionSchedule": (None, [_vp, _vp]),
+    "pglFieldGetRegionProfile": (None, [_vp, ctypes.c_int, _vp]),
 }
+
+
+
+class RegionSchedule(ctypes.Structure):
+    """PGLRegionSchedule (include/openpgl_amd/openpgl.h)."""
+    _fields_ = [("perRegionLaunches", ctypes.c_int), ("grid", ctypes.c_uint32), ("balance", ctypes.c_float),
+                ("minChunksPerRank", ctypes.c_uint32), ("maxTeam", ctypes.c_uint32)]
+
 
 _lib = None
 
@@ -266,8 +276,35 @@ class Field:
     def reset(self):
         self.lib.pglFieldReset(self.h)
 
-    def set_region_schedule(self, schedule):
-        self.lib.pglFieldSetRegionSchedule(self.h, schedule)
+    def set_region_schedule(self, per_region_launches=None, grid=None, balance=None, min_chunks_per_rank=None,
+                            max_team=None):
+        """Directional scheduling (PGLRegionSchedule); None keeps the current value."""
+        sc = RegionSchedule()
+        self.lib.pglFieldGetRegionSchedule(self.h, ctypes.byref(sc))
+        if per_region_launches is not None:
+            sc.perRegionLaunches = int(per_region_launches)
+        if grid is not None:
+            sc.grid = int(grid)
+        if balance is not None:
+            sc.balance = float(balance)
+        if min_chunks_per_rank is not None:
+            sc.minChunksPerRank = int(min_chunks_per_rank)
+        if max_team is not None:
+            sc.maxTeam = int(max_team)
+        self.lib.pglFieldSetRegionSchedule(self.h, ctypes.byref(sc))
+
+    def region_profile(self, kind=0):
+        """(regions, 8): us of total, E-step, split, distance passes, barrier waits, merging,
+        splitting logic; team size.  Storage order."""
+        out = np.zeros((self.num_regions(kind), 8), np.float32)
+        self.lib.pglFieldGetRegionProfile(self.h, kind, _ptr(out))
+        return out
+
+    def region_schedule(self):
+        sc = RegionSchedule()
+        self.lib.pglFieldGetRegionSchedule(self.h, ctypes.byref(sc))
+        return {"per_region_launches": sc.perRegionLaunches, "grid": sc.grid, "balance": sc.balance,
+                "min_chunks_per_rank": sc.minChunksPerRank, "max_team": sc.maxTeam}
 
     # --- introspection
     def num_regions(self, kind=0):
@@ -298,10 +335,11 @@ class Field:
         return out.reshape(-1, 8)
 
     def last_update_info(self, kind=0):
-        o = np.zeros(14, np.float64)
-        self.lib.pglFieldGetLastUpdateInfo(self.h, kind, _ptr(o))
+        o = np.zeros(18, np.float64)
+        self.lib.pglFieldGetLastUpdateInfo(self.h, kind, _ptr(o), 18)
         keys = ["wall_ms", "spatial_ms", "directional_ms", "knn_ms", "gather_ms", "sort_ms", "kd_rounds",
-                "region_launches", "estep_evals", "dist_evals", "em_iterations", "samples", "split_evals", "regions"]
+                "region_launches", "estep_evals", "dist_evals", "em_iterations", "samples", "split_evals", "regions",
+                "region_jobs", "queue_items", "max_team", "grid"]
         return dict(zip(keys, o.tolist()))
 
     # --- queries

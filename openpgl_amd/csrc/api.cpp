@@ -722,9 +722,10 @@ void pglFieldGetKNN(PGLField field, int kind, uint32_t* nb) {
     }
 }
 
-void pglFieldGetLastUpdateInfo(PGLField field, int kind, double* o) {
+void pglFieldGetLastUpdateInfo(PGLField field, int kind, double* out, size_t count) {
   KindField& k = kind_of(field, kind);
   const UpdateInfo& i = k.lastInfo();
+  double o[18];
   o[0] = i.wallMs; o[1] = i.spatialMs; o[2] = i.directionalMs; o[3] = i.knnMs; o[4] = i.gatherMs; o[5] = i.sortMs;
   o[6] = i.kdRounds; o[7] = i.regionLaunches;
   k.syncMirror(field->gf->stream);
@@ -737,11 +738,50 @@ void pglFieldGetLastUpdateInfo(PGLField field, int kind, double* o) {
     it += S.emIterations + S.partialEmIterations;
   }
   o[8] = ee; o[9] = de; o[10] = it; o[11] = i.samples; o[12] = se; o[13] = k.numRegions();
+  o[14] = i.regionJobs; o[15] = i.regionItems; o[16] = i.maxTeam; o[17] = i.regionGrid;
+  for (size_t j = 0; j < count && j < 18; j++) out[j] = o[j];
 }
 
-void pglFieldSetRegionSchedule(PGLField field, int schedule) {
-  field->gf->surface.setSchedule(schedule);
-  field->gf->volume.setSchedule(schedule);
+void pglFieldGetRegionProfile(PGLField field, int kind, float* out) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(field, "field");
+  KindField& k = kind_of(field, kind);
+  k.syncMirror(field->gf->stream);
+  int dev = 0, rateKHz = 0;
+  PGL_HIP_CHECK(hipGetDevice(&dev));
+  PGL_HIP_CHECK(hipDeviceGetAttribute(&rateKHz, hipDeviceAttributeWallClockRate, dev));
+  const float toUs = rateKHz > 0 ? 1000.f / (float)rateKHz : 0.01f;
+  for (uint32_t r = 0; r < k.numRegions(); r++) {
+    const RegionState& S = k.mirrorRegion(r);
+    for (int i = 0; i < 7; i++) out[r * 8 + i] = S.prof[i] * toUs;
+    out[r * 8 + 7] = S.prof[7];
+  }
 }
+PGL_CATCH_END_VOID
+
+void pglFieldSetRegionSchedule(PGLField field, const PGLRegionSchedule* sc) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(field, "field");
+  THROW_IF_NULL(sc, "schedule");
+  for (KindField* k : {&field->gf->surface, &field->gf->volume}) {
+    RegionSchedule& r = k->schedule();
+    r.perRegionLaunches = sc->perRegionLaunches;
+    r.grid = sc->grid;
+    r.balance = sc->balance;
+    r.minChunksPerRank = sc->minChunksPerRank;
+    r.maxTeam = sc->maxTeam;
+  }
+}
+PGL_CATCH_END_VOID
+
+void pglFieldGetRegionSchedule(PGLField field, PGLRegionSchedule* sc) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(field, "field");
+  THROW_IF_NULL(sc, "schedule");
+  const RegionSchedule& r = field->gf->surface.schedule();
+  sc->perRegionLaunches = r.perRegionLaunches;
+  sc->grid = r.grid;
+  sc->balance = r.balance;
+  sc->minChunksPerRank = r.minChunksPerRank;
+  sc->maxTeam = r.maxTeam;
+}
+PGL_CATCH_END_VOID
 
 }  // extern "C"

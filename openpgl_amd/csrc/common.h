@@ -15,6 +15,10 @@
 namespace pglgpu {
 
 static const int kMaxComponents = 32;   // PGL_VMM_MAX_COMPONENTS (include/openpgl/common.h:26)
+// Directional passes: samples of a region are reduced in fixed chunks of
+// kChunk samples; one chunk partial holds kPartStride floats.
+static const uint32_t kChunk = 1024;
+static const int kPartStride = 256;
 static const int kMergeWidth = 8;       // merge loop floor = SIMD width of the CPU_8 device
 static const float kOneOverFourPi = 0.07957747154594767f;
 static const float kMaxKappaSplit = 32000.0f;  // OPENPGL_MAX_KAPPA used by the splitter
@@ -95,7 +99,11 @@ struct RegionState {
   float estepEvals;     // sample x component evaluations in E-steps of the last update
   float auxEvals;       // sample x component evaluations in distance passes
   float splitEvals;     // sample x component evaluations in split-statistics passes
-  uint32_t pad[317];
+  // wall-clock ticks of the last update (rank 0 of the region's team):
+  // [0] total, [1] E-step passes, [2] split passes, [3] distance passes,
+  // [4] team barrier waits, [5] merging, [6] splitting logic, [7] team size
+  float prof[8];
+  uint32_t pad[309];
 };
 static_assert(sizeof(RegionState) == 4096, "RegionState must be 4 KiB");
 
@@ -116,6 +124,14 @@ struct RegionInput {
   float ssMean[3], ssVar[3], ssN;
   uint32_t begin, count;
   int32_t mode, splitFlag;
+};
+
+// One region of a directional launch: team = workgroups working on it,
+// chunks = ceil(samples / kChunk), partialOffset = float offset of its
+// 2 x chunks x kPartStride partial buffer.
+struct RegionJob {
+  uint32_t region, team, chunks, pad;
+  unsigned long long partialOffset;
 };
 
 // Node being pushed down one level in a kd round.

@@ -552,22 +552,16 @@ void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
   std::stable_sort(work.begin(), work.end(), [](const std::pair<uint32_t, uint32_t>& a, const std::pair<uint32_t, uint32_t>& b) {
     return a.first > b.first;
   });
-  std::vector<uint32_t> list(work.size());
-  for (size_t i = 0; i < work.size(); i++) list[i] = work[i].second;
-  dRegionList_.ensure(std::max<size_t>(list.size(), 1) * 4);
-  PGL_HIP_CHECK(hipMemcpyAsync(dRegionList_.p, list.data(), list.size() * 4, hipMemcpyHostToDevice, s));
-  if (schedule_ == 1) {
-    for (size_t i = 0; i < list.size(); i++)
-      launch_region_update(dStates_.as<RegionState>(), dRegionList_.as<uint32_t>() + i, 1, dDx_.as<float>(),
-                           dDy_.as<float>(), dDz_.as<float>(), dW_.as<float>(), dPdf_.as<float>(), dDist_.as<float>(),
-                           fc_, dUniformDirs_.as<float>(), s);
-    info_.regionLaunches = (double)list.size();
-  } else {
-    launch_region_update(dStates_.as<RegionState>(), dRegionList_.as<uint32_t>(), (uint32_t)list.size(),
-                         dDx_.as<float>(), dDy_.as<float>(), dDz_.as<float>(), dW_.as<float>(), dPdf_.as<float>(),
-                         dDist_.as<float>(), fc_, dUniformDirs_.as<float>(), s);
-    info_.regionLaunches = list.empty() ? 0 : 1;
-  }
+  RegionRunner& rr = runner();
+  rr.launch(dStates_.as<RegionState>(), work,
+            RegionSamplesView{dDx_.as<float>(), dDy_.as<float>(), dDz_.as<float>(), dW_.as<float>(),
+                              dPdf_.as<float>(), dDist_.as<float>()},
+            fc_, dUniformDirs_.as<float>(), s);
+  info_.regionLaunches = rr.lastLaunches;
+  info_.regionJobs = rr.lastJobs;
+  info_.regionItems = rr.lastItems;
+  info_.maxTeam = rr.lastMaxTeam;
+  info_.regionGrid = rr.lastGrid;
   PGL_HIP_CHECK(hipEventRecord(ev_[4], s));
 
   // KNN neighbourhoods of the region pivots (sample-statistics means)
@@ -583,6 +577,7 @@ void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
   }
   PGL_HIP_CHECK(hipEventRecord(ev_[5], s));
   PGL_HIP_CHECK(hipStreamSynchronize(s));
+  rr.check();
   for (auto& r : regions_) { r.splitFlag = false; r.copyFrom = -1; }
   initialized_ = true;
   iteration_++;
@@ -595,6 +590,11 @@ void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
   PGL_HIP_CHECK(hipEventElapsedTime(&ms, ev_[3], ev_[4])); info_.directionalMs = ms;
   PGL_HIP_CHECK(hipEventElapsedTime(&ms, ev_[4], ev_[5])); info_.knnMs = ms;
   info_.wallMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+RegionRunner& KindField::runner() {
+  if (!runner_) runner_.reset(new RegionRunner());
+  return *runner_;
 }
 
 void KindField::syncMirror(hipStream_t s) const {

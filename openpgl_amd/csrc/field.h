@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -12,6 +13,7 @@
 #include "common.h"
 #include "kernels.h"
 #include "query_math.h"
+#include "region_sched.h"
 
 namespace pglgpu {
 
@@ -58,6 +60,7 @@ struct DevBuf {
 struct UpdateInfo {
   double wallMs = 0, spatialMs = 0, directionalMs = 0, knnMs = 0, gatherMs = 0, sortMs = 0;
   double kdRounds = 0, regionLaunches = 0, estepEvals = 0, auxEvals = 0, emIterations = 0, samples = 0;
+  double regionJobs = 0, regionItems = 0, maxTeam = 0, regionGrid = 0;
 };
 
 // One guiding field (surface or volume): Field<> of field/Field.h.
@@ -86,7 +89,8 @@ class KindField {
   void exportRegion(uint32_t storageIdx, struct PGLRegionRecordHost* out) const;
   void knnHost(std::vector<uint32_t>& ids) const;
   const UpdateInfo& lastInfo() const { return info_; }
-  void setSchedule(int s) { schedule_ = s; }
+  RegionSchedule& schedule() { return runner().schedule; }
+  RegionRunner& runner();
   bool validate() const;
 
   struct RegionHost {
@@ -109,7 +113,6 @@ class KindField {
   FactoryConfig fc_;
   bool deterministic_;
   int vectorWidth_;
-  int schedule_ = 0;
 
   std::vector<KDNode> nodes_;
   std::vector<RegionHost> regions_;
@@ -126,7 +129,8 @@ class KindField {
   DevBuf dDx_, dDy_, dDz_, dW_, dPdf_, dDist_;
   DevBuf dAos_;
   DevBuf dNodes_, dFrontier_, dFrontierSlot_, dAcc_, dPartial_, dStatPartial_;
-  DevBuf dStates_, dRegionList_, dPivots_, dNbIds_, dNbPts_, dUniformDirs_, dCopyPairs_, dInputs_;
+  std::unique_ptr<RegionRunner> runner_;
+  DevBuf dStates_, dPivots_, dNbIds_, dNbPts_, dUniformDirs_, dCopyPairs_, dInputs_;
   uint32_t stateCapacity_ = 0;
 
   mutable std::vector<RegionState> hostStates_;

@@ -4,7 +4,16 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <stdexcept>
+#include <string>
+
 #include "common.h"
+
+#define PGL_KERNEL_CHECK(x)                                                                       \
+  do {                                                                                            \
+    const hipError_t e_ = (x);                                                                    \
+    if (e_ != hipSuccess) throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
 
 namespace pglgpu {
 
@@ -46,9 +55,17 @@ void launch_gather_reproject(const void* aos, const uint32_t* sortedKeys, const 
 void launch_knn8(const float4* pivots, uint32_t numRegions, uint32_t* nbIds, float4* nbPts, hipStream_t s);
 
 // directional
-void launch_region_update(RegionState* states, const uint32_t* regionList, uint32_t numRegions, const float* dx,
-                          const float* dy, const float* dz, const float* w, const float* pdf, const float* dist,
-                          const FactoryConfig& cfg, const float* uniformDirs, hipStream_t s);
+// Directional update of the regions listed in jobs (persistent kernel, see
+// kernels_region.hip).  items: (job, rank) pairs, ranks of a job consecutive;
+// queueAndCounters: numJobs + 1 words (cleared here); partial: per job
+// 2 x chunks x kPartStride floats at job.partialOffset; err: set to 1 when a
+// team barrier waited longer than timeoutTicks of the wall clock (wall_clock64).  Every job's team must be <= grid.
+int region_kernel_blocks_per_cu();
+void launch_region_update(RegionState* states, const RegionJob* jobs, uint32_t numJobs, const uint2* items,
+                          uint32_t numItems, uint32_t grid, uint32_t* queueAndCounters, float* partial, int* err,
+                          unsigned long long timeoutTicks, const float* dx, const float* dy, const float* dz, const float* w, const float* pdf,
+                          const float* dist, const FactoryConfig& cfg, const float* uniformDirs, bool cooperative,
+                          hipStream_t s);
 
 void launch_copy_states(RegionState* states, const uint32_t* pairs /* dst, src */, uint32_t numPairs, hipStream_t s);
 void launch_set_inputs(RegionState* states, const RegionInput* inputs, uint32_t count, hipStream_t s);

@@ -1,21 +1,31 @@
-// Directional training of Field::Update on gfx950: one 256-thread workgroup
-// owns one region and runs the whole adaptive split-and-merge update for it
-// on the device — weighted EM (E-step over the region's sample stream, MAP
-// M-step), chi-square split statistics, recursive/partial splitting with
-// partial re-fit, pairwise chi-square merging and the harmonic-mean parallax
-// distances — with no host round trip.  All regions of an update run in one
-// launch (grid = regions, largest first).
+// Directional training of Field::Update on gfx950: the whole adaptive
+// split-and-merge update of a region — weighted EM (E-step over the region's
+// sample stream, MAP M-step), chi-square split statistics, recursive/partial
+// splitting with partial re-fit, pairwise chi-square merging and the
+// harmonic-mean parallax distances — runs on the device with no host round
+// trip.  All regions of an update run in one persistent launch.
 //
-// Work layout: a quad of 4 lanes evaluates one sample; lane q of the quad owns
-// mixture components k = 4j + q (j < 8), so a wave evaluates 16 samples x 32
-// components per step with the component parameters held in registers.  The
-// per-sample pdf is a 2-step quad reduction; the weighted sufficient
-// statistics are accumulated per lane in FP32 blocks of 8 samples folded into
-// Kahan-compensated totals, then reduced over quads (xor-shuffles) and waves
-// (fixed order in LDS).  Per-component M-step / split / merge logic runs in
-// wave 0 with one lane per component, serial decisions in lane 0.
-// The mean-cosine -> kappa inversion and the log-likelihood convergence test
-// are evaluated in FP64; everything else is FP32.
+// Teams: a region is worked on by a team of T workgroups (T = 1 for small
+// regions).  Its samples are cut into fixed chunks of kChunk samples; every
+// sample pass computes one partial per chunk (team rank r takes chunks r,
+// r+T, ...), the partials are folded in chunk order with Kahan compensation
+// (the log-likelihood in FP64), and every rank of the team then runs the same
+// per-region logic on identical LDS state.  Because the chunking and the fold
+// order do not depend on T, the result is bit-identical for every team size
+// and schedule.  Ranks exchange partials through global memory (double
+// buffered) and meet at a counter barrier with a bounded wait.  Work items
+// (region, rank) are taken from a queue in order, with every rank of a team
+// queued consecutively, by a grid that is co-resident on the device.
+//
+// Work layout inside a chunk: a quad of 4 lanes evaluates one sample; lane q
+// of the quad owns mixture components k = 4j + q (j < 8), so a wave evaluates
+// 16 samples x 32 components per step with the component parameters held in
+// registers.  The per-sample pdf is a 2-step quad reduction; per-lane sums over
+// the chunk are reduced over quads (xor-shuffles) and waves (fixed order).
+// Per-component M-step / split / merge logic runs in wave 0 with one lane per
+// component, serial decisions in lane 0.  The mean-cosine -> kappa inversion
+// and the log-likelihood convergence test are evaluated in FP64; everything
+// else is FP32.
 //
 // Reference behaviour followed (directional/vmm/, Open PGL 0.6.0):
 //   AdaptiveSplitandMergeFactory.h          fit 303-347, update 352-462
@@ -39,6 +49,9 @@ namespace pglgpu {
 
 #define RT 256
 #define RWAVES 4
+#define K_ESTEP_COPY 32
+// value layout of a pass result: field f of component k at f * 32 + k, scalars from kScal
+static const int kScal = 192;
 static const float kTwoPi = 6.283185307179586f;
 
 struct RegionSamples {
@@ -69,6 +82,20 @@ struct Shared {
   int32_t found, ci, cj;
   int32_t order[32];
   int32_t W;  // SIMD width of the emulated reference device
+  float fin[kPartStride];  // folded pass results (layout: see the pass functions)
+  double finLL;            // folded log-likelihood
+  int32_t abort;           // a team barrier timed out: finish without waiting
+  uint32_t item;           // queue item of this workgroup
+};
+
+// Team context of a workgroup (uniform over the workgroup).
+struct Team {
+  uint32_t rank, T, nc;
+  float* part;          // 2 x nc x kPartStride floats of this region
+  uint32_t* counter;    // barrier counter of this region
+  uint32_t epoch;       // barriers passed
+  int* err;
+  unsigned long long timeout;  // longest barrier wait, wall-clock ticks
 };
 
 // ----------------------------------------------------------------- helpers
@@ -182,117 +209,170 @@ __device__ __forceinline__ float quad_group_sum(float v) {  // over the 16 quads
   return v;
 }
 
-__device__ __forceinline__ void kahan_fold(float& sum, float& comp, float& blk) {
-  const float y = blk - comp;
+// Kahan step of the cross-chunk fold
+__device__ __forceinline__ void kfold(float& sum, float& comp, float x) {
+  const float y = x - comp;
   const float t = sum + y;
   comp = (t - sum) - y;
   sum = t;
-  blk = 0.f;
 }
 
-// E-step (WEM:847-892): fills sh.cur (sums for k < K), sh.ll, sh.un.
-__device__ void pass_estep(Shared& sh, const RegionSamples& rs, uint32_t base, uint32_t n) {
+// value v of the current chunk from the per-wave partials (fixed order)
+__device__ __forceinline__ float chunk_value(const Shared& sh, int v, int F, int NS) {
+  if (v < 32 * F) {
+    const int f = v >> 5, k = v & 31;
+    return (sh.red[0][k][f] + sh.red[1][k][f]) + (sh.red[2][k][f] + sh.red[3][k][f]);
+  }
+  if (v >= kScal && v < kScal + NS) {
+    const int i = v - kScal;
+    return (sh.scal[0][i] + sh.scal[1][i]) + (sh.scal[2][i] + sh.scal[3][i]);
+  }
+  return 0.f;
+}
+
+// Team barrier.  Partials are written with device-coherent (agent-scope
+// relaxed atomic, write-through) stores and read with device-coherent loads,
+// so no cache writeback or invalidation is needed: each wave waits for its
+// stores to complete (s_waitcnt), the workgroup arrives on the counter and
+// waits until all T ranks have arrived.  The wait is bounded: after
+// tm.timeout ticks of the constant-rate wall clock, or as soon as any
+// workgroup has raised the error flag, the workgroup raises the flag and stops
+// waiting at this and every later barrier (it still arrives, so no teammate
+// waits for it).  The host turns the flag into an error.
+__device__ void team_barrier(Shared& sh, Team& tm) {
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  tm.epoch++;
+  if (threadIdx.x == 0) {
+    const unsigned long long tb = wall_clock64();
+    atomicAdd(tm.counter, 1u);
+    if (!sh.abort) {
+      const uint32_t target = tm.epoch * tm.T;
+      const unsigned long long t0 = tb;
+      while (__hip_atomic_load(tm.counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        __builtin_amdgcn_s_sleep(2);
+        if (__hip_atomic_load(tm.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 ||
+            wall_clock64() - t0 > tm.timeout) {
+          atomicExch(tm.err, 1);
+          sh.abort = 1;
+          break;
+        }
+      }
+    }
+    sh.S.prof[4] += (float)(wall_clock64() - tb);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  __syncthreads();
+}
+
+// End of a chunk: value v = threadIdx.x of chunk c goes into the running fold
+// (T = 1, chunks arrive in order) or to the team's partial buffer.
+__device__ __forceinline__ void chunk_out(const Shared& sh, const Team& tm, uint32_t c, int F, int NS, int llIdx,
+                                          float& fs, float& fc, double& fd) {
+  const int v = threadIdx.x;
+  const float x = chunk_value(sh, v, F, NS);
+  if (tm.T == 1) {
+    if (v == llIdx) fd += (double)x;
+    else kfold(fs, fc, x);
+  } else {
+    __hip_atomic_store(tm.part + ((size_t)(tm.epoch & 1u) * tm.nc + c) * kPartStride + v, x, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// End of a pass: fold the chunk partials of all ranks in chunk order, publish in sh.fin.
+__device__ void pass_finish(Shared& sh, Team& tm, int llIdx, float fs, float fc, double fd) {
+  const int v = threadIdx.x;
+  if (tm.T > 1) {
+    const uint32_t buf = tm.epoch & 1u;
+    team_barrier(sh, tm);
+    const float* p = tm.part + (size_t)buf * tm.nc * kPartStride + v;
+    for (uint32_t c = 0; c < tm.nc; c++) {
+      const float x = __hip_atomic_load(p + (size_t)c * kPartStride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (v == llIdx) fd += (double)x;
+      else kfold(fs, fc, x);
+    }
+  }
+  sh.fin[v] = fs;
+  if (v == llIdx) sh.finLL = fd;
+  __syncthreads();
+}
+
+// E-step (WEM:847-892).  Result: fin[f*32+k] = sum over samples of
+// gamma_k w (dx, dy, dz, 1) for f = 0..3, finLL = sum w log pdf,
+// fin[kScal+0..3] = unassigned (w, w dx, w dy, w dz).
+__device__ void pass_estep(Shared& sh, Team& tm, const RegionSamples& rs, uint32_t base, uint32_t n) {
+  const unsigned long long tPass = wall_clock64();
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, q = tid & 3, quad = tid >> 2;
   const VMMParams& V = sh.S.vmm;
   const int K = V.K;
   const int J = (K + 3) >> 2;
   CompRegs c;
   load_comps(V, q, c);
-  float bx[8], by[8], bz[8], bw[8], sx[8], sy[8], sz[8], sw[8], cx[8], cy[8], cz[8], cw[8];
+  float fs = 0.f, fc = 0.f;
+  double fd = 0.0;
+  for (uint32_t ch = tm.rank; ch < tm.nc; ch += tm.T) {
+    const uint32_t cb = ch * kChunk, ce = min(n, cb + kChunk);
+    float bx[8], by[8], bz[8], bw[8];
 #pragma unroll
-  for (int j = 0; j < 8; j++) {
-    bx[j] = by[j] = bz[j] = bw[j] = 0.f;
-    sx[j] = sy[j] = sz[j] = sw[j] = 0.f;
-    cx[j] = cy[j] = cz[j] = cw[j] = 0.f;
-  }
-  float llb = 0.f, lls = 0.f, llc = 0.f;
-  float u0 = 0.f, u1 = 0.f, u2 = 0.f, u3 = 0.f;
-  int blk = 0;
-  for (uint32_t s = quad; s < n; s += RT / 4) {
-    const uint32_t i = base + s;
-    const float dx = rs.dx[i], dy = rs.dy[i], dz = rs.dz[i], w = rs.w[i];
-    float e[8];
-    const float pdf = soft_eval(c, J, dx, dy, dz, e);
-    if (pdf <= 1e-16f) {
-      if (q == 0) { u0 += w; u1 += dx * w; u2 += dy * w; u3 += dz * w; }
-    } else {
-      if (q == 0) llb += w * logf(pdf);
-      const float r = w / pdf;
+    for (int j = 0; j < 8; j++) bx[j] = by[j] = bz[j] = bw[j] = 0.f;
+    float ll = 0.f, u0 = 0.f, u1 = 0.f, u2 = 0.f, u3 = 0.f;
+    for (uint32_t s = cb + quad; s < ce; s += RT / 4) {
+      const uint32_t i = base + s;
+      const float dx = rs.dx[i], dy = rs.dy[i], dz = rs.dz[i], w = rs.w[i];
+      float e[8];
+      const float pdf = soft_eval(c, J, dx, dy, dz, e);
+      if (pdf <= 1e-16f) {
+        if (q == 0) { u0 += w; u1 += dx * w; u2 += dy * w; u3 += dz * w; }
+      } else {
+        if (q == 0) ll += w * logf(pdf);
+        const float r = w / pdf;
 #pragma unroll
-      for (int j = 0; j < 8; j++) {
-        const float g = e[j] * r;
-        bx[j] = fmaf(g, dx, bx[j]);
-        by[j] = fmaf(g, dy, by[j]);
-        bz[j] = fmaf(g, dz, bz[j]);
-        bw[j] += g;
+        for (int j = 0; j < 8; j++) {
+          const float g = e[j] * r;
+          bx[j] = fmaf(g, dx, bx[j]);
+          by[j] = fmaf(g, dy, by[j]);
+          bz[j] = fmaf(g, dz, bz[j]);
+          bw[j] += g;
+        }
       }
     }
-    if (++blk == 8) {
 #pragma unroll
-      for (int j = 0; j < 8; j++) {
-        kahan_fold(sx[j], cx[j], bx[j]);
-        kahan_fold(sy[j], cy[j], by[j]);
-        kahan_fold(sz[j], cz[j], bz[j]);
-        kahan_fold(sw[j], cw[j], bw[j]);
+    for (int j = 0; j < 8; j++) {
+      const float vx = quad_group_sum(bx[j]);
+      const float vy = quad_group_sum(by[j]);
+      const float vz = quad_group_sum(bz[j]);
+      const float vw = quad_group_sum(bw[j]);
+      if (lane < 4) {
+        const int k = 4 * j + lane;
+        sh.red[wave][k][0] = vx;
+        sh.red[wave][k][1] = vy;
+        sh.red[wave][k][2] = vz;
+        sh.red[wave][k][3] = vw;
       }
-      kahan_fold(lls, llc, llb);
-      blk = 0;
     }
-  }
-#pragma unroll
-  for (int j = 0; j < 8; j++) {
-    kahan_fold(sx[j], cx[j], bx[j]);
-    kahan_fold(sy[j], cy[j], by[j]);
-    kahan_fold(sz[j], cz[j], bz[j]);
-    kahan_fold(sw[j], cw[j], bw[j]);
-  }
-  kahan_fold(lls, llc, llb);
-#pragma unroll
-  for (int j = 0; j < 8; j++) {
-    const float vx = quad_group_sum(sx[j] - cx[j]);
-    const float vy = quad_group_sum(sy[j] - cy[j]);
-    const float vz = quad_group_sum(sz[j] - cz[j]);
-    const float vw = quad_group_sum(sw[j] - cw[j]);
-    if (lane < 4) {
-      const int k = 4 * j + lane;
-      sh.red[wave][k][0] = vx;
-      sh.red[wave][k][1] = vy;
-      sh.red[wave][k][2] = vz;
-      sh.red[wave][k][3] = vw;
+    // lanes q != 0 hold zeros for the per-sample scalars
+    const float vll = wave_sum(ll);
+    const float v0 = wave_sum(u0), v1 = wave_sum(u1), v2 = wave_sum(u2), v3 = wave_sum(u3);
+    if (lane == 0) {
+      sh.scal[wave][0] = vll;
+      sh.scal[wave][1] = v0;
+      sh.scal[wave][2] = v1;
+      sh.scal[wave][3] = v2;
+      sh.scal[wave][4] = v3;
     }
+    __syncthreads();
+    chunk_out(sh, tm, ch, 4, 5, kScal, fs, fc, fd);
+    __syncthreads();
   }
-  // lanes q != 0 hold zeros for the per-sample scalars
-  const float vll = wave_sum(lls - llc);
-  const float v0 = wave_sum(u0), v1 = wave_sum(u1), v2 = wave_sum(u2), v3 = wave_sum(u3);
-  if (lane == 0) {
-    sh.scal[wave][0] = vll;
-    sh.scal[wave][1] = v0;
-    sh.scal[wave][2] = v1;
-    sh.scal[wave][3] = v2;
-    sh.scal[wave][4] = v3;
-  }
-  __syncthreads();
-  if (tid < 128) {
-    const int k = tid >> 2, cc = tid & 3;
-    if (k < K) {
-      const float v = (sh.red[0][k][cc] + sh.red[1][k][cc]) + (sh.red[2][k][cc] + sh.red[3][k][cc]);
-      if (cc == 0) sh.cur.swdx[k] = v;
-      else if (cc == 1) sh.cur.swdy[k] = v;
-      else if (cc == 2) sh.cur.swdz[k] = v;
-      else sh.cur.sw[k] = v;
-    }
-  }
-  if (tid == 0) {
-    sh.ll = ((double)sh.scal[0][0] + (double)sh.scal[1][0]) + ((double)sh.scal[2][0] + (double)sh.scal[3][0]);
-    for (int c2 = 0; c2 < 4; c2++)
-      sh.un[c2] = (sh.scal[0][1 + c2] + sh.scal[1][1 + c2]) + (sh.scal[2][1 + c2] + sh.scal[3][1 + c2]);
-    sh.S.estepEvals += (float)n * (float)K;
-  }
-  __syncthreads();
+  pass_finish(sh, tm, kScal, fs, fc, fd);
+  if (tid == 0) { sh.S.estepEvals += (float)n * (float)K; sh.S.prof[1] += (float)(wall_clock64() - tPass); }
 }
 
-// Distances pass (WEM:1199-1326): per component sum of w g_k (g_k pdf / w_k) / dist and of the weights.
-__device__ void pass_distances(Shared& sh, const RegionSamples& rs, uint32_t base, uint32_t n) {
+// Distances pass (WEM:1199-1326).  Result: fin[k] = sum of w g_k (g_k pdf / w_k) / dist,
+// fin[32+k] = sum of w g_k (g_k pdf / w_k).
+__device__ void pass_distances(Shared& sh, Team& tm, const RegionSamples& rs, uint32_t base, uint32_t n) {
+  const unsigned long long tPass = wall_clock64();
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, q = tid & 3, quad = tid >> 2;
   const VMMParams& V = sh.S.vmm;
   const int K = V.K;
@@ -305,122 +385,135 @@ __device__ void pass_distances(Shared& sh, const RegionSamples& rs, uint32_t bas
     const int k = 4 * j + q;
     invw[j] = k < K ? 1.f / V.w[k] : 0.f;
   }
-  float ad[8], aw[8];
+  float fs = 0.f, fc = 0.f;
+  double fd = 0.0;
+  for (uint32_t ch = tm.rank; ch < tm.nc; ch += tm.T) {
+    const uint32_t cb = ch * kChunk, ce = min(n, cb + kChunk);
+    float ad[8], aw[8];
 #pragma unroll
-  for (int j = 0; j < 8; j++) ad[j] = aw[j] = 0.f;
-  for (uint32_t s = quad; s < n; s += RT / 4) {
-    const uint32_t i = base + s;
-    const float dx = rs.dx[i], dy = rs.dy[i], dz = rs.dz[i], w = rs.w[i];
-    const float invd = 1.f / rs.dist[i];
-    float e[8];
-    const float pdf = soft_eval(c, J, dx, dy, dz, e);
-    if (pdf > 1e-16f) {
-      const float ip = 1.f / pdf;
+    for (int j = 0; j < 8; j++) ad[j] = aw[j] = 0.f;
+    for (uint32_t s = cb + quad; s < ce; s += RT / 4) {
+      const uint32_t i = base + s;
+      const float dx = rs.dx[i], dy = rs.dy[i], dz = rs.dz[i], w = rs.w[i];
+      const float invd = 1.f / rs.dist[i];
+      float e[8];
+      const float pdf = soft_eval(c, J, dx, dy, dz, e);
+      if (pdf > 1e-16f) {
+        const float ip = 1.f / pdf;
 #pragma unroll
-      for (int j = 0; j < 8; j++) {
-        const float g = e[j] * ip;
-        const float wt = w * g * (e[j] * invw[j]);
-        ad[j] = fmaf(wt, invd, ad[j]);
-        aw[j] += wt;
+        for (int j = 0; j < 8; j++) {
+          const float g = e[j] * ip;
+          const float wt = w * g * (e[j] * invw[j]);
+          ad[j] = fmaf(wt, invd, ad[j]);
+          aw[j] += wt;
+        }
       }
     }
-  }
 #pragma unroll
-  for (int j = 0; j < 8; j++) {
-    const float vd = quad_group_sum(ad[j]);
-    const float vw = quad_group_sum(aw[j]);
-    if (lane < 4) {
-      sh.red[wave][4 * j + lane][0] = vd;
-      sh.red[wave][4 * j + lane][1] = vw;
+    for (int j = 0; j < 8; j++) {
+      const float vd = quad_group_sum(ad[j]);
+      const float vw = quad_group_sum(aw[j]);
+      if (lane < 4) {
+        sh.red[wave][4 * j + lane][0] = vd;
+        sh.red[wave][4 * j + lane][1] = vw;
+      }
     }
+    __syncthreads();
+    chunk_out(sh, tm, ch, 2, 0, -1, fs, fc, fd);
+    __syncthreads();
   }
-  __syncthreads();
-  if (tid == 0) sh.S.auxEvals += (float)n * (float)K;
+  pass_finish(sh, tm, -1, fs, fc, fd);
+  if (tid == 0) { sh.S.auxEvals += (float)n * (float)K; sh.S.prof[3] += (float)(wall_clock64() - tPass); }
 }
 
 // Split-statistics pass (Splitter:439-523): chi^2 terms, assigned mass and the
-// zero-mean tangent-plane covariance in each component's frame.
-__device__ void pass_split(Shared& sh, const RegionSamples& rs, uint32_t base, uint32_t n, float mcEst) {
+// zero-mean tangent-plane covariance in each component's frame.  Result:
+// fin[f*32+k] for f = chi, assigned, weight, cov xx, cov yy, cov xy;
+// fin[kScal] = number of samples with pdf > 0.
+__device__ void pass_split(Shared& sh, Team& tm, const RegionSamples& rs, uint32_t base, uint32_t n, float mcEst) {
+  const unsigned long long tPass = wall_clock64();
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, q = tid & 3, quad = tid >> 2;
   const VMMParams& V = sh.S.vmm;
   const int K = V.K;
   const int J = (K + 3) >> 2;
   CompRegs c;
   load_comps(V, q, c);
-  float achi[8], asa[8], asw[8], acx[8], acy[8], acz[8];
-#pragma unroll
-  for (int j = 0; j < 8; j++) achi[j] = asa[j] = asw[j] = acx[j] = acy[j] = acz[j] = 0.f;
-  float cnt = 0.f;
   const float inv_mc = 1.f / mcEst;
-  for (uint32_t s = quad; s < n; s += RT / 4) {
-    const uint32_t i = base + s;
-    const float dx = rs.dx[i], dy = rs.dy[i], dz = rs.dz[i], w = rs.w[i], spdf = rs.pdf[i];
-    float e[8];
-    const float pdf = soft_eval(c, J, dx, dy, dz, e);
-    if (pdf > 1e-16f) {
-      cnt += 1.f;
-      const float ip = 1.f / pdf;
-      const float value = w * spdf;
-      const float a = value * inv_mc * ip;  // value / (mc pdf)
-      const float ispdf = 1.f / spdf;
+  float fs = 0.f, fc = 0.f;
+  double fd = 0.0;
+  for (uint32_t ch = tm.rank; ch < tm.nc; ch += tm.T) {
+    const uint32_t cb = ch * kChunk, ce = min(n, cb + kChunk);
+    float achi[8], asa[8], asw[8], acx[8], acy[8], acz[8];
 #pragma unroll
-      for (int j = 0; j < 8; j++) {
-        if (j < J) {
-          const int k = 4 * j + q;
-          const float g = e[j] * ip;
-          const float vmf = e[j];
-          // (value^2 vmf / (mc pdf)^2 - 2 vmf value / (mc pdf) + vmf) / spdf
-          float chi = vmf * (a * a - 2.f * a + 1.f) * ispdf;
-          chi = g > 0.f ? chi : 0.f;
-          achi[j] += chi;
-          asa[j] += g;
-          const float lx = dot3f(sh.fxx[k], sh.fxy[k], sh.fxz[k], dx, dy, dz);
-          const float ly = dot3f(sh.fyx[k], sh.fyy[k], sh.fyz[k], dx, dy, dz);
-          const float aw = g * w;
-          asw[j] += aw;
-          acx[j] = fmaf(aw, lx * lx, acx[j]);
-          acy[j] = fmaf(aw, ly * ly, acy[j]);
-          acz[j] = fmaf(aw, lx * ly, acz[j]);
+    for (int j = 0; j < 8; j++) achi[j] = asa[j] = asw[j] = acx[j] = acy[j] = acz[j] = 0.f;
+    float cnt = 0.f;
+    for (uint32_t s = cb + quad; s < ce; s += RT / 4) {
+      const uint32_t i = base + s;
+      const float dx = rs.dx[i], dy = rs.dy[i], dz = rs.dz[i], w = rs.w[i], spdf = rs.pdf[i];
+      float e[8];
+      const float pdf = soft_eval(c, J, dx, dy, dz, e);
+      if (pdf > 1e-16f) {
+        cnt += 1.f;
+        const float ip = 1.f / pdf;
+        const float value = w * spdf;
+        const float a = value * inv_mc * ip;  // value / (mc pdf)
+        const float ispdf = 1.f / spdf;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          if (j < J) {
+            const int k = 4 * j + q;
+            const float g = e[j] * ip;
+            const float vmf = e[j];
+            // (value^2 vmf / (mc pdf)^2 - 2 vmf value / (mc pdf) + vmf) / spdf
+            float chi = vmf * (a * a - 2.f * a + 1.f) * ispdf;
+            chi = g > 0.f ? chi : 0.f;
+            achi[j] += chi;
+            asa[j] += g;
+            const float lx = dot3f(sh.fxx[k], sh.fxy[k], sh.fxz[k], dx, dy, dz);
+            const float ly = dot3f(sh.fyx[k], sh.fyy[k], sh.fyz[k], dx, dy, dz);
+            const float aw = g * w;
+            asw[j] += aw;
+            acx[j] = fmaf(aw, lx * lx, acx[j]);
+            acy[j] = fmaf(aw, ly * ly, acy[j]);
+            acz[j] = fmaf(aw, lx * ly, acz[j]);
+          }
         }
       }
     }
-  }
 #pragma unroll
-  for (int j = 0; j < 8; j++) {
-    const float v0 = quad_group_sum(achi[j]);
-    const float v1 = quad_group_sum(asa[j]);
-    const float v2 = quad_group_sum(asw[j]);
-    const float v3 = quad_group_sum(acx[j]);
-    const float v4 = quad_group_sum(acy[j]);
-    const float v5 = quad_group_sum(acz[j]);
-    if (lane < 4) {
-      const int k = 4 * j + lane;
-      sh.red[wave][k][0] = v0;
-      sh.red[wave][k][1] = v1;
-      sh.red[wave][k][2] = v2;
-      sh.red[wave][k][3] = v3;
-      sh.red[wave][k][4] = v4;
-      sh.red[wave][k][5] = v5;
+    for (int j = 0; j < 8; j++) {
+      const float v0 = quad_group_sum(achi[j]);
+      const float v1 = quad_group_sum(asa[j]);
+      const float v2 = quad_group_sum(asw[j]);
+      const float v3 = quad_group_sum(acx[j]);
+      const float v4 = quad_group_sum(acy[j]);
+      const float v5 = quad_group_sum(acz[j]);
+      if (lane < 4) {
+        const int k = 4 * j + lane;
+        sh.red[wave][k][0] = v0;
+        sh.red[wave][k][1] = v1;
+        sh.red[wave][k][2] = v2;
+        sh.red[wave][k][3] = v3;
+        sh.red[wave][k][4] = v4;
+        sh.red[wave][k][5] = v5;
+      }
     }
+    // every lane of a quad counted the sample: divide the wave sum by 4
+    const float vc = wave_sum(cnt) * 0.25f;
+    if (lane == 0) sh.scal[wave][0] = vc;
+    __syncthreads();
+    chunk_out(sh, tm, ch, 6, 1, -1, fs, fc, fd);
+    __syncthreads();
   }
-  // every lane of a quad counted the sample: divide the wave sum by 4
-  const float vc = wave_sum(cnt) * 0.25f;
-  if (lane == 0) sh.scal[wave][5] = vc;
-  __syncthreads();
-  if (tid == 0) {
-    sh.nAssigned = (sh.scal[0][5] + sh.scal[1][5]) + (sh.scal[2][5] + sh.scal[3][5]);
-    sh.S.splitEvals += (float)n * (float)K;
-  }
-  __syncthreads();
+  pass_finish(sh, tm, -1, fs, fc, fd);
+  if (tid == 0) { sh.S.splitEvals += (float)n * (float)K; sh.S.prof[2] += (float)(wall_clock64() - tPass); }
 }
 
-__device__ __forceinline__ float red4(const Shared& sh, int k, int c) {
-  return (sh.red[0][k][c] + sh.red[1][k][c]) + (sh.red[2][k][c] + sh.red[3][k][c]);
-}
+__device__ __forceinline__ float fin(const Shared& sh, int k, int f) { return sh.fin[f * 32 + k]; }
 
 // ------------------------------------------------------ region-level steps
 // UpdateSplitStatistics into SS (S.split or the local recursive-split copy).
-__device__ void update_split_stats(Shared& sh, SplitStatsState& SS, const RegionSamples& rs, uint32_t base,
+__device__ void update_split_stats(Shared& sh, Team& tm, SplitStatsState& SS, const RegionSamples& rs, uint32_t base,
                                    uint32_t n, float mcEst) {
   const int tid = threadIdx.x;
   const VMMParams& V = sh.S.vmm;
@@ -432,21 +525,21 @@ __device__ void update_split_stats(Shared& sh, SplitStatsState& SS, const Region
     sh.fyx[k] = yx; sh.fyy[k] = yy; sh.fyz[k] = yz;
   }
   __syncthreads();
-  pass_split(sh, rs, base, n, mcEst);
+  pass_split(sh, tm, rs, base, n, mcEst);
   if (tid < 32) {
     const int k = tid;
-    const float nA = sh.nAssigned;
+    const float nA = sh.fin[kScal];
     const int lim = vec_lim(sh.W, SS.K);
     if (k < SS.K) {
       const float nOld = SS.sn[k];
       const float nNew = nOld + nA;
-      SS.ssa[k] += red4(sh, k, 1);
-      SS.ssw[k] += red4(sh, k, 2);
-      SS.covx[k] += red4(sh, k, 3);
-      SS.covy[k] += red4(sh, k, 4);
-      SS.covz[k] += red4(sh, k, 5);
+      SS.ssa[k] += fin(sh, k, 1);
+      SS.ssw[k] += fin(sh, k, 2);
+      SS.covx[k] += fin(sh, k, 3);
+      SS.covy[k] += fin(sh, k, 4);
+      SS.covz[k] += fin(sh, k, 5);
       // running mean chi += (x - chi)/count over the assigned samples, in closed form
-      if (nA > 0.f) SS.chi[k] = (SS.chi[k] * nOld + red4(sh, k, 0)) / nNew;
+      if (nA > 0.f) SS.chi[k] = (SS.chi[k] * nOld + fin(sh, k, 0)) / nNew;
       SS.sn[k] = nNew;
     } else if (k < lim) {
       SS.sn[k] += nA;
@@ -455,15 +548,16 @@ __device__ void update_split_stats(Shared& sh, SplitStatsState& SS, const Region
   __syncthreads();
 }
 
-__device__ void distances(Shared& sh, SuffStats& ST, const RegionSamples& rs, uint32_t base, uint32_t n, bool init) {
-  pass_distances(sh, rs, base, n);
+__device__ void distances(Shared& sh, Team& tm, SuffStats& ST, const RegionSamples& rs, uint32_t base, uint32_t n,
+                          bool init) {
+  pass_distances(sh, tm, rs, base, n);
   const int tid = threadIdx.x;
   VMMParams& V = sh.S.vmm;
   if (tid < 32) {
     const int k = tid;
     const int lim = vec_lim(sh.W, V.K);
     if (k < V.K) {
-      const float bd = red4(sh, k, 0), bw = red4(sh, k, 1);
+      const float bd = fin(sh, k, 0), bw = fin(sh, k, 1);
       if (init) {
         ST.sdw[k] = bw;
         V.dist[k] = ST.sdw[k] / bd;
@@ -483,7 +577,7 @@ __device__ void distances(Shared& sh, SuffStats& ST, const RegionSamples& rs, ui
 
 // EM loop (updateMixture / partialUpdateMixture). prev = region statistics or
 // the partial-refit temporary; masked = partial fit over sh.mask.
-__device__ int em(Shared& sh, SuffStats& prev, const RegionSamples& rs, uint32_t base, uint32_t n,
+__device__ int em(Shared& sh, Team& tm, SuffStats& prev, const RegionSamples& rs, uint32_t base, uint32_t n,
                   const FactoryConfig& cfg, bool masked) {
   const int tid = threadIdx.x;
   VMMParams& V = sh.S.vmm;
@@ -499,12 +593,21 @@ __device__ int em(Shared& sh, SuffStats& prev, const RegionSamples& rs, uint32_t
     if (tid < 32) suff_zero_slot(cur, tid);
     if (tid == 0) { cur.K = V.K; cur.numSamples = (float)n; cur.sumWeights = 0.f; cur.normalized = 0; }
     __syncthreads();
-    pass_estep(sh, rs, base, n);
+    pass_estep(sh, tm, rs, base, n);
+    if (tid < K_ESTEP_COPY) {
+      const int k = tid;
+      if (k < V.K) {
+        cur.swdx[k] = fin(sh, k, 0); cur.swdy[k] = fin(sh, k, 1); cur.swdz[k] = fin(sh, k, 2); cur.sw[k] = fin(sh, k, 3);
+      }
+    }
+    if (tid == 0) sh.ll = sh.finLL;
+    __syncthreads();
     // unassigned mass becomes a new component (WEM:732-748)
-    if (tid == 0 && sh.un[0] > 0.f && cur.K < kMaxComponents) {
+    const float un0 = sh.fin[kScal + 1];
+    if (tid == 0 && un0 > 0.f && cur.K < kMaxComponents) {
       const int k = cur.K;
       cur.K++;
-      cur.sw[k] = sh.un[0]; cur.swdx[k] = sh.un[1]; cur.swdy[k] = sh.un[2]; cur.swdz[k] = sh.un[3];
+      cur.sw[k] = un0; cur.swdx[k] = sh.fin[kScal + 2]; cur.swdy[k] = sh.fin[kScal + 3]; cur.swdz[k] = sh.fin[kScal + 4];
       prev.K++;
       prev.sw[k] = 0.f; prev.swdx[k] = 0.f; prev.swdy[k] = 0.f; prev.swdz[k] = 0.f;
       V.K++;
@@ -735,8 +838,8 @@ __device__ void clear_split(Shared& sh, SplitStatsState& s, int K) {
 }
 
 // PerformRecursiveSplitting (Splitter:293-356)
-__device__ int recursive_splitting(Shared& sh, const RegionSamples& rs, uint32_t base, uint32_t n, float mcEst,
-                                   const FactoryConfig& cfg, int& pits) {
+__device__ int recursive_splitting(Shared& sh, Team& tm, const RegionSamples& rs, uint32_t base, uint32_t n,
+                                   float mcEst, const FactoryConfig& cfg, int& pits) {
   const int tid = threadIdx.x;
   VMMParams& V = sh.S.vmm;
   int total = 0;
@@ -746,8 +849,9 @@ __device__ int recursive_splitting(Shared& sh, const RegionSamples& rs, uint32_t
   while (V.K < kMaxComponents && sh.numSplits != 0) {
     clear_split(sh, sh.lsplit, 32);
     clear_split(sh, sh.lsplit, V.K);
-    update_split_stats(sh, sh.lsplit, rs, base, n, mcEst);
+    update_split_stats(sh, tm, sh.lsplit, rs, base, n, mcEst);
     if (tid == 0) {
+      const unsigned long long tS = wall_clock64();
       sort_candidates(sh.lsplit, sh.order);
       sh.mask = 0u;
       int ns = 0;
@@ -762,11 +866,12 @@ __device__ int recursive_splitting(Shared& sh, const RegionSamples& rs, uint32_t
         }
       }
       sh.numSplits = ns;
+      sh.S.prof[6] += (float)(wall_clock64() - tS);
     }
     __syncthreads();
     if (sh.numSplits > 0) {
       clear_suff(sh, sh.tmp, V.K);
-      pits += em(sh, sh.tmp, rs, base, n, cfg, true);
+      pits += em(sh, tm, sh.tmp, rs, base, n, cfg, true);
       if (tid == 0) sh.S.suff.K = V.K;
       __syncthreads();
       masked_replace(sh, sh.S.suff, sh.tmp);
@@ -937,28 +1042,44 @@ __device__ void suff_merge(SuffStats& ST, int i, int j) {  // WEM:485-518
 
 // PerformMerging with ThresholdedMergeNext (Merger:45-89, 218-279)
 __device__ int perform_merging(Shared& sh, SuffStats& ST, SplitStatsState& SS, const FactoryConfig& cfg) {
+  const unsigned long long tMerge = wall_clock64();
   const int tid = threadIdx.x;
   VMMParams& V = sh.S.vmm;
   int total = 0;
   while (V.K > sh.W) {
     const int K = V.K;
+    // costs of all pairs i < j < K; the merge candidate is the pair of least
+    // cost below the threshold with both components holding split samples,
+    // ties to the first pair in (i, j) order -- the serial scan of
+    // ThresholdedMergeNext (Merger:218-279) as a (cost, pair index) arg-min
+    float best = 3.402823466e38f;
+    int bestP = 0x7fffffff;
     for (int p = tid; p < 32 * 32; p += RT) {
       const int i = p >> 5, j = p & 31;
-      sh.cost[p] = (i < j && j < K) ? merge_cost(V, i, j) : 3.402823466e38f;
+      if (i < j && j < K) {
+        const float cst = merge_cost(V, i, j);
+        if (cst < cfg.mergingThreshold && SS.sn[i] > 0.f && SS.sn[j] > 0.f && (cst < best || (cst == best && p < bestP))) {
+          best = cst;
+          bestP = p;
+        }
+      }
     }
+    for (int off = 32; off > 0; off >>= 1) {
+      const float ob = __shfl_xor(best, off);
+      const int op = __shfl_xor(bestP, off);
+      if (ob < best || (ob == best && op < bestP)) { best = ob; bestP = op; }
+    }
+    if ((tid & 63) == 0) { sh.cost[tid >> 6] = best; sh.order[tid >> 6] = bestP; }
     __syncthreads();
     if (tid == 0) {
-      float minCost = 3.402823466e38f;
-      int ci = 0, cj = 0, found = 0;
-      for (int i = 0; i < K - 1; i++)
-        for (int j = i + 1; j < K; j++) {
-          const float cst = sh.cost[i * 32 + j];
-          if (cst < cfg.mergingThreshold && cst < minCost && SS.sn[i] > 0.f && SS.sn[j] > 0.f) {
-            ci = i; cj = j; minCost = cst; found = 1;
-          }
-        }
+      float minCost = sh.cost[0];
+      int pm = sh.order[0];
+      for (int w = 1; w < RWAVES; w++)
+        if (sh.cost[w] < minCost || (sh.cost[w] == minCost && sh.order[w] < pm)) { minCost = sh.cost[w]; pm = sh.order[w]; }
+      const int found = pm != 0x7fffffff;
       sh.found = found;
       if (found) {
+        const int ci = pm >> 5, cj = pm & 31;
         const float mIx = V.mux[ci], mIy = V.muy[ci], mIz = V.muz[ci], wI = V.w[ci];
         const float mJx = V.mux[cj], mJy = V.muy[cj], mJz = V.muz[cj], wJ = V.w[cj];
         vmm_merge(V, ci, cj);
@@ -970,6 +1091,7 @@ __device__ int perform_merging(Shared& sh, SuffStats& ST, SplitStatsState& SS, c
     if (!sh.found) break;
     total++;
   }
+  if (tid == 0) sh.S.prof[5] += (float)(wall_clock64() - tMerge);
   return total;
 }
 
@@ -1003,8 +1125,8 @@ __device__ bool region_valid(Shared& sh, bool checkSuff) {
 }
 
 // ---- ASM::fit (ASM:303-347)
-__device__ void region_fit(Shared& sh, const RegionSamples& rs, uint32_t base, uint32_t n, const FactoryConfig& cfg,
-                           const float* __restrict__ uniformDirs) {
+__device__ void region_fit(Shared& sh, Team& tm, const RegionSamples& rs, uint32_t base, uint32_t n,
+                           const FactoryConfig& cfg, const float* __restrict__ uniformDirs) {
   const int tid = threadIdx.x;
   VMMParams& V = sh.S.vmm;
   const int K0 = cfg.initK;
@@ -1034,17 +1156,17 @@ __device__ void region_fit(Shared& sh, const RegionSamples& rs, uint32_t base, u
   clear_suff(sh, sh.S.suff, K0);
   if (tid == 0) sh.S.suff.normalized = 1;
   __syncthreads();
-  const int its = em(sh, sh.S.suff, rs, base, n, cfg, false);
-  distances(sh, sh.S.suff, rs, base, n, true);
+  const int its = em(sh, tm, sh.S.suff, rs, base, n, cfg, false);
+  distances(sh, tm, sh.S.suff, rs, base, n, true);
   int splits = 0, merges = 0, pits = 0;
   if (cfg.useSplitAndMerge) {
     const float mcEst = sh.S.suff.sumWeights / sh.S.suff.numSamples;
-    splits = recursive_splitting(sh, rs, base, n, mcEst, cfg, pits);
+    splits = recursive_splitting(sh, tm, rs, base, n, mcEst, cfg, pits);
     clear_split(sh, sh.S.split, V.K);
-    update_split_stats(sh, sh.S.split, rs, base, n, mcEst);
+    update_split_stats(sh, tm, sh.S.split, rs, base, n, mcEst);
     merges = perform_merging(sh, sh.S.suff, sh.S.split, cfg);
   }
-  distances(sh, sh.S.suff, rs, base, n, true);
+  distances(sh, tm, sh.S.suff, rs, base, n, true);
   if (tid == 0) {
     sh.S.afterSplitLo = sh.S.afterSplitHi = sh.S.afterMergeLo = sh.S.afterMergeHi = 0;
     sh.S.emIterations = its; sh.S.partialEmIterations = pits; sh.S.numSplits = splits; sh.S.numMerges = merges;
@@ -1057,13 +1179,13 @@ __device__ __forceinline__ unsigned long long get64(uint32_t lo, uint32_t hi) {
 }
 
 // ---- ASM::update (ASM:352-462)
-__device__ void region_update_body(Shared& sh, const RegionSamples& rs, uint32_t base, uint32_t n,
+__device__ void region_update_body(Shared& sh, Team& tm, const RegionSamples& rs, uint32_t base, uint32_t n,
                                    const FactoryConfig& cfg) {
   const int tid = threadIdx.x;
   VMMParams& V = sh.S.vmm;
   RegionState& S = sh.S;
   const int prevK = V.K;
-  const int its = em(sh, S.suff, rs, base, n, cfg, false);
+  const int its = em(sh, tm, S.suff, rs, base, n, cfg, false);
   if (tid == 0 && prevK < V.K) S.split.K = V.K;
   __syncthreads();
   int splits = 0, merges = 0, pits = 0;
@@ -1071,9 +1193,10 @@ __device__ void region_update_body(Shared& sh, const RegionSamples& rs, uint32_t
     const float mcEst = S.suff.sumWeights / S.suff.numSamples;
     unsigned long long afterSplit = get64(S.afterSplitLo, S.afterSplitHi) + n;
     unsigned long long afterMerge = get64(S.afterMergeLo, S.afterMergeHi) + n;
-    update_split_stats(sh, S.split, rs, base, n, mcEst);
+    update_split_stats(sh, tm, S.split, rs, base, n, mcEst);
     if (afterSplit >= (unsigned long long)(cfg.minSamplesForSplitting < 0 ? 0 : cfg.minSamplesForSplitting)) {
       if (tid == 0) {
+        const unsigned long long tS = wall_clock64();
         sort_candidates(S.split, sh.order);
         sh.mask = 0u;
         int ns = 0;
@@ -1088,13 +1211,14 @@ __device__ void region_update_body(Shared& sh, const RegionSamples& rs, uint32_t
           }
         }
         sh.numSplits = ns;
+        S.prof[6] += (float)(wall_clock64() - tS);
       }
       __syncthreads();
       splits = sh.numSplits;
       if (splits > 0 && cfg.partialReFit && n >= (uint32_t)(cfg.minSamplesForPartialRefitting < 0 ? 0 : cfg.minSamplesForPartialRefitting)) {
         copy_suff(sh, sh.tmp, S.suff);
         clear_suff(sh, sh.tmp, V.K);
-        pits = em(sh, sh.tmp, rs, base, n, cfg, true);
+        pits = em(sh, tm, sh.tmp, rs, base, n, cfg, true);
         if (tid == 0) S.suff.K = V.K;
         __syncthreads();
         masked_replace(sh, S.suff, sh.tmp);
@@ -1113,17 +1237,17 @@ __device__ void region_update_body(Shared& sh, const RegionSamples& rs, uint32_t
     }
     __syncthreads();
   }
-  distances(sh, S.suff, rs, base, n, false);
+  distances(sh, tm, S.suff, rs, base, n, false);
   if (tid == 0) { S.emIterations = its; S.partialEmIterations = pits; S.numSplits = splits; S.numMerges = merges; }
   __syncthreads();
 }
 
-__global__ void __launch_bounds__(RT) k_region_update(RegionState* __restrict__ states,
-                                                      const uint32_t* __restrict__ regionList, RegionSamples rs,
-                                                      FactoryConfig cfg, const float* __restrict__ uniformDirs) {
-  __shared__ Shared sh;
+// One region on one team rank: load the state into LDS, run fit or update,
+// rank 0 writes the state back.
+__device__ void region_job(Shared& sh, Team& tm, RegionState* __restrict__ states, uint32_t r, const RegionSamples& rs,
+                           const FactoryConfig& cfg, const float* __restrict__ uniformDirs) {
+  const unsigned long long tJob = wall_clock64();
   const int tid = threadIdx.x;
-  const uint32_t r = regionList[blockIdx.x];
   {
     const uint32_t* src = (const uint32_t*)(states + r);
     uint32_t* dst = (uint32_t*)&sh.S;
@@ -1136,6 +1260,8 @@ __global__ void __launch_bounds__(RT) k_region_update(RegionState* __restrict__ 
   if (tid == 0) {
     S.emIterations = 0; S.partialEmIterations = 0; S.numSplits = 0; S.numMerges = 0;
     S.estepEvals = 0.f; S.auxEvals = 0.f; S.splitEvals = 0.f;
+    for (int i = 0; i < 8; i++) S.prof[i] = 0.f;
+    S.prof[7] = (float)tm.T;
     sh.W = cfg.vectorWidth;
   }
   __syncthreads();
@@ -1185,11 +1311,11 @@ __global__ void __launch_bounds__(RT) k_region_update(RegionState* __restrict__ 
       __syncthreads();
       if (tid == 0) { S.vmm.pivot[0] -= shx; S.vmm.pivot[1] -= shy; S.vmm.pivot[2] -= shz; }
       __syncthreads();
-      region_update_body(sh, rs, base, n, cfg);
+      region_update_body(sh, tm, rs, base, n, cfg);
       const bool ok = region_valid(sh, true);
       if (tid == 0) S.valid = ok ? 1 : 0;
     } else if (mode == 1) {
-      region_fit(sh, rs, base, n, cfg, uniformDirs);
+      region_fit(sh, tm, rs, base, n, cfg, uniformDirs);
       if (tid == 0) { S.vmm.pivot[0] = S.ssMean[0]; S.vmm.pivot[1] = S.ssMean[1]; S.vmm.pivot[2] = S.ssMean[2]; }
       const bool ok = region_valid(sh, false);
       if (tid == 0) { S.valid = ok ? 1 : 0; S.splitFlag = 0; S.fitted = 1; }
@@ -1197,11 +1323,49 @@ __global__ void __launch_bounds__(RT) k_region_update(RegionState* __restrict__ 
   } else if (mode == 1) {
     if (tid == 0) { S.valid = 0; S.splitFlag = 0; }
   }
+  if (tid == 0) sh.S.prof[0] = (float)(wall_clock64() - tJob);
   __syncthreads();
-  {
+  if (tm.rank == 0) {
     uint32_t* dst = (uint32_t*)(states + r);
     const uint32_t* src = (const uint32_t*)&sh.S;
     for (int i = tid; i < (int)(sizeof(RegionState) / 4); i += RT) dst[i] = src[i];
+  }
+}
+
+
+// Persistent region kernel: each workgroup takes queue items (job, rank) in
+// order until the queue is empty; the ranks of a job are queued
+// consecutively, so a team is always completed by the next free workgroups
+// and every wait ends (given T <= grid, which the host guarantees, and a
+// co-resident grid).
+__global__ void __launch_bounds__(RT) k_region_update(RegionState* __restrict__ states,
+                                                      const RegionJob* __restrict__ jobs,
+                                                      const uint2* __restrict__ items, uint32_t numItems,
+                                                      uint32_t* __restrict__ queue, uint32_t* __restrict__ counters,
+                                                      float* __restrict__ partial, int* __restrict__ err,
+                                                      unsigned long long timeout, RegionSamples rs, FactoryConfig cfg,
+                                                      const float* __restrict__ uniformDirs) {
+  __shared__ Shared sh;
+  const int tid = threadIdx.x;
+  if (tid == 0) sh.abort = 0;
+  for (;;) {
+    if (tid == 0) sh.item = atomicAdd(queue, 1u);
+    __syncthreads();
+    const uint32_t it = sh.item;
+    if (it >= numItems) break;
+    const uint2 item = items[it];
+    const RegionJob job = jobs[item.x];
+    Team tm;
+    tm.rank = item.y;
+    tm.T = job.team;
+    tm.nc = job.chunks;
+    tm.part = partial + job.partialOffset;
+    tm.counter = counters + item.x;
+    tm.epoch = 0;
+    tm.err = err;
+    tm.timeout = timeout;
+    region_job(sh, tm, states, job.region, rs, cfg, uniformDirs);
+    __syncthreads();
   }
 }
 
@@ -1242,12 +1406,36 @@ void launch_set_inputs(RegionState* states, const RegionInput* inputs, uint32_t 
   hipLaunchKernelGGL(k_set_inputs, dim3((count + 255) / 256), dim3(256), 0, s, states, inputs, count);
 }
 
-void launch_region_update(RegionState* states, const uint32_t* regionList, uint32_t numRegions, const float* dx,
-                          const float* dy, const float* dz, const float* w, const float* pdf, const float* dist,
-                          const FactoryConfig& cfg, const float* uniformDirs, hipStream_t s) {
-  if (numRegions == 0) return;
+__global__ void __launch_bounds__(256) k_zero_u32(uint32_t* __restrict__ p, uint32_t count) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = 0u;
+}
+
+int region_kernel_blocks_per_cu() {
+  int b = 0;
+  PGL_KERNEL_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_region_update, RT, 0));
+  return b;
+}
+
+void launch_region_update(RegionState* states, const RegionJob* jobs, uint32_t numJobs, const uint2* items,
+                          uint32_t numItems, uint32_t grid, uint32_t* queueAndCounters, float* partial, int* err,
+                          unsigned long long timeoutTicks, const float* dx, const float* dy, const float* dz, const float* w, const float* pdf,
+                          const float* dist, const FactoryConfig& cfg, const float* uniformDirs, bool cooperative,
+                          hipStream_t s) {
+  if (numItems == 0) return;
+  const uint32_t nz = numJobs + 1;
+  hipLaunchKernelGGL(k_zero_u32, dim3((nz + 255) / 256), dim3(256), 0, s, queueAndCounters, nz);
   RegionSamples rs{dx, dy, dz, w, pdf, dist};
-  hipLaunchKernelGGL(k_region_update, dim3(numRegions), dim3(RT), 0, s, states, regionList, rs, cfg, uniformDirs);
+  uint32_t* queue = queueAndCounters;
+  uint32_t* counters = queueAndCounters + 1;
+  if (cooperative) {
+    void* args[] = {&states,  &jobs, &items,        &numItems, &queue,      &counters,
+                    &partial, &err,  &timeoutTicks, &rs,       (void*)&cfg, &uniformDirs};
+    PGL_KERNEL_CHECK(hipLaunchCooperativeKernel((const void*)k_region_update, dim3(grid), dim3(RT), args, 0, s));
+  } else {
+    hipLaunchKernelGGL(k_region_update, dim3(grid), dim3(RT), 0, s, states, jobs, items, numItems, queue, counters,
+                       partial, err, timeoutTicks, rs, cfg, uniformDirs);
+  }
 }
 
 }  // namespace pglgpu

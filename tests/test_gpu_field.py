@@ -115,3 +115,46 @@ def test_query_parity_surface_and_volume():
         print("volume" if volume else "surface", "queries", valid.sum(), "median rel", np.median(rel),
               "frac<1e-3", np.mean(rel < 1e-3))
         assert np.mean(rel < 1e-3) >= 0.95
+
+
+_DIRECTIONAL_KEYS = ["K", "weights", "kappas", "meanDirections", "normalizations", "meanCosines", "distances",
+                     "pivot", "sumOfWeightedDirections", "sumOfWeightedStats", "sumOfDistanceWeights",
+                     "chiSquareMCEstimates", "splitCovariances", "splitNumSamples", "emIterations",
+                     "partialEmIterations", "numSplits", "numMerges", "valid"]
+
+
+def _directional_state(field):
+    out = []
+    for i in range(field.num_regions()):
+        r = field.region(i)
+        out.append({k: np.asarray(r[k]).copy() for k in _DIRECTIONAL_KEYS})
+    return out
+
+
+def test_schedule_independence():
+    """The directional update is bit-identical for the per-region path, the
+    default team schedule and forced large teams (chunking and fold order do
+    not depend on the team size)."""
+    import openpgl_amd as pgl
+    dev = pgl.Device()
+    frames = [generate(120000, seed=4, frame=f) for f in range(3)]
+    schedules = [dict(per_region_launches=1),
+                 dict(per_region_launches=0),
+                 dict(per_region_launches=0, balance=64.0, min_chunks_per_rank=1, max_team=64),
+                 dict(per_region_launches=0, grid=7, balance=2.0)]
+    states, teams = [], []
+    for sc in schedules:
+        f = pgl.Field(dev, max_samples_per_leaf=8000, deterministic=True)
+        f.set_region_schedule(**sc)
+        print("schedule", sc, flush=True)
+        for fr in frames:
+            f.update(pgl.SampleStorage(fr))
+            teams.append(f.last_update_info()["max_team"])
+        states.append(_directional_state(f))
+    print("largest teams per update", teams)
+    assert max(teams) > 1
+    for s in states[1:]:
+        assert len(s) == len(states[0])
+        for i, (a, b) in enumerate(zip(states[0], s)):
+            for k in _DIRECTIONAL_KEYS:
+                assert np.asarray(a[k]).tobytes() == np.asarray(b[k]).tobytes(), (i, k, a[k], b[k])
