@@ -11,11 +11,14 @@ HBM before timing (pglFieldUpdateSurfaceDevice).
 value = timed samples / timed seconds.  The roofline object reports the
 region kernel (the dominant kernel: weighted EM + split/merge for all regions
 in one launch) against the FP32 vector peak; its time comes from HIP events
-recorded inside the library on the kernel's stream.  cpu_baseline times the
-oracle (a CPU restatement of the reference, std::thread over regions) on a
-bounded sample of the same workload.  A closed-loop serving run (batches of
-log-uniform size in [1k, 256k] queries) reports guided samples/s and per-batch
-p50/p99 latency.
+recorded inside the library around the kernel launch on the kernel's stream.
+traffic (HBM bytes per launch) is read from profiles/k_region_update_traffic.txt,
+which holds the rocprofv3 FETCH_SIZE / WRITE_SIZE measurements of the same
+workload and the copy-kernel calibration (--calibrate-copy).  cpu_baseline
+times the oracle (a CPU restatement of the reference, std::thread over
+regions) on a bounded sample of the same workload.  A closed-loop serving run
+(batches of log-uniform size in [1k, 256k] queries) reports guided samples/s
+and per-batch p50/p99 latency.
 """
 import argparse
 import json
@@ -107,19 +110,53 @@ def serving(field, n_batches=40, seed=0):
             "valid_fraction": float(np.mean(pdf[:sizes[-1]] > 0)), "mode": "closed-loop, device-resident inputs"}
 
 
-def cpu_baseline(frames_host, threads):
-    """Oracle on a bounded sample: frame 0 (build, untimed) then frame 1 (timed update)."""
+def cpu_baseline(frames_host, threads, timed_frames=3):
+    """Oracle on a bounded sample: frame 0 (build, untimed), then the incremental
+    updates of frames 1..timed_frames timed."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from field_compare import OracleField
     from oracle_ctypes import default_config
     o = OracleField(default_config(32000))
     o.update(frames_host[0], threads)
     t0 = time.perf_counter()
-    o.update(frames_host[1], threads)
+    n = 0
+    for f in range(1, timed_frames + 1):
+        o.update(frames_host[f], threads)
+        n += len(frames_host[f])
     dt = time.perf_counter() - t0
-    return {"value": len(frames_host[1]) / dt, "unit": "samples/s", "cores": threads, "kind": "port",
-            "sample": "oracle (CPU restatement of Open PGL 0.6.0), one incremental Update of frame 1 "
-                      "(1M samples) after the frame-0 build; %.1f s" % dt}
+    return {"value": n / dt, "unit": "samples/s", "cores": threads, "kind": "port",
+            "sample": "oracle (CPU restatement of Open PGL 0.6.0, std::thread over regions), incremental "
+                      "Updates of frames 1-%d (%d samples) after the frame-0 build; %.2f s" % (timed_frames, n, dt)}
+
+
+def read_traffic():
+    """HBM bytes per region-kernel launch from the committed rocprofv3 summary."""
+    path = os.path.join(ROOT, "profiles", "k_region_update_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if ":" in line and not line.lstrip().startswith("#"):
+            k, v = line.split(":", 1)
+            try:
+                vals[k.strip()] = float(v.split()[0])
+            except (ValueError, IndexError):
+                pass
+    if "traffic_bytes_per_launch" not in vals:
+        return None
+    return vals["traffic_bytes_per_launch"]
+
+
+def calibrate_copy(nbytes=1 << 30, reps=5):
+    """Plain device-to-device copy of known size (the FETCH_SIZE / WRITE_SIZE
+    calibration kernel for rocprofv3 --pmc)."""
+    from openpgl_amd.hip_runtime import DeviceArray, hip, synchronize
+    a = DeviceArray(nbytes)
+    b = DeviceArray(nbytes)
+    for _ in range(reps):
+        hip().hipMemcpy(b.ptr, a.ptr, nbytes, 3)
+    synchronize()
+    print(json.dumps({"copy_bytes": nbytes, "reps": reps}), flush=True)
 
 
 def main():
@@ -130,11 +167,16 @@ def main():
     ap.add_argument("--samples", type=int, default=N_PER_FRAME)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 1 GiB device copy (rocprofv3 --pmc calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md for how it would "
               "shard); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     nframes = args.warmup + args.steps
     log("generating", nframes, "frames of", args.samples, "samples")
     frames = make_frames(args.samples, nframes)
@@ -142,7 +184,7 @@ def main():
     timed = sum(len(frames[args.warmup + s]) for s in range(args.steps))
     value = timed / dt
     # roofline of the region kernel over the timed frames
-    kern_ms = sum(i["directional_ms"] for i in infos)
+    kern_ms = sum(i["region_kernel_ms"] for i in infos)
     flops = sum(FLOPS_ESTEP * i["estep_evals"] + FLOPS_SPLIT * i["split_evals"] + FLOPS_DIST * i["dist_evals"]
                 for i in infos)
     achieved = flops / (kern_ms * 1e-3) / 1e12
@@ -155,6 +197,7 @@ def main():
         "knn_ms_avg": float(np.mean([i["knn_ms"] for i in infos])),
         "wall_ms_avg": float(np.mean([i["wall_ms"] for i in infos])),
         "regions_last": int(infos[-1]["regions"]),
+        "directional_ms_avg": float(np.mean([i["directional_ms"] for i in infos])),
     }
     out = {
         "metric": "Field::Update samples/s; guided samples/s + p50/p99 at fixed PDF error",
@@ -173,14 +216,14 @@ def main():
                                "split-and-merge on, maxK 32, kd max depth 32, maxSamplesPerLeaf 32000",
                    "samples_per_frame": args.samples, "frames_timed": args.steps, "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None, "kernel": "k_region_update",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": read_traffic(), "kernel": "k_region_update",
                      **per_launch},
     }
     if not args.no_serving:
         out["serving"] = serving(field)
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
-        out["cpu_baseline"] = cpu_baseline(frames, threads)
+        out["cpu_baseline"] = cpu_baseline(frames, threads, timed_frames=min(3, nframes - 1))
     print(json.dumps(out), flush=True)
 
 

@@ -267,8 +267,9 @@ void pglFieldGetKNN(PGLField field, int kind, uint32_t* neighbours /* 8 per regi
  * [7] region kernel launches, [8] E-step sample x component evaluations,
  * [9] distance-pass evaluations, [10] EM iterations (sum), [11] samples,
  * [12] split-statistics evaluations, [13] regions, [14] region jobs,
- * [15] queue items (sum of team sizes), [16] largest team, [17] grid.
- * Writes min(count, 18) values. */
+ * [15] queue items (sum of team sizes), [16] largest team, [17] grid,
+ * [18] region kernel GPU time (events around its launch only).
+ * Writes min(count, 19) values. */
 void pglFieldGetLastUpdateInfo(PGLField field, int kind, double* out, size_t count);
 
 /* EXTENSION: scheduling of the directional update.  None of the fields
@@ -288,10 +289,11 @@ typedef struct {
   uint32_t maxTeam;
 } PGLRegionSchedule;
 void pglFieldSetRegionSchedule(PGLField field, const PGLRegionSchedule* schedule);
-/* EXTENSION: per-region profile of the last update, 8 floats per region in
+/* EXTENSION: per-region profile of the last update, 12 floats per region in
  * storage order: microseconds of [0] the whole region, [1] E-step passes,
  * [2] split-statistics passes, [3] distance passes, [4] team barrier waits,
- * [5] merging, [6] splitting logic; [7] team size. */
+ * [5] merging, [6] splitting logic, [9] M-steps and convergence tests,
+ * [10] cross-chunk folds; [7] team size, [8] E-step passes, [11] samples. */
 void pglFieldGetRegionProfile(PGLField field, int kind, float* out);
 void pglFieldGetRegionSchedule(PGLField field, PGLRegionSchedule* schedule);
 

@@ -294,9 +294,10 @@ class Field:
         self.lib.pglFieldSetRegionSchedule(self.h, ctypes.byref(sc))
 
     def region_profile(self, kind=0):
-        """(regions, 8): us of total, E-step, split, distance passes, barrier waits, merging,
-        splitting logic; team size.  Storage order."""
-        out = np.zeros((self.num_regions(kind), 8), np.float32)
+        """(regions, 12): us of total, E-step, split, distance passes, barrier waits, merging,
+        splitting logic, [7] team size, [8] E-step passes, [9] M-steps, [10] folds, [11] samples.
+        Storage order."""
+        out = np.zeros((self.num_regions(kind), 12), np.float32)
         self.lib.pglFieldGetRegionProfile(self.h, kind, _ptr(out))
         return out
 
@@ -335,11 +336,11 @@ class Field:
         return out.reshape(-1, 8)
 
     def last_update_info(self, kind=0):
-        o = np.zeros(18, np.float64)
-        self.lib.pglFieldGetLastUpdateInfo(self.h, kind, _ptr(o), 18)
+        o = np.zeros(19, np.float64)
+        self.lib.pglFieldGetLastUpdateInfo(self.h, kind, _ptr(o), 19)
         keys = ["wall_ms", "spatial_ms", "directional_ms", "knn_ms", "gather_ms", "sort_ms", "kd_rounds",
                 "region_launches", "estep_evals", "dist_evals", "em_iterations", "samples", "split_evals", "regions",
-                "region_jobs", "queue_items", "max_team", "grid"]
+                "region_jobs", "queue_items", "max_team", "grid", "region_kernel_ms"]
         return dict(zip(keys, o.tolist()))
 
     # --- queries

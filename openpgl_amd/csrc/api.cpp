@@ -725,21 +725,14 @@ void pglFieldGetKNN(PGLField field, int kind, uint32_t* nb) {
 void pglFieldGetLastUpdateInfo(PGLField field, int kind, double* out, size_t count) {
   KindField& k = kind_of(field, kind);
   const UpdateInfo& i = k.lastInfo();
-  double o[18];
+  double o[19];
   o[0] = i.wallMs; o[1] = i.spatialMs; o[2] = i.directionalMs; o[3] = i.knnMs; o[4] = i.gatherMs; o[5] = i.sortMs;
   o[6] = i.kdRounds; o[7] = i.regionLaunches;
-  k.syncMirror(field->gf->stream);
-  double ee = 0, de = 0, se = 0, it = 0;
-  for (uint32_t r = 0; r < k.numRegions(); r++) {
-    const RegionState& S = k.mirrorRegion(r);
-    ee += S.estepEvals;
-    de += S.auxEvals;
-    se += S.splitEvals;
-    it += S.emIterations + S.partialEmIterations;
-  }
-  o[8] = ee; o[9] = de; o[10] = it; o[11] = i.samples; o[12] = se; o[13] = k.numRegions();
+  o[8] = i.estepEvals; o[9] = i.auxEvals; o[10] = i.emIterations; o[11] = i.samples; o[12] = i.splitEvals;
+  o[13] = k.numRegions();
   o[14] = i.regionJobs; o[15] = i.regionItems; o[16] = i.maxTeam; o[17] = i.regionGrid;
-  for (size_t j = 0; j < count && j < 18; j++) out[j] = o[j];
+  o[18] = i.regionKernelMs;
+  for (size_t j = 0; j < count && j < 19; j++) out[j] = o[j];
 }
 
 void pglFieldGetRegionProfile(PGLField field, int kind, float* out) PGL_CATCH_BEGIN {
@@ -752,8 +745,10 @@ void pglFieldGetRegionProfile(PGLField field, int kind, float* out) PGL_CATCH_BE
   const float toUs = rateKHz > 0 ? 1000.f / (float)rateKHz : 0.01f;
   for (uint32_t r = 0; r < k.numRegions(); r++) {
     const RegionState& S = k.mirrorRegion(r);
-    for (int i = 0; i < 7; i++) out[r * 8 + i] = S.prof[i] * toUs;
-    out[r * 8 + 7] = S.prof[7];
+    for (int i = 0; i < 12; i++) {
+      const bool count = i == 7 || i == 8 || i == 11;
+      out[r * 12 + i] = count ? S.prof[i] : S.prof[i] * toUs;
+    }
   }
 }
 PGL_CATCH_END_VOID

@@ -99,11 +99,13 @@ struct RegionState {
   float estepEvals;     // sample x component evaluations in E-steps of the last update
   float auxEvals;       // sample x component evaluations in distance passes
   float splitEvals;     // sample x component evaluations in split-statistics passes
-  // wall-clock ticks of the last update (rank 0 of the region's team):
-  // [0] total, [1] E-step passes, [2] split passes, [3] distance passes,
-  // [4] team barrier waits, [5] merging, [6] splitting logic, [7] team size
-  float prof[8];
-  uint32_t pad[309];
+  // profile of the last update (rank 0 of the region's team), wall-clock
+  // ticks: [0] total, [1] E-step passes, [2] split passes, [3] distance
+  // passes, [4] team barrier waits, [5] merging, [6] splitting logic,
+  // [9] M-step and convergence test, [10] cross-chunk folds;
+  // counts: [7] team size, [8] E-step passes, [11] samples
+  float prof[12];
+  uint32_t pad[305];
 };
 static_assert(sizeof(RegionState) == 4096, "RegionState must be 4 KiB");
 

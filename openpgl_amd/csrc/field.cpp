@@ -174,6 +174,7 @@ void KindField::reset() {  // Field::resetField (Field.h:222-232)
   nodes_.clear();
   regions_.clear();
   hostStates_.clear();
+  hintRegions_ = 0;
   mirrorDirty_ = true;
 }
 
@@ -548,15 +549,19 @@ void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
                           dDist_.as<float>(), s);
   PGL_HIP_CHECK(hipEventRecord(ev_[3], s));
 
-  // directional update: largest regions first
-  std::stable_sort(work.begin(), work.end(), [](const std::pair<uint32_t, uint32_t>& a, const std::pair<uint32_t, uint32_t>& b) {
-    return a.first > b.first;
-  });
+  // directional update: teams sized by the region's samples.  (The number of
+  // EM iterations a region needs is not predictable from its last update, so
+  // the evaluations per sample recorded in dCostHint_ are kept for reporting
+  // only.)
+  std::vector<RegionWork> rw;
+  rw.reserve(work.size());
+  for (const auto& w : work) rw.push_back(RegionWork{w.second, w.first, (float)w.first});
+  dCostHint_.ensure(std::max<size_t>(R, 1) * 4, true);
   RegionRunner& rr = runner();
-  rr.launch(dStates_.as<RegionState>(), work,
+  rr.launch(dStates_.as<RegionState>(), rw,
             RegionSamplesView{dDx_.as<float>(), dDy_.as<float>(), dDz_.as<float>(), dW_.as<float>(),
                               dPdf_.as<float>(), dDist_.as<float>()},
-            fc_, dUniformDirs_.as<float>(), s);
+            fc_, dUniformDirs_.as<float>(), dCostHint_.as<float>(), s);
   info_.regionLaunches = rr.lastLaunches;
   info_.regionJobs = rr.lastJobs;
   info_.regionItems = rr.lastItems;
@@ -576,8 +581,15 @@ void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
     launch_knn8(tmp.as<float4>(), R, dNbIds_.as<uint32_t>(), dNbPts_.as<float4>(), s);
   }
   PGL_HIP_CHECK(hipEventRecord(ev_[5], s));
+  hCostHint_.ensure(std::max<size_t>(R, 1) * 4);
+  PGL_HIP_CHECK(hipMemcpyAsync(hCostHint_.p, dCostHint_.p, size_t(R) * 4, hipMemcpyDeviceToHost, s));
   PGL_HIP_CHECK(hipStreamSynchronize(s));
+  hintRegions_ = R;
   rr.check();
+  info_.estepEvals = (double)rr.stats().estepEvals;
+  info_.auxEvals = (double)rr.stats().distEvals;
+  info_.splitEvals = (double)rr.stats().splitEvals;
+  info_.emIterations = (double)rr.stats().emIterations;
   for (auto& r : regions_) { r.splitFlag = false; r.copyFrom = -1; }
   initialized_ = true;
   iteration_++;
@@ -588,8 +600,25 @@ void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
   PGL_HIP_CHECK(hipEventElapsedTime(&ms, ev_[1], ev_[2])); info_.sortMs = ms;
   PGL_HIP_CHECK(hipEventElapsedTime(&ms, ev_[2], ev_[3])); info_.gatherMs = ms;
   PGL_HIP_CHECK(hipEventElapsedTime(&ms, ev_[3], ev_[4])); info_.directionalMs = ms;
+  info_.regionKernelMs = rr.kernelMs();
   PGL_HIP_CHECK(hipEventElapsedTime(&ms, ev_[4], ev_[5])); info_.knnMs = ms;
   info_.wallMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void PinnedBuf::ensure(size_t b) {
+  if (b <= bytes) return;
+  void* np = nullptr;
+  PGL_HIP_CHECK(hipHostMalloc(&np, b, hipHostMallocDefault));
+  if (p) {
+    std::memcpy(np, p, bytes);
+    (void)hipHostFree(p);
+  }
+  p = np;
+  bytes = b;
+}
+
+PinnedBuf::~PinnedBuf() {
+  if (p) (void)hipHostFree(p);
 }
 
 RegionRunner& KindField::runner() {

@@ -57,10 +57,19 @@ struct DevBuf {
   template <class T> T* as() const { return (T*)p; }
 };
 
+// Growable pinned host allocation (hipHostMalloc).
+struct PinnedBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void ensure(size_t b);
+  ~PinnedBuf();
+  template <class T> T* as() const { return (T*)p; }
+};
+
 struct UpdateInfo {
   double wallMs = 0, spatialMs = 0, directionalMs = 0, knnMs = 0, gatherMs = 0, sortMs = 0;
   double kdRounds = 0, regionLaunches = 0, estepEvals = 0, auxEvals = 0, emIterations = 0, samples = 0;
-  double regionJobs = 0, regionItems = 0, maxTeam = 0, regionGrid = 0;
+  double regionJobs = 0, regionItems = 0, maxTeam = 0, regionGrid = 0, splitEvals = 0, regionKernelMs = 0;
 };
 
 // One guiding field (surface or volume): Field<> of field/Field.h.
@@ -130,6 +139,9 @@ class KindField {
   DevBuf dAos_;
   DevBuf dNodes_, dFrontier_, dFrontierSlot_, dAcc_, dPartial_, dStatPartial_;
   std::unique_ptr<RegionRunner> runner_;
+  DevBuf dCostHint_;       // per region: evaluations per sample of its last update (written by the region kernel)
+  PinnedBuf hCostHint_;    // host copy, read when the next update plans its teams
+  uint32_t hintRegions_ = 0;
   DevBuf dStates_, dPivots_, dNbIds_, dNbPts_, dUniformDirs_, dCopyPairs_, dInputs_;
   uint32_t stateCapacity_ = 0;
 

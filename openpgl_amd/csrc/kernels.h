@@ -57,15 +57,31 @@ void launch_knn8(const float4* pivots, uint32_t numRegions, uint32_t* nbIds, flo
 // directional
 // Directional update of the regions listed in jobs (persistent kernel, see
 // kernels_region.hip).  items: (job, rank) pairs, ranks of a job consecutive;
-// queueAndCounters: numJobs + 1 words (cleared here); partial: per job
-// 2 x chunks x kPartStride floats at job.partialOffset; err: set to 1 when a
-// team barrier waited longer than timeoutTicks of the wall clock (wall_clock64).  Every job's team must be <= grid.
+// queueAndCounters: numJobs + 1 words (cleared by the launcher); partial: per
+// job 2 x chunks x kPartStride floats at job.partialOffset; err: set to 1 when
+// a team barrier waited longer than timeoutTicks of the wall clock
+// (wall_clock64); stats: E-step, distance and split evaluations and EM
+// iterations are added to stats[0..3]; costHint (optional, per region):
+// evaluations per sample of this update.  Every job's team must be <= grid.
+struct RegionLaunch {
+  RegionState* states;
+  const RegionJob* jobs;
+  uint32_t numJobs;
+  const uint2* items;
+  uint32_t numItems;
+  uint32_t grid;
+  uint32_t* queueAndCounters;
+  float* partial;
+  int* err;
+  unsigned long long timeoutTicks;
+  unsigned long long* stats;
+  float* costHint;
+  const float *dx, *dy, *dz, *w, *pdf, *dist;
+  const float* uniformDirs;
+  bool cooperative;
+};
 int region_kernel_blocks_per_cu();
-void launch_region_update(RegionState* states, const RegionJob* jobs, uint32_t numJobs, const uint2* items,
-                          uint32_t numItems, uint32_t grid, uint32_t* queueAndCounters, float* partial, int* err,
-                          unsigned long long timeoutTicks, const float* dx, const float* dy, const float* dz, const float* w, const float* pdf,
-                          const float* dist, const FactoryConfig& cfg, const float* uniformDirs, bool cooperative,
-                          hipStream_t s);
+void launch_region_update(const RegionLaunch& launch, const FactoryConfig& cfg, hipStream_t s);
 
 void launch_copy_states(RegionState* states, const uint32_t* pairs /* dst, src */, uint32_t numPairs, hipStream_t s);
 void launch_set_inputs(RegionState* states, const RegionInput* inputs, uint32_t count, hipStream_t s);

@@ -286,12 +286,14 @@ __device__ void pass_finish(Shared& sh, Team& tm, int llIdx, float fs, float fc,
   if (tm.T > 1) {
     const uint32_t buf = tm.epoch & 1u;
     team_barrier(sh, tm);
+    const unsigned long long tF = wall_clock64();
     const float* p = tm.part + (size_t)buf * tm.nc * kPartStride + v;
     for (uint32_t c = 0; c < tm.nc; c++) {
       const float x = __hip_atomic_load(p + (size_t)c * kPartStride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (v == llIdx) fd += (double)x;
       else kfold(fs, fc, x);
     }
+    if (v == 0) sh.S.prof[10] += (float)(wall_clock64() - tF);
   }
   sh.fin[v] = fs;
   if (v == llIdx) sh.finLL = fd;
@@ -366,7 +368,11 @@ __device__ void pass_estep(Shared& sh, Team& tm, const RegionSamples& rs, uint32
     __syncthreads();
   }
   pass_finish(sh, tm, kScal, fs, fc, fd);
-  if (tid == 0) { sh.S.estepEvals += (float)n * (float)K; sh.S.prof[1] += (float)(wall_clock64() - tPass); }
+  if (tid == 0) {
+    sh.S.estepEvals += (float)n * (float)K;
+    sh.S.prof[1] += (float)(wall_clock64() - tPass);
+    sh.S.prof[8] += 1.f;
+  }
 }
 
 // Distances pass (WEM:1199-1326).  Result: fin[k] = sum of w g_k (g_k pdf / w_k) / dist,
@@ -594,6 +600,7 @@ __device__ int em(Shared& sh, Team& tm, SuffStats& prev, const RegionSamples& rs
     if (tid == 0) { cur.K = V.K; cur.numSamples = (float)n; cur.sumWeights = 0.f; cur.normalized = 0; }
     __syncthreads();
     pass_estep(sh, tm, rs, base, n);
+    const unsigned long long tM = wall_clock64();
     if (tid < K_ESTEP_COPY) {
       const int k = tid;
       if (k < V.K) {
@@ -678,6 +685,7 @@ __device__ int em(Shared& sh, Team& tm, SuffStats& prev, const RegionSamples& rs
       sh.prevLL = sh.ll;
       sh.invPrevLL = 1.0 / fabs(sh.ll);
     }
+    if (tid == 0) sh.S.prof[9] += (float)(wall_clock64() - tM);
     __syncthreads();
   }
   // previousStats += currentStats (WEM:575-593)
@@ -1046,18 +1054,28 @@ __device__ int perform_merging(Shared& sh, SuffStats& ST, SplitStatsState& SS, c
   const int tid = threadIdx.x;
   VMMParams& V = sh.S.vmm;
   int total = 0;
+  // Pair costs live in sh.cost[i * 32 + j] (i < j).  After a merge only the
+  // pairs of the merged slot ci and of slot cj (which receives the last
+  // component) are recomputed; every other pair has unchanged inputs, so the
+  // table equals a full recomputation bit for bit.
+  int ri = -1, rj = -1;
   while (V.K > sh.W) {
     const int K = V.K;
-    // costs of all pairs i < j < K; the merge candidate is the pair of least
-    // cost below the threshold with both components holding split samples,
-    // ties to the first pair in (i, j) order -- the serial scan of
-    // ThresholdedMergeNext (Merger:218-279) as a (cost, pair index) arg-min
+    for (int p = tid; p < 32 * 32; p += RT) {
+      const int i = p >> 5, j = p & 31;
+      if (i < j && j < K && (ri < 0 || i == ri || j == ri || i == rj || j == rj)) sh.cost[p] = merge_cost(V, i, j);
+    }
+    __syncthreads();
+    // merge candidate: least cost below the threshold with both components
+    // holding split samples, ties to the first pair in (i, j) order -- the
+    // serial scan of ThresholdedMergeNext (Merger:218-279) as an arg-min over
+    // (cost, pair index)
     float best = 3.402823466e38f;
     int bestP = 0x7fffffff;
     for (int p = tid; p < 32 * 32; p += RT) {
       const int i = p >> 5, j = p & 31;
       if (i < j && j < K) {
-        const float cst = merge_cost(V, i, j);
+        const float cst = sh.cost[p];
         if (cst < cfg.mergingThreshold && SS.sn[i] > 0.f && SS.sn[j] > 0.f && (cst < best || (cst == best && p < bestP))) {
           best = cst;
           bestP = p;
@@ -1069,17 +1087,19 @@ __device__ int perform_merging(Shared& sh, SuffStats& ST, SplitStatsState& SS, c
       const int op = __shfl_xor(bestP, off);
       if (ob < best || (ob == best && op < bestP)) { best = ob; bestP = op; }
     }
-    if ((tid & 63) == 0) { sh.cost[tid >> 6] = best; sh.order[tid >> 6] = bestP; }
+    if ((tid & 63) == 0) { sh.scal[tid >> 6][0] = best; sh.order[tid >> 6] = bestP; }
     __syncthreads();
     if (tid == 0) {
-      float minCost = sh.cost[0];
+      float minCost = sh.scal[0][0];
       int pm = sh.order[0];
       for (int w = 1; w < RWAVES; w++)
-        if (sh.cost[w] < minCost || (sh.cost[w] == minCost && sh.order[w] < pm)) { minCost = sh.cost[w]; pm = sh.order[w]; }
+        if (sh.scal[w][0] < minCost || (sh.scal[w][0] == minCost && sh.order[w] < pm)) { minCost = sh.scal[w][0]; pm = sh.order[w]; }
       const int found = pm != 0x7fffffff;
       sh.found = found;
       if (found) {
         const int ci = pm >> 5, cj = pm & 31;
+        sh.ci = ci;
+        sh.cj = cj;
         const float mIx = V.mux[ci], mIy = V.muy[ci], mIz = V.muz[ci], wI = V.w[ci];
         const float mJx = V.mux[cj], mJy = V.muy[cj], mJz = V.muz[cj], wJ = V.w[cj];
         vmm_merge(V, ci, cj);
@@ -1089,6 +1109,8 @@ __device__ int perform_merging(Shared& sh, SuffStats& ST, SplitStatsState& SS, c
     }
     __syncthreads();
     if (!sh.found) break;
+    ri = sh.ci;
+    rj = sh.cj;
     total++;
   }
   if (tid == 0) sh.S.prof[5] += (float)(wall_clock64() - tMerge);
@@ -1260,8 +1282,9 @@ __device__ void region_job(Shared& sh, Team& tm, RegionState* __restrict__ state
   if (tid == 0) {
     S.emIterations = 0; S.partialEmIterations = 0; S.numSplits = 0; S.numMerges = 0;
     S.estepEvals = 0.f; S.auxEvals = 0.f; S.splitEvals = 0.f;
-    for (int i = 0; i < 8; i++) S.prof[i] = 0.f;
+    for (int i = 0; i < 12; i++) S.prof[i] = 0.f;
     S.prof[7] = (float)tm.T;
+    S.prof[11] = (float)S.sampleCount;
     sh.W = cfg.vectorWidth;
   }
   __syncthreads();
@@ -1343,8 +1366,10 @@ __global__ void __launch_bounds__(RT) k_region_update(RegionState* __restrict__ 
                                                       const uint2* __restrict__ items, uint32_t numItems,
                                                       uint32_t* __restrict__ queue, uint32_t* __restrict__ counters,
                                                       float* __restrict__ partial, int* __restrict__ err,
-                                                      unsigned long long timeout, RegionSamples rs, FactoryConfig cfg,
-                                                      const float* __restrict__ uniformDirs) {
+                                                      unsigned long long timeout,
+                                                      unsigned long long* __restrict__ stats,
+                                                      float* __restrict__ costHint, RegionSamples rs,
+                                                      FactoryConfig cfg, const float* __restrict__ uniformDirs) {
   __shared__ Shared sh;
   const int tid = threadIdx.x;
   if (tid == 0) sh.abort = 0;
@@ -1365,6 +1390,17 @@ __global__ void __launch_bounds__(RT) k_region_update(RegionState* __restrict__ 
     tm.err = err;
     tm.timeout = timeout;
     region_job(sh, tm, states, job.region, rs, cfg, uniformDirs);
+    if (tid == 0 && tm.rank == 0) {
+      const RegionState& S = sh.S;
+      atomicAdd(&stats[0], (unsigned long long)S.estepEvals);
+      atomicAdd(&stats[1], (unsigned long long)S.auxEvals);
+      atomicAdd(&stats[2], (unsigned long long)S.splitEvals);
+      atomicAdd(&stats[3], (unsigned long long)(S.emIterations + S.partialEmIterations));
+      if (costHint) {
+        const float n = (float)S.sampleCount;
+        costHint[job.region] = n > 0.f ? (S.estepEvals + S.auxEvals + S.splitEvals) / n : 0.f;
+      }
+    }
     __syncthreads();
   }
 }
@@ -1417,24 +1453,31 @@ int region_kernel_blocks_per_cu() {
   return b;
 }
 
-void launch_region_update(RegionState* states, const RegionJob* jobs, uint32_t numJobs, const uint2* items,
-                          uint32_t numItems, uint32_t grid, uint32_t* queueAndCounters, float* partial, int* err,
-                          unsigned long long timeoutTicks, const float* dx, const float* dy, const float* dz, const float* w, const float* pdf,
-                          const float* dist, const FactoryConfig& cfg, const float* uniformDirs, bool cooperative,
-                          hipStream_t s) {
-  if (numItems == 0) return;
-  const uint32_t nz = numJobs + 1;
-  hipLaunchKernelGGL(k_zero_u32, dim3((nz + 255) / 256), dim3(256), 0, s, queueAndCounters, nz);
-  RegionSamples rs{dx, dy, dz, w, pdf, dist};
-  uint32_t* queue = queueAndCounters;
-  uint32_t* counters = queueAndCounters + 1;
-  if (cooperative) {
-    void* args[] = {&states,  &jobs, &items,        &numItems, &queue,      &counters,
-                    &partial, &err,  &timeoutTicks, &rs,       (void*)&cfg, &uniformDirs};
-    PGL_KERNEL_CHECK(hipLaunchCooperativeKernel((const void*)k_region_update, dim3(grid), dim3(RT), args, 0, s));
+void launch_region_update(const RegionLaunch& L, const FactoryConfig& cfg, hipStream_t s) {
+  if (L.numItems == 0) return;
+  const uint32_t nz = L.numJobs + 1;
+  hipLaunchKernelGGL(k_zero_u32, dim3((nz + 255) / 256), dim3(256), 0, s, L.queueAndCounters, nz);
+  RegionSamples rs{L.dx, L.dy, L.dz, L.w, L.pdf, L.dist};
+  RegionState* states = L.states;
+  const RegionJob* jobs = L.jobs;
+  const uint2* items = L.items;
+  uint32_t numItems = L.numItems;
+  uint32_t* queue = L.queueAndCounters;
+  uint32_t* counters = L.queueAndCounters + 1;
+  float* partial = L.partial;
+  int* err = L.err;
+  unsigned long long timeout = L.timeoutTicks;
+  unsigned long long* stats = L.stats;
+  float* costHint = L.costHint;
+  const float* uniformDirs = L.uniformDirs;
+  FactoryConfig c = cfg;
+  if (L.cooperative) {
+    void* args[] = {&states, &jobs,    &items, &numItems, &queue, &counters,   &partial,
+                    &err,    &timeout, &stats, &costHint, &rs,    (void*)&c, &uniformDirs};
+    PGL_KERNEL_CHECK(hipLaunchCooperativeKernel((const void*)k_region_update, dim3(L.grid), dim3(RT), args, 0, s));
   } else {
-    hipLaunchKernelGGL(k_region_update, dim3(grid), dim3(RT), 0, s, states, jobs, items, numItems, queue, counters,
-                       partial, err, timeoutTicks, rs, cfg, uniformDirs);
+    hipLaunchKernelGGL(k_region_update, dim3(L.grid), dim3(RT), 0, s, states, jobs, items, numItems, queue, counters,
+                       partial, err, timeout, stats, costHint, rs, c, uniformDirs);
   }
 }
 

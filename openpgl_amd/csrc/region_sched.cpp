@@ -2,6 +2,7 @@
 #include "region_sched.h"
 
 #include <algorithm>
+#include <cmath>
 #include <stdexcept>
 #include <string>
 
@@ -39,6 +40,11 @@ RegionRunner::RegionRunner() {
   PGL_SCHED_CHECK(hipMalloc((void**)&dErr_, sizeof(int)));
   PGL_SCHED_CHECK(hipHostMalloc((void**)&hErr_, sizeof(int), hipHostMallocDefault));
   *hErr_ = 0;
+  PGL_SCHED_CHECK(hipMalloc((void**)&dStats_, sizeof(RegionLaunchStats)));
+  PGL_SCHED_CHECK(hipHostMalloc((void**)&hStats_, sizeof(RegionLaunchStats), hipHostMallocDefault));
+  *hStats_ = RegionLaunchStats{0, 0, 0, 0};
+  PGL_SCHED_CHECK(hipEventCreate(&evBegin_));
+  PGL_SCHED_CHECK(hipEventCreate(&evEnd_));
 }
 
 RegionRunner::~RegionRunner() {
@@ -48,35 +54,42 @@ RegionRunner::~RegionRunner() {
   (void)hipFree(dPartial_);
   (void)hipFree(dErr_);
   (void)hipHostFree(hErr_);
+  (void)hipFree(dStats_);
+  (void)hipHostFree(hStats_);
+  if (evBegin_) (void)hipEventDestroy(evBegin_);
+  if (evEnd_) (void)hipEventDestroy(evEnd_);
 }
 
-void RegionRunner::launch(RegionState* states, const std::vector<std::pair<uint32_t, uint32_t>>& work,
-                          const RegionSamplesView& rs, const FactoryConfig& cfg, const float* uniformDirs,
-                          hipStream_t s) {
+void RegionRunner::launch(RegionState* states, std::vector<RegionWork> work, const RegionSamplesView& rs,
+                          const FactoryConfig& cfg, const float* uniformDirs, float* costHint, hipStream_t s) {
   lastLaunches = lastJobs = lastItems = lastMaxTeam = lastGrid = 0;
   *hErr_ = 0;
+  *hStats_ = RegionLaunchStats{0, 0, 0, 0};
+  timed_ = false;
   if (work.empty()) return;
+  std::stable_sort(work.begin(), work.end(), [](const RegionWork& a, const RegionWork& b) { return a.cost > b.cost; });
   const uint32_t J = (uint32_t)work.size();
   const bool perRegion = schedule.perRegionLaunches != 0;
   const uint32_t G = perRegion ? 1u : (schedule.grid ? std::min(schedule.grid, resident_) : resident_);
-  uint64_t total = 0;
-  for (const auto& w : work) total += w.first;
+  double total = 0.0;
+  for (const auto& w : work) total += std::max(w.cost, 0.f);
   const double bal = schedule.balance > 0.f ? schedule.balance : 1.f;
-  const uint64_t minTarget = (uint64_t)kChunk * std::max<uint32_t>(1, schedule.minChunksPerRank);
-  const uint64_t target = std::max<uint64_t>(minTarget, (uint64_t)((double)total / ((double)G * bal)) + 1);
+  const double target = std::max(total / ((double)G * bal), 1e-30);
+  const uint32_t minChunks = std::max<uint32_t>(1, schedule.minChunksPerRank);
   const uint32_t teamCap = std::max<uint32_t>(1, std::min(schedule.maxTeam, G));
 
   jobs_.resize(J);
   items_.clear();
   unsigned long long partialFloats = 0;
   for (uint32_t j = 0; j < J; j++) {
-    const uint32_t n = work[j].first;
+    const uint32_t n = work[j].samples;
     RegionJob& jb = jobs_[j];
-    jb.region = work[j].second;
+    jb.region = work[j].region;
     jb.chunks = std::max<uint32_t>(1, (n + kChunk - 1) / kChunk);
     uint32_t T = 1;
     if (!perRegion) {
-      T = (uint32_t)std::min<uint64_t>((n + target - 1) / target, jb.chunks);
+      const double want = std::ceil(std::max(work[j].cost, 0.f) / target);
+      T = (uint32_t)std::min<double>(std::max(want, 1.0), (double)std::max<uint32_t>(1, jb.chunks / minChunks));
       T = std::max<uint32_t>(1, std::min(T, teamCap));
       // fewest ranks with the same largest number of chunks per rank
       const uint32_t per = (jb.chunks + T - 1) / T;
@@ -99,25 +112,54 @@ void RegionRunner::launch(RegionState* states, const std::vector<std::pair<uint3
   PGL_SCHED_CHECK(hipMemcpyAsync(dJobs_, jobs_.data(), J * sizeof(RegionJob), hipMemcpyHostToDevice, s));
   PGL_SCHED_CHECK(hipMemcpyAsync(dItems_, items_.data(), items_.size() * sizeof(uint2), hipMemcpyHostToDevice, s));
   PGL_SCHED_CHECK(hipMemsetAsync(dErr_, 0, sizeof(int), s));
+  PGL_SCHED_CHECK(hipMemsetAsync(dStats_, 0, sizeof(RegionLaunchStats), s));
 
-  const RegionJob* dj = (const RegionJob*)dJobs_;
-  const uint2* di = (const uint2*)dItems_;
+  RegionLaunch L;
+  L.states = states;
+  L.jobs = (const RegionJob*)dJobs_;
+  L.numJobs = J;
+  L.items = (const uint2*)dItems_;
+  L.numItems = (uint32_t)items_.size();
+  L.grid = std::min<uint32_t>(G, L.numItems);
+  L.queueAndCounters = (uint32_t*)dQueue_;
+  L.partial = (float*)dPartial_;
+  L.err = dErr_;
+  L.timeoutTicks = timeoutTicks_;
+  L.stats = (unsigned long long*)dStats_;
+  L.costHint = costHint;
+  L.dx = rs.dx; L.dy = rs.dy; L.dz = rs.dz; L.w = rs.w; L.pdf = rs.pdf; L.dist = rs.dist;
+  L.uniformDirs = uniformDirs;
+  L.cooperative = cooperative_;
+  PGL_SCHED_CHECK(hipEventRecord(evBegin_, s));
   if (perRegion) {
-    for (uint32_t j = 0; j < J; j++)
-      launch_region_update(states, dj + j, 1, di, 1, 1, (uint32_t*)dQueue_, (float*)dPartial_, dErr_, timeoutTicks_, rs.dx, rs.dy,
-                           rs.dz, rs.w, rs.pdf, rs.dist, cfg, uniformDirs, false, s);
+    L.cooperative = false;
+    L.numJobs = 1;
+    L.numItems = 1;
+    L.grid = 1;
+    for (uint32_t j = 0; j < J; j++) {
+      L.jobs = (const RegionJob*)dJobs_ + j;
+      launch_region_update(L, cfg, s);
+    }
     lastLaunches = J;
     lastGrid = 1;
   } else {
-    const uint32_t grid = std::min<uint32_t>(G, (uint32_t)items_.size());
-    launch_region_update(states, dj, J, di, (uint32_t)items_.size(), grid, (uint32_t*)dQueue_, (float*)dPartial_,
-                         dErr_, timeoutTicks_, rs.dx, rs.dy, rs.dz, rs.w, rs.pdf, rs.dist, cfg, uniformDirs, cooperative_, s);
+    launch_region_update(L, cfg, s);
     lastLaunches = 1;
-    lastGrid = grid;
+    lastGrid = L.grid;
   }
+  PGL_SCHED_CHECK(hipEventRecord(evEnd_, s));
+  timed_ = true;
   lastJobs = J;
   lastItems = (uint32_t)items_.size();
   PGL_SCHED_CHECK(hipMemcpyAsync(hErr_, dErr_, sizeof(int), hipMemcpyDeviceToHost, s));
+  PGL_SCHED_CHECK(hipMemcpyAsync(hStats_, dStats_, sizeof(RegionLaunchStats), hipMemcpyDeviceToHost, s));
+}
+
+float RegionRunner::kernelMs() const {
+  if (!timed_) return 0.f;
+  float ms = 0.f;
+  PGL_SCHED_CHECK(hipEventElapsedTime(&ms, evBegin_, evEnd_));
+  return ms;
 }
 
 void RegionRunner::check() const {

@@ -3,7 +3,6 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
-#include <utility>
 #include <vector>
 
 #include "common.h"
@@ -27,6 +26,19 @@ struct RegionSamplesView {
   const float *dx, *dy, *dz, *w, *pdf, *dist;
 };
 
+// One region of a launch: its samples start at state.sampleBegin; cost is the
+// predicted work (any unit, e.g. samples x evaluations per sample).
+struct RegionWork {
+  uint32_t region;
+  uint32_t samples;
+  float cost;
+};
+
+// Totals of a launch (read after the stream has completed).
+struct RegionLaunchStats {
+  unsigned long long estepEvals, distEvals, splitEvals, emIterations;
+};
+
 class RegionRunner {
  public:
   RegionRunner();
@@ -34,12 +46,17 @@ class RegionRunner {
   RegionRunner(const RegionRunner&) = delete;
   RegionRunner& operator=(const RegionRunner&) = delete;
 
-  // work: (samples, region) with the region's samples at state.sampleBegin
-  // of the sample arrays.  Jobs are ordered by samples, largest first.
-  void launch(RegionState* states, const std::vector<std::pair<uint32_t, uint32_t>>& work,
-              const RegionSamplesView& rs, const FactoryConfig& cfg, const float* uniformDirs, hipStream_t s);
+  // Jobs are ordered by cost, largest first; team sizes follow the cost.
+  // costHint (optional, device, per region) receives the evaluations per
+  // sample of this launch.
+  void launch(RegionState* states, std::vector<RegionWork> work, const RegionSamplesView& rs,
+              const FactoryConfig& cfg, const float* uniformDirs, float* costHint, hipStream_t s);
   // after the stream has completed: throws if a team barrier timed out
   void check() const;
+  const RegionLaunchStats& stats() const { return *hStats_; }
+  // GPU time of the region kernel(s) of the last launch (HIP events around
+  // the kernel launches only), valid after the stream has completed
+  float kernelMs() const;
 
   RegionSchedule schedule;
   uint32_t residentGrid() const { return resident_; }
@@ -60,6 +77,10 @@ class RegionRunner {
   size_t partialBytes_ = 0;
   int* dErr_ = nullptr;
   int* hErr_ = nullptr;
+  RegionLaunchStats* dStats_ = nullptr;
+  RegionLaunchStats* hStats_ = nullptr;
+  hipEvent_t evBegin_ = nullptr, evEnd_ = nullptr;
+  bool timed_ = false;
   std::vector<RegionJob> jobs_;
   std::vector<uint2> items_;
 };
