@@ -129,6 +129,73 @@ def cpu_baseline(frames_host, threads, timed_frames=3):
                       "Updates of frames 1-%d (%d samples) after the frame-0 build; %.2f s" % (timed_frames, n, dt)}
 
 
+def batched_em(num_regions=4096, cpu=True, threads=16):
+    """Batched uneven EM: num_regions pre-partitioned regions, samples per region
+    log-uniform in [64, 65536], initial K in {8, 16, 32}, one fit step in one
+    persistent launch.  The autotuner re-runs the launch under every candidate
+    schedule (results are schedule-independent); the plain per-region path
+    (one launch of one workgroup per region) is timed the same way."""
+    import openpgl_amd as pgl
+    from openpgl_amd.hip_runtime import DeviceArray, synchronize
+    from openpgl_amd.synth import batched_regions
+    t0 = time.perf_counter()
+    recs, counts, stats7, init_k = batched_regions(num_regions, seed=0)
+    log("batched EM: %d regions, %d samples generated in %.1f s" % (num_regions, len(recs), time.perf_counter() - t0))
+    dev = pgl.Device()
+    d = DeviceArray.from_numpy(recs)
+    synchronize()
+    batch = pgl.RegionBatch(dev, max_samples_per_leaf=32000, deterministic=True)
+    batch.step_device(d.value, counts, stats7, init_k)
+    first = batch.last_info()
+    best, table = batch.autotune(None, reps=2)
+    _, per_region = batch.autotune([dict(per_region_launches=1)], reps=1)
+    # leave the tuned schedule selected for reporting
+    batch.set_schedule(**table[best][0])
+    best_ms = table[best][1]
+    default_ms = [ms for sc, ms in table if sc == {"per_region_launches": 0, "grid": 0, "balance": 1.0,
+                                                    "min_chunks_per_rank": 1, "max_team": 64}]
+    n = int(len(recs))
+    flops = FLOPS_ESTEP * first["estep_evals"] + FLOPS_SPLIT * first["split_evals"] + FLOPS_DIST * first["dist_evals"]
+    out = {"workload": "%d regions, samples log-U[64, 65536] (%d total), initial K in {8,16,32}, fit step"
+                       % (num_regions, n),
+           "samples": n, "batched_ms": best_ms, "batched_samples_per_s": n / (best_ms * 1e-3),
+           "default_schedule_ms": default_ms[0] if default_ms else None,
+           "per_region_path_ms": per_region[0][1],
+           "speedup_vs_per_region": per_region[0][1] / best_ms,
+           "tuned_schedule": table[best][0],
+           "autotune_top5": sorted([(ms, sc) for sc, ms in table], key=lambda x: x[0])[:5],
+           "achieved_tflops": flops / (best_ms * 1e-3) / 1e12,
+           "em_iterations": first["em_iterations"]}
+    if cpu:
+        out["cpu"] = batched_cpu_baseline(recs, counts, stats7, init_k, threads)
+        out["speedup_vs_cpu"] = out["batched_samples_per_s"] / out["cpu"]["value"]
+    return out
+
+
+def batched_cpu_baseline(recs, counts, stats7, init_k, threads, sample_regions=256):
+    """Oracle region fits (std library threads via ctypes, GIL released) on
+    every (num_regions / sample_regions)-th region."""
+    import concurrent.futures
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_ctypes import OracleRegion, default_config
+    offs = np.concatenate([[0], np.cumsum(counts.astype(np.int64))])
+    step = max(1, len(counts) // sample_regions)
+    regions = list(range(0, len(counts), step))
+    jobs = []
+    for r in regions:
+        cfg = default_config(32000)
+        cfg.initK = int(init_k[r])
+        jobs.append((OracleRegion(cfg), recs[offs[r]:offs[r + 1]], stats7[r]))
+    n = int(sum(len(j[1]) for j in jobs))
+    t0 = time.perf_counter()
+    with concurrent.futures.ThreadPoolExecutor(max_workers=threads) as ex:
+        list(ex.map(lambda j: j[0].step(j[1], j[2][0:3], j[2][3:6], float(j[2][6])), jobs))
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "samples/s", "cores": threads, "kind": "port",
+            "sample": "oracle region fits of %d of the regions (every %d-th, %d samples), %d threads, %.2f s"
+                      % (len(regions), step, n, threads, dt)}
+
+
 def read_traffic():
     """HBM bytes per region-kernel launch from the committed rocprofv3 summary."""
     path = os.path.join(ROOT, "profiles", "k_region_update_traffic.txt")
@@ -167,6 +234,7 @@ def main():
     ap.add_argument("--samples", type=int, default=N_PER_FRAME)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched uneven EM workload")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (rocprofv3 --pmc calibration)")
     args = ap.parse_args()
@@ -221,6 +289,8 @@ def main():
     }
     if not args.no_serving:
         out["serving"] = serving(field)
+    if not args.no_batched:
+        out["batched_em"] = batched_em(cpu=not args.no_cpu, threads=min(os.cpu_count() or 1, 16))
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
         out["cpu_baseline"] = cpu_baseline(frames, threads, timed_frames=min(3, nframes - 1))

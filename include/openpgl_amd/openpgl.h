@@ -297,6 +297,41 @@ void pglFieldSetRegionSchedule(PGLField field, const PGLRegionSchedule* schedule
 void pglFieldGetRegionProfile(PGLField field, int kind, float* out);
 void pglFieldGetRegionSchedule(PGLField field, PGLRegionSchedule* schedule);
 
+/* ------------------------------------------------------------------------
+ * EXTENSION: batched directional training of pre-partitioned regions.
+ * The Field::fitRegions / updateRegions step (field/Field.h:389-511) for a
+ * caller-given set of regions, without a spatial structure, in one persistent
+ * GPU launch.  Arguments as for pglDeviceNewField (the sub-structs are
+ * deleted; only the directional arguments are used).
+ *   pglRegionBatchStep: deviceRecords = region-contiguous PGLSampleData in
+ *   device memory (region r starts at the sum of counts[0..r-1]);
+ *   sampleStats7 = per region the float sample statistics of the positions
+ *   (mean[3], variance * n[3], n), host memory; initK (optional) = initial
+ *   components per region (0 = the arguments' initK); splitFlags
+ *   (optional). The first step fits, later steps update.
+ *   pglRegionBatchAutotune re-runs the last step's launch from its input
+ *   state for each candidate schedule (NULL = the built-in set from
+ *   pglRegionScheduleCandidates, where grid == 1 means one workgroup per CU),
+ *   keeps the fastest of reps runs per candidate, selects the fastest
+ *   candidate and returns its index (-1 on error); msOut receives the kernel
+ *   time of every candidate.  Schedules never change results.
+ * ------------------------------------------------------------------------ */
+typedef struct PGLRegionBatchImpl* PGLRegionBatch;
+PGLRegionBatch pglNewRegionBatch(PGLDevice device, PGLFieldArguments args);
+void pglReleaseRegionBatch(PGLRegionBatch batch);
+void pglRegionBatchStep(PGLRegionBatch batch, uint32_t numRegions, const PGLSampleData* deviceRecords,
+                        const uint32_t* counts, const float* sampleStats7, const uint32_t* initK,
+                        const int* splitFlags);
+uint32_t pglRegionBatchGetNumRegions(PGLRegionBatch batch);
+void pglRegionBatchExportRegion(PGLRegionBatch batch, uint32_t region, PGLRegionRecord* out);
+void pglRegionBatchSetSchedule(PGLRegionBatch batch, const PGLRegionSchedule* schedule);
+void pglRegionBatchGetSchedule(PGLRegionBatch batch, PGLRegionSchedule* schedule);
+size_t pglRegionScheduleCandidates(PGLRegionSchedule* out, size_t maxCount);
+int pglRegionBatchAutotune(PGLRegionBatch batch, const PGLRegionSchedule* candidates, size_t numCandidates, int reps,
+                           float* msOut);
+/* same layout as pglFieldGetLastUpdateInfo ([1], [3]-[6] are 0) */
+void pglRegionBatchGetLastInfo(PGLRegionBatch batch, double* out, size_t count);
+
 #ifdef __cplusplus
 }  /* extern "C" */
 #endif

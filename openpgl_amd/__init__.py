@@ -151,6 +151,16 @@ _SIGS = {
     "pglFieldSetRegionSchedule": (None, [_vp, _vp]),
     "pglFieldGetRegionSchedule": (None, [_vp, _vp]),
     "pglFieldGetRegionProfile": (None, [_vp, ctypes.c_int, _vp]),
+    "pglNewRegionBatch": (_vp, [_vp, PGLFieldArguments]),
+    "pglReleaseRegionBatch": (None, [_vp]),
+    "pglRegionBatchStep": (None, [_vp, _u32, _vp, _vp, _vp, _vp, _vp]),
+    "pglRegionBatchGetNumRegions": (_u32, [_vp]),
+    "pglRegionBatchExportRegion": (None, [_vp, _u32, ctypes.POINTER(PGLRegionRecord)]),
+    "pglRegionBatchSetSchedule": (None, [_vp, _vp]),
+    "pglRegionBatchGetSchedule": (None, [_vp, _vp]),
+    "pglRegionScheduleCandidates": (ctypes.c_size_t, [_vp, ctypes.c_size_t]),
+    "pglRegionBatchAutotune": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t, ctypes.c_int, _vp]),
+    "pglRegionBatchGetLastInfo": (None, [_vp, _vp, ctypes.c_size_t]),
 }
 
 
@@ -222,25 +232,111 @@ class SampleStorage:
             self.h = None
 
 
+def _field_arguments(lib, max_samples_per_leaf, deterministic, knn=True, overrides=None):
+    args = PGLFieldArguments()
+    lib.pglFieldArgumentsSetDefaults(ctypes.byref(args), PGL_SPATIAL_STRUCTURE_KDTREE,
+                                     PGL_DIRECTIONAL_DISTRIBUTION_PARALLAX_AWARE_VMM, deterministic,
+                                     max_samples_per_leaf)
+    kd = ctypes.cast(args.spatialSturctureArguments, ctypes.POINTER(PGLKDTreeArguments)).contents
+    kd.knnLookup = knn
+    if overrides:
+        v = ctypes.cast(args.directionalDistributionArguments, ctypes.POINTER(PGLVMMFactoryArguments)).contents
+        for k, val in overrides.items():
+            if hasattr(kd, k):
+                setattr(kd, k, val)
+            else:
+                setattr(v, k, val)
+    return args
+
+
+_INFO_KEYS = ["wall_ms", "spatial_ms", "directional_ms", "knn_ms", "gather_ms", "sort_ms", "kd_rounds",
+              "region_launches", "estep_evals", "dist_evals", "em_iterations", "samples", "split_evals", "regions",
+              "region_jobs", "queue_items", "max_team", "grid", "region_kernel_ms"]
+
+
+def _schedule_dict(sc):
+    return {"per_region_launches": sc.perRegionLaunches, "grid": sc.grid, "balance": sc.balance,
+            "min_chunks_per_rank": sc.minChunksPerRank, "max_team": sc.maxTeam}
+
+
+def _schedule_struct(d):
+    return RegionSchedule(int(d.get("per_region_launches", 0)), int(d.get("grid", 0)), float(d.get("balance", 1.0)),
+                          int(d.get("min_chunks_per_rank", 1)), int(d.get("max_team", 64)))
+
+
+class RegionBatch:
+    """Batched directional training of pre-partitioned regions (pglRegionBatch*)."""
+
+    def __init__(self, device, max_samples_per_leaf=32000, deterministic=True, overrides=None):
+        self.lib = load_library()
+        self.device = device
+        args = _field_arguments(self.lib, max_samples_per_leaf, deterministic, True, overrides)
+        self.h = self.lib.pglNewRegionBatch(device.h, args)
+        if not self.h:
+            raise RuntimeError("pglNewRegionBatch failed")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.pglReleaseRegionBatch(self.h)
+            self.h = None
+
+    def step_device(self, device_ptr, counts, stats7, init_k=None, split_flags=None):
+        """device_ptr: region-contiguous PGLSampleData in device memory."""
+        counts = np.ascontiguousarray(counts, np.uint32)
+        stats7 = np.ascontiguousarray(stats7, np.float32).reshape(-1, 7)
+        ik = None if init_k is None else np.ascontiguousarray(init_k, np.uint32)
+        sf = None if split_flags is None else np.ascontiguousarray(split_flags, np.int32)
+        self.lib.pglRegionBatchStep(self.h, len(counts), ctypes.c_void_p(device_ptr), _ptr(counts), _ptr(stats7),
+                                    _ptr(ik), _ptr(sf))
+
+    def num_regions(self):
+        return self.lib.pglRegionBatchGetNumRegions(self.h)
+
+    def region(self, idx):
+        r = PGLRegionRecord()
+        self.lib.pglRegionBatchExportRegion(self.h, idx, ctypes.byref(r))
+        return r.to_dict()
+
+    def set_schedule(self, **kw):
+        sc = RegionSchedule()
+        self.lib.pglRegionBatchGetSchedule(self.h, ctypes.byref(sc))
+        d = _schedule_dict(sc)
+        d.update(kw)
+        self.lib.pglRegionBatchSetSchedule(self.h, ctypes.byref(_schedule_struct(d)))
+
+    def schedule(self):
+        sc = RegionSchedule()
+        self.lib.pglRegionBatchGetSchedule(self.h, ctypes.byref(sc))
+        return _schedule_dict(sc)
+
+    def autotune(self, candidates=None, reps=2):
+        """Returns (best index, [(schedule dict, kernel ms)])."""
+        if candidates is None:
+            n = self.lib.pglRegionScheduleCandidates(None, 0)
+            arr = (RegionSchedule * n)()
+            self.lib.pglRegionScheduleCandidates(arr, n)
+        else:
+            n = len(candidates)
+            arr = (RegionSchedule * n)(*[_schedule_struct(c) for c in candidates])
+        ms = np.zeros(n, np.float32)
+        best = self.lib.pglRegionBatchAutotune(self.h, arr, n, reps, _ptr(ms))
+        if best < 0:
+            raise RuntimeError("pglRegionBatchAutotune failed")
+        return best, [(_schedule_dict(arr[i]), float(ms[i])) for i in range(n)]
+
+    def last_info(self):
+        o = np.zeros(19, np.float64)
+        self.lib.pglRegionBatchGetLastInfo(self.h, _ptr(o), 19)
+        return dict(zip(_INFO_KEYS, o.tolist()))
+
+
 class Field:
     """openpgl::cpp::Field over the MI355X backend."""
 
     def __init__(self, device, max_samples_per_leaf=32000, deterministic=True, knn=True, overrides=None):
         self.lib = load_library()
         self.device = device
-        args = PGLFieldArguments()
-        self.lib.pglFieldArgumentsSetDefaults(ctypes.byref(args), PGL_SPATIAL_STRUCTURE_KDTREE,
-                                              PGL_DIRECTIONAL_DISTRIBUTION_PARALLAX_AWARE_VMM, deterministic,
-                                              max_samples_per_leaf)
-        kd = ctypes.cast(args.spatialSturctureArguments, ctypes.POINTER(PGLKDTreeArguments)).contents
-        kd.knnLookup = knn
-        if overrides:
-            v = ctypes.cast(args.directionalDistributionArguments, ctypes.POINTER(PGLVMMFactoryArguments)).contents
-            for k, val in overrides.items():
-                if hasattr(kd, k):
-                    setattr(kd, k, val)
-                else:
-                    setattr(v, k, val)
+        args = _field_arguments(self.lib, max_samples_per_leaf, deterministic, knn, overrides)
         self.h = self.lib.pglDeviceNewField(device.h, args)
         if not self.h:
             raise RuntimeError("pglDeviceNewField failed")
@@ -338,10 +434,7 @@ class Field:
     def last_update_info(self, kind=0):
         o = np.zeros(19, np.float64)
         self.lib.pglFieldGetLastUpdateInfo(self.h, kind, _ptr(o), 19)
-        keys = ["wall_ms", "spatial_ms", "directional_ms", "knn_ms", "gather_ms", "sort_ms", "kd_rounds",
-                "region_launches", "estep_evals", "dist_evals", "em_iterations", "samples", "split_evals", "regions",
-                "region_jobs", "queue_items", "max_team", "grid", "region_kernel_ms"]
-        return dict(zip(keys, o.tolist()))
+        return dict(zip(_INFO_KEYS, o.tolist()))
 
     # --- queries
     def sample_batch(self, pos, nrm, u2, u1=None, g=None, is_volume=None):

@@ -25,6 +25,8 @@ SOURCES = [
                    "-Xclang", "-packed-fp32-ops"]),
     ("region_sched.cpp", ["-x", "hip", "--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang",
                           "-packed-fp32-ops"]),
+    ("region_batch.cpp", ["-x", "hip", "--offload-arch=" + ARCH, "-ffp-contract=off", "-Xclang", "-target-feature",
+                          "-Xclang", "-packed-fp32-ops"]),
     ("api.cpp", ["-x", "hip", "--offload-arch=" + ARCH, "-ffp-contract=off", "-Xclang", "-target-feature",
                  "-Xclang", "-packed-fp32-ops"]),
 ]

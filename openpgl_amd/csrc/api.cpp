@@ -17,6 +17,7 @@
 
 #include "../../include/openpgl_amd/openpgl.h"
 #include "field.h"
+#include "region_batch.h"
 
 using namespace pglgpu;
 
@@ -237,19 +238,7 @@ void pglReleaseDevice(PGLDevice device) PGL_CATCH_BEGIN {
 }
 PGL_CATCH_END_VOID
 
-PGLField pglDeviceNewField(PGLDevice device, PGLFieldArguments args) PGL_CATCH_BEGIN {
-  THROW_IF_NULL(device, "handle");
-  if (args.directionalDistributionType != PGL_DIRECTIONAL_DISTRIBUTION_PARALLAX_AWARE_VMM)
-    throw std::runtime_error("openpgl_amd: only the parallax-aware VMM field is implemented");
-  PGLKDTreeArguments* kd = (PGLKDTreeArguments*)args.spatialSturctureArguments;
-  PGLVMMFactoryArguments* v = (PGLVMMFactoryArguments*)args.directionalDistributionArguments;
-  THROW_IF_NULL(kd, "spatial structure arguments");
-  THROW_IF_NULL(v, "directional distribution arguments");
-  KdSettings ks;
-  ks.knnLookup = kd->knnLookup;
-  ks.minSamples = (uint32_t)kd->minSamples;
-  ks.maxSamples = (uint32_t)kd->maxSamples;
-  ks.maxDepth = (uint32_t)kd->maxDepth;
+static FactoryConfig factory_config(const PGLVMMFactoryArguments* v, int vectorWidth) {
   FactoryConfig fc{};
   fc.initK = (int)v->initK;
   fc.initKappa = v->initKappa;
@@ -268,8 +257,25 @@ PGLField pglDeviceNewField(PGLDevice device, PGLFieldArguments args) PGL_CATCH_B
   fc.minSamplesForMerging = v->minSamplesForMerging;
   fc.decayOnSpatialSplit = 0.25f;
   fc.maxMeanCosineSplit = 1.f / std::tanh(32000.0f) - 1.f / 32000.0f;
-  fc.vectorWidth = device->vectorWidth;
+  fc.vectorWidth = vectorWidth;
   if (fc.initK < 1 || fc.initK > 32) throw std::runtime_error("openpgl_amd: initK must be in [1, 32]");
+  return fc;
+}
+
+PGLField pglDeviceNewField(PGLDevice device, PGLFieldArguments args) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(device, "handle");
+  if (args.directionalDistributionType != PGL_DIRECTIONAL_DISTRIBUTION_PARALLAX_AWARE_VMM)
+    throw std::runtime_error("openpgl_amd: only the parallax-aware VMM field is implemented");
+  PGLKDTreeArguments* kd = (PGLKDTreeArguments*)args.spatialSturctureArguments;
+  PGLVMMFactoryArguments* v = (PGLVMMFactoryArguments*)args.directionalDistributionArguments;
+  THROW_IF_NULL(kd, "spatial structure arguments");
+  THROW_IF_NULL(v, "directional distribution arguments");
+  KdSettings ks;
+  ks.knnLookup = kd->knnLookup;
+  ks.minSamples = (uint32_t)kd->minSamples;
+  ks.maxSamples = (uint32_t)kd->maxSamples;
+  ks.maxDepth = (uint32_t)kd->maxDepth;
+  const FactoryConfig fc = factory_config(v, device->vectorWidth);
   // the reference deletes the argument structs it was handed (Device.h:109, 136)
   delete kd;
   delete v;
@@ -776,6 +782,137 @@ void pglFieldGetRegionSchedule(PGLField field, PGLRegionSchedule* sc) PGL_CATCH_
   sc->balance = r.balance;
   sc->minChunksPerRank = r.minChunksPerRank;
   sc->maxTeam = r.maxTeam;
+}
+PGL_CATCH_END_VOID
+
+
+/* ------------------------------------------------------------------ batched regions (EXTENSION) */
+
+struct PGLRegionBatchImpl {
+  RegionBatch* b = nullptr;
+  hipStream_t stream = nullptr;
+};
+
+static void to_sched(const PGLRegionSchedule* sc, RegionSchedule& r) {
+  r.perRegionLaunches = sc->perRegionLaunches;
+  r.grid = sc->grid;
+  r.balance = sc->balance;
+  r.minChunksPerRank = sc->minChunksPerRank;
+  r.maxTeam = sc->maxTeam;
+}
+static void from_sched(const RegionSchedule& r, PGLRegionSchedule* sc) {
+  sc->perRegionLaunches = r.perRegionLaunches;
+  sc->grid = r.grid;
+  sc->balance = r.balance;
+  sc->minChunksPerRank = r.minChunksPerRank;
+  sc->maxTeam = r.maxTeam;
+}
+
+PGLRegionBatch pglNewRegionBatch(PGLDevice device, PGLFieldArguments args) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(device, "handle");
+  if (args.directionalDistributionType != PGL_DIRECTIONAL_DISTRIBUTION_PARALLAX_AWARE_VMM)
+    throw std::runtime_error("openpgl_amd: only the parallax-aware VMM is implemented");
+  PGLKDTreeArguments* kd = (PGLKDTreeArguments*)args.spatialSturctureArguments;
+  PGLVMMFactoryArguments* v = (PGLVMMFactoryArguments*)args.directionalDistributionArguments;
+  THROW_IF_NULL(v, "directional distribution arguments");
+  const FactoryConfig fc = factory_config(v, device->vectorWidth);
+  delete kd;
+  delete v;
+  PGLRegionBatchImpl* B = new PGLRegionBatchImpl();
+  PGL_HIP_CHECK(hipStreamCreate(&B->stream));
+  B->b = new RegionBatch(fc);
+  return B;
+}
+PGL_CATCH_END(nullptr)
+
+void pglReleaseRegionBatch(PGLRegionBatch batch) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(batch, "handle");
+  delete batch->b;
+  (void)hipStreamDestroy(batch->stream);
+  delete batch;
+}
+PGL_CATCH_END_VOID
+
+void pglRegionBatchStep(PGLRegionBatch batch, uint32_t numRegions, const PGLSampleData* deviceRecords,
+                        const uint32_t* counts, const float* sampleStats7, const uint32_t* initK,
+                        const int* splitFlags) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(batch, "handle");
+  batch->b->step(numRegions, deviceRecords, counts, sampleStats7, initK, splitFlags, batch->stream);
+}
+PGL_CATCH_END_VOID
+
+uint32_t pglRegionBatchGetNumRegions(PGLRegionBatch batch) { return batch->b->numRegions(); }
+
+void pglRegionBatchExportRegion(PGLRegionBatch batch, uint32_t region, PGLRegionRecord* out) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(batch, "handle");
+  THROW_IF_NULL(out, "record");
+  static_assert(sizeof(PGLRegionRecord) == sizeof(PGLRegionRecordHost), "record layouts differ");
+  batch->b->exportRegion(region, (PGLRegionRecordHost*)out, batch->stream);
+}
+PGL_CATCH_END_VOID
+
+void pglRegionBatchSetSchedule(PGLRegionBatch batch, const PGLRegionSchedule* sc) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(batch, "handle");
+  THROW_IF_NULL(sc, "schedule");
+  to_sched(sc, batch->b->runner().schedule);
+}
+PGL_CATCH_END_VOID
+
+void pglRegionBatchGetSchedule(PGLRegionBatch batch, PGLRegionSchedule* sc) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(batch, "handle");
+  THROW_IF_NULL(sc, "schedule");
+  from_sched(batch->b->runner().schedule, sc);
+}
+PGL_CATCH_END_VOID
+
+size_t pglRegionScheduleCandidates(PGLRegionSchedule* out, size_t maxCount) {
+  // search space of the autotuner: grid (all co-resident workgroups or one per
+  // CU), team balance, chunks per rank, largest team
+  std::vector<PGLRegionSchedule> c;
+  for (uint32_t gridHalf = 0; gridHalf < 2; gridHalf++)
+    for (float bal : {0.5f, 1.f, 2.f, 4.f})
+      for (uint32_t mc : {1u, 2u, 4u})
+        for (uint32_t mt : {16u, 64u}) c.push_back(PGLRegionSchedule{0, gridHalf ? 1u : 0u, bal, mc, mt});
+  if (out)
+    for (size_t i = 0; i < c.size() && i < maxCount; i++) out[i] = c[i];
+  return c.size();
+}
+
+int pglRegionBatchAutotune(PGLRegionBatch batch, const PGLRegionSchedule* candidates, size_t numCandidates, int reps,
+                           float* msOut) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(batch, "handle");
+  std::vector<PGLRegionSchedule> cs;
+  if (candidates) {
+    cs.assign(candidates, candidates + numCandidates);
+  } else {
+    cs.resize(pglRegionScheduleCandidates(nullptr, 0));
+    pglRegionScheduleCandidates(cs.data(), cs.size());
+  }
+  // grid == 1 in a candidate means "one workgroup per CU"
+  const uint32_t resident = batch->b->runner().residentGrid();
+  int cus = 0, dev = 0;
+  PGL_HIP_CHECK(hipGetDevice(&dev));
+  PGL_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  std::vector<RegionSchedule> rs(cs.size());
+  for (size_t i = 0; i < cs.size(); i++) {
+    to_sched(&cs[i], rs[i]);
+    if (rs[i].grid == 1) rs[i].grid = std::min<uint32_t>(resident, (uint32_t)cus);
+  }
+  std::vector<float> ms;
+  const int best = batch->b->autotune(rs, reps, ms, batch->stream);
+  if (msOut)
+    for (size_t i = 0; i < ms.size(); i++) msOut[i] = ms[i];
+  return best;
+}
+PGL_CATCH_END(-1)
+
+void pglRegionBatchGetLastInfo(PGLRegionBatch batch, double* out, size_t count) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(batch, "handle");
+  const UpdateInfo& i = batch->b->lastInfo();
+  double o[19] = {i.wallMs, 0.0, i.directionalMs, 0.0, 0.0, 0.0, 0.0, i.regionLaunches, i.estepEvals, i.auxEvals,
+                  i.emIterations, i.samples, i.splitEvals, (double)batch->b->numRegions(), i.regionJobs,
+                  i.regionItems, i.maxTeam, i.regionGrid, i.regionKernelMs};
+  for (size_t j = 0; j < count && j < 19; j++) out[j] = o[j];
 }
 PGL_CATCH_END_VOID
 

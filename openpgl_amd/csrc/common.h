@@ -105,7 +105,8 @@ struct RegionState {
   // [9] M-step and convergence test, [10] cross-chunk folds;
   // counts: [7] team size, [8] E-step passes, [11] samples
   float prof[12];
-  uint32_t pad[305];
+  int32_t initK;        // components of the initial mixture of a fit (0: the factory's initK)
+  uint32_t pad[304];
 };
 static_assert(sizeof(RegionState) == 4096, "RegionState must be 4 KiB");
 
@@ -126,6 +127,7 @@ struct RegionInput {
   float ssMean[3], ssVar[3], ssN;
   uint32_t begin, count;
   int32_t mode, splitFlag;
+  int32_t initK;        // 0: the factory's initK
 };
 
 // One region of a directional launch: team = workgroups working on it,

@@ -125,7 +125,12 @@ void DevBuf::release() {
 }
 
 // ------------------------------------------------------------ KindField
-static void uniform_directions(std::vector<float>& out) {  // WEM:668-700 spherical Fibonacci
+float reprojection_min_distance(const float var[3]) {  // WEM:1184-1196, |.| with fused dot
+  const float l = std::sqrt(std::fmaf(var[0], var[0], std::fmaf(var[1], var[1], var[2] * var[2])));
+  return 3.f * 3.f * std::sqrt(l);
+}
+
+void uniform_directions(std::vector<float>& out) {  // WEM:668-700 spherical Fibonacci
   out.assign(32 * 32 * 3, 0.f);
   const float gr = 1.618033988749895f;
   for (int l = 0; l < 32; l++)
@@ -524,13 +529,11 @@ void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
     in.count = rg.count;
     in.mode = first ? 1 : 2;
     in.splitFlag = rg.splitFlag ? 1 : 0;
+    in.initK = 0;
     begin += rg.count;
-    // prepareSamples minimum distance (WEM:1184-1196): 9 sqrt(|variance|), |.| with fused dot
     float var[3];
     rg.ss.variance(var);
-    float md = std::sqrt(std::fmaf(var[0], var[0], std::fmaf(var[1], var[1], var[2] * var[2])));
-    md = 3.f * 3.f * std::sqrt(md);
-    pivots[r] = make_float4(rg.ss.mean[0], rg.ss.mean[1], rg.ss.mean[2], md);
+    pivots[r] = make_float4(rg.ss.mean[0], rg.ss.mean[1], rg.ss.mean[2], reprojection_min_distance(var));
     if (first || rg.count > 0 || rg.splitFlag) work.push_back({rg.count, r});
   }
   if (begin != n) throw std::runtime_error("kd update lost samples");
@@ -709,9 +712,7 @@ void KindField::canonicalOrder(std::vector<KDNode>& out, std::vector<uint32_t>& 
   }
 }
 
-void KindField::exportRegion(uint32_t r, PGLRegionRecordHost* o) const {
-  const RegionState& S = hostStates_[r];
-  const RegionHost& h = regions_[r];
+void export_state(const RegionState& S, PGLRegionRecordHost* o) {
   memset(o, 0, sizeof(*o));
   o->K = S.vmm.K;
   for (int k = 0; k < 32; k++) {
@@ -732,15 +733,22 @@ void KindField::exportRegion(uint32_t r, PGLRegionRecordHost* o) const {
   o->suffK = S.suff.K; o->normalized = S.suff.normalized; o->splitK = S.split.K;
   o->numSamplesAfterLastSplit = ((uint64_t)S.afterSplitHi << 32) | S.afterSplitLo;
   o->numSamplesAfterLastMerge = ((uint64_t)S.afterMergeHi << 32) | S.afterMergeLo;
+  o->valid = S.valid;
+  o->numSamplesThisUpdate = S.sampleCount;
+  o->emIterations = S.emIterations; o->partialEmIterations = S.partialEmIterations;
+  o->numSplits = S.numSplits; o->numMerges = S.numMerges;
+}
+
+void KindField::exportRegion(uint32_t r, PGLRegionRecordHost* o) const {
+  const RegionState& S = hostStates_[r];
+  const RegionHost& h = regions_[r];
+  export_state(S, o);
   for (int j = 0; j < 3; j++) {
     o->ssMean[j] = h.ss.mean[j]; o->ssVariance[j] = h.ss.var[j]; o->ssLower[j] = h.ss.lo[j]; o->ssUpper[j] = h.ss.hi[j];
     o->regionLower[j] = h.lo[j]; o->regionUpper[j] = h.hi[j];
   }
   o->ssNumSamples = h.ss.n;
-  o->splitFlag = h.splitFlag; o->valid = S.valid;
-  o->numSamplesThisUpdate = S.sampleCount;
-  o->emIterations = S.emIterations; o->partialEmIterations = S.partialEmIterations;
-  o->numSplits = S.numSplits; o->numMerges = S.numMerges;
+  o->splitFlag = h.splitFlag;
 }
 
 void KindField::knnHost(std::vector<uint32_t>& ids) const { ids = hostNbIds_; }

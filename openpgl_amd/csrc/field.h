@@ -186,5 +186,11 @@ class GuidingField {
 };
 
 const HGRep* host_hg_table();
+// spherical-Fibonacci initial means (InitUniformVMM, WEM:632-666): 32 x 32 x 3 floats
+void uniform_directions(std::vector<float>& out);
+// directional part of a region record
+void export_state(const RegionState& S, PGLRegionRecordHost* out);
+// minimum reprojection distance of prepareSamples (WEM:1184-1196): 9 sqrt(|variance|) with a fused dot
+float reprojection_min_distance(const float var[3]);
 
 }  // namespace pglgpu

@@ -49,6 +49,9 @@ size_t sort_pairs_temp_bytes(uint32_t n);
 void sort_pairs(void* temp, size_t tempBytes, const uint32_t* keysIn, uint32_t* keysOut, const uint32_t* valsIn,
                 uint32_t* valsOut, uint32_t n, int endBit, hipStream_t s);
 void launch_region_ranges(const uint32_t* sortedKeys, uint32_t n, uint32_t* begin, uint32_t* end, hipStream_t s);
+// keys/values of region-contiguous records (begin: numRegions ascending offsets)
+void launch_region_keys(const uint32_t* begin, uint32_t numRegions, uint32_t n, uint32_t* keys, uint32_t* vals,
+                        hipStream_t s);
 void launch_gather_reproject(const void* aos, const uint32_t* sortedKeys, const uint32_t* sortedIdx, uint32_t n,
                              const float4* regionPivot, float* dx, float* dy, float* dz, float* w, float* pdf,
                              float* dist, hipStream_t s);

@@ -1151,7 +1151,7 @@ __device__ void region_fit(Shared& sh, Team& tm, const RegionSamples& rs, uint32
                            const FactoryConfig& cfg, const float* __restrict__ uniformDirs) {
   const int tid = threadIdx.x;
   VMMParams& V = sh.S.vmm;
-  const int K0 = cfg.initK;
+  const int K0 = sh.S.initK > 0 ? min(sh.S.initK, kMaxComponents) : cfg.initK;
   clear_suff(sh, sh.S.suff, K0);
   clear_split(sh, sh.S.split, K0);
   // InitUniformVMM (WEM:632-666): spherical Fibonacci means for K0 components
@@ -1428,6 +1428,7 @@ __global__ void __launch_bounds__(256) k_set_inputs(RegionState* __restrict__ st
   S.sampleCount = v.count;
   S.mode = v.mode;
   S.splitFlag = v.splitFlag;
+  S.initK = v.initK;
   S.emIterations = 0; S.partialEmIterations = 0; S.numSplits = 0; S.numMerges = 0;
   S.estepEvals = 0.f; S.auxEvals = 0.f; S.splitEvals = 0.f;
 }

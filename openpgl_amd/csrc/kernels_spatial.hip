@@ -448,6 +448,27 @@ void launch_region_ranges(const uint32_t* sortedKeys, uint32_t n, uint32_t* begi
   hipLaunchKernelGGL(k_region_ranges, dim3(grid_for(n, 256)), dim3(256), 0, s, sortedKeys, n, begin, end);
 }
 
+// Keys of region-contiguous records: key[i] = region r with begin[r] <= i < begin[r+1], value[i] = i.
+__global__ void __launch_bounds__(256) k_region_keys(const uint32_t* __restrict__ begin, uint32_t numRegions,
+                                                     uint32_t n, uint32_t* __restrict__ keys,
+                                                     uint32_t* __restrict__ vals) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t lo = 0, hi = numRegions;  // last r with begin[r] <= i
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (begin[mid] <= i) lo = mid; else hi = mid;
+  }
+  keys[i] = lo;
+  vals[i] = i;
+}
+
+void launch_region_keys(const uint32_t* begin, uint32_t numRegions, uint32_t n, uint32_t* keys, uint32_t* vals,
+                        hipStream_t s) {
+  if (n == 0 || numRegions == 0) return;
+  hipLaunchKernelGGL(k_region_keys, dim3((n + 255) / 256), dim3(256), 0, s, begin, numRegions, n, keys, vals);
+}
+
 void launch_gather_reproject(const void* aos, const uint32_t* sortedKeys, const uint32_t* sortedIdx, uint32_t n,
                              const float4* regionPivot, float* dx, float* dy, float* dz, float* w, float* pdf,
                              float* dist, hipStream_t s) {

@@ -111,3 +111,58 @@ def queries(n, seed=0, scene_seed=0, num_blobs=16, volume_fraction=0.3):
     is_vol = rng.random(n) < volume_fraction
     g = rng.uniform(-0.9, 0.9, size=n).astype(np.float32)
     return pos, nrm, u2, is_vol, g
+
+
+def _region_block(counts, mu, kappa, seed, step, block):
+    """Records of consecutive regions (one RNG stream per block of regions)."""
+    rng = np.random.Generator(np.random.PCG64([seed, 778, step, block]))
+    total = int(counts.sum())
+    region = np.repeat(np.arange(len(counts)), counts)
+    lobe = rng.integers(0, 4, size=total)
+    out = np.zeros(total, dtype=SAMPLE_DTYPE)
+    out["pos"] = rng.normal(scale=0.05, size=(total, 3)).astype(np.float32)
+    out["dir"] = sample_vmf(rng, mu[region, lobe], kappa[region, lobe]).astype(np.float32)
+    out["weight"] = np.exp(rng.normal(size=total)).astype(np.float32)
+    out["pdf"] = np.float32(1.0 / (4.0 * np.pi))
+    out["distance"] = rng.exponential(1.0, size=total).astype(np.float32)
+    stats = np.zeros((len(counts), 7), np.float32)
+    off = 0
+    for r, n in enumerate(counts):
+        n = int(n)
+        if n:
+            p = out["pos"][off:off + n].astype(np.float64)
+            m = p.mean(axis=0)
+            stats[r, 0:3] = m
+            stats[r, 3:6] = ((p - m) ** 2).sum(axis=0)
+        stats[r, 6] = n
+        off += n
+    return out, stats
+
+
+def batched_regions(num_regions=4096, seed=0, nmin=64, nmax=65536, ks=(8, 16, 32), step=0, threads=None):
+    """Pre-partitioned regions for batched directional training.  Region r has
+    floor(exp(U[ln nmin, ln nmax])) samples (fixed per region across steps),
+    an initial component count drawn from ks, positions in a 0.05-sigma blob
+    and directions from its own 4-lobe vMF mixture (kappa log-U[5, 500]);
+    weights lognormal(0, 1), pdf 1/(4 pi), distance Exp(1).  step selects an
+    independent draw of new samples from the same mixtures.  Generated in
+    blocks of 64 regions (one random stream each, so the result does not
+    depend on the thread count).  Returns (records region-contiguous,
+    counts u32, stats7 f32 (mean[3], variance * n[3], n) of the positions,
+    initK u32)."""
+    import concurrent.futures
+    import os
+    rng = np.random.Generator(np.random.PCG64([seed, 777]))
+    counts = np.floor(np.exp(rng.uniform(np.log(nmin), np.log(nmax), size=num_regions))).astype(np.uint32)
+    init_k = rng.choice(np.asarray(ks, np.uint32), size=num_regions)
+    mu = _unit_vectors(rng, num_regions * 4).reshape(num_regions, 4, 3)
+    kappa = np.exp(rng.uniform(np.log(5.0), np.log(500.0), size=(num_regions, 4)))
+    B = 64
+    blocks = [(b, slice(b * B, min(num_regions, (b + 1) * B))) for b in range((num_regions + B - 1) // B)]
+    workers = threads or min(16, os.cpu_count() or 1)
+    with concurrent.futures.ThreadPoolExecutor(max_workers=workers) as ex:
+        parts = list(ex.map(lambda bs: _region_block(counts[bs[1]], mu[bs[1]], kappa[bs[1]], seed, step, bs[0]),
+                            blocks))
+    records = np.concatenate([p[0] for p in parts])
+    stats7 = np.concatenate([p[1] for p in parts])
+    return records, counts, stats7, init_k
