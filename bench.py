@@ -110,6 +110,90 @@ def serving(field, n_batches=40, seed=0):
             "valid_fraction": float(np.mean(pdf[:sizes[-1]] > 0)), "mode": "closed-loop, device-resident inputs"}
 
 
+def serving_field():
+    """Field of the query-serving workload: one Update with 200k surface records
+    (the single-update generator configuration) and 200k volume records."""
+    import openpgl_amd as pgl
+    from openpgl_amd.synth import generate
+    dev = pgl.Device()
+    f = pgl.Field(dev, max_samples_per_leaf=32000, deterministic=True)
+    recs = np.concatenate([generate(200000, seed=0), generate(200000, seed=1, flags=2)])
+    f.update(pgl.SampleStorage(recs))
+    return dev, f
+
+
+def serving_poisson(field, offered_qps, duration_s=1.5, seed=0, pool=1 << 22, ring=1 << 23, slots=4):
+    """Open-loop serving: requests of floor(exp(U[ln 1k, ln 256k])) queries arrive
+    as a Poisson process at offered_qps / E[size] requests/s; each query is a
+    position near a blob, a normal uniform on the sphere, 2 uniforms, no KNN
+    random (u1 = -1); 30% are volume queries with g uniform in [-0.9, 0.9].
+    Inputs are slices of a pinned pool, outputs go to a pinned ring.
+    Latency = submission (arrival) -> results in host memory, per request."""
+    import openpgl_amd as pgl
+    from openpgl_amd.hip_runtime import pinned_copy, pinned_empty
+    from openpgl_amd.synth import queries
+    rng = np.random.default_rng(seed)
+    pos, nrm, u2, is_vol, g = queries(pool, seed=seed + 11)
+    pos, nrm, u2, g = (pinned_copy(a) for a in (pos, nrm, u2, g))
+    vol = pinned_copy(is_vol.astype(np.uint8))
+    odir = pinned_empty((ring, 3), np.float32)
+    opdf = pinned_empty(ring, np.float32)
+    oid = pinned_empty(ring, np.uint32)
+    oint = pinned_empty(ring, np.float32)
+    mean_size = (256000 - 1000) / np.log(256.0)
+    rate = offered_qps / mean_size
+    count = max(8, int(duration_s * rate))
+    sizes = np.floor(np.exp(rng.uniform(np.log(1000), np.log(256000), count))).astype(np.int64)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, count))
+    starts = rng.integers(0, pool - sizes)
+    server = pgl.Server(field, memory_budget=1 << 30, slots=slots)
+    # warm-up request (arena allocation)
+    rid, _ = server.submit(pos[:256000], nrm[:256000], u2[:256000], g=g[:256000], is_volume=vol[:256000],
+                           out=(odir[:256000], opdf[:256000], oid[:256000], oint[:256000]))
+    server.wait(rid)
+    server.release(rid)
+    ids, live = [], []
+    rpos = 0
+    t0 = time.perf_counter()
+    for i in range(count):
+        n = int(sizes[i])
+        if rpos + n > ring:
+            rpos = 0
+        # the ring slice must not be in use by an unfinished request (slices are
+        # taken in order, so the oldest live request is the next one to overlap)
+        while live and live[0][0] < rpos + n and live[0][1] > rpos:
+            server.wait(live[0][2])
+            live.pop(0)
+        target = t0 + arrivals[i]
+        while True:
+            now = time.perf_counter()
+            if now >= target:
+                break
+            if target - now > 2e-4:
+                time.sleep((target - now) - 1e-4)
+        a = int(starts[i])
+        sl = slice(a, a + n)
+        o = slice(rpos, rpos + n)
+        rid, _ = server.submit(pos[sl], nrm[sl], u2[sl], g=g[sl], is_volume=vol[sl],
+                               out=(odir[o], opdf[o], oid[o], oint[o]))
+        ids.append(rid)
+        live.append((rpos, rpos + n, rid))
+        rpos += n
+    server.drain()
+    t = np.array([server.timing(r) for r in ids])
+    lat = t[:, 2] - t[:, 0]
+    wall = (t[:, 2].max() - t[:, 0].min()) * 1e-3
+    st = server.stats()
+    valid = float(np.mean(opdf[:min(ring, rpos)] > 0))
+    server.close()
+    return {"offered_queries_per_s": offered_qps, "achieved_guided_samples_per_s": float(sizes.sum() / wall),
+            "requests": int(count), "queries": int(sizes.sum()), "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()),
+            "mean_batch_queries": st["queries"] / max(1.0, st["batches"]), "batches": int(st["batches"]),
+            "peak_pending_requests": int(st["peak_pending"]), "valid_fraction": valid,
+            "mode": "open loop, Poisson arrivals, continuous batching, host pinned in/out, %d slots" % slots}
+
+
 def cpu_baseline(frames_host, threads, timed_frames=3):
     """Oracle on a bounded sample: frame 0 (build, untimed), then the incremental
     updates of frames 1..timed_frames timed."""
@@ -289,6 +373,8 @@ def main():
     }
     if not args.no_serving:
         out["serving"] = serving(field)
+        sdev, sfield = serving_field()
+        out["serving_poisson"] = [serving_poisson(sfield, q) for q in (1.0e8, 2.0e8)]
     if not args.no_batched:
         out["batched_em"] = batched_em(cpu=not args.no_cpu, threads=min(os.cpu_count() or 1, 16))
     if not args.no_cpu:

@@ -332,6 +332,48 @@ int pglRegionBatchAutotune(PGLRegionBatch batch, const PGLRegionSchedule* candid
 /* same layout as pglFieldGetLastUpdateInfo ([1], [3]-[6] are 0) */
 void pglRegionBatchGetLastInfo(PGLRegionBatch batch, double* out, size_t count);
 
+/* ------------------------------------------------------------------------
+ * EXTENSION: continuous-batching query server.
+ * A request is n queries (the batched Init + cosine/HG product + sample +
+ * pdf of pglFieldSampleBatch) with host pointers that stay valid until the
+ * request is done; pinned host memory (hipHostMalloc) gives asynchronous
+ * copies.  The device-memory budget is split into one arena per in-flight
+ * slot (68 bytes per query), allocated at creation.  Requests are queued as
+ * pieces of at most 32768 queries; whenever a slot is free, the pending
+ * pieces are admitted in arrival order while its arena has room and run as
+ * one batch, so batches on different slots overlap on the device and large
+ * requests pipeline across slots.  Each query's result depends only on its own inputs,
+ * so a request's results are the same alone or in any mix.  The server
+ * serves a snapshot of the field taken at creation and by pglServerRefresh
+ * (call it from the thread that updates the field, after the update), so the
+ * field may be updated while requests are served.
+ *   pglServerSubmit returns the request id (0 if rejected: larger than the
+ *   budget); pglServerWait returns 1 when done, 0 on
+ *   timeout (timeoutMs < 0 waits forever); pglServerGetTiming: milliseconds
+ *   since server creation of submit, admission (batch issue) and completion
+ *   (results in host memory); pglServerGetStats: [0] requests done,
+ *   [1] queries done, [2] batches, [3] peak in-flight bytes, [4] peak pending
+ *   requests, [5] largest batch.
+ * ------------------------------------------------------------------------ */
+typedef struct {
+  size_t memoryBudgetBytes; /* 0: 512 MiB */
+  uint32_t slots;           /* 0: 4 */
+  uint32_t maxBatchQueries; /* 0: 2^20 */
+} PGLServerConfig;
+typedef struct PGLServerImpl* PGLServer;
+PGLServer pglNewServer(PGLField field, const PGLServerConfig* config);
+void pglReleaseServer(PGLServer server);
+uint64_t pglServerSubmit(PGLServer server, size_t n, const float* pos, const float* normalOrDir, const float* g,
+                         const float* u2, const float* u1, const uint8_t* isVolume, float* dirOut, float* pdfOut,
+                         uint32_t* idOut, float* integralOut);
+int pglServerPoll(PGLServer server, uint64_t request);
+int pglServerWait(PGLServer server, uint64_t request, double timeoutMs);
+void pglServerDrain(PGLServer server);
+int pglServerGetTiming(PGLServer server, uint64_t request, double* out3);
+void pglServerForget(PGLServer server, uint64_t request);
+void pglServerRefresh(PGLServer server);
+void pglServerGetStats(PGLServer server, double* out, size_t count);
+
 #ifdef __cplusplus
 }  /* extern "C" */
 #endif

@@ -161,6 +161,16 @@ _SIGS = {
     "pglRegionScheduleCandidates": (ctypes.c_size_t, [_vp, ctypes.c_size_t]),
     "pglRegionBatchAutotune": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t, ctypes.c_int, _vp]),
     "pglRegionBatchGetLastInfo": (None, [_vp, _vp, ctypes.c_size_t]),
+    "pglNewServer": (_vp, [_vp, _vp]),
+    "pglReleaseServer": (None, [_vp]),
+    "pglServerSubmit": (ctypes.c_uint64, [_vp, ctypes.c_size_t, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "pglServerPoll": (ctypes.c_int, [_vp, ctypes.c_uint64]),
+    "pglServerWait": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.c_double]),
+    "pglServerDrain": (None, [_vp]),
+    "pglServerGetTiming": (ctypes.c_int, [_vp, ctypes.c_uint64, _vp]),
+    "pglServerForget": (None, [_vp, ctypes.c_uint64]),
+    "pglServerRefresh": (None, [_vp]),
+    "pglServerGetStats": (None, [_vp, _vp, ctypes.c_size_t]),
 }
 
 
@@ -262,6 +272,80 @@ def _schedule_dict(sc):
 def _schedule_struct(d):
     return RegionSchedule(int(d.get("per_region_launches", 0)), int(d.get("grid", 0)), float(d.get("balance", 1.0)),
                           int(d.get("min_chunks_per_rank", 1)), int(d.get("max_team", 64)))
+
+
+class ServerConfig(ctypes.Structure):
+    """PGLServerConfig (include/openpgl_amd/openpgl.h)."""
+    _fields_ = [("memoryBudgetBytes", ctypes.c_size_t), ("slots", ctypes.c_uint32),
+                ("maxBatchQueries", ctypes.c_uint32)]
+
+
+class Server:
+    """Continuous-batching query server (pglServer*).  Arrays passed to submit
+    must stay alive (and unchanged) until the request is done; pinned arrays
+    (hip_runtime.pinned_empty) give asynchronous copies."""
+
+    def __init__(self, field, memory_budget=0, slots=0, max_batch=0):
+        self.lib = load_library()
+        self.field = field
+        cfg = ServerConfig(memory_budget, slots, max_batch)
+        self.h = self.lib.pglNewServer(field.h, ctypes.byref(cfg))
+        if not self.h:
+            raise RuntimeError("pglNewServer failed")
+        self._live = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.pglReleaseServer(self.h)
+            self.h = None
+            self._live.clear()
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, pos, nrm, u2, u1=None, g=None, is_volume=None, out=None):
+        """Returns (request id, (dir, pdf, ids, integral) output arrays)."""
+        n = len(pos)
+        if out is None:
+            out = (np.zeros((n, 3), np.float32), np.zeros(n, np.float32), np.zeros(n, np.uint32),
+                   np.zeros(n, np.float32))
+        rid = self.lib.pglServerSubmit(self.h, n, _ptr(pos), _ptr(nrm), _ptr(g), _ptr(u2), _ptr(u1),
+                                       _ptr(is_volume), _ptr(out[0]), _ptr(out[1]), _ptr(out[2]), _ptr(out[3]))
+        if rid == 0:
+            raise RuntimeError("pglServerSubmit rejected the request")
+        self._live[rid] = (pos, nrm, u2, u1, g, is_volume, out)
+        return rid, out
+
+    def wait(self, rid, timeout_ms=-1.0):
+        ok = self.lib.pglServerWait(self.h, rid, timeout_ms)
+        if ok < 0:
+            raise RuntimeError("pglServerWait failed")
+        return ok == 1
+
+    def poll(self, rid):
+        return self.lib.pglServerPoll(self.h, rid) == 1
+
+    def timing(self, rid):
+        t = np.zeros(3, np.float64)
+        if self.lib.pglServerGetTiming(self.h, rid, _ptr(t)) != 1:
+            return None
+        return t
+
+    def release(self, rid):
+        self.lib.pglServerForget(self.h, rid)
+        self._live.pop(rid, None)
+
+    def drain(self):
+        self.lib.pglServerDrain(self.h)
+
+    def refresh(self):
+        self.lib.pglServerRefresh(self.h)
+
+    def stats(self):
+        o = np.zeros(6, np.float64)
+        self.lib.pglServerGetStats(self.h, _ptr(o), 6)
+        return dict(zip(["requests", "queries", "batches", "peak_inflight_bytes", "peak_pending", "max_batch"],
+                        o.tolist()))
 
 
 class RegionBatch:

@@ -18,6 +18,7 @@
 #include "../../include/openpgl_amd/openpgl.h"
 #include "field.h"
 #include "region_batch.h"
+#include "server.h"
 
 using namespace pglgpu;
 
@@ -913,6 +914,87 @@ void pglRegionBatchGetLastInfo(PGLRegionBatch batch, double* out, size_t count) 
                   i.emIterations, i.samples, i.splitEvals, (double)batch->b->numRegions(), i.regionJobs,
                   i.regionItems, i.maxTeam, i.regionGrid, i.regionKernelMs};
   for (size_t j = 0; j < count && j < 19; j++) out[j] = o[j];
+}
+PGL_CATCH_END_VOID
+
+
+/* ------------------------------------------------------------------ query server (EXTENSION) */
+
+struct PGLServerImpl {
+  QueryServer* s = nullptr;
+};
+
+PGLServer pglNewServer(PGLField field, const PGLServerConfig* cfg) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(field, "handle");
+  ServerConfig c;
+  if (cfg) {
+    if (cfg->memoryBudgetBytes) c.memoryBudget = cfg->memoryBudgetBytes;
+    if (cfg->slots) c.slots = cfg->slots;
+    if (cfg->maxBatchQueries) c.maxBatchQueries = cfg->maxBatchQueries;
+  }
+  PGLServerImpl* S = new PGLServerImpl();
+  S->s = new QueryServer(field->gf, c);
+  return S;
+}
+PGL_CATCH_END(nullptr)
+
+void pglReleaseServer(PGLServer server) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(server, "handle");
+  delete server->s;
+  delete server;
+}
+PGL_CATCH_END_VOID
+
+uint64_t pglServerSubmit(PGLServer server, size_t n, const float* pos, const float* normalOrDir, const float* g,
+                         const float* u2, const float* u1, const uint8_t* isVolume, float* dirOut, float* pdfOut,
+                         uint32_t* idOut, float* integralOut) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(server, "handle");
+  QueryRequest r;
+  r.n = n; r.pos = pos; r.nrm = normalOrDir; r.g = g; r.u2 = u2; r.u1 = u1; r.isVolume = isVolume;
+  r.dir = dirOut; r.pdf = pdfOut; r.id = idOut; r.integral = integralOut;
+  return server->s->submit(r);
+}
+PGL_CATCH_END(0)
+
+int pglServerPoll(PGLServer server, uint64_t request) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(server, "handle");
+  return server->s->poll(request) ? 1 : 0;
+}
+PGL_CATCH_END(-1)
+
+int pglServerWait(PGLServer server, uint64_t request, double timeoutMs) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(server, "handle");
+  return server->s->wait(request, timeoutMs) ? 1 : 0;
+}
+PGL_CATCH_END(-1)
+
+void pglServerDrain(PGLServer server) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(server, "handle");
+  server->s->drain();
+}
+PGL_CATCH_END_VOID
+
+int pglServerGetTiming(PGLServer server, uint64_t request, double* out3) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(server, "handle");
+  return server->s->timing(request, out3[0], out3[1], out3[2]) ? 1 : 0;
+}
+PGL_CATCH_END(-1)
+
+void pglServerForget(PGLServer server, uint64_t request) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(server, "handle");
+  server->s->forget(request);
+}
+PGL_CATCH_END_VOID
+
+void pglServerRefresh(PGLServer server) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(server, "handle");
+  server->s->refresh();
+}
+PGL_CATCH_END_VOID
+
+void pglServerGetStats(PGLServer server, double* out, size_t count) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(server, "handle");
+  server->s->stats(out, count);
 }
 PGL_CATCH_END_VOID
 

@@ -90,6 +90,18 @@ void launch_copy_states(RegionState* states, const uint32_t* pairs /* dst, src *
 void launch_set_inputs(RegionState* states, const RegionInput* inputs, uint32_t count, hipStream_t s);
 
 // query
+// One piece of a query batch: queries [begin, begin + n) of the batch read
+// their inputs and write their results through these pointers (device or
+// device-accessible pinned host memory); g, u1, vol, id, integral may be null.
+struct QueryPiece {
+  const float *pos, *nrm, *g, *u2, *u1;
+  const uint8_t* vol;
+  float *dir, *pdf, *integral;
+  uint32_t* id;
+  uint32_t begin, n;
+};
+void launch_query_pieces(const QueryField& surf, const QueryField& vol, const HGRep* hg, const QueryPiece* pieces,
+                         uint32_t numPieces, uint32_t n, hipStream_t s);
 void launch_query(const QueryField& surf, const QueryField& vol, const HGRep* hg, uint32_t n, const float* pos,
                   const float* nrm, const float* g, const float* u2, const float* u1, const uint8_t* isVolume,
                   float* dirOut, float* pdfOut, uint32_t* idOut, float* integralOut, hipStream_t s);

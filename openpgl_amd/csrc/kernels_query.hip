@@ -89,42 +89,31 @@ __device__ float mixture_pdf(const VMMParams& V, float sx, float sy, float sz, f
   return pdf;
 }
 
-__global__ void __launch_bounds__(256) k_query(QueryField surf, QueryField vol, const HGRep* __restrict__ hg,
-                                               uint32_t n, const float* __restrict__ pos, const float* __restrict__ nrm,
-                                               const float* __restrict__ gArr, const float* __restrict__ u2,
-                                               const float* __restrict__ u1, const uint8_t* __restrict__ isVolume,
-                                               float* __restrict__ dirOut, float* __restrict__ pdfOut,
-                                               uint32_t* __restrict__ idOut, float* __restrict__ integralOut) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const bool volume = isVolume ? isVolume[i] != 0 : false;
+// One query: lookup (+ KNN pick), parallax shift, cosine (surface) or 3-lobe
+// HG (volume) product, one sample and its pdf.  Invalid queries (outside the
+// field, untrained field, invalid region) give dir 0, pdf 0, id ~0.
+__device__ __forceinline__ void query_one(const QueryField& surf, const QueryField& vol, const HGRep* __restrict__ hg,
+                                          bool volume, float x, float y, float z, float nx, float ny, float nz,
+                                          float g, float ux, float uy, float u, float out[3], float& pdf,
+                                          uint32_t& id, float& integral) {
   const QueryField& f = volume ? vol : surf;
-  const float x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
-  float u = u1 ? u1[i] : -1.f;
-  uint32_t id = 0xffffffffu;
+  id = 0xffffffffu;
+  pdf = 0.f;
+  integral = 0.f;
+  out[0] = out[1] = out[2] = 0.f;
   const int r = lookup_region(f, x, y, z, &u, id);
   if (r < 0 || !f.regions[r].valid) {
-    dirOut[3 * i] = 0.f;
-    dirOut[3 * i + 1] = 0.f;
-    dirOut[3 * i + 2] = 0.f;
-    pdfOut[i] = 0.f;
-    if (idOut) idOut[i] = 0xffffffffu;
-    if (integralOut) integralOut[i] = 0.f;
+    id = 0xffffffffu;
     return;
   }
   const VMMParams& V = f.regions[r].vmm;
   const float sx = V.pivot[0] - x, sy = V.pivot[1] - y, sz = V.pivot[2] - z;
-  const float nx = nrm[3 * i], ny = nrm[3 * i + 1], nz = nrm[3 * i + 2];
-  const float ux = u2[2 * i], uy = u2[2 * i + 1];
-  float out[3];
-  float pdf = 0.f, integral = 0.f;
   if (!volume) {
     const float kn = kCosineKappa, nn = cosine_normalization();
     integral = mixture_integral(V, sx, sy, sz, 1.f, nx, ny, nz, kn, nn);
     mixture_sample(V, sx, sy, sz, 1.f, nx, ny, nz, kn, nn, integral, ux, uy, out);
     pdf = mixture_pdf(V, sx, sy, sz, 1.f, nx, ny, nz, kn, nn, integral, out[0], out[1], out[2]);
   } else {
-    const float g = gArr ? gArr[i] : 0.f;
     const HGRep& rep = hg[hg_index(g)];
     const int KP = (int)rep.K;
     float integ[4], sgn[4];
@@ -150,12 +139,62 @@ __global__ void __launch_bounds__(256) k_query(QueryField surf, QueryField vol, 
                                  rep.norm[j], integ[j], out[0], out[1], out[2]);
     integral = sum;
   }
+}
+
+__global__ void __launch_bounds__(256) k_query(QueryField surf, QueryField vol, const HGRep* __restrict__ hg,
+                                               uint32_t n, const float* __restrict__ pos, const float* __restrict__ nrm,
+                                               const float* __restrict__ gArr, const float* __restrict__ u2,
+                                               const float* __restrict__ u1, const uint8_t* __restrict__ isVolume,
+                                               float* __restrict__ dirOut, float* __restrict__ pdfOut,
+                                               uint32_t* __restrict__ idOut, float* __restrict__ integralOut) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const bool volume = isVolume ? isVolume[i] != 0 : false;
+  float out[3], pdf, integral;
+  uint32_t id;
+  query_one(surf, vol, hg, volume, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], nrm[3 * i], nrm[3 * i + 1],
+            nrm[3 * i + 2], gArr ? gArr[i] : 0.f, u2[2 * i], u2[2 * i + 1], u1 ? u1[i] : -1.f, out, pdf, id, integral);
   dirOut[3 * i] = out[0];
   dirOut[3 * i + 1] = out[1];
   dirOut[3 * i + 2] = out[2];
   pdfOut[i] = pdf;
   if (idOut) idOut[i] = id;
   if (integralOut) integralOut[i] = integral;
+}
+
+// Batch of pieces: thread i of the batch finds its piece (pieces ordered by
+// begin) and works on query i - begin of it through the piece's pointers.
+__global__ void __launch_bounds__(256) k_query_pieces(QueryField surf, QueryField vol, const HGRep* __restrict__ hg,
+                                                      const QueryPiece* __restrict__ pieces, uint32_t numPieces,
+                                                      uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t lo = 0, hi = numPieces;  // last piece with begin <= i
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (pieces[mid].begin <= i) lo = mid; else hi = mid;
+  }
+  const QueryPiece& P = pieces[lo];
+  const uint32_t j = i - P.begin;
+  if (j >= P.n) return;
+  const bool volume = P.vol ? P.vol[j] != 0 : false;
+  float out[3], pdf, integral;
+  uint32_t id;
+  query_one(surf, vol, hg, volume, P.pos[3 * j], P.pos[3 * j + 1], P.pos[3 * j + 2], P.nrm[3 * j], P.nrm[3 * j + 1],
+            P.nrm[3 * j + 2], P.g ? P.g[j] : 0.f, P.u2[2 * j], P.u2[2 * j + 1], P.u1 ? P.u1[j] : -1.f, out, pdf, id,
+            integral);
+  P.dir[3 * j] = out[0];
+  P.dir[3 * j + 1] = out[1];
+  P.dir[3 * j + 2] = out[2];
+  P.pdf[j] = pdf;
+  if (P.id) P.id[j] = id;
+  if (P.integral) P.integral[j] = integral;
+}
+
+void launch_query_pieces(const QueryField& surf, const QueryField& vol, const HGRep* hg, const QueryPiece* pieces,
+                         uint32_t numPieces, uint32_t n, hipStream_t s) {
+  if (n == 0 || numPieces == 0) return;
+  hipLaunchKernelGGL(k_query_pieces, dim3((n + 255) / 256), dim3(256), 0, s, surf, vol, hg, pieces, numPieces, n);
 }
 
 void launch_query(const QueryField& surf, const QueryField& vol, const HGRep* hg, uint32_t n, const float* pos,

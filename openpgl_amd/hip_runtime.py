@@ -78,3 +78,37 @@ class DeviceArray:
 
 def synchronize():
     check(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+
+class _PinnedBlock:
+    """Owner of a hipHostMalloc allocation (freed with the last numpy view)."""
+
+    def __init__(self, nbytes):
+        self.ptr = ctypes.c_void_p()
+        check(hip().hipHostMalloc(ctypes.byref(self.ptr), max(1, nbytes), 0), "hipHostMalloc")
+        self.nbytes = nbytes
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                hip().hipHostFree(self.ptr)
+        except Exception:
+            pass
+
+
+def pinned_empty(shape, dtype):
+    """numpy array in pinned host memory (asynchronous DMA source/target); the
+    allocation lives as long as the array."""
+    dtype = np.dtype(dtype)
+    count = int(np.prod(shape))
+    block = _PinnedBlock(count * dtype.itemsize)
+    buf = (ctypes.c_char * max(1, count * dtype.itemsize)).from_address(block.ptr.value)
+    buf._owner = block
+    return np.frombuffer(buf, dtype=dtype, count=count).reshape(shape)
+
+
+def pinned_copy(a):
+    a = np.ascontiguousarray(a)
+    out = pinned_empty(a.shape, a.dtype)
+    out[...] = a
+    return out
