@@ -194,6 +194,126 @@ def serving_poisson(field, offered_qps, duration_s=1.5, seed=0, pool=1 << 22, ri
             "mode": "open loop, Poisson arrivals, continuous batching, host pinned in/out, %d slots" % slots}
 
 
+def _device_mem_used():
+    import ctypes
+    from openpgl_amd.hip_runtime import hip
+    free, total = ctypes.c_size_t(), ctypes.c_size_t()
+    hip().hipMemGetInfo(ctypes.byref(free), ctypes.byref(total))
+    return total.value - free.value
+
+
+def mixed_workload(frames=3, surf=16_000_000, vol=4_000_000, offered_qps=1.0e8, server_budget=1 << 30):
+    """Large mixed workload: 16M surface + 4M volume records per frame (64 blobs),
+    frame 0 builds the field; frames 1.. are timed (a) alone and (b) while the
+    query server runs an open Poisson load of offered_qps on the previous
+    frame's field (refreshed after every update).  The region kernel keeps one
+    workgroup per CU in (b) so that query waves fit beside it."""
+    import threading
+    import openpgl_amd as pgl
+    from openpgl_amd.hip_runtime import DeviceArray, pinned_copy, pinned_empty, synchronize
+    from openpgl_amd.synth import generate_large, queries
+    t0 = time.perf_counter()
+    fs = [generate_large(surf, seed=0, frame=f, num_blobs=64) for f in range(frames)]
+    fv = [generate_large(vol, seed=1, frame=f, num_blobs=64, flags=2) for f in range(frames)]
+    log("mixed workload: %d frames of %d + %d records generated in %.1f s" % (frames, surf, vol,
+                                                                                time.perf_counter() - t0))
+    dev = pgl.Device()
+    mem0 = _device_mem_used()
+    ds = [DeviceArray.from_numpy(a) for a in fs]
+    dv = [DeviceArray.from_numpy(a) for a in fv]
+    del fs, fv
+    synchronize()
+    mem_inputs = _device_mem_used() - mem0
+    out = {"workload": "%d surface + %d volume records/frame, 64 blobs, %d frames (frame 0 builds)"
+                       % (surf, vol, frames)}
+    # (a) alone
+    fa = pgl.Field(dev, max_samples_per_leaf=32000, deterministic=True)
+    fa.update_surface_device(ds[0].value, surf)
+    fa.update_volume_device(dv[0].value, vol)
+    synchronize()
+    t = time.perf_counter()
+    for f in range(1, frames):
+        fa.update_surface_device(ds[f].value, surf)
+        fa.update_volume_device(dv[f].value, vol)
+    synchronize()
+    alone_s = time.perf_counter() - t
+    mem_field = _device_mem_used() - mem0 - mem_inputs
+    regions = (fa.num_regions(0), fa.num_regions(1))
+    del fa
+    # (b) with serving
+    fb = pgl.Field(dev, max_samples_per_leaf=32000, deterministic=True)
+    fb.set_region_schedule(grid=256)
+    fb.update_surface_device(ds[0].value, surf)
+    fb.update_volume_device(dv[0].value, vol)
+    synchronize()
+    server = pgl.Server(fb, memory_budget=server_budget, slots=4)
+    pool = 1 << 22
+    pos, nrm, u2, is_vol, g = queries(pool, seed=5, num_blobs=64)
+    pos, nrm, u2, g = (pinned_copy(a) for a in (pos, nrm, u2, g))
+    volf = pinned_copy(is_vol.astype(np.uint8))
+    ring = 1 << 23
+    odir, opdf = pinned_empty((ring, 3), np.float32), pinned_empty(ring, np.float32)
+    oid, oint = pinned_empty(ring, np.uint32), pinned_empty(ring, np.float32)
+    rng = np.random.default_rng(3)
+    mean_size = (256000 - 1000) / np.log(256.0)
+    rate = offered_qps / mean_size
+    stop = threading.Event()
+    ids = []
+
+    def load():
+        rpos = 0
+        live = []
+        tn = time.perf_counter()
+        while not stop.is_set():
+            tn += rng.exponential(1.0 / rate)
+            n = int(np.floor(np.exp(rng.uniform(np.log(1000), np.log(256000)))))
+            if rpos + n > ring:
+                rpos = 0
+            while live and live[0][0] < rpos + n and live[0][1] > rpos:
+                server.wait(live[0][2])
+                live.pop(0)
+            while time.perf_counter() < tn and not stop.is_set():
+                time.sleep(1e-4)
+            a = int(rng.integers(0, pool - n))
+            rid, _ = server.submit(pos[a:a + n], nrm[a:a + n], u2[a:a + n], g=g[a:a + n], is_volume=volf[a:a + n],
+                                   out=(odir[rpos:rpos + n], opdf[rpos:rpos + n], oid[rpos:rpos + n],
+                                        oint[rpos:rpos + n]))
+            ids.append((rid, n))
+            live.append((rpos, rpos + n, rid))
+            rpos += n
+
+    th = threading.Thread(target=load)
+    th.start()
+    time.sleep(0.05)
+    t = time.perf_counter()
+    for f in range(1, frames):
+        fb.update_surface_device(ds[f].value, surf)
+        fb.update_volume_device(dv[f].value, vol)
+        server.refresh()
+    synchronize()
+    mixed_s = time.perf_counter() - t
+    stop.set()
+    th.join()
+    server.drain()
+    mem_peak = _device_mem_used() - mem0
+    tm = np.array([server.timing(r) for r, _ in ids])
+    lat = tm[:, 2] - tm[:, 0]
+    nq = sum(n for _, n in ids)
+    span = (tm[:, 2].max() - tm[:, 0].min()) * 1e-3
+    server.close()
+    samples = (frames - 1) * (surf + vol)
+    out.update({"regions_surface_volume": regions,
+                "update_alone_samples_per_s": samples / alone_s,
+                "update_while_serving_samples_per_s": samples / mixed_s,
+                "serving_during_updates": {"offered_queries_per_s": offered_qps, "requests": len(ids),
+                                           "achieved_guided_samples_per_s": nq / span,
+                                           "p50_ms": float(np.percentile(lat, 50)),
+                                           "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max())},
+                "hbm_bytes": {"input_frames": int(mem_inputs), "field": int(mem_field),
+                              "server_budget": int(server_budget), "peak_total": int(mem_peak)}})
+    return out
+
+
 def cpu_baseline(frames_host, threads, timed_frames=3):
     """Oracle on a bounded sample: frame 0 (build, untimed), then the incremental
     updates of frames 1..timed_frames timed."""
@@ -319,6 +439,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched uneven EM workload")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the large mixed update + serving workload")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (rocprofv3 --pmc calibration)")
     args = ap.parse_args()
@@ -377,6 +498,8 @@ def main():
         out["serving_poisson"] = [serving_poisson(sfield, q) for q in (1.0e8, 2.0e8)]
     if not args.no_batched:
         out["batched_em"] = batched_em(cpu=not args.no_cpu, threads=min(os.cpu_count() or 1, 16))
+    if not args.no_mixed:
+        out["mixed_update_and_serving"] = mixed_workload()
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
         out["cpu_baseline"] = cpu_baseline(frames, threads, timed_frames=min(3, nframes - 1))

@@ -66,8 +66,16 @@ def generate(n, seed=0, frame=0, num_blobs=16, flags=0, scene_seed=None, blob_fr
     pos[:nb] = centres[blob[:nb]] + rng.normal(scale=0.05, size=(nb, 3))
     pos[nb:] = rng.random((n - nb, 3))
     if n - nb > 0:
-        d2 = ((pos[nb:, None, :] - centres[None, :, :]) ** 2).sum(-1)
-        blob[nb:] = np.argmin(d2, axis=1)
+        # nearest blob centre (first minimum on ties)
+        q = pos[nb:]
+        best = np.full(n - nb, np.inf)
+        arg = np.zeros(n - nb, dtype=blob.dtype)
+        for b in range(num_blobs):
+            d2 = (q[:, 0] - centres[b, 0]) ** 2 + (q[:, 1] - centres[b, 1]) ** 2 + (q[:, 2] - centres[b, 2]) ** 2
+            closer = d2 < best
+            best[closer] = d2[closer]
+            arg[closer] = b
+        blob[nb:] = arg
     lobe = rng.integers(0, scene.lobes, size=n)
     mu = scene.lobe_mu[blob, lobe]
     kappa = scene.lobe_kappa[blob, lobe]
@@ -81,6 +89,24 @@ def generate(n, seed=0, frame=0, num_blobs=16, flags=0, scene_seed=None, blob_fr
     out["flags"] = flags
     perm = rng.permutation(n)
     return out[perm]
+
+
+def generate_large(n, seed=0, frame=0, num_blobs=64, flags=0, block=1 << 20, threads=None):
+    """generate() for large n: blocks of `block` records from independent random
+    streams over the same scene (scene_seed = seed), produced on a thread pool
+    (the result does not depend on the thread count)."""
+    import concurrent.futures
+    import os
+    counts = [min(block, n - i) for i in range(0, n, block)]
+    workers = threads or min(16, os.cpu_count() or 1)
+
+    def part(b):
+        return generate(counts[b], seed=seed * 1000003 + b + 1, frame=frame, num_blobs=num_blobs, flags=flags,
+                        scene_seed=seed)
+
+    with concurrent.futures.ThreadPoolExecutor(max_workers=workers) as ex:
+        parts = list(ex.map(part, range(len(counts))))
+    return np.concatenate(parts)
 
 
 def region_samples(n, seed=0, lobes=4):

@@ -87,3 +87,49 @@ def unit_dirs(n, seed=7):
     rng = np.random.default_rng(seed)
     d = rng.normal(size=(n, 3))
     return d / np.linalg.norm(d, axis=1, keepdims=True)
+
+
+def precision_report(records_test, records_ref, dirs):
+    """Errors of a set of region records against reference records (same
+    region order): per region the pdf relative error at dirs (median and 99th
+    percentile over directions); where both have the same K and the same EM /
+    split / merge history, per component (matched by a minimum-cost assignment)
+    the weight absolute error, the mean direction angle (radians) and the mean
+    cosine relative error.  Returns a
+    dict of medians / 99th percentiles over regions and the fraction of
+    regions whose discrete history agrees."""
+    pdf_med, pdf_p99, w_err, ang, mc_rel = [], [], [], [], []
+    agree = 0
+    valid = 0
+    for a, b in zip(records_test, records_ref):
+        if not (a["valid"] and b["valid"]):
+            continue
+        valid += 1
+        pa, pb = vmm_pdf(a, dirs), vmm_pdf(b, dirs)
+        rel = np.abs(pa - pb) / pb
+        pdf_med.append(np.median(rel))
+        pdf_p99.append(np.percentile(rel, 99))
+        same = (a["K"] == b["K"] and a["emIterations"] == b["emIterations"] and a["numSplits"] == b["numSplits"]
+                and a["numMerges"] == b["numMerges"] and a["partialEmIterations"] == b["partialEmIterations"])
+        if not same:
+            continue
+        agree += 1
+        K = int(a["K"])
+        # components may sit in other slots (merge choices at tied costs):
+        # match them by a minimum-cost assignment on angle and weight
+        from scipy.optimize import linear_sum_assignment
+        da = a["meanDirections"][:K].astype(np.float64)
+        db = b["meanDirections"][:K].astype(np.float64)
+        cosm = np.clip(da @ db.T, -1.0, 1.0)
+        cost = np.arccos(cosm) + 10.0 * np.abs(a["weights"][:K][:, None] - b["weights"][:K][None, :])
+        ia, ib = linear_sum_assignment(cost)
+        w_err.extend(np.abs(a["weights"][:K][ia].astype(np.float64) - b["weights"][:K][ib]).tolist())
+        ang.extend(np.arccos(cosm[ia, ib]).tolist())
+        mb = b["meanCosines"][:K][ib].astype(np.float64)
+        mc_rel.extend((np.abs(a["meanCosines"][:K][ia] - mb) / np.maximum(mb, 1e-30)).tolist())
+    q = lambda v, p: float(np.percentile(v, p)) if len(v) else float("nan")
+    return {"regions": valid, "history_agrees": agree / max(valid, 1),
+            "pdf_rel_median": q(pdf_med, 50), "pdf_rel_p99_of_regions_p99": q(pdf_p99, 99),
+            "weight_abs_median": q(w_err, 50), "weight_abs_p99": q(w_err, 99),
+            "mean_dir_angle_median": q(ang, 50), "mean_dir_angle_p99": q(ang, 99),
+            "mean_cosine_rel_median": q(mc_rel, 50), "mean_cosine_rel_p99": q(mc_rel, 99)}
