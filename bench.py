@@ -49,25 +49,27 @@ def make_frames(n, frames, seed=0):
     return [generate(n, seed=seed, frame=f) for f in range(frames)]
 
 
-def run_gpu(frames_host, warmup, steps):
+def run_gpu(frames_host, warmup, steps, schedule=None):
     import openpgl_amd as pgl
     from openpgl_amd.hip_runtime import DeviceArray, synchronize
     dev = pgl.Device()
     field = pgl.Field(dev, max_samples_per_leaf=32000, deterministic=True)
+    if schedule:
+        field.set_region_schedule(**schedule)
     dframes = [DeviceArray.from_numpy(f) for f in frames_host]
     synchronize()
     for f in range(warmup):
         field.update_surface_device(dframes[f].value, len(frames_host[f]))
         log("warmup frame", f, "regions", field.num_regions(), field.last_update_info())
     synchronize()
-    infos = []
     t0 = time.perf_counter()
     for s in range(steps):
         f = warmup + s
         field.update_surface_device(dframes[f].value, len(frames_host[f]))
-        infos.append(field.last_update_info())
     synchronize()
     t1 = time.perf_counter()
+    # updates are queued; their figures are collected afterwards
+    infos = field.update_history(steps)
     for s, i in enumerate(infos):
         log("timed frame", warmup + s, i)
     return field, dev, (t1 - t0), infos
@@ -357,7 +359,7 @@ def batched_em(num_regions=4096, cpu=True, threads=16):
     batch.set_schedule(**table[best][0])
     best_ms = table[best][1]
     default_ms = [ms for sc, ms in table if sc == {"per_region_launches": 0, "grid": 0, "balance": 1.0,
-                                                    "min_chunks_per_rank": 1, "max_team": 64}]
+                                                    "min_chunks_per_rank": 1, "max_team": 64, "cooperative": 1}]
     n = int(len(recs))
     flops = FLOPS_ESTEP * first["estep_evals"] + FLOPS_SPLIT * first["split_evals"] + FLOPS_DIST * first["dist_evals"]
     out = {"workload": "%d regions, samples log-U[64, 65536] (%d total), initial K in {8,16,32}, fit step"
@@ -440,6 +442,9 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched uneven EM workload")
     ap.add_argument("--no-mixed", action="store_true", help="skip the large mixed update + serving workload")
+    ap.add_argument("--region-grid", type=int, default=None, help="region kernel workgroups (default: all co-resident)")
+    ap.add_argument("--region-balance", type=float, default=None, help="region team balance (default 1)")
+    ap.add_argument("--region-cooperative", type=int, default=None, help="1: cooperative region launch (default)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (rocprofv3 --pmc calibration)")
     args = ap.parse_args()
@@ -453,7 +458,14 @@ def main():
     nframes = args.warmup + args.steps
     log("generating", nframes, "frames of", args.samples, "samples")
     frames = make_frames(args.samples, nframes)
-    field, dev, dt, infos = run_gpu(frames, args.warmup, args.steps)
+    sched = {}
+    if args.region_grid is not None:
+        sched["grid"] = args.region_grid
+    if args.region_balance is not None:
+        sched["balance"] = args.region_balance
+    if args.region_cooperative is not None:
+        sched["cooperative"] = args.region_cooperative
+    field, dev, dt, infos = run_gpu(frames, args.warmup, args.steps, sched or None)
     timed = sum(len(frames[args.warmup + s]) for s in range(args.steps))
     value = timed / dt
     # roofline of the region kernel over the timed frames

@@ -214,7 +214,10 @@ PGLRegion pglVolumeSamplingGetRegion(PGLVolumeSamplingDistribution volumeSamplin
 
 /* EXTENSION: Field::Update of one field kind from records already resident in
  * device memory (same semantics as pglFieldUpdateSurface/Volume; no flag
- * routing).  Used by GPU-side producers and by the benchmark. */
+ * routing).  Used by GPU-side producers and by the benchmark.  Updates are
+ * queued: the call returns once the training work is queued on the device;
+ * the next update, any query or read of the field, and pglReleaseField wait
+ * for it.  The device records must stay unchanged until then. */
 void pglFieldUpdateSurfaceDevice(PGLField field, const PGLSampleData* deviceSamples, size_t numSamples);
 void pglFieldUpdateVolumeDevice(PGLField field, const PGLSampleData* deviceSamples, size_t numSamples);
 
@@ -271,6 +274,12 @@ void pglFieldGetKNN(PGLField field, int kind, uint32_t* neighbours /* 8 per regi
  * [18] region kernel GPU time (events around its launch only).
  * Writes min(count, 19) values. */
 void pglFieldGetLastUpdateInfo(PGLField field, int kind, double* out, size_t count);
+/* EXTENSION: the same figures for the last min(maxEntries, 256) updates of
+ * one kind, oldest first, 19 doubles each ([13] is 0); returns the count.
+ * Updates are queued (pglFieldUpdate* returns once the directional phase is
+ * queued, the next update or any read of the field waits for it), so this
+ * is how a caller gets per-update figures without waiting after each one. */
+size_t pglFieldGetUpdateHistory(PGLField field, int kind, double* out, size_t maxEntries);
 
 /* EXTENSION: scheduling of the directional update.  None of the fields
  * changes the result (bit-identical for every schedule).
@@ -280,13 +289,17 @@ void pglFieldGetLastUpdateInfo(PGLField field, int kind, double* out, size_t cou
  *   balance            a team gets one workgroup per
  *                      total samples / (grid * balance) of its region
  *   minChunksPerRank   at least this many 1024-sample chunks per workgroup
- *   maxTeam            largest team */
+ *   maxTeam            largest team
+ *   cooperative        1: cooperative launch (the runtime guarantees that the
+ *                      whole grid is resident); 0: ordinary launch (waits
+ *                      stay bounded; other streams' kernels may share CUs) */
 typedef struct {
   int perRegionLaunches;
   uint32_t grid;
   float balance;
   uint32_t minChunksPerRank;
   uint32_t maxTeam;
+  int cooperative;
 } PGLRegionSchedule;
 void pglFieldSetRegionSchedule(PGLField field, const PGLRegionSchedule* schedule);
 /* EXTENSION: per-region profile of the last update, 12 floats per region in

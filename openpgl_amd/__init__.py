@@ -151,6 +151,7 @@ _SIGS = {
     "pglFieldSetRegionSchedule": (None, [_vp, _vp]),
     "pglFieldGetRegionSchedule": (None, [_vp, _vp]),
     "pglFieldGetRegionProfile": (None, [_vp, ctypes.c_int, _vp]),
+    "pglFieldGetUpdateHistory": (ctypes.c_size_t, [_vp, ctypes.c_int, _vp, ctypes.c_size_t]),
     "pglNewRegionBatch": (_vp, [_vp, PGLFieldArguments]),
     "pglReleaseRegionBatch": (None, [_vp]),
     "pglRegionBatchStep": (None, [_vp, _u32, _vp, _vp, _vp, _vp, _vp]),
@@ -178,7 +179,7 @@ _SIGS = {
 class RegionSchedule(ctypes.Structure):
     """PGLRegionSchedule (include/openpgl_amd/openpgl.h)."""
     _fields_ = [("perRegionLaunches", ctypes.c_int), ("grid", ctypes.c_uint32), ("balance", ctypes.c_float),
-                ("minChunksPerRank", ctypes.c_uint32), ("maxTeam", ctypes.c_uint32)]
+                ("minChunksPerRank", ctypes.c_uint32), ("maxTeam", ctypes.c_uint32), ("cooperative", ctypes.c_int)]
 
 
 _lib = None
@@ -266,12 +267,13 @@ _INFO_KEYS = ["wall_ms", "spatial_ms", "directional_ms", "knn_ms", "gather_ms", 
 
 def _schedule_dict(sc):
     return {"per_region_launches": sc.perRegionLaunches, "grid": sc.grid, "balance": sc.balance,
-            "min_chunks_per_rank": sc.minChunksPerRank, "max_team": sc.maxTeam}
+            "min_chunks_per_rank": sc.minChunksPerRank, "max_team": sc.maxTeam, "cooperative": sc.cooperative}
 
 
 def _schedule_struct(d):
     return RegionSchedule(int(d.get("per_region_launches", 0)), int(d.get("grid", 0)), float(d.get("balance", 1.0)),
-                          int(d.get("min_chunks_per_rank", 1)), int(d.get("max_team", 64)))
+                          int(d.get("min_chunks_per_rank", 1)), int(d.get("max_team", 64)),
+                          int(d.get("cooperative", 1)))
 
 
 class ServerConfig(ctypes.Structure):
@@ -457,7 +459,7 @@ class Field:
         self.lib.pglFieldReset(self.h)
 
     def set_region_schedule(self, per_region_launches=None, grid=None, balance=None, min_chunks_per_rank=None,
-                            max_team=None):
+                            max_team=None, cooperative=None):
         """Directional scheduling (PGLRegionSchedule); None keeps the current value."""
         sc = RegionSchedule()
         self.lib.pglFieldGetRegionSchedule(self.h, ctypes.byref(sc))
@@ -471,7 +473,15 @@ class Field:
             sc.minChunksPerRank = int(min_chunks_per_rank)
         if max_team is not None:
             sc.maxTeam = int(max_team)
+        if cooperative is not None:
+            sc.cooperative = int(cooperative)
         self.lib.pglFieldSetRegionSchedule(self.h, ctypes.byref(sc))
+
+    def update_history(self, count=256, kind=0):
+        """Figures of the last `count` updates (oldest first), as last_update_info dicts."""
+        o = np.zeros((count, 19), np.float64)
+        n = self.lib.pglFieldGetUpdateHistory(self.h, kind, _ptr(o), count)
+        return [dict(zip(_INFO_KEYS, o[i].tolist())) for i in range(n)]
 
     def region_profile(self, kind=0):
         """(regions, 12): us of total, E-step, split, distance passes, barrier waits, merging,
@@ -485,7 +495,7 @@ class Field:
         sc = RegionSchedule()
         self.lib.pglFieldGetRegionSchedule(self.h, ctypes.byref(sc))
         return {"per_region_launches": sc.perRegionLaunches, "grid": sc.grid, "balance": sc.balance,
-                "min_chunks_per_rank": sc.minChunksPerRank, "max_team": sc.maxTeam}
+                "min_chunks_per_rank": sc.minChunksPerRank, "max_team": sc.maxTeam, "cooperative": sc.cooperative}
 
     # --- introspection
     def num_regions(self, kind=0):

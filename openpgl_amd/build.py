@@ -21,6 +21,7 @@ SOURCES = [
     ("kernels_spatial.hip", HIPFLAGS),
     ("kernels_region.hip", HIPFLAGS),
     ("kernels_query.hip", HIPFLAGS),
+    ("kernels_sort.hip", HIPFLAGS),
     ("field.cpp", ["-x", "hip", "--offload-arch=" + ARCH, "-ffp-contract=off", "-Xclang", "-target-feature",
                    "-Xclang", "-packed-fp32-ops"]),
     ("region_sched.cpp", ["-x", "hip", "--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang",

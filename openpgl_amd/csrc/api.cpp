@@ -184,10 +184,7 @@ static float dist_pdf_li(const HostDist* d, const float dir[3]) {
 
 static void field_update_host(PGLFieldImpl* F, KindField& kf, const std::vector<PGLSampleData>& samples) {
   if (samples.empty()) return;
-  F->dStage.ensure(samples.size() * sizeof(PGLSampleData));
-  PGL_HIP_CHECK(hipMemcpyAsync(F->dStage.p, samples.data(), samples.size() * sizeof(PGLSampleData), hipMemcpyHostToDevice,
-                               F->gf->stream));
-  kf.update(F->dStage.p, (uint32_t)samples.size(), F->gf->stream);
+  kf.updateHost(samples.data(), (uint32_t)samples.size(), F->gf->stream);
 }
 
 extern "C" {
@@ -296,6 +293,8 @@ PGL_CATCH_END(nullptr)
 
 void pglReleaseField(PGLField field) PGL_CATCH_BEGIN {
   THROW_IF_NULL(field, "handle");
+  field->gf->surface.finish();
+  field->gf->volume.finish();
   if (field->gf->stream) (void)hipStreamDestroy(field->gf->stream);
   delete field->gf;
   delete field;
@@ -729,16 +728,30 @@ void pglFieldGetKNN(PGLField field, int kind, uint32_t* nb) {
     }
 }
 
+static void info_values(const UpdateInfo& i, uint32_t regions, double* o) {
+  o[0] = i.wallMs; o[1] = i.spatialMs; o[2] = i.directionalMs; o[3] = i.knnMs; o[4] = i.gatherMs; o[5] = i.sortMs;
+  o[6] = i.kdRounds; o[7] = i.regionLaunches;
+  o[8] = i.estepEvals; o[9] = i.auxEvals; o[10] = i.emIterations; o[11] = i.samples; o[12] = i.splitEvals;
+  o[13] = regions;
+  o[14] = i.regionJobs; o[15] = i.regionItems; o[16] = i.maxTeam; o[17] = i.regionGrid;
+  o[18] = i.regionKernelMs;
+}
+
+size_t pglFieldGetUpdateHistory(PGLField field, int kind, double* out, size_t maxEntries) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(field, "handle");
+  KindField& k = kind_of(field, kind);
+  const std::vector<UpdateInfo>& h = k.history();
+  const size_t n = std::min(h.size(), maxEntries);
+  for (size_t e = 0; e < n; e++) info_values(h[h.size() - n + e], 0, out + 19 * e);
+  return n;
+}
+PGL_CATCH_END(0)
+
 void pglFieldGetLastUpdateInfo(PGLField field, int kind, double* out, size_t count) {
   KindField& k = kind_of(field, kind);
   const UpdateInfo& i = k.lastInfo();
   double o[19];
-  o[0] = i.wallMs; o[1] = i.spatialMs; o[2] = i.directionalMs; o[3] = i.knnMs; o[4] = i.gatherMs; o[5] = i.sortMs;
-  o[6] = i.kdRounds; o[7] = i.regionLaunches;
-  o[8] = i.estepEvals; o[9] = i.auxEvals; o[10] = i.emIterations; o[11] = i.samples; o[12] = i.splitEvals;
-  o[13] = k.numRegions();
-  o[14] = i.regionJobs; o[15] = i.regionItems; o[16] = i.maxTeam; o[17] = i.regionGrid;
-  o[18] = i.regionKernelMs;
+  info_values(i, k.numRegions(), o);
   for (size_t j = 0; j < count && j < 19; j++) out[j] = o[j];
 }
 
@@ -770,6 +783,7 @@ void pglFieldSetRegionSchedule(PGLField field, const PGLRegionSchedule* sc) PGL_
     r.balance = sc->balance;
     r.minChunksPerRank = sc->minChunksPerRank;
     r.maxTeam = sc->maxTeam;
+    r.cooperative = sc->cooperative;
   }
 }
 PGL_CATCH_END_VOID
@@ -783,6 +797,7 @@ void pglFieldGetRegionSchedule(PGLField field, PGLRegionSchedule* sc) PGL_CATCH_
   sc->balance = r.balance;
   sc->minChunksPerRank = r.minChunksPerRank;
   sc->maxTeam = r.maxTeam;
+  sc->cooperative = r.cooperative;
 }
 PGL_CATCH_END_VOID
 
@@ -800,6 +815,7 @@ static void to_sched(const PGLRegionSchedule* sc, RegionSchedule& r) {
   r.balance = sc->balance;
   r.minChunksPerRank = sc->minChunksPerRank;
   r.maxTeam = sc->maxTeam;
+  r.cooperative = sc->cooperative;
 }
 static void from_sched(const RegionSchedule& r, PGLRegionSchedule* sc) {
   sc->perRegionLaunches = r.perRegionLaunches;
@@ -807,6 +823,7 @@ static void from_sched(const RegionSchedule& r, PGLRegionSchedule* sc) {
   sc->balance = r.balance;
   sc->minChunksPerRank = r.minChunksPerRank;
   sc->maxTeam = r.maxTeam;
+  sc->cooperative = r.cooperative;
 }
 
 PGLRegionBatch pglNewRegionBatch(PGLDevice device, PGLFieldArguments args) PGL_CATCH_BEGIN {
@@ -873,7 +890,7 @@ size_t pglRegionScheduleCandidates(PGLRegionSchedule* out, size_t maxCount) {
   for (uint32_t gridHalf = 0; gridHalf < 2; gridHalf++)
     for (float bal : {0.5f, 1.f, 2.f, 4.f})
       for (uint32_t mc : {1u, 2u, 4u})
-        for (uint32_t mt : {16u, 64u}) c.push_back(PGLRegionSchedule{0, gridHalf ? 1u : 0u, bal, mc, mt});
+        for (uint32_t mt : {16u, 64u}) c.push_back(PGLRegionSchedule{0, gridHalf ? 1u : 0u, bal, mc, mt, 1});
   if (out)
     for (size_t i = 0; i < c.size() && i < maxCount; i++) out[i] = c[i];
   return c.size();

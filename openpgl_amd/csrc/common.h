@@ -19,6 +19,7 @@ static const int kMaxComponents = 32;   // PGL_VMM_MAX_COMPONENTS (include/openp
 // kChunk samples; one chunk partial holds kPartStride floats.
 static const uint32_t kChunk = 1024;
 static const int kPartStride = 256;
+static const size_t kSampleRecordBytes = 40;  // PGLSampleData (include/openpgl/data.h:21-49)
 static const int kMergeWidth = 8;       // merge loop floor = SIMD width of the CPU_8 device
 static const float kOneOverFourPi = 0.07957747154594767f;
 static const float kMaxKappaSplit = 32000.0f;  // OPENPGL_MAX_KAPPA used by the splitter

@@ -108,9 +108,11 @@ static void bounds_params(const float lo[3], const float hi[3], float center[3],
 }
 
 // ------------------------------------------------------------------ DevBuf
+// Growth is geometric (2x, at least 64 KiB): hipFree synchronises the device,
+// so buffers whose size follows the region count must not reallocate often.
 void DevBuf::ensure(size_t b, bool keep) {
   if (b <= bytes) return;
-  size_t nb = std::max(b, bytes + bytes / 2);
+  size_t nb = std::max(std::max(b, bytes * 2), size_t(65536));
   void* np = nullptr;
   PGL_HIP_CHECK(hipMalloc(&np, nb));
   if (keep && p && bytes) PGL_HIP_CHECK(hipMemcpy(np, p, bytes, hipMemcpyDeviceToDevice));
@@ -152,7 +154,11 @@ void uniform_directions(std::vector<float>& out) {  // WEM:668-700 spherical Fib
 KindField::KindField(const KdSettings& kd, const FactoryConfig& fc, bool deterministic, int vectorWidth)
     : kd_(kd), fc_(fc), deterministic_(deterministic), vectorWidth_(vectorWidth) {
   for (int i = 0; i < 3; i++) { sceneLo_[i] = sceneHi_[i] = treeLo_[i] = treeHi_[i] = 0.f; }
-  for (auto& e : ev_) PGL_HIP_CHECK(hipEventCreate(&e));
+  for (int p = 0; p < 2; p++) {
+    for (auto& e : evSp_[p]) PGL_HIP_CHECK(hipEventCreate(&e));
+    for (auto& e : evDir_[p]) PGL_HIP_CHECK(hipEventCreate(&e));
+  }
+  PGL_HIP_CHECK(hipStreamCreate(&sp_));
   std::vector<float> ud;
   uniform_directions(ud);
   dUniformDirs_.ensure(ud.size() * 4);
@@ -160,7 +166,16 @@ KindField::KindField(const KdSettings& kd, const FactoryConfig& fc, bool determi
 }
 
 KindField::~KindField() {
-  for (auto& e : ev_) (void)hipEventDestroy(e);
+  try {
+    finish();
+  } catch (...) {
+  }
+  (void)hipStreamSynchronize(sp_);
+  for (int p = 0; p < 2; p++) {
+    for (auto& e : evSp_[p]) (void)hipEventDestroy(e);
+    for (auto& e : evDir_[p]) (void)hipEventDestroy(e);
+  }
+  (void)hipStreamDestroy(sp_);
 }
 
 void KindField::setSceneBounds(const float lo[3], const float hi[3]) {
@@ -173,6 +188,7 @@ void KindField::sceneBounds(float lo[3], float hi[3]) const {
 }
 
 void KindField::reset() {  // Field::resetField (Field.h:222-232)
+  finish();
   iteration_ = 0;
   boundsSet_ = false;
   initialized_ = false;
@@ -325,16 +341,25 @@ void KindField::kdUpdate(uint32_t n, hipStream_t s) {
   dFrontierSlot_.ensure(std::max<size_t>(leafSlot.size(), 1) * sizeof(int32_t));
   dFrontier_.ensure(std::max<size_t>(params.size() * 4, sizeof(FrontierNode)));
   dAcc_.ensure(std::max<size_t>(L, 2) * sizeof(IntStatsAcc));
-  PGL_HIP_CHECK(hipMemcpyAsync(dNodes_.p, nodes_.data(), nodes_.size() * sizeof(KDNode), hipMemcpyHostToDevice, s));
-  PGL_HIP_CHECK(hipMemcpyAsync(dFrontierSlot_.p, leafSlot.data(), leafSlot.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
-  PGL_HIP_CHECK(hipMemcpyAsync(dFrontier_.p, params.data(), params.size() * 4, hipMemcpyHostToDevice, s));
+  {
+    const size_t nb = nodes_.size() * sizeof(KDNode), sb = leafSlot.size() * sizeof(int32_t), pb = params.size() * 4;
+    kdUp_.ensure(nb + sb + pb + 64);
+    char* up = kdUp_.as<char>();
+    memcpy(up, nodes_.data(), nb);
+    memcpy(up + nb, leafSlot.data(), sb);
+    memcpy(up + nb + sb, params.data(), pb);
+    PGL_HIP_CHECK(hipMemcpyAsync(dNodes_.p, up, nb, hipMemcpyHostToDevice, s));
+    PGL_HIP_CHECK(hipMemcpyAsync(dFrontierSlot_.p, up + nb, sb, hipMemcpyHostToDevice, s));
+    PGL_HIP_CHECK(hipMemcpyAsync(dFrontier_.p, up + nb + sb, pb, hipMemcpyHostToDevice, s));
+  }
   dStatPartial_.ensure(spatial_partial_bytes(n));
   launch_descend_stats(dPx_.as<float>(), dPy_.as<float>(), dPz_.as<float>(), n, dNodes_.as<KDNode>(),
                        dFrontierSlot_.as<int32_t>(), dFrontier_.p, L, dNodeOf_.as<uint32_t>(),
                        dStatPartial_.as<IntStatsAcc>(), dAcc_.as<IntStatsAcc>(), s);
   {
-    std::vector<IntStatsAcc> acc(L);
-    PGL_HIP_CHECK(hipMemcpyAsync(acc.data(), dAcc_.p, L * sizeof(IntStatsAcc), hipMemcpyDeviceToHost, s));
+    kdDown_.ensure(std::max<size_t>(L, 1) * sizeof(IntStatsAcc));
+    const IntStatsAcc* acc = kdDown_.as<IntStatsAcc>();
+    PGL_HIP_CHECK(hipMemcpyAsync(kdDown_.p, dAcc_.p, L * sizeof(IntStatsAcc), hipMemcpyDeviceToHost, s));
     PGL_HIP_CHECK(hipStreamSynchronize(s));
     for (uint32_t i = 0; i < L; i++) {
       if (acc[i].n == 0) continue;
@@ -345,7 +370,6 @@ void KindField::kdUpdate(uint32_t n, hipStream_t s) {
   }
   std::vector<FrontierNode> fn;
   std::vector<int32_t> slot;
-  std::vector<IntStatsAcc> acc;
   std::vector<float> halves;
   while (!frontier.items.empty()) {
     rounds++;
@@ -366,13 +390,21 @@ void KindField::kdUpdate(uint32_t n, hipStream_t s) {
     dFrontier_.ensure(F * sizeof(FrontierNode));
     dFrontierSlot_.ensure(slot.size() * sizeof(int32_t));
     dAcc_.ensure(2 * F * sizeof(IntStatsAcc));
-    PGL_HIP_CHECK(hipMemcpyAsync(dFrontier_.p, fn.data(), F * sizeof(FrontierNode), hipMemcpyHostToDevice, s));
-    PGL_HIP_CHECK(hipMemcpyAsync(dFrontierSlot_.p, slot.data(), slot.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    {
+      const size_t fb = F * sizeof(FrontierNode), sb = slot.size() * sizeof(int32_t);
+      kdUp_.ensure(fb + sb + 64);
+      char* up = kdUp_.as<char>();
+      memcpy(up, fn.data(), fb);
+      memcpy(up + fb, slot.data(), sb);
+      PGL_HIP_CHECK(hipMemcpyAsync(dFrontier_.p, up, fb, hipMemcpyHostToDevice, s));
+      PGL_HIP_CHECK(hipMemcpyAsync(dFrontierSlot_.p, up + fb, sb, hipMemcpyHostToDevice, s));
+    }
     launch_kd_round(dPx_.as<float>(), dPy_.as<float>(), dPz_.as<float>(), n, dNodeOf_.as<uint32_t>(),
                     dFrontierSlot_.as<int32_t>(), dFrontier_.as<FrontierNode>(), F, dStatPartial_.as<IntStatsAcc>(),
                     dAcc_.as<IntStatsAcc>(), s);
-    acc.resize(2 * F);
-    PGL_HIP_CHECK(hipMemcpyAsync(acc.data(), dAcc_.p, 2 * F * sizeof(IntStatsAcc), hipMemcpyDeviceToHost, s));
+    kdDown_.ensure(2 * F * sizeof(IntStatsAcc));
+    const IntStatsAcc* acc = kdDown_.as<IntStatsAcc>();
+    PGL_HIP_CHECK(hipMemcpyAsync(kdDown_.p, dAcc_.p, 2 * F * sizeof(IntStatsAcc), hipMemcpyDeviceToHost, s));
     PGL_HIP_CHECK(hipStreamSynchronize(s));
     next.items.clear();
     for (uint32_t f = 0; f < F; f++) {
@@ -399,7 +431,7 @@ void KindField::kdUpdate(uint32_t n, hipStream_t s) {
     }
     std::swap(frontier, next);
   }
-  info_.kdRounds = rounds;
+  lastKdRounds_ = rounds;
 }
 
 void KindField::ensureRegionCapacity(uint32_t r, hipStream_t s) {
@@ -463,65 +495,91 @@ void KindField::renumberDepthFirst(uint32_t nodesBefore, uint32_t regionsBefore)
   regions_.swap(nr);
 }
 
+// Field::updateField (Field.h:187-220) for records already in device memory.
+//
+// Pipelined over consecutive updates: the spatial phase (ingest, kd update
+// with its host decisions, leaf keys, sort, gather + reprojection) runs on
+// the field's spatial stream and writes one of two region-ordered sample
+// buffers; the directional phase (state copies and inputs, the region kernel,
+// KNN) is queued on the field's stream `s` behind an event of the gather.  The
+// call returns once the directional phase is queued; the previous update's
+// directional phase is finished (its statistics collected) just before this
+// one is queued, and every read of device state finishes the pending update.
 void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
   if (n == 0) return;
   const auto t0 = std::chrono::steady_clock::now();
-  info_ = UpdateInfo();
-  info_.samples = n;
+  const int P = parity_;
+  UpdateInfo inf;
+  inf.samples = n;
   dPx_.ensure(n * 4); dPy_.ensure(n * 4); dPz_.ensure(n * 4); dNodeOf_.ensure(n * 4);
   dKeys_.ensure(n * 4); dVals_.ensure(n * 4); dKeys2_.ensure(n * 4); dVals2_.ensure(n * 4);
-  dDx_.ensure(n * 4); dDy_.ensure(n * 4); dDz_.ensure(n * 4); dW_.ensure(n * 4); dPdf_.ensure(n * 4); dDist_.ensure(n * 4);
+  for (int j = 0; j < 6; j++) dSoa_[P][j].ensure(n * 4);
   const size_t sortBytes = sort_pairs_temp_bytes(n);
   dSortTmp_.ensure(sortBytes);
+  hipStream_t sp = sp_;
 
-  PGL_HIP_CHECK(hipEventRecord(ev_[0], s));
-  launch_ingest(deviceSamples, n, dPx_.as<float>(), dPy_.as<float>(), dPz_.as<float>(), dNodeOf_.as<uint32_t>(), s);
+  // ---------------------------------------------------------------- spatial phase
+  PGL_HIP_CHECK(hipEventRecord(evSp_[P][0], sp));
+  launch_ingest(deviceSamples, n, dPx_.as<float>(), dPy_.as<float>(), dPz_.as<float>(), dNodeOf_.as<uint32_t>(), sp);
   const bool first = !initialized_;
   if (first) {
-    if (!boundsSet_) estimateSceneBounds(dPx_.as<float>(), dPy_.as<float>(), dPz_.as<float>(), n, s);
+    if (!boundsSet_) estimateSceneBounds(dPx_.as<float>(), dPy_.as<float>(), dPz_.as<float>(), n, sp);
     for (int i = 0; i < 3; i++) { treeLo_[i] = sceneLo_[i]; treeHi_[i] = sceneHi_[i]; }
     nodes_.assign(1, make_leaf(0));
     regions_.assign(1, RegionHost());
     for (int i = 0; i < 3; i++) { regions_[0].lo[i] = sceneLo_[i]; regions_[0].hi[i] = sceneHi_[i]; }
-    ensureRegionCapacity(1, s);
-    PGL_HIP_CHECK(hipMemsetAsync(dStates_.p, 0, sizeof(RegionState), s));
   }
   for (auto& r : regions_) { r.count = 0; r.copyFrom = -1; }
   const uint32_t regionsBefore = (uint32_t)regions_.size();
   const uint32_t nodesBefore = first ? 1u : (uint32_t)nodes_.size();
   splitRegion_.assign(nodes_.size(), -1);
-  kdUpdate(n, s);
+  kdUpdate(n, sp);
+  inf.kdRounds = lastKdRounds_;
   renumberDepthFirst(nodesBefore, regionsBefore);
   const uint32_t R = (uint32_t)regions_.size();
-  PGL_HIP_CHECK(hipEventRecord(ev_[1], s));
+  PGL_HIP_CHECK(hipEventRecord(evSp_[P][1], sp));
 
   // region-contiguous order: stable radix sort of (region, sample index)
-  dNodes_.ensure(nodes_.size() * sizeof(KDNode));
-  PGL_HIP_CHECK(hipMemcpyAsync(dNodes_.p, keyNodes_.data(), keyNodes_.size() * sizeof(KDNode), hipMemcpyHostToDevice, s));
-  launch_leaf_keys(dNodeOf_.as<uint32_t>(), dNodes_.as<KDNode>(), n, dKeys_.as<uint32_t>(), dVals_.as<uint32_t>(), s);
+  dKeyNodes_.ensure(keyNodes_.size() * sizeof(KDNode));
+  PGL_HIP_CHECK(hipMemcpyAsync(dKeyNodes_.p, pinned(pin_[P][4], keyNodes_.data(), keyNodes_.size()),
+                               keyNodes_.size() * sizeof(KDNode), hipMemcpyHostToDevice, sp));
+  launch_leaf_keys(dNodeOf_.as<uint32_t>(), dKeyNodes_.as<KDNode>(), n, dKeys_.as<uint32_t>(), dVals_.as<uint32_t>(),
+                   sp);
   // the query kernels walk the tree in the final numbering
-  PGL_HIP_CHECK(hipStreamSynchronize(s));
-  PGL_HIP_CHECK(hipMemcpyAsync(dNodes_.p, nodes_.data(), nodes_.size() * sizeof(KDNode), hipMemcpyHostToDevice, s));
-  int endBit = 1;
-  while ((1u << endBit) < R && endBit < 32) endBit++;
-  sort_pairs(dSortTmp_.p, sortBytes, dKeys_.as<uint32_t>(), dKeys2_.as<uint32_t>(), dVals_.as<uint32_t>(),
-             dVals2_.as<uint32_t>(), n, endBit, s);
-  PGL_HIP_CHECK(hipEventRecord(ev_[2], s));
+  dNodes_.ensure(nodes_.size() * sizeof(KDNode));
+  PGL_HIP_CHECK(hipMemcpyAsync(dNodes_.p, pinned(pin_[P][5], nodes_.data(), nodes_.size()),
+                               nodes_.size() * sizeof(KDNode), hipMemcpyHostToDevice, sp));
+  if (R <= kSortMaxKeys) {
+    // stable counting sort: every region's first position is known from the kd counts
+    hBegin_.resize(R);
+    uint32_t acc = 0;
+    for (uint32_t r = 0; r < R; r++) { hBegin_[r] = acc; acc += regions_[r].count; }
+    if (acc != n) throw std::runtime_error("kd update lost samples");
+    dRegionBegin_.ensure(size_t(R) * 4);
+    PGL_HIP_CHECK(hipMemcpyAsync(dRegionBegin_.p, pinned(pin_[P][6], hBegin_.data(), R), size_t(R) * 4,
+                                 hipMemcpyHostToDevice, sp));
+    dCountSortTmp_.ensure(counting_sort_temp_bytes(n, R));
+    counting_sort_pairs(dCountSortTmp_.p, dKeys_.as<uint32_t>(), dRegionBegin_.as<uint32_t>(), R, n,
+                        dKeys2_.as<uint32_t>(), dVals2_.as<uint32_t>(), sp);
+  } else {
+    int endBit = 1;
+    while ((1u << endBit) < R && endBit < 32) endBit++;
+    sort_pairs(dSortTmp_.p, sortBytes, dKeys_.as<uint32_t>(), dKeys2_.as<uint32_t>(), dVals_.as<uint32_t>(),
+               dVals2_.as<uint32_t>(), n, endBit, sp);
+  }
+  PGL_HIP_CHECK(hipEventRecord(evSp_[P][2], sp));
 
-  // directional state of regions created by spatial splits: copies of their ancestor
-  ensureRegionCapacity(R, s);
-  std::vector<uint32_t> copies;
+  // per-region inputs (host arrays stay alive until the next update finishes this one)
+  hInputs_.assign(R, RegionInput());
+  hPivots_.assign(R, make_float4(0.f, 0.f, 0.f, 0.f));
+  hCopies_.clear();
   for (uint32_t r = regionsBefore; r < R; r++)
-    if (regions_[r].copyFrom >= 0) { copies.push_back(r); copies.push_back((uint32_t)regions_[r].copyFrom); }
-  // per-region inputs
-  typedef RegionInput Input;
-  std::vector<Input> inputs(R);
-  std::vector<float4> pivots(R);
-  std::vector<std::pair<uint32_t, uint32_t>> work;
+    if (regions_[r].copyFrom >= 0) { hCopies_.push_back(r); hCopies_.push_back((uint32_t)regions_[r].copyFrom); }
+  hWork_.clear();
   uint32_t begin = 0;
   for (uint32_t r = 0; r < R; r++) {
     const RegionHost& rg = regions_[r];
-    Input& in = inputs[r];
+    RegionInput& in = hInputs_[r];
     in.region = r;
     for (int i = 0; i < 3; i++) { in.ssMean[i] = rg.ss.mean[i]; in.ssVar[i] = rg.ss.var[i]; }
     in.ssN = rg.ss.n;
@@ -533,91 +591,133 @@ void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
     begin += rg.count;
     float var[3];
     rg.ss.variance(var);
-    pivots[r] = make_float4(rg.ss.mean[0], rg.ss.mean[1], rg.ss.mean[2], reprojection_min_distance(var));
-    if (first || rg.count > 0 || rg.splitFlag) work.push_back({rg.count, r});
+    hPivots_[r] = make_float4(rg.ss.mean[0], rg.ss.mean[1], rg.ss.mean[2], reprojection_min_distance(var));
+    // teams sized by the region's samples (the number of EM iterations a region
+    // needs is not predictable from its last update)
+    if (first || rg.count > 0 || rg.splitFlag) hWork_.push_back(RegionWork{r, rg.count, (float)rg.count});
   }
   if (begin != n) throw std::runtime_error("kd update lost samples");
-  dCopyPairs_.ensure(std::max<size_t>(copies.size(), 2) * 4);
-  dInputs_.ensure(std::max<size_t>(R, 1) * sizeof(Input));
   dPivots_.ensure(std::max<size_t>(R, 1) * sizeof(float4));
-  if (!copies.empty()) {
-    PGL_HIP_CHECK(hipMemcpyAsync(dCopyPairs_.p, copies.data(), copies.size() * 4, hipMemcpyHostToDevice, s));
-    launch_copy_states(dStates_.as<RegionState>(), dCopyPairs_.as<uint32_t>(), (uint32_t)(copies.size() / 2), s);
-  }
-  PGL_HIP_CHECK(hipMemcpyAsync(dInputs_.p, inputs.data(), R * sizeof(Input), hipMemcpyHostToDevice, s));
-  launch_set_inputs(dStates_.as<RegionState>(), dInputs_.as<RegionInput>(), R, s);
-  PGL_HIP_CHECK(hipMemcpyAsync(dPivots_.p, pivots.data(), R * sizeof(float4), hipMemcpyHostToDevice, s));
+  PGL_HIP_CHECK(hipMemcpyAsync(dPivots_.p, pinned(pin_[P][1], hPivots_.data(), R), R * sizeof(float4),
+                               hipMemcpyHostToDevice, sp));
+  float* const* soa = nullptr;
+  float* sv[6];
+  for (int j = 0; j < 6; j++) sv[j] = dSoa_[P][j].as<float>();
+  soa = sv;
   launch_gather_reproject(deviceSamples, dKeys2_.as<uint32_t>(), dVals2_.as<uint32_t>(), n, dPivots_.as<float4>(),
-                          dDx_.as<float>(), dDy_.as<float>(), dDz_.as<float>(), dW_.as<float>(), dPdf_.as<float>(),
-                          dDist_.as<float>(), s);
-  PGL_HIP_CHECK(hipEventRecord(ev_[3], s));
+                          soa[0], soa[1], soa[2], soa[3], soa[4], soa[5], sp);
+  PGL_HIP_CHECK(hipEventRecord(evSp_[P][3], sp));
 
-  // directional update: teams sized by the region's samples.  (The number of
-  // EM iterations a region needs is not predictable from its last update, so
-  // the evaluations per sample recorded in dCostHint_ are kept for reporting
-  // only.)
-  std::vector<RegionWork> rw;
-  rw.reserve(work.size());
-  for (const auto& w : work) rw.push_back(RegionWork{w.second, w.first, (float)w.first});
-  dCostHint_.ensure(std::max<size_t>(R, 1) * 4, true);
-  RegionRunner& rr = runner();
-  rr.launch(dStates_.as<RegionState>(), rw,
-            RegionSamplesView{dDx_.as<float>(), dDy_.as<float>(), dDz_.as<float>(), dW_.as<float>(),
-                              dPdf_.as<float>(), dDist_.as<float>()},
-            fc_, dUniformDirs_.as<float>(), dCostHint_.as<float>(), s);
-  info_.regionLaunches = rr.lastLaunches;
-  info_.regionJobs = rr.lastJobs;
-  info_.regionItems = rr.lastItems;
-  info_.maxTeam = rr.lastMaxTeam;
-  info_.regionGrid = rr.lastGrid;
-  PGL_HIP_CHECK(hipEventRecord(ev_[4], s));
+  // ------------------------------------------------ finish the previous update
+  finish();
 
-  // KNN neighbourhoods of the region pivots (sample-statistics means)
+  // ---------------------------------------------------------- directional phase
+  // (queued on s; waits for this update's gather)
+  if (first) {
+    ensureRegionCapacity(1, s);
+    PGL_HIP_CHECK(hipMemsetAsync(dStates_.p, 0, sizeof(RegionState), s));
+  }
+  ensureRegionCapacity(R, s);
+  PGL_HIP_CHECK(hipEventRecord(evDir_[P][0], s));
+  dCopyPairs_.ensure(std::max<size_t>(hCopies_.size(), 2) * 4);
+  dInputs_.ensure(std::max<size_t>(R, 1) * sizeof(RegionInput));
+  if (!hCopies_.empty()) {
+    PGL_HIP_CHECK(hipMemcpyAsync(dCopyPairs_.p, pinned(pin_[P][2], hCopies_.data(), hCopies_.size()),
+                                 hCopies_.size() * 4, hipMemcpyHostToDevice, s));
+    launch_copy_states(dStates_.as<RegionState>(), dCopyPairs_.as<uint32_t>(), (uint32_t)(hCopies_.size() / 2), s);
+  }
+  PGL_HIP_CHECK(hipMemcpyAsync(dInputs_.p, pinned(pin_[P][0], hInputs_.data(), R), R * sizeof(RegionInput),
+                               hipMemcpyHostToDevice, s));
+  launch_set_inputs(dStates_.as<RegionState>(), dInputs_.as<RegionInput>(), R, s);
+  // KNN neighbourhoods of the region pivots (sample-statistics means); every
+  // upload is queued before the region kernel so that nothing after it waits
   if (kd_.knnLookup) {
-    std::vector<float4> piv(R);
-    for (uint32_t r = 0; r < R; r++) piv[r] = make_float4(regions_[r].ss.mean[0], regions_[r].ss.mean[1], regions_[r].ss.mean[2], 0.f);
+    hKnnPiv_.resize(R);
+    for (uint32_t r = 0; r < R; r++)
+      hKnnPiv_[r] = make_float4(regions_[r].ss.mean[0], regions_[r].ss.mean[1], regions_[r].ss.mean[2], 0.f);
     dNbIds_.ensure(size_t(R) * 8 * 4);
     dNbPts_.ensure(size_t(R) * 8 * sizeof(float4));
-    DevBuf& tmp = dPartial_;
-    tmp.ensure(size_t(R) * sizeof(float4));
-    PGL_HIP_CHECK(hipMemcpyAsync(tmp.p, piv.data(), R * sizeof(float4), hipMemcpyHostToDevice, s));
-    launch_knn8(tmp.as<float4>(), R, dNbIds_.as<uint32_t>(), dNbPts_.as<float4>(), s);
+    dKnnPiv_.ensure(size_t(R) * sizeof(float4));
+    PGL_HIP_CHECK(hipMemcpyAsync(dKnnPiv_.p, pinned(pin_[P][3], hKnnPiv_.data(), R), R * sizeof(float4),
+                                 hipMemcpyHostToDevice, s));
   }
-  PGL_HIP_CHECK(hipEventRecord(ev_[5], s));
+  PGL_HIP_CHECK(hipStreamWaitEvent(s, evSp_[P][3], 0));
+  dCostHint_.ensure(std::max<size_t>(R, 1) * 4, true);
+  RegionRunner& rr = runner();
+  rr.launch(dStates_.as<RegionState>(), hWork_,
+            RegionSamplesView{soa[0], soa[1], soa[2], soa[3], soa[4], soa[5]}, fc_, dUniformDirs_.as<float>(),
+            dCostHint_.as<float>(), s);
+  inf.regionLaunches = rr.lastLaunches;
+  inf.regionJobs = rr.lastJobs;
+  inf.regionItems = rr.lastItems;
+  inf.maxTeam = rr.lastMaxTeam;
+  inf.regionGrid = rr.lastGrid;
+  PGL_HIP_CHECK(hipEventRecord(evDir_[P][1], s));
+
+  if (kd_.knnLookup) launch_knn8(dKnnPiv_.as<float4>(), R, dNbIds_.as<uint32_t>(), dNbPts_.as<float4>(), s);
   hCostHint_.ensure(std::max<size_t>(R, 1) * 4);
   PGL_HIP_CHECK(hipMemcpyAsync(hCostHint_.p, dCostHint_.p, size_t(R) * 4, hipMemcpyDeviceToHost, s));
-  PGL_HIP_CHECK(hipStreamSynchronize(s));
-  hintRegions_ = R;
-  rr.check();
-  info_.estepEvals = (double)rr.stats().estepEvals;
-  info_.auxEvals = (double)rr.stats().distEvals;
-  info_.splitEvals = (double)rr.stats().splitEvals;
-  info_.emIterations = (double)rr.stats().emIterations;
+  PGL_HIP_CHECK(hipEventRecord(evDir_[P][2], s));
+
   for (auto& r : regions_) { r.splitFlag = false; r.copyFrom = -1; }
   initialized_ = true;
   iteration_++;
   mirrorDirty_ = true;
+  pendingInfo_ = inf;
+  pendingRegions_ = R;
+  pendingT0_ = t0;
+  pendingParity_ = P;
+  pending_ = true;
+  parity_ = 1 - P;
+}
 
+// Completes the queued directional phase: waits for it, checks the region
+// kernel's error flag and records the update's figures.
+void KindField::finish() const {
+  if (!pending_) return;
+  const int P = pendingParity_;
+  PGL_HIP_CHECK(hipEventSynchronize(evDir_[P][2]));
+  pending_ = false;
+  RegionRunner& rr = *runner_;
+  hintRegions_ = pendingRegions_;
+  UpdateInfo& i = pendingInfo_;
+  i.estepEvals = (double)rr.stats().estepEvals;
+  i.auxEvals = (double)rr.stats().distEvals;
+  i.splitEvals = (double)rr.stats().splitEvals;
+  i.emIterations = (double)rr.stats().emIterations;
   float ms = 0.f;
-  PGL_HIP_CHECK(hipEventElapsedTime(&ms, ev_[0], ev_[1])); info_.spatialMs = ms;
-  PGL_HIP_CHECK(hipEventElapsedTime(&ms, ev_[1], ev_[2])); info_.sortMs = ms;
-  PGL_HIP_CHECK(hipEventElapsedTime(&ms, ev_[2], ev_[3])); info_.gatherMs = ms;
-  PGL_HIP_CHECK(hipEventElapsedTime(&ms, ev_[3], ev_[4])); info_.directionalMs = ms;
-  info_.regionKernelMs = rr.kernelMs();
-  PGL_HIP_CHECK(hipEventElapsedTime(&ms, ev_[4], ev_[5])); info_.knnMs = ms;
-  info_.wallMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  PGL_HIP_CHECK(hipEventElapsedTime(&ms, evSp_[P][0], evSp_[P][1])); i.spatialMs = ms;
+  PGL_HIP_CHECK(hipEventElapsedTime(&ms, evSp_[P][1], evSp_[P][2])); i.sortMs = ms;
+  PGL_HIP_CHECK(hipEventElapsedTime(&ms, evSp_[P][2], evSp_[P][3])); i.gatherMs = ms;
+  PGL_HIP_CHECK(hipEventElapsedTime(&ms, evDir_[P][0], evDir_[P][1])); i.directionalMs = ms;
+  PGL_HIP_CHECK(hipEventElapsedTime(&ms, evDir_[P][1], evDir_[P][2])); i.knnMs = ms;
+  i.regionKernelMs = rr.kernelMs();
+  i.wallMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - pendingT0_).count();
+  info_ = i;
+  history_.push_back(i);
+  if (history_.size() > 256) history_.erase(history_.begin());
+  rr.check();
+}
+
+void KindField::updateFromHost(const void* hostSamples, uint32_t n) {
+  if (n == 0) return;
+  dAos_.ensure(size_t(n) * kSampleRecordBytes);
+  // on the spatial stream, after the previous update's gather read the staging buffer
+  PGL_HIP_CHECK(hipMemcpyAsync(dAos_.p, hostSamples, size_t(n) * kSampleRecordBytes, hipMemcpyHostToDevice, sp_));
+  PGL_HIP_CHECK(hipStreamSynchronize(sp_));
 }
 
 void PinnedBuf::ensure(size_t b) {
   if (b <= bytes) return;
+  const size_t nb = std::max(std::max(b, bytes * 2), size_t(65536));
   void* np = nullptr;
-  PGL_HIP_CHECK(hipHostMalloc(&np, b, hipHostMallocDefault));
+  PGL_HIP_CHECK(hipHostMalloc(&np, nb, hipHostMallocDefault));
   if (p) {
     std::memcpy(np, p, bytes);
     (void)hipHostFree(p);
   }
   p = np;
-  bytes = b;
+  bytes = nb;
 }
 
 PinnedBuf::~PinnedBuf() {
@@ -630,6 +730,7 @@ RegionRunner& KindField::runner() {
 }
 
 void KindField::syncMirror(hipStream_t s) const {
+  finish();
   std::lock_guard<std::mutex> lock(mirrorMutex_);
   if (!mirrorDirty_) return;
   const uint32_t R = (uint32_t)regions_.size();
@@ -672,6 +773,7 @@ int KindField::lookupRegion(const float p[3], float* u, uint32_t& id) const {
 }
 
 QueryField KindField::queryView() const {
+  finish();
   QueryField q;
   q.nodes = dNodes_.as<KDNode>();
   q.regions = dStates_.as<RegionState>();

@@ -4,6 +4,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <chrono>
+#include <cstring>
+#include <algorithm>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -79,7 +82,15 @@ class KindField {
   ~KindField();
 
   void setSceneBounds(const float lo[3], const float hi[3]);
+  // queues an update (see field.cpp); deviceSamples must stay valid until the next call
   void update(const void* deviceSamples, uint32_t n, hipStream_t s);
+  // update from host records (copied to the field's own staging buffer first)
+  void updateHost(const void* hostSamples, uint32_t n, hipStream_t s) {
+    updateFromHost(hostSamples, n);
+    update(dAos_.p, n, s);
+  }
+  // waits for a queued update and collects its figures
+  void finish() const;
   void reset();
 
   // host mirror of the device state for scalar queries and exports
@@ -97,7 +108,8 @@ class KindField {
   void canonicalOrder(std::vector<KDNode>& nodesOut, std::vector<uint32_t>& regionOrder) const;
   void exportRegion(uint32_t storageIdx, struct PGLRegionRecordHost* out) const;
   void knnHost(std::vector<uint32_t>& ids) const;
-  const UpdateInfo& lastInfo() const { return info_; }
+  const UpdateInfo& lastInfo() const { finish(); return info_; }
+  const std::vector<UpdateInfo>& history() const { finish(); return history_; }
   RegionSchedule& schedule() { return runner().schedule; }
   RegionRunner& runner();
   bool validate() const;
@@ -117,6 +129,7 @@ class KindField {
                  const HostSampleStats& st, struct FrontierItemList& next);
   void ensureRegionCapacity(uint32_t r, hipStream_t s);
   void renumberDepthFirst(uint32_t nodesBefore, uint32_t regionsBefore);
+  void updateFromHost(const void* hostSamples, uint32_t n);
 
   KdSettings kd_;
   FactoryConfig fc_;
@@ -135,13 +148,40 @@ class KindField {
 
   // device buffers
   DevBuf dPx_, dPy_, dPz_, dNodeOf_, dKeys_, dVals_, dKeys2_, dVals2_, dSortTmp_;
-  DevBuf dDx_, dDy_, dDz_, dW_, dPdf_, dDist_;
-  DevBuf dAos_;
+  DevBuf dSoa_[2][6];     // region-ordered dx, dy, dz, w, pdf, dist of two consecutive updates
+  DevBuf dAos_;           // staging of host records
+  DevBuf dKeyNodes_, dKnnPiv_;
+  hipStream_t sp_ = nullptr;  // spatial phase
+  int parity_ = 0;
+  hipEvent_t evSp_[2][4], evDir_[2][3];
+  // pinned staging of the per-update uploads, by frame parity: [0] inputs,
+  // [1] pivots, [2] copy pairs, [3] KNN pivots, [4] key nodes, [5] nodes,
+  // [6] region begins
+  PinnedBuf pin_[2][7];
+  DevBuf dRegionBegin_, dCountSortTmp_;
+  std::vector<uint32_t> hBegin_;
+  PinnedBuf kdUp_, kdDown_;  // kd rounds (synchronous)
+  template <class T> const T* pinned(PinnedBuf& b, const T* src, size_t count) {
+    b.ensure(std::max<size_t>(count, 1) * sizeof(T));
+    if (count) memcpy(b.p, src, count * sizeof(T));
+    return b.as<T>();
+  }
+  std::vector<RegionInput> hInputs_;
+  std::vector<float4> hPivots_, hKnnPiv_;
+  std::vector<uint32_t> hCopies_;
+  std::vector<RegionWork> hWork_;
+  uint32_t lastKdRounds_ = 0;
+  mutable bool pending_ = false;
+  mutable UpdateInfo pendingInfo_;
+  mutable uint32_t pendingRegions_ = 0;
+  mutable int pendingParity_ = 0;
+  std::chrono::steady_clock::time_point pendingT0_;
+  mutable std::vector<UpdateInfo> history_;
   DevBuf dNodes_, dFrontier_, dFrontierSlot_, dAcc_, dPartial_, dStatPartial_;
   std::unique_ptr<RegionRunner> runner_;
   DevBuf dCostHint_;       // per region: evaluations per sample of its last update (written by the region kernel)
-  PinnedBuf hCostHint_;    // host copy, read when the next update plans its teams
-  uint32_t hintRegions_ = 0;
+  PinnedBuf hCostHint_;    // host copy
+  mutable uint32_t hintRegions_ = 0;
   DevBuf dStates_, dPivots_, dNbIds_, dNbPts_, dUniformDirs_, dCopyPairs_, dInputs_;
   uint32_t stateCapacity_ = 0;
 
@@ -150,8 +190,7 @@ class KindField {
   mutable std::vector<float4> hostNbPts_;
   mutable bool mirrorDirty_ = true;
   mutable std::mutex mirrorMutex_;
-  UpdateInfo info_;
-  hipEvent_t ev_[8];
+  mutable UpdateInfo info_;
 };
 
 // Region record layout of the EXTENSION export (mirrors PGLRegionRecord).

@@ -49,6 +49,13 @@ size_t sort_pairs_temp_bytes(uint32_t n);
 void sort_pairs(void* temp, size_t tempBytes, const uint32_t* keysIn, uint32_t* keysOut, const uint32_t* valsIn,
                 uint32_t* valsOut, uint32_t n, int endBit, hipStream_t s);
 void launch_region_ranges(const uint32_t* sortedKeys, uint32_t n, uint32_t* begin, uint32_t* end, hipStream_t s);
+// Stable sort of (key, index) pairs for keys < numKeys <= kSortMaxKeys
+// (kernels_sort.hip); keyBegin[k] = first output position of key k (the
+// running sum of the key counts); keys >= numKeys are dropped.
+static const uint32_t kSortMaxKeys = 4096;
+size_t counting_sort_temp_bytes(uint32_t n, uint32_t numKeys);
+void counting_sort_pairs(void* temp, const uint32_t* keys, const uint32_t* keyBegin, uint32_t numKeys, uint32_t n,
+                         uint32_t* keysOut, uint32_t* valsOut, hipStream_t s);
 // keys/values of region-contiguous records (begin: numRegions ascending offsets)
 void launch_region_keys(const uint32_t* begin, uint32_t numRegions, uint32_t n, uint32_t* keys, uint32_t* vals,
                         hipStream_t s);

@@ -341,15 +341,20 @@ __global__ void __launch_bounds__(256) k_gather_reproject(const SampleAoS* __res
 // Brute-force 8 nearest region pivots per region (exact, ties by index), the
 // precomputed neighbourhood of KNearestRegionsSearchTree::buildRegionNeighbours
 // (spatial/KNN.h:212-266).
+// One wave per region: every lane keeps the 8 nearest of candidates lane,
+// lane + 64, ... in (d^2, index) order; 8 wave-wide arg-min rounds over the
+// lanes' heads merge them.  Same total order as a serial scan, so the lists
+// are identical.
 __global__ void __launch_bounds__(256) k_knn8(const float4* __restrict__ pivots, uint32_t numRegions,
                                               uint32_t* __restrict__ nbIds, float4* __restrict__ nbPts) {
-  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= numRegions) return;
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t r = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r >= numRegions) return;  // uniform over the wave
   const float4 q = pivots[r];
   float bd[8];
   uint32_t bi[8];
   for (int i = 0; i < 8; i++) { bd[i] = 3.402823466e38f; bi[i] = 0xffffffffu; }
-  for (uint32_t i = 0; i < numRegions; i++) {
+  for (uint32_t i = lane; i < numRegions; i += 64) {
     const float4 p = pivots[i];
     const float dx = __fsub_rn(q.x, p.x), dy = __fsub_rn(q.y, p.y), dz = __fsub_rn(q.z, p.z);
     const float d = __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)), __fmul_rn(dz, dz));
@@ -364,10 +369,25 @@ __global__ void __launch_bounds__(256) k_knn8(const float4* __restrict__ pivots,
       bi[k] = i;
     }
   }
-  for (int i = 0; i < 8; i++) {
-    nbIds[r * 8 + i] = bi[i];
-    const float4 p = bi[i] == 0xffffffffu ? make_float4(0.f, 0.f, 0.f, 0.f) : pivots[bi[i]];
-    nbPts[r * 8 + i] = p;
+  int head = 0;
+  for (int k = 0; k < 8; k++) {
+    float d = 3.402823466e38f;
+    uint32_t id = 0xffffffffu;
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+      if (j == head) { d = bd[j]; id = bi[j]; }
+    float wd = d;
+    uint32_t wi = id;
+    for (int off = 32; off > 0; off >>= 1) {
+      const float od = __shfl_xor(wd, off);
+      const uint32_t oi = __shfl_xor(wi, off);
+      if (od < wd || (od == wd && oi < wi)) { wd = od; wi = oi; }
+    }
+    if (wi != 0xffffffffu && wi == id && head < 8) head++;
+    if (lane == 0) {
+      nbIds[r * 8 + k] = wi;
+      nbPts[r * 8 + k] = wi == 0xffffffffu ? make_float4(0.f, 0.f, 0.f, 0.f) : pivots[wi];
+    }
   }
 }
 
@@ -477,7 +497,8 @@ void launch_gather_reproject(const void* aos, const uint32_t* sortedKeys, const 
 }
 
 void launch_knn8(const float4* pivots, uint32_t numRegions, uint32_t* nbIds, float4* nbPts, hipStream_t s) {
-  hipLaunchKernelGGL(k_knn8, dim3((numRegions + 255) / 256), dim3(256), 0, s, pivots, numRegions, nbIds, nbPts);
+  if (numRegions == 0) return;
+  hipLaunchKernelGGL(k_knn8, dim3((numRegions + 3) / 4), dim3(256), 0, s, pivots, numRegions, nbIds, nbPts);
 }
 
 }  // namespace pglgpu

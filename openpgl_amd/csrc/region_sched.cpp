@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <stdexcept>
 #include <string>
 
@@ -15,11 +16,20 @@ namespace pglgpu {
                                                    " in " #expr);                                         \
   } while (0)
 
+static void grow_pinned(void*& p, size_t& have, size_t need) {
+  if (need <= have) return;
+  if (p) PGL_SCHED_CHECK(hipHostFree(p));
+  p = nullptr;
+  need = std::max<size_t>(std::max(need, have * 2), 65536);
+  PGL_SCHED_CHECK(hipHostMalloc(&p, need, hipHostMallocDefault));
+  have = need;
+}
+
 static void grow(void*& p, size_t& have, size_t need) {
   if (need <= have) return;
   if (p) PGL_SCHED_CHECK(hipFree(p));
   p = nullptr;
-  need = std::max<size_t>(need, 256);
+  need = std::max<size_t>(std::max(need, have * 2), 65536);
   PGL_SCHED_CHECK(hipMalloc(&p, need));
   have = need;
 }
@@ -50,6 +60,8 @@ RegionRunner::RegionRunner() {
 RegionRunner::~RegionRunner() {
   (void)hipFree(dJobs_);
   (void)hipFree(dItems_);
+  (void)hipHostFree(hJobs_);
+  (void)hipHostFree(hItems_);
   (void)hipFree(dQueue_);
   (void)hipFree(dPartial_);
   (void)hipFree(dErr_);
@@ -109,8 +121,12 @@ void RegionRunner::launch(RegionState* states, std::vector<RegionWork> work, con
   grow(dItems_, itemsBytes_, items_.size() * sizeof(uint2));
   grow(dQueue_, queueBytes_, (size_t)(J + 1) * 4);
   grow(dPartial_, partialBytes_, std::max<unsigned long long>(partialFloats, 1) * 4);
-  PGL_SCHED_CHECK(hipMemcpyAsync(dJobs_, jobs_.data(), J * sizeof(RegionJob), hipMemcpyHostToDevice, s));
-  PGL_SCHED_CHECK(hipMemcpyAsync(dItems_, items_.data(), items_.size() * sizeof(uint2), hipMemcpyHostToDevice, s));
+  grow_pinned(hJobs_, hJobsBytes_, J * sizeof(RegionJob));
+  grow_pinned(hItems_, hItemsBytes_, items_.size() * sizeof(uint2));
+  memcpy(hJobs_, jobs_.data(), J * sizeof(RegionJob));
+  memcpy(hItems_, items_.data(), items_.size() * sizeof(uint2));
+  PGL_SCHED_CHECK(hipMemcpyAsync(dJobs_, hJobs_, J * sizeof(RegionJob), hipMemcpyHostToDevice, s));
+  PGL_SCHED_CHECK(hipMemcpyAsync(dItems_, hItems_, items_.size() * sizeof(uint2), hipMemcpyHostToDevice, s));
   PGL_SCHED_CHECK(hipMemsetAsync(dErr_, 0, sizeof(int), s));
   PGL_SCHED_CHECK(hipMemsetAsync(dStats_, 0, sizeof(RegionLaunchStats), s));
 
@@ -129,7 +145,7 @@ void RegionRunner::launch(RegionState* states, std::vector<RegionWork> work, con
   L.costHint = costHint;
   L.dx = rs.dx; L.dy = rs.dy; L.dz = rs.dz; L.w = rs.w; L.pdf = rs.pdf; L.dist = rs.dist;
   L.uniformDirs = uniformDirs;
-  L.cooperative = cooperative_;
+  L.cooperative = cooperative_ && schedule.cooperative != 0;
   PGL_SCHED_CHECK(hipEventRecord(evBegin_, s));
   if (perRegion) {
     L.cooperative = false;

@@ -20,6 +20,7 @@ struct RegionSchedule {
   float balance = 1.f;        // team target = total samples / (grid * balance), at least minChunksPerRank chunks
   uint32_t minChunksPerRank = 1;
   uint32_t maxTeam = 64;
+  int cooperative = 1;        // launch with hipLaunchCooperativeKernel (co-residency guaranteed by the runtime)
 };
 
 struct RegionSamplesView {
@@ -48,7 +49,7 @@ class RegionRunner {
 
   // Jobs are ordered by cost, largest first; team sizes follow the cost.
   // costHint (optional, device, per region) receives the evaluations per
-  // sample of this launch.
+  // sample of this launch.  The previous launch must have completed.
   void launch(RegionState* states, std::vector<RegionWork> work, const RegionSamplesView& rs,
               const FactoryConfig& cfg, const float* uniformDirs, float* costHint, hipStream_t s);
   // after the stream has completed: throws if a team barrier timed out
@@ -83,6 +84,11 @@ class RegionRunner {
   bool timed_ = false;
   std::vector<RegionJob> jobs_;
   std::vector<uint2> items_;
+  // pinned staging of the job and item tables (reused once the previous launch has completed)
+  void* hJobs_ = nullptr;
+  size_t hJobsBytes_ = 0;
+  void* hItems_ = nullptr;
+  size_t hItemsBytes_ = 0;
 };
 
 }  // namespace pglgpu
