@@ -26,6 +26,12 @@ import os
 import sys
 import time
 
+# Serving next to training needs a hardware queue per concurrently busy stream
+# (a field uses 3 streams, a query server slots + 1); HIP's default is 4 per
+# process, and streams sharing a queue serialise (a query batch would wait
+# behind a region kernel).  Must be set before the HIP runtime starts.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))

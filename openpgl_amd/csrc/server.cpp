@@ -128,13 +128,23 @@ void QueryServer::snapshotKind(const KindField& k, Snapshot& snap, hipStream_t s
 }
 
 void QueryServer::refresh() {
-  std::unique_lock<std::mutex> lk(mu_);
-  cvDone_.wait(lk, [&] { return inFlightBatches_ == 0; });
-  // the field's own stream orders the copy after its last update
-  PGL_HIP_CHECK(hipStreamSynchronize(gf_->stream));
-  snapshotKind(gf_->surface, surf_, copyStream_);
-  snapshotKind(gf_->volume, vol_, copyStream_);
+  std::lock_guard<std::mutex> serial(refreshMu_);
+  // the field's queued update (its own streams), outside the admission lock
+  gf_->surface.finish();
+  gf_->volume.finish();
+  int next;
+  {
+    std::unique_lock<std::mutex> lk(mu_);
+    next = 1 - active_;
+    cvDone_.wait(lk, [&] { return inUse_[next] == 0; });
+  }
+  snapshotKind(gf_->surface, surf_[next], copyStream_);
+  snapshotKind(gf_->volume, vol_[next], copyStream_);
   PGL_HIP_CHECK(hipStreamSynchronize(copyStream_));
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    active_ = next;
+  }
 }
 
 uint64_t QueryServer::submit(const QueryRequest& r) {
@@ -193,6 +203,8 @@ void QueryServer::dispatchLoop() {
       inFlightBatches_++;
       slot->busy = true;
       slot->issued = false;
+      slot->snap = active_;
+      inUse_[active_]++;
       slot->bytes = bytes;
       slot->queries = queries;
       statPeakBytes_ = std::max(statPeakBytes_, (double)inFlightBytes_);
@@ -262,8 +274,8 @@ void QueryServer::issue(Slot& s, std::vector<Piece>& batch, size_t N) {
   }
   PGL_HIP_CHECK(hipMemcpyAsync(s.devTable, s.hostTable, batch.size() * sizeof(QueryPiece), hipMemcpyHostToDevice,
                                s.stream));
-  launch_query_pieces(surf_.view, vol_.view, gf_->dHG.as<HGRep>(), s.devTable, (uint32_t)batch.size(), (uint32_t)N,
-                      s.stream);
+  launch_query_pieces(surf_[s.snap].view, vol_[s.snap].view, gf_->dHG.as<HGRep>(), s.devTable, (uint32_t)batch.size(),
+                      (uint32_t)N, s.stream);
   PGL_HIP_CHECK(hipGetLastError());
   off = 0;
   for (auto& pc : batch) {
@@ -317,6 +329,7 @@ void QueryServer::completionLoop() {
         s.pieces.clear();
         inFlightBytes_ -= s.bytes;
         inFlightBatches_--;
+        inUse_[s.snap]--;
         s.busy = false;
         s.issued = false;
         s.bytes = 0;

@@ -17,7 +17,8 @@
 // request gets bit-identical results alone or in any mix.
 //
 // The server serves a snapshot of the field's device state taken at creation
-// and on refresh(), so the field can be updated while the server runs.
+// and on refresh(), so the field can be updated while the server runs; the
+// snapshot is double-buffered so that a refresh does not stall admission.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -69,8 +70,9 @@ class QueryServer {
   // milliseconds since server start: submitted, admitted (batch issued), done
   bool timing(uint64_t id, double& submitMs, double& admitMs, double& doneMs);
   void forget(uint64_t id);
-  // copy the field's current device state into the server's snapshot
-  // (waits for in-flight batches first)
+  // snapshot the field's current device state: waits for the field's queued
+  // update, copies into the inactive snapshot (once no batch reads it) and
+  // makes it active; batches keep being admitted meanwhile
   void refresh();
   // [0] requests done, [1] queries done, [2] batches, [3] peak in-flight bytes,
   // [4] peak pending requests, [5] largest batch (queries)
@@ -101,6 +103,7 @@ class QueryServer {
     size_t bytes = 0;
     size_t queries = 0;
     bool issued = false;
+    int snap = 0;  // snapshot the batch reads
     std::vector<Piece> pieces;
   };
   struct Snapshot {
@@ -117,7 +120,11 @@ class QueryServer {
   GuidingField* gf_;
   ServerConfig cfg_;
   std::vector<Slot> slots_;
-  Snapshot surf_, vol_;
+  // two snapshots: batches read the active one while refresh() fills the other
+  Snapshot surf_[2], vol_[2];
+  int active_ = 0;
+  size_t inUse_[2] = {0, 0};  // batches in flight per snapshot
+  std::mutex refreshMu_;
   hipStream_t copyStream_ = nullptr;
   std::chrono::steady_clock::time_point t0_;
 
