@@ -101,6 +101,8 @@ _SIGS = {
     "pglReleaseDevice": (None, [_vp]),
     "pglDeviceNewField": (_vp, [_vp, PGLFieldArguments]),
     "pglReleaseField": (None, [_vp]),
+    "pglDeviceNewFieldFromFile": (_vp, [_vp, ctypes.c_char_p]),
+    "pglFieldStoreToFile": (ctypes.c_bool, [_vp, ctypes.c_char_p]),
     "pglFieldGetIteration": (ctypes.c_size_t, [_vp]),
     "pglFieldSetSceneBounds": (None, [_vp, Box3]),
     "pglFieldGetSceneBounds": (Box3, [_vp]),
@@ -427,10 +429,29 @@ class Field:
         if not self.h:
             raise RuntimeError("pglDeviceNewField failed")
 
+    @classmethod
+    def from_file(cls, device, path):
+        """pglDeviceNewFieldFromFile: a field read from an Open PGL field file
+        (in the layout of the device's SIMD width)."""
+        self = cls.__new__(cls)
+        self.lib = load_library()
+        self.device = device
+        self.h = self.lib.pglDeviceNewFieldFromFile(device.h, os.fsencode(path))
+        if not self.h:
+            raise RuntimeError("pglDeviceNewFieldFromFile failed: %s" % path)
+        return self
+
     def __del__(self):
         if getattr(self, "h", None):
             self.lib.pglReleaseField(self.h)
             self.h = None
+
+    def store(self, path):
+        """pglFieldStoreToFile (layout of the field's device's SIMD width)."""
+        return bool(self.lib.pglFieldStoreToFile(self.h, os.fsencode(path)))
+
+    def compare(self, other):
+        return bool(self.lib.pglFieldCompare(self.h, other.h))
 
     def set_scene_bounds(self, lower, upper):
         b = Box3(Vec3(*lower), Vec3(*upper))

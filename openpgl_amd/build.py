@@ -28,6 +28,8 @@ SOURCES = [
                           "-packed-fp32-ops"]),
     ("region_batch.cpp", ["-x", "hip", "--offload-arch=" + ARCH, "-ffp-contract=off", "-Xclang", "-target-feature",
                           "-Xclang", "-packed-fp32-ops"]),
+    ("field_io.cpp", ["-x", "hip", "--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang",
+                      "-packed-fp32-ops"]),
     ("server.cpp", ["-x", "hip", "--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]),
     ("api.cpp", ["-x", "hip", "--offload-arch=" + ARCH, "-ffp-contract=off", "-Xclang", "-target-feature",
                  "-Xclang", "-packed-fp32-ops"]),

@@ -694,6 +694,10 @@ void KindField::finish() const {
   i.regionKernelMs = rr.kernelMs();
   i.wallMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - pendingT0_).count();
   info_ = i;
+  lastTimes_[0] = (float)i.wallMs;
+  lastTimes_[1] = 0.f;  // records are read in place: no copy step
+  lastTimes_[2] = (float)(i.spatialMs + i.sortMs + i.gatherMs);
+  lastTimes_[3] = (float)(i.directionalMs + i.knnMs);
   history_.push_back(i);
   if (history_.size() > 256) history_.erase(history_.begin());
   rr.check();

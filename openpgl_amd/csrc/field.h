@@ -6,6 +6,7 @@
 
 #include <chrono>
 #include <cstring>
+#include <iosfwd>
 #include <algorithm>
 #include <memory>
 #include <mutex>
@@ -48,6 +49,17 @@ struct KdSettings {
   uint32_t maxDepth = 32;
   uint32_t minSamples = 100;
   bool knnLookup = true;
+};
+
+// Settings a field file carries that the training kernels do not read
+// (WeightedEMFactory::Configuration maxK/maxKappa, WEM:33-42;
+// AdaptiveSplitAndMergeFactory::Configuration maxSplitItr,
+// AdaptiveSplitandMergeFactory.h:46; Field::m_fitRegions, Field.h:650).
+struct FieldFileSettings {
+  uint64_t maxK = 32;
+  float maxKappa = 320000.f;
+  int32_t maxSplitItr = 1;
+  bool fitRegions = true;
 };
 
 // Growable device allocation.
@@ -113,6 +125,14 @@ class KindField {
   RegionSchedule& schedule() { return runner().schedule; }
   RegionRunner& runner();
   bool validate() const;
+  int vectorWidth() const { return vectorWidth_; }
+
+  // Field::serialize / Field::deserialize (field/Field.h:240-305) in the
+  // layout of a reference device of SIMD width W (4, 8 or 16); readFile
+  // replaces the whole state and uploads it to the device.
+  void writeFile(std::ostream& os, int W, bool isSurface, hipStream_t s) const;
+  void readFile(std::istream& is, int W, hipStream_t s);
+  FieldFileSettings fileSettings;
 
   struct RegionHost {
     HostSampleStats ss;
@@ -191,6 +211,9 @@ class KindField {
   mutable bool mirrorDirty_ = true;
   mutable std::mutex mirrorMutex_;
   mutable UpdateInfo info_;
+  // Field::m_timeLastUpdate, ...CopySamples, ...SpatialStructureUpdate,
+  // ...DirectionalDistriubtionUpdate (ms): the last update's, or a loaded file's
+  mutable float lastTimes_[4] = {0.f, 0.f, 0.f, 0.f};
 };
 
 // Region record layout of the EXTENSION export (mirrors PGLRegionRecord).
@@ -227,6 +250,9 @@ class GuidingField {
 const HGRep* host_hg_table();
 // spherical-Fibonacci initial means (InitUniformVMM, WEM:632-666): 32 x 32 x 3 floats
 void uniform_directions(std::vector<float>& out);
+// field files in the layout of a reference device of SIMD width W
+void store_field_file(const GuidingField& gf, const char* path, int W);
+void load_field_file(GuidingField& gf, const char* path, int W);
 // directional part of a region record
 void export_state(const RegionState& S, PGLRegionRecordHost* out);
 // minimum reprojection distance of prepareSamples (WEM:1184-1196): 9 sqrt(|variance|) with a fused dot

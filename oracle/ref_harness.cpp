@@ -8,7 +8,9 @@
 // not built here.
 //
 // Mirrors, for a single region, the per-region body of Field::fitRegions
-// (field/Field.h:389-439) and Field::updateRegions (field/Field.h:441-511).
+// (field/Field.h:389-439) and Field::updateRegions (field/Field.h:441-511),
+// and exposes the reference's own Region serialisation (spatial/Region.h:63-80,
+// data/Range.h:29-38) for SIMD widths 4 and 8 to pin the field-file layout.
 #include <sstream>
 #include <vector>
 #include <algorithm>
@@ -22,6 +24,8 @@
 #include "directional/vmm/AdaptiveSplitandMergeFactory.h"
 #include "directional/vmm/VMMSurfaceSamplingDistribution.h"
 #include "directional/vmm/VMMVolumeSamplingDistribution.h"
+#include "spatial/Region.h"
+#include "data/Range.h"
 
 #include "pgl_oracle.h"
 
@@ -60,6 +64,182 @@ static void toCfg(const orc_config* c, ASM8::Configuration& a) {
   a.minSamplesForSplitting = c->minSamplesForSplitting;
   a.minSamplesForPartialRefitting = c->minSamplesForPartialRefitting;
   a.minSamplesForMerging = c->minSamplesForMerging;
+}
+
+
+// ---------------------------------------------------------------- field-file records
+// A storage entry of Field::m_regionStorageContainer (Region + Range) of a
+// device of SIMD width W, trained by the reference's own factory.
+template <int W>
+struct FileRegion {
+  typedef ParallaxAwareVonMisesFisherMixture<W, 32, true> VMM;
+  typedef AdaptiveSplitAndMergeFactory<VMM> ASM;
+  Region<VMM, typename ASM::Statistics> region;
+  Range range;
+  typename ASM::Configuration cfg;
+  bool first = true;
+};
+
+template <int W>
+static void* file_region_new(const orc_config* c) {
+  typedef FileRegion<W> FR;
+  void* mem = nullptr;
+  if (posix_memalign(&mem, 64, sizeof(FR)) != 0) return nullptr;
+  FR* r = new (mem) FR();
+  auto& a = r->cfg;
+  a.weightedEMCfg.initK = c->initK;
+  a.weightedEMCfg.initKappa = c->initKappa;
+  a.weightedEMCfg.maxK = c->maxK;
+  a.weightedEMCfg.maxEMIterrations = c->maxEMIterations;
+  a.weightedEMCfg.maxKappa = c->maxKappa;
+  a.weightedEMCfg.maxMeanCosine = KappaToMeanCosine<float>(c->maxKappa);
+  a.weightedEMCfg.convergenceThreshold = c->convergenceThreshold;
+  a.weightedEMCfg.weightPrior = c->weightPrior;
+  a.weightedEMCfg.meanCosinePriorStrength = c->meanCosinePriorStrength;
+  a.weightedEMCfg.meanCosinePrior = c->meanCosinePrior;
+  a.splittingThreshold = c->splittingThreshold;
+  a.mergingThreshold = c->mergingThreshold;
+  a.partialReFit = c->partialReFit;
+  a.maxSplitItr = 1;
+  a.useSplitAndMerge = c->useSplitAndMerge;
+  a.minSamplesForSplitting = c->minSamplesForSplitting;
+  a.minSamplesForPartialRefitting = c->minSamplesForPartialRefitting;
+  a.minSamplesForMerging = c->minSamplesForMerging;
+  return r;
+}
+
+// fit (first call) or update of the region with the given sample statistics
+// (13 floats: mean, sampleVariance, numSamples, lower, upper) and region bounds
+template <int W>
+static void file_region_step(void* p, const orc_sample* s, size_t n, const float* ss13, const float* bounds6,
+                             int splitFlag) {
+  typedef FileRegion<W> FR;
+  FR* r = (FR*)p;
+  auto& R = r->region;
+  R.sampleStatistics.mean = Point3(ss13[0], ss13[1], ss13[2]);
+  R.sampleStatistics.sampleVariance = Vector3(ss13[3], ss13[4], ss13[5]);
+  R.sampleStatistics.numSamples = ss13[6];
+  R.sampleStatistics.sampleBounds.lower = Vector3(ss13[7], ss13[8], ss13[9]);
+  R.sampleStatistics.sampleBounds.upper = Vector3(ss13[10], ss13[11], ss13[12]);
+  R.regionBounds.lower = Vector3(bounds6[0], bounds6[1], bounds6[2]);
+  R.regionBounds.upper = Vector3(bounds6[3], bounds6[4], bounds6[5]);
+  std::vector<SampleData> data(n);
+  memcpy(data.data(), s, n * sizeof(SampleData));
+  std::sort(data.begin(), data.end(), SampleDataLess);
+  typename FR::ASM factory;
+  typename FR::ASM::FittingStatistics fit;
+  const Point3 sampleMean = R.sampleStatistics.mean;
+  if (r->first) {
+    factory.prepareSamples(data.data(), n, R.sampleStatistics, r->cfg);
+    factory.fit(R.distribution, R.trainingStatistics, data.data(), n, r->cfg, fit);
+    R.distribution._pivotPosition = sampleMean;
+    R.valid = R.distribution.isValid();
+    R.splitFlag = false;
+    r->first = false;
+  } else {
+    if (splitFlag) R.trainingStatistics.decay(0.25f);
+    R.trainingStatistics.sufficientStatistics.applyParallaxShift(R.distribution, R.distribution._pivotPosition - sampleMean);
+    R.distribution.performRelativeParallaxShift(R.distribution._pivotPosition - sampleMean);
+    factory.prepareSamples(data.data(), n, R.sampleStatistics, r->cfg);
+    factory.update(R.distribution, R.trainingStatistics, data.data(), n, r->cfg, fit);
+    R.valid = R.distribution.isValid() && R.trainingStatistics.isValid();
+    R.splitFlag = false;
+  }
+  r->range.reset();
+}
+
+template <int W>
+static size_t file_region_serialize(void* p, char* buf, size_t cap) {
+  FileRegion<W>* r = (FileRegion<W>*)p;
+  std::ostringstream os;
+  r->region.serialize(os);
+  r->range.serialize(os);
+  const std::string b = os.str();
+  if (buf && cap >= b.size()) memcpy(buf, b.data(), b.size());
+  return b.size();
+}
+
+template <int W>
+static void file_region_export(void* p, orc_region_record* o) {
+  FileRegion<W>* r = (FileRegion<W>*)p;
+  const auto& R = r->region;
+  memset(o, 0, sizeof(*o));
+  const auto& v = R.distribution;
+  const auto& su = R.trainingStatistics.sufficientStatistics;
+  const auto& sp = R.trainingStatistics.splittingStatistics;
+  o->K = v._numComponents;
+  for (int k = 0; k < 32; k++) {
+    const int q = k / W, l = k % W;
+    o->weights[k] = v._weights[q][l];
+    o->kappas[k] = v._kappas[q][l];
+    o->meanDirections[k][0] = v._meanDirections[q].x[l];
+    o->meanDirections[k][1] = v._meanDirections[q].y[l];
+    o->meanDirections[k][2] = v._meanDirections[q].z[l];
+    o->normalizations[k] = v._normalizations[q][l];
+    o->eMinus2Kappa[k] = v._eMinus2Kappa[q][l];
+    o->meanCosines[k] = v._meanCosines[q][l];
+    o->distances[k] = v._distances[q][l];
+    o->sumOfWeightedDirections[k][0] = su.sumOfWeightedDirections[q].x[l];
+    o->sumOfWeightedDirections[k][1] = su.sumOfWeightedDirections[q].y[l];
+    o->sumOfWeightedDirections[k][2] = su.sumOfWeightedDirections[q].z[l];
+    o->sumOfWeightedStats[k] = su.sumOfWeightedStats[q][l];
+    o->sumOfDistanceWeights[k] = su.sumOfDistanceWeightes[q][l];
+    o->chiSquareMCEstimates[k] = sp.chiSquareMCEstimates[q][l];
+    o->splitMeans[k][0] = sp.splitMeans[q].x[l];
+    o->splitMeans[k][1] = sp.splitMeans[q].y[l];
+    o->splitCovariances[k][0] = sp.splitWeightedSampleCovariances[q].x[l];
+    o->splitCovariances[k][1] = sp.splitWeightedSampleCovariances[q].y[l];
+    o->splitCovariances[k][2] = sp.splitWeightedSampleCovariances[q].z[l];
+    o->splitNumSamples[k] = sp.numSamples[q][l];
+    o->splitSumWeights[k] = sp.sumWeights[q][l];
+    o->splitSumAssignedSamples[k] = sp.sumAssignedSamples[q][l];
+  }
+  o->pivot[0] = v._pivotPosition.x; o->pivot[1] = v._pivotPosition.y; o->pivot[2] = v._pivotPosition.z;
+  o->sumWeights = su.sumWeights; o->numSamples = su.numSamples; o->overallNumSamples = su.overallNumSamples;
+  o->suffK = su.numComponents; o->normalized = su.normalized;
+  o->splitK = sp.numComponents;
+  o->numSamplesAfterLastSplit = R.trainingStatistics.numSamplesAfterLastSplit;
+  o->numSamplesAfterLastMerge = R.trainingStatistics.numSamplesAfterLastMerge;
+  for (int j = 0; j < 3; j++) {
+    o->ssMean[j] = R.sampleStatistics.mean[j];
+    o->ssVariance[j] = R.sampleStatistics.sampleVariance[j];
+    o->ssLower[j] = R.sampleStatistics.sampleBounds.lower[j];
+    o->ssUpper[j] = R.sampleStatistics.sampleBounds.upper[j];
+    o->regionLower[j] = R.regionBounds.lower[j];
+    o->regionUpper[j] = R.regionBounds.upper[j];
+  }
+  o->ssNumSamples = R.sampleStatistics.numSamples;
+  o->splitFlag = R.splitFlag;
+  o->valid = R.valid;
+}
+
+template <int W>
+static float file_region_pdf(void* p, const float* d) {
+  return ((FileRegion<W>*)p)->region.distribution.pdf(Vector3(d[0], d[1], d[2]));
+}
+
+template <int W>
+static void file_region_release(void* p) {
+  ((FileRegion<W>*)p)->~FileRegion<W>();
+  free(p);
+}
+
+// a storage entry read back with the reference's Region/Range deserialisation
+template <int W>
+static void* file_region_deserialize(const char* buf, size_t n) {
+  typedef FileRegion<W> FR;
+  void* mem = nullptr;
+  if (posix_memalign(&mem, 64, sizeof(FR)) != 0) return nullptr;
+  FR* r = new (mem) FR();
+  std::istringstream is(std::string(buf, n));
+  r->region.deserialize(is);
+  r->range.deserialize(is);
+  r->first = false;
+  if (!is) {
+    file_region_release<W>(r);
+    return nullptr;
+  }
+  return r;
 }
 
 extern "C" {
@@ -274,6 +454,43 @@ void ref_stats_merge_split(const float* a13, const float* b13, int dim, float po
 }
 
 float ref_rcp(float x) { return embree::rcp(x); }
+
+// field-file storage entries for SIMD width W (4 or 8; 16 needs AVX-512 builds)
+void* ref_file_region_new(const orc_config* c, int W) {
+  return W == 4 ? file_region_new<4>(c) : (W == 8 ? file_region_new<8>(c) : nullptr);
+}
+void ref_file_region_step(void* p, int W, const orc_sample* s, size_t n, const float* ss13, const float* bounds6,
+                          int splitFlag) {
+  if (W == 4) file_region_step<4>(p, s, n, ss13, bounds6, splitFlag);
+  else file_region_step<8>(p, s, n, ss13, bounds6, splitFlag);
+}
+size_t ref_file_region_serialize(void* p, int W, char* buf, size_t cap) {
+  return W == 4 ? file_region_serialize<4>(p, buf, cap) : file_region_serialize<8>(p, buf, cap);
+}
+void* ref_file_region_deserialize(const char* buf, size_t n, int W) {
+  return W == 4 ? file_region_deserialize<4>(buf, n) : (W == 8 ? file_region_deserialize<8>(buf, n) : nullptr);
+}
+void ref_file_region_export(void* p, int W, orc_region_record* o) {
+  if (W == 4) file_region_export<4>(p, o);
+  else file_region_export<8>(p, o);
+}
+float ref_file_region_pdf(void* p, int W, const float* d) {
+  return W == 4 ? file_region_pdf<4>(p, d) : file_region_pdf<8>(p, d);
+}
+void ref_file_region_release(void* p, int W) {
+  if (W == 4) file_region_release<4>(p);
+  else file_region_release<8>(p);
+}
+// AdaptiveSplitAndMergeFactory::Configuration::serialize of the given settings
+size_t ref_file_config_serialize(const orc_config* c, char* buf, size_t cap) {
+  FileRegion<8>* r = (FileRegion<8>*)file_region_new<8>(c);
+  std::ostringstream os;
+  r->cfg.serialize(os);
+  file_region_release<8>(r);
+  const std::string b = os.str();
+  if (buf && cap >= b.size()) memcpy(buf, b.data(), b.size());
+  return b.size();
+}
 
 void ref_hg_table(float* t) {
   VMMSingleLobeHenyeyGreensteinOracle::init();

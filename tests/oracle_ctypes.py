@@ -133,6 +133,19 @@ def reference():
         lib.ref_rcp.restype = ctypes.c_float
         lib.ref_rcp.argtypes = [ctypes.c_float]
         lib.ref_hg_table.argtypes = [vp]
+        lib.ref_file_region_new.restype = vp
+        lib.ref_file_region_new.argtypes = [ctypes.POINTER(Config), ctypes.c_int]
+        lib.ref_file_region_step.argtypes = [vp, ctypes.c_int, vp, ctypes.c_size_t, vp, vp, ctypes.c_int]
+        lib.ref_file_region_serialize.restype = ctypes.c_size_t
+        lib.ref_file_region_serialize.argtypes = [vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]
+        lib.ref_file_region_deserialize.restype = vp
+        lib.ref_file_region_deserialize.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_int]
+        lib.ref_file_region_export.argtypes = [vp, ctypes.c_int, ctypes.POINTER(RegionRecord)]
+        lib.ref_file_region_pdf.restype = ctypes.c_float
+        lib.ref_file_region_pdf.argtypes = [vp, ctypes.c_int, vp]
+        lib.ref_file_region_release.argtypes = [vp, ctypes.c_int]
+        lib.ref_file_config_serialize.restype = ctypes.c_size_t
+        lib.ref_file_config_serialize.argtypes = [ctypes.POINTER(Config), ctypes.c_char_p, ctypes.c_size_t]
         _ref = lib
     return _ref
 
@@ -243,3 +256,53 @@ def region_query(region, volume, pos, nrm, g, u2, pdf_dir):
     integ = lib.orc_region_query(region.h, int(volume), ptr(pos), ptr(nrm), float(g), ptr(u2), ptr(pdf_dir),
                                  ptr(pa), ptr(d), ptr(p))
     return integ, float(pa[0]), d, float(p[0])
+
+
+class RefFileRegion:
+    """A field-file storage entry (Region + Range) of a reference device of
+    SIMD width W (4 or 8), trained and (de)serialised by the reference's code."""
+
+    def __init__(self, cfg=None, width=8, data=None):
+        self.lib = reference()
+        self.W = int(width)
+        if data is not None:
+            self.h = self.lib.ref_file_region_deserialize(data, len(data), self.W)
+        else:
+            self.h = self.lib.ref_file_region_new(ctypes.byref(cfg), self.W)
+        if not self.h:
+            raise RuntimeError("reference region could not be created")
+
+    def step(self, samples, ss13, bounds6, split_flag=0):
+        ss = np.ascontiguousarray(ss13, np.float32)
+        b = np.ascontiguousarray(bounds6, np.float32)
+        self.lib.ref_file_region_step(self.h, self.W, ptr(samples), len(samples), ptr(ss), ptr(b), int(split_flag))
+
+    def serialize(self):
+        n = self.lib.ref_file_region_serialize(self.h, self.W, None, 0)
+        buf = ctypes.create_string_buffer(n)
+        self.lib.ref_file_region_serialize(self.h, self.W, buf, n)
+        return buf.raw[:n]
+
+    def record(self):
+        r = RegionRecord()
+        self.lib.ref_file_region_export(self.h, self.W, ctypes.byref(r))
+        return r.to_dict()
+
+    def pdf(self, d):
+        d = np.ascontiguousarray(d, np.float32)
+        return self.lib.ref_file_region_pdf(self.h, self.W, ptr(d))
+
+    def __del__(self):
+        try:
+            self.lib.ref_file_region_release(self.h, self.W)
+        except Exception:
+            pass
+
+
+def ref_config_bytes(cfg):
+    """AdaptiveSplitAndMergeFactory::Configuration::serialize of cfg, by the reference's code."""
+    lib = reference()
+    n = lib.ref_file_config_serialize(ctypes.byref(cfg), None, 0)
+    buf = ctypes.create_string_buffer(n)
+    lib.ref_file_config_serialize(ctypes.byref(cfg), buf, n)
+    return buf.raw[:n]

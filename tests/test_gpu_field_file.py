@@ -1,0 +1,250 @@
+"""Field files on the GPU: pglFieldStoreToFile / pglDeviceNewFieldFromFile.
+
+  * a GPU-trained field written by the library parses, with the restated
+    format (tests/field_file.py, pinned against the reference's serialiser),
+    into exactly the library's own exports (kd nodes, every region record,
+    KNN points): bit-exact;
+  * every storage entry of that file deserialises through the reference's
+    own Region code (oracle/_ref) into the same state, and the reference's
+    mixture pdf of it matches the GPU query pdf at the region pivot (1e-3);
+  * store -> load -> store is byte-identical, the loaded field answers
+    batched queries bit-identically, and training continued from the loaded
+    field gives bit-identical results to training continued in memory;
+  * a field file written from the oracle's CPU-trained state loads on the
+    GPU and serves queries that match the oracle's (ids exact, pdf 1e-3).
+"""
+import numpy as np
+import pytest
+
+import field_file as ff
+from field_compare import OracleField, unit_dirs
+from oracle_ctypes import RefFileRegion, default_config, reference_available
+from openpgl_amd.synth import generate, queries
+
+pytestmark = pytest.mark.gpu
+
+FLAG_VOLUME = 2  # PGLSampleData::EInsideVolume (include/openpgl/data.h:28)
+
+
+def _trained(dev, frames=3, n=30000, max_samples=2000, start=0):
+    import openpgl_amd as pgl
+    f = pgl.Field(dev, max_samples_per_leaf=max_samples, deterministic=True)
+    for fr in range(start, start + frames):
+        s = np.concatenate([generate(n, seed=1, frame=fr), generate(n // 3, seed=2, frame=fr, flags=FLAG_VOLUME)])
+        print("frame", fr, "samples", len(s), flush=True)
+        f.update(pgl.SampleStorage(s))
+    return f
+
+
+def _frame(fr, n=30000):
+    return np.concatenate([generate(n, seed=1, frame=fr), generate(n // 3, seed=2, frame=fr, flags=FLAG_VOLUME)])
+
+
+def _bits(a):
+    return np.asarray(a, np.float32).view(np.uint32)
+
+
+def _same(a, b):
+    """bit-exact for float data, exact for integers"""
+    a, b = np.asarray(a), np.asarray(b)
+    if a.dtype.kind == "f" or b.dtype.kind == "f":
+        return np.array_equal(_bits(a), _bits(b))
+    return np.array_equal(a.astype(np.int64), b.astype(np.int64))
+
+
+def _check_against_exports(field, parsed):
+    for kind, name in ((0, "surface"), (1, "volume")):
+        k = parsed[name]
+        # the library exports nodes and regions in the canonical depth-first order
+        piv, dims, cod = field.nodes(kind)
+        (fp, fd, fc), order = ff.canonical(k["nodes"])
+        assert np.array_equal(_bits(fp), _bits(piv)) and np.array_equal(fd, dims) and np.array_equal(fc, cod)
+        R = field.num_regions(kind)
+        assert len(k["regions"]) == R and R > 1 and sorted(order) == list(range(R))
+        for c in range(R):
+            i = order[c]
+            a, b = k["regions"][i], field.region(c, kind)
+            for key, v in a.items():
+                if key in ("rangeBegin", "rangeEnd"):
+                    assert v == 0
+                elif key == "splitMeans":
+                    assert not np.any(v)
+                else:
+                    assert _same(v, b[key]), (name, i, key)
+        assert k["knnBuilt"] == 1
+        means = np.array([r["ssMean"] for r in k["regions"]], np.float32)
+        assert np.array_equal(_bits(k["knnPoints"][:, :3]), _bits(means))
+        assert not np.any(k["knnPoints"][:, 3])
+        assert k["initialized"] == 1 and k["treeInit"] == 1
+
+
+def test_store_parses_into_the_library_state(tmp_path):
+    import openpgl_amd as pgl
+    dev = pgl.Device()
+    f = _trained(dev)
+    path = str(tmp_path / "field.pgl")
+    assert f.store(path)
+    data = open(path, "rb").read()
+    parsed = ff.read_file(data, 8)
+    assert parsed["header"] == ff.HEADER
+    assert parsed["spatialType"] == 0 and parsed["directionalType"] == 0
+    assert parsed["iteration"] == f.iteration() == 3
+    assert parsed["surface"]["isSurface"] == 1 and parsed["volume"]["isSurface"] == 0
+    assert parsed["surface"]["iteration"] == 3
+    cfg = parsed["surface"]["config"]
+    assert cfg["initK"] == 16 and cfg["maxK"] == 32 and cfg["maxSplitItr"] == 1
+    assert parsed["surface"]["maxSamples"] == 2000 and parsed["surface"]["maxDepth"] == 32
+    _check_against_exports(f, parsed)
+
+
+@pytest.mark.skipif(not reference_available(), reason="oracle/_ref not built")
+def test_entries_read_back_by_reference_code(tmp_path):
+    import openpgl_amd as pgl
+    dev = pgl.Device()
+    f = _trained(dev)
+    path = str(tmp_path / "field.pgl")
+    assert f.store(path)
+    data = open(path, "rb").read()
+    parsed = ff.read_file(data, 8)
+    # byte offsets of the surface storage entries
+    inp = ff._In(data)
+    inp.take(len(ff.HEADER) + 8 + 16)
+    for key, fmt in ff.FIELD_HEAD:
+        inp.unpack(fmt)
+    ff.read_config(inp)
+    inp.unpack("3Q")
+    inp.unpack("B")
+    inp.unpack("6f")
+    inp.take(8 * inp.unpack("Q"))
+    R = inp.unpack("Q")
+    dirs = unit_dirs(64)
+    worst = 0.0
+    checked = 0
+    for i in range(R):
+        entry = data[inp.o:inp.o + ff.REGION_BYTES]
+        inp.o += ff.REGION_BYTES
+        ref = RefFileRegion(width=8, data=entry)
+        rec = ref.record()
+        mine = parsed["surface"]["regions"][i]
+        for key in ("K", "weights", "kappas", "meanDirections", "pivot", "ssMean", "regionLower", "valid"):
+            assert np.array_equal(np.asarray(rec[key], np.float64), np.asarray(mine[key], np.float64)), (i, key)
+        if not rec["valid"]:
+            continue
+        # the GPU's distribution at the pivot (no parallax shift), no product
+        d = f.new_surface_distribution()
+        ok, _ = d.init(rec["pivot"])
+        if not ok or d.id() != i:
+            continue
+        p_gpu = np.array([d.pdf(x) for x in dirs])
+        p_ref = np.array([ref.pdf(x) for x in dirs])
+        worst = max(worst, float(np.max(np.abs(p_gpu - p_ref) / p_ref)))
+        checked += 1
+    assert checked > R // 2, (checked, R)
+    assert worst <= 1e-3, worst
+
+
+def test_store_load_store_and_continued_training(tmp_path):
+    import openpgl_amd as pgl
+    dev = pgl.Device()
+    a = _trained(dev, frames=2)
+    p1 = str(tmp_path / "a.pgl")
+    assert a.store(p1)
+    b = pgl.Field.from_file(dev, p1)
+    assert b.iteration() == a.iteration() == 2
+    p2 = str(tmp_path / "b.pgl")
+    assert b.store(p2)
+    assert open(p1, "rb").read() == open(p2, "rb").read()
+    assert b.compare(a) and b.validate()
+    # identical batched queries
+    pos, nrm, u2, is_vol, g = queries(20000, seed=5)
+    u1 = np.random.default_rng(5).random(len(pos)).astype(np.float32)
+    ra = a.sample_batch(pos, nrm, u2, u1=u1, g=g, is_volume=is_vol)
+    rb = b.sample_batch(pos, nrm, u2, u1=u1, g=g, is_volume=is_vol)
+    for x, y in zip(ra, rb):
+        assert np.array_equal(np.asarray(x).view(np.uint32), np.asarray(y).view(np.uint32))
+    # one more update of both from the same records
+    s = _frame(2)
+    print("continue training", len(s), flush=True)
+    a.update(pgl.SampleStorage(s))
+    b.update(pgl.SampleStorage(s))
+    pa, pb = str(tmp_path / "a3.pgl"), str(tmp_path / "b3.pgl")
+    assert a.store(pa) and b.store(pb)
+    fa, fb = ff.read_file(open(pa, "rb").read(), 8), ff.read_file(open(pb, "rb").read(), 8)
+    for name in ("surface", "volume"):
+        assert np.array_equal(fa[name]["nodes"], fb[name]["nodes"])
+        assert len(fa[name]["regions"]) == len(fb[name]["regions"])
+        for ra_, rb_ in zip(fa[name]["regions"], fb[name]["regions"]):
+            for key in ra_:
+                assert _same(ra_[key], rb_[key]), (name, key)
+
+
+def test_width4_device_layout(tmp_path):
+    import openpgl_amd as pgl
+    dev = pgl.Device(pgl.PGL_DEVICE_TYPE_CPU_4)
+    f = _trained(dev, frames=1)
+    path = str(tmp_path / "w4.pgl")
+    assert f.store(path)
+    parsed = ff.read_file(open(path, "rb").read(), 4)
+    _check_against_exports(f, parsed)
+    g = pgl.Field.from_file(dev, path)
+    p2 = str(tmp_path / "w4b.pgl")
+    assert g.store(p2)
+    assert open(path, "rb").read() == open(p2, "rb").read()
+
+
+def _oracle_file(orc, cfg, iteration, max_samples):
+    """A field file of the oracle's (surface) state, written with the restated format."""
+    piv, dims, cod = orc.nodes()
+    nodes = np.zeros(len(piv), dtype=[("pivot", "<f4"), ("bits", "<u4")])
+    nodes["pivot"] = piv
+    nodes["bits"] = (dims.astype(np.uint32) << 30) | cod
+    regions = []
+    for i in range(orc.num_regions()):
+        r = orc.region(i)
+        r["splitMeans"] = np.zeros((32, 2), np.float32)
+        r["rangeBegin"] = r["rangeEnd"] = 0
+        regions.append(r)
+    lo, hi = np.zeros(3, np.float32), np.zeros(3, np.float32)
+    orc.lib.orc_field_get_scene_bounds(orc.h, lo.ctypes.data, hi.ctypes.data)
+    fcfg = {"initK": cfg.initK, "initKappa": cfg.initKappa, "maxK": cfg.maxK, "maxEMIterrations": cfg.maxEMIterations,
+            "maxKappa": cfg.maxKappa, "maxMeanCosine": float(np.float32(1.0 / np.tanh(np.float32(cfg.maxKappa))
+                                                                         - 1.0 / np.float32(cfg.maxKappa))),
+            "convergenceThreshold": cfg.convergenceThreshold, "weightPrior": cfg.weightPrior,
+            "meanCosinePriorStrength": cfg.meanCosinePriorStrength, "meanCosinePrior": cfg.meanCosinePrior,
+            "splittingThreshold": cfg.splittingThreshold, "mergingThreshold": cfg.mergingThreshold,
+            "partialReFit": cfg.partialReFit, "maxSplitItr": 1, "minSamplesForSplitting": cfg.minSamplesForSplitting,
+            "minSamplesForMerging": cfg.minSamplesForMerging,
+            "minSamplesForPartialRefitting": cfg.minSamplesForPartialRefitting}
+    surf = ff.empty_field(True, fcfg, 100, max_samples, 32)
+    surf.update({"iteration": iteration, "isSceneBoundsSet": 1, "sceneBounds": (*lo, *hi), "initialized": 1,
+                 "treeInit": 1, "treeBounds": (*lo, *hi), "nodes": nodes, "regions": regions, "knnBuilt": 1,
+                 "knnPoints": np.array([[*r["ssMean"], 0.0] for r in regions], np.float32)})
+    vol = ff.empty_field(False, fcfg, 100, max_samples, 32)
+    return ff.write_file({"iteration": iteration, "surface": surf, "volume": vol}, 8)
+
+
+def test_oracle_trained_field_served_on_gpu(tmp_path):
+    import openpgl_amd as pgl
+    cfg = default_config(2000)
+    orc = OracleField(cfg)
+    for fr in range(2):
+        orc.update(generate(30000, seed=1, frame=fr))
+    path = str(tmp_path / "oracle.pgl")
+    with open(path, "wb") as fh:
+        fh.write(_oracle_file(orc, cfg, 2, 2000))
+    dev = pgl.Device()
+    f = pgl.Field.from_file(dev, path)
+    assert f.iteration() == 2 and f.num_regions() == orc.num_regions()
+    pos, nrm, u2, _, _ = queries(20000, seed=11, volume_fraction=0.0)
+    u1 = np.random.default_rng(11).random(len(pos)).astype(np.float32)
+    gd, gp, gi, gint = f.sample_batch(pos, nrm, u2, u1=u1)
+    od, op, oi, oint = orc.query_batch(0, pos, nrm, u2, u1=u1)
+    # the file holds the oracle's regions in canonical order; the oracle reports storage ids
+    s2c = np.zeros(orc.num_regions(), np.uint32)
+    orc.lib.orc_field_region_canonical_map(orc.h, s2c.ctypes.data)
+    ok = op > 0
+    assert ok.sum() > len(op) // 2
+    assert np.array_equal(gp > 0, ok)
+    assert np.array_equal(gi[ok], s2c[oi[ok]])
+    rel = np.abs(gp[ok] - op[ok]) / op[ok]
+    assert np.mean(rel <= 1e-3) >= 0.95, np.percentile(rel, [50, 95, 99])
