@@ -18,6 +18,8 @@ LIB_PATH = os.path.join(HERE, "libopenpgl_amd.so")
 PGL_DEVICE_TYPE_CPU_4, PGL_DEVICE_TYPE_CPU_8, PGL_DEVICE_TYPE_CPU_16, PGL_DEVICE_TYPE_NONE, PGL_DEVICE_TYPE_GPU_MI355X = range(5)
 PGL_SPATIAL_STRUCTURE_KDTREE = 0
 PGL_DIRECTIONAL_DISTRIBUTION_PARALLAX_AWARE_VMM = 0
+PGL_DIRECTIONAL_DISTRIBUTION_QUADTREE = 1
+PGL_DIRECTIONAL_DISTRIBUTION_VMM = 2
 
 _vp = ctypes.c_void_p
 _f = ctypes.c_float
@@ -245,11 +247,11 @@ class SampleStorage:
             self.h = None
 
 
-def _field_arguments(lib, max_samples_per_leaf, deterministic, knn=True, overrides=None):
+def _field_arguments(lib, max_samples_per_leaf, deterministic, knn=True, overrides=None,
+                     directional_type=PGL_DIRECTIONAL_DISTRIBUTION_PARALLAX_AWARE_VMM):
     args = PGLFieldArguments()
-    lib.pglFieldArgumentsSetDefaults(ctypes.byref(args), PGL_SPATIAL_STRUCTURE_KDTREE,
-                                     PGL_DIRECTIONAL_DISTRIBUTION_PARALLAX_AWARE_VMM, deterministic,
-                                     max_samples_per_leaf)
+    lib.pglFieldArgumentsSetDefaults(ctypes.byref(args), PGL_SPATIAL_STRUCTURE_KDTREE, directional_type,
+                                     deterministic, max_samples_per_leaf)
     kd = ctypes.cast(args.spatialSturctureArguments, ctypes.POINTER(PGLKDTreeArguments)).contents
     kd.knnLookup = knn
     if overrides:
@@ -421,10 +423,11 @@ class RegionBatch:
 class Field:
     """openpgl::cpp::Field over the MI355X backend."""
 
-    def __init__(self, device, max_samples_per_leaf=32000, deterministic=True, knn=True, overrides=None):
+    def __init__(self, device, max_samples_per_leaf=32000, deterministic=True, knn=True, overrides=None,
+                 directional_type=PGL_DIRECTIONAL_DISTRIBUTION_PARALLAX_AWARE_VMM):
         self.lib = load_library()
         self.device = device
-        args = _field_arguments(self.lib, max_samples_per_leaf, deterministic, knn, overrides)
+        args = _field_arguments(self.lib, max_samples_per_leaf, deterministic, knn, overrides, directional_type)
         self.h = self.lib.pglDeviceNewField(device.h, args)
         if not self.h:
             raise RuntimeError("pglDeviceNewField failed")

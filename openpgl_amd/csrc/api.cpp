@@ -117,7 +117,7 @@ static bool dist_init(HostDist* d, const KindField& f, hipStream_t s, const floa
   const float sx = V.pivot[0] - pos[0], sy = V.pivot[1] - pos[1], sz = V.pivot[2] - pos[2];
   for (int k = 0; k < V.K; k++) {
     float dist;
-    shift_comp(V, k, sx, sy, sz, d->lmx[k], d->lmy[k], d->lmz[k], dist);
+    shift_comp(V, k, sx, sy, sz, d->lmx[k], d->lmy[k], d->lmz[k], dist, f.parallaxAware());
     d->lw[k] = V.w[k];
     d->lkappa[k] = V.kappa[k];
     d->lnorm[k] = V.norm[k];
@@ -262,8 +262,9 @@ static FactoryConfig factory_config(const PGLVMMFactoryArguments* v, int vectorW
 
 // Device::newField for the parallax-aware VMM (device/Device.h:90-138)
 static PGLFieldImpl* new_field(PGLDevice device, PGLFieldArguments args) {
-  if (args.directionalDistributionType != PGL_DIRECTIONAL_DISTRIBUTION_PARALLAX_AWARE_VMM)
-    throw std::runtime_error("openpgl_amd: only the parallax-aware VMM field is implemented");
+  if (args.directionalDistributionType != PGL_DIRECTIONAL_DISTRIBUTION_PARALLAX_AWARE_VMM &&
+      args.directionalDistributionType != PGL_DIRECTIONAL_DISTRIBUTION_VMM)
+    throw std::runtime_error("openpgl_amd: only the VMM field types are implemented (no quadtree)");
   PGLKDTreeArguments* kd = (PGLKDTreeArguments*)args.spatialSturctureArguments;
   PGLVMMFactoryArguments* v = (PGLVMMFactoryArguments*)args.directionalDistributionArguments;
   THROW_IF_NULL(kd, "spatial structure arguments");
@@ -273,7 +274,9 @@ static PGLFieldImpl* new_field(PGLDevice device, PGLFieldArguments args) {
   ks.minSamples = (uint32_t)kd->minSamples;
   ks.maxSamples = (uint32_t)kd->maxSamples;
   ks.maxDepth = (uint32_t)kd->maxDepth;
-  const FactoryConfig fc = factory_config(v, device->vectorWidth);
+  FactoryConfig fc = factory_config(v, device->vectorWidth);
+  // the plain VMM (Device.h:139-184): ParallaxAwareVonMisesFisherMixture<W, 32, false>
+  fc.noParallax = args.directionalDistributionType == PGL_DIRECTIONAL_DISTRIBUTION_VMM ? 1 : 0;
   FieldFileSettings fs;
   fs.maxK = v->maxK;
   fs.maxKappa = v->maxKappa;
@@ -865,12 +868,14 @@ static void from_sched(const RegionSchedule& r, PGLRegionSchedule* sc) {
 
 PGLRegionBatch pglNewRegionBatch(PGLDevice device, PGLFieldArguments args) PGL_CATCH_BEGIN {
   THROW_IF_NULL(device, "handle");
-  if (args.directionalDistributionType != PGL_DIRECTIONAL_DISTRIBUTION_PARALLAX_AWARE_VMM)
-    throw std::runtime_error("openpgl_amd: only the parallax-aware VMM is implemented");
+  if (args.directionalDistributionType != PGL_DIRECTIONAL_DISTRIBUTION_PARALLAX_AWARE_VMM &&
+      args.directionalDistributionType != PGL_DIRECTIONAL_DISTRIBUTION_VMM)
+    throw std::runtime_error("openpgl_amd: only the VMM field types are implemented (no quadtree)");
   PGLKDTreeArguments* kd = (PGLKDTreeArguments*)args.spatialSturctureArguments;
   PGLVMMFactoryArguments* v = (PGLVMMFactoryArguments*)args.directionalDistributionArguments;
   THROW_IF_NULL(v, "directional distribution arguments");
-  const FactoryConfig fc = factory_config(v, device->vectorWidth);
+  FactoryConfig fc = factory_config(v, device->vectorWidth);
+  fc.noParallax = args.directionalDistributionType == PGL_DIRECTIONAL_DISTRIBUTION_VMM ? 1 : 0;
   delete kd;
   delete v;
   PGLRegionBatchImpl* B = new PGLRegionBatchImpl();

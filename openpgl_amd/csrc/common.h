@@ -55,7 +55,8 @@ struct FactoryConfig {
   float decayOnSpatialSplit;
   float maxMeanCosineSplit;      // KappaToMeanCosine(32000)
   int32_t vectorWidth;           // SIMD width of the emulated reference device (merge floor, clear granularity)
-  float pad[2];
+  int32_t noParallax;            // 1: plain VMM field (PGL_DIRECTIONAL_DISTRIBUTION_VMM): no shift before updates
+  float pad[1];
 };
 
 // ParallaxAwareVonMisesFisherMixture (PAVMM:22-87) as SoA over 32 slots.

@@ -605,7 +605,7 @@ void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
   for (int j = 0; j < 6; j++) sv[j] = dSoa_[P][j].as<float>();
   soa = sv;
   launch_gather_reproject(deviceSamples, dKeys2_.as<uint32_t>(), dVals2_.as<uint32_t>(), n, dPivots_.as<float4>(),
-                          soa[0], soa[1], soa[2], soa[3], soa[4], soa[5], sp);
+                          soa[0], soa[1], soa[2], soa[3], soa[4], soa[5], fc_.noParallax == 0, sp);
   PGL_HIP_CHECK(hipEventRecord(evSp_[P][3], sp));
 
   // ------------------------------------------------ finish the previous update
@@ -786,6 +786,7 @@ QueryField KindField::queryView() const {
   q.nbSize = std::min<uint32_t>(8, (uint32_t)regions_.size());
   q.knn = kd_.knnLookup ? 1 : 0;
   q.trained = (initialized_ && iteration_ > 0) ? 1 : 0;
+  q.parallax = fc_.noParallax ? 0 : 1;
   for (int i = 0; i < 3; i++) { q.lo[i] = treeLo_[i]; q.hi[i] = treeHi_[i]; }
   return q;
 }

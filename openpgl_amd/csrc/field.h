@@ -126,12 +126,14 @@ class KindField {
   RegionRunner& runner();
   bool validate() const;
   int vectorWidth() const { return vectorWidth_; }
+  // false for the plain VMM field type (PGL_DIRECTIONAL_DISTRIBUTION_VMM)
+  bool parallaxAware() const { return fc_.noParallax == 0; }
 
   // Field::serialize / Field::deserialize (field/Field.h:240-305) in the
   // layout of a reference device of SIMD width W (4, 8 or 16); readFile
   // replaces the whole state and uploads it to the device.
   void writeFile(std::ostream& os, int W, bool isSurface, hipStream_t s) const;
-  void readFile(std::istream& is, int W, hipStream_t s);
+  void readFile(std::istream& is, int W, bool parallaxAware, hipStream_t s);
   FieldFileSettings fileSettings;
 
   struct RegionHost {

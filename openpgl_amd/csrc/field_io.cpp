@@ -286,7 +286,7 @@ void KindField::writeFile(std::ostream& os, int W, bool isSurface, hipStream_t s
   }
 }
 
-void KindField::readFile(std::istream& is, int W, hipStream_t s) {
+void KindField::readFile(std::istream& is, int W, bool parallaxAware, hipStream_t s) {
   check_width(W);
   finish();
   Reader r{is};
@@ -326,6 +326,7 @@ void KindField::readFile(std::istream& is, int W, hipStream_t s) {
   fc.useSplitAndMerge = 1;  // not stored: a loaded Configuration keeps its default (AdaptiveSplitandMergeFactory.h:43)
   fc.decayOnSpatialSplit = decay;
   fc.vectorWidth = W;
+  fc.noParallax = parallaxAware ? 0 : 1;
   if (fc.initK < 1) throw std::runtime_error("error: invalid field file (initK)");
   fs.fitRegions = fitRegions;
 
@@ -430,7 +431,7 @@ void store_field_file(const GuidingField& gf, const char* path, int W) {
   Writer w{os};
   w.bytes(kFieldHeader, sizeof(kFieldHeader));
   w.put<int32_t>(kKdTree);
-  w.put<int32_t>(kParallaxAwareVMM);
+  w.put<int32_t>(gf.surface.parallaxAware() ? kParallaxAwareVMM : kVMM);
   w.size(gf.iteration);
   w.size(0);  // m_totalSPP
   gf.surface.writeFile(os, W, true, gf.stream);
@@ -453,12 +454,13 @@ void load_field_file(GuidingField& gf, const char* path, int W) {
   const int32_t directional = r.get<int32_t>();
   if (spatial != kKdTree || (directional != kParallaxAwareVMM && directional != kVMM && directional != kQuadtree))
     throw std::runtime_error("error: unrecognized field type");
-  if (directional != kParallaxAwareVMM)
-    throw std::runtime_error("openpgl_amd: only parallax-aware VMM field files can be loaded");
+  if (directional == kQuadtree)
+    throw std::runtime_error("openpgl_amd: quadtree field files are not supported (VMM field types only)");
+  const bool parallaxAware = directional == kParallaxAwareVMM;
   const uint64_t iteration = r.size();
   (void)r.size();  // m_totalSPP
-  gf.surface.readFile(is, W, gf.stream);
-  gf.volume.readFile(is, W, gf.stream);
+  gf.surface.readFile(is, W, parallaxAware, gf.stream);
+  gf.volume.readFile(is, W, parallaxAware, gf.stream);
   gf.iteration = (uint32_t)iteration;
 }
 

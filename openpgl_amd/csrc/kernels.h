@@ -29,6 +29,7 @@ struct QueryField {
   int32_t knn;
   int32_t trained;
   float lo[3], hi[3];
+  int32_t parallax;  // 0: plain VMM field, no shift to the query position
 };
 
 // spatial
@@ -61,7 +62,7 @@ void launch_region_keys(const uint32_t* begin, uint32_t numRegions, uint32_t n, 
                         hipStream_t s);
 void launch_gather_reproject(const void* aos, const uint32_t* sortedKeys, const uint32_t* sortedIdx, uint32_t n,
                              const float4* regionPivot, float* dx, float* dy, float* dz, float* w, float* pdf,
-                             float* dist, hipStream_t s);
+                             float* dist, bool reproject, hipStream_t s);
 void launch_knn8(const float4* pivots, uint32_t numRegions, uint32_t* nbIds, float4* nbPts, hipStream_t s);
 
 // directional

@@ -47,26 +47,26 @@ __device__ __forceinline__ int lookup_region(const QueryField& f, float x, float
 }
 
 // integral of the product of the shifted mixture with lobe (weight, n, kn, nn)
-__device__ float mixture_integral(const VMMParams& V, float sx, float sy, float sz, float weight, float nx, float ny,
-                                  float nz, float kn, float nn) {
+__device__ float mixture_integral(const VMMParams& V, bool par, float sx, float sy, float sz, float weight, float nx,
+                                  float ny, float nz, float kn, float nn) {
   float integral = 0.f;
   for (int k = 0; k < V.K; k++) {
     float mx, my, mz, d;
-    shift_comp(V, k, sx, sy, sz, mx, my, mz, d);
+    shift_comp(V, k, sx, sy, sz, mx, my, mz, d, par);
     const ProdComp p = product_comp(V.w[k], V.kappa[k], mx, my, mz, V.norm[k], weight, nx, ny, nz, kn, nn);
     integral += p.scale;
   }
   return integral;
 }
 
-__device__ void mixture_sample(const VMMParams& V, float sx, float sy, float sz, float weight, float nx, float ny,
-                               float nz, float kn, float nn, float integral, float u0, float u1, float* out) {
+__device__ void mixture_sample(const VMMParams& V, bool par, float sx, float sy, float sz, float weight, float nx,
+                               float ny, float nz, float kn, float nn, float integral, float u0, float u1, float* out) {
   float sum = 0.f;
   ProdComp sel;
   float cdf = 0.f;
   for (int k = 0; k < V.K; k++) {
     float mx, my, mz, d;
-    shift_comp(V, k, sx, sy, sz, mx, my, mz, d);
+    shift_comp(V, k, sx, sy, sz, mx, my, mz, d, par);
     sel = product_comp(V.w[k], V.kappa[k], mx, my, mz, V.norm[k], weight, nx, ny, nz, kn, nn);
     cdf = sel.scale / integral;
     if (sum + cdf >= u1 || k + 1 >= V.K) break;
@@ -76,13 +76,13 @@ __device__ void mixture_sample(const VMMParams& V, float sx, float sy, float sz,
   sample_comp(sel.kappa, sel.em2k, sel.mx, sel.my, sel.mz, u0, us, out);
 }
 
-__device__ float mixture_pdf(const VMMParams& V, float sx, float sy, float sz, float weight, float nx, float ny,
-                             float nz, float kn, float nn, float integral, float dx, float dy, float dz) {
+__device__ float mixture_pdf(const VMMParams& V, bool par, float sx, float sy, float sz, float weight, float nx,
+                             float ny, float nz, float kn, float nn, float integral, float dx, float dy, float dz) {
   float pdf = 0.f;
   const float inv = 1.f / integral;
   for (int k = 0; k < V.K; k++) {
     float mx, my, mz, d;
-    shift_comp(V, k, sx, sy, sz, mx, my, mz, d);
+    shift_comp(V, k, sx, sy, sz, mx, my, mz, d, par);
     const ProdComp p = product_comp(V.w[k], V.kappa[k], mx, my, mz, V.norm[k], weight, nx, ny, nz, kn, nn);
     pdf += eval_comp(p.scale * inv, p.norm, p.kappa, p.mx, p.my, p.mz, dx, dy, dz);
   }
@@ -107,12 +107,13 @@ __device__ __forceinline__ void query_one(const QueryField& surf, const QueryFie
     return;
   }
   const VMMParams& V = f.regions[r].vmm;
+  const bool par = f.parallax != 0;
   const float sx = V.pivot[0] - x, sy = V.pivot[1] - y, sz = V.pivot[2] - z;
   if (!volume) {
     const float kn = kCosineKappa, nn = cosine_normalization();
-    integral = mixture_integral(V, sx, sy, sz, 1.f, nx, ny, nz, kn, nn);
-    mixture_sample(V, sx, sy, sz, 1.f, nx, ny, nz, kn, nn, integral, ux, uy, out);
-    pdf = mixture_pdf(V, sx, sy, sz, 1.f, nx, ny, nz, kn, nn, integral, out[0], out[1], out[2]);
+    integral = mixture_integral(V, par, sx, sy, sz, 1.f, nx, ny, nz, kn, nn);
+    mixture_sample(V, par, sx, sy, sz, 1.f, nx, ny, nz, kn, nn, integral, ux, uy, out);
+    pdf = mixture_pdf(V, par, sx, sy, sz, 1.f, nx, ny, nz, kn, nn, integral, out[0], out[1], out[2]);
   } else {
     const HGRep& rep = hg[hg_index(g)];
     const int KP = (int)rep.K;
@@ -120,7 +121,7 @@ __device__ __forceinline__ void query_one(const QueryField& surf, const QueryFie
     float sum = 0.f;
     for (int m = 0; m < KP; m++) {
       sgn[m] = (g * rep.mc[m] > 0.f) ? 1.f : -1.f;
-      integ[m] = mixture_integral(V, sx, sy, sz, rep.w[m], nx * sgn[m], ny * sgn[m], nz * sgn[m], rep.kappa[m], rep.norm[m]);
+      integ[m] = mixture_integral(V, par, sx, sy, sz, rep.w[m], nx * sgn[m], ny * sgn[m], nz * sgn[m], rep.kappa[m], rep.norm[m]);
       sum += integ[m];
     }
     float mw[4];
@@ -132,10 +133,10 @@ __device__ __forceinline__ void query_one(const QueryField& surf, const QueryFie
       if (nw > ux) break;
       acc = nw;
     }
-    mixture_sample(V, sx, sy, sz, rep.w[m], nx * sgn[m], ny * sgn[m], nz * sgn[m], rep.kappa[m], rep.norm[m], integ[m],
+    mixture_sample(V, par, sx, sy, sz, rep.w[m], nx * sgn[m], ny * sgn[m], nz * sgn[m], rep.kappa[m], rep.norm[m], integ[m],
                    (ux - acc) / mw[m], uy, out);
     for (int j = 0; j < KP; j++)
-      pdf += mw[j] * mixture_pdf(V, sx, sy, sz, rep.w[j], nx * sgn[j], ny * sgn[j], nz * sgn[j], rep.kappa[j],
+      pdf += mw[j] * mixture_pdf(V, par, sx, sy, sz, rep.w[j], nx * sgn[j], ny * sgn[j], nz * sgn[j], rep.kappa[j],
                                  rep.norm[j], integ[j], out[0], out[1], out[2]);
     integral = sum;
   }

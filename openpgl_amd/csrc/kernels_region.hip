@@ -1304,36 +1304,39 @@ __device__ void region_job(Shared& sh, Team& tm, RegionState* __restrict__ state
   if (n > 0) {
     if (mode == 2) {
       // parallax shift of the statistics and of the mixture to the new pivot
-      // (WEM:265-283, PAVMM:809-831)
-      const float shx = S.vmm.pivot[0] - S.ssMean[0], shy = S.vmm.pivot[1] - S.ssMean[1],
-                  shz = S.vmm.pivot[2] - S.ssMean[2];
-      if (tid < 32) {
-        const int k = tid;
-        VMMParams& V = S.vmm;
-        if (k < V.K) {
-          const float dk = V.dist[k];
-          if (dk > 0.f) {
-            float sx = S.suff.swdx[k], sy = S.suff.swdy[k], sz = S.suff.swdz[k];
-            const float mcos = sqrtf(dot3f(sx, sy, sz, sx, sy, sz));
-            if (mcos > 0.f) {
-              sx = (sx / mcos) * dk + shx;
-              sy = (sy / mcos) * dk + shy;
-              sz = (sz / mcos) * dk + shz;
-              const float l = sqrtf(dot3f(sx, sy, sz, sx, sy, sz));
-              S.suff.swdx[k] = (sx / l) * mcos;
-              S.suff.swdy[k] = (sy / l) * mcos;
-              S.suff.swdz[k] = (sz / l) * mcos;
+      // (WEM:265-283, PAVMM:809-831); the plain VMM keeps its fit-time pivot
+      // (Field.h:476)
+      if (!cfg.noParallax) {
+        const float shx = S.vmm.pivot[0] - S.ssMean[0], shy = S.vmm.pivot[1] - S.ssMean[1],
+                    shz = S.vmm.pivot[2] - S.ssMean[2];
+        if (tid < 32) {
+          const int k = tid;
+          VMMParams& V = S.vmm;
+          if (k < V.K) {
+            const float dk = V.dist[k];
+            if (dk > 0.f) {
+              float sx = S.suff.swdx[k], sy = S.suff.swdy[k], sz = S.suff.swdz[k];
+              const float mcos = sqrtf(dot3f(sx, sy, sz, sx, sy, sz));
+              if (mcos > 0.f) {
+                sx = (sx / mcos) * dk + shx;
+                sy = (sy / mcos) * dk + shy;
+                sz = (sz / mcos) * dk + shz;
+                const float l = sqrtf(dot3f(sx, sy, sz, sx, sy, sz));
+                S.suff.swdx[k] = (sx / l) * mcos;
+                S.suff.swdy[k] = (sy / l) * mcos;
+                S.suff.swdz[k] = (sz / l) * mcos;
+              }
             }
+            const float px = V.mux[k] * dk + shx, py = V.muy[k] * dk + shy, pz = V.muz[k] * dk + shz;
+            const float l = sqrtf(dot3f(px, py, pz, px, py, pz));
+            if (dk > 0.f && isfinite(dk)) { V.mux[k] = px / l; V.muy[k] = py / l; V.muz[k] = pz / l; }
+            V.dist[k] = dk > 0.f ? l : 0.f;
           }
-          const float px = V.mux[k] * dk + shx, py = V.muy[k] * dk + shy, pz = V.muz[k] * dk + shz;
-          const float l = sqrtf(dot3f(px, py, pz, px, py, pz));
-          if (dk > 0.f && isfinite(dk)) { V.mux[k] = px / l; V.muy[k] = py / l; V.muz[k] = pz / l; }
-          V.dist[k] = dk > 0.f ? l : 0.f;
         }
+        __syncthreads();
+        if (tid == 0) { S.vmm.pivot[0] -= shx; S.vmm.pivot[1] -= shy; S.vmm.pivot[2] -= shz; }
+        __syncthreads();
       }
-      __syncthreads();
-      if (tid == 0) { S.vmm.pivot[0] -= shx; S.vmm.pivot[1] -= shy; S.vmm.pivot[2] -= shz; }
-      __syncthreads();
       region_update_body(sh, tm, rs, base, n, cfg);
       const bool ok = region_valid(sh, true);
       if (tid == 0) S.valid = ok ? 1 : 0;

@@ -305,20 +305,22 @@ __global__ void __launch_bounds__(256) k_region_ranges(const uint32_t* __restric
 // Gather into region-contiguous SoA and reproject each sample to its region's
 // pivot: origin = p + d * max(minDist, dist); d' = normalise(origin - pivot).
 // distance == inf keeps the direction (position moves to the pivot);
-// distance <= 0 keeps the sample unchanged.
+// distance <= 0 keeps the sample unchanged.  Without reprojection (plain VMM
+// field, WEM:1186) the samples are gathered as recorded.
 __global__ void __launch_bounds__(256) k_gather_reproject(const SampleAoS* __restrict__ in,
                                                           const uint32_t* __restrict__ sortedKeys,
                                                           const uint32_t* __restrict__ sortedIdx, uint32_t n,
                                                           const float4* __restrict__ regionPivot /* xyz, minDist */,
                                                           float* __restrict__ odx, float* __restrict__ ody,
                                                           float* __restrict__ odz, float* __restrict__ ow,
-                                                          float* __restrict__ opdf, float* __restrict__ odist) {
+                                                          float* __restrict__ opdf, float* __restrict__ odist,
+                                                          int reproject) {
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
     const SampleAoS s = in[sortedIdx[j]];
     const float4 pv = regionPivot[sortedKeys[j]];
     float dx = s.dx, dy = s.dy, dz = s.dz, dist = s.dist;
-    if (!isinf(s.dist) && s.dist > 0.f) {
+    if (reproject && !isinf(s.dist) && s.dist > 0.f) {
       const float dd = fmaxf(pv.w, s.dist);
       const float ox = s.px + s.dx * dd - pv.x;
       const float oy = s.py + s.dy * dd - pv.y;
@@ -491,9 +493,9 @@ void launch_region_keys(const uint32_t* begin, uint32_t numRegions, uint32_t n, 
 
 void launch_gather_reproject(const void* aos, const uint32_t* sortedKeys, const uint32_t* sortedIdx, uint32_t n,
                              const float4* regionPivot, float* dx, float* dy, float* dz, float* w, float* pdf,
-                             float* dist, hipStream_t s) {
+                             float* dist, bool reproject, hipStream_t s) {
   hipLaunchKernelGGL(k_gather_reproject, dim3(grid_for(n, 256)), dim3(256), 0, s, (const SampleAoS*)aos, sortedKeys,
-                     sortedIdx, n, regionPivot, dx, dy, dz, w, pdf, dist);
+                     sortedIdx, n, regionPivot, dx, dy, dz, w, pdf, dist, reproject ? 1 : 0);
 }
 
 void launch_knn8(const float4* pivots, uint32_t numRegions, uint32_t* nbIds, float4* nbPts, hipStream_t s) {

@@ -32,13 +32,18 @@ PGL_HD inline float cosine_normalization() {
 }
 
 // component k of a region's mixture after the parallax shift to a query
-// position (shift = pivot - position)
+// position (shift = pivot - position); unshifted for the plain VMM field
+// (VMMSurfaceSamplingDistribution.h:41, VMMVolumeSamplingDistribution.h:43)
 PGL_HD inline void shift_comp(const VMMParams& V, int k, float sx, float sy, float sz, float& mx, float& my,
-                              float& mz, float& dist) {
+                              float& mz, float& dist, bool parallax = true) {
   const float d = V.dist[k];
   mx = V.mux[k];
   my = V.muy[k];
   mz = V.muz[k];
+  if (!parallax) {
+    dist = d;
+    return;
+  }
   const float px = mx * d + sx, py = my * d + sy, pz = mz * d + sz;
   const float l = sqrtf(qdot(px, py, pz, px, py, pz));
   if (d > 0.f && isfinite(d)) {

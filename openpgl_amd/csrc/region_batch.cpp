@@ -74,7 +74,7 @@ void RegionBatch::step(uint32_t R, const void* records, const uint32_t* counts, 
   launch_region_keys(dBegin_.as<uint32_t>(), R, n, dKeys_.as<uint32_t>(), dVals_.as<uint32_t>(), s);
   launch_gather_reproject(records, dKeys_.as<uint32_t>(), dVals_.as<uint32_t>(), n, dPivots_.as<float4>(),
                           dDx_.as<float>(), dDy_.as<float>(), dDz_.as<float>(), dW_.as<float>(), dPdf_.as<float>(),
-                          dDist_.as<float>(), s);
+                          dDist_.as<float>(), fc_.noParallax == 0, s);
   PGL_HIP_CHECK(hipMemcpyAsync(dSnapshot_.p, dStates_.p, size_t(R) * sizeof(RegionState), hipMemcpyDeviceToDevice, s));
   PGL_HIP_CHECK(hipEventRecord(ev_[0], s));
   runner_.launch(dStates_.as<RegionState>(), work,

@@ -28,7 +28,7 @@
 //
 // How each part is pinned (tests/):
 //   * IntegerSampleStatistics / rcp: golden vectors written by the reference's own
-//     SampleStatistics.h compiled in oracle/_ref (test_oracle_ref.py), plus closed forms.
+//     SampleStatistics.h compiled in oracle/_ref (tests/test_oracle.py), plus closed forms.
 //   * kd build: unbuildable reference (needs oneTBB headers); pinned by invariants
 //     (partition/lookup agreement, leaf capacity, split = mean of argmax variance of
 //     the merged statistics, brute-force recomputation of every leaf's statistics).
@@ -36,6 +36,9 @@
 //   * directional fit/update, products, sample, pdf: the reference's own directional
 //     headers compiled in oracle/_ref (golden KATs in tests/golden), K=1 closed-form
 //     MLE, FP64 twin agreement, integral of pdf = 1.
+//   * plain VMM field type (orc_config.parallax = 0): the reference's
+//     ParallaxAwareVonMisesFisherMixture<8, 32, false> path compiled in oracle/_ref
+//     (tests/golden/region_kat_np.npz and a live comparison).
 #include "pgl_oracle.h"
 
 #include <immintrin.h>
@@ -1053,7 +1056,8 @@ static bool sampleLess(const orc_sample& a, const orc_sample& b) {  // SampleDat
 
 // reprojectSample / prepareSamples (WEM:1151-1196)
 template <class R>
-static void prepare(const orc_sample* in, size_t n, const SampleStats& ss, std::vector<PreparedSample<R>>& out) {
+static void prepare(const orc_sample* in, size_t n, const SampleStats& ss, std::vector<PreparedSample<R>>& out,
+                    bool parallax) {
   out.resize(n);
   float var[3];
   ss.getVariance(var);
@@ -1066,6 +1070,7 @@ static void prepare(const orc_sample* in, size_t n, const SampleStats& ss, std::
     o.w = s.weight; o.pdf = s.pdf;
     o.d[0] = s.dx; o.d[1] = s.dy; o.d[2] = s.dz;
     o.dist = s.distance;
+    if (!parallax) continue;  // the plain VMM trains on the samples as recorded
     if (std::isinf(s.distance)) continue;
     if (!(s.distance > 0.f)) continue;
     const R dd = std::max(R(minDist), R(s.distance));
@@ -1247,12 +1252,12 @@ template <class R> struct Field {
       if (cfg.deterministic) std::sort(samples.begin() + reg.begin, samples.begin() + reg.end, sampleLess);
       const float sm[3] = {reg.ss.mean[0], reg.ss.mean[1], reg.ss.mean[2]};
       std::vector<PreparedSample<R>> ps;
-      if (!fit) {
+      if (!fit && cfg.parallax) {
         const R shift[3] = {reg.vmm.pivot[0] - R(sm[0]), reg.vmm.pivot[1] - R(sm[1]), reg.vmm.pivot[2] - R(sm[2])};
         suffShift(reg.ts.suff, reg.vmm, shift);
         vmmShift(reg.vmm, shift);
       }
-      prepare<R>(samples.data() + reg.begin, n, reg.ss, ps);
+      prepare<R>(samples.data() + reg.begin, n, reg.ss, ps, cfg.parallax != 0);
       if (fit) {
         factory.fit(reg.vmm, reg.ts, ps.data(), n, &reg.info);
         for (int j = 0; j < 3; j++) reg.vmm.pivot[j] = R(sm[j]);
@@ -1449,8 +1454,10 @@ template <class R> static bool initQuery(const Field<R>& f, const float pos[3], 
   const Region<R>* reg = f.getRegion(pos, u, id);
   if (!reg || !reg->valid) return false;
   q.li = reg->vmm;
-  const R shift[3] = {q.li.pivot[0] - R(pos[0]), q.li.pivot[1] - R(pos[1]), q.li.pivot[2] - R(pos[2])};
-  vmmShift(q.li, shift);
+  if (f.cfg.parallax) {
+    const R shift[3] = {q.li.pivot[0] - R(pos[0]), q.li.pivot[1] - R(pos[1]), q.li.pivot[2] - R(pos[2])};
+    vmmShift(q.li, shift);
+  }
   q.d[0] = q.li;
   q.weights[0] = 1;
   q.num = 1;
@@ -1572,6 +1579,7 @@ void orc_config_defaults(orc_config* c, uint32_t maxSamplesPerLeaf) {
   c->minSamplesForPartialRefitting = int32_t(maxSamplesPerLeaf / 8);
   c->minSamplesForMerging = int32_t(maxSamplesPerLeaf / 4);
   c->decayOnSpatialSplit = 0.25f; c->deterministic = 1; c->vectorWidth = 8;
+  c->parallax = 1;
 }
 
 orc_field orc_field_new(const orc_config* cfg, int precision) {
@@ -1716,7 +1724,7 @@ static void regionStep(Region<R>& reg, const Factory<R>& fac, bool& first, const
   std::vector<PreparedSample<R>> ps;
   reg.info.numSamplesThisUpdate = uint32_t(n);
   if (first) {
-    prepare<R>(sorted.data(), n, reg.ss, ps);
+    prepare<R>(sorted.data(), n, reg.ss, ps, fac.c.parallax != 0);
     fac.fit(reg.vmm, reg.ts, ps.data(), n, &reg.info);
     for (int j = 0; j < 3; j++) reg.vmm.pivot[j] = R(m[j]);
     reg.valid = vmmValid(reg.vmm);
@@ -1731,10 +1739,12 @@ static void regionStep(Region<R>& reg, const Factory<R>& fac, bool& first, const
       const int lim = std::min(MAXK, ((sp.K + 7) / 8) * 8);
       for (int k = 0; k < lim; k++) { sp.cov[k][0] *= a; sp.cov[k][1] *= a; sp.cov[k][2] *= a; sp.n[k] *= a; sp.sw[k] *= a; sp.sa[k] *= a; }
     }
-    const R shift[3] = {reg.vmm.pivot[0] - R(m[0]), reg.vmm.pivot[1] - R(m[1]), reg.vmm.pivot[2] - R(m[2])};
-    suffShift(reg.ts.suff, reg.vmm, shift);
-    vmmShift(reg.vmm, shift);
-    prepare<R>(sorted.data(), n, reg.ss, ps);
+    if (fac.c.parallax) {
+      const R shift[3] = {reg.vmm.pivot[0] - R(m[0]), reg.vmm.pivot[1] - R(m[1]), reg.vmm.pivot[2] - R(m[2])};
+      suffShift(reg.ts.suff, reg.vmm, shift);
+      vmmShift(reg.vmm, shift);
+    }
+    prepare<R>(sorted.data(), n, reg.ss, ps, fac.c.parallax != 0);
     fac.update(reg.vmm, reg.ts, ps.data(), n, &reg.info);
     reg.valid = vmmValid(reg.vmm) && suffValid(reg.ts.suff);
   }
@@ -1764,12 +1774,14 @@ float orc_region_pdf(orc_region r, const float* d) {
 }  // extern "C"
 
 template <class R>
-static float regionQuery(const Region<R>& reg, int volume, const float* pos, const float* nrm, float g, const float* u2,
-                         const float* pdfDir, float* pdfAtDir, float* dirOut, float* pdfOut) {
+static float regionQuery(const Region<R>& reg, bool parallax, int volume, const float* pos, const float* nrm, float g,
+                         const float* u2, const float* pdfDir, float* pdfAtDir, float* dirOut, float* pdfOut) {
   QueryDist<R> q;
   q.li = reg.vmm;
-  const R shift[3] = {q.li.pivot[0] - R(pos[0]), q.li.pivot[1] - R(pos[1]), q.li.pivot[2] - R(pos[2])};
-  vmmShift(q.li, shift);
+  if (parallax) {
+    const R shift[3] = {q.li.pivot[0] - R(pos[0]), q.li.pivot[1] - R(pos[1]), q.li.pivot[2] - R(pos[2])};
+    vmmShift(q.li, shift);
+  }
   q.d[0] = q.li; q.weights[0] = 1; q.num = 1; q.integral = 1;
   if (volume) applyHG(q, nrm, g); else applyCosine(q, nrm);
   const R pd[3] = {R(pdfDir[0]), R(pdfDir[1]), R(pdfDir[2])};
@@ -1785,8 +1797,9 @@ extern "C" {
 
 float orc_region_query(orc_region r, int volume, const float* pos, const float* nrm, float g, const float* u2,
                        const float* pdfDir, float* pdfAtDir, float* dirOut, float* pdfOut) {
-  if (r->precision == 0) return regionQuery(r->r32, volume, pos, nrm, g, u2, pdfDir, pdfAtDir, dirOut, pdfOut);
-  return regionQuery(r->r64, volume, pos, nrm, g, u2, pdfDir, pdfAtDir, dirOut, pdfOut);
+  const bool par = r->cfg.parallax != 0;
+  if (r->precision == 0) return regionQuery(r->r32, par, volume, pos, nrm, g, u2, pdfDir, pdfAtDir, dirOut, pdfOut);
+  return regionQuery(r->r64, par, volume, pos, nrm, g, u2, pdfDir, pdfAtDir, dirOut, pdfOut);
 }
 
 void orc_stats_merge_split(const float* a, const float* b, int dim, float pos, float* m, float* l, float* r) {

@@ -51,6 +51,11 @@ typedef struct {
   float decayOnSpatialSplit; uint32_t deterministic;
   // merge loop stops at K <= vectorWidth (VMMChiSquareComponentMerger.h:72)
   uint32_t vectorWidth;
+  // 1: parallax-aware mixture (PGL_DIRECTIONAL_DISTRIBUTION_PARALLAX_AWARE_VMM);
+  // 0: plain VMM (PGL_DIRECTIONAL_DISTRIBUTION_VMM, Device.h:139-184): no
+  // reprojection (WEM:1186), no shift before updates (Field.h:476) or queries
+  // (VMMSurfaceSamplingDistribution.h:41, VMMVolumeSamplingDistribution.h:43)
+  uint32_t parallax;
 } orc_config;
 
 // flat per-region export record used by tests (canonical DFS order)

@@ -31,21 +31,27 @@
 
 using namespace openpgl;
 
-typedef ParallaxAwareVonMisesFisherMixture<8, 32, true> VMM8;
-typedef AdaptiveSplitAndMergeFactory<VMM8> ASM8;
-
-struct RefRegion {
-  VMM8 vmm;
-  ASM8::Statistics stats;
+// P = parallax compensation: the reference's PARALLAX_AWARE_VMM (true) and
+// VMM (false) field types (device/Device.h:93-184)
+template <bool P>
+struct RefRegionT {
+  typedef ParallaxAwareVonMisesFisherMixture<8, 32, P> VMM;
+  typedef AdaptiveSplitAndMergeFactory<VMM> ASM;
+  VMM vmm;
+  typename ASM::Statistics stats;
   SampleStatistics ss;
-  ASM8::Configuration cfg;
+  typename ASM::Configuration cfg;
   bool first = true;
   bool valid = false;
-  ASM8::FittingStatistics fit;
-  RefRegion() : vmm(), stats(), ss() {}
+  typename ASM::FittingStatistics fit;
+  RefRegionT() : vmm(), stats(), ss() {}
 };
+typedef RefRegionT<true> RefRegion;
+typedef RefRegion::VMM VMM8;
+typedef RefRegion::ASM ASM8;
 
-static void toCfg(const orc_config* c, ASM8::Configuration& a) {
+template <class Cfg>
+static void toCfg(const orc_config* c, Cfg& a) {
   a.weightedEMCfg.initK = c->initK;
   a.weightedEMCfg.initKappa = c->initKappa;
   a.weightedEMCfg.maxK = c->maxK;
@@ -66,6 +72,141 @@ static void toCfg(const orc_config* c, ASM8::Configuration& a) {
   a.minSamplesForMerging = c->minSamplesForMerging;
 }
 
+
+template <bool P>
+static void* region_new(const orc_config* c) {
+  static bool hgInit = false;
+  if (!hgInit) { VMMSingleLobeHenyeyGreensteinOracle::init(); hgInit = true; }
+  void* mem = nullptr;
+  if (posix_memalign(&mem, 64, sizeof(RefRegionT<P>)) != 0) return nullptr;
+  RefRegionT<P>* r = new (mem) RefRegionT<P>();  // vfloat<8> members need 32-byte alignment
+  toCfg(c, r->cfg);
+  return r;
+}
+template <bool P>
+static void region_release(void* p) {
+  ((RefRegionT<P>*)p)->~RefRegionT<P>();
+  free(p);
+}
+
+// the per-region body of Field::fitRegions / Field::updateRegions
+// (field/Field.h:405-438, 455-511)
+template <bool P>
+static void region_step(void* p, const orc_sample* s, size_t n, const float* m, const float* v, float ns, int splitFlag) {
+  typedef RefRegionT<P> RR;
+  RR* r = (RR*)p;
+  r->ss.mean = Point3(m[0], m[1], m[2]);
+  r->ss.sampleVariance = Vector3(v[0], v[1], v[2]);
+  r->ss.numSamples = ns;
+  std::vector<SampleData> data(n);
+  memcpy(data.data(), s, n * sizeof(SampleData));
+  std::sort(data.begin(), data.end(), SampleDataLess);
+  typename RR::ASM factory;
+  r->fit = typename RR::ASM::FittingStatistics();
+  const Point3 sampleMean = r->ss.mean;
+  if (r->first) {
+    factory.prepareSamples(data.data(), n, r->ss, r->cfg);
+    factory.fit(r->vmm, r->stats, data.data(), n, r->cfg, r->fit);
+    r->vmm._pivotPosition = sampleMean;
+    r->valid = r->vmm.isValid();
+    r->first = false;
+  } else {
+    if (splitFlag) r->stats.decay(0.25f);
+    if (RR::VMM::ParallaxCompensation == 1) {
+      r->stats.sufficientStatistics.applyParallaxShift(r->vmm, r->vmm._pivotPosition - sampleMean);
+      r->vmm.performRelativeParallaxShift(r->vmm._pivotPosition - sampleMean);
+    }
+    factory.prepareSamples(data.data(), n, r->ss, r->cfg);
+    factory.update(r->vmm, r->stats, data.data(), n, r->cfg, r->fit);
+    r->valid = r->vmm.isValid() && r->stats.isValid();
+  }
+}
+
+template <bool P>
+static void region_export(void* p, orc_region_record* o) {
+  RefRegionT<P>* r = (RefRegionT<P>*)p;
+  memset(o, 0, sizeof(*o));
+  const typename RefRegionT<P>::VMM& v = r->vmm;
+  o->K = v._numComponents;
+  const auto& su = r->stats.sufficientStatistics;
+  const auto& sp = r->stats.splittingStatistics;
+  for (int k = 0; k < 32; k++) {
+    const int q = k / 8, l = k % 8;
+    o->weights[k] = v._weights[q][l];
+    o->kappas[k] = v._kappas[q][l];
+    o->meanDirections[k][0] = v._meanDirections[q].x[l];
+    o->meanDirections[k][1] = v._meanDirections[q].y[l];
+    o->meanDirections[k][2] = v._meanDirections[q].z[l];
+    o->normalizations[k] = v._normalizations[q][l];
+    o->eMinus2Kappa[k] = v._eMinus2Kappa[q][l];
+    o->meanCosines[k] = v._meanCosines[q][l];
+    o->distances[k] = v._distances[q][l];
+    o->sumOfWeightedDirections[k][0] = su.sumOfWeightedDirections[q].x[l];
+    o->sumOfWeightedDirections[k][1] = su.sumOfWeightedDirections[q].y[l];
+    o->sumOfWeightedDirections[k][2] = su.sumOfWeightedDirections[q].z[l];
+    o->sumOfWeightedStats[k] = su.sumOfWeightedStats[q][l];
+    o->sumOfDistanceWeights[k] = su.sumOfDistanceWeightes[q][l];
+    o->chiSquareMCEstimates[k] = sp.chiSquareMCEstimates[q][l];
+    o->splitMeans[k][0] = sp.splitMeans[q].x[l];
+    o->splitMeans[k][1] = sp.splitMeans[q].y[l];
+    o->splitCovariances[k][0] = sp.splitWeightedSampleCovariances[q].x[l];
+    o->splitCovariances[k][1] = sp.splitWeightedSampleCovariances[q].y[l];
+    o->splitCovariances[k][2] = sp.splitWeightedSampleCovariances[q].z[l];
+    o->splitNumSamples[k] = sp.numSamples[q][l];
+    o->splitSumWeights[k] = sp.sumWeights[q][l];
+    o->splitSumAssignedSamples[k] = sp.sumAssignedSamples[q][l];
+  }
+  o->pivot[0] = v._pivotPosition.x; o->pivot[1] = v._pivotPosition.y; o->pivot[2] = v._pivotPosition.z;
+  o->sumWeights = su.sumWeights; o->numSamples = su.numSamples; o->overallNumSamples = su.overallNumSamples;
+  o->suffK = su.numComponents; o->normalized = su.normalized;
+  o->splitK = sp.numComponents;
+  o->numSamplesAfterLastSplit = r->stats.numSamplesAfterLastSplit;
+  o->numSamplesAfterLastMerge = r->stats.numSamplesAfterLastMerge;
+  for (int j = 0; j < 3; j++) {
+    o->ssMean[j] = r->ss.mean[j]; o->ssVariance[j] = r->ss.sampleVariance[j];
+  }
+  o->ssNumSamples = r->ss.numSamples;
+  o->valid = r->valid;
+  o->numSplits = r->fit.numSplits;
+  o->numMerges = r->fit.numMerges;
+  o->emIterations = r->fit.numUpdateWEMIterations;
+  o->partialEmIterations = r->fit.numPartialUpdateWEMIterations;
+}
+
+template <bool P>
+static float region_pdf(void* p, const float* d) {
+  RefRegionT<P>* r = (RefRegionT<P>*)p;
+  return r->vmm.pdf(Vector3(d[0], d[1], d[2]));
+}
+
+// init at pos (parallax shift) + cosine (surface) or HG (volume) product, then
+// pdf(dir) and sample(u) -> dirOut/pdfOut.  Returns the product integral.
+template <bool P>
+static float region_query(void* p, int volume, const float* pos, const float* nrm, float g, const float* u2, const float* pdfDir,
+                       float* pdfAtDir, float* dirOut, float* pdfOut) {
+  RefRegionT<P>* r = (RefRegionT<P>*)p;
+  float integral = 1.f;
+  if (volume) {
+    VMMVolumeSamplingDistribution<typename RefRegionT<P>::VMM, P> d;
+    d.init(&r->vmm, Point3(pos[0], pos[1], pos[2]));
+    d.applySingleLobeHenyeyGreensteinProduct(Vector3(nrm[0], nrm[1], nrm[2]), g);
+    integral = d.m_productIntegral;
+    *pdfAtDir = d.pdf(Vector3(pdfDir[0], pdfDir[1], pdfDir[2]));
+    Vector3 o = d.sample(Point2(u2[0], u2[1]));
+    dirOut[0] = o.x; dirOut[1] = o.y; dirOut[2] = o.z;
+    *pdfOut = d.pdf(o);
+  } else {
+    VMMSurfaceSamplingDistribution<typename RefRegionT<P>::VMM, P> d;
+    d.init(&r->vmm, Point3(pos[0], pos[1], pos[2]));
+    d.applyCosineProduct(Vector3(nrm[0], nrm[1], nrm[2]));
+    integral = d.m_productIntegral;
+    *pdfAtDir = d.pdf(Vector3(pdfDir[0], pdfDir[1], pdfDir[2]));
+    Vector3 o = d.sample(Point2(u2[0], u2[1]));
+    dirOut[0] = o.x; dirOut[1] = o.y; dirOut[2] = o.z;
+    *pdfOut = d.pdf(o);
+  }
+  return integral;
+}
 
 // ---------------------------------------------------------------- field-file records
 // A storage entry of Field::m_regionStorageContainer (Region + Range) of a
@@ -244,45 +385,15 @@ static void* file_region_deserialize(const char* buf, size_t n) {
 
 extern "C" {
 
-void* ref_region_new(const orc_config* c) {
-  static bool hgInit = false;
-  if (!hgInit) { VMMSingleLobeHenyeyGreensteinOracle::init(); hgInit = true; }
-  void* mem = nullptr;
-  if (posix_memalign(&mem, 64, sizeof(RefRegion)) != 0) return nullptr;
-  RefRegion* r = new (mem) RefRegion();  // vfloat<8> members need 32-byte alignment
-  toCfg(c, r->cfg);
-  return r;
-}
-void ref_region_release(void* p) {
-  ((RefRegion*)p)->~RefRegion();
-  free(p);
-}
-
+void* ref_region_new(const orc_config* c) { return region_new<true>(c); }
+void ref_region_release(void* p) { region_release<true>(p); }
 void ref_region_step(void* p, const orc_sample* s, size_t n, const float* m, const float* v, float ns, int splitFlag) {
-  RefRegion* r = (RefRegion*)p;
-  r->ss.mean = Point3(m[0], m[1], m[2]);
-  r->ss.sampleVariance = Vector3(v[0], v[1], v[2]);
-  r->ss.numSamples = ns;
-  std::vector<SampleData> data(n);
-  memcpy(data.data(), s, n * sizeof(SampleData));
-  std::sort(data.begin(), data.end(), SampleDataLess);
-  ASM8 factory;
-  r->fit = ASM8::FittingStatistics();
-  const Point3 sampleMean = r->ss.mean;
-  if (r->first) {
-    factory.prepareSamples(data.data(), n, r->ss, r->cfg);
-    factory.fit(r->vmm, r->stats, data.data(), n, r->cfg, r->fit);
-    r->vmm._pivotPosition = sampleMean;
-    r->valid = r->vmm.isValid();
-    r->first = false;
-  } else {
-    if (splitFlag) r->stats.decay(0.25f);
-    r->stats.sufficientStatistics.applyParallaxShift(r->vmm, r->vmm._pivotPosition - sampleMean);
-    r->vmm.performRelativeParallaxShift(r->vmm._pivotPosition - sampleMean);
-    factory.prepareSamples(data.data(), n, r->ss, r->cfg);
-    factory.update(r->vmm, r->stats, data.data(), n, r->cfg, r->fit);
-    r->valid = r->vmm.isValid() && r->stats.isValid();
-  }
+  region_step<true>(p, s, n, m, v, ns, splitFlag);
+}
+void* ref_region_np_new(const orc_config* c) { return region_new<false>(c); }
+void ref_region_np_release(void* p) { region_release<false>(p); }
+void ref_region_np_step(void* p, const orc_sample* s, size_t n, const float* m, const float* v, float ns, int splitFlag) {
+  region_step<false>(p, s, n, m, v, ns, splitFlag);
 }
 
 // ASM::fit stopped after a phase: 1 EM + distances, 2 recursive splitting,
@@ -332,87 +443,17 @@ float ref_region_merge_cost(void* p, int i, int j) {
   return merger.CalculateMergeCost(r->vmm, i, j);
 }
 
-void ref_region_export(void* p, orc_region_record* o) {
-  RefRegion* r = (RefRegion*)p;
-  memset(o, 0, sizeof(*o));
-  const VMM8& v = r->vmm;
-  o->K = v._numComponents;
-  const auto& su = r->stats.sufficientStatistics;
-  const auto& sp = r->stats.splittingStatistics;
-  for (int k = 0; k < 32; k++) {
-    const int q = k / 8, l = k % 8;
-    o->weights[k] = v._weights[q][l];
-    o->kappas[k] = v._kappas[q][l];
-    o->meanDirections[k][0] = v._meanDirections[q].x[l];
-    o->meanDirections[k][1] = v._meanDirections[q].y[l];
-    o->meanDirections[k][2] = v._meanDirections[q].z[l];
-    o->normalizations[k] = v._normalizations[q][l];
-    o->eMinus2Kappa[k] = v._eMinus2Kappa[q][l];
-    o->meanCosines[k] = v._meanCosines[q][l];
-    o->distances[k] = v._distances[q][l];
-    o->sumOfWeightedDirections[k][0] = su.sumOfWeightedDirections[q].x[l];
-    o->sumOfWeightedDirections[k][1] = su.sumOfWeightedDirections[q].y[l];
-    o->sumOfWeightedDirections[k][2] = su.sumOfWeightedDirections[q].z[l];
-    o->sumOfWeightedStats[k] = su.sumOfWeightedStats[q][l];
-    o->sumOfDistanceWeights[k] = su.sumOfDistanceWeightes[q][l];
-    o->chiSquareMCEstimates[k] = sp.chiSquareMCEstimates[q][l];
-    o->splitMeans[k][0] = sp.splitMeans[q].x[l];
-    o->splitMeans[k][1] = sp.splitMeans[q].y[l];
-    o->splitCovariances[k][0] = sp.splitWeightedSampleCovariances[q].x[l];
-    o->splitCovariances[k][1] = sp.splitWeightedSampleCovariances[q].y[l];
-    o->splitCovariances[k][2] = sp.splitWeightedSampleCovariances[q].z[l];
-    o->splitNumSamples[k] = sp.numSamples[q][l];
-    o->splitSumWeights[k] = sp.sumWeights[q][l];
-    o->splitSumAssignedSamples[k] = sp.sumAssignedSamples[q][l];
-  }
-  o->pivot[0] = v._pivotPosition.x; o->pivot[1] = v._pivotPosition.y; o->pivot[2] = v._pivotPosition.z;
-  o->sumWeights = su.sumWeights; o->numSamples = su.numSamples; o->overallNumSamples = su.overallNumSamples;
-  o->suffK = su.numComponents; o->normalized = su.normalized;
-  o->splitK = sp.numComponents;
-  o->numSamplesAfterLastSplit = r->stats.numSamplesAfterLastSplit;
-  o->numSamplesAfterLastMerge = r->stats.numSamplesAfterLastMerge;
-  for (int j = 0; j < 3; j++) {
-    o->ssMean[j] = r->ss.mean[j]; o->ssVariance[j] = r->ss.sampleVariance[j];
-  }
-  o->ssNumSamples = r->ss.numSamples;
-  o->valid = r->valid;
-  o->numSplits = r->fit.numSplits;
-  o->numMerges = r->fit.numMerges;
-  o->emIterations = r->fit.numUpdateWEMIterations;
-  o->partialEmIterations = r->fit.numPartialUpdateWEMIterations;
-}
-
-float ref_region_pdf(void* p, const float* d) {
-  RefRegion* r = (RefRegion*)p;
-  return r->vmm.pdf(Vector3(d[0], d[1], d[2]));
-}
-
-// init at pos (parallax shift) + cosine (surface) or HG (volume) product, then
-// pdf(dir) and sample(u) -> dirOut/pdfOut.  Returns the product integral.
+void ref_region_export(void* p, orc_region_record* o) { region_export<true>(p, o); }
+float ref_region_pdf(void* p, const float* d) { return region_pdf<true>(p, d); }
 float ref_region_query(void* p, int volume, const float* pos, const float* nrm, float g, const float* u2, const float* pdfDir,
                        float* pdfAtDir, float* dirOut, float* pdfOut) {
-  RefRegion* r = (RefRegion*)p;
-  float integral = 1.f;
-  if (volume) {
-    VMMVolumeSamplingDistribution<VMM8, true> d;
-    d.init(&r->vmm, Point3(pos[0], pos[1], pos[2]));
-    d.applySingleLobeHenyeyGreensteinProduct(Vector3(nrm[0], nrm[1], nrm[2]), g);
-    integral = d.m_productIntegral;
-    *pdfAtDir = d.pdf(Vector3(pdfDir[0], pdfDir[1], pdfDir[2]));
-    Vector3 o = d.sample(Point2(u2[0], u2[1]));
-    dirOut[0] = o.x; dirOut[1] = o.y; dirOut[2] = o.z;
-    *pdfOut = d.pdf(o);
-  } else {
-    VMMSurfaceSamplingDistribution<VMM8, true> d;
-    d.init(&r->vmm, Point3(pos[0], pos[1], pos[2]));
-    d.applyCosineProduct(Vector3(nrm[0], nrm[1], nrm[2]));
-    integral = d.m_productIntegral;
-    *pdfAtDir = d.pdf(Vector3(pdfDir[0], pdfDir[1], pdfDir[2]));
-    Vector3 o = d.sample(Point2(u2[0], u2[1]));
-    dirOut[0] = o.x; dirOut[1] = o.y; dirOut[2] = o.z;
-    *pdfOut = d.pdf(o);
-  }
-  return integral;
+  return region_query<true>(p, volume, pos, nrm, g, u2, pdfDir, pdfAtDir, dirOut, pdfOut);
+}
+void ref_region_np_export(void* p, orc_region_record* o) { region_export<false>(p, o); }
+float ref_region_np_pdf(void* p, const float* d) { return region_pdf<false>(p, d); }
+float ref_region_np_query(void* p, int volume, const float* pos, const float* nrm, float g, const float* u2,
+                          const float* pdfDir, float* pdfAtDir, float* dirOut, float* pdfOut) {
+  return region_query<false>(p, volume, pos, nrm, g, u2, pdfDir, pdfAtDir, dirOut, pdfOut);
 }
 
 void ref_integer_stats(const float* pos, size_t n, const float* lo, const float* hi, float* out, int64_t* raw) {

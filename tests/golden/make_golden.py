@@ -7,6 +7,9 @@ the reference tree; the fixtures are data (inputs and the reference's outputs):
   rcp.npz            embree::rcp (rcpss + Newton) of this host
   region_kat.npz     per-region fit/update trajectories (samples + outputs)
   query_kat.npz      init + cosine/HG product + sample + pdf on a trained region
+  region_kat_np.npz  the same trajectories and queries for the plain VMM
+                     field type (no parallax compensation)
+Usage: make_golden.py [fixture-name ...]   (default: all)
 """
 import ctypes
 import os
@@ -29,7 +32,54 @@ def hg_table(ref):
     return t.reshape(128, 18)
 
 
+def plain_vmm_kat():
+    """Trajectories and queries of the VMM field type (cfg.parallax = 0)."""
+    cfg = default_config(32000)
+    cfg.minSamplesForSplitting = 1000
+    cfg.minSamplesForPartialRefitting = 1000
+    cfg.minSamplesForMerging = 2000
+    cfg.parallax = 0
+    dirs = np.random.default_rng(98).normal(size=(256, 3))
+    dirs = (dirs / np.linalg.norm(dirs, axis=1, keepdims=True)).astype(np.float32)
+    kat = {"dirs": dirs}
+    for reg in range(2):
+        rr = RefRegion(cfg)
+        for step in range(3):
+            s = region_samples(2500, seed=300 + 100 * reg + step)
+            m, v, n = region_sample_stats(s)
+            split = 1 if step == 2 else 0
+            rr.step(s, m, v, n, split)
+            rec = rr.record()
+            key = "r%d_s%d_" % (reg, step)
+            kat[key + "samples"] = s
+            kat[key + "ss"] = np.concatenate([m, v, [n]]).astype(np.float32)
+            kat[key + "split"] = np.int32(split)
+            for f in ["K", "weights", "kappas", "meanDirections", "distances", "pivot", "numSplits", "numMerges",
+                      "emIterations", "valid"]:
+                kat[key + f] = np.asarray(rec[f])
+            kat[key + "pdf"] = np.array([rr.pdf(d) for d in dirs], np.float32)
+        qrng = np.random.default_rng(7 + reg)
+        qp = (m + qrng.normal(scale=0.5, size=(100, 3))).astype(np.float32)
+        qn = qrng.normal(size=(100, 3))
+        qn = (qn / np.linalg.norm(qn, axis=1, keepdims=True)).astype(np.float32)
+        qu = qrng.random((100, 2)).astype(np.float32)
+        qg = qrng.uniform(-0.9, 0.9, 100).astype(np.float32)
+        kat["q%d_pos" % reg], kat["q%d_nrm" % reg], kat["q%d_u2" % reg], kat["q%d_g" % reg] = qp, qn, qu, qg
+        for vol in (0, 1):
+            pa = np.zeros(100, np.float32); d = np.zeros((100, 3), np.float32); p = np.zeros(100, np.float32)
+            for i in range(100):
+                _, pa[i], d[i], p[i] = rr.query(vol, qp[i], qn[i], qg[i], qu[i], dirs[i])
+            kat["q%d_pdf_at_dir_%d" % (reg, vol)] = pa
+            kat["q%d_dir_%d" % (reg, vol)] = d
+            kat["q%d_pdf_%d" % (reg, vol)] = p
+    np.savez_compressed(os.path.join(HERE, "region_kat_np.npz"), **kat)
+
+
 def main():
+    which = sys.argv[1:]
+    if which == ["region_kat_np"]:
+        plain_vmm_kat()
+        return
     ref = reference()
     np.save(os.path.join(HERE, "hg_table.npy"), hg_table(ref))
 
@@ -122,6 +172,7 @@ def main():
             np.savez_compressed(os.path.join(HERE, "query_kat.npz"), **out)
     kat["steps"] = np.array(steps)
     np.savez_compressed(os.path.join(HERE, "region_kat.npz"), **kat)
+    plain_vmm_kat()
     print("golden fixtures written to", HERE)
 
 

@@ -24,7 +24,7 @@ class Config(ctypes.Structure):
         ("mergingThreshold", ctypes.c_float), ("partialReFit", ctypes.c_uint32),
         ("minSamplesForSplitting", ctypes.c_int32), ("minSamplesForPartialRefitting", ctypes.c_int32),
         ("minSamplesForMerging", ctypes.c_int32), ("decayOnSpatialSplit", ctypes.c_float),
-        ("deterministic", ctypes.c_uint32), ("vectorWidth", ctypes.c_uint32)]]
+        ("deterministic", ctypes.c_uint32), ("vectorWidth", ctypes.c_uint32), ("parallax", ctypes.c_uint32)]]
 
 
 F32 = ctypes.c_float
@@ -133,6 +133,15 @@ def reference():
         lib.ref_rcp.restype = ctypes.c_float
         lib.ref_rcp.argtypes = [ctypes.c_float]
         lib.ref_hg_table.argtypes = [vp]
+        lib.ref_region_np_new.restype = vp
+        lib.ref_region_np_new.argtypes = [ctypes.POINTER(Config)]
+        lib.ref_region_np_release.argtypes = [vp]
+        lib.ref_region_np_step.argtypes = [vp, vp, ctypes.c_size_t, vp, vp, ctypes.c_float, ctypes.c_int]
+        lib.ref_region_np_export.argtypes = [vp, ctypes.POINTER(RegionRecord)]
+        lib.ref_region_np_pdf.restype = ctypes.c_float
+        lib.ref_region_np_pdf.argtypes = [vp, vp]
+        lib.ref_region_np_query.restype = ctypes.c_float
+        lib.ref_region_np_query.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_float, vp, vp, vp, vp, vp]
         lib.ref_file_region_new.restype = vp
         lib.ref_file_region_new.argtypes = [ctypes.POINTER(Config), ctypes.c_int]
         lib.ref_file_region_step.argtypes = [vp, ctypes.c_int, vp, ctypes.c_size_t, vp, vp, ctypes.c_int]
@@ -185,31 +194,44 @@ class OracleRegion:
 
 
 class RefRegion:
+    """One region of the reference's CPU_8 device, parallax-aware VMM
+    (cfg.parallax = 1) or plain VMM (cfg.parallax = 0)."""
+
     def __init__(self, cfg):
         self.lib = reference()
-        self.h = self.lib.ref_region_new(ctypes.byref(cfg))
+        self.pre = "ref_region_" if getattr(cfg, "parallax", 1) else "ref_region_np_"
+        self.h = getattr(self.lib, self.pre + "new")(ctypes.byref(cfg))
+
+    def _fn(self, name):
+        return getattr(self.lib, self.pre + name)
 
     def step(self, samples, mean, var, n_stats, split_flag=0):
         m = np.ascontiguousarray(mean, np.float32)
         v = np.ascontiguousarray(var, np.float32)
-        self.lib.ref_region_step(self.h, ptr(samples), len(samples), ptr(m), ptr(v), float(n_stats), int(split_flag))
+        self._fn("step")(self.h, ptr(samples), len(samples), ptr(m), ptr(v), float(n_stats), int(split_flag))
 
     def record(self):
         r = RegionRecord()
-        self.lib.ref_region_export(self.h, ctypes.byref(r))
+        self._fn("export")(self.h, ctypes.byref(r))
         return r.to_dict()
 
     def pdf(self, d):
         d = np.ascontiguousarray(d, np.float32)
-        return self.lib.ref_region_pdf(self.h, ptr(d))
+        return self._fn("pdf")(self.h, ptr(d))
 
     def query(self, volume, pos, nrm, g, u2, pdf_dir):
         pos = np.ascontiguousarray(pos, np.float32); nrm = np.ascontiguousarray(nrm, np.float32)
         u2 = np.ascontiguousarray(u2, np.float32); pdf_dir = np.ascontiguousarray(pdf_dir, np.float32)
         pa = np.zeros(1, np.float32); d = np.zeros(3, np.float32); p = np.zeros(1, np.float32)
-        integ = self.lib.ref_region_query(self.h, int(volume), ptr(pos), ptr(nrm), float(g), ptr(u2), ptr(pdf_dir),
-                                          ptr(pa), ptr(d), ptr(p))
+        integ = self._fn("query")(self.h, int(volume), ptr(pos), ptr(nrm), float(g), ptr(u2), ptr(pdf_dir),
+                                  ptr(pa), ptr(d), ptr(p))
         return integ, float(pa[0]), d, float(p[0])
+
+    def __del__(self):
+        try:
+            self._fn("release")(self.h)
+        except Exception:
+            pass
 
     def __del__(self):
         try:

@@ -351,6 +351,101 @@ def test_query_kat_matches_reference_vectors():
         print("vol", vol, "identical sample directions", np.mean(ang > 0.9999))
 
 
+def _plain_cfg():
+    cfg = default_config(32000)
+    cfg.minSamplesForSplitting = 1000
+    cfg.minSamplesForPartialRefitting = 1000
+    cfg.minSamplesForMerging = 2000
+    cfg.parallax = 0
+    return cfg
+
+
+def test_plain_vmm_trajectories_match_reference_vectors():
+    """PGL_DIRECTIONAL_DISTRIBUTION_VMM: no reprojection, no shift before an
+    update (the pivot stays where the fit put it) or at a query."""
+    load_hg_table_into_oracle()
+    g = _load("region_kat_np.npz")
+    dirs = g["dirs"]
+    for reg in range(2):
+        o, o64 = OracleRegion(_plain_cfg()), OracleRegion(_plain_cfg(), 1)
+        for step in range(3):
+            k = "r%d_s%d_" % (reg, step)
+            ss = g[k + "ss"]
+            for x in (o, o64):
+                x.step(np.ascontiguousarray(g[k + "samples"]), ss[0:3], ss[3:6], ss[6], int(g[k + "split"]))
+            rec = o.record()
+            assert rec["K"] == g[k + "K"], k
+            if step > 0:
+                assert rec["emIterations"] == g[k + "emIterations"], k
+                assert rec["numSplits"] == g[k + "numSplits"] and rec["numMerges"] == g[k + "numMerges"], k
+            assert np.array_equal(rec["pivot"], g[k + "pivot"]), k
+            assert np.array_equal(rec["pivot"], g["r%d_s0_ss" % reg][0:3]), k  # fit-time mean, never shifted
+            pdf = np.array([o.pdf(d) for d in dirs])
+            rel = np.abs(pdf - g[k + "pdf"]) / g[k + "pdf"]
+            # Where the reference's own FP32 result is off the FP64 twin (the
+            # third step here: 1.4e-4 after a spatial-split decay), the bar is
+            # the twin's distance to the reference, not a fixed 1e-4.
+            pdf64 = np.array([o64.pdf(d) for d in dirs])
+            ref_err = np.median(np.abs(pdf64 - g[k + "pdf"]) / g[k + "pdf"])
+            bar = max(1e-4, 2.0 * ref_err)
+            assert np.median(rel) < bar and np.max(rel) < 5e-3, (k, np.median(rel), np.max(rel), ref_err)
+        for vol in (0, 1):
+            rel = []
+            for i in range(100):
+                _, pa, _, _ = region_query(o, vol, g["q%d_pos" % reg][i], g["q%d_nrm" % reg][i], g["q%d_g" % reg][i],
+                                           g["q%d_u2" % reg][i], dirs[i])
+                ref = g["q%d_pdf_at_dir_%d" % (reg, vol)][i]
+                rel.append(abs(pa - ref) / ref)
+            rel = np.array(rel)
+            assert np.median(rel) < bar and np.mean(rel < 1e-3) > 0.97, (reg, vol, np.median(rel), bar)
+
+
+def test_plain_vmm_differs_from_parallax_aware():
+    """The switch matters on these inputs: the parallax-aware fit of the same
+    samples is a different mixture."""
+    g = _load("region_kat_np.npz")
+    cfg_p = _plain_cfg()
+    cfg_p.parallax = 1
+    a, b = OracleRegion(_plain_cfg()), OracleRegion(cfg_p)
+    for step in range(2):
+        k = "r0_s%d_" % step
+        ss = g[k + "ss"]
+        for o in (a, b):
+            o.step(np.ascontiguousarray(g[k + "samples"]), ss[0:3], ss[3:6], ss[6], int(g[k + "split"]))
+    dirs = g["dirs"]
+    pa = np.array([a.pdf(d) for d in dirs])
+    pb = np.array([b.pdf(d) for d in dirs])
+    assert np.median(np.abs(pa - pb) / pb) > 1e-2
+
+
+@pytest.mark.skipif(not reference_available(), reason="reference build absent")
+def test_plain_vmm_live_against_reference_build():
+    """Six regions x three steps.  A split decision near its threshold can
+    flip between any two FP32 evaluation orders (libm vs the reference's
+    polynomial exp/log here) and the trajectory then departs; as for the GPU,
+    the bar is per case: equal K and median pdf error below
+    max(1e-4, 2 x the FP64 twin's distance to the reference) in >= 15 of 18."""
+    cfg = _plain_cfg()
+    good, cases = 0, []
+    for seed in range(6):
+        o, o64, r = OracleRegion(cfg), OracleRegion(cfg, 1), RefRegion(cfg)
+        for step in range(3):
+            s = region_samples(4000, seed=2000 + 10 * seed + step)
+            m, v, n = region_sample_stats(s)
+            for x in (o, o64, r):
+                x.step(s, m, v, n, int(step == 2))
+            dirs = unit_dirs(256, seed)
+            pa = np.array([o.pdf(d) for d in dirs])
+            pb = np.array([r.pdf(d) for d in dirs])
+            pc = np.array([o64.pdf(d) for d in dirs])
+            bar = max(1e-4, 2.0 * np.median(np.abs(pc - pb) / pb))
+            err = np.median(np.abs(pa - pb) / pb)
+            ok = o.record()["K"] == r.record()["K"] and err < bar
+            good += ok
+            cases.append((seed, step, o.record()["K"], r.record()["K"], float(err), float(bar)))
+    assert good >= 15, cases
+
+
 def test_sampling_follows_pdf():
     """Chi-square test: directions drawn by init + cosine product + sample fall
     into equal-area sphere cells with the probabilities the pdf integrates to."""
