@@ -44,6 +44,30 @@ typedef struct PGLSampleData {
 #define PGL_SAMPLE_FLAG_SPLATTED 1u
 #define PGL_SAMPLE_FLAG_INSIDE_VOLUME 2u
 
+/* data.h:50-113: one recorded path vertex (136 bytes).  A default-constructed
+ * vertex (pglPathSegmentStorageNextSegment) has directionIn (0,1,0),
+ * directionOut (1,0,0), normal (0,0,1), pdfDirectionIn, miWeight,
+ * russianRouletteProbability, eta, roughness 1, unit weights and zero
+ * contributions. */
+typedef struct PGLPathSegmentData {
+  pgl_point3f position;
+  pgl_vec3f directionIn;
+  pgl_vec3f directionOut;
+  pgl_vec3f normal;
+  bool volumeScatter;
+  float pdfDirectionIn;
+  bool isDelta;
+  pgl_vec3f scatteringWeight;
+  pgl_vec3f transmittanceWeight;
+  pgl_vec3f directContribution;
+  float miWeight;
+  pgl_vec3f scatteredContribution;
+  float russianRouletteProbability;
+  float eta;
+  float roughness;
+  const void* regionPtr;
+} PGLPathSegmentData;
+
 /* data.h:106-109 */
 typedef struct PGLString { char* m_str; size_t m_size; } PGLString;
 
@@ -113,6 +137,7 @@ typedef struct PGLSurfaceSamplingDistributionImpl* PGLSurfaceSamplingDistributio
 typedef struct PGLVolumeSamplingDistributionImpl* PGLVolumeSamplingDistribution;
 typedef struct PGLRegionImpl* PGLRegion;
 typedef struct PGLFieldStatisticsImpl* PGLFieldStatistics;
+typedef struct PGLPathSegmentStorageImpl* PGLPathSegmentStorage;
 
 /* config.h:120, api.cpp:755-787: heap-allocates the two argument structs;
  * pglDeviceNewField takes ownership and deletes them (Device.h:109,136) */
@@ -172,6 +197,34 @@ PGLSampleData pglSampleStorageGetSampleSurface(PGLSampleStorage sampleStorage, c
 PGLSampleData pglSampleStorageGetSampleVolume(PGLSampleStorage sampleStorage, const int idx);
 bool pglSampleStorageValidate(PGLSampleStorage sampleStorage);
 bool pglSampleStorageCompare(PGLSampleStorage sampleStorageA, PGLSampleStorage sampleStorageB);
+
+/* pathsegmentstorage.h:35-68 (the C++ default arguments of the reference are
+ * useNEEMiWeights = false, guideDirectLight = false,
+ * rrAffectsDirectContribution = true).  Per-path storage on the host, as in
+ * the reference; vertices and samples beyond the reserved capacity are
+ * dropped. */
+PGLPathSegmentStorage pglNewPathSegmentStorage(void);
+void pglReleasePathSegmentStorage(PGLPathSegmentStorage pathSegmentStorage);
+void pglPathSegmentStorageReserve(PGLPathSegmentStorage pathSegmentStorage, size_t size);
+void pglPathSegmentStorageClear(PGLPathSegmentStorage pathSegmentStorage);
+void pglPathSegmentSetMaxDistance(PGLPathSegmentStorage pathSegmentStorage, float maxDistance);
+float pglPathSegmentGetMaxDistance(PGLPathSegmentStorage pathSegmentStorage);
+int pglPathSegmentGetNumSegments(PGLPathSegmentStorage pathSegmentStorage);
+int pglPathSegmentGetNumSamples(PGLPathSegmentStorage pathSegmentStorage);
+size_t pglPathSegmentStoragePrepareSamples(PGLPathSegmentStorage pathSegmentStorage, const bool useNEEMiWeights,
+                                           const bool guideDirectLight, const bool rrAffectsDirectContribution);
+pgl_vec3f pglPathSegmentStorageCalculatePixelEstimate(PGLPathSegmentStorage pathSegmentStorage,
+                                                      const bool rrAffectsDirectContribution);
+const PGLSampleData* pglPathSegmentStorageGetSamples(PGLPathSegmentStorage pathSegmentStorage,
+                                                     PGL_REF(size_t) nSamples);
+void pglPathSegmentStorageAddSample(PGLPathSegmentStorage pathSegmentStorage, PGLSampleData sample);
+PGLPathSegmentData* pglPathSegmentStorageNextSegment(PGLPathSegmentStorage pathSegmentStorage);
+void pglPathSegmentStorageAddSegment(PGLPathSegmentStorage pathSegmentStorage, PGLPathSegmentData segment);
+bool pglPathSegmentStorageValidateSamples(PGLPathSegmentStorage pathSegmentStorage);
+bool pglPathSegmentStorageValidateSegments(PGLPathSegmentStorage pathSegmentStorage);
+void pglPathSegmentStoragePropagateSamples(PGLPathSegmentStorage pathSegmentStorage, PGLSampleStorage sampleStorage,
+                                           bool guideDirectLight, bool useNEEMiWeights,
+                                           bool rrAffectsDirectContribution);
 
 /* surfacesamplingdistribution.h:24-46 */
 void pglReleaseSurfaceSamplingDistribution(PGLSurfaceSamplingDistribution surfaceSamplingDistribution);
@@ -386,6 +439,39 @@ int pglServerGetTiming(PGLServer server, uint64_t request, double* out3);
 void pglServerForget(PGLServer server, uint64_t request);
 void pglServerRefresh(PGLServer server);
 void pglServerGetStats(PGLServer server, double* out, size_t count);
+
+/* ---------------------------------------------------------------------------
+ * EXTENSION: path segments of many paths on the GPU.  Path p is the vertex
+ * range [pathOffsets[p], pathOffsets[p+1]) of the numSegments vertices
+ * `segments` (device memory, the PGLPathSegmentData layout; ranges must be
+ * non-decreasing and within numSegments, else the call fails); its training samples are those of
+ * pglPathSegmentStoragePrepareSamples on that path with the given max
+ * distance (pglPathSegmentSetMaxDistance, 1e6 by default) and flags.
+ *
+ * pglPathSegmentBatchPrepareSamplesDevice writes them to device arrays,
+ * surface and volume samples apart, in the order per-path
+ * pglPathSegmentStoragePropagateSamples calls in path order would append
+ * them to one sample storage; returns false (with a message) when a
+ * capacity is too small, writing nothing.  Blocks until written.
+ *
+ * pglFieldUpdateFromPathSegmentsDevice trains the field with those samples
+ * without a host round trip: pglFieldUpdate semantics (surface and volume
+ * field, one iteration).  The path arrays may be reused once it returns.
+ * ------------------------------------------------------------------------ */
+typedef struct PGLPathSegmentBatchImpl* PGLPathSegmentBatch;
+PGLPathSegmentBatch pglNewPathSegmentBatch(void);
+void pglReleasePathSegmentBatch(PGLPathSegmentBatch batch);
+bool pglPathSegmentBatchPrepareSamplesDevice(PGLPathSegmentBatch batch, const PGLPathSegmentData* segments,
+                                             size_t numSegments, const uint32_t* pathOffsets, uint32_t numPaths,
+                                             float maxDistance,
+                                             bool useNEEMiWeights, bool guideDirectLight,
+                                             bool rrAffectsDirectContribution, PGLSampleData* surfaceOut,
+                                             size_t surfaceCapacity, PGLSampleData* volumeOut, size_t volumeCapacity,
+                                             size_t* numSurfaceOut, size_t* numVolumeOut);
+void pglFieldUpdateFromPathSegmentsDevice(PGLField field, const PGLPathSegmentData* segments, size_t numSegments,
+                                          const uint32_t* pathOffsets, uint32_t numPaths, float maxDistance,
+                                          bool useNEEMiWeights, bool guideDirectLight,
+                                          bool rrAffectsDirectContribution);
 
 #ifdef __cplusplus
 }  /* extern "C" */

@@ -10,7 +10,7 @@ import os
 
 import numpy as np
 
-from .synth import SAMPLE_DTYPE
+from .synth import PATH_SEGMENT_DTYPE, SAMPLE_DTYPE
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libopenpgl_amd.so")
@@ -41,6 +41,16 @@ class Box3(ctypes.Structure):
 class PGLSampleData(ctypes.Structure):
     _fields_ = [("position", Vec3), ("direction", Vec3), ("weight", _f), ("pdf", _f), ("distance", _f),
                 ("flags", _u32)]
+
+
+class PGLPathSegmentData(ctypes.Structure):
+    """include/openpgl/data.h:50-113 (136 bytes)."""
+    _fields_ = [("position", Vec3), ("directionIn", Vec3), ("directionOut", Vec3), ("normal", Vec3),
+                ("volumeScatter", ctypes.c_bool), ("pdfDirectionIn", ctypes.c_float), ("isDelta", ctypes.c_bool),
+                ("scatteringWeight", Vec3), ("transmittanceWeight", Vec3), ("directContribution", Vec3),
+                ("miWeight", ctypes.c_float), ("scatteredContribution", Vec3),
+                ("russianRouletteProbability", ctypes.c_float), ("eta", ctypes.c_float),
+                ("roughness", ctypes.c_float), ("regionPtr", ctypes.c_void_p)]
 
 
 class PGLDebugArguments(ctypes.Structure):
@@ -143,6 +153,30 @@ _SIGS = {
     "pglNewSampleStorageFromFile": (_vp, [ctypes.c_char_p]),
     "pglSampleStorageCompare": (ctypes.c_bool, [_vp, _vp]),
     "pglSampleStorageValidate": (ctypes.c_bool, [_vp]),
+    "pglNewPathSegmentStorage": (_vp, []),
+    "pglReleasePathSegmentStorage": (None, [_vp]),
+    "pglPathSegmentStorageReserve": (None, [_vp, ctypes.c_size_t]),
+    "pglPathSegmentStorageClear": (None, [_vp]),
+    "pglPathSegmentSetMaxDistance": (None, [_vp, _f]),
+    "pglPathSegmentGetMaxDistance": (_f, [_vp]),
+    "pglPathSegmentGetNumSegments": (ctypes.c_int, [_vp]),
+    "pglPathSegmentGetNumSamples": (ctypes.c_int, [_vp]),
+    "pglPathSegmentStoragePrepareSamples": (ctypes.c_size_t, [_vp, ctypes.c_bool, ctypes.c_bool, ctypes.c_bool]),
+    "pglPathSegmentStorageCalculatePixelEstimate": (Vec3, [_vp, ctypes.c_bool]),
+    "pglPathSegmentStorageGetSamples": (_vp, [_vp, ctypes.POINTER(ctypes.c_size_t)]),
+    "pglPathSegmentStorageAddSample": (None, [_vp, PGLSampleData]),
+    "pglPathSegmentStorageNextSegment": (_vp, [_vp]),
+    "pglPathSegmentStorageAddSegment": (None, [_vp, PGLPathSegmentData]),
+    "pglPathSegmentStorageValidateSamples": (ctypes.c_bool, [_vp]),
+    "pglPathSegmentStorageValidateSegments": (ctypes.c_bool, [_vp]),
+    "pglPathSegmentStoragePropagateSamples": (None, [_vp, _vp, ctypes.c_bool, ctypes.c_bool, ctypes.c_bool]),
+    "pglNewPathSegmentBatch": (_vp, []),
+    "pglReleasePathSegmentBatch": (None, [_vp]),
+    "pglPathSegmentBatchPrepareSamplesDevice": (ctypes.c_bool, [_vp, _vp, ctypes.c_size_t, _vp, _u32, _f,
+                                                                 ctypes.c_bool, ctypes.c_bool, ctypes.c_bool, _vp,
+                                                                 ctypes.c_size_t, _vp, ctypes.c_size_t, _vp, _vp]),
+    "pglFieldUpdateFromPathSegmentsDevice": (None, [_vp, _vp, ctypes.c_size_t, _vp, _u32, _f, ctypes.c_bool,
+                                                    ctypes.c_bool, ctypes.c_bool]),
     "pglFieldSampleBatch": (None, [_vp, ctypes.c_size_t] + [_vp] * 10),
     "pglFieldSampleBatchDevice": (None, [_vp, ctypes.c_size_t] + [_vp] * 11),
     "pglFieldGetNumNodes": (_u32, [_vp, ctypes.c_int]),
@@ -473,6 +507,12 @@ class Field:
     def update_volume_device(self, device_ptr, n):
         self.lib.pglFieldUpdateVolumeDevice(self.h, ctypes.c_void_p(device_ptr), n)
 
+    def update_from_path_segments_device(self, segments_ptr, num_segments, offsets_ptr, num_paths, max_distance=1e6,
+                                         use_nee_mi_weights=False, guide_direct_light=False, rr_affects_direct=True):
+        self.lib.pglFieldUpdateFromPathSegmentsDevice(self.h, ctypes.c_void_p(segments_ptr), num_segments,
+                                                      ctypes.c_void_p(offsets_ptr), num_paths, float(max_distance),
+                                                      use_nee_mi_weights, guide_direct_light, rr_affects_direct)
+
     def iteration(self):
         return self.lib.pglFieldGetIteration(self.h)
 
@@ -629,3 +669,91 @@ class VolumeSamplingDistribution(_Dist):
     def apply_hg_product(self, d, g):
         self.lib.pglVolumeSamplingDistributionApplySingleLobeHenyeyGreensteinProduct(
             self.h, Vec3(*[float(x) for x in d]), float(g))
+
+
+class PathSegmentStorage:
+    """pglPathSegmentStorage*: one recorded path on the host (the reference's
+    per-render-thread storage)."""
+
+    def __init__(self, capacity=64, max_distance=None):
+        self.lib = load_library()
+        self.h = self.lib.pglNewPathSegmentStorage()
+        self.lib.pglPathSegmentStorageReserve(self.h, capacity)
+        if max_distance is not None:
+            self.lib.pglPathSegmentSetMaxDistance(self.h, float(max_distance))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.pglReleasePathSegmentStorage(self.h)
+            self.h = None
+
+    def add_segments(self, segments):
+        seg = np.ascontiguousarray(segments, PATH_SEGMENT_DTYPE)
+        raw = seg.view(np.uint8).reshape(len(seg), 136)
+        for i in range(len(seg)):
+            self.lib.pglPathSegmentStorageAddSegment(self.h, PGLPathSegmentData.from_buffer_copy(raw[i].tobytes()))
+
+    def next_segment(self):
+        """pglPathSegmentStorageNextSegment: address of a default vertex (0 when full)."""
+        return self.lib.pglPathSegmentStorageNextSegment(self.h) or 0
+
+    def clear(self):
+        self.lib.pglPathSegmentStorageClear(self.h)
+
+    def num_segments(self):
+        return self.lib.pglPathSegmentGetNumSegments(self.h)
+
+    def num_samples(self):
+        return self.lib.pglPathSegmentGetNumSamples(self.h)
+
+    def prepare_samples(self, use_nee_mi_weights=False, guide_direct_light=False, rr_affects_direct=True):
+        return self.lib.pglPathSegmentStoragePrepareSamples(self.h, use_nee_mi_weights, guide_direct_light,
+                                                            rr_affects_direct)
+
+    def samples(self):
+        n = ctypes.c_size_t(0)
+        p = self.lib.pglPathSegmentStorageGetSamples(self.h, ctypes.byref(n))
+        if n.value == 0:
+            return np.zeros(0, SAMPLE_DTYPE)
+        return np.frombuffer((ctypes.c_char * (40 * n.value)).from_address(p), SAMPLE_DTYPE).copy()
+
+    def pixel_estimate(self, rr_affects_direct=True):
+        v = self.lib.pglPathSegmentStorageCalculatePixelEstimate(self.h, rr_affects_direct)
+        return np.array([v.x, v.y, v.z], np.float32)
+
+    def propagate(self, storage, guide_direct_light=False, use_nee_mi_weights=False, rr_affects_direct=True):
+        self.lib.pglPathSegmentStoragePropagateSamples(self.h, storage.h, guide_direct_light, use_nee_mi_weights,
+                                                       rr_affects_direct)
+
+    def validate_segments(self):
+        return self.lib.pglPathSegmentStorageValidateSegments(self.h)
+
+    def validate_samples(self):
+        return self.lib.pglPathSegmentStorageValidateSamples(self.h)
+
+
+class PathSegmentBatch:
+    """EXTENSION: training samples of many recorded paths, on the GPU."""
+
+    def __init__(self):
+        self.lib = load_library()
+        self.h = self.lib.pglNewPathSegmentBatch()
+        if not self.h:
+            raise RuntimeError("pglNewPathSegmentBatch failed")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.pglReleasePathSegmentBatch(self.h)
+            self.h = None
+
+    def prepare_device(self, segments_ptr, num_segments, offsets_ptr, num_paths, surface_ptr, surface_capacity,
+                       volume_ptr, volume_capacity, max_distance=1e6, use_nee_mi_weights=False,
+                       guide_direct_light=False, rr_affects_direct=True):
+        """Returns (surface count, volume count), or None on error (capacity too small, bad ranges)."""
+        ns, nv = ctypes.c_size_t(0), ctypes.c_size_t(0)
+        ok = self.lib.pglPathSegmentBatchPrepareSamplesDevice(
+            self.h, ctypes.c_void_p(segments_ptr), num_segments, ctypes.c_void_p(offsets_ptr), num_paths,
+            float(max_distance),
+            use_nee_mi_weights, guide_direct_light, rr_affects_direct, ctypes.c_void_p(surface_ptr),
+            surface_capacity, ctypes.c_void_p(volume_ptr), volume_capacity, ctypes.byref(ns), ctypes.byref(nv))
+        return (ns.value, nv.value) if ok else None

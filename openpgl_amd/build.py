@@ -22,6 +22,7 @@ SOURCES = [
     ("kernels_region.hip", HIPFLAGS),
     ("kernels_query.hip", HIPFLAGS),
     ("kernels_sort.hip", HIPFLAGS),
+    ("kernels_path.hip", HIPFLAGS),
     ("field.cpp", ["-x", "hip", "--offload-arch=" + ARCH, "-ffp-contract=off", "-Xclang", "-target-feature",
                    "-Xclang", "-packed-fp32-ops"]),
     ("region_sched.cpp", ["-x", "hip", "--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang",
@@ -30,6 +31,10 @@ SOURCES = [
                           "-Xclang", "-packed-fp32-ops"]),
     ("field_io.cpp", ["-x", "hip", "--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang",
                       "-packed-fp32-ops"]),
+    ("path_batch.cpp", ["-x", "hip", "--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang",
+                        "-packed-fp32-ops"]),
+    ("api_paths.cpp", ["-x", "hip", "--offload-arch=" + ARCH, "-ffp-contract=off", "-Xclang", "-target-feature",
+                       "-Xclang", "-packed-fp32-ops"]),
     ("server.cpp", ["-x", "hip", "--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]),
     ("api.cpp", ["-x", "hip", "--offload-arch=" + ARCH, "-ffp-contract=off", "-Xclang", "-target-feature",
                  "-Xclang", "-packed-fp32-ops"]),

@@ -16,60 +16,12 @@
 #include <vector>
 
 #include "../../include/openpgl_amd/openpgl.h"
+#include "api_impl.h"
 #include "field.h"
 #include "region_batch.h"
 #include "server.h"
 
 using namespace pglgpu;
-
-#define PGL_CATCH_BEGIN try {
-#define PGL_CATCH_END(ret)                                            \
-  }                                                                   \
-  catch (const std::bad_alloc&) {                                     \
-    std::cout << "Open PGL was unable to allocate memory" << std::endl; \
-    return ret;                                                       \
-  }                                                                   \
-  catch (const std::exception& e) {                                   \
-    std::cout << e.what() << std::endl;                               \
-    return ret;                                                       \
-  }                                                                   \
-  catch (...) {                                                       \
-    std::cout << "an unrecognized exception was caught" << std::endl; \
-    return ret;                                                       \
-  }
-#define PGL_CATCH_END_VOID                                            \
-  }                                                                   \
-  catch (const std::bad_alloc&) {                                     \
-    std::cout << "Open PGL was unable to allocate memory" << std::endl; \
-    return;                                                           \
-  }                                                                   \
-  catch (const std::exception& e) {                                   \
-    std::cout << e.what() << std::endl;                               \
-    return;                                                           \
-  }                                                                   \
-  catch (...) {                                                       \
-    std::cout << "an unrecognized exception was caught" << std::endl; \
-    return;                                                           \
-  }
-#define THROW_IF_NULL(obj, name) \
-  if (obj == nullptr) throw std::runtime_error(std::string("null ") + name + " provided to " + __FUNCTION__)
-
-struct PGLDeviceImpl {
-  int type;
-  int vectorWidth;
-};
-
-struct PGLFieldImpl {
-  GuidingField* gf;
-  int deviceType;
-  std::vector<PGLSampleData>* staging = nullptr;
-  DevBuf dStage;
-};
-
-struct PGLSampleStorageImpl {
-  std::vector<PGLSampleData> surface, volume;
-  std::mutex mutex;
-};
 
 struct HostDist {
   int kind = 0;
@@ -294,6 +246,7 @@ static PGLFieldImpl* new_field(PGLDevice device, PGLFieldArguments args) {
 }
 
 static void release_field(PGLFieldImpl* field) {
+  delete field->paths;
   field->gf->surface.finish();
   field->gf->volume.finish();
   if (field->gf->stream) (void)hipStreamDestroy(field->gf->stream);

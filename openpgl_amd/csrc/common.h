@@ -80,6 +80,31 @@ struct SplitStatsState {
   int32_t K;
 };
 
+// One path vertex as the renderer records it: PGLPathSegmentData
+// (include/openpgl/data.h:50-113), 136 bytes with the same member offsets.
+struct PathVertex {
+  float px, py, pz;        // position
+  float dix, diy, diz;     // directionIn (towards the next vertex)
+  float dox, doy, doz;     // directionOut (towards the previous vertex)
+  float nx, ny, nz;        // normal
+  uint8_t volumeScatter;
+  uint8_t pad0[3];
+  float pdfDirectionIn;
+  uint8_t isDelta;
+  uint8_t pad1[3];
+  float scatteringWeight[3];
+  float transmittanceWeight[3];
+  float directContribution[3];
+  float miWeight;
+  float scatteredContribution[3];
+  float russianRouletteProbability;
+  float eta;
+  float roughness;
+  uint32_t pad2;
+  uint64_t regionPtr;
+};
+static_assert(sizeof(PathVertex) == 136, "PathVertex is PGLPathSegmentData (136 B)");
+
 // Per-region directional state kept resident in HBM (one 4 KiB record per region).
 struct RegionState {
   VMMParams vmm;

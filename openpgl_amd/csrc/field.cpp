@@ -607,6 +607,7 @@ void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
   launch_gather_reproject(deviceSamples, dKeys2_.as<uint32_t>(), dVals2_.as<uint32_t>(), n, dPivots_.as<float4>(),
                           soa[0], soa[1], soa[2], soa[3], soa[4], soa[5], fc_.noParallax == 0, sp);
   PGL_HIP_CHECK(hipEventRecord(evSp_[P][3], sp));
+  lastGatherParity_ = P;
 
   // ------------------------------------------------ finish the previous update
   finish();

@@ -128,6 +128,8 @@ class KindField {
   int vectorWidth() const { return vectorWidth_; }
   // false for the plain VMM field type (PGL_DIRECTIONAL_DISTRIBUTION_VMM)
   bool parallaxAware() const { return fc_.noParallax == 0; }
+  // completes when the last queued update has read its input records (null before the first)
+  hipEvent_t lastGatherEvent() const { return lastGatherParity_ < 0 ? nullptr : evSp_[lastGatherParity_][3]; }
 
   // Field::serialize / Field::deserialize (field/Field.h:240-305) in the
   // layout of a reference device of SIMD width W (4, 8 or 16); readFile
@@ -175,6 +177,7 @@ class KindField {
   DevBuf dKeyNodes_, dKnnPiv_;
   hipStream_t sp_ = nullptr;  // spatial phase
   int parity_ = 0;
+  int lastGatherParity_ = -1;
   hipEvent_t evSp_[2][4], evDir_[2][3];
   // pinned staging of the per-update uploads, by frame parity: [0] inputs,
   // [1] pivots, [2] copy pairs, [3] KNN pivots, [4] key nodes, [5] nodes,

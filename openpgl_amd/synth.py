@@ -192,3 +192,72 @@ def batched_regions(num_regions=4096, seed=0, nmin=64, nmax=65536, ks=(8, 16, 32
     records = np.concatenate([p[0] for p in parts])
     stats7 = np.concatenate([p[1] for p in parts])
     return records, counts, stats7, init_k
+
+
+# PGLPathSegmentData (include/openpgl/data.h:50-113), 136 bytes
+PATH_SEGMENT_DTYPE = np.dtype({
+    "names": ["position", "directionIn", "directionOut", "normal", "volumeScatter", "pdfDirectionIn", "isDelta",
+              "scatteringWeight", "transmittanceWeight", "directContribution", "miWeight", "scatteredContribution",
+              "russianRouletteProbability", "eta", "roughness", "regionPtr"],
+    "formats": [("<f4", 3), ("<f4", 3), ("<f4", 3), ("<f4", 3), "u1", "<f4", "u1", ("<f4", 3), ("<f4", 3),
+                ("<f4", 3), "<f4", ("<f4", 3), "<f4", "<f4", "<f4", "<u8"],
+    "offsets": [0, 12, 24, 36, 48, 52, 56, 60, 72, 84, 96, 100, 112, 116, 120, 128],
+    "itemsize": 136})
+
+
+def random_paths(num_paths, seed=0, max_len=12, scene_seed=0, num_blobs=16):
+    """Recorded paths of a synthetic scene: vertex positions near the scene's
+    blobs, consecutive vertices joined by their directions, a mix of diffuse,
+    glossy and delta (some refractive) vertices, volume vertices, sparse
+    emission and scattered contributions, Russian roulette after the third
+    vertex (sometimes terminating with probability 0).  Returns the vertex
+    array (PATH_SEGMENT_DTYPE) and the path offsets (num_paths + 1, uint32)."""
+    rng = np.random.Generator(np.random.PCG64([seed, 4243]))
+    scene = SceneModel(scene_seed, num_blobs)
+    lengths = rng.integers(0, max_len + 1, size=num_paths)
+    offsets = np.zeros(num_paths + 1, np.uint32)
+    offsets[1:] = np.cumsum(lengths)
+    n = int(offsets[-1])
+    seg = np.zeros(n, PATH_SEGMENT_DTYPE)
+    if n == 0:
+        return seg, offsets
+    path_of = np.repeat(np.arange(num_paths), lengths)
+    index = np.arange(n) - offsets[path_of].astype(np.int64)
+    blob = rng.integers(0, num_blobs, size=n)
+    pos = scene.centres[blob] + rng.normal(scale=0.08, size=(n, 3))
+    seg["position"] = pos.astype(np.float32)
+    last = index == lengths[path_of] - 1
+    nxt = np.where(last, np.arange(n), np.arange(n) + 1)
+    d_in = pos[nxt] - pos
+    d_in[last] = _unit_vectors(rng, int(last.sum()))
+    d_in /= np.linalg.norm(d_in, axis=1, keepdims=True)
+    first = index == 0
+    prv = np.where(first, np.arange(n), np.arange(n) - 1)
+    d_out = pos[prv] - pos
+    d_out[first] = _unit_vectors(rng, int(first.sum()))
+    d_out /= np.linalg.norm(d_out, axis=1, keepdims=True)
+    seg["directionIn"] = d_in.astype(np.float32)
+    seg["directionOut"] = d_out.astype(np.float32)
+    seg["normal"] = _unit_vectors(rng, n).astype(np.float32)
+    seg["volumeScatter"] = rng.random(n) < 0.2
+    delta = rng.random(n) < 0.15
+    seg["isDelta"] = delta
+    rough = rng.random(n).astype(np.float32)
+    rough[rng.random(n) < 0.1] = 0.005
+    seg["roughness"] = rough
+    eta = np.ones(n, np.float32)
+    eta[delta & (rng.random(n) < 0.5)] = 1.5
+    seg["eta"] = eta
+    seg["pdfDirectionIn"] = np.exp(rng.uniform(np.log(0.01), np.log(5.0), n)).astype(np.float32)
+    seg["scatteringWeight"] = rng.uniform(0.0, 1.5, (n, 3)).astype(np.float32)
+    seg["transmittanceWeight"] = np.where(rng.random((n, 1)) < 0.7, 1.0,
+                                          rng.uniform(0.3, 1.0, (n, 3))).astype(np.float32)
+    emit = rng.random(n) < 0.2
+    seg["directContribution"] = (rng.uniform(0.0, 4.0, (n, 3)) * emit[:, None]).astype(np.float32)
+    sc = rng.random(n) < 0.3
+    seg["scatteredContribution"] = (rng.uniform(0.0, 2.0, (n, 3)) * sc[:, None]).astype(np.float32)
+    seg["miWeight"] = rng.random(n).astype(np.float32)
+    rr = np.where(index < 3, 1.0, rng.uniform(0.2, 1.0, n))
+    rr[(index >= 3) & (rng.random(n) < 0.05)] = 0.0
+    seg["russianRouletteProbability"] = rr.astype(np.float32)
+    return seg, offsets

@@ -1855,3 +1855,120 @@ int orc_hg_rep(float g, float* w, float* mc, float* kappa, float* norm) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- path segments
+// PathSegmentDataStorage::prepareSamples / calculatePixelEstimate
+// (data/PathSegmentDataStorage.h:160-393) for one path.  Vector dot products
+// are embree's madd chain (fused under the CPU_8 build), everything else
+// separate IEEE operations in the reference's order.
+namespace orc {
+
+static inline float vdot(const float a[3], const float b[3]) { return std::fma(a[0], b[0], std::fma(a[1], b[1], a[2] * b[2])); }
+
+}  // namespace orc
+
+extern "C" {
+
+size_t orc_path_prepare_samples(const orc_path_segment* segs, size_t n, float maxDistance, int useNEEMiWeights,
+                                int guideDirectLight, int rrAffectsDirectContribution, orc_sample* out, size_t cap) {
+  using orc::vdot;
+  size_t written = 0;
+  float lastDistance = 0.0f;
+  for (long i = long(n) - 2; i >= 0; --i) {
+    const orc_path_segment& cur = segs[i];
+    float d[3];
+    for (int k = 0; k < 3; k++) d[k] = segs[i + 1].position[k] - cur.position[k];
+    const float currentDistance = std::sqrt(vdot(d, d));
+    const float distance = std::fmin(currentDistance + lastDistance, 2.0f * maxDistance);
+    if (!cur.isDelta && cur.roughness >= 0.3f) {
+      lastDistance = 0.0f;
+    } else {
+      lastDistance = distance;
+      if (cur.eta != 1.0f) {
+        const float cosThetaI = vdot(cur.normal, cur.directionOut);
+        const float cosThetaO = vdot(cur.normal, cur.directionIn);
+        lastDistance *= std::fabs(cosThetaI / (cosThetaO * cur.eta));
+      }
+    }
+    if (cur.isDelta || !(cur.roughness > 0.01f)) continue;
+    const float pdf = std::max(0.1f, cur.pdfDirectionIn);
+    float throughput[3] = {1.f, 1.f, 1.f}, contribution[3] = {0.f, 0.f, 0.f};
+    float previousRR = 1.0f;
+    for (size_t j = size_t(i) + 1; j < n; ++j) {
+      const orc_path_segment& nxt = segs[j];
+      float clamped[3];
+      for (int k = 0; k < 3; k++) {
+        throughput[k] = throughput[k] * segs[j - 1].transmittanceWeight[k];
+        clamped[k] = throughput[k] < 10.0f ? throughput[k] : 10.0f;  // embree::min
+        const float add = clamped[k] * nxt.scatteredContribution[k];
+        contribution[k] = contribution[k] + add;
+      }
+      float direct[3];
+      for (int k = 0; k < 3; k++) direct[k] = rrAffectsDirectContribution ? nxt.directContribution[k]
+                                                                          : nxt.directContribution[k] * previousRR;
+      const bool first = j == size_t(i) + 1;
+      if (first && !useNEEMiWeights) {
+        if (guideDirectLight)
+          for (int k = 0; k < 3; k++) contribution[k] = contribution[k] + clamped[k] * direct[k];
+      } else if (!first || guideDirectLight) {
+        for (int k = 0; k < 3; k++) {
+          const float cw = clamped[k] * nxt.miWeight;
+          contribution[k] = contribution[k] + cw * direct[k];
+        }
+      }
+      for (int k = 0; k < 3; k++) throughput[k] = throughput[k] * nxt.scatteringWeight[k];
+      if (nxt.russianRouletteProbability > 0.f) {
+        for (int k = 0; k < 3; k++) throughput[k] = throughput[k] / nxt.russianRouletteProbability;
+      } else {
+        for (int k = 0; k < 3; k++) throughput[k] = 0.f;
+      }
+      previousRR = nxt.russianRouletteProbability;
+    }
+    const bool positive = contribution[0] > 0.0f || contribution[1] > 0.0f || contribution[2] > 0.0f;
+    if (!positive || !(distance > 0)) continue;
+    if (written < cap) {
+      orc_sample& s = out[written];
+      s.px = cur.position[0]; s.py = cur.position[1]; s.pz = cur.position[2];
+      s.dx = cur.directionIn[0]; s.dy = cur.directionIn[1]; s.dz = cur.directionIn[2];
+      const float spectrum = (contribution[0] + contribution[1] + contribution[2]) / 3.0f;
+      s.weight = spectrum / pdf;
+      s.pdf = pdf;
+      s.distance = distance;
+      s.flags = cur.volumeScatter ? 2u : 0u;
+    }
+    written++;
+  }
+  return written;
+}
+
+void orc_path_pixel_estimate(const orc_path_segment* segs, size_t n, int rrAffectsDirectContribution, float* rgb) {
+  rgb[0] = rgb[1] = rgb[2] = 0.f;
+  if (n == 0) return;
+  float throughput[3] = {1.f, 1.f, 1.f}, contribution[3] = {0.f, 0.f, 0.f};
+  float previousRR = 1.0f;
+  for (size_t j = 1; j < n; ++j) {
+    const orc_path_segment& nxt = segs[j];
+    for (int k = 0; k < 3; k++) {
+      throughput[k] = throughput[k] * segs[j - 1].transmittanceWeight[k];
+      contribution[k] = contribution[k] + throughput[k] * nxt.scatteredContribution[k];
+    }
+    for (int k = 0; k < 3; k++) {
+      const float direct = rrAffectsDirectContribution ? nxt.directContribution[k] : nxt.directContribution[k] * previousRR;
+      const float tw = throughput[k] * nxt.miWeight;
+      contribution[k] = contribution[k] + tw * direct;
+    }
+    for (int k = 0; k < 3; k++) throughput[k] = throughput[k] * nxt.scatteringWeight[k];
+    if (nxt.russianRouletteProbability > 0.f) {
+      for (int k = 0; k < 3; k++) throughput[k] = throughput[k] / nxt.russianRouletteProbability;
+    } else {
+      for (int k = 0; k < 3; k++) throughput[k] = 0.f;
+    }
+    previousRR = nxt.russianRouletteProbability;
+  }
+  for (int k = 0; k < 3; k++) {
+    const float own = segs[0].directContribution[k] + segs[0].scatteredContribution[k];
+    rgb[k] = n == 1 ? own : own + segs[0].scatteringWeight[k] * contribution[k];
+  }
+}
+
+}  // extern "C"

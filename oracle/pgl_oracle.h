@@ -58,6 +58,21 @@ typedef struct {
   uint32_t parallax;
 } orc_config;
 
+// one recorded path vertex, same 136-byte layout as PGLPathSegmentData
+// (include/openpgl/data.h:50-113)
+typedef struct {
+  float position[3], directionIn[3], directionOut[3], normal[3];
+  uint8_t volumeScatter; uint8_t pad0[3];
+  float pdfDirectionIn;
+  uint8_t isDelta; uint8_t pad1[3];
+  float scatteringWeight[3], transmittanceWeight[3], directContribution[3];
+  float miWeight;
+  float scatteredContribution[3];
+  float russianRouletteProbability, eta, roughness;
+  uint32_t pad2;
+  uint64_t regionPtr;
+} orc_path_segment;
+
 // flat per-region export record used by tests (canonical DFS order)
 typedef struct {
   // VMM
@@ -147,6 +162,13 @@ float orc_rcp(float x);
 // HG phase-function representation lookup (fills K and 4x{w, meanCosine, kappa, norm})
 int orc_hg_rep(float g, float* w, float* mc, float* kappa, float* norm);
 void orc_set_hg_table(const float* table);  // 128 x (K, g, 4w, 4mc, 4kappa, 4norm) = 18 floats
+
+// PathSegmentDataStorage::prepareSamples of one path (n vertices): writes at
+// most cap samples, returns how many the path yields
+size_t orc_path_prepare_samples(const orc_path_segment* segs, size_t n, float maxDistance, int useNEEMiWeights,
+                                int guideDirectLight, int rrAffectsDirectContribution, orc_sample* out, size_t cap);
+// PathSegmentDataStorage::calculatePixelEstimate of one path
+void orc_path_pixel_estimate(const orc_path_segment* segs, size_t n, int rrAffectsDirectContribution, float* rgb);
 
 #ifdef __cplusplus
 }

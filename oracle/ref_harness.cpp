@@ -496,6 +496,29 @@ void ref_stats_merge_split(const float* a13, const float* b13, int dim, float po
 
 float ref_rcp(float x) { return embree::rcp(x); }
 
+// sizeof(PGLPathSegmentData) and the offsets of its members in declaration order
+// (include/openpgl/data.h:50-113).  PathSegmentDataStorage itself is not built
+// here: it includes SampleDataStorage.h, which needs oneTBB.
+void ref_path_segment_layout(size_t* out) {
+  out[0] = sizeof(PGLPathSegmentData);
+  out[1] = offsetof(PGLPathSegmentData, position);
+  out[2] = offsetof(PGLPathSegmentData, directionIn);
+  out[3] = offsetof(PGLPathSegmentData, directionOut);
+  out[4] = offsetof(PGLPathSegmentData, normal);
+  out[5] = offsetof(PGLPathSegmentData, volumeScatter);
+  out[6] = offsetof(PGLPathSegmentData, pdfDirectionIn);
+  out[7] = offsetof(PGLPathSegmentData, isDelta);
+  out[8] = offsetof(PGLPathSegmentData, scatteringWeight);
+  out[9] = offsetof(PGLPathSegmentData, transmittanceWeight);
+  out[10] = offsetof(PGLPathSegmentData, directContribution);
+  out[11] = offsetof(PGLPathSegmentData, miWeight);
+  out[12] = offsetof(PGLPathSegmentData, scatteredContribution);
+  out[13] = offsetof(PGLPathSegmentData, russianRouletteProbability);
+  out[14] = offsetof(PGLPathSegmentData, eta);
+  out[15] = offsetof(PGLPathSegmentData, roughness);
+  out[16] = offsetof(PGLPathSegmentData, regionPtr);
+}
+
 // field-file storage entries for SIMD width W (4 or 8; 16 needs AVX-512 builds)
 void* ref_file_region_new(const orc_config* c, int W) {
   return W == 4 ? file_region_new<4>(c) : (W == 8 ? file_region_new<8>(c) : nullptr);

@@ -328,3 +328,48 @@ def ref_config_bytes(cfg):
     buf = ctypes.create_string_buffer(n)
     lib.ref_file_config_serialize(ctypes.byref(cfg), buf, n)
     return buf.raw[:n]
+
+
+def oracle_path_samples(segments, offsets, max_distance=1e6, use_nee_mi_weights=False, guide_direct_light=False,
+                        rr_affects_direct=True):
+    """The oracle's PathSegmentDataStorage::prepareSamples applied to each path
+    [offsets[p], offsets[p+1]) in order; returns (surface, volume) records in
+    the order per-path propagation into one sample storage appends them."""
+    from openpgl_amd.synth import SAMPLE_DTYPE
+    lib = oracle()
+    fn = lib.orc_path_prepare_samples
+    fn.restype = ctypes.c_size_t
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                   ctypes.c_void_p, ctypes.c_size_t]
+    segments = np.ascontiguousarray(segments)
+    out = np.zeros(max(1, len(segments)), SAMPLE_DTYPE)
+    parts = []
+    base = segments.ctypes.data
+    for p in range(len(offsets) - 1):
+        b, e = int(offsets[p]), int(offsets[p + 1])
+        n = e - b
+        k = fn(base + 136 * b, n, float(max_distance), int(use_nee_mi_weights), int(guide_direct_light),
+               int(rr_affects_direct), out.ctypes.data, len(out))
+        if k:
+            parts.append(out[:k].copy())
+    allv = np.concatenate(parts) if parts else np.zeros(0, SAMPLE_DTYPE)
+    vol = (allv["flags"] & 2) != 0
+    return allv[~vol], allv[vol]
+
+
+def oracle_pixel_estimate(segments, rr_affects_direct=True):
+    lib = oracle()
+    fn = lib.orc_path_pixel_estimate
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p]
+    segments = np.ascontiguousarray(segments)
+    out = np.zeros(3, np.float32)
+    fn(segments.ctypes.data if len(segments) else None, len(segments), int(rr_affects_direct), out.ctypes.data)
+    return out
+
+
+def ref_path_segment_layout():
+    lib = reference()
+    out = np.zeros(17, np.uint64)
+    lib.ref_path_segment_layout.argtypes = [ctypes.c_void_p]
+    lib.ref_path_segment_layout(out.ctypes.data)
+    return out
