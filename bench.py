@@ -408,6 +408,53 @@ def batched_cpu_baseline(recs, counts, stats7, init_k, threads, sample_regions=2
                       % (len(regions), step, n, threads, dt)}
 
 
+def path_segments(num_paths=1_000_000, reps=5, cpu=True):
+    """Batched path segments -> training samples (pglPathSegmentBatchPrepareSamplesDevice)
+    on 1M recorded paths of up to 12 vertices, inputs resident in HBM; the CPU
+    line is the oracle's per-path restatement on one core over 50k paths."""
+    import openpgl_amd as pgl
+    from openpgl_amd.hip_runtime import DeviceArray
+    from openpgl_amd.synth import random_paths
+    seg, off = random_paths(num_paths, seed=21, max_len=12)
+    log("path segments:", num_paths, "paths,", len(seg), "vertices")
+    dseg, doff = DeviceArray.from_numpy(seg), DeviceArray.from_numpy(off)
+    cap = len(seg)
+    ds, dv = DeviceArray(40 * cap), DeviceArray(40 * cap)
+    batch = pgl.PathSegmentBatch()
+    r = batch.prepare_device(dseg.value, len(seg), doff.value, num_paths, ds.value, cap, dv.value, cap)
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        r = batch.prepare_device(dseg.value, len(seg), doff.value, num_paths, ds.value, cap, dv.value, cap)
+        times.append(time.perf_counter() - t0)
+    dt = float(np.median(times))
+    ns, nv = r
+    # algorithmic bytes: every 136 B vertex read once, every 40 B record written once
+    nbytes = 136 * len(seg) + 40 * (ns + nv)
+    out = {"paths": num_paths, "vertices": int(len(seg)), "samples": int(ns + nv), "ms": dt * 1e3,
+           "paths_per_s": num_paths / dt, "samples_per_s": (ns + nv) / dt,
+           "hbm_gbs_algorithmic": nbytes / dt / 1e9, "hbm_frac": nbytes / dt / 1e9 / PEAK_HBM_GBS}
+    if cpu:
+        import ctypes
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_ctypes import oracle
+        lib = oracle()
+        fn = lib.orc_path_prepare_batch
+        fn.restype = ctypes.c_size_t
+        fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_float, ctypes.c_int, ctypes.c_int,
+                       ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t]
+        sample_paths = 200_000
+        buf = np.zeros(int(off[sample_paths]) * 40 + 40, np.uint8)
+        t0 = time.perf_counter()
+        k = fn(seg.ctypes.data, off.ctypes.data, sample_paths, 1e6, 0, 0, 1, buf.ctypes.data, len(buf) // 40)
+        ct = time.perf_counter() - t0
+        out["cpu"] = {"value": sample_paths / ct, "unit": "paths/s", "cores": 1, "kind": "port",
+                      "sample": "oracle prepareSamples over the first 200k of the same paths, one thread (%d samples)"
+                                % k}
+        out["speedup_vs_cpu"] = out["paths_per_s"] / out["cpu"]["value"]
+    return out
+
+
 def read_traffic():
     """HBM bytes per region-kernel launch from the committed rocprofv3 summary."""
     path = os.path.join(ROOT, "profiles", "k_region_update_traffic.txt")
@@ -448,6 +495,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched uneven EM workload")
     ap.add_argument("--no-mixed", action="store_true", help="skip the large mixed update + serving workload")
+    ap.add_argument("--no-paths", action="store_true", help="skip the path-segment workload")
     ap.add_argument("--region-grid", type=int, default=None, help="region kernel workgroups (default: all co-resident)")
     ap.add_argument("--region-balance", type=float, default=None, help="region team balance (default 1)")
     ap.add_argument("--region-cooperative", type=int, default=None, help="1: cooperative region launch (default)")
@@ -518,6 +566,8 @@ def main():
         out["batched_em"] = batched_em(cpu=not args.no_cpu, threads=min(os.cpu_count() or 1, 16))
     if not args.no_mixed:
         out["mixed_update_and_serving"] = mixed_workload()
+    if not args.no_paths:
+        out["path_segments"] = path_segments(cpu=not args.no_cpu)
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
         out["cpu_baseline"] = cpu_baseline(frames, threads, timed_frames=min(3, nframes - 1))

@@ -114,16 +114,18 @@ void launch_query(const QueryField& surf, const QueryField& vol, const HGRep* hg
                   const float* nrm, const float* g, const float* u2, const float* u1, const uint8_t* isVolume,
                   float* dirOut, float* pdfOut, uint32_t* idOut, float* integralOut, hipStream_t s);
 
-// path segments -> training samples (kernels_path.hip); counts are packed as
-// surface | volume << 32 per path
+// path segments -> training samples (kernels_path.hip).  Per vertex: its
+// record, a flag packed as surface | volume << 32 and its path; totals:
+// [0] surface records, [1] volume records, [2] 1 if a path range was malformed
 struct PathSampleOptions;
 struct PathSample;
-size_t path_scan_temp_bytes(uint32_t numPaths);
-void launch_path_count(const PathVertex* segs, const uint32_t* offsets, uint32_t numPaths, uint32_t numSegments,
-                       const PathSampleOptions& o, unsigned long long* packed, unsigned long long* scanned,
-                       void* scanTemp, size_t scanBytes, unsigned long long* totals, hipStream_t s);
-void launch_path_emit(const PathVertex* segs, const uint32_t* offsets, uint32_t numPaths, uint32_t numSegments,
-                      const PathSampleOptions& o, const unsigned long long* scanned, PathSample* surfOut,
-                      PathSample* volOut, hipStream_t s);
+size_t path_scan_temp_bytes(uint32_t numSegments);
+void launch_path_records(const PathVertex* segs, uint32_t numSegments, const uint32_t* offsets, uint32_t numPaths,
+                         const PathSampleOptions& o, PathSample* records, unsigned long long* flags, uint32_t* pathOf,
+                         unsigned long long* incl, void* scanTemp, size_t scanBytes, unsigned long long* totals,
+                         hipStream_t s);
+void launch_path_scatter(const PathSample* records, const unsigned long long* flags, const uint32_t* pathOf,
+                         const unsigned long long* incl, const uint32_t* offsets, uint32_t numSegments,
+                         PathSample* surfOut, PathSample* volOut, hipStream_t s);
 
 }  // namespace pglgpu

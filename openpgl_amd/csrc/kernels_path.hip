@@ -4,11 +4,16 @@
 // PGLSampleData records, surface and volume apart, in the order the
 // per-path calls would append them to one sample storage.
 //
-// One thread per path (a path is short and its samples depend on a running
-// backward chain): pass 1 counts each path's surface and volume samples,
-// an exclusive scan of the packed counts gives every path's output offsets,
-// pass 2 recomputes and writes.  Both passes run the same routine
-// (path_samples.h), so the counts and the records agree.
+// One thread per vertex: the lanes of a wave cover consecutive vertices, so
+// they read one contiguous stretch of the vertex array (lanes of one path
+// read the same later vertices) instead of one path per lane with loads
+// scattered over many paths.  A vertex's thread redoes the path's backward
+// distance walk from the last vertex down to its own (the exact sequence of
+// float operations of the serial walk) and then its own contribution
+// (path_samples.h), and stores its record and a packed surface | volume << 32
+// flag.  An inclusive scan of the flags gives every record its place: the
+// records of earlier paths, plus those of the same path at later vertices
+// (the walk emits vertex n-2 first).
 #pragma clang fp contract(off)
 #include <hip/hip_runtime.h>
 
@@ -22,83 +27,117 @@ namespace pglgpu {
 namespace {
 
 constexpr int kPathThreads = 256;
+constexpr uint32_t kNoPath = 0xffffffffu;
 
-__global__ void __launch_bounds__(kPathThreads) k_path_count(const PathVertex* __restrict__ segs,
-                                                             const uint32_t* __restrict__ offsets, uint32_t numPaths,
-                                                             uint32_t numSegments, PathSampleOptions o,
-                                                             unsigned long long* __restrict__ packed,
+// the path holding vertex v: the last p with offsets[p] <= v
+__device__ __forceinline__ uint32_t path_of(const uint32_t* __restrict__ offsets, uint32_t numPaths, uint32_t v) {
+  uint32_t lo = 0, hi = numPaths;
+  while (hi - lo > 1) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (offsets[mid] <= v) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(kPathThreads) k_path_check(const uint32_t* __restrict__ offsets, uint32_t numPaths,
+                                                             uint32_t numSegments,
                                                              unsigned long long* __restrict__ totals) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= numPaths) return;
-  const uint32_t b = offsets[p], e = offsets[p + 1];
-  uint32_t surf = 0, vol = 0;
-  if (b > e || e > numSegments) {
-    totals[2] = 1ull;  // malformed range: nothing is read, the host reports the error
-  } else if (e > b) {
-    path_samples(segs + b, (int)(e - b), o, [&](const PathSample& s) {
-      if (s.flags & 2u) vol++; else surf++;
-    });
-  }
-  packed[p] = (unsigned long long)surf | ((unsigned long long)vol << 32);
+  if (offsets[p] > offsets[p + 1] || offsets[p + 1] > numSegments) totals[2] = 1ull;
 }
 
-__global__ void k_path_totals(const unsigned long long* __restrict__ packed,
-                              const unsigned long long* __restrict__ scanned, uint32_t numPaths,
+__global__ void __launch_bounds__(kPathThreads) k_path_vertex(const PathVertex* __restrict__ segs, uint32_t numSegments,
+                                                              const uint32_t* __restrict__ offsets, uint32_t numPaths,
+                                                              PathSampleOptions o, PathSample* __restrict__ records,
+                                                              unsigned long long* __restrict__ flags,
+                                                              uint32_t* __restrict__ pathOf) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < numSegments; v += stride) {
+    unsigned long long flag = 0ull;
+    uint32_t p = kNoPath;
+    const uint32_t q = path_of(offsets, numPaths, v);
+    const uint32_t b = offsets[q], e = offsets[q + 1];
+    if (b <= v && v < e && e <= numSegments) {
+      p = q;
+      const int n = (int)(e - b), i = (int)(v - b);
+      PathSample s;
+      if (i <= n - 2 && path_vertex_sample(segs + b, n, i, o, s)) {
+        records[v] = s;
+        flag = (s.flags & 2u) ? (1ull << 32) : 1ull;
+      }
+    }
+    flags[v] = flag;
+    pathOf[v] = p;
+  }
+}
+
+__global__ void k_path_totals(const unsigned long long* __restrict__ incl, uint32_t numSegments,
                               unsigned long long* __restrict__ totals) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
-    const unsigned long long t = numPaths ? scanned[numPaths - 1] + packed[numPaths - 1] : 0ull;
+    const unsigned long long t = numSegments ? incl[numSegments - 1] : 0ull;
     totals[0] = t & 0xffffffffull;
     totals[1] = t >> 32;
   }
 }
 
-__global__ void __launch_bounds__(kPathThreads) k_path_emit(const PathVertex* __restrict__ segs,
-                                                            const uint32_t* __restrict__ offsets, uint32_t numPaths,
-                                                            uint32_t numSegments, PathSampleOptions o,
-                                                            const unsigned long long* __restrict__ scanned,
-                                                            PathSample* __restrict__ surfOut,
-                                                            PathSample* __restrict__ volOut) {
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= numPaths) return;
-  const uint32_t b = offsets[p], e = offsets[p + 1];
-  if (e <= b || e > numSegments) return;
-  uint32_t si = (uint32_t)(scanned[p] & 0xffffffffull);
-  uint32_t vi = (uint32_t)(scanned[p] >> 32);
-  path_samples(segs + b, (int)(e - b), o, [&](const PathSample& s) {
-    if (s.flags & 2u) volOut[vi++] = s; else surfOut[si++] = s;
-  });
+__global__ void __launch_bounds__(kPathThreads) k_path_scatter(const PathSample* __restrict__ records,
+                                                               const unsigned long long* __restrict__ flags,
+                                                               const uint32_t* __restrict__ pathOf,
+                                                               const unsigned long long* __restrict__ incl,
+                                                               const uint32_t* __restrict__ offsets,
+                                                               uint32_t numSegments, PathSample* __restrict__ surfOut,
+                                                               PathSample* __restrict__ volOut) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < numSegments; v += stride) {
+    const unsigned long long f = flags[v];
+    if (!f) continue;
+    const uint32_t p = pathOf[v];
+    const uint32_t b = offsets[p], e = offsets[p + 1];
+    const unsigned long long before = b > 0 ? incl[b - 1] : 0ull;  // records of earlier paths
+    const unsigned long long last = incl[e - 1], mine = incl[v];
+    const int hi = (f >> 32) ? 32 : 0;
+    // records of earlier paths + those of this path at later vertices
+    const uint32_t pos = (uint32_t)(before >> hi) + ((uint32_t)(last >> hi) - (uint32_t)(mine >> hi));
+    if (hi) volOut[pos] = records[v]; else surfOut[pos] = records[v];
+  }
+}
+
+uint32_t path_grid(uint32_t n) {
+  const uint32_t blocks = (n + kPathThreads - 1) / kPathThreads;
+  return blocks < 65536u ? (blocks ? blocks : 1u) : 65536u;
 }
 
 }  // namespace
 
-size_t path_scan_temp_bytes(uint32_t numPaths) {
+size_t path_scan_temp_bytes(uint32_t numSegments) {
   size_t bytes = 0;
   const unsigned long long* in = nullptr;
   unsigned long long* out = nullptr;
-  PGL_KERNEL_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)numPaths));
+  PGL_KERNEL_CHECK(hipcub::DeviceScan::InclusiveSum(nullptr, bytes, in, out, (int)numSegments));
   return bytes;
 }
 
-// totals: [0] surface samples, [1] volume samples, [2] 1 if a path range was malformed
-void launch_path_count(const PathVertex* segs, const uint32_t* offsets, uint32_t numPaths, uint32_t numSegments,
-                       const PathSampleOptions& o, unsigned long long* packed, unsigned long long* scanned,
-                       void* scanTemp, size_t scanBytes, unsigned long long* totals, hipStream_t s) {
+void launch_path_records(const PathVertex* segs, uint32_t numSegments, const uint32_t* offsets, uint32_t numPaths,
+                         const PathSampleOptions& o, PathSample* records, unsigned long long* flags, uint32_t* pathOf,
+                         unsigned long long* incl, void* scanTemp, size_t scanBytes, unsigned long long* totals,
+                         hipStream_t s) {
   PGL_KERNEL_CHECK(hipMemsetAsync(totals, 0, 3 * sizeof(unsigned long long), s));
-  if (numPaths == 0) return;
-  const uint32_t blocks = (numPaths + kPathThreads - 1) / kPathThreads;
-  hipLaunchKernelGGL(k_path_count, dim3(blocks), dim3(kPathThreads), 0, s, segs, offsets, numPaths, numSegments, o,
-                     packed, totals);
-  PGL_KERNEL_CHECK(hipcub::DeviceScan::ExclusiveSum(scanTemp, scanBytes, packed, scanned, (int)numPaths, s));
-  hipLaunchKernelGGL(k_path_totals, dim3(1), dim3(64), 0, s, packed, scanned, numPaths, totals);
+  if (numPaths == 0 || numSegments == 0) return;
+  hipLaunchKernelGGL(k_path_check, dim3((numPaths + kPathThreads - 1) / kPathThreads), dim3(kPathThreads), 0, s,
+                     offsets, numPaths, numSegments, totals);
+  hipLaunchKernelGGL(k_path_vertex, dim3(path_grid(numSegments)), dim3(kPathThreads), 0, s, segs, numSegments, offsets,
+                     numPaths, o, records, flags, pathOf);
+  PGL_KERNEL_CHECK(hipcub::DeviceScan::InclusiveSum(scanTemp, scanBytes, flags, incl, (int)numSegments, s));
+  hipLaunchKernelGGL(k_path_totals, dim3(1), dim3(64), 0, s, incl, numSegments, totals);
 }
 
-void launch_path_emit(const PathVertex* segs, const uint32_t* offsets, uint32_t numPaths, uint32_t numSegments,
-                      const PathSampleOptions& o, const unsigned long long* scanned, PathSample* surfOut,
-                      PathSample* volOut, hipStream_t s) {
-  if (numPaths == 0) return;
-  const uint32_t blocks = (numPaths + kPathThreads - 1) / kPathThreads;
-  hipLaunchKernelGGL(k_path_emit, dim3(blocks), dim3(kPathThreads), 0, s, segs, offsets, numPaths, numSegments, o,
-                     scanned, surfOut, volOut);
+void launch_path_scatter(const PathSample* records, const unsigned long long* flags, const uint32_t* pathOf,
+                         const unsigned long long* incl, const uint32_t* offsets, uint32_t numSegments,
+                         PathSample* surfOut, PathSample* volOut, hipStream_t s) {
+  if (numSegments == 0) return;
+  hipLaunchKernelGGL(k_path_scatter, dim3(path_grid(numSegments)), dim3(kPathThreads), 0, s, records, flags, pathOf,
+                     incl, offsets, numSegments, surfOut, volOut);
 }
 
 }  // namespace pglgpu

@@ -21,22 +21,32 @@ PathSampleBatch::~PathSampleBatch() {
 void PathSampleBatch::count(const PathVertex* segs, uint32_t numSegments, const uint32_t* offsets, uint32_t numPaths,
                             const PathSampleOptions& o, size_t& numSurf, size_t& numVol) {
   numSurf = numVol = 0;
-  if (numPaths == 0) return;
-  packed_.ensure(size_t(numPaths) * 8);
-  scanned_.ensure(size_t(numPaths) * 8);
-  const size_t scanBytes = path_scan_temp_bytes(numPaths);
-  scanTemp_.ensure(std::max<size_t>(scanBytes, 16));
+  offsets_ = offsets;
+  numSegments_ = numPaths ? numSegments : 0;
   totals_.ensure(24);
   hTotals_.ensure(24);
-  launch_path_count(segs, offsets, numPaths, numSegments, o, packed_.as<unsigned long long>(),
-                    scanned_.as<unsigned long long>(), scanTemp_.p, scanBytes, totals_.as<unsigned long long>(),
-                    stream_);
+  const size_t V = std::max<size_t>(numSegments_, 1);
+  records_.ensure(V * sizeof(PathSample));
+  flags_.ensure(V * 8);
+  incl_.ensure(V * 8);
+  pathOf_.ensure(V * 4);
+  const size_t scanBytes = path_scan_temp_bytes(std::max<uint32_t>(numSegments_, 1));
+  scanTemp_.ensure(std::max<size_t>(scanBytes, 16));
+  launch_path_records(segs, numSegments_, offsets, numPaths, o, records_.as<PathSample>(),
+                      flags_.as<unsigned long long>(), pathOf_.as<uint32_t>(), incl_.as<unsigned long long>(),
+                      scanTemp_.p, scanBytes, totals_.as<unsigned long long>(), stream_);
   PGL_HIP_CHECK(hipMemcpyAsync(hTotals_.p, totals_.p, 24, hipMemcpyDeviceToHost, stream_));
   PGL_HIP_CHECK(hipStreamSynchronize(stream_));
   const unsigned long long* t = hTotals_.as<unsigned long long>();
   if (t[2]) throw std::runtime_error("openpgl_amd: path offsets must be non-decreasing and within the segments");
   numSurf = (size_t)t[0];
   numVol = (size_t)t[1];
+}
+
+void PathSampleBatch::scatter(PathSample* surfOut, PathSample* volOut) {
+  launch_path_scatter(records_.as<PathSample>(), flags_.as<unsigned long long>(), pathOf_.as<uint32_t>(),
+                      incl_.as<unsigned long long>(), offsets_, numSegments_, surfOut, volOut, stream_);
+  PGL_HIP_CHECK(hipStreamSynchronize(stream_));
 }
 
 void PathSampleBatch::prepare(const PathVertex* segs, uint32_t numSegments, const uint32_t* offsets,
@@ -47,9 +57,7 @@ void PathSampleBatch::prepare(const PathVertex* segs, uint32_t numSegments, cons
     throw std::runtime_error("openpgl_amd: path samples exceed the output capacity (" + std::to_string(numSurf) +
                              " surface, " + std::to_string(numVol) + " volume)");
   if (numSurf == 0 && numVol == 0) return;
-  launch_path_emit(segs, offsets, numPaths, numSegments, o, scanned_.as<unsigned long long>(), surfOut, volOut,
-                   stream_);
-  PGL_HIP_CHECK(hipStreamSynchronize(stream_));
+  scatter(surfOut, volOut);
 }
 
 void PathSampleBatch::prepareOwned(const PathVertex* segs, uint32_t numSegments, const uint32_t* offsets,
@@ -71,11 +79,7 @@ void PathSampleBatch::prepareOwned(const PathVertex* segs, uint32_t numSegments,
   out_[b][1].ensure(nv * sizeof(PathSample));
   surf = out_[b][0].as<PathSample>();
   vol = out_[b][1].as<PathSample>();
-  if (numSurf || numVol) {
-    launch_path_emit(segs, offsets, numPaths, numSegments, o, scanned_.as<unsigned long long>(),
-                     out_[b][0].as<PathSample>(), out_[b][1].as<PathSample>(), stream_);
-    PGL_HIP_CHECK(hipStreamSynchronize(stream_));
-  }
+  if (numSurf || numVol) scatter(out_[b][0].as<PathSample>(), out_[b][1].as<PathSample>());
   lastBuffer_ = b;
 }
 

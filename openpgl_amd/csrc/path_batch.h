@@ -40,9 +40,12 @@ class PathSampleBatch {
  private:
   void count(const PathVertex* segs, uint32_t numSegments, const uint32_t* offsets, uint32_t numPaths,
              const PathSampleOptions& o, size_t& numSurf, size_t& numVol);
+  void scatter(PathSample* surfOut, PathSample* volOut);
 
   hipStream_t stream_ = nullptr;
-  DevBuf packed_, scanned_, scanTemp_, totals_;
+  DevBuf records_, flags_, pathOf_, incl_, scanTemp_, totals_;
+  const uint32_t* offsets_ = nullptr;  // of the last count()
+  uint32_t numSegments_ = 0;
   PinnedBuf hTotals_;
   DevBuf out_[2][2];  // [buffer][surface, volume]
   hipEvent_t released_[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};

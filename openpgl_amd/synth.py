@@ -218,46 +218,59 @@ def random_paths(num_paths, seed=0, max_len=12, scene_seed=0, num_blobs=16):
     offsets = np.zeros(num_paths + 1, np.uint32)
     offsets[1:] = np.cumsum(lengths)
     n = int(offsets[-1])
-    seg = np.zeros(n, PATH_SEGMENT_DTYPE)
+    # built as 34 float32 columns (one 136-byte record per row): single-column
+    # strided writes are fast, 3-wide sub-array field writes are not
+    raw = np.zeros((n, 34), np.float32)
     if n == 0:
-        return seg, offsets
+        return raw.view(PATH_SEGMENT_DTYPE).reshape(0), offsets
+    col = {name: PATH_SEGMENT_DTYPE.fields[name][1] // 4 for name in PATH_SEGMENT_DTYPE.names}
+
+    def put(name, values):
+        values = np.asarray(values, np.float32)
+        if values.ndim == 1:
+            raw[:, col[name]] = values
+        else:
+            for k in range(values.shape[1]):
+                raw[:, col[name] + k] = values[:, k]
+
+    def unit(v):
+        return v / np.sqrt((v * v).sum(1))[:, None]
+
     path_of = np.repeat(np.arange(num_paths), lengths)
     index = np.arange(n) - offsets[path_of].astype(np.int64)
     blob = rng.integers(0, num_blobs, size=n)
     pos = scene.centres[blob] + rng.normal(scale=0.08, size=(n, 3))
-    seg["position"] = pos.astype(np.float32)
+    put("position", pos)
     last = index == lengths[path_of] - 1
     nxt = np.where(last, np.arange(n), np.arange(n) + 1)
     d_in = pos[nxt] - pos
-    d_in[last] = _unit_vectors(rng, int(last.sum()))
-    d_in /= np.linalg.norm(d_in, axis=1, keepdims=True)
+    d_in[last] = rng.normal(size=(int(last.sum()), 3))
+    put("directionIn", unit(d_in))
     first = index == 0
     prv = np.where(first, np.arange(n), np.arange(n) - 1)
     d_out = pos[prv] - pos
-    d_out[first] = _unit_vectors(rng, int(first.sum()))
-    d_out /= np.linalg.norm(d_out, axis=1, keepdims=True)
-    seg["directionIn"] = d_in.astype(np.float32)
-    seg["directionOut"] = d_out.astype(np.float32)
-    seg["normal"] = _unit_vectors(rng, n).astype(np.float32)
-    seg["volumeScatter"] = rng.random(n) < 0.2
+    d_out[first] = rng.normal(size=(int(first.sum()), 3))
+    put("directionOut", unit(d_out))
+    put("normal", unit(rng.normal(size=(n, 3))))
+    flags = raw.view(np.uint8)
+    flags[:, PATH_SEGMENT_DTYPE.fields["volumeScatter"][1]] = rng.random(n) < 0.2
     delta = rng.random(n) < 0.15
-    seg["isDelta"] = delta
-    rough = rng.random(n).astype(np.float32)
+    flags[:, PATH_SEGMENT_DTYPE.fields["isDelta"][1]] = delta
+    rough = rng.random(n)
     rough[rng.random(n) < 0.1] = 0.005
-    seg["roughness"] = rough
-    eta = np.ones(n, np.float32)
+    put("roughness", rough)
+    eta = np.ones(n)
     eta[delta & (rng.random(n) < 0.5)] = 1.5
-    seg["eta"] = eta
-    seg["pdfDirectionIn"] = np.exp(rng.uniform(np.log(0.01), np.log(5.0), n)).astype(np.float32)
-    seg["scatteringWeight"] = rng.uniform(0.0, 1.5, (n, 3)).astype(np.float32)
-    seg["transmittanceWeight"] = np.where(rng.random((n, 1)) < 0.7, 1.0,
-                                          rng.uniform(0.3, 1.0, (n, 3))).astype(np.float32)
+    put("eta", eta)
+    put("pdfDirectionIn", np.exp(rng.uniform(np.log(0.01), np.log(5.0), n)))
+    put("scatteringWeight", rng.uniform(0.0, 1.5, (n, 3)))
+    put("transmittanceWeight", np.where(rng.random((n, 1)) < 0.7, 1.0, rng.uniform(0.3, 1.0, (n, 3))))
     emit = rng.random(n) < 0.2
-    seg["directContribution"] = (rng.uniform(0.0, 4.0, (n, 3)) * emit[:, None]).astype(np.float32)
+    put("directContribution", rng.uniform(0.0, 4.0, (n, 3)) * emit[:, None])
     sc = rng.random(n) < 0.3
-    seg["scatteredContribution"] = (rng.uniform(0.0, 2.0, (n, 3)) * sc[:, None]).astype(np.float32)
-    seg["miWeight"] = rng.random(n).astype(np.float32)
+    put("scatteredContribution", rng.uniform(0.0, 2.0, (n, 3)) * sc[:, None])
+    put("miWeight", rng.random(n))
     rr = np.where(index < 3, 1.0, rng.uniform(0.2, 1.0, n))
     rr[(index >= 3) & (rng.random(n) < 0.05)] = 0.0
-    seg["russianRouletteProbability"] = rr.astype(np.float32)
-    return seg, offsets
+    put("russianRouletteProbability", rr)
+    return raw.view(PATH_SEGMENT_DTYPE).reshape(n), offsets

@@ -1941,6 +1941,19 @@ size_t orc_path_prepare_samples(const orc_path_segment* segs, size_t n, float ma
   return written;
 }
 
+size_t orc_path_prepare_batch(const orc_path_segment* segs, const uint32_t* offsets, size_t numPaths,
+                              float maxDistance, int useNEEMiWeights, int guideDirectLight,
+                              int rrAffectsDirectContribution, orc_sample* out, size_t cap) {
+  size_t total = 0;
+  for (size_t p = 0; p < numPaths; p++) {
+    const size_t b = offsets[p], e = offsets[p + 1];
+    const size_t room = total < cap ? cap - total : 0;
+    total += orc_path_prepare_samples(segs + b, e - b, maxDistance, useNEEMiWeights, guideDirectLight,
+                                      rrAffectsDirectContribution, out + (total < cap ? total : cap), room);
+  }
+  return total;
+}
+
 void orc_path_pixel_estimate(const orc_path_segment* segs, size_t n, int rrAffectsDirectContribution, float* rgb) {
   rgb[0] = rgb[1] = rgb[2] = 0.f;
   if (n == 0) return;

@@ -167,6 +167,11 @@ void orc_set_hg_table(const float* table);  // 128 x (K, g, 4w, 4mc, 4kappa, 4no
 // most cap samples, returns how many the path yields
 size_t orc_path_prepare_samples(const orc_path_segment* segs, size_t n, float maxDistance, int useNEEMiWeights,
                                 int guideDirectLight, int rrAffectsDirectContribution, orc_sample* out, size_t cap);
+// prepareSamples of numPaths paths [offsets[p], offsets[p+1]) in order, one
+// thread; returns the total (at most cap written to out)
+size_t orc_path_prepare_batch(const orc_path_segment* segs, const uint32_t* offsets, size_t numPaths,
+                              float maxDistance, int useNEEMiWeights, int guideDirectLight,
+                              int rrAffectsDirectContribution, orc_sample* out, size_t cap);
 // PathSegmentDataStorage::calculatePixelEstimate of one path
 void orc_path_pixel_estimate(const orc_path_segment* segs, size_t n, int rrAffectsDirectContribution, float* rgb);
 
