@@ -721,11 +721,11 @@ void pglFieldGetKNN(PGLField field, int kind, uint32_t* nb) {
     }
 }
 
-static void info_values(const UpdateInfo& i, uint32_t regions, double* o) {
+static void info_values(const UpdateInfo& i, double* o) {
   o[0] = i.wallMs; o[1] = i.spatialMs; o[2] = i.directionalMs; o[3] = i.knnMs; o[4] = i.gatherMs; o[5] = i.sortMs;
   o[6] = i.kdRounds; o[7] = i.regionLaunches;
   o[8] = i.estepEvals; o[9] = i.auxEvals; o[10] = i.emIterations; o[11] = i.samples; o[12] = i.splitEvals;
-  o[13] = regions;
+  o[13] = i.regions;
   o[14] = i.regionJobs; o[15] = i.regionItems; o[16] = i.maxTeam; o[17] = i.regionGrid;
   o[18] = i.regionKernelMs;
 }
@@ -735,7 +735,7 @@ size_t pglFieldGetUpdateHistory(PGLField field, int kind, double* out, size_t ma
   KindField& k = kind_of(field, kind);
   const std::vector<UpdateInfo>& h = k.history();
   const size_t n = std::min(h.size(), maxEntries);
-  for (size_t e = 0; e < n; e++) info_values(h[h.size() - n + e], 0, out + 19 * e);
+  for (size_t e = 0; e < n; e++) info_values(h[h.size() - n + e], out + 19 * e);
   return n;
 }
 PGL_CATCH_END(0)
@@ -744,7 +744,7 @@ void pglFieldGetLastUpdateInfo(PGLField field, int kind, double* out, size_t cou
   KindField& k = kind_of(field, kind);
   const UpdateInfo& i = k.lastInfo();
   double o[19];
-  info_values(i, k.numRegions(), o);
+  info_values(i, o);
   for (size_t j = 0; j < count && j < 19; j++) out[j] = o[j];
 }
 

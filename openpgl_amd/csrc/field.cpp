@@ -693,6 +693,7 @@ void KindField::finish() const {
   PGL_HIP_CHECK(hipEventElapsedTime(&ms, evDir_[P][0], evDir_[P][1])); i.directionalMs = ms;
   PGL_HIP_CHECK(hipEventElapsedTime(&ms, evDir_[P][1], evDir_[P][2])); i.knnMs = ms;
   i.regionKernelMs = rr.kernelMs();
+  i.regions = pendingRegions_;
   i.wallMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - pendingT0_).count();
   info_ = i;
   lastTimes_[0] = (float)i.wallMs;

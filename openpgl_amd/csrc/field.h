@@ -85,6 +85,7 @@ struct UpdateInfo {
   double wallMs = 0, spatialMs = 0, directionalMs = 0, knnMs = 0, gatherMs = 0, sortMs = 0;
   double kdRounds = 0, regionLaunches = 0, estepEvals = 0, auxEvals = 0, emIterations = 0, samples = 0;
   double regionJobs = 0, regionItems = 0, maxTeam = 0, regionGrid = 0, splitEvals = 0, regionKernelMs = 0;
+  double regions = 0;  // regions after the update
 };
 
 // One guiding field (surface or volume): Field<> of field/Field.h.

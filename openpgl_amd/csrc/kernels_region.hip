@@ -71,6 +71,10 @@ struct Shared {
   float red[RWAVES][32][8];
   float fxx[32], fxy[32], fxz[32], fyx[32], fyy[32], fyz[32];
   float cost[32 * 32];
+  float selfp[32][6];      // merging: product of each component with itself (direction, kappa, norm, scale)
+  float term[64][3];       // merging: the three cost terms of up to 64 pairs
+  float mlobe[8];          // merging: the merged component of the chosen pair
+  float pre[8];            // merging: weight and direction of the chosen pair before the merge
   float scal[RWAVES][8];
   double ll;
   double prevLL, invPrevLL;
@@ -932,37 +936,71 @@ __device__ float integrated_division(float m0x, float m0y, float m0z, float k0, 
   return scale;
 }
 
-// CalculateMergeCost (Merger:93-177)
-__device__ float merge_cost(const VMMParams& V, int i0, int i1) {
-  const float w0 = V.w[i0], w1 = V.w[i1];
-  const float k0 = V.kappa[i0], k1 = V.kappa[i1];
-  const float m0x = V.mux[i0], m0y = V.muy[i0], m0z = V.muz[i0];
-  const float m1x = V.mux[i1], m1y = V.muy[i1], m1z = V.muz[i1];
-  const float c0 = V.mc[i0], c1 = V.mc[i1], n0 = V.norm[i0], n1 = V.norm[i1];
-  const float w = w0 + w1;
-  float kappa = 0.f, nrm = kOneOverFourPi, em2k = 1.f;
-  float mdx = (w0 * c0 * m0x + w1 * c1 * m1x) / w;
-  float mdy = (w0 * c0 * m0y + w1 * c1 * m1y) / w;
-  float mdz = (w0 * c0 * m0z + w1 * c1 * m1z) / w;
-  float mcos = mdx * mdx + mdy * mdy + mdz * mdz;
+// CalculateMergeCost (Merger:93-177), split into its parts: the
+// moment-matched merged lobe of the pair, the product of each component with
+// itself (cached per component while merging) and the three integrated
+// divisions, each a term of the chi-square estimate.
+struct MergedLobe {
+  float w, dx, dy, dz, kappa, nrm, em2k, mc;
+};
+
+__device__ __forceinline__ MergedLobe merged_lobe(const VMMParams& V, int i0, int i1) {
+  const float w0 = V.w[i0], w1 = V.w[i1], c0 = V.mc[i0], c1 = V.mc[i1];
+  MergedLobe m;
+  m.w = w0 + w1;
+  m.kappa = 0.f;
+  m.nrm = kOneOverFourPi;
+  m.em2k = 1.f;
+  m.dx = (w0 * c0 * V.mux[i0] + w1 * c1 * V.mux[i1]) / m.w;
+  m.dy = (w0 * c0 * V.muy[i0] + w1 * c1 * V.muy[i1]) / m.w;
+  m.dz = (w0 * c0 * V.muz[i0] + w1 * c1 * V.muz[i1]) / m.w;
+  float mcos = m.dx * m.dx + m.dy * m.dy + m.dz * m.dz;
+  m.mc = mcos;
   if (mcos > 0.f) {
     mcos = sqrtf(mcos);
-    kappa = mc_to_kappa(mcos);
-    kappa = kappa < 1e-3f ? 0.f : kappa;
-    em2k = expf(-2.f * kappa);
-    nrm = kappa / (kTwoPi * (1.f - em2k));
-    mdx /= mcos; mdy /= mcos; mdz /= mcos;
+    m.mc = mcos;
+    m.kappa = mc_to_kappa(mcos);
+    m.kappa = m.kappa < 1e-3f ? 0.f : m.kappa;
+    m.em2k = expf(-2.f * m.kappa);
+    m.nrm = m.kappa / (kTwoPi * (1.f - m.em2k));
+    m.dx /= mcos; m.dy /= mcos; m.dz /= mcos;
   } else {
-    mdx = m0x; mdy = m0y; mdz = m0z;
+    m.dx = V.mux[i0]; m.dy = V.muy[i0]; m.dz = V.muz[i0];
   }
-  float a0x, a0y, a0z, ak, an, b0x, b0y, b0z, bk, bn, c0x, c0y, c0z, ck, cn;
-  const float s00 = product_scale(m0x, m0y, m0z, k0, n0, m0x, m0y, m0z, k0, n0, a0x, a0y, a0z, ak, an);
-  const float s11 = product_scale(m1x, m1y, m1z, k1, n1, m1x, m1y, m1z, k1, n1, b0x, b0y, b0z, bk, bn);
-  const float s01 = product_scale(m0x, m0y, m0z, k0, n0, m1x, m1y, m1z, k1, n1, c0x, c0y, c0z, ck, cn);
+  return m;
+}
+
+// product of component k with itself: {direction x, y, z, kappa, norm, scale}
+__device__ __forceinline__ void self_product(const VMMParams& V, int k, float* out) {
+  const float mx = V.mux[k], my = V.muy[k], mz = V.muz[k], kk = V.kappa[k], nn = V.norm[k];
+  out[5] = product_scale(mx, my, mz, kk, nn, mx, my, mz, kk, nn, out[0], out[1], out[2], out[3], out[4]);
+}
+
+// term t of the cost of pair (i0, i1): 0 and 1 the self terms of i0 and i1,
+// 2 the cross term (both orders counted)
+__device__ __forceinline__ float merge_term(const VMMParams& V, const float (*selfp)[6], int i0, int i1, int t,
+                                            const MergedLobe& m) {
+  if (t < 2) {
+    const int k = t == 0 ? i0 : i1;
+    const float wk = V.w[k];
+    const float* sp = selfp[k];
+    return ((wk * wk) / m.w) *
+           (sp[5] * integrated_division(sp[0], sp[1], sp[2], sp[3], sp[4], -m.dx, -m.dy, -m.dz, m.kappa, m.nrm, m.em2k));
+  }
+  float cx, cy, cz, ck, cn;
+  const float s01 = product_scale(V.mux[i0], V.muy[i0], V.muz[i0], V.kappa[i0], V.norm[i0], V.mux[i1], V.muy[i1],
+                                  V.muz[i1], V.kappa[i1], V.norm[i1], cx, cy, cz, ck, cn);
+  return 2.f * ((V.w[i0] * V.w[i1]) / m.w) *
+         (s01 * integrated_division(cx, cy, cz, ck, cn, -m.dx, -m.dy, -m.dz, m.kappa, m.nrm, m.em2k));
+}
+
+// chi = ((t0 + t1) + t2) - w, each term rounded on its own
+__device__ __forceinline__ float merge_chi(float t0, float t1, float t2, float w) {
+#pragma clang fp contract(off)
   float chi = 0.f;
-  chi += ((w0 * w0) / w) * (s00 * integrated_division(a0x, a0y, a0z, ak, an, -mdx, -mdy, -mdz, kappa, nrm, em2k));
-  chi += ((w1 * w1) / w) * (s11 * integrated_division(b0x, b0y, b0z, bk, bn, -mdx, -mdy, -mdz, kappa, nrm, em2k));
-  chi += 2.f * ((w0 * w1) / w) * (s01 * integrated_division(c0x, c0y, c0z, ck, cn, -mdx, -mdy, -mdz, kappa, nrm, em2k));
+  chi += t0;
+  chi += t1;
+  chi += t2;
   chi -= w;
   return chi;
 }
@@ -975,29 +1013,13 @@ __device__ void vmm_swap(VMMParams& V, int a, int b) {
 #undef SWP
 }
 
-// PAVMM::mergeComponents (PAVMM:191-257)
-__device__ void vmm_merge(VMMParams& V, int i0, int i1) {
-  const float w0 = V.w[i0], w1 = V.w[i1], c0 = V.mc[i0], c1 = V.mc[i1];
-  float kappa = 0.f, nrm = kOneOverFourPi, e = 1.f;
-  const float w = w0 + w1;
-  float mdx = w0 * c0 * V.mux[i0] + w1 * c1 * V.mux[i1];
-  float mdy = w0 * c0 * V.muy[i0] + w1 * c1 * V.muy[i1];
-  float mdz = w0 * c0 * V.muz[i0] + w1 * c1 * V.muz[i1];
-  mdx /= w; mdy /= w; mdz /= w;
-  float mcos = mdx * mdx + mdy * mdy + mdz * mdz;
-  if (mcos > 0.f) {
-    mcos = sqrtf(mcos);
-    kappa = mc_to_kappa(mcos);
-    kappa = kappa < 1e-3f ? 0.f : kappa;
-    e = expf(-2.f * kappa);
-    nrm = kappa / (kTwoPi * (1.f - e));
-    mdx /= mcos; mdy /= mcos; mdz /= mcos;
-  } else {
-    mdx = V.mux[i0]; mdy = V.muy[i0]; mdz = V.muz[i0];
-  }
+// PAVMM::mergeComponents (PAVMM:191-257): component i0 becomes the merged
+// lobe m, the last component moves into slot i1
+__device__ void vmm_merge(VMMParams& V, int i0, int i1, const MergedLobe& m) {
+  const float w0 = V.w[i0], w1 = V.w[i1];
   const float d0 = V.dist[i0], d1 = V.dist[i1];
-  V.w[i0] = w; V.kappa[i0] = kappa; V.mc[i0] = mcos; V.norm[i0] = nrm; V.em2k[i0] = e;
-  V.mux[i0] = mdx; V.muy[i0] = mdy; V.muz[i0] = mdz;
+  V.w[i0] = m.w; V.kappa[i0] = m.kappa; V.mc[i0] = m.mc; V.norm[i0] = m.nrm; V.em2k[i0] = m.em2k;
+  V.mux[i0] = m.dx; V.muy[i0] = m.dy; V.muz[i0] = m.dz;
   float nd = w0 * d0 + w1 * d1;
   nd /= (w0 + w1);
   V.dist[i0] = nd;
@@ -1049,23 +1071,38 @@ __device__ void suff_merge(SuffStats& ST, int i, int j) {  // WEM:485-518
 }
 
 // PerformMerging with ThresholdedMergeNext (Merger:45-89, 218-279)
+//
+// Pair costs live in sh.cost[i * 32 + j] (i < j).  After a merge only the
+// pairs of the merged slot ci and of slot cj (which receives the last
+// component) change: every other pair has unchanged inputs, so the table
+// equals a full recomputation bit for bit.  The merge loop is sequential by
+// definition; its latency per step is cut by caching each component's
+// product with itself, by computing the three terms of a changed pair's
+// cost on three lanes, and by merging the mixture, the split statistics and
+// the sufficient statistics on three waves at once.
 __device__ int perform_merging(Shared& sh, SuffStats& ST, SplitStatsState& SS, const FactoryConfig& cfg) {
   const unsigned long long tMerge = wall_clock64();
   const int tid = threadIdx.x;
   VMMParams& V = sh.S.vmm;
   int total = 0;
-  // Pair costs live in sh.cost[i * 32 + j] (i < j).  After a merge only the
-  // pairs of the merged slot ci and of slot cj (which receives the last
-  // component) are recomputed; every other pair has unchanged inputs, so the
-  // table equals a full recomputation bit for bit.
-  int ri = -1, rj = -1;
-  while (V.K > sh.W) {
-    const int K = V.K;
+  if (V.K > sh.W) {
+    for (int k = tid; k < V.K; k += RT) self_product(V, k, sh.selfp[k]);
+    __syncthreads();
+    // full table
     for (int p = tid; p < 32 * 32; p += RT) {
       const int i = p >> 5, j = p & 31;
-      if (i < j && j < K && (ri < 0 || i == ri || j == ri || i == rj || j == rj)) sh.cost[p] = merge_cost(V, i, j);
+      if (i < j && j < V.K) {
+        const MergedLobe m = merged_lobe(V, i, j);
+        const float t0 = merge_term(V, sh.selfp, i, j, 0, m);
+        const float t1 = merge_term(V, sh.selfp, i, j, 1, m);
+        const float t2 = merge_term(V, sh.selfp, i, j, 2, m);
+        sh.cost[p] = merge_chi(t0, t1, t2, m.w);
+      }
     }
     __syncthreads();
+  }
+  while (V.K > sh.W) {
+    const int K = V.K;
     // merge candidate: least cost below the threshold with both components
     // holding split samples, ties to the first pair in (i, j) order -- the
     // serial scan of ThresholdedMergeNext (Merger:218-279) as an arg-min over
@@ -1094,24 +1131,54 @@ __device__ int perform_merging(Shared& sh, SuffStats& ST, SplitStatsState& SS, c
       int pm = sh.order[0];
       for (int w = 1; w < RWAVES; w++)
         if (sh.scal[w][0] < minCost || (sh.scal[w][0] == minCost && sh.order[w] < pm)) { minCost = sh.scal[w][0]; pm = sh.order[w]; }
-      const int found = pm != 0x7fffffff;
-      sh.found = found;
-      if (found) {
+      sh.found = pm != 0x7fffffff;
+      if (sh.found) {
         const int ci = pm >> 5, cj = pm & 31;
         sh.ci = ci;
         sh.cj = cj;
-        const float mIx = V.mux[ci], mIy = V.muy[ci], mIz = V.muz[ci], wI = V.w[ci];
-        const float mJx = V.mux[cj], mJy = V.muy[cj], mJz = V.muz[cj], wJ = V.w[cj];
-        vmm_merge(V, ci, cj);
-        split_stats_merge(SS, ci, cj, wI, mIx, mIy, mIz, wJ, mJx, mJy, mJz, V.w[ci], V.mux[ci], V.muy[ci], V.muz[ci]);
-        suff_merge(ST, ci, cj);
+        const MergedLobe m = merged_lobe(V, ci, cj);
+        sh.mlobe[0] = m.w; sh.mlobe[1] = m.dx; sh.mlobe[2] = m.dy; sh.mlobe[3] = m.dz;
+        sh.mlobe[4] = m.kappa; sh.mlobe[5] = m.nrm; sh.mlobe[6] = m.em2k; sh.mlobe[7] = m.mc;
+        sh.pre[0] = V.w[ci]; sh.pre[1] = V.mux[ci]; sh.pre[2] = V.muy[ci]; sh.pre[3] = V.muz[ci];
+        sh.pre[4] = V.w[cj]; sh.pre[5] = V.mux[cj]; sh.pre[6] = V.muy[cj]; sh.pre[7] = V.muz[cj];
       }
     }
     __syncthreads();
     if (!sh.found) break;
-    ri = sh.ci;
-    rj = sh.cj;
+    const int ci = sh.ci, cj = sh.cj;
+    // the three merges read the pre-merge slots and write disjoint state
+    if ((tid & 63) == 0) {
+      const int wave = tid >> 6;
+      if (wave == 0) {
+        const MergedLobe m{sh.mlobe[0], sh.mlobe[1], sh.mlobe[2], sh.mlobe[3], sh.mlobe[4], sh.mlobe[5], sh.mlobe[6],
+                           sh.mlobe[7]};
+        vmm_merge(V, ci, cj, m);
+      } else if (wave == 1) {
+        split_stats_merge(SS, ci, cj, sh.pre[0], sh.pre[1], sh.pre[2], sh.pre[3], sh.pre[4], sh.pre[5], sh.pre[6],
+                          sh.pre[7], sh.mlobe[0], sh.mlobe[1], sh.mlobe[2], sh.mlobe[3]);
+      } else if (wave == 2) {
+        suff_merge(ST, ci, cj);
+      }
+    }
+    __syncthreads();
     total++;
+    if (V.K <= sh.W) break;
+    // changed rows: ci (merged) and cj (the former last component)
+    const int Kn = V.K;
+    if (tid < 2 && (tid == 0 || cj < Kn)) self_product(V, tid == 0 ? ci : cj, sh.selfp[tid == 0 ? ci : cj]);
+    __syncthreads();
+    // pair q of the changed rows: q < Kn -> (ci, q), else (cj, q - Kn) unless
+    // that pair was already listed; three lanes per pair, one term each
+    const int q = tid / 3, t = tid - 3 * (tid / 3);
+    int i = -1, j = -1;
+    if (q < 2 * Kn) {
+      const int a0 = q < Kn ? ci : cj, b0 = q < Kn ? q : q - Kn;
+      if (a0 < Kn && b0 != a0 && !(q >= Kn && b0 == ci)) { i = min(a0, b0); j = max(a0, b0); }
+    }
+    if (i >= 0) sh.term[q][t] = merge_term(V, sh.selfp, i, j, t, merged_lobe(V, i, j));
+    __syncthreads();
+    if (i >= 0 && t == 0) sh.cost[i * 32 + j] = merge_chi(sh.term[q][0], sh.term[q][1], sh.term[q][2], V.w[i] + V.w[j]);
+    __syncthreads();
   }
   if (tid == 0) sh.S.prof[5] += (float)(wall_clock64() - tMerge);
   return total;
@@ -1364,7 +1431,7 @@ __device__ void region_job(Shared& sh, Team& tm, RegionState* __restrict__ state
 // consecutively, so a team is always completed by the next free workgroups
 // and every wait ends (given T <= grid, which the host guarantees, and a
 // co-resident grid).
-__global__ void __launch_bounds__(RT) k_region_update(RegionState* __restrict__ states,
+__global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(2, 2))) k_region_update(RegionState* __restrict__ states,
                                                       const RegionJob* __restrict__ jobs,
                                                       const uint2* __restrict__ items, uint32_t numItems,
                                                       uint32_t* __restrict__ queue, uint32_t* __restrict__ counters,
