@@ -76,6 +76,7 @@ struct Shared {
   float mlobe[8];          // merging: the merged component of the chosen pair
   float pre[8];            // merging: weight and direction of the chosen pair before the merge
   float scal[RWAVES][8];
+  float stage[5][kChunk];  // sample pass: the current chunk's dx, dy, dz, w and pdf or distance
   double ll;
   double prevLL, invPrevLL;
   float un[4];
@@ -170,7 +171,7 @@ __device__ __forceinline__ int vec_lim(int W, int K) { int l = ((K + W - 1) / W)
 // ------------------------------------------------------------ sample passes
 // Loads this lane's component parameters (k = 4j + q) into registers.
 struct CompRegs {
-  float x[8], y[8], z[8], kap[8], wn[8];
+  float x[8], y[8], z[8], kap2[8], wn[8];  // kap2 = kappa log2(e)
 };
 
 __device__ __forceinline__ void load_comps(const VMMParams& V, int q, CompRegs& c) {
@@ -182,27 +183,34 @@ __device__ __forceinline__ void load_comps(const VMMParams& V, int q, CompRegs& 
     c.x[j] = a ? V.mux[k] : 0.f;
     c.y[j] = a ? V.muy[k] : 0.f;
     c.z[j] = a ? V.muz[k] : 1.f;
-    c.kap[j] = a ? V.kappa[k] : 0.f;
+    c.kap2[j] = a ? V.kappa[k] * 1.44269504088896341f : 0.f;
     c.wn[j] = a ? V.w[k] * V.norm[k] : 0.f;
   }
 }
 
-// e_k = w_k norm_k exp(kappa_k min(mu_k.d - 1, 0)); returns the quad-wide pdf
+// sum over the 4 lanes of a quad (DPP quad permutes: lane ^ 1, then lane ^ 2);
+// every lane of the quad gets the same bits, as IEEE addition commutes
+__device__ __forceinline__ float quad_sum(float v) {
+  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, true));
+  v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, true));
+  return v;
+}
+
+// e_k = w_k norm_k exp(kappa_k min(mu_k.d - 1, 0)), evaluated as
+// w_k norm_k 2^min(kap2_k mu_k.d - kap2_k, 0); returns the quad-wide pdf
 __device__ __forceinline__ float soft_eval(const CompRegs& c, int J, float dx, float dy, float dz, float e[8]) {
   float part = 0.f;
 #pragma unroll
   for (int j = 0; j < 8; j++) {
     if (j < J) {
       const float ct = dot3f(dx, dy, dz, c.x[j], c.y[j], c.z[j]);
-      e[j] = c.wn[j] * __expf(c.kap[j] * fminf(ct - 1.f, 0.f));
+      e[j] = c.wn[j] * __builtin_amdgcn_exp2f(fminf(fmaf(c.kap2[j], ct, -c.kap2[j]), 0.f));
       part += e[j];
     } else {
       e[j] = 0.f;
     }
   }
-  float pdf = part + __shfl_xor(part, 1);
-  pdf = pdf + __shfl_xor(pdf, 2);
-  return pdf;
+  return quad_sum(part);
 }
 
 __device__ __forceinline__ float quad_group_sum(float v) {  // over the 16 quads of a wave (same q)
@@ -304,6 +312,37 @@ __device__ void pass_finish(Shared& sh, Team& tm, int llIdx, float fs, float fc,
   __syncthreads();
 }
 
+// Copies samples [first, first + cnt) of the region's stream into the
+// staging arrays (dx, dy, dz, w, and pdf or distance when extra is set):
+// every thread issues all of its coalesced loads at once, so a chunk costs
+// one memory latency instead of one per sample step.  Callers synchronise
+// before reading the stage.
+__device__ __forceinline__ void stage_chunk(Shared& sh, const RegionSamples& rs, uint32_t first, uint32_t cnt,
+                                            const float* __restrict__ extra) {
+  const int tid = threadIdx.x;
+  float v[kChunk / RT][5];
+#pragma unroll
+  for (int m = 0; m < (int)(kChunk / RT); m++) {
+    const uint32_t o = tid + m * RT;
+    if (o < cnt) {
+      const uint32_t i = first + o;
+      v[m][0] = rs.dx[i];
+      v[m][1] = rs.dy[i];
+      v[m][2] = rs.dz[i];
+      v[m][3] = rs.w[i];
+      v[m][4] = extra ? extra[i] : 0.f;
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < (int)(kChunk / RT); m++) {
+    const uint32_t o = tid + m * RT;
+    if (o < cnt) {
+#pragma unroll
+      for (int a = 0; a < 5; a++) sh.stage[a][o] = v[m][a];
+    }
+  }
+}
+
 // E-step (WEM:847-892).  Result: fin[f*32+k] = sum over samples of
 // gamma_k w (dx, dy, dz, 1) for f = 0..3, finLL = sum w log pdf,
 // fin[kScal+0..3] = unassigned (w, w dx, w dy, w dz).
@@ -319,27 +358,28 @@ __device__ void pass_estep(Shared& sh, Team& tm, const RegionSamples& rs, uint32
   double fd = 0.0;
   for (uint32_t ch = tm.rank; ch < tm.nc; ch += tm.T) {
     const uint32_t cb = ch * kChunk, ce = min(n, cb + kChunk);
+    stage_chunk(sh, rs, base + cb, ce - cb, nullptr);
+    __syncthreads();
     float bx[8], by[8], bz[8], bw[8];
 #pragma unroll
     for (int j = 0; j < 8; j++) bx[j] = by[j] = bz[j] = bw[j] = 0.f;
     float ll = 0.f, u0 = 0.f, u1 = 0.f, u2 = 0.f, u3 = 0.f;
-    for (uint32_t s = cb + quad; s < ce; s += RT / 4) {
-      const uint32_t i = base + s;
-      const float dx = rs.dx[i], dy = rs.dy[i], dz = rs.dz[i], w = rs.w[i];
+    for (uint32_t s = quad; s < ce - cb; s += RT / 4) {
+      const float dx = sh.stage[0][s], dy = sh.stage[1][s], dz = sh.stage[2][s], w = sh.stage[3][s];
       float e[8];
       const float pdf = soft_eval(c, J, dx, dy, dz, e);
       if (pdf <= 1e-16f) {
         if (q == 0) { u0 += w; u1 += dx * w; u2 += dy * w; u3 += dz * w; }
       } else {
         if (q == 0) ll += w * logf(pdf);
-        const float r = w / pdf;
+        const float r = w * __builtin_amdgcn_rcpf(pdf);
+        const float rx = r * dx, ry = r * dy, rz = r * dz;
 #pragma unroll
         for (int j = 0; j < 8; j++) {
-          const float g = e[j] * r;
-          bx[j] = fmaf(g, dx, bx[j]);
-          by[j] = fmaf(g, dy, by[j]);
-          bz[j] = fmaf(g, dz, bz[j]);
-          bw[j] += g;
+          bx[j] = fmaf(e[j], rx, bx[j]);
+          by[j] = fmaf(e[j], ry, by[j]);
+          bz[j] = fmaf(e[j], rz, bz[j]);
+          bw[j] = fmaf(e[j], r, bw[j]);
         }
       }
     }
@@ -399,30 +439,31 @@ __device__ void pass_distances(Shared& sh, Team& tm, const RegionSamples& rs, ui
   double fd = 0.0;
   for (uint32_t ch = tm.rank; ch < tm.nc; ch += tm.T) {
     const uint32_t cb = ch * kChunk, ce = min(n, cb + kChunk);
+    stage_chunk(sh, rs, base + cb, ce - cb, rs.dist);
+    __syncthreads();
     float ad[8], aw[8];
 #pragma unroll
     for (int j = 0; j < 8; j++) ad[j] = aw[j] = 0.f;
-    for (uint32_t s = cb + quad; s < ce; s += RT / 4) {
-      const uint32_t i = base + s;
-      const float dx = rs.dx[i], dy = rs.dy[i], dz = rs.dz[i], w = rs.w[i];
-      const float invd = 1.f / rs.dist[i];
+    for (uint32_t s = quad; s < ce - cb; s += RT / 4) {
+      const float dx = sh.stage[0][s], dy = sh.stage[1][s], dz = sh.stage[2][s], w = sh.stage[3][s];
       float e[8];
       const float pdf = soft_eval(c, J, dx, dy, dz, e);
       if (pdf > 1e-16f) {
-        const float ip = 1.f / pdf;
+        // w g_k (g_k pdf / w_k) = (w / pdf) e_k^2 / w_k; 1 / w_k is applied to the chunk sums
+        const float wip = w * __builtin_amdgcn_rcpf(pdf);
+        const float wipd = wip * __builtin_amdgcn_rcpf(sh.stage[4][s]);
 #pragma unroll
         for (int j = 0; j < 8; j++) {
-          const float g = e[j] * ip;
-          const float wt = w * g * (e[j] * invw[j]);
-          ad[j] = fmaf(wt, invd, ad[j]);
-          aw[j] += wt;
+          const float e2 = e[j] * e[j];
+          ad[j] = fmaf(e2, wipd, ad[j]);
+          aw[j] = fmaf(e2, wip, aw[j]);
         }
       }
     }
 #pragma unroll
     for (int j = 0; j < 8; j++) {
-      const float vd = quad_group_sum(ad[j]);
-      const float vw = quad_group_sum(aw[j]);
+      const float vd = quad_group_sum(ad[j]) * invw[j];
+      const float vw = quad_group_sum(aw[j]) * invw[j];
       if (lane < 4) {
         sh.red[wave][4 * j + lane][0] = vd;
         sh.red[wave][4 * j + lane][1] = vw;
@@ -453,35 +494,35 @@ __device__ void pass_split(Shared& sh, Team& tm, const RegionSamples& rs, uint32
   double fd = 0.0;
   for (uint32_t ch = tm.rank; ch < tm.nc; ch += tm.T) {
     const uint32_t cb = ch * kChunk, ce = min(n, cb + kChunk);
+    stage_chunk(sh, rs, base + cb, ce - cb, rs.pdf);
+    __syncthreads();
     float achi[8], asa[8], asw[8], acx[8], acy[8], acz[8];
 #pragma unroll
     for (int j = 0; j < 8; j++) achi[j] = asa[j] = asw[j] = acx[j] = acy[j] = acz[j] = 0.f;
     float cnt = 0.f;
-    for (uint32_t s = cb + quad; s < ce; s += RT / 4) {
-      const uint32_t i = base + s;
-      const float dx = rs.dx[i], dy = rs.dy[i], dz = rs.dz[i], w = rs.w[i], spdf = rs.pdf[i];
+    for (uint32_t s = quad; s < ce - cb; s += RT / 4) {
+      const float dx = sh.stage[0][s], dy = sh.stage[1][s], dz = sh.stage[2][s], w = sh.stage[3][s];
+      const float spdf = sh.stage[4][s];
       float e[8];
       const float pdf = soft_eval(c, J, dx, dy, dz, e);
       if (pdf > 1e-16f) {
         cnt += 1.f;
-        const float ip = 1.f / pdf;
+        const float ip = __builtin_amdgcn_rcpf(pdf);
         const float value = w * spdf;
         const float a = value * inv_mc * ip;  // value / (mc pdf)
-        const float ispdf = 1.f / spdf;
+        // (value^2 vmf / (mc pdf)^2 - 2 vmf value / (mc pdf) + vmf) / spdf = vmf cf
+        const float cf = (a * a - 2.f * a + 1.f) * __builtin_amdgcn_rcpf(spdf);
+        const float wip = w * ip;
 #pragma unroll
         for (int j = 0; j < 8; j++) {
           if (j < J) {
             const int k = 4 * j + q;
-            const float g = e[j] * ip;
             const float vmf = e[j];
-            // (value^2 vmf / (mc pdf)^2 - 2 vmf value / (mc pdf) + vmf) / spdf
-            float chi = vmf * (a * a - 2.f * a + 1.f) * ispdf;
-            chi = g > 0.f ? chi : 0.f;
-            achi[j] += chi;
-            asa[j] += g;
+            achi[j] += vmf > 0.f ? vmf * cf : 0.f;
+            asa[j] = fmaf(vmf, ip, asa[j]);
             const float lx = dot3f(sh.fxx[k], sh.fxy[k], sh.fxz[k], dx, dy, dz);
             const float ly = dot3f(sh.fyx[k], sh.fyy[k], sh.fyz[k], dx, dy, dz);
-            const float aw = g * w;
+            const float aw = vmf * wip;
             asw[j] += aw;
             acx[j] = fmaf(aw, lx * lx, acx[j]);
             acy[j] = fmaf(aw, ly * ly, acy[j]);
