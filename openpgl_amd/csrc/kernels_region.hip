@@ -213,12 +213,35 @@ __device__ __forceinline__ float soft_eval(const CompRegs& c, int J, float dx, f
   return quad_sum(part);
 }
 
-__device__ __forceinline__ float quad_group_sum(float v) {  // over the 16 quads of a wave (same q)
-  v += __shfl_xor(v, 4);
-  v += __shfl_xor(v, 8);
-  v += __shfl_xor(v, 16);
-  v += __shfl_xor(v, 32);
-  return v;
+// Cross-lane sums without LDS traffic: DPP row rotations inside each row of
+// 16 lanes, then the gfx950 row swaps (permlane16_swap exchanges the odd rows
+// of its first operand with the even rows of its second, permlane32_swap the
+// upper half of the first with the lower half of the second; with both
+// operands equal, the two results added give lane i the sum with lane i ^ 16
+// or i ^ 32).
+#define PGL_ROW_ROR(v, n) __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x120 + (n), 0xF, 0xF, false))
+__device__ __forceinline__ float cross_row_sum(float v) {
+  const auto a = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+  v = __uint_as_float(a[0]) + __uint_as_float(a[1]);
+  const auto b = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+  return __uint_as_float(b[0]) + __uint_as_float(b[1]);
+}
+
+// sum over the 16 lanes of a wave with the same lane % 4 (the quads' lane q);
+// lanes 0..3 hold the sums of q = 0..3
+__device__ __forceinline__ float quad_group_sum(float v) {
+  v += PGL_ROW_ROR(v, 4);
+  v += PGL_ROW_ROR(v, 8);
+  return cross_row_sum(v);
+}
+
+// sum over the 64 lanes of a wave, valid in lane 0
+__device__ __forceinline__ float wave_sum0(float v) {
+  v += PGL_ROW_ROR(v, 1);
+  v += PGL_ROW_ROR(v, 2);
+  v += PGL_ROW_ROR(v, 4);
+  v += PGL_ROW_ROR(v, 8);
+  return cross_row_sum(v);
 }
 
 // Kahan step of the cross-chunk fold
@@ -398,8 +421,8 @@ __device__ void pass_estep(Shared& sh, Team& tm, const RegionSamples& rs, uint32
       }
     }
     // lanes q != 0 hold zeros for the per-sample scalars
-    const float vll = wave_sum(ll);
-    const float v0 = wave_sum(u0), v1 = wave_sum(u1), v2 = wave_sum(u2), v3 = wave_sum(u3);
+    const float vll = wave_sum0(ll);
+    const float v0 = wave_sum0(u0), v1 = wave_sum0(u1), v2 = wave_sum0(u2), v3 = wave_sum0(u3);
     if (lane == 0) {
       sh.scal[wave][0] = vll;
       sh.scal[wave][1] = v0;
@@ -550,7 +573,7 @@ __device__ void pass_split(Shared& sh, Team& tm, const RegionSamples& rs, uint32
       }
     }
     // every lane of a quad counted the sample: divide the wave sum by 4
-    const float vc = wave_sum(cnt) * 0.25f;
+    const float vc = wave_sum0(cnt) * 0.25f;
     if (lane == 0) sh.scal[wave][0] = vc;
     __syncthreads();
     chunk_out(sh, tm, ch, 6, 1, -1, fs, fc, fd);
