@@ -42,6 +42,8 @@
 //   field/Field.h                           updateRegions 441-511 (decay, parallax shift)
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "common.h"
 #include "kernels.h"
 
@@ -194,6 +196,36 @@ __device__ __forceinline__ float quad_sum(float v) {
   v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, true));
   v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, true));
   return v;
+}
+
+// Calls f with std::integral_constant<int, JT> for the smallest even JT >= J
+// (J = component groups in use), so that the sample loops are unrolled over
+// a fixed number of groups with no branch between components; the groups in
+// [J, JT) hold zero-weight components.
+template <class F>
+__device__ __forceinline__ void with_groups(int J, F&& f) {
+  if (J > 6) f(std::integral_constant<int, 8>());
+  else if (J > 4) f(std::integral_constant<int, 6>());
+  else if (J > 2) f(std::integral_constant<int, 4>());
+  else f(std::integral_constant<int, 2>());
+}
+
+// soft_eval for the first JT component groups of this lane (e[j] = 0 for
+// j >= JT), with no branch between the components
+template <int JT>
+__device__ __forceinline__ float soft_eval_t(const CompRegs& c, float dx, float dy, float dz, float e[8]) {
+  float part = 0.f;
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    if (j < JT) {
+      const float ct = dot3f(dx, dy, dz, c.x[j], c.y[j], c.z[j]);
+      e[j] = c.wn[j] * __builtin_amdgcn_exp2f(fminf(fmaf(c.kap2[j], ct, -c.kap2[j]), 0.f));
+      part += e[j];
+    } else {
+      e[j] = 0.f;
+    }
+  }
+  return quad_sum(part);
 }
 
 // e_k = w_k norm_k exp(kappa_k min(mu_k.d - 1, 0)), evaluated as
@@ -366,6 +398,55 @@ __device__ __forceinline__ void stage_chunk(Shared& sh, const RegionSamples& rs,
   }
 }
 
+// Per-lane E-step sums of a chunk.
+struct EStepAcc {
+  float* bx;
+  float* by;
+  float* bz;
+  float* bw;
+  float& ll;
+  float& u0;
+  float& u1;
+  float& u2;
+  float& u3;
+};
+
+// One sample of the E-step, without branches: a sample the mixture does not
+// cover (pdf <= 1e-16) adds zero to the component sums and its weight to the
+// unassigned sums (lane q = 0 of the quad, like the log-likelihood).
+template <int JT>
+__device__ __forceinline__ void estep_sample(const CompRegs& c, int q, float dx, float dy, float dz, float w,
+                                             EStepAcc& a) {
+  float e[8];
+  const float pdf = soft_eval_t<JT>(c, dx, dy, dz, e);
+  const bool ok = pdf > 1e-16f;
+  const float wu = (!ok && q == 0) ? w : 0.f;
+  a.u0 += wu;
+  a.u1 = fmaf(dx, wu, a.u1);
+  a.u2 = fmaf(dy, wu, a.u2);
+  a.u3 = fmaf(dz, wu, a.u3);
+  const float lp = fmaf(w, logf(pdf), a.ll);
+  a.ll = (ok && q == 0) ? lp : a.ll;
+  const float r = ok ? w * __builtin_amdgcn_rcpf(pdf) : 0.f;
+  const float rx = r * dx, ry = r * dy, rz = r * dz;
+#pragma unroll
+  for (int j = 0; j < JT; j++) {
+    a.bx[j] = fmaf(e[j], rx, a.bx[j]);
+    a.by[j] = fmaf(e[j], ry, a.by[j]);
+    a.bz[j] = fmaf(e[j], rz, a.bz[j]);
+    a.bw[j] = fmaf(e[j], r, a.bw[j]);
+  }
+}
+
+// The staged chunk's samples s = quad, quad + 64, ... in that order, two per
+// step so that the evaluation of one overlaps the sums of the other.
+template <int JT>
+__device__ __forceinline__ void estep_chunk(const Shared& sh, const CompRegs& c, int q, int quad, uint32_t cnt,
+                                            EStepAcc& a) {
+  for (uint32_t s = quad; s < cnt; s += RT / 4)
+    estep_sample<JT>(c, q, sh.stage[0][s], sh.stage[1][s], sh.stage[2][s], sh.stage[3][s], a);
+}
+
 // E-step (WEM:847-892).  Result: fin[f*32+k] = sum over samples of
 // gamma_k w (dx, dy, dz, 1) for f = 0..3, finLL = sum w log pdf,
 // fin[kScal+0..3] = unassigned (w, w dx, w dy, w dz).
@@ -387,25 +468,8 @@ __device__ void pass_estep(Shared& sh, Team& tm, const RegionSamples& rs, uint32
 #pragma unroll
     for (int j = 0; j < 8; j++) bx[j] = by[j] = bz[j] = bw[j] = 0.f;
     float ll = 0.f, u0 = 0.f, u1 = 0.f, u2 = 0.f, u3 = 0.f;
-    for (uint32_t s = quad; s < ce - cb; s += RT / 4) {
-      const float dx = sh.stage[0][s], dy = sh.stage[1][s], dz = sh.stage[2][s], w = sh.stage[3][s];
-      float e[8];
-      const float pdf = soft_eval(c, J, dx, dy, dz, e);
-      if (pdf <= 1e-16f) {
-        if (q == 0) { u0 += w; u1 += dx * w; u2 += dy * w; u3 += dz * w; }
-      } else {
-        if (q == 0) ll += w * logf(pdf);
-        const float r = w * __builtin_amdgcn_rcpf(pdf);
-        const float rx = r * dx, ry = r * dy, rz = r * dz;
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-          bx[j] = fmaf(e[j], rx, bx[j]);
-          by[j] = fmaf(e[j], ry, by[j]);
-          bz[j] = fmaf(e[j], rz, bz[j]);
-          bw[j] = fmaf(e[j], r, bw[j]);
-        }
-      }
-    }
+    EStepAcc a{bx, by, bz, bw, ll, u0, u1, u2, u3};
+    with_groups(J, [&](auto jt) { estep_chunk<decltype(jt)::value>(sh, c, q, quad, ce - cb, a); });
 #pragma unroll
     for (int j = 0; j < 8; j++) {
       const float vx = quad_group_sum(bx[j]);
@@ -467,22 +531,25 @@ __device__ void pass_distances(Shared& sh, Team& tm, const RegionSamples& rs, ui
     float ad[8], aw[8];
 #pragma unroll
     for (int j = 0; j < 8; j++) ad[j] = aw[j] = 0.f;
-    for (uint32_t s = quad; s < ce - cb; s += RT / 4) {
-      const float dx = sh.stage[0][s], dy = sh.stage[1][s], dz = sh.stage[2][s], w = sh.stage[3][s];
-      float e[8];
-      const float pdf = soft_eval(c, J, dx, dy, dz, e);
-      if (pdf > 1e-16f) {
+    const uint32_t cnt = ce - cb;
+    with_groups(J, [&](auto jt) {
+      constexpr int JT = decltype(jt)::value;
+      for (uint32_t s = quad; s < cnt; s += RT / 4) {
+        const float dx = sh.stage[0][s], dy = sh.stage[1][s], dz = sh.stage[2][s], w = sh.stage[3][s];
+        float e[8];
+        const float pdf = soft_eval_t<JT>(c, dx, dy, dz, e);
+        const bool ok = pdf > 1e-16f;  // other samples add zero
         // w g_k (g_k pdf / w_k) = (w / pdf) e_k^2 / w_k; 1 / w_k is applied to the chunk sums
-        const float wip = w * __builtin_amdgcn_rcpf(pdf);
-        const float wipd = wip * __builtin_amdgcn_rcpf(sh.stage[4][s]);
+        const float wip = ok ? w * __builtin_amdgcn_rcpf(pdf) : 0.f;
+        const float wipd = ok ? wip * __builtin_amdgcn_rcpf(sh.stage[4][s]) : 0.f;
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
+        for (int j = 0; j < JT; j++) {
           const float e2 = e[j] * e[j];
           ad[j] = fmaf(e2, wipd, ad[j]);
           aw[j] = fmaf(e2, wip, aw[j]);
         }
       }
-    }
+    });
 #pragma unroll
     for (int j = 0; j < 8; j++) {
       const float vd = quad_group_sum(ad[j]) * invw[j];
@@ -1069,34 +1136,14 @@ __device__ __forceinline__ float merge_chi(float t0, float t1, float t2, float w
   return chi;
 }
 
-__device__ void vmm_swap(VMMParams& V, int a, int b) {
-  if (a == b) return;
-  float t;
-#define SWP(arr) t = V.arr[a]; V.arr[a] = V.arr[b]; V.arr[b] = t;
-  SWP(w) SWP(kappa) SWP(em2k) SWP(mc) SWP(norm) SWP(mux) SWP(muy) SWP(muz) SWP(dist)
-#undef SWP
-}
-
-// PAVMM::mergeComponents (PAVMM:191-257): component i0 becomes the merged
-// lobe m, the last component moves into slot i1
-__device__ void vmm_merge(VMMParams& V, int i0, int i1, const MergedLobe& m) {
-  const float w0 = V.w[i0], w1 = V.w[i1];
-  const float d0 = V.dist[i0], d1 = V.dist[i1];
-  V.w[i0] = m.w; V.kappa[i0] = m.kappa; V.mc[i0] = m.mc; V.norm[i0] = m.nrm; V.em2k[i0] = m.em2k;
-  V.mux[i0] = m.dx; V.muy[i0] = m.dy; V.muz[i0] = m.dz;
-  float nd = w0 * d0 + w1 * d1;
-  nd /= (w0 + w1);
-  V.dist[i0] = nd;
-  vmm_swap(V, i1, V.K - 1);
-  clear_comp(V, V.K - 1);
-  V.K -= 1;
-}
-
-// ComponentSplitStatistics::mergeComponentStats, zero-mean variant (Splitter:994-1100)
-__device__ void split_stats_merge(SplitStatsState& SS, int i, int j, float wI, float mIx, float mIy, float mIz,
-                                  float wJ, float mJx, float mJy, float mJz, float wK, float mKx, float mKy,
-                                  float mKz) {
-  const int L = SS.K - 1;
+// ComponentSplitStatistics::mergeComponentStats, zero-mean variant
+// (Splitter:994-1100): the statistics of slot i after merging slot j into it.
+struct SplitSlot {
+  float covx, covy, covz, ssw, sn, ssa, chi;
+};
+__device__ __forceinline__ SplitSlot split_merged_slot(const SplitStatsState& SS, int i, int j, float wI, float mIx,
+                                                       float mIy, float mIz, float wJ, float mJx, float mJy,
+                                                       float mJz, float wK, float mKx, float mKy, float mKz) {
   float xx, xy, xz, yx, yy, yz;
   frame_of(mKx, mKy, mKz, xx, xy, xz, yx, yy, yz);
   float aix, aiy, ajx, ajy;
@@ -1113,25 +1160,115 @@ __device__ void split_stats_merge(SplitStatsState& SS, int i, int j, float wI, f
   const float mJJ[3] = {ajx * ajx, ajy * ajy, ajx * ajy};
   float cK[3];
   for (int d = 0; d < 3; d++) cK[d] = (wI * cI[d] + wI * mII[d] + wJ * cJ[d] + wJ * mJJ[d]) * invWK;
-  const float saK = SS.ssa[i] + SS.ssa[j];
-  const float nK = invWK * (wI * SS.sn[i] + wJ * SS.sn[j]);
-  const float chiK = SS.chi[i] + SS.chi[j];
-  SS.covx[i] = cK[0] * swK; SS.covy[i] = cK[1] * swK; SS.covz[i] = cK[2] * swK;
-  SS.ssw[i] = swK; SS.sn[i] = nK; SS.ssa[i] = saK; SS.chi[i] = chiK;
-  SS.covx[j] = SS.covx[L]; SS.covy[j] = SS.covy[L]; SS.covz[j] = SS.covz[L];
-  SS.ssw[j] = SS.ssw[L]; SS.sn[j] = SS.sn[L]; SS.ssa[j] = SS.ssa[L]; SS.chi[j] = SS.chi[L];
-  split_zero_slot(SS, L);
-  SS.K--;
+  SplitSlot o;
+  o.ssa = SS.ssa[i] + SS.ssa[j];
+  o.sn = invWK * (wI * SS.sn[i] + wJ * SS.sn[j]);
+  o.chi = SS.chi[i] + SS.chi[j];
+  o.covx = cK[0] * swK; o.covy = cK[1] * swK; o.covz = cK[2] * swK;
+  o.ssw = swK;
+  return o;
 }
 
-__device__ void suff_merge(SuffStats& ST, int i, int j) {  // WEM:485-518
-  const int L = ST.K - 1;
-  ST.swdx[i] += ST.swdx[j]; ST.swdy[i] += ST.swdy[j]; ST.swdz[i] += ST.swdz[j];
-  ST.sw[i] += ST.sw[j]; ST.sdw[i] += ST.sdw[j];
-  ST.swdx[j] = ST.swdx[L]; ST.swdy[j] = ST.swdy[L]; ST.swdz[j] = ST.swdz[L];
-  ST.sw[j] = ST.sw[L]; ST.sdw[j] = ST.sdw[L];
-  suff_zero_slot(ST, L);
-  ST.K--;
+// lexicographic (cost, pair index) minimum
+__device__ __forceinline__ void argmin_take(float& best, int& bestP, float ob, int op) {
+  if (ob < best || (ob == best && op < bestP)) { best = ob; bestP = op; }
+}
+#define PGL_ROW_ROR_I(v, n) __builtin_amdgcn_mov_dpp((v), 0x120 + (n), 0xF, 0xF, false)
+
+// wave-wide lexicographic minimum of (best, bestP): DPP row rotations, then
+// the row swaps; every lane ends with the same pair index
+__device__ __forceinline__ int wave_argmin(float best, int bestP) {
+  argmin_take(best, bestP, PGL_ROW_ROR(best, 1), PGL_ROW_ROR_I(bestP, 1));
+  argmin_take(best, bestP, PGL_ROW_ROR(best, 2), PGL_ROW_ROR_I(bestP, 2));
+  argmin_take(best, bestP, PGL_ROW_ROR(best, 4), PGL_ROW_ROR_I(bestP, 4));
+  argmin_take(best, bestP, PGL_ROW_ROR(best, 8), PGL_ROW_ROR_I(bestP, 8));
+  {
+    const auto a = __builtin_amdgcn_permlane16_swap(__float_as_uint(best), __float_as_uint(best), false, false);
+    const auto b = __builtin_amdgcn_permlane16_swap((unsigned)bestP, (unsigned)bestP, false, false);
+    best = __uint_as_float(a[0]);
+    bestP = (int)b[0];
+    argmin_take(best, bestP, __uint_as_float(a[1]), (int)b[1]);
+  }
+  {
+    const auto a = __builtin_amdgcn_permlane32_swap(__float_as_uint(best), __float_as_uint(best), false, false);
+    const auto b = __builtin_amdgcn_permlane32_swap((unsigned)bestP, (unsigned)bestP, false, false);
+    best = __uint_as_float(a[0]);
+    bestP = (int)b[0];
+    argmin_take(best, bestP, __uint_as_float(a[1]), (int)b[1]);
+  }
+  return __builtin_amdgcn_readfirstlane(bestP);
+}
+
+// One merge on wave 0 (lanes 0..31 own component slots): pair (ci, cj) is
+// merged into slot ci (PAVMM::mergeComponents PAVMM:191-257, the split
+// statistics' mergeComponentStats and SufficientStatistics::mergeComponents
+// WEM:485-518) and the last slot of each state moves into slot cj.  Every
+// lane computes the merged values alike; each slot lane reads all old values
+// before it writes, and LDS operations of one wave complete in order.
+__device__ void merge_step_wave0(Shared& sh, SuffStats& ST, SplitStatsState& SS, int ci, int cj) {
+  const int lane = threadIdx.x & 63;
+  VMMParams& V = sh.S.vmm;
+  const int LV = V.K - 1, LS = SS.K - 1, LT = ST.K - 1;
+  const MergedLobe m = merged_lobe(V, ci, cj);
+  float msp[6];  // product of the merged lobe with itself
+  msp[5] = product_scale(m.dx, m.dy, m.dz, m.kappa, m.nrm, m.dx, m.dy, m.dz, m.kappa, m.nrm, msp[0], msp[1], msp[2],
+                         msp[3], msp[4]);
+  const float w0 = V.w[ci], w1 = V.w[cj];
+  const float d0 = V.dist[ci], d1 = V.dist[cj];
+  float nd = w0 * d0 + w1 * d1;
+  nd /= (w0 + w1);
+  const SplitSlot ns = split_merged_slot(SS, ci, cj, w0, V.mux[ci], V.muy[ci], V.muz[ci], w1, V.mux[cj], V.muy[cj],
+                                         V.muz[cj], m.w, m.dx, m.dy, m.dz);
+  // merged sufficient statistics of slot ci
+  const float tdx = ST.swdx[ci] + ST.swdx[cj], tdy = ST.swdy[ci] + ST.swdy[cj], tdz = ST.swdz[ci] + ST.swdz[cj];
+  const float tsw = ST.sw[ci] + ST.sw[cj], tsd = ST.sdw[ci] + ST.sdw[cj];
+  if (lane < 32) {
+    const int k = lane;
+    // the last slots as they stand once slot ci holds the merge
+    const float lw = V.w[LV], lk = V.kappa[LV], lmc = V.mc[LV], ln = V.norm[LV], le = V.em2k[LV];
+    const float lx = V.mux[LV], ly = V.muy[LV], lz = V.muz[LV], ld = V.dist[LV];
+    const bool sI = LS == ci;
+    const float schi = sI ? ns.chi : SS.chi[LS], scx = sI ? ns.covx : SS.covx[LS], scy = sI ? ns.covy : SS.covy[LS];
+    const float scz = sI ? ns.covz : SS.covz[LS], ssn = sI ? ns.sn : SS.sn[LS], ssw = sI ? ns.ssw : SS.ssw[LS];
+    const float ssa = sI ? ns.ssa : SS.ssa[LS];
+    const bool tI = LT == ci;
+    const float udx = tI ? tdx : ST.swdx[LT], udy = tI ? tdy : ST.swdy[LT], udz = tI ? tdz : ST.swdz[LT];
+    const float usw = tI ? tsw : ST.sw[LT], usd = tI ? tsd : ST.sdw[LT];
+    float lsp[6];
+    for (int a = 0; a < 6; a++) lsp[a] = sh.selfp[LV][a];
+    if (k == LV) {
+      clear_comp(V, k);
+    } else if (k == cj) {
+      V.w[k] = lw; V.kappa[k] = lk; V.mc[k] = lmc; V.norm[k] = ln; V.em2k[k] = le;
+      V.mux[k] = lx; V.muy[k] = ly; V.muz[k] = lz; V.dist[k] = ld;
+      for (int a = 0; a < 6; a++) sh.selfp[k][a] = lsp[a];
+    } else if (k == ci) {
+      V.w[k] = m.w; V.kappa[k] = m.kappa; V.mc[k] = m.mc; V.norm[k] = m.nrm; V.em2k[k] = m.em2k;
+      V.mux[k] = m.dx; V.muy[k] = m.dy; V.muz[k] = m.dz; V.dist[k] = nd;
+      for (int a = 0; a < 6; a++) sh.selfp[k][a] = msp[a];
+    }
+    if (k == LS) {
+      split_zero_slot(SS, k);
+    } else if (k == cj) {
+      SS.chi[k] = schi; SS.covx[k] = scx; SS.covy[k] = scy; SS.covz[k] = scz; SS.sn[k] = ssn; SS.ssw[k] = ssw;
+      SS.ssa[k] = ssa;
+    } else if (k == ci) {
+      SS.chi[k] = ns.chi; SS.covx[k] = ns.covx; SS.covy[k] = ns.covy; SS.covz[k] = ns.covz; SS.sn[k] = ns.sn;
+      SS.ssw[k] = ns.ssw; SS.ssa[k] = ns.ssa;
+    }
+    if (k == LT) {
+      suff_zero_slot(ST, k);
+    } else if (k == cj) {
+      ST.swdx[k] = udx; ST.swdy[k] = udy; ST.swdz[k] = udz; ST.sw[k] = usw; ST.sdw[k] = usd;
+    } else if (k == ci) {
+      ST.swdx[k] = tdx; ST.swdy[k] = tdy; ST.swdz[k] = tdz; ST.sw[k] = tsw; ST.sdw[k] = tsd;
+    }
+  }
+  if (lane == 0) {
+    V.K = LV;
+    SS.K = LS;
+    ST.K = LT;
+  }
 }
 
 // PerformMerging with ThresholdedMergeNext (Merger:45-89, 218-279)
@@ -1140,13 +1277,16 @@ __device__ void suff_merge(SuffStats& ST, int i, int j) {  // WEM:485-518
 // pairs of the merged slot ci and of slot cj (which receives the last
 // component) change: every other pair has unchanged inputs, so the table
 // equals a full recomputation bit for bit.  The merge loop is sequential by
-// definition; its latency per step is cut by caching each component's
-// product with itself, by computing the three terms of a changed pair's
-// cost on three lanes, and by merging the mixture, the split statistics and
-// the sufficient statistics on three waves at once.
+// definition.  A step's choice and merge run on wave 0 alone, with no
+// workgroup barrier: the arg-min over the table (least cost below the
+// threshold with both components holding split samples, ties to the first
+// pair in (i, j) order -- the serial scan of ThresholdedMergeNext), then the
+// merge itself.  The three cost terms of the changed pairs are computed on
+// waves 0..2, one pair per lane, between two barriers.  Each component's
+// product with itself is cached and moves with the component.
 __device__ int perform_merging(Shared& sh, SuffStats& ST, SplitStatsState& SS, const FactoryConfig& cfg) {
   const unsigned long long tMerge = wall_clock64();
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   VMMParams& V = sh.S.vmm;
   int total = 0;
   if (V.K > sh.W) {
@@ -1164,85 +1304,49 @@ __device__ int perform_merging(Shared& sh, SuffStats& ST, SplitStatsState& SS, c
       }
     }
     __syncthreads();
-  }
-  while (V.K > sh.W) {
-    const int K = V.K;
-    // merge candidate: least cost below the threshold with both components
-    // holding split samples, ties to the first pair in (i, j) order -- the
-    // serial scan of ThresholdedMergeNext (Merger:218-279) as an arg-min over
-    // (cost, pair index)
-    float best = 3.402823466e38f;
-    int bestP = 0x7fffffff;
-    for (int p = tid; p < 32 * 32; p += RT) {
-      const int i = p >> 5, j = p & 31;
-      if (i < j && j < K) {
-        const float cst = sh.cost[p];
-        if (cst < cfg.mergingThreshold && SS.sn[i] > 0.f && SS.sn[j] > 0.f && (cst < best || (cst == best && p < bestP))) {
-          best = cst;
-          bestP = p;
-        }
-      }
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-      const float ob = __shfl_xor(best, off);
-      const int op = __shfl_xor(bestP, off);
-      if (ob < best || (ob == best && op < bestP)) { best = ob; bestP = op; }
-    }
-    if ((tid & 63) == 0) { sh.scal[tid >> 6][0] = best; sh.order[tid >> 6] = bestP; }
-    __syncthreads();
-    if (tid == 0) {
-      float minCost = sh.scal[0][0];
-      int pm = sh.order[0];
-      for (int w = 1; w < RWAVES; w++)
-        if (sh.scal[w][0] < minCost || (sh.scal[w][0] == minCost && sh.order[w] < pm)) { minCost = sh.scal[w][0]; pm = sh.order[w]; }
-      sh.found = pm != 0x7fffffff;
-      if (sh.found) {
-        const int ci = pm >> 5, cj = pm & 31;
-        sh.ci = ci;
-        sh.cj = cj;
-        const MergedLobe m = merged_lobe(V, ci, cj);
-        sh.mlobe[0] = m.w; sh.mlobe[1] = m.dx; sh.mlobe[2] = m.dy; sh.mlobe[3] = m.dz;
-        sh.mlobe[4] = m.kappa; sh.mlobe[5] = m.nrm; sh.mlobe[6] = m.em2k; sh.mlobe[7] = m.mc;
-        sh.pre[0] = V.w[ci]; sh.pre[1] = V.mux[ci]; sh.pre[2] = V.muy[ci]; sh.pre[3] = V.muz[ci];
-        sh.pre[4] = V.w[cj]; sh.pre[5] = V.mux[cj]; sh.pre[6] = V.muy[cj]; sh.pre[7] = V.muz[cj];
-      }
-    }
-    __syncthreads();
-    if (!sh.found) break;
-    const int ci = sh.ci, cj = sh.cj;
-    // the three merges read the pre-merge slots and write disjoint state
-    if ((tid & 63) == 0) {
-      const int wave = tid >> 6;
+    for (;;) {
       if (wave == 0) {
-        const MergedLobe m{sh.mlobe[0], sh.mlobe[1], sh.mlobe[2], sh.mlobe[3], sh.mlobe[4], sh.mlobe[5], sh.mlobe[6],
-                           sh.mlobe[7]};
-        vmm_merge(V, ci, cj, m);
-      } else if (wave == 1) {
-        split_stats_merge(SS, ci, cj, sh.pre[0], sh.pre[1], sh.pre[2], sh.pre[3], sh.pre[4], sh.pre[5], sh.pre[6],
-                          sh.pre[7], sh.mlobe[0], sh.mlobe[1], sh.mlobe[2], sh.mlobe[3]);
-      } else if (wave == 2) {
-        suff_merge(ST, ci, cj);
+        const int K = V.K;
+        const uint32_t hasSplit = (uint32_t)__ballot(lane < 32 && SS.sn[lane & 31] > 0.f);
+        const float thr = cfg.mergingThreshold;
+        float best = 3.402823466e38f;
+        int bestP = 0x7fffffff;
+        // lane l scans pairs l, l + 64, ... in increasing order: rows 2r and 2r + 1
+        for (int r = 0; r < 16 && 2 * r < K; r++) {
+          const int p = lane + 64 * r;
+          const int i = p >> 5, j = p & 31;
+          if (i < j && j < K && ((hasSplit >> i) & (hasSplit >> j) & 1u)) {
+            const float cst = sh.cost[p];
+            if (cst < thr && cst < best) { best = cst; bestP = p; }
+          }
+        }
+        const int pm = wave_argmin(best, bestP);
+        int more = 0;
+        if (pm != 0x7fffffff) {
+          const int ci = pm >> 5, cj = pm & 31;
+          merge_step_wave0(sh, ST, SS, ci, cj);
+          total++;
+          more = V.K > sh.W;
+          if (lane == 0) { sh.ci = ci; sh.cj = cj; }
+        }
+        if (lane == 0) sh.found = more;
       }
+      __syncthreads();
+      if (!sh.found) break;
+      // pair q of the changed rows: lanes 0..31 (ci, q), lanes 32..63 (cj, q - 32)
+      // unless that pair was already listed; wave t computes term t
+      const int ci = sh.ci, cj = sh.cj, Kn = V.K;
+      const int a0 = lane < 32 ? ci : cj, b0 = lane & 31;
+      int i = -1, j = -1;
+      if (a0 < Kn && b0 < Kn && b0 != a0 && !(lane >= 32 && b0 == ci)) { i = min(a0, b0); j = max(a0, b0); }
+      if (wave < 3 && i >= 0) sh.term[lane][wave] = merge_term(V, sh.selfp, i, j, wave, merged_lobe(V, i, j));
+      __syncthreads();
+      if (wave == 0 && i >= 0) sh.cost[i * 32 + j] = merge_chi(sh.term[lane][0], sh.term[lane][1], sh.term[lane][2], V.w[i] + V.w[j]);
+      // the next arg-min runs on wave 0, after these writes of its own
     }
+    if (tid == 0) sh.ci = total;
     __syncthreads();
-    total++;
-    if (V.K <= sh.W) break;
-    // changed rows: ci (merged) and cj (the former last component)
-    const int Kn = V.K;
-    if (tid < 2 && (tid == 0 || cj < Kn)) self_product(V, tid == 0 ? ci : cj, sh.selfp[tid == 0 ? ci : cj]);
-    __syncthreads();
-    // pair q of the changed rows: q < Kn -> (ci, q), else (cj, q - Kn) unless
-    // that pair was already listed; three lanes per pair, one term each
-    const int q = tid / 3, t = tid - 3 * (tid / 3);
-    int i = -1, j = -1;
-    if (q < 2 * Kn) {
-      const int a0 = q < Kn ? ci : cj, b0 = q < Kn ? q : q - Kn;
-      if (a0 < Kn && b0 != a0 && !(q >= Kn && b0 == ci)) { i = min(a0, b0); j = max(a0, b0); }
-    }
-    if (i >= 0) sh.term[q][t] = merge_term(V, sh.selfp, i, j, t, merged_lobe(V, i, j));
-    __syncthreads();
-    if (i >= 0 && t == 0) sh.cost[i * 32 + j] = merge_chi(sh.term[q][0], sh.term[q][1], sh.term[q][2], V.w[i] + V.w[j]);
-    __syncthreads();
+    total = sh.ci;
   }
   if (tid == 0) sh.S.prof[5] += (float)(wall_clock64() - tMerge);
   return total;
