@@ -65,6 +65,16 @@ struct RegionSamples {
   const float* __restrict__ dist;
 };
 
+// Team context of a workgroup (kept in LDS; thread 0 advances the epoch).
+struct Team {
+  uint32_t rank, T, nc;
+  float* part;          // 2 x nc x kPartStride floats of this region
+  uint32_t* counter;    // barrier counter of this region
+  uint32_t epoch;       // barriers passed
+  int* err;
+  unsigned long long timeout;  // longest barrier wait, wall-clock ticks
+};
+
 struct Shared {
   RegionState S;
   SuffStats cur;
@@ -93,17 +103,12 @@ struct Shared {
   double finLL;            // folded log-likelihood
   int32_t abort;           // a team barrier timed out: finish without waiting
   uint32_t item;           // queue item of this workgroup
+  FactoryConfig cfg;       // launch constants, read from LDS by the region logic
+  RegionSamples rs;
+  Team tm;                 // this workgroup's place in its region's team
+  uint32_t jobRegion, jobIndex;
 };
 
-// Team context of a workgroup (uniform over the workgroup).
-struct Team {
-  uint32_t rank, T, nc;
-  float* part;          // 2 x nc x kPartStride floats of this region
-  uint32_t* counter;    // barrier counter of this region
-  uint32_t epoch;       // barriers passed
-  int* err;
-  unsigned long long timeout;  // longest barrier wait, wall-clock ticks
-};
 
 // ----------------------------------------------------------------- helpers
 __device__ __forceinline__ float dot3f(float ax, float ay, float az, float bx, float by, float bz) {
@@ -309,8 +314,8 @@ __device__ __forceinline__ float chunk_value(const Shared& sh, int v, int F, int
 __device__ void team_barrier(Shared& sh, Team& tm) {
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
-  tm.epoch++;
   if (threadIdx.x == 0) {
+    tm.epoch++;  // read by the other threads after the closing barrier
     const unsigned long long tb = wall_clock64();
     atomicAdd(tm.counter, 1u);
     if (!sh.abort) {
@@ -1311,13 +1316,21 @@ __device__ int perform_merging(Shared& sh, SuffStats& ST, SplitStatsState& SS, c
         const float thr = cfg.mergingThreshold;
         float best = 3.402823466e38f;
         int bestP = 0x7fffffff;
-        // lane l scans pairs l, l + 64, ... in increasing order: rows 2r and 2r + 1
-        for (int r = 0; r < 16 && 2 * r < K; r++) {
-          const int p = lane + 64 * r;
-          const int i = p >> 5, j = p & 31;
-          if (i < j && j < K && ((hasSplit >> i) & (hasSplit >> j) & 1u)) {
-            const float cst = sh.cost[p];
-            if (cst < thr && cst < best) { best = cst; bestP = p; }
+        // lane l scans pairs l, l + 64, ... in increasing order (rows 2r and
+        // 2r + 1), reading four at a time
+        for (int r0 = 0; r0 < 16 && 2 * r0 < K; r0 += 4) {
+          float cs[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) cs[u] = sh.cost[lane + 64 * (r0 + u)];
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const int p = lane + 64 * (r0 + u);
+            const int i = p >> 5, j = p & 31;
+            const float cst = cs[u];
+            if (i < j && j < K && ((hasSplit >> i) & (hasSplit >> j) & 1u) && cst < thr && cst < best) {
+              best = cst;
+              bestP = p;
+            }
           }
         }
         const int pm = wave_argmin(best, bestP);
@@ -1610,24 +1623,35 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(2, 2)))
                                                       FactoryConfig cfg, const float* __restrict__ uniformDirs) {
   __shared__ Shared sh;
   const int tid = threadIdx.x;
-  if (tid == 0) sh.abort = 0;
+  if (tid == 0) {
+    sh.abort = 0;
+    sh.cfg = cfg;
+    sh.rs = rs;
+  }
   for (;;) {
-    if (tid == 0) sh.item = atomicAdd(queue, 1u);
+    if (tid == 0) {
+      const uint32_t it = atomicAdd(queue, 1u);
+      sh.item = it;
+      if (it < numItems) {
+        const uint2 item = items[it];
+        const RegionJob job = jobs[item.x];
+        Team& tm = sh.tm;
+        tm.rank = item.y;
+        tm.T = job.team;
+        tm.nc = job.chunks;
+        tm.part = partial + job.partialOffset;
+        tm.counter = counters + item.x;
+        tm.epoch = 0;
+        tm.err = err;
+        tm.timeout = timeout;
+        sh.jobRegion = job.region;
+        sh.jobIndex = item.x;
+      }
+    }
     __syncthreads();
-    const uint32_t it = sh.item;
-    if (it >= numItems) break;
-    const uint2 item = items[it];
-    const RegionJob job = jobs[item.x];
-    Team tm;
-    tm.rank = item.y;
-    tm.T = job.team;
-    tm.nc = job.chunks;
-    tm.part = partial + job.partialOffset;
-    tm.counter = counters + item.x;
-    tm.epoch = 0;
-    tm.err = err;
-    tm.timeout = timeout;
-    region_job(sh, tm, states, job.region, rs, cfg, uniformDirs);
+    if (sh.item >= numItems) break;
+    Team& tm = sh.tm;
+    region_job(sh, tm, states, sh.jobRegion, sh.rs, sh.cfg, uniformDirs);
     if (tid == 0 && tm.rank == 0) {
       const RegionState& S = sh.S;
       atomicAdd(&stats[0], (unsigned long long)S.estepEvals);
@@ -1636,7 +1660,7 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(2, 2)))
       atomicAdd(&stats[3], (unsigned long long)(S.emIterations + S.partialEmIterations));
       if (costHint) {
         const float n = (float)S.sampleCount;
-        costHint[job.region] = n > 0.f ? (S.estepEvals + S.auxEvals + S.splitEvals) / n : 0.f;
+        costHint[sh.jobRegion] = n > 0.f ? (S.estepEvals + S.auxEvals + S.splitEvals) / n : 0.f;
       }
     }
     __syncthreads();
