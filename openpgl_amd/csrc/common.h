@@ -154,7 +154,8 @@ struct RegionInput {
   float ssMean[3], ssVar[3], ssN;
   uint32_t begin, count;
   int32_t mode, splitFlag;
-  int32_t initK;        // 0: the factory's initK
+  int32_t initK;          // 0: the factory's initK
+  int32_t copyFrom = -1;  // >= 0: a new region whose state starts as a copy of this ancestor's
 };
 
 // One region of a directional launch: team = workgroups working on it,

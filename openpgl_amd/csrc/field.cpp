@@ -337,24 +337,24 @@ void KindField::kdUpdate(uint32_t n, hipStream_t s) {
   std::vector<float> params(6 * L);
   for (uint32_t i = 0; i < L; i++)
     for (int j = 0; j < 3; j++) { params[6 * i + j] = leaves[i].center[j]; params[6 * i + 3 + j] = leaves[i].invHalf[j]; }
-  dNodes_.ensure(nodes_.size() * sizeof(KDNode));
-  dFrontierSlot_.ensure(std::max<size_t>(leafSlot.size(), 1) * sizeof(int32_t));
-  dFrontier_.ensure(std::max<size_t>(params.size() * 4, sizeof(FrontierNode)));
   dAcc_.ensure(std::max<size_t>(L, 2) * sizeof(IntStatsAcc));
+  // one upload: nodes | leaf slots | leaf parameters (16-byte aligned parts)
+  const size_t nb = nodes_.size() * sizeof(KDNode), sb = leafSlot.size() * sizeof(int32_t), pb = params.size() * 4;
+  const size_t offSlot = (nb + 15) & ~size_t(15), offPar = (offSlot + sb + 15) & ~size_t(15);
+  const size_t upB = offPar + std::max<size_t>(pb, sizeof(FrontierNode));
+  kdUp_.ensure(upB);
+  dKdUp_.ensure(upB);
   {
-    const size_t nb = nodes_.size() * sizeof(KDNode), sb = leafSlot.size() * sizeof(int32_t), pb = params.size() * 4;
-    kdUp_.ensure(nb + sb + pb + 64);
     char* up = kdUp_.as<char>();
     memcpy(up, nodes_.data(), nb);
-    memcpy(up + nb, leafSlot.data(), sb);
-    memcpy(up + nb + sb, params.data(), pb);
-    PGL_HIP_CHECK(hipMemcpyAsync(dNodes_.p, up, nb, hipMemcpyHostToDevice, s));
-    PGL_HIP_CHECK(hipMemcpyAsync(dFrontierSlot_.p, up + nb, sb, hipMemcpyHostToDevice, s));
-    PGL_HIP_CHECK(hipMemcpyAsync(dFrontier_.p, up + nb + sb, pb, hipMemcpyHostToDevice, s));
+    memcpy(up + offSlot, leafSlot.data(), sb);
+    memcpy(up + offPar, params.data(), pb);
+    PGL_HIP_CHECK(hipMemcpyAsync(dKdUp_.p, up, offPar + pb, hipMemcpyHostToDevice, s));
   }
+  char* const kd = dKdUp_.as<char>();
   dStatPartial_.ensure(spatial_partial_bytes(n));
-  launch_descend_stats(dPx_.as<float>(), dPy_.as<float>(), dPz_.as<float>(), n, dNodes_.as<KDNode>(),
-                       dFrontierSlot_.as<int32_t>(), dFrontier_.p, L, dNodeOf_.as<uint32_t>(),
+  launch_descend_stats(dPx_.as<float>(), dPy_.as<float>(), dPz_.as<float>(), n, (const KDNode*)kd,
+                       (const int32_t*)(kd + offSlot), kd + offPar, L, dNodeOf_.as<uint32_t>(),
                        dStatPartial_.as<IntStatsAcc>(), dAcc_.as<IntStatsAcc>(), s);
   {
     kdDown_.ensure(std::max<size_t>(L, 1) * sizeof(IntStatsAcc));
@@ -387,20 +387,21 @@ void KindField::kdUpdate(uint32_t n, hipStream_t s) {
       bounds_params(it.lo, it.hi, fn[f].center, &halves[3 * f], fn[f].invHalf);
       slot[it.node] = (int32_t)f;
     }
-    dFrontier_.ensure(F * sizeof(FrontierNode));
-    dFrontierSlot_.ensure(slot.size() * sizeof(int32_t));
     dAcc_.ensure(2 * F * sizeof(IntStatsAcc));
+    // one upload: frontier nodes | slots
+    const size_t fb = F * sizeof(FrontierNode), rsb = slot.size() * sizeof(int32_t);
+    const size_t offS = (fb + 15) & ~size_t(15);
+    kdUp_.ensure(offS + rsb);
+    dKdUp_.ensure(offS + rsb);
     {
-      const size_t fb = F * sizeof(FrontierNode), sb = slot.size() * sizeof(int32_t);
-      kdUp_.ensure(fb + sb + 64);
       char* up = kdUp_.as<char>();
       memcpy(up, fn.data(), fb);
-      memcpy(up + fb, slot.data(), sb);
-      PGL_HIP_CHECK(hipMemcpyAsync(dFrontier_.p, up, fb, hipMemcpyHostToDevice, s));
-      PGL_HIP_CHECK(hipMemcpyAsync(dFrontierSlot_.p, up + fb, sb, hipMemcpyHostToDevice, s));
+      memcpy(up + offS, slot.data(), rsb);
+      PGL_HIP_CHECK(hipMemcpyAsync(dKdUp_.p, up, offS + rsb, hipMemcpyHostToDevice, s));
     }
+    char* const kr = dKdUp_.as<char>();
     launch_kd_round(dPx_.as<float>(), dPy_.as<float>(), dPz_.as<float>(), n, dNodeOf_.as<uint32_t>(),
-                    dFrontierSlot_.as<int32_t>(), dFrontier_.as<FrontierNode>(), F, dStatPartial_.as<IntStatsAcc>(),
+                    (const int32_t*)(kr + offS), (const FrontierNode*)kr, F, dStatPartial_.as<IntStatsAcc>(),
                     dAcc_.as<IntStatsAcc>(), s);
     kdDown_.ensure(2 * F * sizeof(IntStatsAcc));
     const IntStatsAcc* acc = kdDown_.as<IntStatsAcc>();
@@ -539,43 +540,12 @@ void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
   const uint32_t R = (uint32_t)regions_.size();
   PGL_HIP_CHECK(hipEventRecord(evSp_[P][1], sp));
 
-  // region-contiguous order: stable radix sort of (region, sample index)
-  dKeyNodes_.ensure(keyNodes_.size() * sizeof(KDNode));
-  PGL_HIP_CHECK(hipMemcpyAsync(dKeyNodes_.p, pinned(pin_[P][4], keyNodes_.data(), keyNodes_.size()),
-                               keyNodes_.size() * sizeof(KDNode), hipMemcpyHostToDevice, sp));
-  launch_leaf_keys(dNodeOf_.as<uint32_t>(), dKeyNodes_.as<KDNode>(), n, dKeys_.as<uint32_t>(), dVals_.as<uint32_t>(),
-                   sp);
-  // the query kernels walk the tree in the final numbering
-  dNodes_.ensure(nodes_.size() * sizeof(KDNode));
-  PGL_HIP_CHECK(hipMemcpyAsync(dNodes_.p, pinned(pin_[P][5], nodes_.data(), nodes_.size()),
-                               nodes_.size() * sizeof(KDNode), hipMemcpyHostToDevice, sp));
-  if (R <= kSortMaxKeys) {
-    // stable counting sort: every region's first position is known from the kd counts
-    hBegin_.resize(R);
-    uint32_t acc = 0;
-    for (uint32_t r = 0; r < R; r++) { hBegin_[r] = acc; acc += regions_[r].count; }
-    if (acc != n) throw std::runtime_error("kd update lost samples");
-    dRegionBegin_.ensure(size_t(R) * 4);
-    PGL_HIP_CHECK(hipMemcpyAsync(dRegionBegin_.p, pinned(pin_[P][6], hBegin_.data(), R), size_t(R) * 4,
-                                 hipMemcpyHostToDevice, sp));
-    dCountSortTmp_.ensure(counting_sort_temp_bytes(n, R));
-    counting_sort_pairs(dCountSortTmp_.p, dKeys_.as<uint32_t>(), dRegionBegin_.as<uint32_t>(), R, n,
-                        dKeys2_.as<uint32_t>(), dVals2_.as<uint32_t>(), sp);
-  } else {
-    int endBit = 1;
-    while ((1u << endBit) < R && endBit < 32) endBit++;
-    sort_pairs(dSortTmp_.p, sortBytes, dKeys_.as<uint32_t>(), dKeys2_.as<uint32_t>(), dVals_.as<uint32_t>(),
-               dVals2_.as<uint32_t>(), n, endBit, sp);
-  }
-  PGL_HIP_CHECK(hipEventRecord(evSp_[P][2], sp));
-
-  // per-region inputs (host arrays stay alive until the next update finishes this one)
+  // host tables of this update: every region's inputs and pivots, the sort's
+  // region starts and the KNN pivots, uploaded with the leaf-key nodes at once
   hInputs_.assign(R, RegionInput());
   hPivots_.assign(R, make_float4(0.f, 0.f, 0.f, 0.f));
-  hCopies_.clear();
-  for (uint32_t r = regionsBefore; r < R; r++)
-    if (regions_[r].copyFrom >= 0) { hCopies_.push_back(r); hCopies_.push_back((uint32_t)regions_[r].copyFrom); }
   hWork_.clear();
+  hBegin_.resize(R);
   uint32_t begin = 0;
   for (uint32_t r = 0; r < R; r++) {
     const RegionHost& rg = regions_[r];
@@ -588,6 +558,8 @@ void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
     in.mode = first ? 1 : 2;
     in.splitFlag = rg.splitFlag ? 1 : 0;
     in.initK = 0;
+    in.copyFrom = r >= regionsBefore ? rg.copyFrom : -1;  // new regions start from their ancestor's state
+    hBegin_[r] = begin;
     begin += rg.count;
     float var[3];
     rg.ss.variance(var);
@@ -597,15 +569,53 @@ void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
     if (first || rg.count > 0 || rg.splitFlag) hWork_.push_back(RegionWork{r, rg.count, (float)rg.count});
   }
   if (begin != n) throw std::runtime_error("kd update lost samples");
-  dPivots_.ensure(std::max<size_t>(R, 1) * sizeof(float4));
-  PGL_HIP_CHECK(hipMemcpyAsync(dPivots_.p, pinned(pin_[P][1], hPivots_.data(), R), R * sizeof(float4),
-                               hipMemcpyHostToDevice, sp));
+  if (kd_.knnLookup) {
+    hKnnPiv_.resize(R);
+    for (uint32_t r = 0; r < R; r++)
+      hKnnPiv_[r] = make_float4(regions_[r].ss.mean[0], regions_[r].ss.mean[1], regions_[r].ss.mean[2], 0.f);
+  }
+  const size_t keyB = keyNodes_.size() * sizeof(KDNode), pivB = size_t(R) * sizeof(float4);
+  const size_t offBegin = (keyB + 15) & ~size_t(15), offPiv = (offBegin + size_t(R) * 4 + 15) & ~size_t(15);
+  const size_t offKnn = offPiv + pivB, tabB = offKnn + (kd_.knnLookup ? pivB : 0);
+  pin_[P][0].ensure(tabB);
+  dTables_[P].ensure(tabB);
+  {
+    char* h = pin_[P][0].as<char>();
+    memcpy(h, keyNodes_.data(), keyB);
+    memcpy(h + offBegin, hBegin_.data(), size_t(R) * 4);
+    memcpy(h + offPiv, hPivots_.data(), pivB);
+    if (kd_.knnLookup) memcpy(h + offKnn, hKnnPiv_.data(), pivB);
+    PGL_HIP_CHECK(hipMemcpyAsync(dTables_[P].p, h, tabB, hipMemcpyHostToDevice, sp));
+  }
+  char* const tab = dTables_[P].as<char>();
+  knnPivots_ = (const float4*)(tab + offKnn);
+
+  // region-contiguous order: stable sort of (region, sample index)
+  launch_leaf_keys(dNodeOf_.as<uint32_t>(), (const KDNode*)tab, n, dKeys_.as<uint32_t>(), dVals_.as<uint32_t>(), sp);
+  // the query kernels walk the tree in the final numbering
+  dNodes_.ensure(nodes_.size() * sizeof(KDNode));
+  PGL_HIP_CHECK(hipMemcpyAsync(dNodes_.p, pinned(pin_[P][5], nodes_.data(), nodes_.size()),
+                               nodes_.size() * sizeof(KDNode), hipMemcpyHostToDevice, sp));
+  if (R <= kSortMaxKeys) {
+    // stable counting sort: every region's first position is known from the kd counts
+    dCountSortTmp_.ensure(counting_sort_temp_bytes(n, R));
+    counting_sort_pairs(dCountSortTmp_.p, dKeys_.as<uint32_t>(), (const uint32_t*)(tab + offBegin), R, n,
+                        dKeys2_.as<uint32_t>(), dVals2_.as<uint32_t>(), sp);
+  } else {
+    int endBit = 1;
+    while ((1u << endBit) < R && endBit < 32) endBit++;
+    sort_pairs(dSortTmp_.p, sortBytes, dKeys_.as<uint32_t>(), dKeys2_.as<uint32_t>(), dVals_.as<uint32_t>(),
+               dVals2_.as<uint32_t>(), n, endBit, sp);
+  }
+  PGL_HIP_CHECK(hipEventRecord(evSp_[P][2], sp));
+
   float* const* soa = nullptr;
   float* sv[6];
   for (int j = 0; j < 6; j++) sv[j] = dSoa_[P][j].as<float>();
   soa = sv;
-  launch_gather_reproject(deviceSamples, dKeys2_.as<uint32_t>(), dVals2_.as<uint32_t>(), n, dPivots_.as<float4>(),
-                          soa[0], soa[1], soa[2], soa[3], soa[4], soa[5], fc_.noParallax == 0, sp);
+  launch_gather_reproject(deviceSamples, dKeys2_.as<uint32_t>(), dVals2_.as<uint32_t>(), n,
+                          (const float4*)(tab + offPiv), soa[0], soa[1], soa[2], soa[3], soa[4], soa[5],
+                          fc_.noParallax == 0, sp);
   PGL_HIP_CHECK(hipEventRecord(evSp_[P][3], sp));
   lastGatherParity_ = P;
 
@@ -620,34 +630,18 @@ void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
   }
   ensureRegionCapacity(R, s);
   PGL_HIP_CHECK(hipEventRecord(evDir_[P][0], s));
-  dCopyPairs_.ensure(std::max<size_t>(hCopies_.size(), 2) * 4);
-  dInputs_.ensure(std::max<size_t>(R, 1) * sizeof(RegionInput));
-  if (!hCopies_.empty()) {
-    PGL_HIP_CHECK(hipMemcpyAsync(dCopyPairs_.p, pinned(pin_[P][2], hCopies_.data(), hCopies_.size()),
-                                 hCopies_.size() * 4, hipMemcpyHostToDevice, s));
-    launch_copy_states(dStates_.as<RegionState>(), dCopyPairs_.as<uint32_t>(), (uint32_t)(hCopies_.size() / 2), s);
-  }
-  PGL_HIP_CHECK(hipMemcpyAsync(dInputs_.p, pinned(pin_[P][0], hInputs_.data(), R), R * sizeof(RegionInput),
-                               hipMemcpyHostToDevice, s));
-  launch_set_inputs(dStates_.as<RegionState>(), dInputs_.as<RegionInput>(), R, s);
-  // KNN neighbourhoods of the region pivots (sample-statistics means); every
-  // upload is queued before the region kernel so that nothing after it waits
+  // KNN neighbourhoods of the region pivots (sample-statistics means, uploaded
+  // with the spatial tables)
   if (kd_.knnLookup) {
-    hKnnPiv_.resize(R);
-    for (uint32_t r = 0; r < R; r++)
-      hKnnPiv_[r] = make_float4(regions_[r].ss.mean[0], regions_[r].ss.mean[1], regions_[r].ss.mean[2], 0.f);
     dNbIds_.ensure(size_t(R) * 8 * 4);
     dNbPts_.ensure(size_t(R) * 8 * sizeof(float4));
-    dKnnPiv_.ensure(size_t(R) * sizeof(float4));
-    PGL_HIP_CHECK(hipMemcpyAsync(dKnnPiv_.p, pinned(pin_[P][3], hKnnPiv_.data(), R), R * sizeof(float4),
-                                 hipMemcpyHostToDevice, s));
   }
   PGL_HIP_CHECK(hipStreamWaitEvent(s, evSp_[P][3], 0));
-  dCostHint_.ensure(std::max<size_t>(R, 1) * 4, true);
+  // the new regions' state copies, the inputs and the launch (one upload, one preparing kernel)
   RegionRunner& rr = runner();
   rr.launch(dStates_.as<RegionState>(), hWork_,
             RegionSamplesView{soa[0], soa[1], soa[2], soa[3], soa[4], soa[5]}, fc_, dUniformDirs_.as<float>(),
-            dCostHint_.as<float>(), s);
+            nullptr, s, hInputs_.data(), R);
   inf.regionLaunches = rr.lastLaunches;
   inf.regionJobs = rr.lastJobs;
   inf.regionItems = rr.lastItems;
@@ -655,9 +649,7 @@ void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
   inf.regionGrid = rr.lastGrid;
   PGL_HIP_CHECK(hipEventRecord(evDir_[P][1], s));
 
-  if (kd_.knnLookup) launch_knn8(dKnnPiv_.as<float4>(), R, dNbIds_.as<uint32_t>(), dNbPts_.as<float4>(), s);
-  hCostHint_.ensure(std::max<size_t>(R, 1) * 4);
-  PGL_HIP_CHECK(hipMemcpyAsync(hCostHint_.p, dCostHint_.p, size_t(R) * 4, hipMemcpyDeviceToHost, s));
+  if (kd_.knnLookup) launch_knn8(knnPivots_, R, dNbIds_.as<uint32_t>(), dNbPts_.as<float4>(), s);
   PGL_HIP_CHECK(hipEventRecord(evDir_[P][2], s));
 
   for (auto& r : regions_) { r.splitFlag = false; r.copyFrom = -1; }

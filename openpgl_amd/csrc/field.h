@@ -175,7 +175,8 @@ class KindField {
   DevBuf dPx_, dPy_, dPz_, dNodeOf_, dKeys_, dVals_, dKeys2_, dVals2_, dSortTmp_;
   DevBuf dSoa_[2][6];     // region-ordered dx, dy, dz, w, pdf, dist of two consecutive updates
   DevBuf dAos_;           // staging of host records
-  DevBuf dKeyNodes_, dKnnPiv_;
+  DevBuf dTables_[2];          // per parity: leaf-key nodes | region starts | pivots | KNN pivots
+  const float4* knnPivots_ = nullptr;  // KNN pivots of the pending update (in dTables_)
   hipStream_t sp_ = nullptr;  // spatial phase
   int parity_ = 0;
   int lastGatherParity_ = -1;
@@ -184,7 +185,7 @@ class KindField {
   // [1] pivots, [2] copy pairs, [3] KNN pivots, [4] key nodes, [5] nodes,
   // [6] region begins
   PinnedBuf pin_[2][7];
-  DevBuf dRegionBegin_, dCountSortTmp_;
+  DevBuf dCountSortTmp_;
   std::vector<uint32_t> hBegin_;
   PinnedBuf kdUp_, kdDown_;  // kd rounds (synchronous)
   template <class T> const T* pinned(PinnedBuf& b, const T* src, size_t count) {
@@ -194,7 +195,6 @@ class KindField {
   }
   std::vector<RegionInput> hInputs_;
   std::vector<float4> hPivots_, hKnnPiv_;
-  std::vector<uint32_t> hCopies_;
   std::vector<RegionWork> hWork_;
   uint32_t lastKdRounds_ = 0;
   mutable bool pending_ = false;
@@ -203,12 +203,10 @@ class KindField {
   mutable int pendingParity_ = 0;
   std::chrono::steady_clock::time_point pendingT0_;
   mutable std::vector<UpdateInfo> history_;
-  DevBuf dNodes_, dFrontier_, dFrontierSlot_, dAcc_, dPartial_, dStatPartial_;
+  DevBuf dNodes_, dKdUp_, dAcc_, dPartial_, dStatPartial_;  // dKdUp_: the kd rounds' uploads
   std::unique_ptr<RegionRunner> runner_;
-  DevBuf dCostHint_;       // per region: evaluations per sample of its last update (written by the region kernel)
-  PinnedBuf hCostHint_;    // host copy
   mutable uint32_t hintRegions_ = 0;
-  DevBuf dStates_, dPivots_, dNbIds_, dNbPts_, dUniformDirs_, dCopyPairs_, dInputs_;
+  DevBuf dStates_, dNbIds_, dNbPts_, dUniformDirs_;
   uint32_t stateCapacity_ = 0;
 
   mutable std::vector<RegionState> hostStates_;

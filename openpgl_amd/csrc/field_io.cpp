@@ -412,9 +412,9 @@ void KindField::readFile(std::istream& is, int W, bool parallaxAware, hipStream_
     }
     dNbIds_.ensure(size_t(R) * 8 * 4);
     dNbPts_.ensure(size_t(R) * 8 * sizeof(float4));
-    dKnnPiv_.ensure(size_t(R) * sizeof(float4));
-    PGL_HIP_CHECK(hipMemcpyAsync(dKnnPiv_.p, knnPts.data(), size_t(R) * sizeof(float4), hipMemcpyHostToDevice, s));
-    launch_knn8(dKnnPiv_.as<float4>(), R, dNbIds_.as<uint32_t>(), dNbPts_.as<float4>(), s);
+    dTables_[0].ensure(size_t(R) * sizeof(float4));  // no update is pending: the tables are free
+    PGL_HIP_CHECK(hipMemcpyAsync(dTables_[0].p, knnPts.data(), size_t(R) * sizeof(float4), hipMemcpyHostToDevice, s));
+    launch_knn8(dTables_[0].as<float4>(), R, dNbIds_.as<uint32_t>(), dNbPts_.as<float4>(), s);
   }
   PGL_HIP_CHECK(hipStreamSynchronize(s));
   {

@@ -90,12 +90,20 @@ struct RegionLaunch {
   const float *dx, *dy, *dz, *w, *pdf, *dist;
   const float* uniformDirs;
   bool cooperative;
+  bool zeroQueue;  // zero the queue and team counters before the launch (else done by launch_prepare_regions)
 };
 int region_kernel_blocks_per_cu();
 void launch_region_update(const RegionLaunch& launch, const FactoryConfig& cfg, hipStream_t s);
 
 void launch_copy_states(RegionState* states, const uint32_t* pairs /* dst, src */, uint32_t numPairs, hipStream_t s);
 void launch_set_inputs(RegionState* states, const RegionInput* inputs, uint32_t count, hipStream_t s);
+// One kernel before a persistent region launch: zeroes numZero words at zero
+// (queue and team counters), the error flag and the four launch totals; with
+// inputs, every region's state is first copied from its ancestor where
+// copyFrom >= 0 and then given its inputs (launch_copy_states followed by
+// launch_set_inputs).
+void launch_prepare_regions(RegionState* states, const RegionInput* inputs, uint32_t numInputs, uint32_t* zero,
+                            uint32_t numZero, int* err, unsigned long long* stats, hipStream_t s);
 
 // query
 // One piece of a query batch: queries [begin, begin + n) of the batch read

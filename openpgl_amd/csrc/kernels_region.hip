@@ -1695,6 +1695,49 @@ __global__ void __launch_bounds__(256) k_set_inputs(RegionState* __restrict__ st
   S.estepEvals = 0.f; S.auxEvals = 0.f; S.splitEvals = 0.f;
 }
 
+// Block 0 also zeroes the launch's counters; block b < numInputs prepares the
+// region of inputs[b].  A new region's ancestor may receive its own inputs
+// while the copy runs: only input fields can differ, and the new region
+// overwrites all of them.
+__global__ void __launch_bounds__(256) k_prepare_regions(RegionState* __restrict__ states,
+                                                         const RegionInput* __restrict__ in, uint32_t numInputs,
+                                                         uint32_t* __restrict__ zero, uint32_t numZero,
+                                                         int* __restrict__ err, unsigned long long* __restrict__ stats) {
+  const uint32_t b = blockIdx.x;
+  if (b == 0) {
+    for (uint32_t i = threadIdx.x; i < numZero; i += blockDim.x) zero[i] = 0u;
+    if (threadIdx.x == 0) *err = 0;
+    if (threadIdx.x < 4) stats[threadIdx.x] = 0ull;
+  }
+  if (in == nullptr || b >= numInputs) return;
+  const RegionInput v = in[b];
+  if (v.copyFrom >= 0) {  // uniform over the block
+    const uint4* src = (const uint4*)(states + v.copyFrom);
+    uint4* dst = (uint4*)(states + v.region);
+    for (uint32_t i = threadIdx.x; i < sizeof(RegionState) / 16; i += blockDim.x) dst[i] = src[i];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    RegionState& S = states[v.region];
+    for (int j = 0; j < 3; j++) { S.ssMean[j] = v.ssMean[j]; S.ssVar[j] = v.ssVar[j]; }
+    S.ssN = v.ssN;
+    S.sampleBegin = v.begin;
+    S.sampleCount = v.count;
+    S.mode = v.mode;
+    S.splitFlag = v.splitFlag;
+    S.initK = v.initK;
+    S.emIterations = 0; S.partialEmIterations = 0; S.numSplits = 0; S.numMerges = 0;
+    S.estepEvals = 0.f; S.auxEvals = 0.f; S.splitEvals = 0.f;
+  }
+}
+
+void launch_prepare_regions(RegionState* states, const RegionInput* inputs, uint32_t numInputs, uint32_t* zero,
+                            uint32_t numZero, int* err, unsigned long long* stats, hipStream_t s) {
+  const uint32_t blocks = inputs && numInputs ? numInputs : 1u;
+  hipLaunchKernelGGL(k_prepare_regions, dim3(blocks), dim3(256), 0, s, states, inputs, numInputs, zero, numZero, err,
+                     stats);
+}
+
 void launch_copy_states(RegionState* states, const uint32_t* pairs, uint32_t numPairs, hipStream_t s) {
   if (numPairs == 0) return;
   hipLaunchKernelGGL(k_copy_states, dim3(numPairs), dim3(256), 0, s, states, pairs, numPairs);
@@ -1719,7 +1762,7 @@ int region_kernel_blocks_per_cu() {
 void launch_region_update(const RegionLaunch& L, const FactoryConfig& cfg, hipStream_t s) {
   if (L.numItems == 0) return;
   const uint32_t nz = L.numJobs + 1;
-  hipLaunchKernelGGL(k_zero_u32, dim3((nz + 255) / 256), dim3(256), 0, s, L.queueAndCounters, nz);
+  if (L.zeroQueue) hipLaunchKernelGGL(k_zero_u32, dim3((nz + 255) / 256), dim3(256), 0, s, L.queueAndCounters, nz);
   RegionSamples rs{L.dx, L.dy, L.dz, L.w, L.pdf, L.dist};
   RegionState* states = L.states;
   const RegionJob* jobs = L.jobs;

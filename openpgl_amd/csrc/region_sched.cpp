@@ -58,10 +58,8 @@ RegionRunner::RegionRunner() {
 }
 
 RegionRunner::~RegionRunner() {
-  (void)hipFree(dJobs_);
-  (void)hipFree(dItems_);
-  (void)hipHostFree(hJobs_);
-  (void)hipHostFree(hItems_);
+  (void)hipFree(dStage_);
+  (void)hipHostFree(hStage_);
   (void)hipFree(dQueue_);
   (void)hipFree(dPartial_);
   (void)hipFree(dErr_);
@@ -73,12 +71,24 @@ RegionRunner::~RegionRunner() {
 }
 
 void RegionRunner::launch(RegionState* states, std::vector<RegionWork> work, const RegionSamplesView& rs,
-                          const FactoryConfig& cfg, const float* uniformDirs, float* costHint, hipStream_t s) {
+                          const FactoryConfig& cfg, const float* uniformDirs, float* costHint, hipStream_t s,
+                          const RegionInput* inputs, uint32_t numInputs) {
   lastLaunches = lastJobs = lastItems = lastMaxTeam = lastGrid = 0;
   *hErr_ = 0;
   *hStats_ = RegionLaunchStats{0, 0, 0, 0};
   timed_ = false;
-  if (work.empty()) return;
+  if (work.empty()) {
+    if (inputs && numInputs) {  // the states still take their inputs
+      grow(dStage_, stageBytes_, size_t(numInputs) * sizeof(RegionInput));
+      grow_pinned(hStage_, hStageBytes_, size_t(numInputs) * sizeof(RegionInput));
+      memcpy(hStage_, inputs, size_t(numInputs) * sizeof(RegionInput));
+      PGL_SCHED_CHECK(hipMemcpyAsync(dStage_, hStage_, size_t(numInputs) * sizeof(RegionInput),
+                                     hipMemcpyHostToDevice, s));
+      launch_prepare_regions(states, (const RegionInput*)dStage_, numInputs, nullptr, 0, dErr_,
+                             (unsigned long long*)dStats_, s);
+    }
+    return;
+  }
   std::stable_sort(work.begin(), work.end(), [](const RegionWork& a, const RegionWork& b) { return a.cost > b.cost; });
   const uint32_t J = (uint32_t)work.size();
   const bool perRegion = schedule.perRegionLaunches != 0;
@@ -117,24 +127,29 @@ void RegionRunner::launch(RegionState* states, std::vector<RegionWork> work, con
   }
   if (perRegion) items_.assign(1, make_uint2(0, 0));
 
-  grow(dJobs_, jobsBytes_, J * sizeof(RegionJob));
-  grow(dItems_, itemsBytes_, items_.size() * sizeof(uint2));
+  // one upload: jobs | items | inputs (16-byte aligned parts)
+  const size_t jobsB = J * sizeof(RegionJob), itemsB = items_.size() * sizeof(uint2);
+  const size_t inB = inputs ? size_t(numInputs) * sizeof(RegionInput) : 0;
+  const size_t offItems = (jobsB + 15) & ~size_t(15), offIn = (offItems + itemsB + 15) & ~size_t(15);
+  const size_t stageB = offIn + inB;
+  grow(dStage_, stageBytes_, stageB);
   grow(dQueue_, queueBytes_, (size_t)(J + 1) * 4);
   grow(dPartial_, partialBytes_, std::max<unsigned long long>(partialFloats, 1) * 4);
-  grow_pinned(hJobs_, hJobsBytes_, J * sizeof(RegionJob));
-  grow_pinned(hItems_, hItemsBytes_, items_.size() * sizeof(uint2));
-  memcpy(hJobs_, jobs_.data(), J * sizeof(RegionJob));
-  memcpy(hItems_, items_.data(), items_.size() * sizeof(uint2));
-  PGL_SCHED_CHECK(hipMemcpyAsync(dJobs_, hJobs_, J * sizeof(RegionJob), hipMemcpyHostToDevice, s));
-  PGL_SCHED_CHECK(hipMemcpyAsync(dItems_, hItems_, items_.size() * sizeof(uint2), hipMemcpyHostToDevice, s));
-  PGL_SCHED_CHECK(hipMemsetAsync(dErr_, 0, sizeof(int), s));
-  PGL_SCHED_CHECK(hipMemsetAsync(dStats_, 0, sizeof(RegionLaunchStats), s));
+  grow_pinned(hStage_, hStageBytes_, stageB);
+  char* hs = (char*)hStage_;
+  memcpy(hs, jobs_.data(), jobsB);
+  memcpy(hs + offItems, items_.data(), itemsB);
+  if (inB) memcpy(hs + offIn, inputs, inB);
+  PGL_SCHED_CHECK(hipMemcpyAsync(dStage_, hStage_, stageB, hipMemcpyHostToDevice, s));
+  char* ds = (char*)dStage_;
+  launch_prepare_regions(states, inB ? (const RegionInput*)(ds + offIn) : nullptr, inB ? numInputs : 0,
+                         (uint32_t*)dQueue_, J + 1, dErr_, (unsigned long long*)dStats_, s);
 
   RegionLaunch L;
   L.states = states;
-  L.jobs = (const RegionJob*)dJobs_;
+  L.jobs = (const RegionJob*)ds;
   L.numJobs = J;
-  L.items = (const uint2*)dItems_;
+  L.items = (const uint2*)(ds + offItems);
   L.numItems = (uint32_t)items_.size();
   L.grid = std::min<uint32_t>(G, L.numItems);
   L.queueAndCounters = (uint32_t*)dQueue_;
@@ -146,6 +161,7 @@ void RegionRunner::launch(RegionState* states, std::vector<RegionWork> work, con
   L.dx = rs.dx; L.dy = rs.dy; L.dz = rs.dz; L.w = rs.w; L.pdf = rs.pdf; L.dist = rs.dist;
   L.uniformDirs = uniformDirs;
   L.cooperative = cooperative_ && schedule.cooperative != 0;
+  L.zeroQueue = perRegion;
   PGL_SCHED_CHECK(hipEventRecord(evBegin_, s));
   if (perRegion) {
     L.cooperative = false;
@@ -153,7 +169,7 @@ void RegionRunner::launch(RegionState* states, std::vector<RegionWork> work, con
     L.numItems = 1;
     L.grid = 1;
     for (uint32_t j = 0; j < J; j++) {
-      L.jobs = (const RegionJob*)dJobs_ + j;
+      L.jobs = (const RegionJob*)ds + j;
       launch_region_update(L, cfg, s);
     }
     lastLaunches = J;

@@ -49,9 +49,12 @@ class RegionRunner {
 
   // Jobs are ordered by cost, largest first; team sizes follow the cost.
   // costHint (optional, device, per region) receives the evaluations per
-  // sample of this launch.  The previous launch must have completed.
+  // sample of this launch.  inputs (optional, host, numInputs entries) are
+  // applied to the states before the launch (launch_prepare_regions), also
+  // when there is no work.  The previous launch must have completed.
   void launch(RegionState* states, std::vector<RegionWork> work, const RegionSamplesView& rs,
-              const FactoryConfig& cfg, const float* uniformDirs, float* costHint, hipStream_t s);
+              const FactoryConfig& cfg, const float* uniformDirs, float* costHint, hipStream_t s,
+              const RegionInput* inputs = nullptr, uint32_t numInputs = 0);
   // after the stream has completed: throws if a team barrier timed out
   void check() const;
   const RegionLaunchStats& stats() const { return *hStats_; }
@@ -68,10 +71,8 @@ class RegionRunner {
   uint32_t resident_ = 0;
   bool cooperative_ = false;
   unsigned long long timeoutTicks_ = 0;
-  void* dJobs_ = nullptr;
-  size_t jobsBytes_ = 0;
-  void* dItems_ = nullptr;
-  size_t itemsBytes_ = 0;
+  void* dStage_ = nullptr;  // jobs | items | inputs of the current launch
+  size_t stageBytes_ = 0;
   void* dQueue_ = nullptr;
   size_t queueBytes_ = 0;
   void* dPartial_ = nullptr;
@@ -84,11 +85,9 @@ class RegionRunner {
   bool timed_ = false;
   std::vector<RegionJob> jobs_;
   std::vector<uint2> items_;
-  // pinned staging of the job and item tables (reused once the previous launch has completed)
-  void* hJobs_ = nullptr;
-  size_t hJobsBytes_ = 0;
-  void* hItems_ = nullptr;
-  size_t hItemsBytes_ = 0;
+  // pinned staging of the upload (reused once the previous launch has completed)
+  void* hStage_ = nullptr;
+  size_t hStageBytes_ = 0;
 };
 
 }  // namespace pglgpu
