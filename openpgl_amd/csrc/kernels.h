@@ -97,13 +97,15 @@ void launch_region_update(const RegionLaunch& launch, const FactoryConfig& cfg, 
 
 void launch_copy_states(RegionState* states, const uint32_t* pairs /* dst, src */, uint32_t numPairs, hipStream_t s);
 void launch_set_inputs(RegionState* states, const RegionInput* inputs, uint32_t count, hipStream_t s);
-// One kernel before a persistent region launch: zeroes numZero words at zero
-// (queue and team counters), the error flag and the four launch totals; with
-// inputs, every region's state is first copied from its ancestor where
-// copyFrom >= 0 and then given its inputs (launch_copy_states followed by
-// launch_set_inputs).
-void launch_prepare_regions(RegionState* states, const RegionInput* inputs, uint32_t numInputs, uint32_t* zero,
-                            uint32_t numZero, int* err, unsigned long long* stats, hipStream_t s);
+// One kernel before a persistent region launch, reading its tables from
+// pinned host memory: copies tableBytes (a multiple of 16) from hostTables to
+// tables, zeroes numZero words at zero (queue and team counters), the error
+// flag and the four launch totals; with inputs (host, pinned), every region's
+// state is first copied from its ancestor where copyFrom >= 0 and then given
+// its inputs (launch_copy_states followed by launch_set_inputs).
+void launch_prepare_regions(RegionState* states, const void* hostTables, void* tables, size_t tableBytes,
+                            const RegionInput* inputs, uint32_t numInputs, uint32_t* zero, uint32_t numZero, int* err,
+                            unsigned long long* stats, hipStream_t s);
 
 // query
 // One piece of a query batch: queries [begin, begin + n) of the batch read

@@ -42,6 +42,7 @@
 //   field/Field.h                           updateRegions 441-511 (decay, parallax shift)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "common.h"
@@ -1695,15 +1696,20 @@ __global__ void __launch_bounds__(256) k_set_inputs(RegionState* __restrict__ st
   S.estepEvals = 0.f; S.auxEvals = 0.f; S.splitEvals = 0.f;
 }
 
-// Block 0 also zeroes the launch's counters; block b < numInputs prepares the
-// region of inputs[b].  A new region's ancestor may receive its own inputs
-// while the copy runs: only input fields can differ, and the new region
-// overwrites all of them.
+// Reads the launch's tables straight from pinned host memory (no copy
+// operation before it): all blocks copy the job and item tables (copy16
+// 16-byte words) to device memory for the region kernel; block 0 zeroes the
+// launch's counters; block b < numInputs prepares the region of inputs[b].  A
+// new region's ancestor may receive its own inputs while the copy runs: only
+// input fields can differ, and the new region overwrites all of them.
 __global__ void __launch_bounds__(256) k_prepare_regions(RegionState* __restrict__ states,
-                                                         const RegionInput* __restrict__ in, uint32_t numInputs,
-                                                         uint32_t* __restrict__ zero, uint32_t numZero,
-                                                         int* __restrict__ err, unsigned long long* __restrict__ stats) {
+                                                         const uint4* __restrict__ hostTables, uint4* __restrict__ tables,
+                                                         uint32_t copy16, const RegionInput* __restrict__ in,
+                                                         uint32_t numInputs, uint32_t* __restrict__ zero,
+                                                         uint32_t numZero, int* __restrict__ err,
+                                                         unsigned long long* __restrict__ stats) {
   const uint32_t b = blockIdx.x;
+  for (uint32_t i = b * blockDim.x + threadIdx.x; i < copy16; i += gridDim.x * blockDim.x) tables[i] = hostTables[i];
   if (b == 0) {
     for (uint32_t i = threadIdx.x; i < numZero; i += blockDim.x) zero[i] = 0u;
     if (threadIdx.x == 0) *err = 0;
@@ -1731,11 +1737,14 @@ __global__ void __launch_bounds__(256) k_prepare_regions(RegionState* __restrict
   }
 }
 
-void launch_prepare_regions(RegionState* states, const RegionInput* inputs, uint32_t numInputs, uint32_t* zero,
-                            uint32_t numZero, int* err, unsigned long long* stats, hipStream_t s) {
-  const uint32_t blocks = inputs && numInputs ? numInputs : 1u;
-  hipLaunchKernelGGL(k_prepare_regions, dim3(blocks), dim3(256), 0, s, states, inputs, numInputs, zero, numZero, err,
-                     stats);
+void launch_prepare_regions(RegionState* states, const void* hostTables, void* tables, size_t tableBytes,
+                            const RegionInput* inputs, uint32_t numInputs, uint32_t* zero, uint32_t numZero, int* err,
+                            unsigned long long* stats, hipStream_t s) {
+  const uint32_t copy16 = (uint32_t)((tableBytes + 15) / 16);
+  uint32_t blocks = inputs && numInputs ? numInputs : 1u;
+  blocks = std::max<uint32_t>(blocks, std::min<uint32_t>((copy16 + 255) / 256, 1024u));
+  hipLaunchKernelGGL(k_prepare_regions, dim3(blocks), dim3(256), 0, s, states, (const uint4*)hostTables,
+                     (uint4*)tables, copy16, inputs, numInputs, zero, numZero, err, stats);
 }
 
 void launch_copy_states(RegionState* states, const uint32_t* pairs, uint32_t numPairs, hipStream_t s) {

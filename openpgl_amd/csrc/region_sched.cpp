@@ -79,12 +79,11 @@ void RegionRunner::launch(RegionState* states, std::vector<RegionWork> work, con
   timed_ = false;
   if (work.empty()) {
     if (inputs && numInputs) {  // the states still take their inputs
-      grow(dStage_, stageBytes_, size_t(numInputs) * sizeof(RegionInput));
       grow_pinned(hStage_, hStageBytes_, size_t(numInputs) * sizeof(RegionInput));
       memcpy(hStage_, inputs, size_t(numInputs) * sizeof(RegionInput));
-      PGL_SCHED_CHECK(hipMemcpyAsync(dStage_, hStage_, size_t(numInputs) * sizeof(RegionInput),
-                                     hipMemcpyHostToDevice, s));
-      launch_prepare_regions(states, (const RegionInput*)dStage_, numInputs, nullptr, 0, dErr_,
+      void* dev = nullptr;
+      PGL_SCHED_CHECK(hipHostGetDevicePointer(&dev, hStage_, 0));
+      launch_prepare_regions(states, nullptr, nullptr, 0, (const RegionInput*)dev, numInputs, nullptr, 0, dErr_,
                              (unsigned long long*)dStats_, s);
     }
     return;
@@ -127,12 +126,12 @@ void RegionRunner::launch(RegionState* states, std::vector<RegionWork> work, con
   }
   if (perRegion) items_.assign(1, make_uint2(0, 0));
 
-  // one upload: jobs | items | inputs (16-byte aligned parts)
+  // jobs | items | inputs (16-byte aligned parts) in pinned memory, read by the preparing kernel
   const size_t jobsB = J * sizeof(RegionJob), itemsB = items_.size() * sizeof(uint2);
   const size_t inB = inputs ? size_t(numInputs) * sizeof(RegionInput) : 0;
   const size_t offItems = (jobsB + 15) & ~size_t(15), offIn = (offItems + itemsB + 15) & ~size_t(15);
   const size_t stageB = offIn + inB;
-  grow(dStage_, stageBytes_, stageB);
+  grow(dStage_, stageBytes_, offIn);
   grow(dQueue_, queueBytes_, (size_t)(J + 1) * 4);
   grow(dPartial_, partialBytes_, std::max<unsigned long long>(partialFloats, 1) * 4);
   grow_pinned(hStage_, hStageBytes_, stageB);
@@ -140,10 +139,11 @@ void RegionRunner::launch(RegionState* states, std::vector<RegionWork> work, con
   memcpy(hs, jobs_.data(), jobsB);
   memcpy(hs + offItems, items_.data(), itemsB);
   if (inB) memcpy(hs + offIn, inputs, inB);
-  PGL_SCHED_CHECK(hipMemcpyAsync(dStage_, hStage_, stageB, hipMemcpyHostToDevice, s));
+  char* hdev = nullptr;
+  PGL_SCHED_CHECK(hipHostGetDevicePointer((void**)&hdev, hStage_, 0));
   char* ds = (char*)dStage_;
-  launch_prepare_regions(states, inB ? (const RegionInput*)(ds + offIn) : nullptr, inB ? numInputs : 0,
-                         (uint32_t*)dQueue_, J + 1, dErr_, (unsigned long long*)dStats_, s);
+  launch_prepare_regions(states, hdev, ds, offIn, inB ? (const RegionInput*)(hdev + offIn) : nullptr,
+                         inB ? numInputs : 0, (uint32_t*)dQueue_, J + 1, dErr_, (unsigned long long*)dStats_, s);
 
   RegionLaunch L;
   L.states = states;
