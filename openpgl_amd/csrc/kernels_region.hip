@@ -1037,16 +1037,16 @@ __device__ float product_scale(float m0x, float m0y, float m0z, float k0, float 
   pmx = k0 * m0x + k1 * m1x;
   pmy = k0 * m0y + k1 * m1y;
   pmz = k0 * m0z + k1 * m1z;
-  pk = sqrtf(dot3f(pmx, pmy, pmz, pmx, pmy, pmz));
-  pn = 1.f / (4.f * 3.14159265358979f);
-  if (pk > 1e-3f) {
-    const float e = expf(-2.f * pk);
-    pn = pk / (kTwoPi * (1.f - e));
-    pmx /= pk; pmy /= pk; pmz /= pk;
-  } else {
-    pk = 0.f;
-    pmx = m0x; pmy = m0y; pmz = m0z;
-  }
+  // both outcomes computed and selected: no branch, so that independent
+  // chains of the merge step can be interleaved
+  const float r = sqrtf(dot3f(pmx, pmy, pmz, pmx, pmy, pmz));
+  const bool big = r > 1e-3f;
+  const float e = expf(-2.f * r);
+  pn = big ? r / (kTwoPi * (1.f - e)) : 1.f / (4.f * 3.14159265358979f);
+  pmx = big ? pmx / r : m0x;
+  pmy = big ? pmy / r : m0y;
+  pmz = big ? pmz / r : m0z;
+  pk = big ? r : 0.f;
   float scale = (n0 * n1) / pn;
   const float c0 = dot3f(m0x, m0y, m0z, pmx, pmy, pmz), c1 = dot3f(m1x, m1y, m1z, pmx, pmy, pmz);
   scale *= expf(k0 * (c0 - 1.f) + k1 * (c1 - 1.f));
@@ -1056,16 +1056,14 @@ __device__ float product_scale(float m0x, float m0y, float m0z, float k0, float 
 __device__ float integrated_division(float m0x, float m0y, float m0z, float k0, float n0, float m1x, float m1y,
                                      float m1z, float k1, float n1, float em2k1) {
   float pmx = k0 * m0x + k1 * m1x, pmy = k0 * m0y + k1 * m1y, pmz = k0 * m0z + k1 * m1z;
-  float pk = sqrtf(dot3f(pmx, pmy, pmz, pmx, pmy, pmz));
-  float pn = 1.f / (4.f * 3.14159265358979f);
-  if (pk > 1e-3f) {
-    const float e = expf(-2.f * pk);
-    pn = pk / (kTwoPi * (1.f - e));
-    pmx /= pk; pmy /= pk; pmz /= pk;
-  } else {
-    pk = 0.f;
-    pmx = m0x; pmy = m0y; pmz = m0z;
-  }
+  const float r = sqrtf(dot3f(pmx, pmy, pmz, pmx, pmy, pmz));
+  const bool big = r > 1e-3f;  // both outcomes computed and selected (no branch)
+  const float e = expf(-2.f * r);
+  const float pn = big ? r / (kTwoPi * (1.f - e)) : 1.f / (4.f * 3.14159265358979f);
+  pmx = big ? pmx / r : m0x;
+  pmy = big ? pmy / r : m0y;
+  pmz = big ? pmz / r : m0z;
+  const float pk = big ? r : 0.f;
   float scale = (n0 * n1) / pn;
   const float c0 = dot3f(m0x, m0y, m0z, pmx, pmy, pmz), c1 = dot3f(m1x, m1y, m1z, pmx, pmy, pmz);
   scale *= (4.f * 3.14159265358979f * 3.14159265358979f * (1.f - em2k1)) / (k1 * k1);
@@ -1085,25 +1083,23 @@ __device__ __forceinline__ MergedLobe merged_lobe(const VMMParams& V, int i0, in
   const float w0 = V.w[i0], w1 = V.w[i1], c0 = V.mc[i0], c1 = V.mc[i1];
   MergedLobe m;
   m.w = w0 + w1;
-  m.kappa = 0.f;
-  m.nrm = kOneOverFourPi;
-  m.em2k = 1.f;
-  m.dx = (w0 * c0 * V.mux[i0] + w1 * c1 * V.mux[i1]) / m.w;
-  m.dy = (w0 * c0 * V.muy[i0] + w1 * c1 * V.muy[i1]) / m.w;
-  m.dz = (w0 * c0 * V.muz[i0] + w1 * c1 * V.muz[i1]) / m.w;
-  float mcos = m.dx * m.dx + m.dy * m.dy + m.dz * m.dz;
-  m.mc = mcos;
-  if (mcos > 0.f) {
-    mcos = sqrtf(mcos);
-    m.mc = mcos;
-    m.kappa = mc_to_kappa(mcos);
-    m.kappa = m.kappa < 1e-3f ? 0.f : m.kappa;
-    m.em2k = expf(-2.f * m.kappa);
-    m.nrm = m.kappa / (kTwoPi * (1.f - m.em2k));
-    m.dx /= mcos; m.dy /= mcos; m.dz /= mcos;
-  } else {
-    m.dx = V.mux[i0]; m.dy = V.muy[i0]; m.dz = V.muz[i0];
-  }
+  const float dx = (w0 * c0 * V.mux[i0] + w1 * c1 * V.mux[i1]) / m.w;
+  const float dy = (w0 * c0 * V.muy[i0] + w1 * c1 * V.muy[i1]) / m.w;
+  const float dz = (w0 * c0 * V.muz[i0] + w1 * c1 * V.muz[i1]) / m.w;
+  const float mcos = dx * dx + dy * dy + dz * dz;
+  // both outcomes computed and selected (no branch)
+  const bool pos = mcos > 0.f;
+  const float r = sqrtf(mcos);
+  float kap = mc_to_kappa(r);
+  kap = kap < 1e-3f ? 0.f : kap;
+  const float e = expf(-2.f * kap);
+  m.mc = pos ? r : mcos;
+  m.kappa = pos ? kap : 0.f;
+  m.em2k = pos ? e : 1.f;
+  m.nrm = pos ? kap / (kTwoPi * (1.f - e)) : kOneOverFourPi;
+  m.dx = pos ? dx / r : V.mux[i0];
+  m.dy = pos ? dy / r : V.muy[i0];
+  m.dz = pos ? dz / r : V.muz[i0];
   return m;
 }
 
@@ -1159,9 +1155,10 @@ __device__ __forceinline__ SplitSlot split_merged_slot(const SplitStatsState& SS
           ajx, ajy);
   const float invWK = wK > 0.f ? 1.f / wK : 1.f;
   const float swI = SS.ssw[i], swJ = SS.ssw[j], swK = swI + swJ;
-  float cI[3] = {0.f, 0.f, 0.f}, cJ[3] = {0.f, 0.f, 0.f};
-  if (swI > 0.f) { const float r = 1.f / swI; cI[0] = SS.covx[i] * r; cI[1] = SS.covy[i] * r; cI[2] = SS.covz[i] * r; }
-  if (swJ > 0.f) { const float r = 1.f / swJ; cJ[0] = SS.covx[j] * r; cJ[1] = SS.covy[j] * r; cJ[2] = SS.covz[j] * r; }
+  const float rI = 1.f / swI, rJ = 1.f / swJ;  // used where the weight is positive (no branch)
+  const bool pI = swI > 0.f, pJ = swJ > 0.f;
+  const float cI[3] = {pI ? SS.covx[i] * rI : 0.f, pI ? SS.covy[i] * rI : 0.f, pI ? SS.covz[i] * rI : 0.f};
+  const float cJ[3] = {pJ ? SS.covx[j] * rJ : 0.f, pJ ? SS.covy[j] * rJ : 0.f, pJ ? SS.covz[j] * rJ : 0.f};
   const float mII[3] = {aix * aix, aiy * aiy, aix * aiy};
   const float mJJ[3] = {ajx * ajx, ajy * ajy, ajx * ajy};
   float cK[3];
