@@ -29,8 +29,15 @@ import time
 # Serving next to training needs a hardware queue per concurrently busy stream
 # (a field uses 3 streams, a query server slots + 1); HIP's default is 4 per
 # process, and streams sharing a queue serialise (a query batch would wait
-# behind a region kernel).  Must be set before the HIP runtime starts.
-os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+# behind a region kernel).  A smaller value in the environment (such as the
+# default 4 exported explicitly) is raised.  Must be set before the HIP
+# runtime starts.
+try:
+    _hwq = int(os.environ.get("GPU_MAX_HW_QUEUES", "0"))
+except ValueError:
+    _hwq = 0
+if _hwq < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 import numpy as np
 
