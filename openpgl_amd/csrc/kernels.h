@@ -130,10 +130,11 @@ void launch_query(const QueryField& surf, const QueryField& vol, const HGRep* hg
 struct PathSampleOptions;
 struct PathSample;
 size_t path_scan_temp_bytes(uint32_t numSegments);
+size_t path_tile_words(uint32_t numSegments);  // entries of tileFirst
 void launch_path_records(const PathVertex* segs, uint32_t numSegments, const uint32_t* offsets, uint32_t numPaths,
                          const PathSampleOptions& o, PathSample* records, unsigned long long* flags, uint32_t* pathOf,
-                         unsigned long long* incl, void* scanTemp, size_t scanBytes, unsigned long long* totals,
-                         hipStream_t s);
+                         uint32_t* tileFirst, unsigned long long* incl, void* scanTemp, size_t scanBytes,
+                         unsigned long long* totals, hipStream_t s);
 void launch_path_scatter(const PathSample* records, const unsigned long long* flags, const uint32_t* pathOf,
                          const unsigned long long* incl, const uint32_t* offsets, uint32_t numSegments,
                          PathSample* surfOut, PathSample* volOut, hipStream_t s);

@@ -4,16 +4,18 @@
 // PGLSampleData records, surface and volume apart, in the order the
 // per-path calls would append them to one sample storage.
 //
-// One thread per vertex: the lanes of a wave cover consecutive vertices, so
-// they read one contiguous stretch of the vertex array (lanes of one path
-// read the same later vertices) instead of one path per lane with loads
-// scattered over many paths.  A vertex's thread redoes the path's backward
-// distance walk from the last vertex down to its own (the exact sequence of
-// float operations of the serial walk) and then its own contribution
-// (path_samples.h), and stores its record and a packed surface | volume << 32
-// flag.  An inclusive scan of the flags gives every record its place: the
-// records of earlier paths, plus those of the same path at later vertices
-// (the walk emits vertex n-2 first).
+// One thread per vertex, 256 consecutive vertices per workgroup (a tile).
+// The tile's vertices, and the rest of the path its last vertex belongs to
+// (up to 64 more), are first copied to LDS with coalesced loads, together
+// with the offsets of the tile's paths; each vertex's thread then finds its
+// path in LDS, redoes the path's backward distance walk from the last vertex
+// down to its own (the exact sequence of float operations of the serial
+// walk) and its own contribution (path_samples.h) reading LDS, and stores its
+// record and a packed surface | volume << 32 flag.  A path reaching beyond
+// the staged vertices is read from global memory.  The first path of every
+// tile is found beforehand, one thread per tile.  An inclusive scan of the
+// flags gives every record its place: the records of earlier paths, plus
+// those of the same path at later vertices (the walk emits vertex n-2 first).
 #pragma clang fp contract(off)
 #include <hip/hip_runtime.h>
 
@@ -47,29 +49,131 @@ __global__ void __launch_bounds__(kPathThreads) k_path_check(const uint32_t* __r
   if (offsets[p] > offsets[p + 1] || offsets[p + 1] > numSegments) totals[2] = 1ull;
 }
 
+constexpr uint32_t kStage = kPathThreads + 64;  // staged vertices of a tile
+constexpr uint32_t kTileOffsets = 512;           // path offsets of a tile kept in LDS
+constexpr uint32_t kStepBuckets = 64;            // walk lengths >= 63 share the last bucket
+
+// tileFirst[t] = path of vertex t * kPathThreads, tileFirst[numTiles] = path
+// of the last vertex (numSegments >= 1)
+__global__ void __launch_bounds__(kPathThreads) k_path_tiles(const uint32_t* __restrict__ offsets, uint32_t numPaths,
+                                                             uint32_t numSegments, uint32_t numTiles,
+                                                             uint32_t* __restrict__ tileFirst) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > numTiles) return;
+  tileFirst[t] = path_of(offsets, numPaths, t < numTiles ? t * kPathThreads : numSegments - 1);
+}
+
+// the last p in [0, count) with offs[p] <= v (0 if none)
+__device__ __forceinline__ uint32_t tile_path(const uint32_t* offs, uint32_t count, uint32_t v) {
+  uint32_t lo = 0, hi = count;
+  while (hi - lo > 1) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (offs[mid] <= v) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
 __global__ void __launch_bounds__(kPathThreads) k_path_vertex(const PathVertex* __restrict__ segs, uint32_t numSegments,
                                                               const uint32_t* __restrict__ offsets, uint32_t numPaths,
+                                                              const uint32_t* __restrict__ tileFirst,
                                                               PathSampleOptions o, PathSample* __restrict__ records,
                                                               unsigned long long* __restrict__ flags,
                                                               uint32_t* __restrict__ pathOf) {
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < numSegments; v += stride) {
-    unsigned long long flag = 0ull;
-    uint32_t p = kNoPath;
-    const uint32_t q = path_of(offsets, numPaths, v);
-    const uint32_t b = offsets[q], e = offsets[q + 1];
-    if (b <= v && v < e && e <= numSegments) {
-      p = q;
-      const int n = (int)(e - b), i = (int)(v - b);
-      PathSample s;
-      if (i <= n - 2 && path_vertex_sample(segs + b, n, i, o, s)) {
-        records[v] = s;
-        flag = (s.flags & 2u) ? (1ull << 32) : 1ull;
-      }
-    }
-    flags[v] = flag;
-    pathOf[v] = p;
+  __shared__ PathVertex stage[kStage];
+  __shared__ uint32_t offs[kTileOffsets];
+  __shared__ uint32_t stageEnd;
+  __shared__ uint32_t hist[kStepBuckets];
+  __shared__ uint32_t vb[kPathThreads], ve[kPathThreads], vq[kPathThreads];
+  __shared__ uint16_t order[kPathThreads];
+  const uint32_t tid = threadIdx.x, t0 = blockIdx.x * kPathThreads;
+  const uint32_t t1 = min(t0 + kPathThreads, numSegments);
+  const uint32_t v = t0 + tid;
+  // the tile's paths p0 .. p1 (monotone offsets); their offsets p0 .. p1 + 1
+  const uint32_t p0 = tileFirst[blockIdx.x];
+  const uint32_t p1 = max(p0, min(tileFirst[blockIdx.x + 1], numPaths - 1));
+  const uint32_t np = p1 - p0 + 2;
+  const bool inLds = np <= kTileOffsets;
+  if (inLds)
+    for (uint32_t i = tid; i < np; i += kPathThreads) offs[i] = offsets[p0 + i];
+  if (tid == 0) stageEnd = t1;
+  __syncthreads();
+  uint32_t q = kNoPath, b = 0, e = 0;
+  if (v < t1) {
+    q = inLds ? p0 + tile_path(offs, np - 1, v) : path_of(offsets, numPaths, v);
+    b = inLds ? offs[q - p0] : offsets[q];
+    e = inLds ? offs[q - p0 + 1] : offsets[q + 1];
+    // the tile's last vertex extends the staged range to the end of its path
+    if (v + 1 == t1 && e > t1 && e <= numSegments) stageEnd = min(e, t0 + kStage);
   }
+  __syncthreads();
+  const uint32_t nStage = stageEnd - t0;
+  {
+    // 8-byte words (the vertex alignment); every thread issues all its loads
+    // before its first store, so a tile pays one memory latency
+    constexpr uint32_t kWords = (uint32_t)(sizeof(PathVertex) / 8);
+    constexpr uint32_t kIt = (kStage * kWords + kPathThreads - 1) / kPathThreads;
+    const uint2* src = reinterpret_cast<const uint2*>(segs + t0);
+    uint2* dst = reinterpret_cast<uint2*>(stage);
+    const uint32_t words = nStage * kWords;
+    uint2 tmp[kIt];
+#pragma unroll
+    for (uint32_t k = 0; k < kIt; k++) {
+      const uint32_t i = tid + k * kPathThreads;
+      if (i < words) tmp[k] = src[i];
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kIt; k++) {
+      const uint32_t i = tid + k * kPathThreads;
+      if (i < words) dst[i] = tmp[k];
+    }
+  }
+  // a vertex's walk takes e - v - 1 steps: the tile's vertices are handed to
+  // the threads in decreasing order of that length (counting sort in LDS), so
+  // that the lanes of a wave take about the same number of steps
+  const bool valid = v < t1 && b <= v && v < e && e <= numSegments;
+  const uint32_t steps = valid && v + 1 < e ? e - v - 1 : 0u;
+  const uint32_t key = kStepBuckets - 1 - min(steps, kStepBuckets - 1);
+  if (tid < kStepBuckets) hist[tid] = 0u;
+  vb[tid] = b;
+  ve[tid] = e;
+  vq[tid] = valid ? q : kNoPath;
+  __syncthreads();
+  const uint32_t rank = v < t1 ? atomicAdd(&hist[key], 1u) : 0u;
+  __syncthreads();
+  if (tid < kStepBuckets) {  // exclusive prefix of the bucket counts (one wave)
+    const uint32_t c = hist[tid];
+    uint32_t x = c;
+#pragma unroll
+    for (int d = 1; d < (int)kStepBuckets; d <<= 1) {
+      const uint32_t y = __shfl_up(x, d, (int)kStepBuckets);
+      if ((int)tid >= d) x += y;
+    }
+    hist[tid] = x - c;
+  }
+  __syncthreads();
+  if (v < t1) order[hist[key] + rank] = (uint16_t)tid;
+  __syncthreads();
+  if (v >= t1) return;
+  // this thread's vertex (the order within a bucket varies; each record does not)
+  const uint32_t w = order[tid], u = t0 + w;
+  const uint32_t ub = vb[w], ue = ve[w], uq = vq[w];
+  unsigned long long flag = 0ull;
+  if (uq != kNoPath) {
+    PathSample s;
+    bool ok = false;
+    if (u + 1 < ue) {  // vertices n-1 (the last of a path) yield no sample
+      // the walk reads vertices u .. e-1 only: vertex u is index 0 of the suffix
+      if (ue <= t0 + nStage) ok = path_vertex_sample(stage + w, (int)(ue - u), 0, o, s);
+      else ok = path_vertex_sample(segs + u, (int)(ue - u), 0, o, s);
+    }
+    if (ok) {
+      records[u] = s;
+      flag = (s.flags & 2u) ? (1ull << 32) : 1ull;
+    }
+  }
+  (void)ub;
+  flags[u] = flag;
+  pathOf[u] = uq;
 }
 
 __global__ void k_path_totals(const unsigned long long* __restrict__ incl, uint32_t numSegments,
@@ -118,16 +222,21 @@ size_t path_scan_temp_bytes(uint32_t numSegments) {
   return bytes;
 }
 
+size_t path_tile_words(uint32_t numSegments) { return (numSegments + kPathThreads - 1) / kPathThreads + 1; }
+
 void launch_path_records(const PathVertex* segs, uint32_t numSegments, const uint32_t* offsets, uint32_t numPaths,
                          const PathSampleOptions& o, PathSample* records, unsigned long long* flags, uint32_t* pathOf,
-                         unsigned long long* incl, void* scanTemp, size_t scanBytes, unsigned long long* totals,
-                         hipStream_t s) {
+                         uint32_t* tileFirst, unsigned long long* incl, void* scanTemp, size_t scanBytes,
+                         unsigned long long* totals, hipStream_t s) {
   PGL_KERNEL_CHECK(hipMemsetAsync(totals, 0, 3 * sizeof(unsigned long long), s));
   if (numPaths == 0 || numSegments == 0) return;
   hipLaunchKernelGGL(k_path_check, dim3((numPaths + kPathThreads - 1) / kPathThreads), dim3(kPathThreads), 0, s,
                      offsets, numPaths, numSegments, totals);
-  hipLaunchKernelGGL(k_path_vertex, dim3(path_grid(numSegments)), dim3(kPathThreads), 0, s, segs, numSegments, offsets,
-                     numPaths, o, records, flags, pathOf);
+  const uint32_t numTiles = (numSegments + kPathThreads - 1) / kPathThreads;
+  hipLaunchKernelGGL(k_path_tiles, dim3(numTiles / kPathThreads + 1), dim3(kPathThreads), 0, s, offsets, numPaths,
+                     numSegments, numTiles, tileFirst);
+  hipLaunchKernelGGL(k_path_vertex, dim3(numTiles), dim3(kPathThreads), 0, s, segs, numSegments, offsets, numPaths,
+                     tileFirst, o, records, flags, pathOf);
   PGL_KERNEL_CHECK(hipcub::DeviceScan::InclusiveSum(scanTemp, scanBytes, flags, incl, (int)numSegments, s));
   hipLaunchKernelGGL(k_path_totals, dim3(1), dim3(64), 0, s, incl, numSegments, totals);
 }

@@ -30,10 +30,12 @@ void PathSampleBatch::count(const PathVertex* segs, uint32_t numSegments, const 
   flags_.ensure(V * 8);
   incl_.ensure(V * 8);
   pathOf_.ensure(V * 4);
+  tiles_.ensure(path_tile_words((uint32_t)V) * 4);
   const size_t scanBytes = path_scan_temp_bytes(std::max<uint32_t>(numSegments_, 1));
   scanTemp_.ensure(std::max<size_t>(scanBytes, 16));
   launch_path_records(segs, numSegments_, offsets, numPaths, o, records_.as<PathSample>(),
-                      flags_.as<unsigned long long>(), pathOf_.as<uint32_t>(), incl_.as<unsigned long long>(),
+                      flags_.as<unsigned long long>(), pathOf_.as<uint32_t>(), tiles_.as<uint32_t>(),
+                      incl_.as<unsigned long long>(),
                       scanTemp_.p, scanBytes, totals_.as<unsigned long long>(), stream_);
   PGL_HIP_CHECK(hipMemcpyAsync(hTotals_.p, totals_.p, 24, hipMemcpyDeviceToHost, stream_));
   PGL_HIP_CHECK(hipStreamSynchronize(stream_));

@@ -43,7 +43,7 @@ class PathSampleBatch {
   void scatter(PathSample* surfOut, PathSample* volOut);
 
   hipStream_t stream_ = nullptr;
-  DevBuf records_, flags_, pathOf_, incl_, scanTemp_, totals_;
+  DevBuf records_, flags_, pathOf_, tiles_, incl_, scanTemp_, totals_;
   const uint32_t* offsets_ = nullptr;  // of the last count()
   uint32_t numSegments_ = 0;
   PinnedBuf hTotals_;

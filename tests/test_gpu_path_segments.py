@@ -46,6 +46,27 @@ def test_batch_equals_oracle(nee, guide, rr_direct, max_distance):
     assert v.tobytes() == wv.tobytes()
 
 
+def test_batch_long_and_empty_paths():
+    """Paths reaching far beyond a 256-vertex tile (read from global memory
+    past the staged vertices) and tiles holding more paths than the offsets
+    kept in LDS (runs of empty paths) give the oracle's records too."""
+    import openpgl_amd as pgl
+    batch = pgl.PathSegmentBatch()
+    seg, off = random_paths(300, seed=11, max_len=700)
+    print("long paths", len(off) - 1, "vertices", len(seg), "longest", int(np.diff(off).max()), flush=True)
+    s, v = _device_samples(batch, seg, off)
+    ws, wv = oracle_path_samples(seg, off)
+    assert len(s) == len(ws) and len(v) == len(wv) and len(s) > 1000
+    assert s.tobytes() == ws.tobytes() and v.tobytes() == wv.tobytes()
+    seg, off = random_paths(3000, seed=12, max_len=10)
+    off2 = np.concatenate([np.repeat(off[:-1], 25), off[-1:]]).astype(np.uint32)  # 24 empty paths before each
+    print("with empty paths", len(off2) - 1, "vertices", len(seg), flush=True)
+    s, v = _device_samples(batch, seg, off2)
+    ws, wv = oracle_path_samples(seg, off2)
+    assert len(s) == len(ws) and len(v) == len(wv) and len(s) > 1000
+    assert s.tobytes() == ws.tobytes() and v.tobytes() == wv.tobytes()
+
+
 def test_batch_edge_cases():
     import openpgl_amd as pgl
     from openpgl_amd.hip_runtime import DeviceArray
