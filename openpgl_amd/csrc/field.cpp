@@ -438,7 +438,9 @@ void KindField::kdUpdate(uint32_t n, hipStream_t s) {
 void KindField::ensureRegionCapacity(uint32_t r, hipStream_t s) {
   if (r <= stateCapacity_) return;
   uint32_t cap = std::max<uint32_t>(r, stateCapacity_ * 2);
-  cap = std::max<uint32_t>(cap, 64);
+  // reallocation synchronises the stream: start with room for 4096 regions
+  // (16 MiB) so that a growing field does not pay it every few updates
+  cap = std::max<uint32_t>(cap, 4096);
   DevBuf nb;
   nb.ensure(size_t(cap) * sizeof(RegionState));
   PGL_HIP_CHECK(hipMemsetAsync(nb.p, 0, size_t(cap) * sizeof(RegionState), s));
