@@ -372,7 +372,7 @@ def batched_em(num_regions=4096, cpu=True, threads=16):
     batch.set_schedule(**table[best][0])
     best_ms = table[best][1]
     default_ms = [ms for sc, ms in table if sc == {"per_region_launches": 0, "grid": 0, "balance": 1.0,
-                                                    "min_chunks_per_rank": 1, "max_team": 64, "cooperative": 1}]
+                                                    "min_chunks_per_rank": 1, "max_team": 64, "cooperative": 0}]
     n = int(len(recs))
     flops = FLOPS_ESTEP * first["estep_evals"] + FLOPS_SPLIT * first["split_evals"] + FLOPS_DIST * first["dist_evals"]
     out = {"workload": "%d regions, samples log-U[64, 65536] (%d total), initial K in {8,16,32}, fit step"
@@ -505,7 +505,7 @@ def main():
     ap.add_argument("--no-paths", action="store_true", help="skip the path-segment workload")
     ap.add_argument("--region-grid", type=int, default=None, help="region kernel workgroups (default: all co-resident)")
     ap.add_argument("--region-balance", type=float, default=None, help="region team balance (default 1)")
-    ap.add_argument("--region-cooperative", type=int, default=None, help="1: cooperative region launch (default)")
+    ap.add_argument("--region-cooperative", type=int, default=None, help="1: cooperative region launch (default 0: ordinary launch)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (rocprofv3 --pmc calibration)")
     args = ap.parse_args()

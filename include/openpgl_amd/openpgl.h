@@ -344,8 +344,12 @@ size_t pglFieldGetUpdateHistory(PGLField field, int kind, double* out, size_t ma
  *   minChunksPerRank   at least this many 1024-sample chunks per workgroup
  *   maxTeam            largest team
  *   cooperative        1: cooperative launch (the runtime guarantees that the
- *                      whole grid is resident); 0: ordinary launch (waits
- *                      stay bounded; other streams' kernels may share CUs) */
+ *                      whole grid is resident and keeps other kernels off
+ *                      the device meanwhile); 0 (default): ordinary launch
+ *                      (the grid fits the device, so every workgroup becomes
+ *                      resident once other streams' kernels end; waits stay
+ *                      bounded; other streams' kernels share the CUs that
+ *                      finished workgroups free) */
 typedef struct {
   int perRegionLaunches;
   uint32_t grid;

@@ -311,7 +311,7 @@ def _schedule_dict(sc):
 def _schedule_struct(d):
     return RegionSchedule(int(d.get("per_region_launches", 0)), int(d.get("grid", 0)), float(d.get("balance", 1.0)),
                           int(d.get("min_chunks_per_rank", 1)), int(d.get("max_team", 64)),
-                          int(d.get("cooperative", 1)))
+                          int(d.get("cooperative", 0)))
 
 
 class ServerConfig(ctypes.Structure):

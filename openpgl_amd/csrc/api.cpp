@@ -879,13 +879,15 @@ void pglRegionBatchGetSchedule(PGLRegionBatch batch, PGLRegionSchedule* sc) PGL_
 PGL_CATCH_END_VOID
 
 size_t pglRegionScheduleCandidates(PGLRegionSchedule* out, size_t maxCount) {
-  // search space of the autotuner: grid (all co-resident workgroups or one per
-  // CU), team balance, chunks per rank, largest team
+  // search space of the autotuner: launch kind (ordinary or cooperative),
+  // grid (all co-resident workgroups or one per CU), team balance, chunks per
+  // rank, largest team
   std::vector<PGLRegionSchedule> c;
-  for (uint32_t gridHalf = 0; gridHalf < 2; gridHalf++)
-    for (float bal : {0.5f, 1.f, 2.f, 4.f})
-      for (uint32_t mc : {1u, 2u, 4u})
-        for (uint32_t mt : {16u, 64u}) c.push_back(PGLRegionSchedule{0, gridHalf ? 1u : 0u, bal, mc, mt, 1});
+  for (int coop = 0; coop < 2; coop++)
+    for (uint32_t gridHalf = 0; gridHalf < 2; gridHalf++)
+      for (float bal : {0.5f, 1.f, 2.f, 4.f})
+        for (uint32_t mc : {1u, 2u, 4u})
+          for (uint32_t mt : {16u, 64u}) c.push_back(PGLRegionSchedule{0, gridHalf ? 1u : 0u, bal, mc, mt, coop});
   if (out)
     for (size_t i = 0; i < c.size() && i < maxCount; i++) out[i] = c[i];
   return c.size();

@@ -20,7 +20,14 @@ struct RegionSchedule {
   float balance = 1.f;        // team target = total samples / (grid * balance), at least minChunksPerRank chunks
   uint32_t minChunksPerRank = 1;
   uint32_t maxTeam = 64;
-  int cooperative = 1;        // launch with hipLaunchCooperativeKernel (co-residency guaranteed by the runtime)
+  // 1: hipLaunchCooperativeKernel (the runtime makes the whole grid resident
+  // before it starts and keeps other streams' kernels off the device
+  // meanwhile); 0: ordinary launch.  The grid never exceeds what is
+  // resident at once and every other kernel ends, so every workgroup is
+  // eventually resident and every team wait ends; the waits stay bounded in
+  // any case.  The ordinary launch lets the next update's spatial kernels use
+  // the CUs that finished workgroups free.
+  int cooperative = 0;
 };
 
 struct RegionSamplesView {
