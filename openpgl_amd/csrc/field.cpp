@@ -593,17 +593,15 @@ void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
   knnPivots_ = (const float4*)(tab + offKnn);
 
   // region-contiguous order: stable sort of (region, sample index)
-  launch_leaf_keys(dNodeOf_.as<uint32_t>(), (const KDNode*)tab, n, dKeys_.as<uint32_t>(), dVals_.as<uint32_t>(), sp);
-  // the query kernels walk the tree in the final numbering
-  dNodes_.ensure(nodes_.size() * sizeof(KDNode));
-  PGL_HIP_CHECK(hipMemcpyAsync(dNodes_.p, pinned(pin_[P][5], nodes_.data(), nodes_.size()),
-                               nodes_.size() * sizeof(KDNode), hipMemcpyHostToDevice, sp));
   if (R <= kSortMaxKeys) {
-    // stable counting sort: every region's first position is known from the kd counts
+    // stable counting sort keyed by the region of each sample's leaf: every
+    // region's first position is known from the kd counts
     dCountSortTmp_.ensure(counting_sort_temp_bytes(n, R));
-    counting_sort_pairs(dCountSortTmp_.p, dKeys_.as<uint32_t>(), (const uint32_t*)(tab + offBegin), R, n,
-                        dKeys2_.as<uint32_t>(), dVals2_.as<uint32_t>(), sp);
+    counting_sort_leaves(dCountSortTmp_.p, dNodeOf_.as<uint32_t>(), (const KDNode*)tab,
+                         (const uint32_t*)(tab + offBegin), R, n, dKeys2_.as<uint32_t>(), dVals2_.as<uint32_t>(), sp);
   } else {
+    launch_leaf_keys(dNodeOf_.as<uint32_t>(), (const KDNode*)tab, n, dKeys_.as<uint32_t>(), dVals_.as<uint32_t>(),
+                     sp);
     int endBit = 1;
     while ((1u << endBit) < R && endBit < 32) endBit++;
     sort_pairs(dSortTmp_.p, sortBytes, dKeys_.as<uint32_t>(), dKeys2_.as<uint32_t>(), dVals_.as<uint32_t>(),
@@ -652,6 +650,12 @@ void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
   PGL_HIP_CHECK(hipEventRecord(evDir_[P][1], s));
 
   if (kd_.knnLookup) launch_knn8(knnPivots_, R, dNbIds_.as<uint32_t>(), dNbPts_.as<float4>(), s);
+  // the query kernels walk the tree in the final numbering: uploaded behind
+  // the region kernel, off the spatial phase's path; every reader of the
+  // tree first finishes the pending update, which waits for this copy
+  dNodes_.ensure(nodes_.size() * sizeof(KDNode));
+  PGL_HIP_CHECK(hipMemcpyAsync(dNodes_.p, pinned(pin_[P][5], nodes_.data(), nodes_.size()),
+                               nodes_.size() * sizeof(KDNode), hipMemcpyHostToDevice, s));
   PGL_HIP_CHECK(hipEventRecord(evDir_[P][2], s));
 
   for (auto& r : regions_) { r.splitFlag = false; r.copyFrom = -1; }

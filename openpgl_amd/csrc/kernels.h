@@ -57,6 +57,10 @@ static const uint32_t kSortMaxKeys = 4096;
 size_t counting_sort_temp_bytes(uint32_t n, uint32_t numKeys);
 void counting_sort_pairs(void* temp, const uint32_t* keys, const uint32_t* keyBegin, uint32_t numKeys, uint32_t n,
                          uint32_t* keysOut, uint32_t* valsOut, hipStream_t s);
+// the same with the key of sample i computed as keyNodes[nodeOf[i]].idx()
+// (the region of its leaf; launch_leaf_keys and the sort in one)
+void counting_sort_leaves(void* temp, const uint32_t* nodeOf, const KDNode* keyNodes, const uint32_t* keyBegin,
+                          uint32_t numKeys, uint32_t n, uint32_t* keysOut, uint32_t* valsOut, hipStream_t s);
 // keys/values of region-contiguous records (begin: numRegions ascending offsets)
 void launch_region_keys(const uint32_t* begin, uint32_t numRegions, uint32_t n, uint32_t* keys, uint32_t* vals,
                         hipStream_t s);

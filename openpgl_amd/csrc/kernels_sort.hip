@@ -26,7 +26,18 @@ constexpr int kSortThreads = 256;
 constexpr int kSortItems = 4;
 constexpr uint32_t kSortTile = kSortThreads * kSortItems;
 
-__global__ void __launch_bounds__(kSortThreads) k_sort_hist(const uint32_t* __restrict__ keys, uint32_t n,
+// key of element i: keys[i], or the region of the leaf node nodeOf[i]
+// (keyNodes[nodeOf[i]].idx()) when keys is null
+struct SortKeys {
+  const uint32_t* __restrict__ keys;
+  const uint32_t* __restrict__ nodeOf;
+  const KDNode* __restrict__ keyNodes;
+  __device__ __forceinline__ uint32_t operator()(uint64_t i) const {
+    return keys ? keys[i] : keyNodes[nodeOf[i]].idx();
+  }
+};
+
+__global__ void __launch_bounds__(kSortThreads) k_sort_hist(SortKeys keys, uint32_t n,
                                                             uint32_t tilesPerBlock, uint32_t numKeys,
                                                             uint32_t* __restrict__ counts) {
   __shared__ uint32_t hist[kSortMaxKeys];
@@ -35,7 +46,7 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_hist(const uint32_t* __re
   const uint64_t begin = (uint64_t)blockIdx.x * tilesPerBlock * kSortTile;
   const uint64_t end = min<uint64_t>((uint64_t)n, begin + (uint64_t)tilesPerBlock * kSortTile);
   for (uint64_t i = begin + threadIdx.x; i < end; i += kSortThreads) {
-    const uint32_t k = keys[i];
+    const uint32_t k = keys(i);
     if (k < numKeys) atomicAdd(&hist[k], 1u);
   }
   __syncthreads();
@@ -68,7 +79,7 @@ struct MaxOp {
   __device__ __forceinline__ int operator()(int a, int b) const { return a > b ? a : b; }
 };
 
-__global__ void __launch_bounds__(kSortThreads) k_sort_scatter(const uint32_t* __restrict__ keys, uint32_t n,
+__global__ void __launch_bounds__(kSortThreads) k_sort_scatter(SortKeys keys, uint32_t n,
                                                                uint32_t tilesPerBlock, uint32_t numKeys, int bits,
                                                                const uint32_t* __restrict__ offsets,
                                                                uint32_t* __restrict__ keysOut,
@@ -95,7 +106,7 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_scatter(const uint32_t* _
     for (int j = 0; j < kSortItems; j++) {
       const uint64_t i = base + t * kSortItems + j;
       const bool ok = i < n;
-      const uint32_t key = ok ? keys[i] : 0u;
+      const uint32_t key = ok ? keys(i) : 0u;
       const bool real = ok && key < numKeys;  // keys out of range are dropped (never written out of bounds)
       k[j] = real ? key : ((1u << bits) - 1u);  // padding sorts after every real element of its key
       v[j] = real ? (uint32_t)i : pad;
@@ -158,8 +169,8 @@ size_t counting_sort_temp_bytes(uint32_t n, uint32_t numKeys) {
   return 2 * (size_t)blocks * numKeys * sizeof(uint32_t);
 }
 
-void counting_sort_pairs(void* temp, const uint32_t* keys, const uint32_t* keyBegin, uint32_t numKeys, uint32_t n,
-                         uint32_t* keysOut, uint32_t* valsOut, hipStream_t s) {
+static void counting_sort(void* temp, SortKeys keys, const uint32_t* keyBegin, uint32_t numKeys, uint32_t n,
+                          uint32_t* keysOut, uint32_t* valsOut, hipStream_t s) {
   if (n == 0 || numKeys == 0) return;
   const uint32_t tiles = (n + kSortTile - 1) / kSortTile;
   const uint32_t blocks = std::min<uint32_t>(std::max<uint32_t>(tiles, 1), 1024);
@@ -172,6 +183,16 @@ void counting_sort_pairs(void* temp, const uint32_t* keys, const uint32_t* keyBe
   hipLaunchKernelGGL(k_sort_scan, dim3(numKeys), dim3(kSortThreads), 0, s, counts, blocks, numKeys, keyBegin, offsets);
   hipLaunchKernelGGL(k_sort_scatter, dim3(blocks), dim3(kSortThreads), 0, s, keys, n, tilesPerBlock, numKeys, bits,
                      offsets, keysOut, valsOut);
+}
+
+void counting_sort_pairs(void* temp, const uint32_t* keys, const uint32_t* keyBegin, uint32_t numKeys, uint32_t n,
+                         uint32_t* keysOut, uint32_t* valsOut, hipStream_t s) {
+  counting_sort(temp, SortKeys{keys, nullptr, nullptr}, keyBegin, numKeys, n, keysOut, valsOut, s);
+}
+
+void counting_sort_leaves(void* temp, const uint32_t* nodeOf, const KDNode* keyNodes, const uint32_t* keyBegin,
+                          uint32_t numKeys, uint32_t n, uint32_t* keysOut, uint32_t* valsOut, hipStream_t s) {
+  counting_sort(temp, SortKeys{nullptr, nodeOf, keyNodes}, keyBegin, numKeys, n, keysOut, valsOut, s);
 }
 
 }  // namespace pglgpu
