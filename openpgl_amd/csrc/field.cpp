@@ -609,10 +609,10 @@ void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
   }
   PGL_HIP_CHECK(hipEventRecord(evSp_[P][2], sp));
 
-  float* const* soa = nullptr;
+  // gather into the region-ordered SoA, reprojected to the region pivots
   float* sv[6];
   for (int j = 0; j < 6; j++) sv[j] = dSoa_[P][j].as<float>();
-  soa = sv;
+  float* const* soa = sv;
   launch_gather_reproject(deviceSamples, dKeys2_.as<uint32_t>(), dVals2_.as<uint32_t>(), n,
                           (const float4*)(tab + offPiv), soa[0], soa[1], soa[2], soa[3], soa[4], soa[5],
                           fc_.noParallax == 0, sp);

@@ -79,11 +79,20 @@ struct MaxOp {
   __device__ __forceinline__ int operator()(int a, int b) const { return a > b ? a : b; }
 };
 
+// element (key k, index i) at sorted position dst: the (key, index) pair
+struct PairOut {
+  uint32_t* __restrict__ keysOut;
+  uint32_t* __restrict__ valsOut;
+  __device__ __forceinline__ void operator()(uint32_t dst, uint32_t k, uint32_t i) const {
+    keysOut[dst] = k;
+    valsOut[dst] = i;
+  }
+};
+
+template <class Out>
 __global__ void __launch_bounds__(kSortThreads) k_sort_scatter(SortKeys keys, uint32_t n,
                                                                uint32_t tilesPerBlock, uint32_t numKeys, int bits,
-                                                               const uint32_t* __restrict__ offsets,
-                                                               uint32_t* __restrict__ keysOut,
-                                                               uint32_t* __restrict__ valsOut) {
+                                                               const uint32_t* __restrict__ offsets, Out out) {
   typedef hipcub::BlockRadixSort<uint32_t, kSortThreads, kSortItems, uint32_t> Sort;
   typedef hipcub::BlockScan<int, kSortThreads> Scan;
   __shared__ union {
@@ -133,10 +142,7 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_scatter(SortKeys keys, ui
       if (v[j] == pad) continue;
       const int pos = (int)(t * kSortItems + j);
       const uint32_t dst = running[k[j]] + (uint32_t)(pos - runStart[j]);
-      if (dst < n) {
-        keysOut[dst] = k[j];
-        valsOut[dst] = v[j];
-      }
+      if (dst < n) out(dst, k[j], v[j]);
     }
     __syncthreads();
     // the last real element of each run advances its key's position
@@ -169,8 +175,9 @@ size_t counting_sort_temp_bytes(uint32_t n, uint32_t numKeys) {
   return 2 * (size_t)blocks * numKeys * sizeof(uint32_t);
 }
 
-static void counting_sort(void* temp, SortKeys keys, const uint32_t* keyBegin, uint32_t numKeys, uint32_t n,
-                          uint32_t* keysOut, uint32_t* valsOut, hipStream_t s) {
+template <class Out>
+static void counting_sort(void* temp, SortKeys keys, const uint32_t* keyBegin, uint32_t numKeys, uint32_t n, Out out,
+                          hipStream_t s) {
   if (n == 0 || numKeys == 0) return;
   const uint32_t tiles = (n + kSortTile - 1) / kSortTile;
   const uint32_t blocks = std::min<uint32_t>(std::max<uint32_t>(tiles, 1), 1024);
@@ -181,18 +188,18 @@ static void counting_sort(void* temp, SortKeys keys, const uint32_t* keyBegin, u
   uint32_t* offsets = counts + (size_t)blocks * numKeys;
   hipLaunchKernelGGL(k_sort_hist, dim3(blocks), dim3(kSortThreads), 0, s, keys, n, tilesPerBlock, numKeys, counts);
   hipLaunchKernelGGL(k_sort_scan, dim3(numKeys), dim3(kSortThreads), 0, s, counts, blocks, numKeys, keyBegin, offsets);
-  hipLaunchKernelGGL(k_sort_scatter, dim3(blocks), dim3(kSortThreads), 0, s, keys, n, tilesPerBlock, numKeys, bits,
-                     offsets, keysOut, valsOut);
+  hipLaunchKernelGGL(k_sort_scatter<Out>, dim3(blocks), dim3(kSortThreads), 0, s, keys, n, tilesPerBlock, numKeys,
+                     bits, offsets, out);
 }
 
 void counting_sort_pairs(void* temp, const uint32_t* keys, const uint32_t* keyBegin, uint32_t numKeys, uint32_t n,
                          uint32_t* keysOut, uint32_t* valsOut, hipStream_t s) {
-  counting_sort(temp, SortKeys{keys, nullptr, nullptr}, keyBegin, numKeys, n, keysOut, valsOut, s);
+  counting_sort(temp, SortKeys{keys, nullptr, nullptr}, keyBegin, numKeys, n, PairOut{keysOut, valsOut}, s);
 }
 
 void counting_sort_leaves(void* temp, const uint32_t* nodeOf, const KDNode* keyNodes, const uint32_t* keyBegin,
                           uint32_t numKeys, uint32_t n, uint32_t* keysOut, uint32_t* valsOut, hipStream_t s) {
-  counting_sort(temp, SortKeys{nullptr, nodeOf, keyNodes}, keyBegin, numKeys, n, keysOut, valsOut, s);
+  counting_sort(temp, SortKeys{nullptr, nodeOf, keyNodes}, keyBegin, numKeys, n, PairOut{keysOut, valsOut}, s);
 }
 
 }  // namespace pglgpu

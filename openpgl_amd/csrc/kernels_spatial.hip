@@ -15,14 +15,10 @@
 
 #include "common.h"
 #include "kernels.h"
+#include "sample_gather.h"
 
 namespace pglgpu {
 
-struct SampleAoS {
-  float px, py, pz, dx, dy, dz, w, pdf, dist;
-  uint32_t flags;
-};
-static_assert(sizeof(SampleAoS) == 40, "PGLSampleData is 40 B");
 
 // One coalesced pass over the 40-byte records: positions to SoA for the kd
 // rounds, every sample starts at the root.
@@ -302,11 +298,8 @@ __global__ void __launch_bounds__(256) k_region_ranges(const uint32_t* __restric
   }
 }
 
-// Gather into region-contiguous SoA and reproject each sample to its region's
-// pivot: origin = p + d * max(minDist, dist); d' = normalise(origin - pivot).
-// distance == inf keeps the direction (position moves to the pivot);
-// distance <= 0 keeps the sample unchanged.  Without reprojection (plain VMM
-// field, WEM:1186) the samples are gathered as recorded.
+// Gather into region-contiguous SoA, each sample reprojected to its region's
+// pivot (gather_sample).
 __global__ void __launch_bounds__(256) k_gather_reproject(const SampleAoS* __restrict__ in,
                                                           const uint32_t* __restrict__ sortedKeys,
                                                           const uint32_t* __restrict__ sortedIdx, uint32_t n,
@@ -317,26 +310,7 @@ __global__ void __launch_bounds__(256) k_gather_reproject(const SampleAoS* __res
                                                           int reproject) {
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
-    const SampleAoS s = in[sortedIdx[j]];
-    const float4 pv = regionPivot[sortedKeys[j]];
-    float dx = s.dx, dy = s.dy, dz = s.dz, dist = s.dist;
-    if (reproject && !isinf(s.dist) && s.dist > 0.f) {
-      const float dd = fmaxf(pv.w, s.dist);
-      const float ox = s.px + s.dx * dd - pv.x;
-      const float oy = s.py + s.dy * dd - pv.y;
-      const float oz = s.pz + s.dz * dd - pv.z;
-      const float len = sqrtf(ox * ox + oy * oy + oz * oz);
-      dx = ox / len;
-      dy = oy / len;
-      dz = oz / len;
-      dist = len;
-    }
-    odx[j] = dx;
-    ody[j] = dy;
-    odz[j] = dz;
-    ow[j] = s.w;
-    opdf[j] = s.pdf;
-    odist[j] = dist;
+    gather_sample(in[sortedIdx[j]], regionPivot[sortedKeys[j]], reproject, j, odx, ody, odz, ow, opdf, odist);
   }
 }
 
