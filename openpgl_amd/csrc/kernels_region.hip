@@ -86,8 +86,7 @@ struct Shared {
   float cost[32 * 32];
   float selfp[32][6];      // merging: product of each component with itself (direction, kappa, norm, scale)
   float term[64][3];       // merging: the three cost terms of up to 64 pairs
-  float mlobe[8];          // merging: the merged component of the chosen pair
-  float pre[8];            // merging: weight and direction of the chosen pair before the merge
+  float pre[10];           // merging: weight, direction and mean cosine of the chosen pair before the merge
   float scal[RWAVES][8];
   float stage[5][kChunk];  // sample pass: the current chunk's dx, dy, dz, w and pdf or distance
   double ll;
@@ -1079,13 +1078,13 @@ struct MergedLobe {
   float w, dx, dy, dz, kappa, nrm, em2k, mc;
 };
 
-__device__ __forceinline__ MergedLobe merged_lobe(const VMMParams& V, int i0, int i1) {
-  const float w0 = V.w[i0], w1 = V.w[i1], c0 = V.mc[i0], c1 = V.mc[i1];
+__device__ __forceinline__ MergedLobe merged_lobe_of(float w0, float c0, float m0x, float m0y, float m0z, float w1,
+                                                     float c1, float m1x, float m1y, float m1z) {
   MergedLobe m;
   m.w = w0 + w1;
-  const float dx = (w0 * c0 * V.mux[i0] + w1 * c1 * V.mux[i1]) / m.w;
-  const float dy = (w0 * c0 * V.muy[i0] + w1 * c1 * V.muy[i1]) / m.w;
-  const float dz = (w0 * c0 * V.muz[i0] + w1 * c1 * V.muz[i1]) / m.w;
+  const float dx = (w0 * c0 * m0x + w1 * c1 * m1x) / m.w;
+  const float dy = (w0 * c0 * m0y + w1 * c1 * m1y) / m.w;
+  const float dz = (w0 * c0 * m0z + w1 * c1 * m1z) / m.w;
   const float mcos = dx * dx + dy * dy + dz * dz;
   // both outcomes computed and selected (no branch)
   const bool pos = mcos > 0.f;
@@ -1097,10 +1096,15 @@ __device__ __forceinline__ MergedLobe merged_lobe(const VMMParams& V, int i0, in
   m.kappa = pos ? kap : 0.f;
   m.em2k = pos ? e : 1.f;
   m.nrm = pos ? kap / (kTwoPi * (1.f - e)) : kOneOverFourPi;
-  m.dx = pos ? dx / r : V.mux[i0];
-  m.dy = pos ? dy / r : V.muy[i0];
-  m.dz = pos ? dz / r : V.muz[i0];
+  m.dx = pos ? dx / r : m0x;
+  m.dy = pos ? dy / r : m0y;
+  m.dz = pos ? dz / r : m0z;
   return m;
+}
+
+__device__ __forceinline__ MergedLobe merged_lobe(const VMMParams& V, int i0, int i1) {
+  return merged_lobe_of(V.w[i0], V.mc[i0], V.mux[i0], V.muy[i0], V.muz[i0], V.w[i1], V.mc[i1], V.mux[i1], V.muy[i1],
+                        V.muz[i1]);
 }
 
 // product of component k with itself: {direction x, y, z, kappa, norm, scale}
@@ -1202,16 +1206,19 @@ __device__ __forceinline__ int wave_argmin(float best, int bestP) {
   return __builtin_amdgcn_readfirstlane(bestP);
 }
 
-// One merge on wave 0 (lanes 0..31 own component slots): pair (ci, cj) is
-// merged into slot ci (PAVMM::mergeComponents PAVMM:191-257, the split
-// statistics' mergeComponentStats and SufficientStatistics::mergeComponents
-// WEM:485-518) and the last slot of each state moves into slot cj.  Every
-// lane computes the merged values alike; each slot lane reads all old values
-// before it writes, and LDS operations of one wave complete in order.
-__device__ void merge_step_wave0(Shared& sh, SuffStats& ST, SplitStatsState& SS, int ci, int cj) {
+// One merge of pair (ci, cj) into slot ci, the last slot of each state
+// moving into slot cj, on two waves at once (lanes 0..31 own component
+// slots; every lane computes the merged values alike; each slot lane reads
+// all old values before it writes, and LDS operations of one wave complete in
+// order).  Wave 0: the mixture (PAVMM::mergeComponents PAVMM:191-257), the
+// components' self products and the sufficient statistics
+// (SufficientStatistics::mergeComponents WEM:485-518).  Wave 1: the split
+// statistics (mergeComponentStats), from the pair's values stashed in sh.pre
+// before the mixture changes.
+__device__ void merge_step_mixture(Shared& sh, SuffStats& ST, int ci, int cj) {
   const int lane = threadIdx.x & 63;
   VMMParams& V = sh.S.vmm;
-  const int LV = V.K - 1, LS = SS.K - 1, LT = ST.K - 1;
+  const int LV = V.K - 1, LT = ST.K - 1;
   const MergedLobe m = merged_lobe(V, ci, cj);
   float msp[6];  // product of the merged lobe with itself
   msp[5] = product_scale(m.dx, m.dy, m.dz, m.kappa, m.nrm, m.dx, m.dy, m.dz, m.kappa, m.nrm, msp[0], msp[1], msp[2],
@@ -1220,8 +1227,6 @@ __device__ void merge_step_wave0(Shared& sh, SuffStats& ST, SplitStatsState& SS,
   const float d0 = V.dist[ci], d1 = V.dist[cj];
   float nd = w0 * d0 + w1 * d1;
   nd /= (w0 + w1);
-  const SplitSlot ns = split_merged_slot(SS, ci, cj, w0, V.mux[ci], V.muy[ci], V.muz[ci], w1, V.mux[cj], V.muy[cj],
-                                         V.muz[cj], m.w, m.dx, m.dy, m.dz);
   // merged sufficient statistics of slot ci
   const float tdx = ST.swdx[ci] + ST.swdx[cj], tdy = ST.swdy[ci] + ST.swdy[cj], tdz = ST.swdz[ci] + ST.swdz[cj];
   const float tsw = ST.sw[ci] + ST.sw[cj], tsd = ST.sdw[ci] + ST.sdw[cj];
@@ -1230,10 +1235,6 @@ __device__ void merge_step_wave0(Shared& sh, SuffStats& ST, SplitStatsState& SS,
     // the last slots as they stand once slot ci holds the merge
     const float lw = V.w[LV], lk = V.kappa[LV], lmc = V.mc[LV], ln = V.norm[LV], le = V.em2k[LV];
     const float lx = V.mux[LV], ly = V.muy[LV], lz = V.muz[LV], ld = V.dist[LV];
-    const bool sI = LS == ci;
-    const float schi = sI ? ns.chi : SS.chi[LS], scx = sI ? ns.covx : SS.covx[LS], scy = sI ? ns.covy : SS.covy[LS];
-    const float scz = sI ? ns.covz : SS.covz[LS], ssn = sI ? ns.sn : SS.sn[LS], ssw = sI ? ns.ssw : SS.ssw[LS];
-    const float ssa = sI ? ns.ssa : SS.ssa[LS];
     const bool tI = LT == ci;
     const float udx = tI ? tdx : ST.swdx[LT], udy = tI ? tdy : ST.swdy[LT], udz = tI ? tdz : ST.swdz[LT];
     const float usw = tI ? tsw : ST.sw[LT], usd = tI ? tsd : ST.sdw[LT];
@@ -1250,15 +1251,6 @@ __device__ void merge_step_wave0(Shared& sh, SuffStats& ST, SplitStatsState& SS,
       V.mux[k] = m.dx; V.muy[k] = m.dy; V.muz[k] = m.dz; V.dist[k] = nd;
       for (int a = 0; a < 6; a++) sh.selfp[k][a] = msp[a];
     }
-    if (k == LS) {
-      split_zero_slot(SS, k);
-    } else if (k == cj) {
-      SS.chi[k] = schi; SS.covx[k] = scx; SS.covy[k] = scy; SS.covz[k] = scz; SS.sn[k] = ssn; SS.ssw[k] = ssw;
-      SS.ssa[k] = ssa;
-    } else if (k == ci) {
-      SS.chi[k] = ns.chi; SS.covx[k] = ns.covx; SS.covy[k] = ns.covy; SS.covz[k] = ns.covz; SS.sn[k] = ns.sn;
-      SS.ssw[k] = ns.ssw; SS.ssa[k] = ns.ssa;
-    }
     if (k == LT) {
       suff_zero_slot(ST, k);
     } else if (k == cj) {
@@ -1269,9 +1261,34 @@ __device__ void merge_step_wave0(Shared& sh, SuffStats& ST, SplitStatsState& SS,
   }
   if (lane == 0) {
     V.K = LV;
-    SS.K = LS;
     ST.K = LT;
   }
+}
+
+__device__ void merge_step_split(Shared& sh, SplitStatsState& SS, int ci, int cj) {
+  const int lane = threadIdx.x & 63;
+  const int LS = SS.K - 1;
+  const float* pre = sh.pre;  // w, mu x, y, z of ci; the same of cj; mean cosines of ci, cj
+  const MergedLobe m = merged_lobe_of(pre[0], pre[8], pre[1], pre[2], pre[3], pre[4], pre[9], pre[5], pre[6], pre[7]);
+  const SplitSlot ns = split_merged_slot(SS, ci, cj, pre[0], pre[1], pre[2], pre[3], pre[4], pre[5], pre[6], pre[7],
+                                         m.w, m.dx, m.dy, m.dz);
+  if (lane < 32) {
+    const int k = lane;
+    const bool sI = LS == ci;
+    const float schi = sI ? ns.chi : SS.chi[LS], scx = sI ? ns.covx : SS.covx[LS], scy = sI ? ns.covy : SS.covy[LS];
+    const float scz = sI ? ns.covz : SS.covz[LS], ssn = sI ? ns.sn : SS.sn[LS], ssw = sI ? ns.ssw : SS.ssw[LS];
+    const float ssa = sI ? ns.ssa : SS.ssa[LS];
+    if (k == LS) {
+      split_zero_slot(SS, k);
+    } else if (k == cj) {
+      SS.chi[k] = schi; SS.covx[k] = scx; SS.covy[k] = scy; SS.covz[k] = scz; SS.sn[k] = ssn; SS.ssw[k] = ssw;
+      SS.ssa[k] = ssa;
+    } else if (k == ci) {
+      SS.chi[k] = ns.chi; SS.covx[k] = ns.covx; SS.covy[k] = ns.covy; SS.covz[k] = ns.covz; SS.sn[k] = ns.sn;
+      SS.ssw[k] = ns.ssw; SS.ssa[k] = ns.ssa;
+    }
+  }
+  if (lane == 0) SS.K = LS;
 }
 
 // PerformMerging with ThresholdedMergeNext (Merger:45-89, 218-279)
@@ -1280,12 +1297,12 @@ __device__ void merge_step_wave0(Shared& sh, SuffStats& ST, SplitStatsState& SS,
 // pairs of the merged slot ci and of slot cj (which receives the last
 // component) change: every other pair has unchanged inputs, so the table
 // equals a full recomputation bit for bit.  The merge loop is sequential by
-// definition.  A step's choice and merge run on wave 0 alone, with no
-// workgroup barrier: the arg-min over the table (least cost below the
-// threshold with both components holding split samples, ties to the first
-// pair in (i, j) order -- the serial scan of ThresholdedMergeNext), then the
-// merge itself.  The three cost terms of the changed pairs are computed on
-// waves 0..2, one pair per lane, between two barriers.  Each component's
+// definition.  A step: the arg-min over the table on wave 0 (least cost below
+// the threshold with both components holding split samples, ties to the first
+// pair in (i, j) order -- the serial scan of ThresholdedMergeNext), which
+// also stashes the pair's values; the merge on waves 0 and 1 at once; the
+// three cost terms of the changed pairs on waves 0..2, one pair per lane; the
+// new costs written by wave 0 before its next arg-min.  Each component's
 // product with itself is cached and moves with the component.
 __device__ int perform_merging(Shared& sh, SuffStats& ST, SplitStatsState& SS, const FactoryConfig& cfg) {
   const unsigned long long tMerge = wall_clock64();
@@ -1332,18 +1349,28 @@ __device__ int perform_merging(Shared& sh, SuffStats& ST, SplitStatsState& SS, c
           }
         }
         const int pm = wave_argmin(best, bestP);
-        int more = 0;
-        if (pm != 0x7fffffff) {
-          const int ci = pm >> 5, cj = pm & 31;
-          merge_step_wave0(sh, ST, SS, ci, cj);
-          total++;
-          more = V.K > sh.W;
-          if (lane == 0) { sh.ci = ci; sh.cj = cj; }
+        if (lane == 0) {
+          sh.found = pm != 0x7fffffff;
+          if (sh.found) {
+            const int ci = pm >> 5, cj = pm & 31;
+            sh.ci = ci;
+            sh.cj = cj;
+            sh.pre[0] = V.w[ci]; sh.pre[1] = V.mux[ci]; sh.pre[2] = V.muy[ci]; sh.pre[3] = V.muz[ci];
+            sh.pre[4] = V.w[cj]; sh.pre[5] = V.mux[cj]; sh.pre[6] = V.muy[cj]; sh.pre[7] = V.muz[cj];
+            sh.pre[8] = V.mc[ci]; sh.pre[9] = V.mc[cj];
+          }
         }
-        if (lane == 0) sh.found = more;
       }
       __syncthreads();
       if (!sh.found) break;
+      {
+        const int ci = sh.ci, cj = sh.cj;
+        if (wave == 0) merge_step_mixture(sh, ST, ci, cj);
+        else if (wave == 1) merge_step_split(sh, SS, ci, cj);
+      }
+      __syncthreads();
+      total++;
+      if (V.K <= sh.W) break;
       // pair q of the changed rows: lanes 0..31 (ci, q), lanes 32..63 (cj, q - 32)
       // unless that pair was already listed; wave t computes term t
       const int ci = sh.ci, cj = sh.cj, Kn = V.K;
@@ -1355,9 +1382,6 @@ __device__ int perform_merging(Shared& sh, SuffStats& ST, SplitStatsState& SS, c
       if (wave == 0 && i >= 0) sh.cost[i * 32 + j] = merge_chi(sh.term[lane][0], sh.term[lane][1], sh.term[lane][2], V.w[i] + V.w[j]);
       // the next arg-min runs on wave 0, after these writes of its own
     }
-    if (tid == 0) sh.ci = total;
-    __syncthreads();
-    total = sh.ci;
   }
   if (tid == 0) sh.S.prof[5] += (float)(wall_clock64() - tMerge);
   return total;
