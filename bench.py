@@ -269,6 +269,11 @@ def mixed_workload(frames=3, surf=16_000_000, vol=4_000_000, offered_qps=1.0e8, 
     ring = 1 << 23
     odir, opdf = pinned_empty((ring, 3), np.float32), pinned_empty(ring, np.float32)
     oid, oint = pinned_empty(ring, np.uint32), pinned_empty(ring, np.float32)
+    # warm-up request (arena allocation, first launches), as in the serving-alone run
+    rid, _ = server.submit(pos[:256000], nrm[:256000], u2[:256000], g=g[:256000], is_volume=volf[:256000],
+                           out=(odir[:256000], opdf[:256000], oid[:256000], oint[:256000]))
+    server.wait(rid)
+    server.release(rid)
     rng = np.random.default_rng(3)
     mean_size = (256000 - 1000) / np.log(256.0)
     rate = offered_qps / mean_size

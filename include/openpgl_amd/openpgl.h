@@ -338,7 +338,9 @@ size_t pglFieldGetUpdateHistory(PGLField field, int kind, double* out, size_t ma
  * changes the result (bit-identical for every schedule).
  *   perRegionLaunches  1: one launch of one workgroup per region (the plain
  *                      per-region GPU path); 0: one persistent launch
- *   grid               workgroups of the persistent launch (0: all co-resident)
+ *   grid               workgroups of the persistent launch (0: two per CU,
+ *                      which leaves room for other streams' kernels; at
+ *                      most the workgroups that fit at once, three per CU)
  *   balance            a team gets one workgroup per
  *                      total samples / (grid * balance) of its region
  *   minChunksPerRank   at least this many 1024-sample chunks per workgroup
@@ -381,7 +383,8 @@ void pglFieldGetRegionSchedule(PGLField field, PGLRegionSchedule* schedule);
  *   (optional). The first step fits, later steps update.
  *   pglRegionBatchAutotune re-runs the last step's launch from its input
  *   state for each candidate schedule (NULL = the built-in set from
- *   pglRegionScheduleCandidates, where grid == 1 means one workgroup per CU),
+ *   pglRegionScheduleCandidates, where grid == 1 means one workgroup per CU
+ *   and grid == 2 all that fit at once),
  *   keeps the fastest of reps runs per candidate, selects the fastest
  *   candidate and returns its index (-1 on error); msOut receives the kernel
  *   time of every candidate.  Schedules never change results.

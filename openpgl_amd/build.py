@@ -20,6 +20,8 @@ HIPFLAGS = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "
 SOURCES = [
     ("kernels_spatial.hip", HIPFLAGS),
     ("kernels_region.hip", HIPFLAGS),
+    # the region kernel built a second time for two workgroups per CU (k_region_update_wide)
+    ("kernels_region.hip", HIPFLAGS + ["-DPGL_REGION_OCC=2"], "kernels_region_wide.o"),
     ("kernels_query.hip", HIPFLAGS),
     ("kernels_sort.hip", HIPFLAGS),
     ("kernels_path.hip", HIPFLAGS),
@@ -41,8 +43,8 @@ SOURCES = [
 ]
 
 
-def _compile(src, extra):
-    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+def _compile(src, extra, objName=None):
+    obj = os.path.join(BUILD, objName or os.path.splitext(src)[0] + ".o")
     cmd = [HIPCC] + COMMON + extra + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

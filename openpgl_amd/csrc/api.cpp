@@ -880,14 +880,14 @@ PGL_CATCH_END_VOID
 
 size_t pglRegionScheduleCandidates(PGLRegionSchedule* out, size_t maxCount) {
   // search space of the autotuner: launch kind (ordinary or cooperative),
-  // grid (all co-resident workgroups or one per CU), team balance, chunks per
-  // rank, largest team
+  // grid (the default two per CU, one per CU, or all that fit at once), team
+  // balance, chunks per rank, largest team
   std::vector<PGLRegionSchedule> c;
   for (int coop = 0; coop < 2; coop++)
-    for (uint32_t gridHalf = 0; gridHalf < 2; gridHalf++)
+    for (uint32_t gridSel : {0u, 1u, 2u})
       for (float bal : {0.5f, 1.f, 2.f, 4.f})
         for (uint32_t mc : {1u, 2u, 4u})
-          for (uint32_t mt : {16u, 64u}) c.push_back(PGLRegionSchedule{0, gridHalf ? 1u : 0u, bal, mc, mt, coop});
+          for (uint32_t mt : {16u, 64u}) c.push_back(PGLRegionSchedule{0, gridSel, bal, mc, mt, coop});
   if (out)
     for (size_t i = 0; i < c.size() && i < maxCount; i++) out[i] = c[i];
   return c.size();
@@ -903,7 +903,7 @@ int pglRegionBatchAutotune(PGLRegionBatch batch, const PGLRegionSchedule* candid
     cs.resize(pglRegionScheduleCandidates(nullptr, 0));
     pglRegionScheduleCandidates(cs.data(), cs.size());
   }
-  // grid == 1 in a candidate means "one workgroup per CU"
+  // grid == 1 in a candidate means one workgroup per CU, grid == 2 all that fit at once
   const uint32_t resident = batch->b->runner().residentGrid();
   int cus = 0, dev = 0;
   PGL_HIP_CHECK(hipGetDevice(&dev));
@@ -912,6 +912,7 @@ int pglRegionBatchAutotune(PGLRegionBatch batch, const PGLRegionSchedule* candid
   for (size_t i = 0; i < cs.size(); i++) {
     to_sched(&cs[i], rs[i]);
     if (rs[i].grid == 1) rs[i].grid = std::min<uint32_t>(resident, (uint32_t)cus);
+    else if (rs[i].grid == 2) rs[i].grid = resident;
   }
   std::vector<float> ms;
   const int best = batch->b->autotune(rs, reps, ms, batch->stream);

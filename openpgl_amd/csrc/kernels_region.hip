@@ -48,6 +48,15 @@
 #include "common.h"
 #include "kernels.h"
 
+#ifndef PGL_REGION_OCC
+#define PGL_REGION_OCC 3
+#endif
+#if PGL_REGION_OCC == 3
+#define PGL_REGION_KERNEL k_region_update
+#else
+#define PGL_REGION_KERNEL k_region_update_wide
+#endif
+
 namespace pglgpu {
 
 #define RT 256
@@ -1633,8 +1642,15 @@ __device__ void region_job(Shared& sh, Team& tm, RegionState* __restrict__ state
 // order until the queue is empty; the ranks of a job are queued
 // consecutively, so a team is always completed by the next free workgroups
 // and every wait ends (given T <= grid, which the host guarantees, and a
-// co-resident grid).
-__global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(2, 2))) k_region_update(RegionState* __restrict__ states,
+// grid that fits the device).  This file is compiled twice (each build with
+// its own device functions): PGL_REGION_OCC = 3 fits three workgroups on a CU
+// (168 VGPRs) and serves the default launch of two per CU, which leaves the
+// room of the third to other streams' kernels (the next update's spatial
+// phase); PGL_REGION_OCC = 2 (248 VGPRs, the faster code, k_region_update_wide)
+// serves launches of at most one workgroup per CU, which leave half of every
+// CU to other work (serving next to training).
+__global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(PGL_REGION_OCC, PGL_REGION_OCC)))
+PGL_REGION_KERNEL(RegionState* __restrict__ states,
                                                       const RegionJob* __restrict__ jobs,
                                                       const uint2* __restrict__ items, uint32_t numItems,
                                                       uint32_t* __restrict__ queue, uint32_t* __restrict__ counters,
@@ -1688,6 +1704,10 @@ __global__ void __launch_bounds__(RT) __attribute__((amdgpu_waves_per_eu(2, 2)))
     __syncthreads();
   }
 }
+
+#if PGL_REGION_OCC == 3
+// the PGL_REGION_OCC = 2 build of the kernel (its own translation unit)
+const void* region_kernel_wide();
 
 // clone the directional state of the ancestor into regions created by spatial splits
 __global__ void __launch_bounds__(256) k_copy_states(RegionState* __restrict__ states, const uint32_t* __restrict__ pairs,
@@ -1807,14 +1827,20 @@ void launch_region_update(const RegionLaunch& L, const FactoryConfig& cfg, hipSt
   float* costHint = L.costHint;
   const float* uniformDirs = L.uniformDirs;
   FactoryConfig c = cfg;
-  if (L.cooperative) {
-    void* args[] = {&states, &jobs,    &items, &numItems, &queue, &counters,   &partial,
-                    &err,    &timeout, &stats, &costHint, &rs,    (void*)&c, &uniformDirs};
-    PGL_KERNEL_CHECK(hipLaunchCooperativeKernel((const void*)k_region_update, dim3(L.grid), dim3(RT), args, 0, s));
-  } else {
-    hipLaunchKernelGGL(k_region_update, dim3(L.grid), dim3(RT), 0, s, states, jobs, items, numItems, queue, counters,
-                       partial, err, timeout, stats, costHint, rs, c, uniformDirs);
-  }
+  // at most one workgroup per CU: the variant compiled for two per CU
+  int cus = 0, dev = 0;
+  PGL_KERNEL_CHECK(hipGetDevice(&dev));
+  PGL_KERNEL_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const bool wide = L.grid <= (uint32_t)cus;
+  void* args[] = {&states, &jobs,    &items, &numItems, &queue, &counters,   &partial,
+                  &err,    &timeout, &stats, &costHint, &rs,    (void*)&c, &uniformDirs};
+  const void* fn = wide ? region_kernel_wide() : (const void*)k_region_update;
+  if (L.cooperative) PGL_KERNEL_CHECK(hipLaunchCooperativeKernel(fn, dim3(L.grid), dim3(RT), args, 0, s));
+  else PGL_KERNEL_CHECK(hipLaunchKernel(fn, dim3(L.grid), dim3(RT), args, 0, s));
 }
+
+#else
+const void* region_kernel_wide() { return (const void*)k_region_update_wide; }
+#endif
 
 }  // namespace pglgpu

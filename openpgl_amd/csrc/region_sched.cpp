@@ -46,6 +46,7 @@ RegionRunner::RegionRunner() {
   const int perCU = region_kernel_blocks_per_cu();
   if (perCU <= 0 || cus <= 0) throw std::runtime_error("region kernel cannot be resident on this device");
   resident_ = (uint32_t)(perCU * cus);
+  defaultGrid_ = (uint32_t)(std::min(perCU, 2) * cus);
   cooperative_ = coop != 0;
   PGL_SCHED_CHECK(hipMalloc((void**)&dErr_, sizeof(int)));
   PGL_SCHED_CHECK(hipHostMalloc((void**)&hErr_, sizeof(int), hipHostMallocDefault));
@@ -91,7 +92,7 @@ void RegionRunner::launch(RegionState* states, std::vector<RegionWork> work, con
   std::stable_sort(work.begin(), work.end(), [](const RegionWork& a, const RegionWork& b) { return a.cost > b.cost; });
   const uint32_t J = (uint32_t)work.size();
   const bool perRegion = schedule.perRegionLaunches != 0;
-  const uint32_t G = perRegion ? 1u : (schedule.grid ? std::min(schedule.grid, resident_) : resident_);
+  const uint32_t G = perRegion ? 1u : (schedule.grid ? std::min(schedule.grid, resident_) : defaultGrid_);
   double total = 0.0;
   for (const auto& w : work) total += std::max(w.cost, 0.f);
   const double bal = schedule.balance > 0.f ? schedule.balance : 1.f;

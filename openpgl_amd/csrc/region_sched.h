@@ -16,7 +16,7 @@ struct DevBuf;
 // chunking and the fold order of the region kernel do not depend on them.
 struct RegionSchedule {
   int perRegionLaunches = 0;  // 1: one launch of one workgroup per region (plain per-region path)
-  uint32_t grid = 0;          // workgroups of the persistent launch (0: all that are co-resident)
+  uint32_t grid = 0;          // workgroups of the persistent launch (0: two per CU; at most all that fit at once)
   float balance = 1.f;        // team target = total samples / (grid * balance), at least minChunksPerRank chunks
   uint32_t minChunksPerRank = 1;
   uint32_t maxTeam = 64;
@@ -75,7 +75,8 @@ class RegionRunner {
   uint32_t lastLaunches = 0, lastJobs = 0, lastItems = 0, lastMaxTeam = 0, lastGrid = 0;
 
  private:
-  uint32_t resident_ = 0;
+  uint32_t resident_ = 0;     // workgroups that fit on the device at once
+  uint32_t defaultGrid_ = 0;  // two per CU (one more fits per CU, for other streams' kernels)
   bool cooperative_ = false;
   unsigned long long timeoutTicks_ = 0;
   void* dStage_ = nullptr;  // jobs | items | inputs of the current launch
