@@ -53,8 +53,15 @@
 #endif
 #if PGL_REGION_OCC == 3
 #define PGL_REGION_KERNEL k_region_update
+// The split statistics pass uses all 168 VGPRs; called as a function it
+// saves the 33 callee-saved registers among them to scratch per lane on every
+// call, inlined into the update path it spills nothing.  (The EM loop, which
+// saves 47, runs slower inlined and stays a call.)  The fit path (first update
+// of a region) keeps its calls.
+#define PGL_REGION_CALL __forceinline__
 #else
 #define PGL_REGION_KERNEL k_region_update_wide
+#define PGL_REGION_CALL
 #endif
 
 namespace pglgpu {
@@ -668,8 +675,8 @@ __device__ __forceinline__ float fin(const Shared& sh, int k, int f) { return sh
 
 // ------------------------------------------------------ region-level steps
 // UpdateSplitStatistics into SS (S.split or the local recursive-split copy).
-__device__ void update_split_stats(Shared& sh, Team& tm, SplitStatsState& SS, const RegionSamples& rs, uint32_t base,
-                                   uint32_t n, float mcEst) {
+__device__ PGL_REGION_CALL void update_split_stats(Shared& sh, Team& tm, SplitStatsState& SS,
+                                                   const RegionSamples& rs, uint32_t base, uint32_t n, float mcEst) {
   const int tid = threadIdx.x;
   const VMMParams& V = sh.S.vmm;
   if (tid < 32) {
@@ -732,8 +739,8 @@ __device__ void distances(Shared& sh, Team& tm, SuffStats& ST, const RegionSampl
 
 // EM loop (updateMixture / partialUpdateMixture). prev = region statistics or
 // the partial-refit temporary; masked = partial fit over sh.mask.
-__device__ int em(Shared& sh, Team& tm, SuffStats& prev, const RegionSamples& rs, uint32_t base, uint32_t n,
-                  const FactoryConfig& cfg, bool masked) {
+__device__ int em(Shared& sh, Team& tm, SuffStats& prev, const RegionSamples& rs, uint32_t base,
+                                  uint32_t n, const FactoryConfig& cfg, bool masked) {
   const int tid = threadIdx.x;
   VMMParams& V = sh.S.vmm;
   SuffStats& cur = sh.cur;
@@ -1426,8 +1433,8 @@ __device__ bool region_valid(Shared& sh, bool checkSuff) {
 }
 
 // ---- ASM::fit (ASM:303-347)
-__device__ void region_fit(Shared& sh, Team& tm, const RegionSamples& rs, uint32_t base, uint32_t n,
-                           const FactoryConfig& cfg, const float* __restrict__ uniformDirs) {
+__device__ __noinline__ void region_fit(Shared& sh, Team& tm, const RegionSamples& rs, uint32_t base, uint32_t n,
+                                        const FactoryConfig& cfg, const float* __restrict__ uniformDirs) {
   const int tid = threadIdx.x;
   VMMParams& V = sh.S.vmm;
   const int K0 = sh.S.initK > 0 ? min(sh.S.initK, kMaxComponents) : cfg.initK;
