@@ -354,7 +354,7 @@ void KindField::kdUpdate(uint32_t n, hipStream_t s) {
   char* const kd = dKdUp_.as<char>();
   dStatPartial_.ensure(spatial_partial_bytes(n));
   launch_descend_stats(dPx_.as<float>(), dPy_.as<float>(), dPz_.as<float>(), n, (const KDNode*)kd,
-                       (const int32_t*)(kd + offSlot), kd + offPar, L, dNodeOf_.as<uint32_t>(),
+                       (uint32_t)nodes_.size(), (const int32_t*)(kd + offSlot), kd + offPar, L, dNodeOf_.as<uint32_t>(),
                        dStatPartial_.as<IntStatsAcc>(), dAcc_.as<IntStatsAcc>(), s);
   {
     kdDown_.ensure(std::max<size_t>(L, 1) * sizeof(IntStatsAcc));

@@ -39,8 +39,8 @@ uint32_t launch_bounds(const float* px, const float* py, const float* pz, uint32
 // partial: scratch of spatial_partial_bytes(n) bytes for the per-block tables.
 size_t spatial_partial_bytes(uint32_t n);
 void launch_descend_stats(const float* px, const float* py, const float* pz, uint32_t n, const KDNode* nodes,
-                          const int32_t* leafSlot, const void* leafParams, uint32_t numLeaves, uint32_t* nodeOf,
-                          IntStatsAcc* partial, IntStatsAcc* acc, hipStream_t s);
+                          uint32_t numNodes, const int32_t* leafSlot, const void* leafParams, uint32_t numLeaves,
+                          uint32_t* nodeOf, IntStatsAcc* partial, IntStatsAcc* acc, hipStream_t s);
 void launch_kd_round(const float* px, const float* py, const float* pz, uint32_t n, uint32_t* nodeOf,
                      const int32_t* frontierSlot, const FrontierNode* frontier, uint32_t numFrontier,
                      IntStatsAcc* partial, IntStatsAcc* childAcc, hipStream_t s);

@@ -171,6 +171,7 @@ struct LeafParams {
 };
 
 static const int kSlotTile = 448;  // LDS accumulator slots per pass (56 KiB)
+static const uint32_t kLdsNodes = 2048;  // trees up to this many nodes are walked from LDS (16 KiB)
 
 // Statistics pass over all samples with a block-private accumulator table for
 // slots [slotBase, slotBase + kSlotTile); each block writes its table to
@@ -178,7 +179,9 @@ static const int kSlotTile = 448;  // LDS accumulator slots per pass (56 KiB)
 // atomics: device-scope atomics serialise across the XCDs).
 //   kDescend: every sample walks the existing inner nodes to its leaf
 //             (KDTreeBuilder.h:292-349 for the levels that already exist);
-//             statistics of the leaf w.r.t. its parent's bounds.
+//             statistics of the leaf w.r.t. its parent's bounds.  A tree of
+//             at most kLdsNodes nodes is staged in LDS first (each level of
+//             the walk is a dependent load).
 //   else:     one level of the push-down below the frontier nodes: left iff
 //             p[dim] < pivot; child statistics w.r.t. the frontier node's bounds.
 template <bool kDescend>
@@ -189,8 +192,13 @@ __global__ void __launch_bounds__(256) k_spatial_stats(const float* __restrict__
                                                        const LeafParams* __restrict__ leafParams,
                                                        const FrontierNode* __restrict__ frontier,
                                                        uint32_t* __restrict__ nodeOf, int writeNodes, int slotBase,
-                                                       int numSlots, IntStatsAcc* __restrict__ partial) {
+                                                       int numSlots, IntStatsAcc* __restrict__ partial,
+                                                       uint32_t numNodes) {
   __shared__ IntStatsAcc acc[kSlotTile];
+  __shared__ KDNode tree[kDescend ? kLdsNodes : 1];
+  const bool inLds = kDescend && numNodes <= kLdsNodes;
+  if (inLds)
+    for (uint32_t c = threadIdx.x; c < numNodes; c += blockDim.x) tree[c] = nodes[c];
   for (int c = threadIdx.x; c < numSlots; c += blockDim.x) acc_init(&acc[c]);
   __syncthreads();
   const uint32_t stride = gridDim.x * blockDim.x;
@@ -204,12 +212,22 @@ __global__ void __launch_bounds__(256) k_spatial_stats(const float* __restrict__
       float c[3], inv[3];
       if (kDescend) {
         uint32_t nd = 0;
-        KDNode k = nodes[0];
-        while (!k.isLeaf()) {
-          const uint32_t d = k.dim();
-          const float pd = d == 0 ? p[0] : (d == 1 ? p[1] : p[2]);
-          nd = k.idx() + ((pd < k.pivot) ? 0u : 1u);
-          k = nodes[nd];
+        if (inLds) {
+          KDNode k = tree[0];
+          while (!k.isLeaf()) {
+            const uint32_t d = k.dim();
+            const float pd = d == 0 ? p[0] : (d == 1 ? p[1] : p[2]);
+            nd = k.idx() + ((pd < k.pivot) ? 0u : 1u);
+            k = tree[nd];
+          }
+        } else {
+          KDNode k = nodes[0];
+          while (!k.isLeaf()) {
+            const uint32_t d = k.dim();
+            const float pd = d == 0 ? p[0] : (d == 1 ? p[1] : p[2]);
+            nd = k.idx() + ((pd < k.pivot) ? 0u : 1u);
+            k = nodes[nd];
+          }
         }
         if (writeNodes) nodeOf[i] = nd;
         slot = slotOfNode[nd];
@@ -390,9 +408,9 @@ static uint32_t stats_blocks(uint32_t n) { return grid_for(n, 256 * 4, 512); }
 size_t spatial_partial_bytes(uint32_t n) { return (size_t)stats_blocks(n) * kSlotTile * sizeof(IntStatsAcc); }
 
 static void run_stats_passes(bool descend, const float* px, const float* py, const float* pz, uint32_t n,
-                             const KDNode* nodes, const int32_t* slotOfNode, const void* leafParams,
-                             const FrontierNode* frontier, uint32_t* nodeOf, uint32_t numSlots, IntStatsAcc* partial,
-                             IntStatsAcc* out, hipStream_t s) {
+                             const KDNode* nodes, uint32_t numNodes, const int32_t* slotOfNode,
+                             const void* leafParams, const FrontierNode* frontier, uint32_t* nodeOf,
+                             uint32_t numSlots, IntStatsAcc* partial, IntStatsAcc* out, hipStream_t s) {
   const uint32_t blocks = stats_blocks(n);
   for (uint32_t base = 0; base < numSlots || base == 0; base += kSlotTile) {
     const int tile = (int)std::min<uint32_t>(kSlotTile, numSlots - base);
@@ -400,10 +418,12 @@ static void run_stats_passes(bool descend, const float* px, const float* py, con
     const int writeNodes = base + kSlotTile >= numSlots ? 1 : 0;
     if (descend)
       hipLaunchKernelGGL(k_spatial_stats<true>, dim3(blocks), dim3(256), 0, s, px, py, pz, n, nodes, slotOfNode,
-                         (const LeafParams*)leafParams, frontier, nodeOf, writeNodes, (int)base, tile, partial);
+                         (const LeafParams*)leafParams, frontier, nodeOf, writeNodes, (int)base, tile, partial,
+                         numNodes);
     else
       hipLaunchKernelGGL(k_spatial_stats<false>, dim3(blocks), dim3(256), 0, s, px, py, pz, n, nodes, slotOfNode,
-                         (const LeafParams*)leafParams, frontier, nodeOf, writeNodes, (int)base, tile, partial);
+                         (const LeafParams*)leafParams, frontier, nodeOf, writeNodes, (int)base, tile, partial,
+                         0u);
     if (tile > 0)
       hipLaunchKernelGGL(k_reduce_partials, dim3(tile), dim3(256), 0, s, partial, blocks, tile, out + base);
     if (numSlots == 0) break;
@@ -411,16 +431,17 @@ static void run_stats_passes(bool descend, const float* px, const float* py, con
 }
 
 void launch_descend_stats(const float* px, const float* py, const float* pz, uint32_t n, const KDNode* nodes,
-                          const int32_t* leafSlot, const void* params, uint32_t numLeaves, uint32_t* nodeOf,
-                          IntStatsAcc* partial, IntStatsAcc* acc, hipStream_t s) {
-  run_stats_passes(true, px, py, pz, n, nodes, leafSlot, params, nullptr, nodeOf, numLeaves, partial, acc, s);
+                          uint32_t numNodes, const int32_t* leafSlot, const void* params, uint32_t numLeaves,
+                          uint32_t* nodeOf, IntStatsAcc* partial, IntStatsAcc* acc, hipStream_t s) {
+  run_stats_passes(true, px, py, pz, n, nodes, numNodes, leafSlot, params, nullptr, nodeOf, numLeaves, partial, acc,
+                   s);
 }
 
 void launch_kd_round(const float* px, const float* py, const float* pz, uint32_t n, uint32_t* nodeOf,
                      const int32_t* frontierSlot, const FrontierNode* frontier, uint32_t numFrontier,
                      IntStatsAcc* partial, IntStatsAcc* childAcc, hipStream_t s) {
-  run_stats_passes(false, px, py, pz, n, nullptr, frontierSlot, nullptr, frontier, nodeOf, 2 * numFrontier, partial,
-                   childAcc, s);
+  run_stats_passes(false, px, py, pz, n, nullptr, 0, frontierSlot, nullptr, frontier, nodeOf, 2 * numFrontier,
+                   partial, childAcc, s);
 }
 
 void launch_leaf_keys(const uint32_t* nodeOf, const KDNode* nodes, uint32_t n, uint32_t* keys, uint32_t* vals,
