@@ -515,8 +515,8 @@ def main():
                     help="only run a 1 GiB device copy (rocprofv3 --pmc calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md for how it would "
-              "shard); run with --gpus 1", file=sys.stderr)
+        print("bench.py runs on one GPU only: the library has no multi-GPU path; run with --gpus 1",
+              file=sys.stderr)
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy()

@@ -436,8 +436,8 @@ template <class R> struct Factory {
   }
 
   R mcToKappa(R r) const {
-    // FP64 inversion of the mean cosine (north-star precision rule for the
-    // product); the oracle keeps the reference's FP32 formula in float mode.
+    // FP64 inversion of the mean cosine (the GPU path inverts in FP64 too);
+    // the oracle keeps the reference's FP32 formula in float mode.
     return MeanCosineToKappa<R>(r);
   }
 
