@@ -61,6 +61,18 @@ def test_kdtree_bit_exact(frames):
     assert np.array_equal(gpu.knn(), orc.knn())
 
 
+def test_kdtree_bit_exact_large_tree():
+    """A tree of more than 2048 nodes: the descending statistics pass walks
+    it from global memory instead of LDS; the second frame descends it."""
+    dev, gpu, orc = _fields(64, 80000, frames=2, seed=3, knn=False)
+    gp, _, _ = gpu.nodes()
+    print("nodes", len(gp), flush=True)
+    assert len(gp) > 2048
+    _tree_equal(gpu, orc)
+    s = generate(5000, seed=11, frame=2)
+    assert np.array_equal(gpu.leaf_ids(s["pos"]), orc.leaf_ids(s["pos"]))
+
+
 def _mixture_errors(gpu, orc, dirs):
     errs = []
     for i in range(gpu.num_regions()):
