@@ -141,6 +141,7 @@ _SIGS = {
     "pglVolumeSamplingDistributionApplySingleLobeHenyeyGreensteinProduct": (None, [_vp, Vec3, _f]),
     "pglVolumeSamplingDistributionSample": (Vec3, [_vp, Vec2]),
     "pglVolumeSamplingDistributionPDF": (_f, [_vp, Vec3]),
+    "pglVolumeSamplingDistributionIncomingRadiancePDF": (_f, [_vp, Vec3]),
     "pglVolumeSamplingDistributionSamplePDF": (_f, [_vp, Vec2, ctypes.POINTER(Vec3)]),
     "pglVolumeSamplingDistributionGetId": (_u32, [_vp]),
     "pglNewSampleStorage": (_vp, []),
@@ -665,6 +666,9 @@ class SurfaceSamplingDistribution(_Dist):
 
 class VolumeSamplingDistribution(_Dist):
     _kind = "Volume"
+
+    def incoming_radiance_pdf(self, d):
+        return self.lib.pglVolumeSamplingDistributionIncomingRadiancePDF(self.h, Vec3(*[float(x) for x in d]))
 
     def apply_hg_product(self, d, g):
         self.lib.pglVolumeSamplingDistributionApplySingleLobeHenyeyGreensteinProduct(

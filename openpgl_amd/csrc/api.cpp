@@ -83,11 +83,20 @@ static bool dist_init(HostDist* d, const KindField& f, hipStream_t s, const floa
   return true;
 }
 
-static float dist_product(HostDist* d, int m, float weight, float nx, float ny, float nz, float kn, float nn) {
+// PAVMM::product (ParallaxAwareVonMisesFisherMixture.h:609-671) of mixture m
+// with one lobe: of the incident-radiance mixture copied into slot m
+// (fromLi, the volume distribution's HG lobes, VMMVolumeSamplingDistribution.h:64)
+// or of slot m itself in place (the surface distribution's cosine product,
+// VMMSurfaceSamplingDistribution.h:52-60, so repeated products compound).
+static float dist_product(HostDist* d, int m, bool fromLi, float weight, float nx, float ny, float nz, float kn,
+                          float nn) {
   float integral = 0.f;
   for (int k = 0; k < d->K; k++) {
-    const ProdComp p = product_comp(d->lw[k], d->lkappa[k], d->lmx[k], d->lmy[k], d->lmz[k], d->lnorm[k], weight, nx,
-                                    ny, nz, kn, nn);
+    const ProdComp p =
+        fromLi ? product_comp(d->lw[k], d->lkappa[k], d->lmx[k], d->lmy[k], d->lmz[k], d->lnorm[k], weight, nx, ny, nz,
+                              kn, nn)
+               : product_comp(d->w[m][k], d->kappa[m][k], d->mx[m][k], d->my[m][k], d->mz[m][k], d->norm[m][k],
+                              weight, nx, ny, nz, kn, nn);
     d->w[m][k] = p.scale; d->kappa[m][k] = p.kappa; d->mx[m][k] = p.mx; d->my[m][k] = p.my; d->mz[m][k] = p.mz;
     d->norm[m][k] = p.norm; d->em2k[m][k] = p.em2k;
     integral += p.scale;
@@ -136,6 +145,7 @@ static float dist_pdf_li(const HostDist* d, const float dir[3]) {
 
 static void field_update_host(PGLFieldImpl* F, KindField& kf, const std::vector<PGLSampleData>& samples) {
   if (samples.empty()) return;
+  if (samples.size() > kMaxUpdateSamples) throw std::runtime_error("openpgl_amd: at most 2^30 samples per update");
   kf.updateHost(samples.data(), (uint32_t)samples.size(), F->gf->stream);
 }
 
@@ -346,7 +356,7 @@ PGL_CATCH_END_VOID
 
 void pglFieldUpdateSurfaceDevice(PGLField field, const PGLSampleData* samples, size_t n) PGL_CATCH_BEGIN {
   THROW_IF_NULL(field, "handle");
-  if (n > 0xffffffffu) throw std::runtime_error("openpgl_amd: at most 2^32-1 samples per update");
+  if (n > kMaxUpdateSamples) throw std::runtime_error("openpgl_amd: at most 2^30 samples per update");
   field->gf->surface.update(samples, (uint32_t)n, field->gf->stream);
   field->gf->iteration++;
 }
@@ -354,7 +364,7 @@ PGL_CATCH_END_VOID
 
 void pglFieldUpdateVolumeDevice(PGLField field, const PGLSampleData* samples, size_t n) PGL_CATCH_BEGIN {
   THROW_IF_NULL(field, "handle");
-  if (n > 0xffffffffu) throw std::runtime_error("openpgl_amd: at most 2^32-1 samples per update");
+  if (n > kMaxUpdateSamples) throw std::runtime_error("openpgl_amd: at most 2^30 samples per update");
   field->gf->volume.update(samples, (uint32_t)n, field->gf->stream);
   field->gf->iteration++;
 }
@@ -555,7 +565,7 @@ bool pglSampleStorageCompare(PGLSampleStorage a, PGLSampleStorage b) {
 // ---- surface distribution
 void pglReleaseSurfaceSamplingDistribution(PGLSurfaceSamplingDistribution d) { delete d; }
 void pglSurfaceSamplingDistributionApplyCosineProduct(PGLSurfaceSamplingDistribution d, pgl_vec3f n) {
-  if (d->num > 0) d->integral = dist_product(d, 0, 1.f, n.x, n.y, n.z, kCosineKappa, cosine_normalization());
+  if (d->num > 0) d->integral = dist_product(d, 0, false, 1.f, n.x, n.y, n.z, kCosineKappa, cosine_normalization());
 }
 bool pglSurfaceSamplingDistributionSupportsApplyCosineProduct(PGLSurfaceSamplingDistribution) { return true; }
 pgl_vec3f pglSurfaceSamplingDistributionSample(PGLSurfaceSamplingDistribution d, pgl_point2f u) {
@@ -614,7 +624,7 @@ void pglVolumeSamplingDistributionApplySingleLobeHenyeyGreensteinProduct(PGLVolu
   const int KP = (int)rep.K;
   for (int i = 0; i < KP; i++) {
     const float sg = meanCosine * rep.mc[i] > 0.f ? 1.f : -1.f;
-    d->mw[i] = dist_product(d, i, rep.w[i], dir.x * sg, dir.y * sg, dir.z * sg, rep.kappa[i], rep.norm[i]);
+    d->mw[i] = dist_product(d, i, true, rep.w[i], dir.x * sg, dir.y * sg, dir.z * sg, rep.kappa[i], rep.norm[i]);
     sum += d->mw[i];
   }
   for (int i = 0; i < KP; i++) d->mw[i] /= sum;
@@ -833,7 +843,7 @@ PGLRegionBatch pglNewRegionBatch(PGLDevice device, PGLFieldArguments args) PGL_C
   delete v;
   PGLRegionBatchImpl* B = new PGLRegionBatchImpl();
   PGL_HIP_CHECK(hipStreamCreate(&B->stream));
-  B->b = new RegionBatch(fc);
+  B->b = new RegionBatch(fc, args.deterministic);
   return B;
 }
 PGL_CATCH_END(nullptr)

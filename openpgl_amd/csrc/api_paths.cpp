@@ -207,7 +207,7 @@ bool pglPathSegmentBatchPrepareSamplesDevice(PGLPathSegmentBatch batch, const PG
     THROW_IF_NULL(segments, "segments");
     THROW_IF_NULL(pathOffsets, "path offsets");
   }
-  if (numSegments > 0xffffffffu) throw std::runtime_error("openpgl_amd: at most 2^32-1 path vertices per batch");
+  if (numSegments > kMaxUpdateSamples) throw std::runtime_error("openpgl_amd: at most 2^30 path vertices per batch");
   const PathSampleOptions o{maxDistance, useNEEMiWeights, guideDirectLight, rrAffectsDirectContribution};
   size_t ns = 0, nv = 0;
   ((PathSampleBatch*)batch)
@@ -230,14 +230,15 @@ void pglFieldUpdateFromPathSegmentsDevice(PGLField field, const PGLPathSegmentDa
     THROW_IF_NULL(segments, "segments");
     THROW_IF_NULL(pathOffsets, "path offsets");
   }
-  if (numSegments > 0xffffffffu) throw std::runtime_error("openpgl_amd: at most 2^32-1 path vertices per update");
+  if (numSegments > kMaxUpdateSamples) throw std::runtime_error("openpgl_amd: at most 2^30 path vertices per update");
   if (!field->paths) field->paths = new PathSampleBatch();
   const PathSampleOptions o{maxDistance, useNEEMiWeights, guideDirectLight, rrAffectsDirectContribution};
   const PathSample *surf = nullptr, *vol = nullptr;
   size_t ns = 0, nv = 0;
   field->paths->prepareOwned((const PathVertex*)segments, (uint32_t)numSegments, pathOffsets, numPaths, o, surf, ns,
                              vol, nv);
-  if (ns > 0xffffffffu || nv > 0xffffffffu) throw std::runtime_error("openpgl_amd: at most 2^32-1 samples per update");
+  if (ns > kMaxUpdateSamples || nv > kMaxUpdateSamples)
+    throw std::runtime_error("openpgl_amd: at most 2^30 samples per update");
   GuidingField& gf = *field->gf;
   hipEvent_t evS = nullptr, evV = nullptr;
   if (ns) {

@@ -20,6 +20,9 @@ static const int kMaxComponents = 32;   // PGL_VMM_MAX_COMPONENTS (include/openp
 static const uint32_t kChunk = 1024;
 static const int kPartStride = 256;
 static const size_t kSampleRecordBytes = 40;  // PGLSampleData (include/openpgl/data.h:21-49)
+// samples of one update: sample indices, sort keys and the region kernel's
+// sample offsets are 32-bit; 2^30 keeps every grid-stride index below 2^31
+static const uint32_t kMaxUpdateSamples = 1u << 30;
 static const int kMergeWidth = 8;       // merge loop floor = SIMD width of the CPU_8 device
 static const float kOneOverFourPi = 0.07957747154594767f;
 static const float kMaxKappaSplit = 32000.0f;  // OPENPGL_MAX_KAPPA used by the splitter

@@ -64,10 +64,36 @@ void HostSampleStats::variance(float v[3]) const {
   for (int i = 0; i < 3; i++) v[i] = var[i] / n;
 }
 
-__attribute__((target("avx2,fma"))) float host_rcp(float x) {
+// embree::rcp (common/math/emath.h:61-88) as each reference device type
+// compiles it (openpgl/CMakeLists.txt:125-131): CPU_4 (SSE4.1) rcpss and a
+// Newton step without FMA, CPU_8 (AVX2) rcpss and a fused Newton step,
+// CPU_16 (AVX-512VL, AVX2) rcp14 and a fused Newton step.
+__attribute__((target("sse4.1"))) static float rcp_sse4(float x) {
+  const __m128 a = _mm_set_ss(x);
+  const __m128 r = _mm_rcp_ss(a);
+  return _mm_cvtss_f32(_mm_mul_ss(r, _mm_sub_ss(_mm_set_ss(2.0f), _mm_mul_ss(r, a))));
+}
+__attribute__((target("avx2,fma"))) static float rcp_avx2(float x) {
   const __m128 a = _mm_set_ss(x);
   const __m128 r = _mm_rcp_ss(a);
   return _mm_cvtss_f32(_mm_mul_ss(r, _mm_fnmadd_ss(r, a, _mm_set_ss(2.0f))));
+}
+__attribute__((target("avx512f,avx512vl,avx2,fma"))) static float rcp_avx512(float x) {
+  const __m128 a = _mm_set_ss(x);
+  const __m128 r = _mm_rcp14_ss(_mm_set_ss(0.0f), a);
+  return _mm_cvtss_f32(_mm_mul_ss(r, _mm_fnmadd_ss(r, a, _mm_set_ss(2.0f))));
+}
+
+bool host_supports_width(int vectorWidth) {
+  if (vectorWidth == 16) return __builtin_cpu_supports("avx512f") && __builtin_cpu_supports("avx512vl");
+  if (vectorWidth == 8) return __builtin_cpu_supports("avx2") && __builtin_cpu_supports("fma");
+  return __builtin_cpu_supports("sse4.1");
+}
+
+float host_rcp(float x, int vectorWidth) {
+  if (vectorWidth == 4) return rcp_sse4(x);
+  if (vectorWidth == 16) return rcp_avx512(x);
+  return rcp_avx2(x);
 }
 
 // IntegerSampleStatistics::getSampleStatistics (SampleStatistics.h:297-340)
@@ -98,11 +124,12 @@ HostSampleStats int_stats_to_float(const IntStatsAcc& a, const float center[3], 
   return s;
 }
 
-static void bounds_params(const float lo[3], const float hi[3], float center[3], float half[3], float inv[3]) {
+static void bounds_params(const float lo[3], const float hi[3], float center[3], float half[3], float inv[3],
+                          int vectorWidth) {
   for (int i = 0; i < 3; i++) {
     const float ext = hi[i] - lo[i];
     half[i] = ext * 0.5f;
-    inv[i] = host_rcp(half[i]);
+    inv[i] = host_rcp(half[i], vectorWidth);
     center[i] = lo[i] + half[i];
   }
 }
@@ -153,6 +180,10 @@ void uniform_directions(std::vector<float>& out) {  // WEM:668-700 spherical Fib
 
 KindField::KindField(const KdSettings& kd, const FactoryConfig& fc, bool deterministic, int vectorWidth)
     : kd_(kd), fc_(fc), deterministic_(deterministic), vectorWidth_(vectorWidth) {
+  if (!host_supports_width(vectorWidth))
+    throw std::runtime_error("openpgl_amd: the host CPU lacks the instruction set of a SIMD-width-" +
+                             std::to_string(vectorWidth) +
+                             " device (its kd-tree reciprocals follow that device's arithmetic)");
   for (int i = 0; i < 3; i++) { sceneLo_[i] = sceneHi_[i] = treeLo_[i] = treeHi_[i] = 0.f; }
   for (int p = 0; p < 2; p++) {
     for (auto& e : evSp_[p]) PGL_HIP_CHECK(hipEventCreate(&e));
@@ -314,7 +345,7 @@ void KindField::kdUpdate(uint32_t n, hipStream_t s) {
         li.node = it.node;
         li.depth = it.depth;
         for (int i = 0; i < 3; i++) { li.lo[i] = it.lo[i]; li.hi[i] = it.hi[i]; }
-        bounds_params(it.plo, it.phi, li.center, li.half, li.invHalf);
+        bounds_params(it.plo, it.phi, li.center, li.half, li.invHalf, vectorWidth_);
         leafSlot[it.node] = (int32_t)leaves.size();
         leaves.push_back(li);
         continue;
@@ -384,7 +415,7 @@ void KindField::kdUpdate(uint32_t n, hipStream_t s) {
       fn[f].left = nd.idx();
       fn[f].dim = nd.dim();
       fn[f].pivot = nd.pivot;
-      bounds_params(it.lo, it.hi, fn[f].center, &halves[3 * f], fn[f].invHalf);
+      bounds_params(it.lo, it.hi, fn[f].center, &halves[3 * f], fn[f].invHalf, vectorWidth_);
       slot[it.node] = (int32_t)f;
     }
     dAcc_.ensure(2 * F * sizeof(IntStatsAcc));
@@ -514,9 +545,11 @@ void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
   const int P = parity_;
   UpdateInfo inf;
   inf.samples = n;
-  dPx_.ensure(n * 4); dPy_.ensure(n * 4); dPz_.ensure(n * 4); dNodeOf_.ensure(n * 4);
-  dKeys_.ensure(n * 4); dVals_.ensure(n * 4); dKeys2_.ensure(n * 4); dVals2_.ensure(n * 4);
-  for (int j = 0; j < 6; j++) dSoa_[P][j].ensure(n * 4);
+  if (n > kMaxUpdateSamples) throw std::runtime_error("openpgl_amd: at most 2^30 samples per update");
+  const size_t n4 = size_t(n) * 4;
+  dPx_.ensure(n4); dPy_.ensure(n4); dPz_.ensure(n4); dNodeOf_.ensure(n4);
+  dKeys_.ensure(n4); dVals_.ensure(n4); dKeys2_.ensure(n4); dVals2_.ensure(n4);
+  for (int j = 0; j < 6; j++) dSoa_[P][j].ensure(n4);
   const size_t sortBytes = sort_pairs_temp_bytes(n);
   dSortTmp_.ensure(sortBytes);
   hipStream_t sp = sp_;
@@ -592,8 +625,17 @@ void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
   char* const tab = dTables_[P].as<char>();
   knnPivots_ = (const float4*)(tab + offKnn);
 
-  // region-contiguous order: stable sort of (region, sample index)
-  if (R <= kSortMaxKeys) {
+  // region-contiguous order: in deterministic mode every region's samples in
+  // SampleDataLess order (Field.h:409-412, 469-472), else a stable sort of
+  // (region, sample index)
+  if (deterministic_) {
+    dDetTmp_.ensure(det_sort_temp_bytes(n));
+    dDetCounter_.ensure(256);
+    detCounter_.ensure(256);
+    lastDetPasses_ = (uint32_t)det_sort_leaves(dDetTmp_.p, deviceSamples, dNodeOf_.as<uint32_t>(), (const KDNode*)tab,
+                                               R, n, dKeys2_.as<uint32_t>(), dVals2_.as<uint32_t>(),
+                                               dDetCounter_.as<uint32_t>(), detCounter_.as<uint32_t>(), sp);
+  } else if (R <= kSortMaxKeys) {
     // stable counting sort keyed by the region of each sample's leaf: every
     // region's first position is known from the kd counts
     dCountSortTmp_.ensure(counting_sort_temp_bytes(n, R));
@@ -771,7 +813,7 @@ int KindField::lookupRegion(const float p[3], float* u, uint32_t& id) const {
       pz[k] = hostNbPts_[leaf * 8 + k].z;
     }
     const uint32_t nbSize = std::min<uint32_t>(8, (uint32_t)regions_.size());
-    return (int)knn_pick(ids, px, py, pz, nbSize, p[0], p[1], p[2], u);
+    return (int)knn_pick(ids, px, py, pz, nbSize, p[0], p[1], p[2], u, vectorWidth_ != 4);
   }
   return (int)leaf;
 }
@@ -787,6 +829,7 @@ QueryField KindField::queryView() const {
   q.knn = kd_.knnLookup ? 1 : 0;
   q.trained = (initialized_ && iteration_ > 0) ? 1 : 0;
   q.parallax = fc_.noParallax ? 0 : 1;
+  q.knnFused = vectorWidth_ != 4 ? 1 : 0;
   for (int i = 0; i < 3; i++) { q.lo[i] = treeLo_[i]; q.hi[i] = treeHi_[i]; }
   return q;
 }

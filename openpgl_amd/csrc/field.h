@@ -41,7 +41,11 @@ struct HostSampleStats {
   void variance(float v[3]) const;
 };
 
-float host_rcp(float x);  // embree::rcp under AVX2: rcpss + fused Newton step (emath.h:61-88)
+// embree::rcp (emath.h:61-88) of the reference device of this SIMD width:
+// 4 = SSE4 (rcpss, Newton step without FMA), 8 = AVX2 (rcpss, fused Newton
+// step), 16 = AVX-512VL (rcp14, fused Newton step)
+float host_rcp(float x, int vectorWidth);
+bool host_supports_width(int vectorWidth);
 HostSampleStats int_stats_to_float(const IntStatsAcc& a, const float center[3], const float half[3]);
 
 struct KdSettings {
@@ -186,6 +190,9 @@ class KindField {
   // [6] region begins
   PinnedBuf pin_[2][7];
   DevBuf dCountSortTmp_;
+  DevBuf dDetTmp_, dDetCounter_;  // deterministic-mode sort
+  PinnedBuf detCounter_;
+  uint32_t lastDetPasses_ = 0;
   std::vector<uint32_t> hBegin_;
   PinnedBuf kdUp_, kdDown_;  // kd rounds (synchronous)
   template <class T> const T* pinned(PinnedBuf& b, const T* src, size_t count) {

@@ -30,6 +30,7 @@ struct QueryField {
   int32_t trained;
   float lo[3], hi[3];
   int32_t parallax;  // 0: plain VMM field, no shift to the query position
+  int32_t knnFused;  // KNN squared distances with FMA (CPU_8/CPU_16 devices) or without (CPU_4)
 };
 
 // spatial
@@ -61,6 +62,21 @@ void counting_sort_pairs(void* temp, const uint32_t* keys, const uint32_t* keyBe
 // (the region of its leaf; launch_leaf_keys and the sort in one)
 void counting_sort_leaves(void* temp, const uint32_t* nodeOf, const KDNode* keyNodes, const uint32_t* keyBegin,
                           uint32_t numKeys, uint32_t n, uint32_t* keysOut, uint32_t* valsOut, hipStream_t s);
+// Deterministic-mode order (kernels_sort.hip): region-contiguous positions
+// with every region's samples in SampleDataLess order (data/SampleData.h:82-93),
+// as Field::fitRegions/updateRegions sort them (field/Field.h:409-412, 469-472).
+// records: the n PGLSampleData records; the region of record i is
+// keyNodes[nodeOf[i]].idx() (det_sort_leaves) or keys[i] (det_sort_keys), all
+// < numKeys.  keysOut[j] = region and valsOut[j] = record index of sorted
+// position j.  temp: det_sort_temp_bytes(n) bytes; counterHost: pinned host
+// word.  Synchronises the stream once per pass; returns the number of
+// refinement passes (0 unless two records of a region share their weight).
+size_t det_sort_temp_bytes(uint32_t n);
+int det_sort_leaves(void* temp, const void* records, const uint32_t* nodeOf, const KDNode* keyNodes,
+                    uint32_t numKeys, uint32_t n, uint32_t* keysOut, uint32_t* valsOut, uint32_t* counterDev,
+                    uint32_t* counterHost, hipStream_t s);
+int det_sort_keys(void* temp, const void* records, const uint32_t* keys, uint32_t numKeys, uint32_t n,
+                  uint32_t* keysOut, uint32_t* valsOut, uint32_t* counterDev, uint32_t* counterHost, hipStream_t s);
 // keys/values of region-contiguous records (begin: numRegions ascending offsets)
 void launch_region_keys(const uint32_t* begin, uint32_t numRegions, uint32_t n, uint32_t* keys, uint32_t* vals,
                         hipStream_t s);

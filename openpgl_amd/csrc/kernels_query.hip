@@ -41,7 +41,7 @@ __device__ __forceinline__ int lookup_region(const QueryField& f, float x, float
       py[s] = p.y;
       pz[s] = p.z;
     }
-    return (int)knn_pick(ids, px, py, pz, f.nbSize, x, y, z, u);
+    return (int)knn_pick(ids, px, py, pz, f.nbSize, x, y, z, u, f.knnFused != 0);
   }
   return (int)leaf;
 }

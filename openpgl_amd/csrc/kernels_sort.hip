@@ -12,7 +12,10 @@
 #include <hip/hip_runtime.h>
 
 #include <hipcub/block/block_radix_sort.hpp>
+#include <hipcub/block/block_reduce.hpp>
 #include <hipcub/block/block_scan.hpp>
+#include <hipcub/device/device_radix_sort.hpp>
+#include <hipcub/device/device_scan.hpp>
 
 #include <algorithm>
 
@@ -200,6 +203,169 @@ void counting_sort_pairs(void* temp, const uint32_t* keys, const uint32_t* keyBe
 void counting_sort_leaves(void* temp, const uint32_t* nodeOf, const KDNode* keyNodes, const uint32_t* keyBegin,
                           uint32_t numKeys, uint32_t n, uint32_t* keysOut, uint32_t* valsOut, hipStream_t s) {
   counting_sort(temp, SortKeys{nullptr, nodeOf, keyNodes}, keyBegin, numKeys, n, PairOut{keysOut, valsOut}, s);
+}
+
+// ------------------------------------------------------------ deterministic order
+// Field::fitRegions / updateRegions sort every region's samples with
+// std::sort(SampleDataLess) in deterministic mode (field/Field.h:409-412,
+// 469-472).  SampleDataLess (data/SampleData.h:82-93) is lexicographic over
+// weight, pdf, distance, position x/y/z, direction x/y/z (flags ignored;
+// -0 and +0 compare equal).  On the device: a radix sort of
+// (region << 32 | key(weight)) pairs, then, only while equal keys remain,
+// refinement passes that sort (group << 32 | key(next field)) where group
+// numbers the runs of equal keys so far.  The result is the order of the
+// comparator; records equal in all nine fields differ at most in their flags,
+// which training does not read, so their relative order does not matter.
+namespace {
+
+// order-preserving map of a float to uint32 (-0 mapped to +0)
+__device__ __forceinline__ uint32_t order_key(float x) {
+  uint32_t b = __float_as_uint(x);
+  if (b == 0x80000000u) b = 0u;
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+// float offset inside the 40-byte record of the comparator's i-th field
+__device__ __forceinline__ int less_field(int i) {
+  // weight, pdf, distance, pos.x, pos.y, pos.z, dir.x, dir.y, dir.z
+  return i < 3 ? 6 + i : i - 3;
+}
+
+__global__ void __launch_bounds__(256) k_det_first(SortKeys keys, const float* __restrict__ recs, uint32_t n,
+                                                   uint64_t* __restrict__ keyOut, uint32_t* __restrict__ valOut) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint64_t r = keys(i);
+    keyOut[i] = (r << 32) | order_key(recs[(size_t)i * 10 + less_field(0)]);
+    valOut[i] = i;
+  }
+}
+
+// head[j] = 1 where a run of equal keys starts; counter += positions that
+// continue a run; regionOut (first pass only) = key >> 32
+__global__ void __launch_bounds__(256) k_det_heads(const uint64_t* __restrict__ key, uint32_t n,
+                                                   uint32_t* __restrict__ head, uint32_t* __restrict__ counter,
+                                                   uint32_t* __restrict__ regionOut) {
+  typedef hipcub::BlockReduce<uint32_t, 256> Reduce;
+  __shared__ typename Reduce::TempStorage tmp;
+  uint32_t ties = 0;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+    const uint64_t k = key[j];
+    const bool h = j == 0 || key[j - 1] != k;
+    head[j] = h ? 1u : 0u;
+    ties += h ? 0u : 1u;
+    if (regionOut) regionOut[j] = (uint32_t)(k >> 32);
+  }
+  const uint32_t t = Reduce(tmp).Sum(ties);
+  if (threadIdx.x == 0 && t) atomicAdd(counter, t);
+}
+
+// next pass: (group of equal keys so far << 32) | key(field), same values
+__global__ void __launch_bounds__(256) k_det_refine(const float* __restrict__ recs,
+                                                    const uint32_t* __restrict__ valIn,
+                                                    const uint32_t* __restrict__ groupIncl, int field, uint32_t n,
+                                                    uint64_t* __restrict__ keyOut, uint32_t* __restrict__ valOut) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+    const uint32_t v = valIn[j];
+    const uint64_t g = (uint64_t)(groupIncl[j] - 1u);
+    keyOut[j] = (g << 32) | order_key(recs[(size_t)v * 10 + less_field(field)]);
+    valOut[j] = v;
+  }
+}
+
+struct DetLayout {
+  size_t key[2], val, head, incl, sortTmp, scanTmp, sortBytes, scanBytes, total;
+};
+
+DetLayout det_layout(uint32_t n) {
+  auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
+  DetLayout L;
+  size_t sortBytes = 0, scanBytes = 0;
+  hipcub::DoubleBuffer<uint64_t> kb(nullptr, nullptr);
+  hipcub::DoubleBuffer<uint32_t> vb(nullptr, nullptr);
+  PGL_KERNEL_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, sortBytes, kb, vb, (int)n, 0, 64));
+  PGL_KERNEL_CHECK(hipcub::DeviceScan::InclusiveSum(nullptr, scanBytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                                    (int)n));
+  size_t off = 0;
+  L.key[0] = off; off += up(size_t(n) * 8);
+  L.key[1] = off; off += up(size_t(n) * 8);
+  L.val = off; off += up(size_t(n) * 4);
+  L.head = off; off += up(size_t(n) * 4);
+  L.incl = off; off += up(size_t(n) * 4);
+  L.sortTmp = off; off += up(sortBytes);
+  L.scanTmp = off; off += up(scanBytes);
+  L.sortBytes = sortBytes;
+  L.scanBytes = scanBytes;
+  L.total = off;
+  return L;
+}
+
+int bits_for(uint64_t count) {
+  int b = 0;
+  while (b < 32 && (1ull << b) < count) b++;
+  return b;
+}
+
+// the sort of both entry points; returns the number of refinement passes
+int det_sort(void* temp, SortKeys keys, const void* records, uint32_t numKeys, uint32_t n, uint32_t* keysOut,
+             uint32_t* valsOut, uint32_t* counterDev, uint32_t* counterHost, hipStream_t s) {
+  if (n == 0) return 0;
+  const DetLayout L = det_layout(n);
+  char* t = (char*)temp;
+  uint64_t* kA = (uint64_t*)(t + L.key[0]);
+  uint64_t* kB = (uint64_t*)(t + L.key[1]);
+  uint32_t* vB = (uint32_t*)(t + L.val);
+  uint32_t* head = (uint32_t*)(t + L.head);
+  uint32_t* incl = (uint32_t*)(t + L.incl);
+  const float* recs = (const float*)records;
+  const uint32_t g = std::min<uint32_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_det_first, dim3(g), dim3(256), 0, s, keys, recs, n, kA, valsOut);
+  hipcub::DoubleBuffer<uint64_t> K(kA, kB);
+  hipcub::DoubleBuffer<uint32_t> V(valsOut, vB);
+  size_t sortBytes = L.sortBytes;
+  PGL_KERNEL_CHECK(hipcub::DeviceRadixSort::SortPairs(t + L.sortTmp, sortBytes, K, V, (int)n, 0,
+                                                      32 + bits_for(numKeys), s));
+  int passes = 0;
+  for (int field = 0;; field++) {
+    PGL_KERNEL_CHECK(hipMemsetAsync(counterDev, 0, sizeof(uint32_t), s));
+    hipLaunchKernelGGL(k_det_heads, dim3(g), dim3(256), 0, s, K.Current(), n, head, counterDev,
+                       field == 0 ? keysOut : nullptr);
+    PGL_KERNEL_CHECK(hipMemcpyAsync(counterHost, counterDev, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    PGL_KERNEL_CHECK(hipStreamSynchronize(s));
+    const uint32_t ties = *counterHost;
+    if (ties == 0 || field == 8) break;
+    size_t scanBytes = L.scanBytes;
+    PGL_KERNEL_CHECK(hipcub::DeviceScan::InclusiveSum(t + L.scanTmp, scanBytes, head, incl, (int)n, s));
+    hipLaunchKernelGGL(k_det_refine, dim3(g), dim3(256), 0, s, recs, V.Current(), incl, field + 1, n,
+                       K.Alternate(), V.Alternate());
+    K.selector ^= 1;
+    V.selector ^= 1;
+    PGL_KERNEL_CHECK(hipcub::DeviceRadixSort::SortPairs(t + L.sortTmp, sortBytes, K, V, (int)n, 0,
+                                                        32 + bits_for(uint64_t(n) - ties), s));
+    passes++;
+  }
+  if (V.Current() != valsOut)
+    PGL_KERNEL_CHECK(hipMemcpyAsync(valsOut, V.Current(), size_t(n) * 4, hipMemcpyDeviceToDevice, s));
+  return passes;
+}
+
+}  // namespace
+
+size_t det_sort_temp_bytes(uint32_t n) { return det_layout(std::max<uint32_t>(n, 1)).total; }
+
+int det_sort_leaves(void* temp, const void* records, const uint32_t* nodeOf, const KDNode* keyNodes,
+                    uint32_t numKeys, uint32_t n, uint32_t* keysOut, uint32_t* valsOut, uint32_t* counterDev,
+                    uint32_t* counterHost, hipStream_t s) {
+  return det_sort(temp, SortKeys{nullptr, nodeOf, keyNodes}, records, numKeys, n, keysOut, valsOut, counterDev,
+                  counterHost, s);
+}
+
+int det_sort_keys(void* temp, const void* records, const uint32_t* keys, uint32_t numKeys, uint32_t n,
+                  uint32_t* keysOut, uint32_t* valsOut, uint32_t* counterDev, uint32_t* counterHost, hipStream_t s) {
+  return det_sort(temp, SortKeys{keys, nullptr, nullptr}, records, numKeys, n, keysOut, valsOut, counterDev,
+                  counterHost, s);
 }
 
 }  // namespace pglgpu

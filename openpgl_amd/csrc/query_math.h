@@ -144,6 +144,11 @@ PGL_HD inline int hg_index(float g) {
   return idx < 127 ? idx : 127;
 }
 
+PGL_HD inline float knn_d2_unfused(float dx, float dy, float dz) {
+#pragma clang fp contract(off)
+  return dx * dx + (dy * dy + dz * dz);
+}
+
 // KNN draw(): selects one of min(4, size) neighbours and rescales u in place (spatial/KNN.h:22-28)
 PGL_HD inline uint32_t knn_draw(float* u, uint32_t size) {
   size = size < 4u ? size : 4u;
@@ -152,11 +157,20 @@ PGL_HD inline uint32_t knn_draw(float* u, uint32_t size) {
   return sel < size - 1 ? sel : size - 1;
 }
 
-// RegionNeighbours<8>::sampleApproximateClosestRegionIdx (spatial/KNN.h:134-148):
-// squared distances (fused dot) with the low 3 mantissa bits replaced by the
-// slot index, sorted ascending, pick the selected one.
+// embree::dot(d, d) = madd(x, x, madd(y, y, z * z)) (math/vec3.h:204) with
+// madd fused (AVX2 / AVX-512 builds) or a * b + c rounded twice (SSE4 build,
+// simd/vfloat4_sse2.h:439-444)
+PGL_HD inline float knn_d2(float dx, float dy, float dz, bool fused) {
+  if (fused) return fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+  return knn_d2_unfused(dx, dy, dz);
+}
+
+// RegionNeighbours<8>::sampleApproximateClosestRegionIdx (spatial/KNN.h:134-148;
+// RegionNeighbours<4>, :86-113, merges two sorted halves to the same pick):
+// squared distances with the low 3 mantissa bits replaced by the slot index,
+// invalid slots at +inf, sorted ascending, pick the selected one.
 PGL_HD inline uint32_t knn_pick(const uint32_t* ids, const float* px, const float* py, const float* pz, uint32_t size,
-                                float x, float y, float z, float* u) {
+                                float x, float y, float z, float* u, bool fusedD2) {
   const uint32_t sel = knn_draw(u, size);
   float d[8];
   for (int i = 0; i < 8; i++) {
@@ -165,7 +179,7 @@ PGL_HD inline uint32_t knn_pick(const uint32_t* ids, const float* px, const floa
       continue;
     }
     const float dx = px[i] - x, dy = py[i] - y, dz = pz[i] - z;
-    float dd = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    float dd = knn_d2(dx, dy, dz, fusedD2);
     union { float f; uint32_t u; } cv;
     cv.f = dd;
     cv.u = (cv.u & ~7u) | (uint32_t)i;

@@ -8,7 +8,7 @@
 
 namespace pglgpu {
 
-RegionBatch::RegionBatch(const FactoryConfig& fc) : fc_(fc) {
+RegionBatch::RegionBatch(const FactoryConfig& fc, bool deterministic) : fc_(fc), deterministic_(deterministic) {
   std::vector<float> ud;
   uniform_directions(ud);
   dUniformDirs_.ensure(ud.size() * 4);
@@ -61,7 +61,7 @@ void RegionBatch::step(uint32_t R, const void* records, const uint32_t* counts, 
     total += counts[r];
     if (first || counts[r] > 0 || in.splitFlag) work.push_back(RegionWork{r, counts[r], (float)counts[r]});
   }
-  if (total > 0xffffffffull) throw std::runtime_error("region batch: more than 2^32 samples");
+  if (total > kMaxUpdateSamples) throw std::runtime_error("region batch: more than 2^30 samples");
   const uint32_t n = (uint32_t)total;
   dInputs_.ensure(size_t(R) * sizeof(RegionInput));
   dPivots_.ensure(size_t(R) * sizeof(float4));
@@ -72,7 +72,20 @@ void RegionBatch::step(uint32_t R, const void* records, const uint32_t* counts, 
   PGL_HIP_CHECK(hipMemcpyAsync(dBegin_.p, begin.data(), R * 4, hipMemcpyHostToDevice, s));
   launch_set_inputs(dStates_.as<RegionState>(), dInputs_.as<RegionInput>(), R, s);
   launch_region_keys(dBegin_.as<uint32_t>(), R, n, dKeys_.as<uint32_t>(), dVals_.as<uint32_t>(), s);
-  launch_gather_reproject(records, dKeys_.as<uint32_t>(), dVals_.as<uint32_t>(), n, dPivots_.as<float4>(),
+  const uint32_t* gatherKeys = dKeys_.as<uint32_t>();
+  const uint32_t* gatherIdx = dVals_.as<uint32_t>();
+  if (deterministic_ && n > 0) {
+    dSortedKeys_.ensure(size_t(n) * 4);
+    dSortedVals_.ensure(size_t(n) * 4);
+    dDetTmp_.ensure(det_sort_temp_bytes(n));
+    dDetCounter_.ensure(256);
+    detCounter_.ensure(256);
+    det_sort_keys(dDetTmp_.p, records, dKeys_.as<uint32_t>(), R, n, dSortedKeys_.as<uint32_t>(),
+                  dSortedVals_.as<uint32_t>(), dDetCounter_.as<uint32_t>(), detCounter_.as<uint32_t>(), s);
+    gatherKeys = dSortedKeys_.as<uint32_t>();
+    gatherIdx = dSortedVals_.as<uint32_t>();
+  }
+  launch_gather_reproject(records, gatherKeys, gatherIdx, n, dPivots_.as<float4>(),
                           dDx_.as<float>(), dDy_.as<float>(), dDz_.as<float>(), dW_.as<float>(), dPdf_.as<float>(),
                           dDist_.as<float>(), fc_.noParallax == 0, s);
   PGL_HIP_CHECK(hipMemcpyAsync(dSnapshot_.p, dStates_.p, size_t(R) * sizeof(RegionState), hipMemcpyDeviceToDevice, s));

@@ -13,7 +13,9 @@ namespace pglgpu {
 
 class RegionBatch {
  public:
-  RegionBatch(const FactoryConfig& fc);
+  // deterministic: every region's samples are first sorted by SampleDataLess
+  // (Field.h:409-412, 469-472)
+  RegionBatch(const FactoryConfig& fc, bool deterministic);
   ~RegionBatch();
 
   // One step for numRegions regions: the first call fits (ASM::fit), later
@@ -39,6 +41,7 @@ class RegionBatch {
 
  private:
   FactoryConfig fc_;
+  bool deterministic_;
   RegionRunner runner_;
   uint32_t numRegions_ = 0;
   uint32_t steps_ = 0;
@@ -46,6 +49,8 @@ class RegionBatch {
   std::vector<RegionWork> lastWork_;
   DevBuf dStates_, dSnapshot_, dInputs_, dPivots_, dBegin_, dKeys_, dVals_;
   DevBuf dDx_, dDy_, dDz_, dW_, dPdf_, dDist_, dUniformDirs_;
+  DevBuf dSortedKeys_, dSortedVals_, dDetTmp_, dDetCounter_;
+  PinnedBuf detCounter_;
   std::vector<RegionState> hostStates_;
   UpdateInfo info_;
   hipEvent_t ev_[2];

@@ -61,12 +61,25 @@ static const float OPENPGL_MAX_KAPPA_F = 32000.0f;   // WEM:13
 static const double ONE_OVER_FOUR_PI_D = 0.07957747154594767;
 
 // ---------------------------------------------------------------- host rcp
-// embree::rcp under -mavx2 -mfma (emath.h:61-88): rcpss + one fused Newton step.
+// embree::rcp (emath.h:61-88) under each device's flags (CMakeLists.txt:125-131):
+// CPU_8 -mavx2 -mfma: rcpss + one fused Newton step; CPU_4 -msse4.1: rcpss +
+// a Newton step without FMA; CPU_16 -mavx512vl (with AVX2): rcp14 + fused step.
 __attribute__((target("avx2,fma"))) static float rcp_avx2(float x) {
   const __m128 a = _mm_set_ss(x);
   const __m128 r = _mm_rcp_ss(a);
   return _mm_cvtss_f32(_mm_mul_ss(r, _mm_fnmadd_ss(r, a, _mm_set_ss(2.0f))));
 }
+__attribute__((target("sse4.1"))) static float rcp_sse4(float x) {
+  const __m128 a = _mm_set_ss(x);
+  const __m128 r = _mm_rcp_ss(a);
+  return _mm_cvtss_f32(_mm_mul_ss(r, _mm_sub_ss(_mm_set_ss(2.0f), _mm_mul_ss(r, a))));
+}
+__attribute__((target("avx512f,avx512vl,avx2,fma"))) static float rcp_avx512(float x) {
+  const __m128 a = _mm_set_ss(x);
+  const __m128 r = _mm_rcp14_ss(_mm_set_ss(0.0f), a);
+  return _mm_cvtss_f32(_mm_mul_ss(r, _mm_fnmadd_ss(r, a, _mm_set_ss(2.0f))));
+}
+static float rcp_width(float x, int W) { return W == 4 ? rcp_sse4(x) : (W == 16 ? rcp_avx512(x) : rcp_avx2(x)); }
 
 // ------------------------------------------------------- sample statistics
 struct SampleStats {
@@ -132,13 +145,13 @@ struct IntStats {
   uint32_t n = 0;
   int64_t lo[3], hi[3];
   float center[3], half[3], invHalf[3];
-  explicit IntStats(const float lower[3], const float upper[3]) {
+  explicit IntStats(const float lower[3], const float upper[3], int W = 8) {
     for (int i = 0; i < 3; i++) {
       lo[i] = std::numeric_limits<int>::max();
       hi[i] = -std::numeric_limits<int>::max();
       const float ext = upper[i] - lower[i];
       half[i] = ext * 0.5f;
-      invHalf[i] = rcp_avx2(half[i]);
+      invHalf[i] = rcp_width(half[i], W);
       center[i] = lower[i] + half[i];
     }
   }
@@ -1030,6 +1043,50 @@ template <class R> static bool vmmValid(const VMM<R>& v) {
 }
 
 // --------------------------------------------------------------- regions
+// ------------------------------------------------------------------- KNN pick
+static uint32_t knnDraw(float* u, uint32_t size) {  // KNN.h:22-28
+  size = std::min<uint32_t>(4, size);
+  const uint32_t sel = uint32_t(*u * float(size));
+  *u = (*u - float(sel) / float(size)) * float(size);
+  return std::min(sel, size - 1);
+}
+
+// RegionNeighbours<8>::sampleApproximateClosestRegionIdx (KNN.h:134-148);
+// RegionNeighbours<4> (KNN.h:86-113) merges two sorted halves to the same
+// pick.  d^2 = embree::dot(d, d) = madd(x, x, madd(y, y, z * z)) (vec3.h:204),
+// madd fused under AVX2 / AVX-512 and a * b + c under SSE4 (vfloat4_sse2.h:439-444).
+// ids: 8 slots (~0u = empty), pts: x[8], y[8], z[8].
+static uint32_t knnPickSlots(const uint32_t* ids, const float* pts, uint32_t size, const float p[3], float* u,
+                             bool fused) {
+  const uint32_t sel = knnDraw(u, size);
+  float dist[8];
+  for (int i = 0; i < 8; i++) {
+    if (ids[i] == ~0u) { dist[i] = std::numeric_limits<float>::infinity(); continue; }
+    const float dx = pts[0 + i] - p[0];
+    const float dy = pts[8 + i] - p[1];
+    const float dz = pts[16 + i] - p[2];
+    float d;
+    if (fused) {
+      d = std::fma(dx, dx, std::fma(dy, dy, dz * dz));
+    } else {
+      const float zz = dz * dz;
+      const float yy = dy * dy;
+      const float inner = yy + zz;
+      const float xx = dx * dx;
+      d = xx + inner;
+    }
+    uint32_t bits;
+    memcpy(&bits, &d, 4);
+    bits = (bits & ~7u) | uint32_t(i);
+    memcpy(&d, &bits, 4);
+    dist[i] = d;
+  }
+  std::sort(dist, dist + 8);
+  uint32_t bits;
+  memcpy(&bits, &dist[sel], 4);
+  return ids[bits & 7u];
+}
+
 template <class R> struct Region {
   VMM<R> vmm;
   Training<R> ts;
@@ -1132,7 +1189,7 @@ template <class R> struct Field {
   }
 
   SampleStats intStats(size_t b, size_t e, const float lo[3], const float hi[3]) const {
-    IntStats is(lo, hi);
+    IntStats is(lo, hi, int(cfg.vectorWidth));
     for (size_t i = b; i < e; i++) { const float p[3] = {samples[i].px, samples[i].py, samples[i].pz}; is.add(p); }
     return is.toSampleStats();
   }
@@ -1343,33 +1400,9 @@ template <class R> struct Field {
     return nodes[i].idx();
   }
 
-  static uint32_t draw(float* u, uint32_t size) {  // KNN.h:22-28
-    size = std::min<uint32_t>(4, size);
-    const uint32_t sel = uint32_t(*u * float(size));
-    *u = (*u - float(sel) / float(size)) * float(size);
-    return std::min(sel, size - 1);
-  }
 
-  // RegionNeighbours<8>::sampleApproximateClosestRegionIdx (KNN.h:134-148); d^2 fused as dot() under AVX2
   uint32_t knnPick(uint32_t leaf, const float p[3], float* u) const {
-    const uint32_t sel = draw(u, nbSize[leaf]);
-    float dist[8];
-    for (int i = 0; i < 8; i++) {
-      if (nbIds[leaf * 8 + i] == ~0u) { dist[i] = std::numeric_limits<float>::infinity(); continue; }
-      const float dx = nbPts[leaf * 24 + 0 + i] - p[0];
-      const float dy = nbPts[leaf * 24 + 8 + i] - p[1];
-      const float dz = nbPts[leaf * 24 + 16 + i] - p[2];
-      float d = std::fma(dx, dx, std::fma(dy, dy, dz * dz));
-      uint32_t bits;
-      memcpy(&bits, &d, 4);
-      bits = (bits & ~7u) | uint32_t(i);
-      memcpy(&d, &bits, 4);
-      dist[i] = d;
-    }
-    std::sort(dist, dist + 8);
-    uint32_t bits;
-    memcpy(&bits, &dist[sel], 4);
-    return nbIds[leaf * 8 + (bits & 7u)];
+    return knnPickSlots(&nbIds[leaf * 8], &nbPts[leaf * 24], nbSize[leaf], p, u, cfg.vectorWidth != 4);
   }
 
   const Region<R>* getRegion(const float p[3], float* u, uint32_t& id) const {  // Field.h:109-145
@@ -1668,7 +1701,65 @@ static int queryImpl(const Field<R>& fld, int volume, const float* pos, const fl
   return 1;
 }
 
+// The scalar SamplingDistribution calls of a renderer (SurfaceVolumeField.h:50-85,
+// VMM{Surface,Volume}SamplingDistribution.h): Init (u1 updated in place by the
+// KNN draw), `products` cosine / HG products, SamplePDF(u2), then PDF and
+// IncomingRadiancePDF at evalDir.  out: dir[3], pdf(dir), pdf(evalDir),
+// pdfLi(evalDir), product integral.
+template <class R>
+static int scalarSequence(const Field<R>& fld, int volume, const float* pos, float* u1, const float* nrm, float g,
+                          int products, const float* u2, const float* ev, float* out, uint32_t* idOut) {
+  QueryDist<R> q;
+  uint32_t id = ~0u;
+  if (!initQuery(fld, pos, u1, q, id)) return 0;
+  for (int p = 0; p < products; p++) {
+    if (volume) applyHG(q, nrm, g); else applyCosine(q, nrm);
+  }
+  R d[3];
+  querySample(q, R(u2[0]), R(u2[1]), d);
+  float df[3];
+  for (int j = 0; j < 3; j++) { df[j] = float(d[j]); out[j] = df[j]; }
+  const R dd[3] = {R(df[0]), R(df[1]), R(df[2])};
+  out[3] = float(queryPdf(q, dd));
+  const R e[3] = {R(ev[0]), R(ev[1]), R(ev[2])};
+  out[4] = float(queryPdf(q, e));
+  out[5] = float(vmmPdf(q.li, e));
+  out[6] = float(q.integral);
+  if (idOut) *idOut = id;
+  return 1;
+}
+
+template <class R>
+static int pdfAtDirs(const Field<R>& fld, int volume, const float* pos, const float* nrm, float g, size_t n,
+                     const float* dirs, float* pdfOut) {
+  QueryDist<R> q;
+  uint32_t id = ~0u;
+  float u = -1.f;
+  if (!initQuery(fld, pos, &u, q, id)) return 0;
+  if (volume) applyHG(q, nrm, g); else applyCosine(q, nrm);
+  for (size_t i = 0; i < n; i++) {
+    const R d[3] = {R(dirs[3 * i]), R(dirs[3 * i + 1]), R(dirs[3 * i + 2])};
+    pdfOut[i] = float(queryPdf(q, d));
+  }
+  return 1;
+}
+
 extern "C" {
+
+int orc_field_scalar_sequence(orc_field f, int volume, const float* pos, float* u1, const float* nrm, float g,
+                              int products, const float* u2, const float* evalDir, float* out7, uint32_t* id) {
+  int r = 0;
+  DISPATCH(f, r = scalarSequence(fld, volume, pos, u1, nrm, g, products, u2, evalDir, out7, id));
+  return r;
+}
+
+// pdf of one query's distribution (Init at pos without KNN, one product) at n directions
+int orc_field_query_pdf_dirs(orc_field f, int volume, const float* pos, const float* nrm, float g, size_t n,
+                             const float* dirs, float* pdfOut) {
+  int r = 0;
+  DISPATCH(f, r = pdfAtDirs(fld, volume, pos, nrm, g, n, dirs, pdfOut));
+  return r;
+}
 
 int orc_field_query(orc_field f, int volume, const float* pos, const float* nrm, float g, const float* u2, float u1,
                     float* dirOut, float* pdfOut, uint32_t* idOut, float* integral) {
@@ -1836,6 +1927,16 @@ void orc_integer_stats(const float* pos, size_t n, const float* lo, const float*
   }
 }
 float orc_rcp(float x) { return rcp_avx2(x); }
+float orc_rcp_width(float x, int W) { return rcp_width(x, W); }
+void orc_integer_stats_width(const float* pos, size_t n, const float* lo, const float* hi, int W, int64_t* raw) {
+  IntStats is(lo, hi, W);
+  for (size_t i = 0; i < n; i++) is.add(pos + 3 * i);
+  for (int j = 0; j < 3; j++) { raw[j] = is.mean[j]; raw[3 + j] = is.var[j]; raw[7 + j] = is.lo[j]; raw[10 + j] = is.hi[j]; }
+  raw[6] = is.n;
+}
+uint32_t orc_knn_pick(const uint32_t* ids8, const float* pts24, uint32_t size, const float* p, float* u, int fused) {
+  return knnPickSlots(ids8, pts24, size, p, u, fused != 0);
+}
 
 void orc_set_hg_table(const float* t) {
   for (int i = 0; i < 128; i++) {

@@ -133,6 +133,16 @@ void orc_field_query_batch(orc_field f, int volume, size_t n, const float* pos, 
 // pdf of a given direction for an initialised query (used for PDF compare)
 int orc_field_query_pdf(orc_field f, int volume, const float* pos, const float* normalOrDir, float g,
                         float u1, const float* dir, float* pdfOut);
+// the scalar call sequence of a renderer: Init (u1 updated in place by the
+// KNN draw), `products` cosine (surface) or HG (volume) products,
+// SamplePDF(u2), PDF and IncomingRadiancePDF at evalDir.  out7: sampled
+// dir[3], its pdf, pdf(evalDir), pdfLi(evalDir), product integral.
+int orc_field_scalar_sequence(orc_field f, int volume, const float* pos, float* u1, const float* normalOrDir,
+                              float g, int products, const float* u2, const float* evalDir, float* out7,
+                              uint32_t* id);
+// pdf of one query (Init without KNN + one product) at n directions; 0 if Init fails
+int orc_field_query_pdf_dirs(orc_field f, int volume, const float* pos, const float* normalOrDir, float g,
+                             size_t n, const float* dirs, float* pdfOut);
 
 // single-region directional KATs (no spatial structure): fit on the first
 // call, update afterwards, exactly as Field::fitRegions/updateRegions do for
@@ -158,6 +168,15 @@ void orc_stats_merge_split(const float* a13, const float* b13, int dim, float po
 void orc_integer_stats(const float* pos, size_t n, const float* lower3, const float* upper3,
                        float* out13, int64_t* rawOut13);
 float orc_rcp(float x);
+// embree::rcp of the device of SIMD width W (4: SSE4, 8: AVX2, 16: AVX-512VL)
+float orc_rcp_width(float x, int W);
+// integer statistics (raw int64 sums, counts, min/max) with the reciprocal of width W
+void orc_integer_stats_width(const float* pos, size_t n, const float* lower3, const float* upper3, int W,
+                             int64_t* rawOut13);
+// KNN pick on one neighbourhood: ids8 (~0u = empty slot), pts24 = x[8] y[8] z[8],
+// size = neighbours found, p = query position, u updated in place; fused:
+// squared distances with FMA (CPU_8 / CPU_16) or without (CPU_4)
+uint32_t orc_knn_pick(const uint32_t* ids8, const float* pts24, uint32_t size, const float* p, float* u, int fused);
 
 // HG phase-function representation lookup (fills K and 4x{w, meanCosine, kappa, norm})
 int orc_hg_rep(float g, float* w, float* mc, float* kappa, float* norm);

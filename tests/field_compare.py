@@ -65,6 +65,28 @@ class OracleField:
         ok = self.lib.orc_field_query_pdf(self.h, int(volume), ptr(p), ptr(nn), float(g), float(u1), ptr(dd), ptr(out))
         return ok, float(out[0])
 
+    def scalar_sequence(self, volume, pos, u1, nrm, g, products, u2, eval_dir):
+        """Init + products + SamplePDF + PDF/IncomingRadiancePDF at eval_dir:
+        (ok, u1 after Init, id, dir, pdf(dir), pdf(eval_dir), pdfLi(eval_dir), integral)."""
+        p = np.ascontiguousarray(pos, np.float32)
+        u = np.array([u1], np.float32)
+        nn = np.ascontiguousarray(nrm, np.float32)
+        uu = np.ascontiguousarray(u2, np.float32)
+        e = np.ascontiguousarray(eval_dir, np.float32)
+        out = np.zeros(7, np.float32)
+        i = np.zeros(1, np.uint32)
+        ok = self.lib.orc_field_scalar_sequence(self.h, int(volume), ptr(p), ptr(u), ptr(nn), float(g), int(products),
+                                                ptr(uu), ptr(e), ptr(out), ptr(i))
+        return bool(ok), float(u[0]), int(i[0]), out[:3].copy(), float(out[3]), float(out[4]), float(out[5]), float(out[6])
+
+    def pdf_at_dirs(self, volume, pos, nrm, g, dirs):
+        p = np.ascontiguousarray(pos, np.float32)
+        nn = np.ascontiguousarray(nrm, np.float32)
+        d = np.ascontiguousarray(dirs, np.float32)
+        out = np.zeros(len(d), np.float32)
+        ok = self.lib.orc_field_query_pdf_dirs(self.h, int(volume), ptr(p), ptr(nn), float(g), len(d), ptr(d), ptr(out))
+        return bool(ok), out
+
     def __del__(self):
         try:
             self.lib.orc_field_release(self.h)

@@ -91,6 +91,11 @@ def oracle():
         lib.orc_field_query.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_float, vp, ctypes.c_float, vp, vp, vp, vp]
         lib.orc_field_query_pdf.restype = ctypes.c_int
         lib.orc_field_query_pdf.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_float, ctypes.c_float, vp, vp]
+        lib.orc_field_scalar_sequence.restype = ctypes.c_int
+        lib.orc_field_scalar_sequence.argtypes = [vp, ctypes.c_int, vp, vp, vp, ctypes.c_float, ctypes.c_int, vp, vp,
+                                                  vp, vp]
+        lib.orc_field_query_pdf_dirs.restype = ctypes.c_int
+        lib.orc_field_query_pdf_dirs.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_float, ctypes.c_size_t, vp, vp]
         lib.orc_field_query_batch.argtypes = [vp, ctypes.c_int, ctypes.c_size_t, vp, vp, vp, vp, vp, vp, vp, vp, vp, ctypes.c_int]
         lib.orc_region_new.restype = vp
         lib.orc_region_new.argtypes = [ctypes.POINTER(Config), ctypes.c_int]
@@ -102,6 +107,11 @@ def oracle():
         lib.orc_integer_stats.argtypes = [vp, ctypes.c_size_t, vp, vp, vp, vp]
         lib.orc_rcp.restype = ctypes.c_float
         lib.orc_rcp.argtypes = [ctypes.c_float]
+        lib.orc_rcp_width.restype = ctypes.c_float
+        lib.orc_rcp_width.argtypes = [ctypes.c_float, ctypes.c_int]
+        lib.orc_integer_stats_width.argtypes = [vp, ctypes.c_size_t, vp, vp, ctypes.c_int, vp]
+        lib.orc_knn_pick.restype = ctypes.c_uint32
+        lib.orc_knn_pick.argtypes = [vp, vp, ctypes.c_uint32, vp, vp, ctypes.c_int]
         lib.orc_set_hg_table.argtypes = [vp]
         lib.orc_hg_rep.restype = ctypes.c_int
         lib.orc_hg_rep.argtypes = [ctypes.c_float, vp, vp, vp, vp]

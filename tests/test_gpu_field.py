@@ -170,3 +170,46 @@ def test_schedule_independence():
         for i, (a, b) in enumerate(zip(states[0], s)):
             for k in _DIRECTIONAL_KEYS:
                 assert np.asarray(a[k]).tobytes() == np.asarray(b[k]).tobytes(), (i, k, a[k], b[k])
+
+
+@pytest.mark.parametrize("device_type", ["cpu4", "cpu16"])
+def test_device_width_arithmetic(device_type):
+    """A field created on a CPU_4 (SSE4) or CPU_16 (AVX-512) device follows
+    that device's arithmetic where it decides discrete results: the kd-tree
+    reciprocals (embree::rcp per ISA) and the KNN squared distances (unfused
+    under SSE4).  Tree, region statistics and KNN-picked region ids equal
+    the oracle configured for the same width."""
+    import openpgl_amd as pgl
+    width = 4 if device_type == "cpu4" else 16
+    if width == 16:
+        with open("/proc/cpuinfo") as f:
+            flags = f.read()
+        if "avx512vl" not in flags:
+            pytest.skip("host CPU without AVX-512VL")
+    dev = pgl.Device(pgl.PGL_DEVICE_TYPE_CPU_4 if width == 4 else pgl.PGL_DEVICE_TYPE_CPU_16)
+    gpu = pgl.Field(dev, max_samples_per_leaf=2000, deterministic=True)
+    cfg = default_config(2000)
+    cfg.vectorWidth = width
+    orc = OracleField(cfg)
+    for f in range(2):
+        s = generate(40000, seed=17, frame=f)
+        gpu.update(pgl.SampleStorage(s))
+        orc.update(s)
+    _tree_equal(gpu, orc)
+    for i in range(gpu.num_regions()):
+        a, b = gpu.region(i), orc.region(i)
+        for key in ["ssMean", "ssVariance", "ssLower", "ssUpper"]:
+            assert np.array_equal(a[key].view(np.uint32), b[key].view(np.uint32)), (i, key)
+    assert np.array_equal(gpu.knn(), orc.knn())
+    pos, nrm, u2, _, _ = queries(20000, seed=8)
+    u1 = np.random.default_rng(9).random(len(pos)).astype(np.float32)
+    d, pdf, ids, _ = gpu.sample_batch(pos, nrm, u2, u1=u1)
+    _, opdf, oids, _ = orc.query_batch(0, pos, nrm, u2, u1=u1)
+    valid = opdf > 0
+    assert np.array_equal(pdf > 0, valid)
+    assert np.array_equal(ids[valid], oids[valid])
+    # the region the KNN pick lands in: same pdf of the sampled direction
+    rel = np.array([abs(pdf[i] / orc.query_pdf(0, pos[i], nrm[i], 0.0, u1[i], d[i])[1] - 1.0)
+                    for i in np.nonzero(valid)[0][:3000]])
+    print(device_type, "regions", gpu.num_regions(), "max rel pdf", rel.max(), flush=True)
+    assert np.all(rel <= 1e-3)
