@@ -530,6 +530,7 @@ template <class R> struct Factory {
       it++;
       if (it > 1) {
         const R rel = std::fabs(ll - prevLL) * invPrevLL;
+        if (getenv("ORC_DEBUG_EM")) fprintf(stderr, "em it %d K %d ll %.9g rel %.9g\n", it, v.K, double(ll), double(rel));
         if (rel < R(c.convergenceThreshold)) converged = true;
         prevLL = ll;
         invPrevLL = R(1) / std::fabs(ll);
@@ -1298,8 +1299,10 @@ template <class R> struct Field {
       Suff<R>& s = reg.ts.suff;
       for (int k = 0; k < MAXK; k++) { for (int j = 0; j < 3; j++) s.swd[k][j] *= a; s.sw[k] *= a; s.sdw[k] *= a; }
       s.numSamples *= a; s.sumWeights *= a;
+      // ComponentSplitStatistics::decay (Splitter:1146-1162): whole SIMD blocks of the device width
       SplitStats<R>& sp = reg.ts.split;
-      const int lim = std::min(MAXK, ((sp.K + 7) / 8) * 8);
+      const int W = int(cfg.vectorWidth);
+      const int lim = std::min(MAXK, ((sp.K + W - 1) / W) * W);
       for (int k = 0; k < lim; k++) { sp.cov[k][0] *= a; sp.cov[k][1] *= a; sp.cov[k][2] *= a; sp.n[k] *= a; sp.sw[k] *= a; sp.sa[k] *= a; }
       reg.splitFlag = false;
     }
@@ -1826,8 +1829,10 @@ static void regionStep(Region<R>& reg, const Factory<R>& fac, bool& first, const
       Suff<R>& st = reg.ts.suff;
       for (int k = 0; k < MAXK; k++) { for (int j = 0; j < 3; j++) st.swd[k][j] *= a; st.sw[k] *= a; st.sdw[k] *= a; }
       st.numSamples *= a; st.sumWeights *= a;
+      // ComponentSplitStatistics::decay (Splitter:1146-1162): whole SIMD blocks of the device width
       SplitStats<R>& sp = reg.ts.split;
-      const int lim = std::min(MAXK, ((sp.K + 7) / 8) * 8);
+      const int W = int(fac.c.vectorWidth);
+      const int lim = std::min(MAXK, ((sp.K + W - 1) / W) * W);
       for (int k = 0; k < lim; k++) { sp.cov[k][0] *= a; sp.cov[k][1] *= a; sp.cov[k][2] *= a; sp.n[k] *= a; sp.sw[k] *= a; sp.sa[k] *= a; }
     }
     if (fac.c.parallax) {

@@ -155,3 +155,60 @@ def precision_report(records_test, records_ref, dirs):
             "weight_abs_median": q(w_err, 50), "weight_abs_p99": q(w_err, 99),
             "mean_dir_angle_median": q(ang, 50), "mean_dir_angle_p99": q(ang, 99),
             "mean_cosine_rel_median": q(mc_rel, 50), "mean_cosine_rel_p99": q(mc_rel, 99)}
+
+
+def _oracle_kind_state(ff, orc, is_surface, fcfg, iteration, max_samples, knn):
+    piv, dims, cod = orc.nodes()
+    nodes = np.zeros(len(piv), dtype=[("pivot", "<f4"), ("bits", "<u4")])
+    nodes["pivot"] = piv
+    nodes["bits"] = (dims.astype(np.uint32) << 30) | cod
+    regions = []
+    for i in range(orc.num_regions()):
+        r = orc.region(i)
+        r["splitMeans"] = np.zeros((32, 2), np.float32)
+        r["rangeBegin"] = r["rangeEnd"] = 0
+        regions.append(r)
+    lo, hi = np.zeros(3, np.float32), np.zeros(3, np.float32)
+    orc.lib.orc_field_get_scene_bounds(orc.h, lo.ctypes.data, hi.ctypes.data)
+    st = ff.empty_field(is_surface, fcfg, 100, max_samples, 32, knn=knn)
+    st.update({"iteration": iteration, "isSceneBoundsSet": 1, "sceneBounds": (*lo, *hi), "initialized": 1,
+               "treeInit": 1, "treeBounds": (*lo, *hi), "nodes": nodes, "regions": regions, "knnBuilt": int(knn),
+               "knnPoints": (np.array([[*r["ssMean"], 0.0] for r in regions], np.float32) if knn
+                             else np.zeros((0, 4), np.float32))})
+    return st
+
+
+def oracle_field_file(cfg, iteration, max_samples, surface=None, volume=None, W=8):
+    """Bytes of a field file (the reference's stream layout, restated in
+    tests/field_file.py) holding the oracle's CPU-trained surface and/or
+    volume fields, so the GPU library can serve exactly the oracle's mixtures."""
+    import field_file as ff
+    fcfg = {"initK": cfg.initK, "initKappa": cfg.initKappa, "maxK": cfg.maxK, "maxEMIterrations": cfg.maxEMIterations,
+            "maxKappa": cfg.maxKappa, "maxMeanCosine": float(np.float32(1.0 / np.tanh(np.float32(cfg.maxKappa))
+                                                                         - 1.0 / np.float32(cfg.maxKappa))),
+            "convergenceThreshold": cfg.convergenceThreshold, "weightPrior": cfg.weightPrior,
+            "meanCosinePriorStrength": cfg.meanCosinePriorStrength, "meanCosinePrior": cfg.meanCosinePrior,
+            "splittingThreshold": cfg.splittingThreshold, "mergingThreshold": cfg.mergingThreshold,
+            "partialReFit": cfg.partialReFit, "maxSplitItr": 1, "minSamplesForSplitting": cfg.minSamplesForSplitting,
+            "minSamplesForMerging": cfg.minSamplesForMerging,
+            "minSamplesForPartialRefitting": cfg.minSamplesForPartialRefitting}
+    knn = bool(cfg.knnLookup)
+    surf = (_oracle_kind_state(ff, surface, True, fcfg, iteration, max_samples, knn) if surface is not None
+            else ff.empty_field(True, fcfg, 100, max_samples, 32, knn=knn))
+    vol = (_oracle_kind_state(ff, volume, False, fcfg, iteration, max_samples, knn) if volume is not None
+           else ff.empty_field(False, fcfg, 100, max_samples, 32, knn=knn))
+    return ff.write_file({"iteration": iteration, "surface": surf, "volume": vol}, W)
+
+
+def load_oracle_field_on_gpu(dev, path, cfg, iteration, max_samples, surface=None, volume=None):
+    import openpgl_amd as pgl
+    with open(path, "wb") as fh:
+        fh.write(oracle_field_file(cfg, iteration, max_samples, surface, volume))
+    return pgl.Field.from_file(dev, path)
+
+
+def canonical_ids(orc):
+    """storage -> canonical region index map of an oracle field (the file holds canonical order)"""
+    s2c = np.zeros(orc.num_regions(), np.uint32)
+    orc.lib.orc_field_region_canonical_map(orc.h, s2c.ctypes.data)
+    return s2c

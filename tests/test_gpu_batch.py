@@ -2,29 +2,27 @@
 against the oracle, region by region, and schedule independence of the
 batched launch, the per-region path and the autotuner.
 
-Tolerance (as for the field's mixtures): per region the pdf at 512
-directions; median over regions of the median relative error <= 1e-3 and
->= 90% of the regions with the 99th percentile below 1e-2 (a region near a
-split/merge/stop threshold can take another discrete decision under a
-different summation order).
+Tolerance (as for the field's mixtures): three-way against the FP32 and
+FP64 oracles (tests/parity_checks.py).
 """
 import numpy as np
 import pytest
 
 from field_compare import unit_dirs, vmm_pdf
 from oracle_ctypes import OracleRegion, default_config
+from parity_checks import assert_mixture_parity
 from openpgl_amd.synth import batched_regions
 
 pytestmark = pytest.mark.gpu
 
 
-def _oracle_regions(recs, counts, stats7, init_k, steps):
+def _oracle_regions(recs, counts, stats7, init_k, steps, precision=0):
     out = []
     offs = np.concatenate([[0], np.cumsum(counts.astype(np.int64))])
     for r in range(len(counts)):
         cfg = default_config(32000)
         cfg.initK = int(init_k[r])
-        o = OracleRegion(cfg)
+        o = OracleRegion(cfg, precision)
         out.append(o)
     for st in steps:
         recs_s, stats_s = st
@@ -32,19 +30,6 @@ def _oracle_regions(recs, counts, stats7, init_k, steps):
             seg = recs_s[offs[r]:offs[r + 1]]
             o.step(seg, stats_s[r, 0:3], stats_s[r, 3:6], float(stats_s[r, 6]))
     return out
-
-
-def _errors(batch, oracles, dirs):
-    errs = []
-    for r, o in enumerate(oracles):
-        a, b = batch.region(r), o.record()
-        assert a["valid"] == b["valid"], r
-        if not b["valid"]:
-            continue
-        pa, pb = vmm_pdf(a, dirs), vmm_pdf(b, dirs)
-        rel = np.abs(pa - pb) / pb
-        errs.append((np.median(rel), np.percentile(rel, 99), a["K"], b["K"]))
-    return np.array(errs)
 
 
 def test_region_batch_matches_oracle():
@@ -61,12 +46,10 @@ def test_region_batch_matches_oracle():
         d = DeviceArray.from_numpy(recs)
         batch.step_device(d.value, counts, stats, init_k)
         steps.append((recs, stats))
-        oracles = _oracle_regions(recs0, counts, stats0, init_k, steps)
-        e = _errors(batch, oracles, dirs)
-        print("regions", len(e), "median-of-median rel", np.median(e[:, 0]), "frac p99<1e-2",
-              np.mean(e[:, 1] < 1e-2), "K equal", np.mean(e[:, 2] == e[:, 3]), flush=True)
-        assert np.median(e[:, 0]) <= 1e-3
-        assert np.mean(e[:, 1] < 1e-2) >= 0.9
+        o32 = _oracle_regions(recs0, counts, stats0, init_k, steps, 0)
+        o64 = _oracle_regions(recs0, counts, stats0, init_k, steps, 1)
+        assert_mixture_parity([batch.region(r) for r in range(len(counts))], [o.record() for o in o32],
+                              [o.record() for o in o64], dirs, "step %d" % (len(steps) - 1))
 
 
 _KEYS = ["K", "weights", "kappas", "meanDirections", "normalizations", "meanCosines", "distances",

@@ -13,11 +13,12 @@ split-and-merge, deterministic; SURVEY workload generator):
       directions, mean cosines and pdf against the FP64 oracle.
 
 Bars: kd tree, leaf ids, region sample statistics and KNN lists bit-exact
-after every Update; mixtures per region (parity_checks: median of the
-per-region median relative pdf error <= 1e-4, >= 99% of the regions with
-the 99th percentile < 1e-2, >= 99% with the oracle's component count);
-query pdf of the GPU's sampled direction under the oracle's distribution
-within 1e-3 relative for every valid query; region ids exact.
+after every Update; mixtures three-way against the FP32 and FP64 oracles
+(tests/parity_checks.py); queries: the query path alone (the oracle's
+trained field served by the GPU through a field file) gives the oracle's
+region ids exactly and the pdf of every valid query within 1e-3 relative;
+end to end (GPU-trained field against oracle-trained fields) three-way as
+the mixtures (tests/parity_checks.py assert_query_pdfs).
 """
 import json
 import os
@@ -25,9 +26,9 @@ import os
 import numpy as np
 import pytest
 
-from field_compare import OracleField, precision_report, unit_dirs
+from field_compare import OracleField, canonical_ids, load_oracle_field_on_gpu, precision_report, unit_dirs
 from oracle_ctypes import OracleRegion, default_config, load_hg_table_into_oracle
-from parity_checks import assert_mixture_bars, assert_spatial_bit_exact, mixture_errors
+from parity_checks import assert_mixture_parity, assert_query_pdfs, assert_spatial_bit_exact, query_pdf_errors
 from openpgl_amd.synth import batched_regions, generate, generate_large, queries
 
 pytestmark = pytest.mark.gpu
@@ -40,83 +41,82 @@ def _records(field, kind=0):
     return [field.region(i, kind) if kind else field.region(i) for i in range(n)]
 
 
-def _query_pdf_parity(gpu, orcs, n, seed, knn_u):
-    """Batched GPU queries (30% volume) against the oracle: validity and ids
-    exact, pdf of the GPU's sampled direction within 1e-3 for every valid query."""
+def _query_rel(gpu, orcs, n, seed, knn_u, canonical=False, orcs64=None):
+    """Batched GPU queries (30% volume) against the oracle fields (surface,
+    volume; None = untrained): validity and region ids exact; returns the
+    relative pdf errors of the GPU's sampled directions (query_pdf_errors)"""
     pos, nrm, u2, is_vol, g = queries(n, seed=seed)
     u1 = np.random.default_rng(seed).random(n).astype(np.float32) if knn_u else np.full(n, -1.0, np.float32)
     d, pdf, ids, _ = gpu.sample_batch(pos, nrm, u2, u1=u1, g=g, is_volume=is_vol)
-    worst = 0.0
+    out = [[], [], []]
     for volume, o in enumerate(orcs):
+        if o is None:
+            continue
         m = is_vol == bool(volume)
         _, opdf, oids, _ = o.query_batch(volume, pos[m], nrm[m], u2[m], u1=u1[m], g=g[m], threads=THREADS)
         valid = opdf > 0
         assert np.array_equal(pdf[m] > 0, valid)
-        assert np.array_equal(ids[m][valid], oids[valid])
-        idx = np.nonzero(m)[0][valid]
-        rel = np.array([abs(pdf[i] - o.query_pdf(volume, pos[i], nrm[i], g[i], u1[i], d[i])[1]) /
-                        o.query_pdf(volume, pos[i], nrm[i], g[i], u1[i], d[i])[1] for i in idx])
-        print("volume" if volume else "surface", "valid queries", len(idx), "median rel", np.median(rel),
-              "max rel", rel.max(), flush=True)
-        assert len(idx) > 0.5 * m.sum()
-        assert np.all(rel <= 1e-3)
-        worst = max(worst, float(rel.max()))
-    return worst
+        expect = canonical_ids(o)[oids[valid]] if canonical else oids[valid]
+        assert np.array_equal(ids[m][valid], expect)
+        assert valid.sum() > 0.5 * m.sum()
+        o64 = orcs64[volume] if orcs64 is not None else None
+        for k, r in enumerate(query_pdf_errors(d, pdf, np.nonzero(m)[0][valid], volume, pos, nrm, g, u1, o, o64)):
+            if r is not None:
+                out[k].append(r)
+    return tuple(np.concatenate(x) if x else None for x in out)
 
 
-def test_c1_single_update():
+def test_c1_single_update(tmp_path):
     import openpgl_amd as pgl
     load_hg_table_into_oracle()
     dev = pgl.Device()
     gpu = pgl.Field(dev, max_samples_per_leaf=32000, deterministic=True)
-    orc = OracleField(default_config(32000))
+    cfg = default_config(32000)
+    o32, o64 = OracleField(cfg, 0), OracleField(cfg, 1)
     s = generate(200000, seed=0)
     print("c1: 200000 samples", flush=True)
     gpu.update(pgl.SampleStorage(s))
-    orc.update(s, threads=THREADS)
-    R = assert_spatial_bit_exact(gpu, orc, generate(20000, seed=9, frame=1)["pos"])
-    print("regions", R, flush=True)
+    o32.update(s, threads=THREADS)
+    o64.update(s, threads=THREADS)
+    R = assert_spatial_bit_exact(gpu, o32, generate(20000, seed=9, frame=1)["pos"])
     assert R >= 4
-    assert_mixture_bars(mixture_errors(_records(gpu), _records(orc), unit_dirs(512)), "c1")
-    # surface queries only (c1 trains the surface field)
-    pos, nrm, u2, _, _ = queries(20000, seed=5)
-    u1 = np.random.default_rng(6).random(len(pos)).astype(np.float32)
-    d, pdf, ids, _ = gpu.sample_batch(pos, nrm, u2, u1=u1)
-    _, opdf, oids, _ = orc.query_batch(0, pos, nrm, u2, u1=u1, threads=THREADS)
-    valid = opdf > 0
-    assert np.array_equal(pdf > 0, valid) and np.array_equal(ids[valid], oids[valid])
-    rel = np.array([abs(pdf[i] / orc.query_pdf(0, pos[i], nrm[i], 0.0, u1[i], d[i])[1] - 1.0)
-                    for i in np.nonzero(valid)[0]])
-    print("c1 queries valid", int(valid.sum()), "median rel", np.median(rel), "max rel", rel.max(), flush=True)
-    assert valid.mean() > 0.9 and np.all(rel <= 1e-3)
+    assert_mixture_parity(_records(gpu), _records(o32), _records(o64), unit_dirs(512), "c1 regions %d" % R)
+    # end to end (surface only: c1 trains the surface field)
+    r32, r64, r3264 = _query_rel(gpu, (o32, None), 20000, seed=5, knn_u=True, orcs64=(o64, None))
+    assert_query_pdfs(r32, "c1 queries end to end", r64, r3264)
+    # the query path alone: the oracle's field served by the GPU
+    served = load_oracle_field_on_gpu(dev, str(tmp_path / "c1.pgl"), cfg, 1, 32000, surface=o32)
+    rel = _query_rel(served, (o32, None), 20000, seed=5, knn_u=True, canonical=True)[0]
+    print("c1 queries on the oracle's field: median rel", np.median(rel), "max", rel.max(), flush=True)
+    assert np.all(rel <= 1e-3)
 
 
 def test_c2_eight_incremental_frames():
     import openpgl_amd as pgl
     dev = pgl.Device()
     gpu = pgl.Field(dev, max_samples_per_leaf=32000, deterministic=True)
-    orc = OracleField(default_config(32000))
+    cfg = default_config(32000)
+    o32, o64 = OracleField(cfg, 0), OracleField(cfg, 1)
     dirs = unit_dirs(512)
-    summary = []
     for f in range(8):
         s = generate(1_000_000, seed=0, frame=f)
         print("c2: frame", f, flush=True)
         gpu.update(pgl.SampleStorage(s))
-        orc.update(s, threads=THREADS)
-        R = assert_spatial_bit_exact(gpu, orc, generate(20000, seed=13, frame=f + 1)["pos"])
-        st = assert_mixture_bars(mixture_errors(_records(gpu), _records(orc), dirs), "c2 frame %d regions %d" % (f, R))
-        summary.append(st)
-    assert summary[-1]["regions"] > 50
+        o32.update(s, threads=THREADS)
+        o64.update(s, threads=THREADS)
+        R = assert_spatial_bit_exact(gpu, o32, generate(20000, seed=13, frame=f + 1)["pos"])
+        assert_mixture_parity(_records(gpu), _records(o32), _records(o64), dirs, "c2 frame %d regions %d" % (f, R))
+    assert R > 200
 
 
-def _oracle_region_records(recs, counts, stats, init_k, regions):
+def _oracle_region_records(recs, counts, stats, init_k, regions, precision=0):
     import concurrent.futures
     offs = np.concatenate([[0], np.cumsum(counts.astype(np.int64))])
 
     def one(r):
         cfg = default_config(32000)
         cfg.initK = int(init_k[r])
-        o = OracleRegion(cfg)
+        o = OracleRegion(cfg, precision)
         o.step(recs[offs[r]:offs[r + 1]], stats[r, 0:3], stats[r, 3:6], float(stats[r, 6]))
         return o.record()
 
@@ -147,8 +147,9 @@ def test_c3_batched_4096_regions():
         assert abs(float(a["weights"][:K].astype(np.float64).sum()) - 1.0) < 1e-5, r
     # every 16th region against the oracle
     sel = list(range(0, len(counts), 16))
-    orc = _oracle_region_records(recs, counts, stats, init_k, sel)
-    assert_mixture_bars(mixture_errors([gpu[r] for r in sel], orc, unit_dirs(512)), "c3 (every 16th region)")
+    o32 = _oracle_region_records(recs, counts, stats, init_k, sel, 0)
+    o64 = _oracle_region_records(recs, counts, stats, init_k, sel, 1)
+    assert_mixture_parity([gpu[r] for r in sel], o32, o64, unit_dirs(512), "c3 (every 16th region)")
     # another schedule (forced large teams) gives the same state for all regions
     b2 = pgl.RegionBatch(dev)
     b2.set_schedule(per_region_launches=0, balance=8.0, max_team=64, min_chunks_per_rank=1)
@@ -167,21 +168,34 @@ def _c1_field_with_volume(dev, knn=True):
     s = generate(200000, seed=0)
     v = generate(200000, seed=1, flags=2)
     gpu.update(pgl.SampleStorage(np.concatenate([s, v])))
-    orc_s, orc_v = OracleField(cfg), OracleField(cfg)
-    orc_s.update(s, threads=THREADS)
-    orc_v.update(v, threads=THREADS)
-    return gpu, orc_s, orc_v
+    orcs = []
+    for prec in (0, 1):
+        orc_s, orc_v = OracleField(cfg, prec), OracleField(cfg, prec)
+        orc_s.update(s, threads=THREADS)
+        orc_v.update(v, threads=THREADS)
+        orcs.append((orc_s, orc_v))
+    return gpu, orcs[0], orcs[1]
 
 
-def test_c4_server_on_c1_field():
+def test_c4_server_on_c1_field(tmp_path):
     import threading
     import openpgl_amd as pgl
     load_hg_table_into_oracle()
     dev = pgl.Device()
-    gpu, orc_s, orc_v = _c1_field_with_volume(dev)
+    gpu, (orc_s, orc_v), o64s = _c1_field_with_volume(dev)
+    # the query path alone: the oracle's c1 surface and volume fields served by the GPU,
     # queries with u1 = -1 (the workload's two randoms) and with a KNN random
-    _query_pdf_parity(gpu, (orc_s, orc_v), 8000, seed=21, knn_u=False)
-    _query_pdf_parity(gpu, (orc_s, orc_v), 8000, seed=22, knn_u=True)
+    served = load_oracle_field_on_gpu(dev, str(tmp_path / "c1sv.pgl"), default_config(32000), 1, 32000,
+                                      surface=orc_s, volume=orc_v)
+    for knn_u, seed in ((False, 21), (True, 22)):
+        rel = _query_rel(served, (orc_s, orc_v), 8000, seed=seed, knn_u=knn_u, canonical=True)[0]
+        print("c4 queries on the oracle's field, knn", knn_u, "median rel", np.median(rel), "max", rel.max(),
+              flush=True)
+        assert np.all(rel <= 1e-3)
+    # end to end on the GPU-trained field
+    for knn_u, seed in ((False, 23), (True, 24)):
+        r32, r64, r3264 = _query_rel(gpu, (orc_s, orc_v), 8000, seed=seed, knn_u=knn_u, orcs64=o64s)
+        assert_query_pdfs(r32, "c4 end to end knn %s" % knn_u, r64, r3264)
     rng = np.random.default_rng(4)
     sizes = np.floor(np.exp(rng.uniform(np.log(1000), np.log(256000), 12))).astype(int)
     reqs = []
@@ -250,9 +264,5 @@ def test_c5_large_mixed_error_against_fp64():
         name = "volume" if kind else "surface"
         report[name + " GPU vs FP64"] = precision_report(g, r64, dirs)
         report[name + " FP32 oracle vs FP64"] = precision_report(r32, r64, dirs)
-        assert_mixture_bars(mixture_errors(g, r32, dirs), "c5 %s GPU vs FP32 oracle" % name)
+        assert_mixture_parity(g, r32, r64, dirs, "c5 %s regions %d" % (name, R))
     print("c5 precision:", json.dumps(report), flush=True)
-    for name in ("surface", "volume"):
-        gpu_rep, o32_rep = report[name + " GPU vs FP64"], report[name + " FP32 oracle vs FP64"]
-        assert gpu_rep["pdf_rel_median"] <= 1e-4
-        assert gpu_rep["pdf_rel_median"] <= 2 * o32_rep["pdf_rel_median"] + 1e-6

