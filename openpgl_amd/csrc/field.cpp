@@ -581,6 +581,7 @@ void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
   hPivots_.assign(R, make_float4(0.f, 0.f, 0.f, 0.f));
   hWork_.clear();
   hBegin_.resize(R);
+  hCount_.resize(R);
   uint32_t begin = 0;
   for (uint32_t r = 0; r < R; r++) {
     const RegionHost& rg = regions_[r];
@@ -595,6 +596,7 @@ void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
     in.initK = 0;
     in.copyFrom = r >= regionsBefore ? rg.copyFrom : -1;  // new regions start from their ancestor's state
     hBegin_[r] = begin;
+    hCount_[r] = rg.count;
     begin += rg.count;
     float var[3];
     rg.ss.variance(var);
@@ -625,17 +627,8 @@ void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
   char* const tab = dTables_[P].as<char>();
   knnPivots_ = (const float4*)(tab + offKnn);
 
-  // region-contiguous order: in deterministic mode every region's samples in
-  // SampleDataLess order (Field.h:409-412, 469-472), else a stable sort of
-  // (region, sample index)
-  if (deterministic_) {
-    dDetTmp_.ensure(det_sort_temp_bytes(n));
-    dDetCounter_.ensure(256);
-    detCounter_.ensure(256);
-    lastDetPasses_ = (uint32_t)det_sort_leaves(dDetTmp_.p, deviceSamples, dNodeOf_.as<uint32_t>(), (const KDNode*)tab,
-                                               R, n, dKeys2_.as<uint32_t>(), dVals2_.as<uint32_t>(),
-                                               dDetCounter_.as<uint32_t>(), detCounter_.as<uint32_t>(), sp);
-  } else if (R <= kSortMaxKeys) {
+  // region-contiguous order: stable sort of (region, sample index)
+  if (R <= kSortMaxKeys) {
     // stable counting sort keyed by the region of each sample's leaf: every
     // region's first position is known from the kd counts
     dCountSortTmp_.ensure(counting_sort_temp_bytes(n, R));
@@ -648,6 +641,14 @@ void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
     while ((1u << endBit) < R && endBit < 32) endBit++;
     sort_pairs(dSortTmp_.p, sortBytes, dKeys_.as<uint32_t>(), dKeys2_.as<uint32_t>(), dVals_.as<uint32_t>(),
                dVals2_.as<uint32_t>(), n, endBit, sp);
+  }
+  // deterministic mode: every region's samples in SampleDataLess order
+  // (Field.h:409-412, 469-472)
+  if (deterministic_) {
+    dDetTmp_.ensure(det_sort_temp_bytes(n, R));
+    detPin_[P].ensure(det_sort_stage_bytes(n, R));
+    det_sort_segments(dDetTmp_.p, detPin_[P].p, detPin_[P].bytes, deviceSamples, hBegin_.data(), hCount_.data(), R,
+                      n, dVals2_.as<uint32_t>(), sp);
   }
   PGL_HIP_CHECK(hipEventRecord(evSp_[P][2], sp));
 

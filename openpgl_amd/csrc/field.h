@@ -190,9 +190,9 @@ class KindField {
   // [6] region begins
   PinnedBuf pin_[2][7];
   DevBuf dCountSortTmp_;
-  DevBuf dDetTmp_, dDetCounter_;  // deterministic-mode sort
-  PinnedBuf detCounter_;
-  uint32_t lastDetPasses_ = 0;
+  DevBuf dDetTmp_;           // deterministic-mode sort
+  PinnedBuf detPin_[2];      // its segment tables, by frame parity
+  std::vector<uint32_t> hCount_;
   std::vector<uint32_t> hBegin_;
   PinnedBuf kdUp_, kdDown_;  // kd rounds (synchronous)
   template <class T> const T* pinned(PinnedBuf& b, const T* src, size_t count) {

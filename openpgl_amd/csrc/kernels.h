@@ -62,21 +62,20 @@ void counting_sort_pairs(void* temp, const uint32_t* keys, const uint32_t* keyBe
 // (the region of its leaf; launch_leaf_keys and the sort in one)
 void counting_sort_leaves(void* temp, const uint32_t* nodeOf, const KDNode* keyNodes, const uint32_t* keyBegin,
                           uint32_t numKeys, uint32_t n, uint32_t* keysOut, uint32_t* valsOut, hipStream_t s);
-// Deterministic-mode order (kernels_sort.hip): region-contiguous positions
-// with every region's samples in SampleDataLess order (data/SampleData.h:82-93),
-// as Field::fitRegions/updateRegions sort them (field/Field.h:409-412, 469-472).
-// records: the n PGLSampleData records; the region of record i is
-// keyNodes[nodeOf[i]].idx() (det_sort_leaves) or keys[i] (det_sort_keys), all
-// < numKeys.  keysOut[j] = region and valsOut[j] = record index of sorted
-// position j.  temp: det_sort_temp_bytes(n) bytes; counterHost: pinned host
-// word.  Synchronises the stream once per pass; returns the number of
-// refinement passes (0 unless two records of a region share their weight).
-size_t det_sort_temp_bytes(uint32_t n);
-int det_sort_leaves(void* temp, const void* records, const uint32_t* nodeOf, const KDNode* keyNodes,
-                    uint32_t numKeys, uint32_t n, uint32_t* keysOut, uint32_t* valsOut, uint32_t* counterDev,
-                    uint32_t* counterHost, hipStream_t s);
-int det_sort_keys(void* temp, const void* records, const uint32_t* keys, uint32_t numKeys, uint32_t n,
-                  uint32_t* keysOut, uint32_t* valsOut, uint32_t* counterDev, uint32_t* counterHost, hipStream_t s);
+// Deterministic-mode order (kernels_sort.hip): every region's range of a
+// region-contiguous index array put in SampleDataLess order
+// (data/SampleData.h:82-93), as Field::fitRegions/updateRegions sort them
+// (field/Field.h:409-412, 469-472).  records: the PGLSampleData records the
+// indices refer to; region r's indices are vals[begin[r], begin[r] +
+// count[r]) (host arrays), sorted in place.  temp: det_sort_temp_bytes(n,
+// numSegments) bytes of device memory; pinnedHost: det_sort_stage_bytes(n,
+// numSegments) bytes of pinned host memory for the segment tables (not
+// reused before the stream passes this point).  No host synchronisation.
+size_t det_sort_temp_bytes(uint32_t n, uint32_t numSegments);
+size_t det_sort_stage_bytes(uint32_t n, uint32_t numSegments);
+void det_sort_segments(void* temp, void* pinnedHost, size_t pinnedBytes, const void* records,
+                       const uint32_t* begin, const uint32_t* count, uint32_t numSegments, uint32_t n,
+                       uint32_t* vals, hipStream_t s);
 // keys/values of region-contiguous records (begin: numRegions ascending offsets)
 void launch_region_keys(const uint32_t* begin, uint32_t numRegions, uint32_t n, uint32_t* keys, uint32_t* vals,
                         hipStream_t s);

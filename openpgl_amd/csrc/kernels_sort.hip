@@ -12,10 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <hipcub/block/block_radix_sort.hpp>
-#include <hipcub/block/block_reduce.hpp>
 #include <hipcub/block/block_scan.hpp>
-#include <hipcub/device/device_radix_sort.hpp>
-#include <hipcub/device/device_scan.hpp>
 
 #include <algorithm>
 
@@ -210,12 +207,20 @@ void counting_sort_leaves(void* temp, const uint32_t* nodeOf, const KDNode* keyN
 // std::sort(SampleDataLess) in deterministic mode (field/Field.h:409-412,
 // 469-472).  SampleDataLess (data/SampleData.h:82-93) is lexicographic over
 // weight, pdf, distance, position x/y/z, direction x/y/z (flags ignored;
-// -0 and +0 compare equal).  On the device: a radix sort of
-// (region << 32 | key(weight)) pairs, then, only while equal keys remain,
-// refinement passes that sort (group << 32 | key(next field)) where group
-// numbers the runs of equal keys so far.  The result is the order of the
-// comparator; records equal in all nine fields differ at most in their flags,
-// which training does not read, so their relative order does not matter.
+// -0 and +0 compare equal).  Records equal in all nine fields differ at most
+// in their flags, which training does not read, so any order of them gives
+// the same result; here they keep their index order (the total order used
+// below is the nine fields, then the record index).
+//
+// Each region's index range is sorted on the device without host
+// round trips:
+//   * ranges of up to 16384 samples in one workgroup: a stable LDS radix sort
+//     by the weight's order key; only if two weights of the range are equal,
+//     stable LDS sorts by every field from the last to the first (least
+//     significant first), which from index order gives the total order;
+//   * longer ranges as sorted blocks of 16384, then merge passes in which
+//     every element finds its place in the partner block by binary search
+//     under the total order.
 namespace {
 
 // order-preserving map of a float to uint32 (-0 mapped to +0)
@@ -225,147 +230,233 @@ __device__ __forceinline__ uint32_t order_key(float x) {
   return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
 }
 
-// float offset inside the 40-byte record of the comparator's i-th field
-__device__ __forceinline__ int less_field(int i) {
-  // weight, pdf, distance, pos.x, pos.y, pos.z, dir.x, dir.y, dir.z
-  return i < 3 ? 6 + i : i - 3;
+// float offset inside the 40-byte record of the comparator's i-th field:
+// weight, pdf, distance, pos.x, pos.y, pos.z, dir.x, dir.y, dir.z
+__device__ __forceinline__ int less_field(int i) { return i < 3 ? 6 + i : i - 3; }
+
+__device__ __forceinline__ uint32_t field_key(const float* __restrict__ recs, uint32_t idx, int field) {
+  return idx == 0xffffffffu ? 0xffffffffu : order_key(recs[(size_t)idx * 10 + less_field(field)]);
 }
 
-__global__ void __launch_bounds__(256) k_det_first(SortKeys keys, const float* __restrict__ recs, uint32_t n,
-                                                   uint64_t* __restrict__ keyOut, uint32_t* __restrict__ valOut) {
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const uint64_t r = keys(i);
-    keyOut[i] = (r << 32) | order_key(recs[(size_t)i * 10 + less_field(0)]);
-    valOut[i] = i;
+// SampleDataLess, then the record index
+__device__ __forceinline__ bool record_less(const float* __restrict__ recs, uint32_t a, uint32_t b) {
+  for (int f = 0; f < 9; f++) {
+    const uint32_t ka = field_key(recs, a, f), kb = field_key(recs, b, f);
+    if (ka != kb) return ka < kb;
   }
+  return a < b;
 }
 
-// head[j] = 1 where a run of equal keys starts; counter += positions that
-// continue a run; regionOut (first pass only) = key >> 32
-__global__ void __launch_bounds__(256) k_det_heads(const uint64_t* __restrict__ key, uint32_t n,
-                                                   uint32_t* __restrict__ head, uint32_t* __restrict__ counter,
-                                                   uint32_t* __restrict__ regionOut) {
-  typedef hipcub::BlockReduce<uint32_t, 256> Reduce;
-  __shared__ typename Reduce::TempStorage tmp;
-  uint32_t ties = 0;
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
-    const uint64_t k = key[j];
-    const bool h = j == 0 || key[j - 1] != k;
-    head[j] = h ? 1u : 0u;
-    ties += h ? 0u : 1u;
-    if (regionOut) regionOut[j] = (uint32_t)(k >> 32);
-  }
-  const uint32_t t = Reduce(tmp).Sum(ties);
-  if (threadIdx.x == 0 && t) atomicAdd(counter, t);
-}
-
-// next pass: (group of equal keys so far << 32) | key(field), same values
-__global__ void __launch_bounds__(256) k_det_refine(const float* __restrict__ recs,
-                                                    const uint32_t* __restrict__ valIn,
-                                                    const uint32_t* __restrict__ groupIncl, int field, uint32_t n,
-                                                    uint64_t* __restrict__ keyOut, uint32_t* __restrict__ valOut) {
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
-    const uint32_t v = valIn[j];
-    const uint64_t g = (uint64_t)(groupIncl[j] - 1u);
-    keyOut[j] = (g << 32) | order_key(recs[(size_t)v * 10 + less_field(field)]);
-    valOut[j] = v;
-  }
-}
-
-struct DetLayout {
-  size_t key[2], val, head, incl, sortTmp, scanTmp, sortBytes, scanBytes, total;
+struct DetSegment {
+  uint32_t begin, count;
 };
 
-DetLayout det_layout(uint32_t n) {
-  auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
-  DetLayout L;
-  size_t sortBytes = 0, scanBytes = 0;
-  hipcub::DoubleBuffer<uint64_t> kb(nullptr, nullptr);
-  hipcub::DoubleBuffer<uint32_t> vb(nullptr, nullptr);
-  PGL_KERNEL_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, sortBytes, kb, vb, (int)n, 0, 64));
-  PGL_KERNEL_CHECK(hipcub::DeviceScan::InclusiveSum(nullptr, scanBytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                                    (int)n));
-  size_t off = 0;
-  L.key[0] = off; off += up(size_t(n) * 8);
-  L.key[1] = off; off += up(size_t(n) * 8);
-  L.val = off; off += up(size_t(n) * 4);
-  L.head = off; off += up(size_t(n) * 4);
-  L.incl = off; off += up(size_t(n) * 4);
-  L.sortTmp = off; off += up(sortBytes);
-  L.scanTmp = off; off += up(scanBytes);
-  L.sortBytes = sortBytes;
-  L.scanBytes = scanBytes;
-  L.total = off;
-  return L;
-}
+constexpr int kDetLocalRun = 32;  // longer runs of equal weights: full sort by every field
 
-int bits_for(uint64_t count) {
-  int b = 0;
-  while (b < 32 && (1ull << b) < count) b++;
-  return b;
-}
-
-// the sort of both entry points; returns the number of refinement passes
-int det_sort(void* temp, SortKeys keys, const void* records, uint32_t numKeys, uint32_t n, uint32_t* keysOut,
-             uint32_t* valsOut, uint32_t* counterDev, uint32_t* counterHost, hipStream_t s) {
-  if (n == 0) return 0;
-  const DetLayout L = det_layout(n);
-  char* t = (char*)temp;
-  uint64_t* kA = (uint64_t*)(t + L.key[0]);
-  uint64_t* kB = (uint64_t*)(t + L.key[1]);
-  uint32_t* vB = (uint32_t*)(t + L.val);
-  uint32_t* head = (uint32_t*)(t + L.head);
-  uint32_t* incl = (uint32_t*)(t + L.incl);
-  const float* recs = (const float*)records;
-  const uint32_t g = std::min<uint32_t>((n + 255) / 256, 4096);
-  hipLaunchKernelGGL(k_det_first, dim3(g), dim3(256), 0, s, keys, recs, n, kA, valsOut);
-  hipcub::DoubleBuffer<uint64_t> K(kA, kB);
-  hipcub::DoubleBuffer<uint32_t> V(valsOut, vB);
-  size_t sortBytes = L.sortBytes;
-  PGL_KERNEL_CHECK(hipcub::DeviceRadixSort::SortPairs(t + L.sortTmp, sortBytes, K, V, (int)n, 0,
-                                                      32 + bits_for(numKeys), s));
-  int passes = 0;
-  for (int field = 0;; field++) {
-    PGL_KERNEL_CHECK(hipMemsetAsync(counterDev, 0, sizeof(uint32_t), s));
-    hipLaunchKernelGGL(k_det_heads, dim3(g), dim3(256), 0, s, K.Current(), n, head, counterDev,
-                       field == 0 ? keysOut : nullptr);
-    PGL_KERNEL_CHECK(hipMemcpyAsync(counterHost, counterDev, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    PGL_KERNEL_CHECK(hipStreamSynchronize(s));
-    const uint32_t ties = *counterHost;
-    if (ties == 0 || field == 8) break;
-    size_t scanBytes = L.scanBytes;
-    PGL_KERNEL_CHECK(hipcub::DeviceScan::InclusiveSum(t + L.scanTmp, scanBytes, head, incl, (int)n, s));
-    hipLaunchKernelGGL(k_det_refine, dim3(g), dim3(256), 0, s, recs, V.Current(), incl, field + 1, n,
-                       K.Alternate(), V.Alternate());
-    K.selector ^= 1;
-    V.selector ^= 1;
-    PGL_KERNEL_CHECK(hipcub::DeviceRadixSort::SortPairs(t + L.sortTmp, sortBytes, K, V, (int)n, 0,
-                                                        32 + bits_for(uint64_t(n) - ties), s));
-    passes++;
+template <int THREADS, int ITEMS>
+__global__ void __launch_bounds__(THREADS) k_det_block_sort(const float* __restrict__ recs,
+                                                          const DetSegment* __restrict__ segs,
+                                                          uint32_t* __restrict__ vals) {
+  typedef hipcub::BlockRadixSort<uint32_t, THREADS, ITEMS, uint32_t> Sort;
+  constexpr int CAP = THREADS * ITEMS;
+  __shared__ union {
+    typename Sort::TempStorage sort;
+    uint32_t idx[CAP];
+  } sh;
+  __shared__ uint32_t sKey[CAP];
+  __shared__ int longRun;
+  __shared__ uint32_t keyAnd, keyOr;
+  const DetSegment sg = segs[blockIdx.x];
+  const uint32_t t = threadIdx.x;
+  uint32_t key[ITEMS], idx[ITEMS];
+#pragma unroll
+  for (int j = 0; j < ITEMS; j++) {
+    const uint32_t pos = t * ITEMS + j;
+    idx[j] = pos < sg.count ? vals[sg.begin + pos] : 0xffffffffu;
+    key[j] = field_key(recs, idx[j], 0);
   }
-  if (V.Current() != valsOut)
-    PGL_KERNEL_CHECK(hipMemcpyAsync(valsOut, V.Current(), size_t(n) * 4, hipMemcpyDeviceToDevice, s));
-  return passes;
+  if (t == 0) {
+    longRun = 0;
+    keyAnd = 0xffffffffu;
+    keyOr = 0u;
+  }
+  __syncthreads();
+  // only the low bits up to the highest bit in which the range's weight keys
+  // differ are sorted (the padding's all-ones keys still sort last: the sort
+  // is stable and the padding comes last)
+  uint32_t ka = 0xffffffffu, ko = 0u;
+#pragma unroll
+  for (int j = 0; j < ITEMS; j++)
+    if (t * ITEMS + j < sg.count) {
+      ka &= key[j];
+      ko |= key[j];
+    }
+  for (int off = 32; off > 0; off >>= 1) {
+    ka &= __shfl_xor(ka, off);
+    ko |= __shfl_xor(ko, off);
+  }
+  if ((t & 63) == 0) {
+    atomicAnd(&keyAnd, ka);
+    atomicOr(&keyOr, ko);
+  }
+  __syncthreads();
+  const uint32_t differ = keyAnd ^ keyOr;
+  const int endBit = differ ? 32 - __clz(differ) : 1;
+  Sort(sh.sort).Sort(key, idx, 0, endBit);  // blocked arrangement in and out; stable
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < ITEMS; j++) {
+    sKey[t * ITEMS + j] = key[j];
+    sh.idx[t * ITEMS + j] = idx[j];
+  }
+  __syncthreads();
+  // runs of equal weights (rare for continuous weights): short runs put in
+  // order by the thread owning their first element, with the full comparator
+  for (int j = 0; j < ITEMS; j++) {
+    const uint32_t p = t * ITEMS + j;
+    if (p + 1 >= sg.count || sKey[p + 1] != sKey[p] || (p > 0 && sKey[p - 1] == sKey[p])) continue;
+    uint32_t L = 2;
+    while (p + L < sg.count && sKey[p + L] == sKey[p] && L <= kDetLocalRun) L++;
+    if (L > kDetLocalRun) {
+      longRun = 1;
+      continue;
+    }
+    for (uint32_t a = 1; a < L; a++) {  // insertion sort of sh.idx[p, p + L)
+      const uint32_t v = sh.idx[p + a];
+      uint32_t b = a;
+      while (b > 0 && record_less(recs, v, sh.idx[p + b - 1])) {
+        sh.idx[p + b] = sh.idx[p + b - 1];
+        b--;
+      }
+      sh.idx[p + b] = v;
+    }
+  }
+  __syncthreads();
+  if (longRun) {  // uniform over the block: stable sorts by every field, least significant first
+    for (int f = 8; f >= 0; f--) {
+#pragma unroll
+      for (int j = 0; j < ITEMS; j++) key[j] = field_key(recs, idx[j], f);
+      __syncthreads();
+      Sort(sh.sort).Sort(key, idx);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) sh.idx[t * ITEMS + j] = idx[j];
+    __syncthreads();
+  }
+  for (uint32_t p = t; p < sg.count; p += THREADS) vals[sg.begin + p] = sh.idx[p];
+}
+
+// one merge pass over the segments of the table (blockIdx.y): runs
+// [k*2w, k*2w + w) and [k*2w + w, (k+1)*2w) of each segment are merged (a
+// segment of at most w elements is copied); every element binary-searches
+// its rank in the partner run under the total order
+__global__ void __launch_bounds__(256) k_det_merge(const float* __restrict__ recs, const DetSegment* __restrict__ segs,
+                                                   uint32_t w, const uint32_t* __restrict__ inBase,
+                                                   uint32_t* __restrict__ outBase) {
+  const DetSegment sg = segs[blockIdx.y];
+  const uint32_t n = sg.count;
+  const uint32_t* __restrict__ in = inBase + sg.begin;
+  uint32_t* __restrict__ out = outBase + sg.begin;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint32_t pair = i / (2 * w), start = pair * 2 * w;
+    const uint32_t aEnd = min(start + w, n), bEnd = min(start + 2 * w, n);
+    const uint32_t v = in[i];
+    uint32_t lo, hi, rankBase;
+    if (i < aEnd) {  // element of run A: the B elements before it
+      lo = aEnd;
+      hi = bEnd;
+      rankBase = i - start;
+    } else {  // element of run B: the A elements before it
+      lo = start;
+      hi = aEnd;
+      rankBase = i - aEnd;
+    }
+    const uint32_t first = lo;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (record_less(recs, in[mid], v)) lo = mid + 1;
+      else hi = mid;
+    }
+    out[start + rankBase + (lo - first)] = v;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_det_copy(const DetSegment* __restrict__ segs, const uint32_t* __restrict__ in,
+                                                  uint32_t* __restrict__ out) {
+  const DetSegment sg = segs[blockIdx.y];
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < sg.count; i += stride)
+    out[sg.begin + i] = in[sg.begin + i];
 }
 
 }  // namespace
 
-size_t det_sort_temp_bytes(uint32_t n) { return det_layout(std::max<uint32_t>(n, 1)).total; }
+constexpr uint32_t kDetSmall = 256 * 4, kDetMid = 256 * 16, kDetBig = 512 * 16, kDetLarge = 512 * 32;
+constexpr int kDetTables = 5;
 
-int det_sort_leaves(void* temp, const void* records, const uint32_t* nodeOf, const KDNode* keyNodes,
-                    uint32_t numKeys, uint32_t n, uint32_t* keysOut, uint32_t* valsOut, uint32_t* counterDev,
-                    uint32_t* counterHost, hipStream_t s) {
-  return det_sort(temp, SortKeys{nullptr, nodeOf, keyNodes}, records, numKeys, n, keysOut, valsOut, counterDev,
-                  counterHost, s);
+static size_t det_table_entries(uint32_t n, uint32_t numSegments) { return (size_t)numSegments + n / kDetLarge + 3; }
+
+size_t det_sort_stage_bytes(uint32_t n, uint32_t numSegments) {
+  return det_table_entries(n, numSegments) * kDetTables * sizeof(DetSegment);
 }
 
-int det_sort_keys(void* temp, const void* records, const uint32_t* keys, uint32_t numKeys, uint32_t n,
-                  uint32_t* keysOut, uint32_t* valsOut, uint32_t* counterDev, uint32_t* counterHost, hipStream_t s) {
-  return det_sort(temp, SortKeys{keys, nullptr, nullptr}, records, numKeys, n, keysOut, valsOut, counterDev,
-                  counterHost, s);
+size_t det_sort_temp_bytes(uint32_t n, uint32_t numSegments) {
+  // the segment tables and the ping-pong index buffer of the merge passes
+  return ((det_sort_stage_bytes(n, numSegments) + 255) & ~size_t(255)) + size_t(n) * 4 + 256;
+}
+
+void det_sort_segments(void* temp, void* pinnedHost, size_t pinnedBytes, const void* records,
+                       const uint32_t* begin, const uint32_t* count, uint32_t numSegments, uint32_t n,
+                       uint32_t* vals, hipStream_t s) {
+  if (n == 0 || numSegments == 0) return;
+  const float* recs = (const float*)records;
+  // tables: 0 ranges <= 1024, 1 <= 4096, 2 <= 8192, 3 blocks of <= 16384
+  // (ranges up to 16384 and the blocks of longer ranges), 4 ranges longer
+  // than 16384
+  const size_t cap = det_table_entries(n, numSegments);
+  if (pinnedBytes < det_sort_stage_bytes(n, numSegments))
+    throw std::runtime_error("det_sort_segments: staging too small");
+  DetSegment* host = (DetSegment*)pinnedHost;
+  DetSegment* cls[kDetTables];
+  for (int c = 0; c < kDetTables; c++) cls[c] = host + c * cap;
+  uint32_t num[kDetTables] = {0, 0, 0, 0, 0};
+  uint32_t longest = 0;
+  for (uint32_t r = 0; r < numSegments; r++) {
+    const uint32_t c = count[r];
+    if (c < 2) continue;
+    if (c <= kDetSmall) {
+      cls[0][num[0]++] = DetSegment{begin[r], c};
+    } else if (c <= kDetMid) {
+      cls[1][num[1]++] = DetSegment{begin[r], c};
+    } else if (c <= kDetBig) {
+      cls[2][num[2]++] = DetSegment{begin[r], c};
+    } else {
+      for (uint32_t o = 0; o < c; o += kDetLarge) cls[3][num[3]++] = DetSegment{begin[r] + o, std::min(kDetLarge, c - o)};
+      if (c > kDetLarge) {
+        cls[4][num[4]++] = DetSegment{begin[r], c};
+        longest = std::max(longest, c);
+      }
+    }
+  }
+  DetSegment* dev = (DetSegment*)temp;
+  PGL_KERNEL_CHECK(hipMemcpyAsync(dev, host, det_sort_stage_bytes(n, numSegments), hipMemcpyHostToDevice, s));
+  if (num[0]) hipLaunchKernelGGL((k_det_block_sort<256, 4>), dim3(num[0]), dim3(256), 0, s, recs, dev, vals);
+  if (num[1]) hipLaunchKernelGGL((k_det_block_sort<256, 16>), dim3(num[1]), dim3(256), 0, s, recs, dev + cap, vals);
+  if (num[2]) hipLaunchKernelGGL((k_det_block_sort<512, 16>), dim3(num[2]), dim3(512), 0, s, recs, dev + 2 * cap, vals);
+  if (num[3]) hipLaunchKernelGGL((k_det_block_sort<512, 32>), dim3(num[3]), dim3(512), 0, s, recs, dev + 3 * cap, vals);
+  if (num[4] == 0) return;
+  // merge passes over all ranges longer than one block, ping-pong through the temp buffer
+  uint32_t* tmpIdx =
+      (uint32_t*)((char*)temp + ((det_sort_stage_bytes(n, numSegments) + 255) & ~size_t(255)));
+  const dim3 grid(std::min<uint32_t>((longest + 255) / 256, 256), num[4]);
+  uint32_t *a = vals, *b = tmpIdx;
+  for (uint32_t w = kDetLarge; w < longest; w *= 2) {
+    hipLaunchKernelGGL(k_det_merge, grid, dim3(256), 0, s, recs, dev + 4 * cap, w, a, b);
+    std::swap(a, b);
+  }
+  if (a != vals) hipLaunchKernelGGL(k_det_copy, grid, dim3(256), 0, s, dev + 4 * cap, a, vals);
 }
 
 }  // namespace pglgpu

@@ -74,16 +74,11 @@ void RegionBatch::step(uint32_t R, const void* records, const uint32_t* counts, 
   launch_region_keys(dBegin_.as<uint32_t>(), R, n, dKeys_.as<uint32_t>(), dVals_.as<uint32_t>(), s);
   const uint32_t* gatherKeys = dKeys_.as<uint32_t>();
   const uint32_t* gatherIdx = dVals_.as<uint32_t>();
-  if (deterministic_ && n > 0) {
-    dSortedKeys_.ensure(size_t(n) * 4);
-    dSortedVals_.ensure(size_t(n) * 4);
-    dDetTmp_.ensure(det_sort_temp_bytes(n));
-    dDetCounter_.ensure(256);
-    detCounter_.ensure(256);
-    det_sort_keys(dDetTmp_.p, records, dKeys_.as<uint32_t>(), R, n, dSortedKeys_.as<uint32_t>(),
-                  dSortedVals_.as<uint32_t>(), dDetCounter_.as<uint32_t>(), detCounter_.as<uint32_t>(), s);
-    gatherKeys = dSortedKeys_.as<uint32_t>();
-    gatherIdx = dSortedVals_.as<uint32_t>();
+  if (deterministic_ && n > 0) {  // every region's samples in SampleDataLess order (Field.h:409-412, 469-472)
+    dDetTmp_.ensure(det_sort_temp_bytes(n, R));
+    detPin_.ensure(det_sort_stage_bytes(n, R));
+    det_sort_segments(dDetTmp_.p, detPin_.p, detPin_.bytes, records, begin.data(), counts, R, n, dVals_.as<uint32_t>(),
+                      s);
   }
   launch_gather_reproject(records, gatherKeys, gatherIdx, n, dPivots_.as<float4>(),
                           dDx_.as<float>(), dDy_.as<float>(), dDz_.as<float>(), dW_.as<float>(), dPdf_.as<float>(),

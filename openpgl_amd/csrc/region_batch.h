@@ -49,8 +49,8 @@ class RegionBatch {
   std::vector<RegionWork> lastWork_;
   DevBuf dStates_, dSnapshot_, dInputs_, dPivots_, dBegin_, dKeys_, dVals_;
   DevBuf dDx_, dDy_, dDz_, dW_, dPdf_, dDist_, dUniformDirs_;
-  DevBuf dSortedKeys_, dSortedVals_, dDetTmp_, dDetCounter_;
-  PinnedBuf detCounter_;
+  DevBuf dDetTmp_;
+  PinnedBuf detPin_;
   std::vector<RegionState> hostStates_;
   UpdateInfo info_;
   hipEvent_t ev_[2];
