@@ -556,7 +556,12 @@ void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
 
   // ---------------------------------------------------------------- spatial phase
   PGL_HIP_CHECK(hipEventRecord(evSp_[P][0], sp));
-  launch_ingest(deviceSamples, n, dPx_.as<float>(), dPy_.as<float>(), dPz_.as<float>(), dNodeOf_.as<uint32_t>(), sp);
+  if (deterministic_) {
+    dWkey_.ensure(n4);
+    dWkeySorted_.ensure(n4);
+  }
+  launch_ingest(deviceSamples, n, dPx_.as<float>(), dPy_.as<float>(), dPz_.as<float>(),
+                deterministic_ ? dWkey_.as<uint32_t>() : nullptr, sp);
   const bool first = !initialized_;
   if (first) {
     if (!boundsSet_) estimateSceneBounds(dPx_.as<float>(), dPy_.as<float>(), dPz_.as<float>(), n, sp);
@@ -633,7 +638,9 @@ void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
     // region's first position is known from the kd counts
     dCountSortTmp_.ensure(counting_sort_temp_bytes(n, R));
     counting_sort_leaves(dCountSortTmp_.p, dNodeOf_.as<uint32_t>(), (const KDNode*)tab,
-                         (const uint32_t*)(tab + offBegin), R, n, dKeys2_.as<uint32_t>(), dVals2_.as<uint32_t>(), sp);
+                         (const uint32_t*)(tab + offBegin), R, n, dKeys2_.as<uint32_t>(), dVals2_.as<uint32_t>(), sp,
+                         deterministic_ ? dWkey_.as<uint32_t>() : nullptr,
+                         deterministic_ ? dWkeySorted_.as<uint32_t>() : nullptr);
   } else {
     launch_leaf_keys(dNodeOf_.as<uint32_t>(), (const KDNode*)tab, n, dKeys_.as<uint32_t>(), dVals_.as<uint32_t>(),
                      sp);
@@ -647,8 +654,10 @@ void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
   if (deterministic_) {
     dDetTmp_.ensure(det_sort_temp_bytes(n, R));
     detPin_[P].ensure(det_sort_stage_bytes(n, R));
+    // the weight keys travelled with the counting sort (above 4096 regions
+    // the tiles read the weights from the records)
     det_sort_segments(dDetTmp_.p, detPin_[P].p, detPin_[P].bytes, deviceSamples, hBegin_.data(), hCount_.data(), R,
-                      n, dVals2_.as<uint32_t>(), sp);
+                      n, dVals2_.as<uint32_t>(), R <= kSortMaxKeys ? dWkeySorted_.as<uint32_t>() : nullptr, sp);
   }
   PGL_HIP_CHECK(hipEventRecord(evSp_[P][2], sp));
 

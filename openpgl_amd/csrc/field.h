@@ -191,6 +191,7 @@ class KindField {
   PinnedBuf pin_[2][7];
   DevBuf dCountSortTmp_;
   DevBuf dDetTmp_;           // deterministic-mode sort
+  DevBuf dWkey_, dWkeySorted_;  // its weight keys in input and in region order
   PinnedBuf detPin_[2];      // its segment tables, by frame parity
   std::vector<uint32_t> hCount_;
   std::vector<uint32_t> hBegin_;

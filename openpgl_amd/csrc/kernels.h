@@ -34,7 +34,8 @@ struct QueryField {
 };
 
 // spatial
-void launch_ingest(const void* aos, uint32_t n, float* px, float* py, float* pz, uint32_t* nodeOf, hipStream_t s);
+// positions to SoA; wkey (optional): order key of every record's weight (deterministic mode)
+void launch_ingest(const void* aos, uint32_t n, float* px, float* py, float* pz, uint32_t* wkey, hipStream_t s);
 uint32_t launch_bounds(const float* px, const float* py, const float* pz, uint32_t n, float* partial, hipStream_t s);
 // leafParams: per leaf slot {center[3], invHalf[3]} of the leaf's parent bounds.
 // partial: scratch of spatial_partial_bytes(n) bytes for the per-block tables.
@@ -59,9 +60,11 @@ size_t counting_sort_temp_bytes(uint32_t n, uint32_t numKeys);
 void counting_sort_pairs(void* temp, const uint32_t* keys, const uint32_t* keyBegin, uint32_t numKeys, uint32_t n,
                          uint32_t* keysOut, uint32_t* valsOut, hipStream_t s);
 // the same with the key of sample i computed as keyNodes[nodeOf[i]].idx()
-// (the region of its leaf; launch_leaf_keys and the sort in one)
+// (the region of its leaf; launch_leaf_keys and the sort in one); with
+// carryIn (optional), carryOut[j] = carryIn[valsOut[j]] is written as well
 void counting_sort_leaves(void* temp, const uint32_t* nodeOf, const KDNode* keyNodes, const uint32_t* keyBegin,
-                          uint32_t numKeys, uint32_t n, uint32_t* keysOut, uint32_t* valsOut, hipStream_t s);
+                          uint32_t numKeys, uint32_t n, uint32_t* keysOut, uint32_t* valsOut, hipStream_t s,
+                          const uint32_t* carryIn = nullptr, uint32_t* carryOut = nullptr);
 // Deterministic-mode order (kernels_sort.hip): every region's range of a
 // region-contiguous index array put in SampleDataLess order
 // (data/SampleData.h:82-93), as Field::fitRegions/updateRegions sort them
@@ -73,9 +76,12 @@ void counting_sort_leaves(void* temp, const uint32_t* nodeOf, const KDNode* keyN
 // reused before the stream passes this point).  No host synchronisation.
 size_t det_sort_temp_bytes(uint32_t n, uint32_t numSegments);
 size_t det_sort_stage_bytes(uint32_t n, uint32_t numSegments);
+// weightKeys (optional): float_order_key of the weight of the record vals[j]
+// at position j (in the same layout as vals; overwritten); null: read from
+// the records.
 void det_sort_segments(void* temp, void* pinnedHost, size_t pinnedBytes, const void* records,
                        const uint32_t* begin, const uint32_t* count, uint32_t numSegments, uint32_t n,
-                       uint32_t* vals, hipStream_t s);
+                       uint32_t* vals, uint32_t* weightKeys, hipStream_t s);
 // keys/values of region-contiguous records (begin: numRegions ascending offsets)
 void launch_region_keys(const uint32_t* begin, uint32_t numRegions, uint32_t n, uint32_t* keys, uint32_t* vals,
                         hipStream_t s);

@@ -17,6 +17,7 @@
 #include <algorithm>
 
 #include "kernels.h"
+#include "sample_gather.h"
 
 namespace pglgpu {
 
@@ -80,12 +81,16 @@ struct MaxOp {
 };
 
 // element (key k, index i) at sorted position dst: the (key, index) pair
+// and, with carryIn, carryIn[i]
 struct PairOut {
   uint32_t* __restrict__ keysOut;
   uint32_t* __restrict__ valsOut;
+  const uint32_t* __restrict__ carryIn;
+  uint32_t* __restrict__ carryOut;
   __device__ __forceinline__ void operator()(uint32_t dst, uint32_t k, uint32_t i) const {
     keysOut[dst] = k;
     valsOut[dst] = i;
+    if (carryIn) carryOut[dst] = carryIn[i];
   }
 };
 
@@ -194,12 +199,15 @@ static void counting_sort(void* temp, SortKeys keys, const uint32_t* keyBegin, u
 
 void counting_sort_pairs(void* temp, const uint32_t* keys, const uint32_t* keyBegin, uint32_t numKeys, uint32_t n,
                          uint32_t* keysOut, uint32_t* valsOut, hipStream_t s) {
-  counting_sort(temp, SortKeys{keys, nullptr, nullptr}, keyBegin, numKeys, n, PairOut{keysOut, valsOut}, s);
+  counting_sort(temp, SortKeys{keys, nullptr, nullptr}, keyBegin, numKeys, n,
+                PairOut{keysOut, valsOut, nullptr, nullptr}, s);
 }
 
 void counting_sort_leaves(void* temp, const uint32_t* nodeOf, const KDNode* keyNodes, const uint32_t* keyBegin,
-                          uint32_t numKeys, uint32_t n, uint32_t* keysOut, uint32_t* valsOut, hipStream_t s) {
-  counting_sort(temp, SortKeys{nullptr, nodeOf, keyNodes}, keyBegin, numKeys, n, PairOut{keysOut, valsOut}, s);
+                          uint32_t numKeys, uint32_t n, uint32_t* keysOut, uint32_t* valsOut, hipStream_t s,
+                          const uint32_t* carryIn, uint32_t* carryOut) {
+  counting_sort(temp, SortKeys{nullptr, nodeOf, keyNodes}, keyBegin, numKeys, n,
+                PairOut{keysOut, valsOut, carryIn, carryOut}, s);
 }
 
 // ------------------------------------------------------------ deterministic order
@@ -212,35 +220,31 @@ void counting_sort_leaves(void* temp, const uint32_t* nodeOf, const KDNode* keyN
 // the same result; here they keep their index order (the total order used
 // below is the nine fields, then the record index).
 //
-// Each region's index range is sorted on the device without host
-// round trips:
-//   * ranges of up to 16384 samples in one workgroup: a stable LDS radix sort
-//     by the weight's order key; only if two weights of the range are equal,
-//     stable LDS sorts by every field from the last to the first (least
-//     significant first), which from index order gives the total order;
-//   * longer ranges as sorted blocks of 16384, then merge passes in which
-//     every element finds its place in the partner block by binary search
-//     under the total order.
+// Each region's index range is sorted on the device without host round
+// trips, by kernels small enough to share the CUs with the region kernel of
+// the previous update (the spatial phase of an update overlaps it):
+//   * tiles of up to 2048 samples (1024 for short ranges) in one workgroup: a
+//     stable LDS radix sort by the weight's order key over the bits in which
+//     the tile's keys differ; runs of equal weights (rare for continuous
+//     weights) are put in order with the full comparator by one thread each,
+//     and a run longer than 32 makes the tile sort by every field, least
+//     significant first (from index order that gives the total order);
+//   * ranges longer than one tile: merge passes in which every element finds
+//     its place in the partner run by binary search, on the weight keys and
+//     the full comparator where the weights are equal.
 namespace {
-
-// order-preserving map of a float to uint32 (-0 mapped to +0)
-__device__ __forceinline__ uint32_t order_key(float x) {
-  uint32_t b = __float_as_uint(x);
-  if (b == 0x80000000u) b = 0u;
-  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
-}
 
 // float offset inside the 40-byte record of the comparator's i-th field:
 // weight, pdf, distance, pos.x, pos.y, pos.z, dir.x, dir.y, dir.z
 __device__ __forceinline__ int less_field(int i) { return i < 3 ? 6 + i : i - 3; }
 
 __device__ __forceinline__ uint32_t field_key(const float* __restrict__ recs, uint32_t idx, int field) {
-  return idx == 0xffffffffu ? 0xffffffffu : order_key(recs[(size_t)idx * 10 + less_field(field)]);
+  return idx == 0xffffffffu ? 0xffffffffu : float_order_key(recs[(size_t)idx * 10 + less_field(field)]);
 }
 
-// SampleDataLess, then the record index
-__device__ __forceinline__ bool record_less(const float* __restrict__ recs, uint32_t a, uint32_t b) {
-  for (int f = 0; f < 9; f++) {
+// SampleDataLess after the weight (fields 1..8), then the record index
+__device__ __forceinline__ bool record_less_after_weight(const float* __restrict__ recs, uint32_t a, uint32_t b) {
+  for (int f = 1; f < 9; f++) {
     const uint32_t ka = field_key(recs, a, f), kb = field_key(recs, b, f);
     if (ka != kb) return ka < kb;
   }
@@ -251,12 +255,13 @@ struct DetSegment {
   uint32_t begin, count;
 };
 
-constexpr int kDetLocalRun = 32;  // longer runs of equal weights: full sort by every field
+constexpr int kDetLocalRun = 32;  // longer runs of equal weights: the tile sorts by every field
 
 template <int THREADS, int ITEMS>
-__global__ void __launch_bounds__(THREADS) k_det_block_sort(const float* __restrict__ recs,
-                                                          const DetSegment* __restrict__ segs,
-                                                          uint32_t* __restrict__ vals) {
+__global__ void __launch_bounds__(THREADS) k_det_tile_sort(const float* __restrict__ recs,
+                                                         const DetSegment* __restrict__ segs,
+                                                         uint32_t* __restrict__ vals, const uint32_t* __restrict__ keysIn,
+                                                         uint32_t* __restrict__ keysOut) {
   typedef hipcub::BlockRadixSort<uint32_t, THREADS, ITEMS, uint32_t> Sort;
   constexpr int CAP = THREADS * ITEMS;
   __shared__ union {
@@ -273,7 +278,16 @@ __global__ void __launch_bounds__(THREADS) k_det_block_sort(const float* __restr
   for (int j = 0; j < ITEMS; j++) {
     const uint32_t pos = t * ITEMS + j;
     idx[j] = pos < sg.count ? vals[sg.begin + pos] : 0xffffffffu;
-    key[j] = field_key(recs, idx[j], 0);
+  }
+  if (keysIn) {
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+      const uint32_t pos = t * ITEMS + j;
+      key[j] = pos < sg.count ? keysIn[sg.begin + pos] : 0xffffffffu;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) key[j] = field_key(recs, idx[j], 0);
   }
   if (t == 0) {
     longRun = 0;
@@ -281,7 +295,7 @@ __global__ void __launch_bounds__(THREADS) k_det_block_sort(const float* __restr
     keyOr = 0u;
   }
   __syncthreads();
-  // only the low bits up to the highest bit in which the range's weight keys
+  // only the low bits up to the highest bit in which the tile's weight keys
   // differ are sorted (the padding's all-ones keys still sort last: the sort
   // is stable and the padding comes last)
   uint32_t ka = 0xffffffffu, ko = 0u;
@@ -310,8 +324,7 @@ __global__ void __launch_bounds__(THREADS) k_det_block_sort(const float* __restr
     sh.idx[t * ITEMS + j] = idx[j];
   }
   __syncthreads();
-  // runs of equal weights (rare for continuous weights): short runs put in
-  // order by the thread owning their first element, with the full comparator
+  // runs of equal weights: short runs put in order by the thread owning their first element
   for (int j = 0; j < ITEMS; j++) {
     const uint32_t p = t * ITEMS + j;
     if (p + 1 >= sg.count || sKey[p + 1] != sKey[p] || (p > 0 && sKey[p - 1] == sKey[p])) continue;
@@ -324,7 +337,7 @@ __global__ void __launch_bounds__(THREADS) k_det_block_sort(const float* __restr
     for (uint32_t a = 1; a < L; a++) {  // insertion sort of sh.idx[p, p + L)
       const uint32_t v = sh.idx[p + a];
       uint32_t b = a;
-      while (b > 0 && record_less(recs, v, sh.idx[p + b - 1])) {
+      while (b > 0 && record_less_after_weight(recs, v, sh.idx[p + b - 1])) {
         sh.idx[p + b] = sh.idx[p + b - 1];
         b--;
       }
@@ -344,25 +357,30 @@ __global__ void __launch_bounds__(THREADS) k_det_block_sort(const float* __restr
     for (int j = 0; j < ITEMS; j++) sh.idx[t * ITEMS + j] = idx[j];
     __syncthreads();
   }
-  for (uint32_t p = t; p < sg.count; p += THREADS) vals[sg.begin + p] = sh.idx[p];
+  for (uint32_t p = t; p < sg.count; p += THREADS) {
+    vals[sg.begin + p] = sh.idx[p];
+    keysOut[sg.begin + p] = sKey[p];
+  }
 }
 
-// one merge pass over the segments of the table (blockIdx.y): runs
+// one merge pass over the segments of the table (blockIdx.y): sorted runs
 // [k*2w, k*2w + w) and [k*2w + w, (k+1)*2w) of each segment are merged (a
 // segment of at most w elements is copied); every element binary-searches
-// its rank in the partner run under the total order
+// its rank in the partner run: weight keys first, the full comparator where
+// they are equal
 __global__ void __launch_bounds__(256) k_det_merge(const float* __restrict__ recs, const DetSegment* __restrict__ segs,
-                                                   uint32_t w, const uint32_t* __restrict__ inBase,
-                                                   uint32_t* __restrict__ outBase) {
+                                                   uint32_t w, const uint32_t* __restrict__ idxIn,
+                                                   const uint32_t* __restrict__ keyIn, uint32_t* __restrict__ idxOut,
+                                                   uint32_t* __restrict__ keyOut) {
   const DetSegment sg = segs[blockIdx.y];
   const uint32_t n = sg.count;
-  const uint32_t* __restrict__ in = inBase + sg.begin;
-  uint32_t* __restrict__ out = outBase + sg.begin;
+  const uint32_t* __restrict__ inI = idxIn + sg.begin;
+  const uint32_t* __restrict__ inK = keyIn + sg.begin;
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const uint32_t pair = i / (2 * w), start = pair * 2 * w;
     const uint32_t aEnd = min(start + w, n), bEnd = min(start + 2 * w, n);
-    const uint32_t v = in[i];
+    const uint32_t v = inI[i], k = inK[i];
     uint32_t lo, hi, rankBase;
     if (i < aEnd) {  // element of run A: the B elements before it
       lo = aEnd;
@@ -376,10 +394,14 @@ __global__ void __launch_bounds__(256) k_det_merge(const float* __restrict__ rec
     const uint32_t first = lo;
     while (lo < hi) {
       const uint32_t mid = (lo + hi) >> 1;
-      if (record_less(recs, in[mid], v)) lo = mid + 1;
+      const uint32_t km = inK[mid];
+      const bool less = km < k || (km == k && record_less_after_weight(recs, inI[mid], v));
+      if (less) lo = mid + 1;
       else hi = mid;
     }
-    out[start + rankBase + (lo - first)] = v;
+    const uint32_t o = sg.begin + start + rankBase + (lo - first);
+    idxOut[o] = v;
+    keyOut[o] = k;
   }
 }
 
@@ -393,70 +415,72 @@ __global__ void __launch_bounds__(256) k_det_copy(const DetSegment* __restrict__
 
 }  // namespace
 
-constexpr uint32_t kDetSmall = 256 * 4, kDetMid = 256 * 16, kDetBig = 512 * 16, kDetLarge = 512 * 32;
-constexpr int kDetTables = 5;
+constexpr uint32_t kDetSmall = 256 * 4, kDetTile = 256 * 8;
+constexpr int kDetTables = 3;
 
-static size_t det_table_entries(uint32_t n, uint32_t numSegments) { return (size_t)numSegments + n / kDetLarge + 3; }
+static size_t det_table_entries(uint32_t n, uint32_t numSegments) { return (size_t)numSegments + n / kDetTile + 3; }
+static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
 size_t det_sort_stage_bytes(uint32_t n, uint32_t numSegments) {
   return det_table_entries(n, numSegments) * kDetTables * sizeof(DetSegment);
 }
 
 size_t det_sort_temp_bytes(uint32_t n, uint32_t numSegments) {
-  // the segment tables and the ping-pong index buffer of the merge passes
-  return ((det_sort_stage_bytes(n, numSegments) + 255) & ~size_t(255)) + size_t(n) * 4 + 256;
+  // the segment tables, the tiles' weight keys and the merge passes' second index and key buffers
+  return align256(det_sort_stage_bytes(n, numSegments)) + 3 * align256(size_t(n) * 4);
 }
 
 void det_sort_segments(void* temp, void* pinnedHost, size_t pinnedBytes, const void* records,
                        const uint32_t* begin, const uint32_t* count, uint32_t numSegments, uint32_t n,
-                       uint32_t* vals, hipStream_t s) {
+                       uint32_t* vals, uint32_t* weightKeys, hipStream_t s) {
   if (n == 0 || numSegments == 0) return;
   const float* recs = (const float*)records;
-  // tables: 0 ranges <= 1024, 1 <= 4096, 2 <= 8192, 3 blocks of <= 16384
-  // (ranges up to 16384 and the blocks of longer ranges), 4 ranges longer
-  // than 16384
+  // tables: 0 ranges <= 1024, 1 tiles of <= 4096 (ranges up to 4096 and the
+  // tiles of longer ranges), 2 ranges longer than one tile
   const size_t cap = det_table_entries(n, numSegments);
   if (pinnedBytes < det_sort_stage_bytes(n, numSegments))
     throw std::runtime_error("det_sort_segments: staging too small");
   DetSegment* host = (DetSegment*)pinnedHost;
-  DetSegment* cls[kDetTables];
-  for (int c = 0; c < kDetTables; c++) cls[c] = host + c * cap;
-  uint32_t num[kDetTables] = {0, 0, 0, 0, 0};
+  DetSegment* cls[kDetTables] = {host, host + cap, host + 2 * cap};
+  uint32_t num[kDetTables] = {0, 0, 0};
   uint32_t longest = 0;
   for (uint32_t r = 0; r < numSegments; r++) {
     const uint32_t c = count[r];
     if (c < 2) continue;
     if (c <= kDetSmall) {
       cls[0][num[0]++] = DetSegment{begin[r], c};
-    } else if (c <= kDetMid) {
-      cls[1][num[1]++] = DetSegment{begin[r], c};
-    } else if (c <= kDetBig) {
-      cls[2][num[2]++] = DetSegment{begin[r], c};
     } else {
-      for (uint32_t o = 0; o < c; o += kDetLarge) cls[3][num[3]++] = DetSegment{begin[r] + o, std::min(kDetLarge, c - o)};
-      if (c > kDetLarge) {
-        cls[4][num[4]++] = DetSegment{begin[r], c};
+      for (uint32_t o = 0; o < c; o += kDetTile) cls[1][num[1]++] = DetSegment{begin[r] + o, std::min(kDetTile, c - o)};
+      if (c > kDetTile) {
+        cls[2][num[2]++] = DetSegment{begin[r], c};
         longest = std::max(longest, c);
       }
     }
   }
-  DetSegment* dev = (DetSegment*)temp;
+  char* t = (char*)temp;
+  DetSegment* dev = (DetSegment*)t;
+  t += align256(det_sort_stage_bytes(n, numSegments));
+  uint32_t* keysA = weightKeys ? weightKeys : (uint32_t*)t;
+  t += align256(size_t(n) * 4);
+  uint32_t* idxB = (uint32_t*)t;
+  t += align256(size_t(n) * 4);
+  uint32_t* keysB = (uint32_t*)t;
   PGL_KERNEL_CHECK(hipMemcpyAsync(dev, host, det_sort_stage_bytes(n, numSegments), hipMemcpyHostToDevice, s));
-  if (num[0]) hipLaunchKernelGGL((k_det_block_sort<256, 4>), dim3(num[0]), dim3(256), 0, s, recs, dev, vals);
-  if (num[1]) hipLaunchKernelGGL((k_det_block_sort<256, 16>), dim3(num[1]), dim3(256), 0, s, recs, dev + cap, vals);
-  if (num[2]) hipLaunchKernelGGL((k_det_block_sort<512, 16>), dim3(num[2]), dim3(512), 0, s, recs, dev + 2 * cap, vals);
-  if (num[3]) hipLaunchKernelGGL((k_det_block_sort<512, 32>), dim3(num[3]), dim3(512), 0, s, recs, dev + 3 * cap, vals);
-  if (num[4] == 0) return;
-  // merge passes over all ranges longer than one block, ping-pong through the temp buffer
-  uint32_t* tmpIdx =
-      (uint32_t*)((char*)temp + ((det_sort_stage_bytes(n, numSegments) + 255) & ~size_t(255)));
-  const dim3 grid(std::min<uint32_t>((longest + 255) / 256, 256), num[4]);
-  uint32_t *a = vals, *b = tmpIdx;
-  for (uint32_t w = kDetLarge; w < longest; w *= 2) {
-    hipLaunchKernelGGL(k_det_merge, grid, dim3(256), 0, s, recs, dev + 4 * cap, w, a, b);
-    std::swap(a, b);
+  const uint32_t* keysIn = weightKeys;  // in place: a tile reads its keys before writing them
+  if (num[0])
+    hipLaunchKernelGGL((k_det_tile_sort<256, 4>), dim3(num[0]), dim3(256), 0, s, recs, dev, vals, keysIn, keysA);
+  if (num[1])
+    hipLaunchKernelGGL((k_det_tile_sort<256, 8>), dim3(num[1]), dim3(256), 0, s, recs, dev + cap, vals, keysIn, keysA);
+  if (num[2] == 0) return;
+  // merge passes over all ranges longer than one tile, ping-pong between the buffers
+  const dim3 grid(std::min<uint32_t>((longest + 255) / 256, 64), num[2]);
+  uint32_t *ia = vals, *ka = keysA, *ib = idxB, *kb = keysB;
+  for (uint32_t w = kDetTile; w < longest; w *= 2) {
+    hipLaunchKernelGGL(k_det_merge, grid, dim3(256), 0, s, recs, dev + 2 * cap, w, ia, ka, ib, kb);
+    std::swap(ia, ib);
+    std::swap(ka, kb);
   }
-  if (a != vals) hipLaunchKernelGGL(k_det_copy, grid, dim3(256), 0, s, dev + 4 * cap, a, vals);
+  if (ia != vals) hipLaunchKernelGGL(k_det_copy, grid, dim3(256), 0, s, dev + 2 * cap, ia, vals);
 }
 
 }  // namespace pglgpu

@@ -21,16 +21,18 @@ namespace pglgpu {
 
 
 // One coalesced pass over the 40-byte records: positions to SoA for the kd
-// rounds, every sample starts at the root.
+// rounds and, in deterministic mode (wkey not null), the order key of each
+// weight for the within-region sort.
 __global__ void __launch_bounds__(256) k_ingest(const SampleAoS* __restrict__ in, uint32_t n,
                                                 float* __restrict__ px, float* __restrict__ py,
-                                                float* __restrict__ pz, uint32_t* __restrict__ nodeOf) {
+                                                float* __restrict__ pz, uint32_t* __restrict__ wkey) {
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const SampleAoS s = in[i];
     px[i] = s.px;
     py[i] = s.py;
     pz[i] = s.pz;
+    if (wkey) wkey[i] = float_order_key(s.w);
   }
 }
 
@@ -392,9 +394,9 @@ static inline uint32_t grid_for(uint32_t n, uint32_t block, uint32_t maxBlocks =
   return g > maxBlocks ? maxBlocks : g;
 }
 
-void launch_ingest(const void* aos, uint32_t n, float* px, float* py, float* pz, uint32_t* nodeOf, hipStream_t s) {
+void launch_ingest(const void* aos, uint32_t n, float* px, float* py, float* pz, uint32_t* wkey, hipStream_t s) {
   if (n == 0) return;
-  hipLaunchKernelGGL(k_ingest, dim3(grid_for(n, 256)), dim3(256), 0, s, (const SampleAoS*)aos, n, px, py, pz, nodeOf);
+  hipLaunchKernelGGL(k_ingest, dim3(grid_for(n, 256)), dim3(256), 0, s, (const SampleAoS*)aos, n, px, py, pz, wkey);
 }
 
 uint32_t launch_bounds(const float* px, const float* py, const float* pz, uint32_t n, float* partial, hipStream_t s) {

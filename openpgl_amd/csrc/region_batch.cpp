@@ -78,7 +78,7 @@ void RegionBatch::step(uint32_t R, const void* records, const uint32_t* counts, 
     dDetTmp_.ensure(det_sort_temp_bytes(n, R));
     detPin_.ensure(det_sort_stage_bytes(n, R));
     det_sort_segments(dDetTmp_.p, detPin_.p, detPin_.bytes, records, begin.data(), counts, R, n, dVals_.as<uint32_t>(),
-                      s);
+                      nullptr, s);
   }
   launch_gather_reproject(records, gatherKeys, gatherIdx, n, dPivots_.as<float4>(),
                           dDx_.as<float>(), dDy_.as<float>(), dDz_.as<float>(), dW_.as<float>(), dPdf_.as<float>(),

@@ -14,6 +14,14 @@ struct SampleAoS {
 };
 static_assert(sizeof(SampleAoS) == 40, "PGLSampleData is 40 B");
 
+// Order-preserving map of a float to uint32 (-0 mapped to +0, which
+// SampleDataLess treats as equal): a < b as floats iff key(a) < key(b).
+__device__ __forceinline__ uint32_t float_order_key(float x) {
+  uint32_t b = __float_as_uint(x);
+  if (b == 0x80000000u) b = 0u;
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
 // Reprojection to the region's pivot: origin = p + d * max(minDist, dist);
 // d' = normalise(origin - pivot).  distance == inf keeps the direction
 // (position moves to the pivot); distance <= 0 keeps the sample unchanged.
