@@ -88,6 +88,39 @@ def run_gpu(frames_host, warmup, steps, schedule=None):
     return field, dev, (t1 - t0), infos
 
 
+def render_loop(frames_host, warmup, steps, queries_per_frame=262144):
+    """The drop-in use in a guided render loop, synchronous: per frame the
+    records arrive in a host SampleStorage (pglFieldUpdate copies them over
+    PCIe), the update completes, then one batch of guided queries
+    (pglFieldSampleBatch from host arrays, results back in host memory) runs
+    on the new field before the next frame.  Nothing overlaps across frames."""
+    import openpgl_amd as pgl
+    from openpgl_amd.synth import queries
+    dev = pgl.Device()
+    field = pgl.Field(dev, max_samples_per_leaf=32000, deterministic=True)
+    stores = [pgl.SampleStorage(f) for f in frames_host]
+    pos, nrm, u2, is_vol, g = queries(queries_per_frame, seed=17)
+    is_vol[:] = False  # the loop trains the surface field only
+    upd, qry = [], []
+    for f in range(warmup + steps):
+        t0 = time.perf_counter()
+        field.update(stores[f])
+        field.num_regions()  # waits for the update
+        t1 = time.perf_counter()
+        field.sample_batch(pos, nrm, u2, g=g, is_volume=is_vol)
+        t2 = time.perf_counter()
+        if f >= warmup:
+            upd.append(t1 - t0)
+            qry.append(t2 - t1)
+    n = sum(len(frames_host[warmup + s]) for s in range(steps))
+    frame_s = sum(upd) + sum(qry)
+    return {"mode": "synchronous: host SampleStorage -> pglFieldUpdate (PCIe copy included) -> wait -> "
+                    "pglFieldSampleBatch of %d host queries -> next frame" % queries_per_frame,
+            "update_samples_per_s": n / sum(upd), "frame_samples_per_s": n / frame_s,
+            "update_ms_avg": 1e3 * float(np.mean(upd)), "query_batch_ms_avg": 1e3 * float(np.mean(qry)),
+            "frames_timed": steps}
+
+
 def serving(field, n_batches=40, seed=0):
     """Closed loop: each batch is a log-uniform [1k, 256k] set of queries (30%
     volume is not trained here -> surface only), inputs resident on device."""
@@ -143,7 +176,8 @@ def serving_poisson(field, offered_qps, duration_s=1.5, seed=0, pool=1 << 22, ri
     position near a blob, a normal uniform on the sphere, 2 uniforms, no KNN
     random (u1 = -1); 30% are volume queries with g uniform in [-0.9, 0.9].
     Inputs are slices of a pinned pool, outputs go to a pinned ring.
-    Latency = submission (arrival) -> results in host memory, per request."""
+    Latency = scheduled Poisson arrival -> results in host memory, per
+    request (any lag of the submitting thread behind the schedule counts)."""
     import openpgl_amd as pgl
     from openpgl_amd.hip_runtime import pinned_copy, pinned_empty
     from openpgl_amd.synth import queries
@@ -167,7 +201,7 @@ def serving_poisson(field, offered_qps, duration_s=1.5, seed=0, pool=1 << 22, ri
                            out=(odir[:256000], opdf[:256000], oid[:256000], oint[:256000]))
     server.wait(rid)
     server.release(rid)
-    ids, live = [], []
+    ids, live, lag = [], [], []
     rpos = 0
     t0 = time.perf_counter()
     for i in range(count):
@@ -189,6 +223,7 @@ def serving_poisson(field, offered_qps, duration_s=1.5, seed=0, pool=1 << 22, ri
         a = int(starts[i])
         sl = slice(a, a + n)
         o = slice(rpos, rpos + n)
+        lag.append(time.perf_counter() - target)
         rid, _ = server.submit(pos[sl], nrm[sl], u2[sl], g=g[sl], is_volume=vol[sl],
                                out=(odir[o], opdf[o], oid[o], oint[o]))
         ids.append(rid)
@@ -196,7 +231,9 @@ def serving_poisson(field, offered_qps, duration_s=1.5, seed=0, pool=1 << 22, ri
         rpos += n
     server.drain()
     t = np.array([server.timing(r) for r in ids])
-    lat = t[:, 2] - t[:, 0]
+    lat_submit = t[:, 2] - t[:, 0]
+    # from the scheduled (Poisson) arrival: host lag before the submit included
+    lat = lat_submit + 1e3 * np.asarray(lag)
     wall = (t[:, 2].max() - t[:, 0].min()) * 1e-3
     st = server.stats()
     valid = float(np.mean(opdf[:min(ring, rpos)] > 0))
@@ -204,6 +241,8 @@ def serving_poisson(field, offered_qps, duration_s=1.5, seed=0, pool=1 << 22, ri
     return {"offered_queries_per_s": offered_qps, "achieved_guided_samples_per_s": float(sizes.sum() / wall),
             "requests": int(count), "queries": int(sizes.sum()), "p50_ms": float(np.percentile(lat, 50)),
             "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()),
+            "latency_from": "scheduled arrival",
+            "p99_ms_from_submit": float(np.percentile(lat_submit, 99)),
             "mean_batch_queries": st["queries"] / max(1.0, st["batches"]), "batches": int(st["batches"]),
             "peak_pending_requests": int(st["peak_pending"]), "valid_fraction": valid,
             "mode": "open loop, Poisson arrivals, continuous batching, host pinned in/out, %d slots" % slots}
@@ -295,10 +334,11 @@ def mixed_workload(frames=3, surf=16_000_000, vol=4_000_000, offered_qps=1.0e8, 
             while time.perf_counter() < tn and not stop.is_set():
                 time.sleep(1e-4)
             a = int(rng.integers(0, pool - n))
+            lagMs = 1e3 * (time.perf_counter() - tn)
             rid, _ = server.submit(pos[a:a + n], nrm[a:a + n], u2[a:a + n], g=g[a:a + n], is_volume=volf[a:a + n],
                                    out=(odir[rpos:rpos + n], opdf[rpos:rpos + n], oid[rpos:rpos + n],
                                         oint[rpos:rpos + n]))
-            ids.append((rid, n))
+            ids.append((rid, n, lagMs))
             live.append((rpos, rpos + n, rid))
             rpos += n
 
@@ -316,9 +356,9 @@ def mixed_workload(frames=3, surf=16_000_000, vol=4_000_000, offered_qps=1.0e8, 
     th.join()
     server.drain()
     mem_peak = _device_mem_used() - mem0
-    tm = np.array([server.timing(r) for r, _ in ids])
-    lat = tm[:, 2] - tm[:, 0]
-    nq = sum(n for _, n in ids)
+    tm = np.array([server.timing(r) for r, _, _ in ids])
+    lat = tm[:, 2] - tm[:, 0] + np.array([lg for _, _, lg in ids])  # from the scheduled arrival
+    nq = sum(n for _, n, _ in ids)
     span = (tm[:, 2].max() - tm[:, 0].min()) * 1e-3
     server.close()
     samples = (frames - 1) * (surf + vol)
@@ -328,10 +368,30 @@ def mixed_workload(frames=3, surf=16_000_000, vol=4_000_000, offered_qps=1.0e8, 
                 "serving_during_updates": {"offered_queries_per_s": offered_qps, "requests": len(ids),
                                            "achieved_guided_samples_per_s": nq / span,
                                            "p50_ms": float(np.percentile(lat, 50)),
-                                           "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max())},
+                                           "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()),
+                                           "latency_from": "scheduled arrival"},
                 "hbm_bytes": {"input_frames": int(mem_inputs), "field": int(mem_field),
                               "server_budget": int(server_budget), "peak_total": int(mem_peak)}})
     return out
+
+
+def host_cpu():
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "model": model}
+
+
+def cpu_threads():
+    """Worker threads of the CPU baselines: every core of this GPU's host
+    share (16 per GPU on the measurement hosts, whatever nproc reports for the
+    whole machine), or nproc where that is smaller."""
+    return min(os.cpu_count() or 1, 16)
 
 
 def cpu_baseline(frames_host, threads, timed_frames=3):
@@ -350,7 +410,8 @@ def cpu_baseline(frames_host, threads, timed_frames=3):
     dt = time.perf_counter() - t0
     return {"value": n / dt, "unit": "samples/s", "cores": threads, "kind": "port",
             "sample": "oracle (CPU restatement of Open PGL 0.6.0, std::thread over regions), incremental "
-                      "Updates of frames 1-%d (%d samples) after the frame-0 build; %.2f s" % (timed_frames, n, dt)}
+                      "Updates of frames 1-%d (%d samples) after the frame-0 build; %.2f s" % (timed_frames, n, dt),
+            "host": host_cpu()}
 
 
 def batched_em(num_regions=4096, cpu=True, threads=16):
@@ -373,6 +434,7 @@ def batched_em(num_regions=4096, cpu=True, threads=16):
     first = batch.last_info()
     best, table = batch.autotune(None, reps=2)
     _, per_region = batch.autotune([dict(per_region_launches=1)], reps=1)
+    _, per_region_teams = batch.autotune([dict(per_region_launches=2)], reps=2)
     # leave the tuned schedule selected for reporting
     batch.set_schedule(**table[best][0])
     best_ms = table[best][1]
@@ -384,8 +446,11 @@ def batched_em(num_regions=4096, cpu=True, threads=16):
                        % (num_regions, n),
            "samples": n, "batched_ms": best_ms, "batched_samples_per_s": n / (best_ms * 1e-3),
            "default_schedule_ms": default_ms[0] if default_ms else None,
-           "per_region_path_ms": per_region[0][1],
-           "speedup_vs_per_region": per_region[0][1] / best_ms,
+           "per_region_path_ms": per_region_teams[0][1],
+           "per_region_path": "one launch per region, a team of one workgroup per 1024-sample chunk (up to 64), "
+                              "each region alone on the device",
+           "speedup_vs_per_region": per_region_teams[0][1] / best_ms,
+           "one_workgroup_per_region_launch_ms": per_region[0][1],
            "tuned_schedule": table[best][0],
            "autotune_top5": sorted([(ms, sc) for sc, ms in table], key=lambda x: x[0])[:5],
            "achieved_tflops": flops / (best_ms * 1e-3) / 1e12,
@@ -393,15 +458,24 @@ def batched_em(num_regions=4096, cpu=True, threads=16):
     if cpu:
         out["cpu"] = batched_cpu_baseline(recs, counts, stats7, init_k, threads)
         out["speedup_vs_cpu"] = out["batched_samples_per_s"] / out["cpu"]["value"]
+        ref = batched_cpu_baseline(recs, counts, stats7, init_k, threads, reference=True)
+        if ref is not None:
+            out["cpu_reference"] = ref
+            out["speedup_vs_cpu_reference"] = out["batched_samples_per_s"] / ref["value"]
     return out
 
 
-def batched_cpu_baseline(recs, counts, stats7, init_k, threads, sample_regions=256):
-    """Oracle region fits (std library threads via ctypes, GIL released) on
-    every (num_regions / sample_regions)-th region."""
+def batched_cpu_baseline(recs, counts, stats7, init_k, threads, sample_regions=256, reference=False):
+    """Region fits on the CPU (threads via ctypes, GIL released) on every
+    (num_regions / sample_regions)-th region: the oracle restatement, or with
+    reference=True the reference's own directional code (its AVX2 CPU_8
+    build in oracle/_ref: ParallaxAwareVonMisesFisherWeightedEMFactory with
+    adaptive split-and-merge, one region per call; None where not built)."""
     import concurrent.futures
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from oracle_ctypes import OracleRegion, default_config
+    from oracle_ctypes import OracleRegion, RefRegion, default_config, reference_available
+    if reference and not reference_available():
+        return None
     offs = np.concatenate([[0], np.cumsum(counts.astype(np.int64))])
     step = max(1, len(counts) // sample_regions)
     regions = list(range(0, len(counts), step))
@@ -409,15 +483,18 @@ def batched_cpu_baseline(recs, counts, stats7, init_k, threads, sample_regions=2
     for r in regions:
         cfg = default_config(32000)
         cfg.initK = int(init_k[r])
-        jobs.append((OracleRegion(cfg), recs[offs[r]:offs[r + 1]], stats7[r]))
+        jobs.append((RefRegion(cfg) if reference else OracleRegion(cfg), recs[offs[r]:offs[r + 1]], stats7[r]))
     n = int(sum(len(j[1]) for j in jobs))
     t0 = time.perf_counter()
     with concurrent.futures.ThreadPoolExecutor(max_workers=threads) as ex:
         list(ex.map(lambda j: j[0].step(j[1], j[2][0:3], j[2][3:6], float(j[2][6])), jobs))
     dt = time.perf_counter() - t0
-    return {"value": n / dt, "unit": "samples/s", "cores": threads, "kind": "port",
-            "sample": "oracle region fits of %d of the regions (every %d-th, %d samples), %d threads, %.2f s"
-                      % (len(regions), step, n, threads, dt)}
+    who = ("the reference's own directional code (CPU_8 AVX2 build)" if reference
+           else "oracle (CPU restatement)")
+    return {"value": n / dt, "unit": "samples/s", "cores": threads, "kind": "reference" if reference else "port",
+            "sample": "%s region fits of %d of the regions (every %d-th, %d samples), %d threads, %.2f s"
+                      % (who, len(regions), step, n, threads, dt),
+            "host": host_cpu()}
 
 
 def path_segments(num_paths=1_000_000, reps=5, cpu=True):
@@ -508,6 +585,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched uneven EM workload")
     ap.add_argument("--no-mixed", action="store_true", help="skip the large mixed update + serving workload")
     ap.add_argument("--no-paths", action="store_true", help="skip the path-segment workload")
+    ap.add_argument("--no-render-loop", action="store_true", help="skip the synchronous render-loop line")
     ap.add_argument("--region-grid", type=int, default=None, help="region kernel workgroups (default: all co-resident)")
     ap.add_argument("--region-balance", type=float, default=None, help="region team balance (default 1)")
     ap.add_argument("--region-cooperative", type=int, default=None, help="1: cooperative region launch (default 0: ordinary launch)")
@@ -570,19 +648,20 @@ def main():
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": read_traffic(), "kernel": "k_region_update",
                      **per_launch},
     }
+    if not args.no_render_loop:
+        out["render_loop"] = render_loop(frames, args.warmup, args.steps)
     if not args.no_serving:
         out["serving"] = serving(field)
         sdev, sfield = serving_field()
         out["serving_poisson"] = [serving_poisson(sfield, q) for q in (1.0e8, 2.0e8)]
     if not args.no_batched:
-        out["batched_em"] = batched_em(cpu=not args.no_cpu, threads=min(os.cpu_count() or 1, 16))
+        out["batched_em"] = batched_em(cpu=not args.no_cpu, threads=cpu_threads())
     if not args.no_mixed:
         out["mixed_update_and_serving"] = mixed_workload()
     if not args.no_paths:
         out["path_segments"] = path_segments(cpu=not args.no_cpu)
     if not args.no_cpu:
-        threads = min(os.cpu_count() or 1, 16)
-        out["cpu_baseline"] = cpu_baseline(frames, threads, timed_frames=min(3, nframes - 1))
+        out["cpu_baseline"] = cpu_baseline(frames, cpu_threads(), timed_frames=min(3, nframes - 1))
     print(json.dumps(out), flush=True)
 
 

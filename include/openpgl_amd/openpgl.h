@@ -337,7 +337,10 @@ size_t pglFieldGetUpdateHistory(PGLField field, int kind, double* out, size_t ma
 /* EXTENSION: scheduling of the directional update.  None of the fields
  * changes the result (bit-identical for every schedule).
  *   perRegionLaunches  1: one launch of one workgroup per region (the plain
- *                      per-region GPU path); 0: one persistent launch
+ *                      per-region GPU path); 2: one launch per region with a
+ *                      team of one workgroup per 1024-sample chunk (up to
+ *                      maxTeam and the device), each region alone on the
+ *                      device; 0: one persistent launch
  *   grid               workgroups of the persistent launch (0: two per CU,
  *                      which leaves room for other streams' kernels; at
  *                      most the workgroups that fit at once, three per CU)

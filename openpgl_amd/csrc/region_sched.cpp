@@ -91,14 +91,18 @@ void RegionRunner::launch(RegionState* states, std::vector<RegionWork> work, con
   }
   std::stable_sort(work.begin(), work.end(), [](const RegionWork& a, const RegionWork& b) { return a.cost > b.cost; });
   const uint32_t J = (uint32_t)work.size();
+  // perRegionLaunches: 1 = one launch of one workgroup per region; 2 = one
+  // launch per region with a team of one workgroup per chunk (up to maxTeam
+  // and the resident grid), i.e. each region alone on the device
   const bool perRegion = schedule.perRegionLaunches != 0;
+  const bool perRegionTeams = schedule.perRegionLaunches == 2;
   const uint32_t G = perRegion ? 1u : (schedule.grid ? std::min(schedule.grid, resident_) : defaultGrid_);
   double total = 0.0;
   for (const auto& w : work) total += std::max(w.cost, 0.f);
   const double bal = schedule.balance > 0.f ? schedule.balance : 1.f;
   const double target = std::max(total / ((double)G * bal), 1e-30);
   const uint32_t minChunks = std::max<uint32_t>(1, schedule.minChunksPerRank);
-  const uint32_t teamCap = std::max<uint32_t>(1, std::min(schedule.maxTeam, G));
+  const uint32_t teamCap = std::max<uint32_t>(1, std::min(schedule.maxTeam, perRegionTeams ? resident_ : G));
 
   jobs_.resize(J);
   items_.clear();
@@ -109,8 +113,8 @@ void RegionRunner::launch(RegionState* states, std::vector<RegionWork> work, con
     jb.region = work[j].region;
     jb.chunks = std::max<uint32_t>(1, (n + kChunk - 1) / kChunk);
     uint32_t T = 1;
-    if (!perRegion) {
-      const double want = std::ceil(std::max(work[j].cost, 0.f) / target);
+    if (!perRegion || perRegionTeams) {
+      const double want = perRegionTeams ? (double)jb.chunks : std::ceil(std::max(work[j].cost, 0.f) / target);
       T = (uint32_t)std::min<double>(std::max(want, 1.0), (double)std::max<uint32_t>(1, jb.chunks / minChunks));
       T = std::max<uint32_t>(1, std::min(T, teamCap));
       // fewest ranks with the same largest number of chunks per rank
@@ -122,10 +126,10 @@ void RegionRunner::launch(RegionState* states, std::vector<RegionWork> work, con
     jb.partialOffset = T > 1 ? partialFloats : 0ull;
     if (T > 1) partialFloats += 2ull * jb.chunks * kPartStride;
     lastMaxTeam = std::max(lastMaxTeam, T);
-    if (!perRegion)
+    if (!perRegion || perRegionTeams)
       for (uint32_t r = 0; r < T; r++) items_.push_back(make_uint2(j, r));
   }
-  if (perRegion) items_.assign(1, make_uint2(0, 0));
+  if (perRegion && !perRegionTeams) items_.assign(1, make_uint2(0, 0));
 
   // jobs | items | inputs (16-byte aligned parts) in pinned memory, read by the preparing kernel
   const size_t jobsB = J * sizeof(RegionJob), itemsB = items_.size() * sizeof(uint2);
@@ -164,7 +168,23 @@ void RegionRunner::launch(RegionState* states, std::vector<RegionWork> work, con
   L.cooperative = cooperative_ && schedule.cooperative != 0;
   L.zeroQueue = perRegion;
   PGL_SCHED_CHECK(hipEventRecord(evBegin_, s));
-  if (perRegion) {
+  if (perRegionTeams) {
+    // one launch per job: its consecutive items, the queue and counters zeroed before each
+    L.cooperative = false;
+    L.zeroQueue = true;
+    const uint2* allItems = L.items;
+    uint32_t first = 0;
+    for (uint32_t j = 0; j < J; j++) {
+      const uint32_t T = jobs_[j].team;
+      L.items = allItems + first;
+      L.numItems = T;
+      L.grid = T;
+      launch_region_update(L, cfg, s);
+      first += T;
+      lastGrid = std::max(lastGrid, T);
+    }
+    lastLaunches = J;
+  } else if (perRegion) {
     L.cooperative = false;
     L.numJobs = 1;
     L.numItems = 1;

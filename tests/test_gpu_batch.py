@@ -68,7 +68,7 @@ def test_region_batch_schedules_and_autotune_agree():
     recs, counts, stats, init_k = batched_regions(96, seed=5, nmin=64, nmax=20000)
     d = DeviceArray.from_numpy(recs)
     ref = None
-    for sc in [dict(per_region_launches=1), dict(per_region_launches=0),
+    for sc in [dict(per_region_launches=1), dict(per_region_launches=2), dict(per_region_launches=0),
                dict(per_region_launches=0, balance=8.0, max_team=64)]:
         b = pgl.RegionBatch(dev)
         b.set_schedule(**sc)

@@ -143,6 +143,7 @@ def test_schedule_independence():
     dev = pgl.Device()
     frames = [generate(120000, seed=4, frame=f) for f in range(3)]
     schedules = [dict(per_region_launches=1),
+                 dict(per_region_launches=2),
                  dict(per_region_launches=0),
                  dict(per_region_launches=0, balance=64.0, min_chunks_per_rank=1, max_team=64),
                  dict(per_region_launches=0, grid=7, balance=2.0)]
