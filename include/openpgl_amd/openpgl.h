@@ -483,6 +483,25 @@ void pglFieldUpdateFromPathSegmentsDevice(PGLField field, const PGLPathSegmentDa
                                           bool useNEEMiWeights, bool guideDirectLight,
                                           bool rrAffectsDirectContribution);
 
+/* EXTENSION, no counterpart in the reference: the samples of
+ * pglPathSegmentBatchPrepareSamplesDevice (same arguments, results and
+ * errors) without allocating anything, for callers that cannot allocate
+ * device or pinned memory (render or serving loops held to a fixed memory
+ * footprint, hosts that forbid allocation inside the frame).  Its scratch is one static
+ * device array of 15.25 MiB per GPU, held until the process ends; at most
+ * 262144 vertices (numSegments) per call, larger calls fail with a message
+ * and write nothing.  Work is queued on `stream` (hipStream_t, NULL = the
+ * default stream) and the call blocks until the records are written.  All
+ * calls on one GPU share that array: calls that overlap on one GPU (from
+ * several threads) race and must be serialised by the caller. */
+bool pglPathSegmentsPrepareSamplesDeviceNoAlloc(const PGLPathSegmentData* segments, size_t numSegments,
+                                                const uint32_t* pathOffsets, uint32_t numPaths, float maxDistance,
+                                                bool useNEEMiWeights, bool guideDirectLight,
+                                                bool rrAffectsDirectContribution, PGLSampleData* surfaceOut,
+                                                size_t surfaceCapacity, PGLSampleData* volumeOut,
+                                                size_t volumeCapacity, size_t* numSurfaceOut, size_t* numVolumeOut,
+                                                void* stream);
+
 #ifdef __cplusplus
 }  /* extern "C" */
 #endif

@@ -178,6 +178,9 @@ _SIGS = {
                                                                  ctypes.c_size_t, _vp, ctypes.c_size_t, _vp, _vp]),
     "pglFieldUpdateFromPathSegmentsDevice": (None, [_vp, _vp, ctypes.c_size_t, _vp, _u32, _f, ctypes.c_bool,
                                                     ctypes.c_bool, ctypes.c_bool]),
+    "pglPathSegmentsPrepareSamplesDeviceNoAlloc": (ctypes.c_bool, [_vp, ctypes.c_size_t, _vp, _u32, _f, ctypes.c_bool,
+                                                                   ctypes.c_bool, ctypes.c_bool, _vp, ctypes.c_size_t,
+                                                                   _vp, ctypes.c_size_t, _vp, _vp, _vp]),
     "pglFieldSampleBatch": (None, [_vp, ctypes.c_size_t] + [_vp] * 10),
     "pglFieldSampleBatchDevice": (None, [_vp, ctypes.c_size_t] + [_vp] * 11),
     "pglFieldGetNumNodes": (_u32, [_vp, ctypes.c_int]),
@@ -734,6 +737,26 @@ class PathSegmentStorage:
 
     def validate_samples(self):
         return self.lib.pglPathSegmentStorageValidateSamples(self.h)
+
+
+NO_ALLOC_MAX_VERTICES = 1 << 18
+
+
+def prepare_path_samples_device_noalloc(segments_ptr, num_segments, offsets_ptr, num_paths, surface_ptr,
+                                        surface_capacity, volume_ptr, volume_capacity, max_distance=1e6,
+                                        use_nee_mi_weights=False, guide_direct_light=False, rr_affects_direct=True,
+                                        stream=None):
+    """EXTENSION: PathSegmentBatch.prepare_device without any allocation
+    (pglPathSegmentsPrepareSamplesDeviceNoAlloc; at most
+    NO_ALLOC_MAX_VERTICES vertices per call; calls overlapping on one GPU
+    race).  Returns (surface count, volume count), or None on error."""
+    lib = load_library()
+    ns, nv = ctypes.c_size_t(0), ctypes.c_size_t(0)
+    ok = lib.pglPathSegmentsPrepareSamplesDeviceNoAlloc(
+        ctypes.c_void_p(segments_ptr), num_segments, ctypes.c_void_p(offsets_ptr), num_paths, float(max_distance),
+        use_nee_mi_weights, guide_direct_light, rr_affects_direct, ctypes.c_void_p(surface_ptr), surface_capacity,
+        ctypes.c_void_p(volume_ptr), volume_capacity, ctypes.byref(ns), ctypes.byref(nv), ctypes.c_void_p(stream))
+    return (ns.value, nv.value) if ok else None
 
 
 class PathSegmentBatch:

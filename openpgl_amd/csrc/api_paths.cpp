@@ -11,6 +11,8 @@
 #pragma clang fp contract(off)
 #include <cmath>
 #include <cstring>
+#include <stdexcept>
+#include <string>
 
 #include "api_impl.h"
 #include "path_batch.h"
@@ -213,6 +215,34 @@ bool pglPathSegmentBatchPrepareSamplesDevice(PGLPathSegmentBatch batch, const PG
   ((PathSampleBatch*)batch)
       ->prepare((const PathVertex*)segments, (uint32_t)numSegments, pathOffsets, numPaths, o, (PathSample*)surfaceOut,
                 surfaceOut ? surfaceCapacity : 0, (PathSample*)volumeOut, volumeOut ? volumeCapacity : 0, ns, nv);
+  if (numSurfaceOut) *numSurfaceOut = ns;
+  if (numVolumeOut) *numVolumeOut = nv;
+  return true;
+}
+PGL_CATCH_END(false)
+
+bool pglPathSegmentsPrepareSamplesDeviceNoAlloc(const PGLPathSegmentData* segments, size_t numSegments,
+                                                const uint32_t* pathOffsets, uint32_t numPaths, float maxDistance,
+                                                bool useNEEMiWeights, bool guideDirectLight,
+                                                bool rrAffectsDirectContribution, PGLSampleData* surfaceOut,
+                                                size_t surfaceCapacity, PGLSampleData* volumeOut,
+                                                size_t volumeCapacity, size_t* numSurfaceOut, size_t* numVolumeOut,
+                                                void* stream) PGL_CATCH_BEGIN {
+  if (numSurfaceOut) *numSurfaceOut = 0;
+  if (numVolumeOut) *numVolumeOut = 0;
+  if (numPaths > 0) {
+    THROW_IF_NULL(segments, "segments");
+    THROW_IF_NULL(pathOffsets, "path offsets");
+  }
+  if (numSegments > kNoAllocMaxVertices)
+    throw std::runtime_error("openpgl_amd: the allocation-free path batch takes at most " +
+                             std::to_string(kNoAllocMaxVertices) + " vertices per call (got " +
+                             std::to_string(numSegments) + ")");
+  const PathSampleOptions o{maxDistance, useNEEMiWeights, guideDirectLight, rrAffectsDirectContribution};
+  size_t ns = 0, nv = 0;
+  path_prepare_noalloc((const PathVertex*)segments, (uint32_t)numSegments, pathOffsets, numPaths, o,
+                       (PathSample*)surfaceOut, surfaceOut ? surfaceCapacity : 0, (PathSample*)volumeOut,
+                       volumeOut ? volumeCapacity : 0, ns, nv, (hipStream_t)stream);
   if (numSurfaceOut) *numSurfaceOut = ns;
   if (numVolumeOut) *numVolumeOut = nv;
   return true;

@@ -164,4 +164,16 @@ void launch_path_scatter(const PathSample* records, const unsigned long long* fl
                          const unsigned long long* incl, const uint32_t* offsets, uint32_t numSegments,
                          PathSample* surfOut, PathSample* volOut, hipStream_t s);
 
+// The same two passes with every scratch buffer in one static device array
+// per GPU (15.25 MiB, held until the process ends): at most
+// kNoAllocMaxVertices vertices per call, no device or pinned allocation.
+// Calls that overlap on one GPU share that array and race.  Blocks until the
+// records are written; throws on a size above the maximum, malformed ranges
+// or a capacity that is too small (nothing written then).
+constexpr uint32_t kNoAllocMaxVertices = 1u << 18;
+constexpr size_t kNoAllocScratchBytes = 15995392;
+void path_prepare_noalloc(const PathVertex* segs, uint32_t numSegments, const uint32_t* offsets, uint32_t numPaths,
+                          const PathSampleOptions& o, PathSample* surfOut, size_t surfCap, PathSample* volOut,
+                          size_t volCap, size_t& numSurf, size_t& numVol, hipStream_t s);
+
 }  // namespace pglgpu

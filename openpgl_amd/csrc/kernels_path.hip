@@ -21,6 +21,10 @@
 
 #include <hipcub/device/device_scan.hpp>
 
+#include <mutex>
+#include <stdexcept>
+#include <string>
+
 #include "kernels.h"
 #include "path_samples.h"
 
@@ -39,6 +43,10 @@ __device__ __forceinline__ uint32_t path_of(const uint32_t* __restrict__ offsets
     if (offsets[mid] <= v) lo = mid; else hi = mid;
   }
   return lo;
+}
+
+__global__ void k_path_clear(unsigned long long* __restrict__ totals) {
+  if (threadIdx.x < 3) totals[threadIdx.x] = 0ull;
 }
 
 __global__ void __launch_bounds__(kPathThreads) k_path_check(const uint32_t* __restrict__ offsets, uint32_t numPaths,
@@ -228,7 +236,7 @@ void launch_path_records(const PathVertex* segs, uint32_t numSegments, const uin
                          const PathSampleOptions& o, PathSample* records, unsigned long long* flags, uint32_t* pathOf,
                          uint32_t* tileFirst, unsigned long long* incl, void* scanTemp, size_t scanBytes,
                          unsigned long long* totals, hipStream_t s) {
-  PGL_KERNEL_CHECK(hipMemsetAsync(totals, 0, 3 * sizeof(unsigned long long), s));
+  hipLaunchKernelGGL(k_path_clear, dim3(1), dim3(64), 0, s, totals);
   if (numPaths == 0 || numSegments == 0) return;
   hipLaunchKernelGGL(k_path_check, dim3((numPaths + kPathThreads - 1) / kPathThreads), dim3(kPathThreads), 0, s,
                      offsets, numPaths, numSegments, totals);
@@ -247,6 +255,88 @@ void launch_path_scatter(const PathSample* records, const unsigned long long* fl
   if (numSegments == 0) return;
   hipLaunchKernelGGL(k_path_scatter, dim3(path_grid(numSegments)), dim3(kPathThreads), 0, s, records, flags, pathOf,
                      incl, offsets, numSegments, surfOut, volOut);
+}
+
+// ---- the variant that allocates nothing
+//
+// Its scratch (records, flags, their scan, path of each vertex, tile table,
+// scan temporary storage, totals) is carved from one file-scope device array
+// of a fixed size; each GPU holds its own copy for the life of the process.
+// Host code reaches it only through the address hipGetSymbolAddress returns
+// on that device.
+
+namespace {
+
+constexpr size_t na_align(size_t b) { return (b + 255) & ~size_t(255); }
+constexpr size_t kNaRecords = 0;
+constexpr size_t kNaFlags = kNaRecords + na_align(size_t(kNoAllocMaxVertices) * sizeof(PathSample));
+constexpr size_t kNaIncl = kNaFlags + na_align(size_t(kNoAllocMaxVertices) * 8);
+constexpr size_t kNaPathOf = kNaIncl + na_align(size_t(kNoAllocMaxVertices) * 8);
+constexpr size_t kNaTiles = kNaPathOf + na_align(size_t(kNoAllocMaxVertices) * 4);
+constexpr size_t kNaTotals = kNaTiles + na_align(size_t(kNoAllocMaxVertices / kPathThreads + 1) * 4);
+constexpr size_t kNaScan = kNaTotals + 256;
+constexpr size_t kNaScanBytes = size_t(1) << 18;  // the scan's temporary storage (checked at set-up)
+constexpr size_t kNaBytes = kNaScan + kNaScanBytes;
+static_assert(kNaBytes == kNoAllocScratchBytes, "scratch layout and its declared size differ");
+constexpr int kNaMaxDevices = 64;
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+// this device's copy of scratch, looked up once per device (with it current)
+static unsigned char* noalloc_scratch() {
+  static std::mutex lock;
+  static unsigned char* ptr[kNaMaxDevices] = {};
+  int dev = 0;
+  PGL_KERNEL_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNaMaxDevices) throw std::runtime_error("openpgl_amd: device index out of range");
+  std::lock_guard<std::mutex> g(lock);
+  if (!ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+      throw std::runtime_error(std::string("openpgl_amd: hipGetSymbolAddress(scratch) failed: ") +
+                               hipGetErrorString(e));
+    if (path_scan_temp_bytes(kNoAllocMaxVertices) > kNaScanBytes)
+      throw std::runtime_error("openpgl_amd: the path scan needs more temporary storage than the static scratch holds");
+    ptr[dev] = static_cast<unsigned char*>(p);
+  }
+  return ptr[dev];
+}
+
+void path_prepare_noalloc(const PathVertex* segs, uint32_t numSegments, const uint32_t* offsets, uint32_t numPaths,
+                          const PathSampleOptions& o, PathSample* surfOut, size_t surfCap, PathSample* volOut,
+                          size_t volCap, size_t& numSurf, size_t& numVol, hipStream_t s) {
+  numSurf = numVol = 0;
+  if (numSegments > kNoAllocMaxVertices)
+    throw std::runtime_error("openpgl_amd: the allocation-free path batch takes at most " +
+                             std::to_string(kNoAllocMaxVertices) + " vertices per call (got " +
+                             std::to_string(numSegments) + ")");
+  unsigned char* base = noalloc_scratch();
+  const uint32_t n = numPaths ? numSegments : 0;
+  PathSample* records = reinterpret_cast<PathSample*>(base + kNaRecords);
+  unsigned long long* flags = reinterpret_cast<unsigned long long*>(base + kNaFlags);
+  unsigned long long* incl = reinterpret_cast<unsigned long long*>(base + kNaIncl);
+  uint32_t* pathOf = reinterpret_cast<uint32_t*>(base + kNaPathOf);
+  uint32_t* tiles = reinterpret_cast<uint32_t*>(base + kNaTiles);
+  unsigned long long* totals = reinterpret_cast<unsigned long long*>(base + kNaTotals);
+  const size_t scanBytes = path_scan_temp_bytes(n > 0 ? n : 1u);
+  if (scanBytes > kNaScanBytes) throw std::runtime_error("openpgl_amd: path scan storage exceeds the static scratch");
+  launch_path_records(segs, n, offsets, numPaths, o, records, flags, pathOf, tiles, incl, base + kNaScan, scanBytes,
+                      totals, s);
+  unsigned long long t[3] = {0ull, 0ull, 0ull};
+  PGL_KERNEL_CHECK(hipMemcpyAsync(t, totals, sizeof(t), hipMemcpyDeviceToHost, s));
+  PGL_KERNEL_CHECK(hipStreamSynchronize(s));
+  if (t[2]) throw std::runtime_error("openpgl_amd: path offsets must be non-decreasing and within the segments");
+  numSurf = (size_t)t[0];
+  numVol = (size_t)t[1];
+  if (numSurf > surfCap || numVol > volCap)
+    throw std::runtime_error("openpgl_amd: path samples exceed the output capacity (" + std::to_string(numSurf) +
+                             " surface, " + std::to_string(numVol) + " volume)");
+  if (numSurf == 0 && numVol == 0) return;
+  launch_path_scatter(records, flags, pathOf, incl, offsets, n, surfOut, volOut, s);
+  PGL_KERNEL_CHECK(hipStreamSynchronize(s));
 }
 
 }  // namespace pglgpu
