@@ -179,6 +179,14 @@ void pglReleaseFieldStatistics(PGLFieldStatistics fieldStatistics);
 PGLString pglFieldStatisticsToString(PGLFieldStatistics fieldStatistics);
 PGLString pglFieldStatisticsHeaderCSVString(PGLFieldStatistics fieldStatistics);
 PGLString pglFieldStatisticsToCSVString(PGLFieldStatistics fieldStatistics);
+/* EXTENSION: the FieldStatistics figures as numbers (field/FieldStatistics.h:16-30):
+ * [0] numCacheRegions [1] numCacheRegionsReserved [2] sizePerCacheRegions
+ * [3] timeLastUpdate [4] timeLastUpdateCopySamples [5] ...SpatialStructureUpdate
+ * [6] ...DirectionalDistriubtionUpdate (ms) [7] kd maxDepth [8] numberOfNodes
+ * [9] numberOfReservedNodes [10] sizePerNode [11] sizePerDistribution
+ * [12] min [13] max [14] average [15] second moment (root mean square) of the
+ * component counts; count <= 16 values are written. */
+void pglFieldStatisticsGetValues(PGLFieldStatistics fieldStatistics, double* out, size_t count);
 
 /* samplestorage.h:35-65 */
 PGLSampleStorage pglNewSampleStorage(void);
@@ -273,6 +281,14 @@ PGLRegion pglVolumeSamplingGetRegion(PGLVolumeSamplingDistribution volumeSamplin
  * for it.  The device records must stay unchanged until then. */
 void pglFieldUpdateSurfaceDevice(PGLField field, const PGLSampleData* deviceSamples, size_t numSamples);
 void pglFieldUpdateVolumeDevice(PGLField field, const PGLSampleData* deviceSamples, size_t numSamples);
+/* EXTENSION: pglFieldUpdate (api/field.h, Field::updateField) with mixed
+ * surface and volume records in device memory: they are split on the
+ * device by PGL_SAMPLE_FLAG_INSIDE_VOLUME, each part in input order, as a
+ * sample storage routes them (SampleDataStorage.h:72-90), then the surface
+ * and the volume field are updated (an empty part leaves its field as it
+ * is).  Returns once the records have been split (they may be reused then);
+ * the training work is queued as for pglFieldUpdateSurfaceDevice. */
+void pglFieldUpdateDevice(PGLField field, const PGLSampleData* deviceSamples, size_t numSamples);
 
 /* EXTENSION: batched Init + product + Sample + PDF for n queries on the GPU.
  * All pointers are device pointers: pos/normalOrDir (3n floats), g (n, HG mean
@@ -327,6 +343,13 @@ void pglFieldGetKNN(PGLField field, int kind, uint32_t* neighbours /* 8 per regi
  * [18] region kernel GPU time (events around its launch only).
  * Writes min(count, 19) values. */
 void pglFieldGetLastUpdateInfo(PGLField field, int kind, double* out, size_t count);
+/* EXTENSION (testing): the next update of the field kind (0 surface, 1 volume)
+ * is reported as failed once its kernels have run, as a region kernel whose
+ * team barrier timed out is.  A failed update is reported by the call that
+ * completes it; from then on every update, query, export and file write of
+ * the field fails (prints the error, returns false / 0 / nothing) until
+ * pglFieldReset. */
+void pglFieldDebugFailNextUpdate(PGLField field, int kind);
 /* EXTENSION: the same figures for the last min(maxEntries, 256) updates of
  * one kind, oldest first, 19 doubles each ([13] is 0); returns the count.
  * Updates are queued (pglFieldUpdate* returns once the directional phase is

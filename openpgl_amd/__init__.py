@@ -6,6 +6,7 @@ The library is the MI355X implementation; importing this module never falls
 back to a CPU path: if the shared library is missing, loading raises.
 """
 import ctypes
+import weakref
 import os
 
 import numpy as np
@@ -105,8 +106,21 @@ class PGLRegionRecord(ctypes.Structure):
         return out
 
 
+class PGLString(ctypes.Structure):
+    _fields_ = [("m_str", ctypes.c_void_p), ("m_size", ctypes.c_size_t)]
+
+
 # name: (restype, argtypes)
 _SIGS = {
+    "pglReleaseString": (None, [PGLString]),
+    "pglFieldGetSurfaceStatistics": (_vp, [_vp]),
+    "pglFieldGetVolumeStatistics": (_vp, [_vp]),
+    "pglReleaseFieldStatistics": (None, [_vp]),
+    "pglFieldStatisticsToString": (PGLString, [_vp]),
+    "pglFieldStatisticsHeaderCSVString": (PGLString, [_vp]),
+    "pglFieldStatisticsToCSVString": (PGLString, [_vp]),
+    "pglFieldStatisticsGetValues": (None, [_vp, _vp, ctypes.c_size_t]),
+    "pglFieldDebugFailNextUpdate": (None, [_vp, ctypes.c_int]),
     "pglFieldArgumentsSetDefaults": (None, [ctypes.POINTER(PGLFieldArguments), ctypes.c_int, ctypes.c_int,
                                             ctypes.c_bool, ctypes.c_size_t]),
     "pglNewDevice": (_vp, [ctypes.c_int, ctypes.c_size_t]),
@@ -122,6 +136,7 @@ _SIGS = {
     "pglFieldUpdateSurface": (None, [_vp, _vp]),
     "pglFieldUpdateVolume": (None, [_vp, _vp]),
     "pglFieldUpdateSurfaceDevice": (None, [_vp, _vp, ctypes.c_size_t]),
+    "pglFieldUpdateDevice": (None, [_vp, _vp, ctypes.c_size_t]),
     "pglFieldUpdateVolumeDevice": (None, [_vp, _vp, ctypes.c_size_t]),
     "pglFieldReset": (None, [_vp]),
     "pglFieldValidate": (ctypes.c_bool, [_vp]),
@@ -227,6 +242,22 @@ class RegionSchedule(ctypes.Structure):
 _lib = None
 
 
+def _own(obj, release):
+    """Ties the library handle obj.h to obj: released when obj is collected,
+    by obj.close(), or at interpreter exit.  The exit pass runs before the
+    interpreter tears its modules down (whose order is arbitrary) and long
+    before the HIP runtime's own teardown, in reverse order of creation, so
+    a server is released before its field and a field before its device."""
+    obj._fin = weakref.finalize(obj, release, obj.h)
+
+
+def _close(obj):
+    fin = getattr(obj, "_fin", None)
+    if fin is not None:
+        fin()
+    obj.h = None
+
+
 def load_library():
     """Load libopenpgl_amd.so from the package directory (raises if absent)."""
     global _lib
@@ -252,17 +283,19 @@ class Device:
         self.h = self.lib.pglNewDevice(device_type, num_threads)
         if not self.h:
             raise RuntimeError("pglNewDevice failed (no GPU?)")
+        _own(self, self.lib.pglReleaseDevice)
 
-    def __del__(self):
-        if getattr(self, "h", None):
-            self.lib.pglReleaseDevice(self.h)
-            self.h = None
+    def close(self):
+        _close(self)
 
 
 class SampleStorage:
     def __init__(self, samples=None):
         self.lib = load_library()
         self.h = self.lib.pglNewSampleStorage()
+        if not self.h:
+            raise RuntimeError("pglNewSampleStorage failed")
+        _own(self, self.lib.pglReleaseSampleStorage)
         if samples is not None:
             self.add_samples(samples)
 
@@ -279,10 +312,8 @@ class SampleStorage:
     def clear(self):
         self.lib.pglSampleStorageClear(self.h)
 
-    def __del__(self):
-        if getattr(self, "h", None):
-            self.lib.pglReleaseSampleStorage(self.h)
-            self.h = None
+    def close(self):
+        _close(self)
 
 
 def _field_arguments(lib, max_samples_per_leaf, deterministic, knn=True, overrides=None,
@@ -336,16 +367,13 @@ class Server:
         self.h = self.lib.pglNewServer(field.h, ctypes.byref(cfg))
         if not self.h:
             raise RuntimeError("pglNewServer failed")
+        _own(self, self.lib.pglReleaseServer)
         self._live = {}
 
     def close(self):
-        if getattr(self, "h", None):
-            self.lib.pglReleaseServer(self.h)
-            self.h = None
-            self._live.clear()
-
-    def __del__(self):
-        self.close()
+        """Waits for the submitted requests, then releases the server."""
+        _close(self)
+        self._live.clear()
 
     def submit(self, pos, nrm, u2, u1=None, g=None, is_volume=None, out=None):
         """Returns (request id, (dir, pdf, ids, integral) output arrays)."""
@@ -402,11 +430,10 @@ class RegionBatch:
         self.h = self.lib.pglNewRegionBatch(device.h, args)
         if not self.h:
             raise RuntimeError("pglNewRegionBatch failed")
+        _own(self, self.lib.pglReleaseRegionBatch)
 
-    def __del__(self):
-        if getattr(self, "h", None):
-            self.lib.pglReleaseRegionBatch(self.h)
-            self.h = None
+    def close(self):
+        _close(self)
 
     def step_device(self, device_ptr, counts, stats7, init_k=None, split_flags=None):
         """device_ptr: region-contiguous PGLSampleData in device memory."""
@@ -469,6 +496,7 @@ class Field:
         self.h = self.lib.pglDeviceNewField(device.h, args)
         if not self.h:
             raise RuntimeError("pglDeviceNewField failed")
+        _own(self, self.lib.pglReleaseField)
 
     @classmethod
     def from_file(cls, device, path):
@@ -480,12 +508,11 @@ class Field:
         self.h = self.lib.pglDeviceNewFieldFromFile(device.h, os.fsencode(path))
         if not self.h:
             raise RuntimeError("pglDeviceNewFieldFromFile failed: %s" % path)
+        _own(self, self.lib.pglReleaseField)
         return self
 
-    def __del__(self):
-        if getattr(self, "h", None):
-            self.lib.pglReleaseField(self.h)
-            self.h = None
+    def close(self):
+        _close(self)
 
     def store(self, path):
         """pglFieldStoreToFile (layout of the field's device's SIMD width)."""
@@ -511,6 +538,11 @@ class Field:
     def update_volume_device(self, device_ptr, n):
         self.lib.pglFieldUpdateVolumeDevice(self.h, ctypes.c_void_p(device_ptr), n)
 
+    def update_device(self, device_ptr, n):
+        """pglFieldUpdate with mixed records in device memory (split by the
+        EInsideVolume flag on the device)."""
+        self.lib.pglFieldUpdateDevice(self.h, ctypes.c_void_p(device_ptr), n)
+
     def update_from_path_segments_device(self, segments_ptr, num_segments, offsets_ptr, num_paths, max_distance=1e6,
                                          use_nee_mi_weights=False, guide_direct_light=False, rr_affects_direct=True):
         self.lib.pglFieldUpdateFromPathSegmentsDevice(self.h, ctypes.c_void_p(segments_ptr), num_segments,
@@ -525,6 +557,35 @@ class Field:
 
     def reset(self):
         self.lib.pglFieldReset(self.h)
+
+    _STAT_KEYS = ["numCacheRegions", "numCacheRegionsReserved", "sizePerCacheRegions", "timeLastUpdate",
+                  "timeLastUpdateCopySamples", "timeLastUpdateSpatialStructureUpdate",
+                  "timeLastUpdateDirectionalDistriubtionUpdate", "maxDepth", "numberOfNodes",
+                  "numberOfReservedNodes", "sizePerNode", "sizePerDistribution", "minNumberOfComponents",
+                  "maxNumberOfComponents", "averageNumberOfComponents", "secondMomentNumberOfComponents"]
+
+    def statistics(self, kind=0):
+        """pglFieldGet{Surface,Volume}Statistics: (values by name, toString,
+        headerCSVString, toCSVString), or None on error."""
+        h = (self.lib.pglFieldGetVolumeStatistics if kind else self.lib.pglFieldGetSurfaceStatistics)(self.h)
+        if not h:
+            return None
+        try:
+            v = np.zeros(16, np.float64)
+            self.lib.pglFieldStatisticsGetValues(h, v.ctypes.data, 16)
+            strings = []
+            for fn in (self.lib.pglFieldStatisticsToString, self.lib.pglFieldStatisticsHeaderCSVString,
+                       self.lib.pglFieldStatisticsToCSVString):
+                ps = fn(h)
+                strings.append(ctypes.string_at(ps.m_str, ps.m_size).decode() if ps.m_str else "")
+                self.lib.pglReleaseString(ps)
+        finally:
+            self.lib.pglReleaseFieldStatistics(h)
+        return (dict(zip(self._STAT_KEYS, v.tolist())), *strings)
+
+    def debug_fail_next_update(self, kind=0):
+        """pglFieldDebugFailNextUpdate (testing)."""
+        self.lib.pglFieldDebugFailNextUpdate(self.h, kind)
 
     def set_region_schedule(self, per_region_launches=None, grid=None, balance=None, min_chunks_per_rank=None,
                             max_team=None, cooperative=None):
@@ -629,6 +690,9 @@ class _Dist:
         self.lib = field.lib
         self.field = field
         self.h = getattr(self.lib, "pglFieldNew%sSamplingDistribution" % self._kind)(field.h)
+        if not self.h:
+            raise RuntimeError("pglFieldNew%sSamplingDistribution failed" % self._kind)
+        _own(self, getattr(self.lib, "pglRelease%sSamplingDistribution" % self._kind))
 
     def init(self, pos, sample1d=-1.0):
         u = ctypes.c_float(sample1d)
@@ -651,10 +715,8 @@ class _Dist:
     def id(self):
         return getattr(self.lib, "pgl%sSamplingDistributionGetId" % self._kind)(self.h)
 
-    def __del__(self):
-        if getattr(self, "h", None):
-            getattr(self.lib, "pglRelease%sSamplingDistribution" % self._kind)(self.h)
-            self.h = None
+    def close(self):
+        _close(self)
 
 
 class SurfaceSamplingDistribution(_Dist):
@@ -685,14 +747,15 @@ class PathSegmentStorage:
     def __init__(self, capacity=64, max_distance=None):
         self.lib = load_library()
         self.h = self.lib.pglNewPathSegmentStorage()
+        if not self.h:
+            raise RuntimeError("pglNewPathSegmentStorage failed")
+        _own(self, self.lib.pglReleasePathSegmentStorage)
         self.lib.pglPathSegmentStorageReserve(self.h, capacity)
         if max_distance is not None:
             self.lib.pglPathSegmentSetMaxDistance(self.h, float(max_distance))
 
-    def __del__(self):
-        if getattr(self, "h", None):
-            self.lib.pglReleasePathSegmentStorage(self.h)
-            self.h = None
+    def close(self):
+        _close(self)
 
     def add_segments(self, segments):
         seg = np.ascontiguousarray(segments, PATH_SEGMENT_DTYPE)
@@ -767,11 +830,10 @@ class PathSegmentBatch:
         self.h = self.lib.pglNewPathSegmentBatch()
         if not self.h:
             raise RuntimeError("pglNewPathSegmentBatch failed")
+        _own(self, self.lib.pglReleasePathSegmentBatch)
 
-    def __del__(self):
-        if getattr(self, "h", None):
-            self.lib.pglReleasePathSegmentBatch(self.h)
-            self.h = None
+    def close(self):
+        _close(self)
 
     def prepare_device(self, segments_ptr, num_segments, offsets_ptr, num_paths, surface_ptr, surface_capacity,
                        volume_ptr, volume_capacity, max_distance=1e6, use_nee_mi_weights=False,

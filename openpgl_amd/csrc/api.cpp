@@ -3,7 +3,10 @@
 //
 // Error behaviour follows the reference: creation, update, reset, release and
 // file functions catch exceptions, print them to stdout and return
-// nullptr/false/0 (api.cpp:33-72); query functions are unguarded.
+// nullptr/false/0 (api.cpp:33-72).  Field queries, exports and statistics
+// do the same here, since they may have to report a device error or an
+// update that failed (the field then refuses every use until
+// pglFieldReset); the scalar distribution calls cannot fail.
 // Scalar Init/Sample/PDF calls are served from a host mirror of the device
 // field (refreshed lazily after each update); batched queries run on the GPU.
 #include <cstdio>
@@ -39,8 +42,7 @@ struct PGLSurfaceSamplingDistributionImpl : HostDist {};
 struct PGLVolumeSamplingDistributionImpl : HostDist {};
 
 struct PGLFieldStatisticsImpl {
-  UpdateInfo info;
-  uint32_t regions, nodes;
+  KindField::Statistics st;
 };
 
 static PGLString make_string(const std::string& s) {
@@ -257,8 +259,8 @@ static PGLFieldImpl* new_field(PGLDevice device, PGLFieldArguments args) {
 
 static void release_field(PGLFieldImpl* field) {
   delete field->paths;
-  field->gf->surface.finish();
-  field->gf->volume.finish();
+  field->gf->surface.drain();
+  field->gf->volume.drain();
   if (field->gf->stream) (void)hipStreamDestroy(field->gf->stream);
   delete field->gf;
   delete field;
@@ -311,15 +313,16 @@ size_t pglFieldGetIteration(PGLField field) PGL_CATCH_BEGIN {
 }
 PGL_CATCH_END(0)
 
-void pglFieldSetSceneBounds(PGLField field, pgl_box3f bounds) {
+void pglFieldSetSceneBounds(PGLField field, pgl_box3f bounds) PGL_CATCH_BEGIN {
   // SurfaceVolumeField::setSceneBounds: enlarge_by(1.01f) returns a new box and is discarded
   const float lo[3] = {bounds.lower.x, bounds.lower.y, bounds.lower.z};
   const float hi[3] = {bounds.upper.x, bounds.upper.y, bounds.upper.z};
   field->gf->surface.setSceneBounds(lo, hi);
   field->gf->volume.setSceneBounds(lo, hi);
 }
+PGL_CATCH_END_VOID
 
-pgl_box3f pglFieldGetSceneBounds(PGLField field) {
+pgl_box3f pglFieldGetSceneBounds(PGLField field) PGL_CATCH_BEGIN {
   float a[3], b[3], c[3], d[3];
   field->gf->surface.sceneBounds(a, b);
   field->gf->volume.sceneBounds(c, d);
@@ -328,6 +331,7 @@ pgl_box3f pglFieldGetSceneBounds(PGLField field) {
   r.upper.x = std::max(b[0], d[0]); r.upper.y = std::max(b[1], d[1]); r.upper.z = std::max(b[2], d[2]);
   return r;
 }
+PGL_CATCH_END(pgl_box3f{})
 
 void pglFieldUpdate(PGLField field, PGLSampleStorage storage) PGL_CATCH_BEGIN {
   THROW_IF_NULL(field, "handle");
@@ -387,10 +391,11 @@ PGLSurfaceSamplingDistribution pglFieldNewSurfaceSamplingDistribution(PGLField f
 PGL_CATCH_END(nullptr)
 
 bool pglFieldInitSurfaceSamplingDistribution(PGLField field, PGLSurfaceSamplingDistribution d, pgl_point3f position,
-                                             float* sample1D) {
+                                             float* sample1D) PGL_CATCH_BEGIN {
   const float p[3] = {position.x, position.y, position.z};
   return dist_init(d, field->gf->surface, field->gf->stream, p, sample1D);
 }
+PGL_CATCH_END(false)
 
 PGLVolumeSamplingDistribution pglFieldNewVolumeSamplingDistribution(PGLField field) PGL_CATCH_BEGIN {
   THROW_IF_NULL(field, "handle");
@@ -401,17 +406,19 @@ PGLVolumeSamplingDistribution pglFieldNewVolumeSamplingDistribution(PGLField fie
 PGL_CATCH_END(nullptr)
 
 bool pglFieldInitVolumeSamplingDistribution(PGLField field, PGLVolumeSamplingDistribution d, pgl_point3f position,
-                                            float* sample1D) {
+                                            float* sample1D) PGL_CATCH_BEGIN {
   const float p[3] = {position.x, position.y, position.z};
   return dist_init(d, field->gf->volume, field->gf->stream, p, sample1D);
 }
+PGL_CATCH_END(false)
 
-bool pglFieldValidate(PGLField field) {
+bool pglFieldValidate(PGLField field) PGL_CATCH_BEGIN {
   bool ok = true;
   if (field->gf->surface.initialized()) { field->gf->surface.syncMirror(field->gf->stream); ok = ok && field->gf->surface.validate(); }
   if (field->gf->volume.initialized()) { field->gf->volume.syncMirror(field->gf->stream); ok = ok && field->gf->volume.validate(); }
   return ok;
 }
+PGL_CATCH_END(false)
 
 static bool kind_equal(const KindField& a, const KindField& b, hipStream_t sa, hipStream_t sb) {
   if (a.initialized() != b.initialized() || a.iteration() != b.iteration()) return false;
@@ -440,37 +447,107 @@ static bool kind_equal(const KindField& a, const KindField& b, hipStream_t sa, h
   return true;
 }
 
-bool pglFieldCompare(PGLField fieldA, PGLField fieldB) {  // Field::operator== after the canonical reorder
+bool pglFieldCompare(PGLField fieldA, PGLField fieldB) PGL_CATCH_BEGIN {  // Field::operator== after the canonical reorder
   return kind_equal(fieldA->gf->surface, fieldB->gf->surface, fieldA->gf->stream, fieldB->gf->stream) &&
          kind_equal(fieldA->gf->volume, fieldB->gf->volume, fieldA->gf->stream, fieldB->gf->stream) &&
          fieldA->gf->iteration == fieldB->gf->iteration;
 }
+PGL_CATCH_END(false)
 
-static PGLFieldStatistics make_stats(const KindField& k) {
-  PGLFieldStatistics s = new PGLFieldStatisticsImpl();
-  s->info = k.lastInfo();
-  s->regions = k.numRegions();
-  s->nodes = (uint32_t)k.nodes().size();
-  return s;
+static PGLFieldStatistics make_stats(const KindField& k, hipStream_t s) {
+  PGLFieldStatistics r = new PGLFieldStatisticsImpl();
+  try {
+    r->st = k.statistics(s);
+  } catch (...) {
+    delete r;
+    throw;
+  }
+  return r;
 }
-PGLFieldStatistics pglFieldGetSurfaceStatistics(PGLField field) { return make_stats(field->gf->surface); }
-PGLFieldStatistics pglFieldGetVolumeStatistics(PGLField field) { return make_stats(field->gf->volume); }
+PGLFieldStatistics pglFieldGetSurfaceStatistics(PGLField field) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(field, "handle");
+  return make_stats(field->gf->surface, field->gf->stream);
+}
+PGL_CATCH_END(nullptr)
+PGLFieldStatistics pglFieldGetVolumeStatistics(PGLField field) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(field, "handle");
+  return make_stats(field->gf->volume, field->gf->stream);
+}
+PGL_CATCH_END(nullptr)
 void pglReleaseFieldStatistics(PGLFieldStatistics s) { delete s; }
-PGLString pglFieldStatisticsToString(PGLFieldStatistics s) {
+
+// FieldStatistics::toString / headerCSVString / toCSVString with
+// KDTreeStatistics and DirectionalDistributionStatistics
+// (field/FieldStatistics.h:34-97, KDTreeStatistics.h, DirectionalDistributionStatistics.h)
+PGLString pglFieldStatisticsToString(PGLFieldStatistics s) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(s, "handle");
+  const KindField::Statistics& t = s->st;
   std::ostringstream o;
-  o << "FieldStatistics:\n  numRegions: " << s->regions << "\n  numNodes: " << s->nodes
-    << "\n  timeLastUpdate[ms]: " << s->info.wallMs << "\n  timeSpatialStructureUpdate[ms]: " << s->info.spatialMs
-    << "\n  timeDirectionalDistriubtionUpdate[ms]: " << s->info.directionalMs << "\n";
+  o << "FieldStatistics:\n";
+  o << "\tnumCacheRegions = " << t.numRegions << "\n";
+  o << "\tnumCacheRegionsReserved = " << t.regionsReserved << "\n";
+  o << "\tsizePerCacheRegions = " << t.sizePerRegion << " bs\n";
+  o << "\tsizeAllCacheRegionsUsed = " << float(t.sizePerRegion * t.numRegions) / 1024 / 1024 << " Mbs\n";
+  o << "\tsizeAllCacheRegionsReserved = " << float(t.sizePerRegion * t.regionsReserved) / 1024 / 1024 << " Mbs\n";
+  o << "\ttimeUpdate = " << t.timeUpdate << " ms\n";
+  o << "\ttimeCopySamples = " << t.timeCopySamples << " ms\n";
+  o << "\ttimeSpatialStructureUpdate = " << t.timeSpatial << " ms\n";
+  o << "\ttimeDirectionalDistriubtionUpdate= " << t.timeDirectional << " ms\n";
+  o << "KDTreeStatistics: \n";
+  o << "\tnumberOfNodes            = " << t.kdNodes << "\n";
+  o << "\tnumberOfReservedNodes    = " << t.kdNodesReserved << "\n";
+  o << "\tmaxDepth                 = " << t.kdMaxDepth << "\n";
+  o << "\tsizePerNode              = " << t.kdSizePerNode << " bs\n";
+  o << "\tsizeAllNodesUsed         = " << float(t.kdNodes * t.kdSizePerNode) / 1024 << " kbs\n";
+  o << "\tsizeAllNodesReserved     = " << float(t.kdNodesReserved * t.kdSizePerNode) / 1024 << " kbs\n";
+  o << "DirectionalDistributionStatistics: \n";
+  o << "\tsizePerDistribution              = " << t.sizePerDistribution << " bs\n";
+  o << "\tminNumberOfComponents            = " << t.minK << "\n";
+  o << "\tmaxNumberOfComponents            = " << t.maxK << "\n";
+  o << "\taverageNumberOfComponents        = " << t.averageK << "\n";
+  o << "\tsecondMomentNumberOfComponents   = " << t.secondMomentK << "\n";
   return make_string(o.str());
 }
-PGLString pglFieldStatisticsHeaderCSVString(PGLFieldStatistics) {
-  return make_string("numRegions,numNodes,timeLastUpdate,timeSpatialStructureUpdate,timeDirectionalDistriubtionUpdate");
+PGL_CATCH_END(PGLString{})
+PGLString pglFieldStatisticsHeaderCSVString(PGLFieldStatistics s) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(s, "handle");
+  return make_string(
+      "FieldStatistics: , numCacheRegions , numCacheRegionsReserved , sizePerCacheRegions(bs) , "
+      "sizeAllCacheRegionsUsed(Mbs) , sizeAllCacheRegionsReserved(Mbs) , timeUpdate(ms) , timeCopySamples(ms) , "
+      "timeSpatialStructureUpdate(ms) , timeDirectionalDistriubtionUpdate(ms) , KDTreeStatistics: , numberOfNodes , "
+      "numberOfReservedNodes , maxDepth , sizePerNode , sizeAllNodesUsed , sizeAllNodesReserved , "
+      "DirectionalDistributionStatistics: , sizePerDistribution , minNumberOfComponents , maxNumberOfComponents , "
+      "averageNumberOfComponents , secondMomentNumberOfComponents , ");
 }
-PGLString pglFieldStatisticsToCSVString(PGLFieldStatistics s) {
+PGL_CATCH_END(PGLString{})
+PGLString pglFieldStatisticsToCSVString(PGLFieldStatistics s) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(s, "handle");
+  const KindField::Statistics& t = s->st;
   std::ostringstream o;
-  o << s->regions << "," << s->nodes << "," << s->info.wallMs << "," << s->info.spatialMs << "," << s->info.directionalMs;
+  o << "  , " << t.numRegions << " , " << t.regionsReserved << " , " << t.sizePerRegion << " , "
+    << float(t.sizePerRegion * t.numRegions) / 1024 / 1024 << " , "
+    << float(t.sizePerRegion * t.regionsReserved) / 1024 / 1024 << " , " << t.timeUpdate << " , "
+    << t.timeCopySamples << " , " << t.timeSpatial << " , " << t.timeDirectional << " , ";
+  o << "  , " << t.kdNodes << " , " << t.kdNodesReserved << " , " << t.kdMaxDepth << " , " << t.kdSizePerNode
+    << " , " << t.kdNodes * t.kdSizePerNode << " , " << t.kdNodesReserved * t.kdSizePerNode << " , ";
+  o << "  , " << t.sizePerDistribution << " , " << t.minK << " , " << t.maxK << " , " << t.averageK << " , "
+    << t.secondMomentK << " , ";
   return make_string(o.str());
 }
+PGL_CATCH_END(PGLString{})
+
+// EXTENSION (testing): the figures of FieldStatistics as numbers
+void pglFieldStatisticsGetValues(PGLFieldStatistics s, double* out, size_t count) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(s, "handle");
+  const KindField::Statistics& t = s->st;
+  const double v[16] = {(double)t.numRegions, (double)t.regionsReserved, (double)t.sizePerRegion,
+                        t.timeUpdate, t.timeCopySamples, t.timeSpatial, t.timeDirectional,
+                        (double)t.kdMaxDepth, (double)t.kdNodes, (double)t.kdNodesReserved,
+                        (double)t.kdSizePerNode, (double)t.sizePerDistribution, t.minK, t.maxK, t.averageK,
+                        t.secondMomentK};
+  for (size_t j = 0; j < count && j < 16; j++) out[j] = v[j];
+}
+PGL_CATCH_END_VOID
 
 // ---- sample storage
 PGLSampleStorage pglNewSampleStorage(void) PGL_CATCH_BEGIN { return new PGLSampleStorageImpl(); }
@@ -639,12 +716,13 @@ PGLRegion pglVolumeSamplingGetRegion(PGLVolumeSamplingDistribution d) { return (
 // ---- batched queries
 void pglFieldSampleBatchDevice(PGLField field, size_t n, const float* pos, const float* nrm, const float* g,
                                const float* u2, const float* u1, const uint8_t* isVolume, float* dirOut, float* pdfOut,
-                               uint32_t* idOut, float* integralOut, void* stream) {
+                               uint32_t* idOut, float* integralOut, void* stream) PGL_CATCH_BEGIN {
   const QueryField sf = field->gf->surface.queryView();
   const QueryField vf = field->gf->volume.queryView();
   launch_query(sf, vf, field->gf->dHG.as<HGRep>(), (uint32_t)n, pos, nrm, g, u2, u1, isVolume, dirOut, pdfOut, idOut,
                integralOut, (hipStream_t)stream);
 }
+PGL_CATCH_END_VOID
 
 void pglFieldSampleBatch(PGLField field, size_t n, const float* pos, const float* nrm, const float* g, const float* u2,
                          const float* u1, const uint8_t* isVolume, float* dirOut, float* pdfOut, uint32_t* idOut,
@@ -681,17 +759,29 @@ PGL_CATCH_END_VOID
 // ---- introspection
 static KindField& kind_of(PGLField f, int kind) { return kind == 0 ? f->gf->surface : f->gf->volume; }
 
-uint32_t pglFieldGetNumNodes(PGLField field, int kind) { return (uint32_t)kind_of(field, kind).nodes().size(); }
-uint32_t pglFieldGetNumRegions(PGLField field, int kind) { return kind_of(field, kind).numRegions(); }
+// both complete a queued update first (and so report a failed one)
+uint32_t pglFieldGetNumNodes(PGLField field, int kind) PGL_CATCH_BEGIN {
+  const KindField& k = kind_of(field, kind);
+  k.finish();
+  return (uint32_t)k.nodes().size();
+}
+PGL_CATCH_END(0)
+uint32_t pglFieldGetNumRegions(PGLField field, int kind) PGL_CATCH_BEGIN {
+  const KindField& k = kind_of(field, kind);
+  k.finish();
+  return k.numRegions();
+}
+PGL_CATCH_END(0)
 
-void pglFieldExportNodes(PGLField field, int kind, float* pivots, uint32_t* dims, uint32_t* childOrRegion) {
+void pglFieldExportNodes(PGLField field, int kind, float* pivots, uint32_t* dims, uint32_t* childOrRegion) PGL_CATCH_BEGIN {
   std::vector<KDNode> out;
   std::vector<uint32_t> order;
   kind_of(field, kind).canonicalOrder(out, order);
   for (size_t i = 0; i < out.size(); i++) { pivots[i] = out[i].pivot; dims[i] = out[i].dim(); childOrRegion[i] = out[i].idx(); }
 }
+PGL_CATCH_END_VOID
 
-void pglFieldExportRegion(PGLField field, int kind, uint32_t canonicalIdx, PGLRegionRecord* out) {
+void pglFieldExportRegion(PGLField field, int kind, uint32_t canonicalIdx, PGLRegionRecord* out) PGL_CATCH_BEGIN {
   KindField& k = kind_of(field, kind);
   k.syncMirror(field->gf->stream);
   std::vector<KDNode> nodes;
@@ -700,8 +790,9 @@ void pglFieldExportRegion(PGLField field, int kind, uint32_t canonicalIdx, PGLRe
   static_assert(sizeof(PGLRegionRecord) == sizeof(PGLRegionRecordHost), "record layouts agree");
   k.exportRegion(order[canonicalIdx], (PGLRegionRecordHost*)out);
 }
+PGL_CATCH_END_VOID
 
-void pglFieldSampleLeafIds(PGLField field, int kind, const float* positions, size_t n, uint32_t* ids) {
+void pglFieldSampleLeafIds(PGLField field, int kind, const float* positions, size_t n, uint32_t* ids) PGL_CATCH_BEGIN {
   KindField& k = kind_of(field, kind);
   std::vector<KDNode> nodes;
   std::vector<uint32_t> order;
@@ -713,8 +804,9 @@ void pglFieldSampleLeafIds(PGLField field, int kind, const float* positions, siz
     ids[i] = nodes[j].idx();
   }
 }
+PGL_CATCH_END_VOID
 
-void pglFieldGetKNN(PGLField field, int kind, uint32_t* nb) {
+void pglFieldGetKNN(PGLField field, int kind, uint32_t* nb) PGL_CATCH_BEGIN {
   KindField& k = kind_of(field, kind);
   k.syncMirror(field->gf->stream);
   std::vector<KDNode> nodes;
@@ -730,6 +822,7 @@ void pglFieldGetKNN(PGLField field, int kind, uint32_t* nb) {
       nb[c * 8 + i] = id == 0xffffffffu ? 0xffffffffu : s2c[id];
     }
 }
+PGL_CATCH_END_VOID
 
 static void info_values(const UpdateInfo& i, double* o) {
   o[0] = i.wallMs; o[1] = i.spatialMs; o[2] = i.directionalMs; o[3] = i.knnMs; o[4] = i.gatherMs; o[5] = i.sortMs;
@@ -750,13 +843,20 @@ size_t pglFieldGetUpdateHistory(PGLField field, int kind, double* out, size_t ma
 }
 PGL_CATCH_END(0)
 
-void pglFieldGetLastUpdateInfo(PGLField field, int kind, double* out, size_t count) {
+void pglFieldGetLastUpdateInfo(PGLField field, int kind, double* out, size_t count) PGL_CATCH_BEGIN {
   KindField& k = kind_of(field, kind);
   const UpdateInfo& i = k.lastInfo();
   double o[19];
   info_values(i, o);
   for (size_t j = 0; j < count && j < 19; j++) out[j] = o[j];
 }
+PGL_CATCH_END_VOID
+
+void pglFieldDebugFailNextUpdate(PGLField field, int kind) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(field, "handle");
+  kind_of(field, kind).runner().failNext = true;
+}
+PGL_CATCH_END_VOID
 
 void pglFieldGetRegionProfile(PGLField field, int kind, float* out) PGL_CATCH_BEGIN {
   THROW_IF_NULL(field, "field");
@@ -864,7 +964,8 @@ void pglRegionBatchStep(PGLRegionBatch batch, uint32_t numRegions, const PGLSamp
 }
 PGL_CATCH_END_VOID
 
-uint32_t pglRegionBatchGetNumRegions(PGLRegionBatch batch) { return batch->b->numRegions(); }
+uint32_t pglRegionBatchGetNumRegions(PGLRegionBatch batch) PGL_CATCH_BEGIN { return batch->b->numRegions(); }
+PGL_CATCH_END(0)
 
 void pglRegionBatchExportRegion(PGLRegionBatch batch, uint32_t region, PGLRegionRecord* out) PGL_CATCH_BEGIN {
   THROW_IF_NULL(batch, "handle");

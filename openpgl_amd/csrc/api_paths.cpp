@@ -221,6 +221,33 @@ bool pglPathSegmentBatchPrepareSamplesDevice(PGLPathSegmentBatch batch, const PG
 }
 PGL_CATCH_END(false)
 
+// pglFieldUpdate (SurfaceVolumeField::updateField) with the records in
+// device memory: split by EInsideVolume on the device as a sample storage
+// would route them (SampleDataStorage.h:72-90), then one iteration of the
+// surface and the volume field
+void pglFieldUpdateDevice(PGLField field, const PGLSampleData* deviceSamples, size_t numSamples) PGL_CATCH_BEGIN {
+  THROW_IF_NULL(field, "handle");
+  if (numSamples > 0) THROW_IF_NULL(deviceSamples, "samples");
+  if (numSamples > kMaxUpdateSamples) throw std::runtime_error("openpgl_amd: at most 2^30 samples per update");
+  if (!field->paths) field->paths = new PathSampleBatch();
+  const PathSample *surf = nullptr, *vol = nullptr;
+  size_t ns = 0, nv = 0;
+  field->paths->routeOwned((const PathSample*)deviceSamples, (uint32_t)numSamples, surf, ns, vol, nv);
+  GuidingField& gf = *field->gf;
+  hipEvent_t evS = nullptr, evV = nullptr;
+  if (ns) {
+    gf.surface.update(surf, (uint32_t)ns, gf.stream);
+    evS = gf.surface.lastGatherEvent();
+  }
+  if (nv) {
+    gf.volume.update(vol, (uint32_t)nv, gf.stream);
+    evV = gf.volume.lastGatherEvent();
+  }
+  field->paths->setReleased(evS, evV);
+  gf.iteration++;
+}
+PGL_CATCH_END_VOID
+
 bool pglPathSegmentsPrepareSamplesDeviceNoAlloc(const PGLPathSegmentData* segments, size_t numSegments,
                                                 const uint32_t* pathOffsets, uint32_t numPaths, float maxDistance,
                                                 bool useNEEMiWeights, bool guideDirectLight,

@@ -197,10 +197,7 @@ KindField::KindField(const KdSettings& kd, const FactoryConfig& fc, bool determi
 }
 
 KindField::~KindField() {
-  try {
-    finish();
-  } catch (...) {
-  }
+  drain();
   (void)hipStreamSynchronize(sp_);
   for (int p = 0; p < 2; p++) {
     for (auto& e : evSp_[p]) (void)hipEventDestroy(e);
@@ -219,7 +216,9 @@ void KindField::sceneBounds(float lo[3], float hi[3]) const {
 }
 
 void KindField::reset() {  // Field::resetField (Field.h:222-232)
-  finish();
+  drain();
+  failed_ = false;
+  failMsg_.clear();
   iteration_ = 0;
   boundsSet_ = false;
   initialized_ = false;
@@ -541,6 +540,7 @@ void KindField::renumberDepthFirst(uint32_t nodesBefore, uint32_t regionsBefore)
 // one is queued, and every read of device state finishes the pending update.
 void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
   if (n == 0) return;
+  if (failed_) finish();  // throws: a failed field takes no update until reset()
   const auto t0 = std::chrono::steady_clock::now();
   const int P = parity_;
   UpdateInfo inf;
@@ -715,6 +715,8 @@ void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
   iteration_++;
   mirrorDirty_ = true;
   pendingInfo_ = inf;
+  pendingCopyMs_ = copyMs_;
+  copyMs_ = 0.f;
   pendingRegions_ = R;
   pendingT0_ = t0;
   pendingParity_ = P;
@@ -722,9 +724,24 @@ void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
   parity_ = 1 - P;
 }
 
-// Completes the queued directional phase: waits for it, checks the region
-// kernel's error flag and records the update's figures.
 void KindField::finish() const {
+  if (failed_)
+    throw std::runtime_error("openpgl_amd: " + failMsg_ + "; the field is invalid until pglFieldReset");
+  finishPending();
+}
+
+void KindField::drain() const noexcept {
+  try {
+    finishPending();
+  } catch (...) {
+  }
+}
+
+// Completes the queued directional phase: waits for it, checks the region
+// kernel's error flag and records the update's figures.  A failed update
+// leaves region states that no later call may use: the field is marked
+// failed until reset().
+void KindField::finishPending() const {
   if (!pending_) return;
   const int P = pendingParity_;
   PGL_HIP_CHECK(hipEventSynchronize(evDir_[P][2]));
@@ -747,20 +764,76 @@ void KindField::finish() const {
   i.wallMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - pendingT0_).count();
   info_ = i;
   lastTimes_[0] = (float)i.wallMs;
-  lastTimes_[1] = 0.f;  // records are read in place: no copy step
+  lastTimes_[1] = pendingCopyMs_;  // 0 for records already in device memory
   lastTimes_[2] = (float)(i.spatialMs + i.sortMs + i.gatherMs);
   lastTimes_[3] = (float)(i.directionalMs + i.knnMs);
   history_.push_back(i);
   if (history_.size() > 256) history_.erase(history_.begin());
-  rr.check();
+  try {
+    rr.check();
+  } catch (const std::exception& e) {
+    failed_ = true;
+    failMsg_ = std::string("the last update failed (") + e.what() + ")";
+    throw;
+  }
 }
 
 void KindField::updateFromHost(const void* hostSamples, uint32_t n) {
   if (n == 0) return;
+  finish();  // a failed field takes no update (and the staging buffer is free)
+  const auto t0 = std::chrono::steady_clock::now();
   dAos_.ensure(size_t(n) * kSampleRecordBytes);
   // on the spatial stream, after the previous update's gather read the staging buffer
   PGL_HIP_CHECK(hipMemcpyAsync(dAos_.p, hostSamples, size_t(n) * kSampleRecordBytes, hipMemcpyHostToDevice, sp_));
   PGL_HIP_CHECK(hipStreamSynchronize(sp_));
+  copyMs_ = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+KindField::Statistics KindField::statistics(hipStream_t s) const {
+  syncMirror(s);
+  Statistics st;
+  st.numRegions = regions_.size();
+  st.regionsReserved = std::max<size_t>(stateCapacity_, st.numRegions);
+  st.sizePerRegion = sizeof(RegionState);
+  st.timeUpdate = lastTimes_[0];
+  st.timeCopySamples = lastTimes_[1];
+  st.timeSpatial = lastTimes_[2];
+  st.timeDirectional = lastTimes_[3];
+  // KDTree::calculateMaxDepth (KDTree.h:651-662): a leaf counts 1
+  if (!nodes_.empty()) {
+    std::vector<std::pair<uint32_t, uint32_t>> stack{{0u, 1u}};
+    while (!stack.empty()) {
+      const auto [n, d] = stack.back();
+      stack.pop_back();
+      if (nodes_[n].isLeaf()) {
+        st.kdMaxDepth = std::max<size_t>(st.kdMaxDepth, d);
+      } else {
+        stack.push_back({nodes_[n].idx(), d + 1});
+        stack.push_back({nodes_[n].idx() + 1, d + 1});
+      }
+    }
+  }
+  st.kdNodes = nodes_.size();
+  st.kdNodesReserved = std::max(nodes_.capacity(), dNodes_.bytes / sizeof(KDNode));
+  st.kdSizePerNode = sizeof(KDNode);
+  st.sizePerDistribution = sizeof(VMMParams);
+  // Field::getStatistics: float accumulation in region order, then the mean
+  // and the square root of the mean square (0 / 0 for an empty field)
+  float mn = 1e10f, mx = 0.f, avg = 0.f, sec = 0.f;
+  for (size_t r = 0; r < st.numRegions; r++) {
+    const int k = (int)hostStates_[r].vmm.K;
+    mn = std::min(mn, (float)k);
+    mx = std::max(mx, (float)k);
+    avg += k;
+    sec += k * k;
+  }
+  avg /= float(st.numRegions);
+  sec /= float(st.numRegions);
+  st.minK = mn;
+  st.maxK = mx;
+  st.averageK = avg;
+  st.secondMomentK = std::sqrt(sec);
+  return st;
 }
 
 void PinnedBuf::ensure(size_t b) {

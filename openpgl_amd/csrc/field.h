@@ -106,9 +106,25 @@ class KindField {
     updateFromHost(hostSamples, n);
     update(dAos_.p, n, s);
   }
-  // waits for a queued update and collects its figures
+  // waits for a queued update and collects its figures; throws if that
+  // update failed, and from then on at every use until reset()
   void finish() const;
+  // completes a queued update without throwing (release paths)
+  void drain() const noexcept;
+  bool failed() const { return failed_; }
   void reset();
+
+  // FieldStatistics (field/FieldStatistics.h:16-30, Field.h:604-638):
+  // region storage, the last update's times, KDTreeStatistics and
+  // DirectionalDistributionStatistics
+  struct Statistics {
+    size_t numRegions = 0, regionsReserved = 0, sizePerRegion = 0;
+    float timeUpdate = 0.f, timeCopySamples = 0.f, timeSpatial = 0.f, timeDirectional = 0.f;
+    size_t kdMaxDepth = 0, kdNodes = 0, kdNodesReserved = 0, kdSizePerNode = 0;
+    size_t sizePerDistribution = 0;
+    float minK = 0.f, maxK = 0.f, averageK = 0.f, secondMomentK = 0.f;
+  };
+  Statistics statistics(hipStream_t s) const;
 
   // host mirror of the device state for scalar queries and exports
   void syncMirror(hipStream_t s) const;
@@ -226,6 +242,11 @@ class KindField {
   // Field::m_timeLastUpdate, ...CopySamples, ...SpatialStructureUpdate,
   // ...DirectionalDistriubtionUpdate (ms): the last update's, or a loaded file's
   mutable float lastTimes_[4] = {0.f, 0.f, 0.f, 0.f};
+  float copyMs_ = 0.f;                 // host-to-device copy of the next update's records
+  mutable float pendingCopyMs_ = 0.f;  // of the queued update
+  mutable bool failed_ = false;
+  mutable std::string failMsg_;
+  void finishPending() const;
 };
 
 // Region record layout of the EXTENSION export (mirrors PGLRegionRecord).

@@ -164,6 +164,14 @@ void launch_path_scatter(const PathSample* records, const unsigned long long* fl
                          const unsigned long long* incl, const uint32_t* offsets, uint32_t numSegments,
                          PathSample* surfOut, PathSample* volOut, hipStream_t s);
 
+// Records split by their EInsideVolume flag, each part in input order
+// (SampleDataStorage.h:72-90): flags + scan (totals[0] surface, [1] volume),
+// then the scatter.
+void launch_route_count(const PathSample* recs, uint32_t n, unsigned long long* flags, unsigned long long* incl,
+                        void* scanTemp, size_t scanBytes, unsigned long long* totals, hipStream_t s);
+void launch_route_scatter(const PathSample* recs, const unsigned long long* incl, uint32_t n, PathSample* surfOut,
+                          PathSample* volOut, hipStream_t s);
+
 // The same two passes with every scratch buffer in one static device array
 // per GPU (15.25 MiB, held until the process ends): at most
 // kNoAllocMaxVertices vertices per call, no device or pinned allocation.

@@ -215,6 +215,28 @@ __global__ void __launch_bounds__(kPathThreads) k_path_scatter(const PathSample*
   }
 }
 
+// ---- records routed by their EInsideVolume flag (SampleDataStorage.h:72-90):
+// flag = surface | volume << 32 per record, inclusive scan, stable scatter
+__global__ void __launch_bounds__(kPathThreads) k_route_flags(const PathSample* __restrict__ recs, uint32_t n,
+                                                              unsigned long long* __restrict__ flags) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    flags[i] = (recs[i].flags & 2u) ? (1ull << 32) : 1ull;
+}
+
+__global__ void __launch_bounds__(kPathThreads) k_route_scatter(const PathSample* __restrict__ recs,
+                                                                const unsigned long long* __restrict__ incl,
+                                                                uint32_t n, PathSample* __restrict__ surfOut,
+                                                                PathSample* __restrict__ volOut) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const PathSample r = recs[i];
+    const unsigned long long c = incl[i];
+    if (r.flags & 2u) volOut[(uint32_t)(c >> 32) - 1u] = r;
+    else surfOut[(uint32_t)(c & 0xffffffffull) - 1u] = r;
+  }
+}
+
 uint32_t path_grid(uint32_t n) {
   const uint32_t blocks = (n + kPathThreads - 1) / kPathThreads;
   return blocks < 65536u ? (blocks ? blocks : 1u) : 65536u;
@@ -255,6 +277,21 @@ void launch_path_scatter(const PathSample* records, const unsigned long long* fl
   if (numSegments == 0) return;
   hipLaunchKernelGGL(k_path_scatter, dim3(path_grid(numSegments)), dim3(kPathThreads), 0, s, records, flags, pathOf,
                      incl, offsets, numSegments, surfOut, volOut);
+}
+
+void launch_route_count(const PathSample* recs, uint32_t n, unsigned long long* flags, unsigned long long* incl,
+                        void* scanTemp, size_t scanBytes, unsigned long long* totals, hipStream_t s) {
+  hipLaunchKernelGGL(k_path_clear, dim3(1), dim3(64), 0, s, totals);
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_route_flags, dim3(path_grid(n)), dim3(kPathThreads), 0, s, recs, n, flags);
+  PGL_KERNEL_CHECK(hipcub::DeviceScan::InclusiveSum(scanTemp, scanBytes, flags, incl, (int)n, s));
+  hipLaunchKernelGGL(k_path_totals, dim3(1), dim3(64), 0, s, incl, n, totals);
+}
+
+void launch_route_scatter(const PathSample* recs, const unsigned long long* incl, uint32_t n, PathSample* surfOut,
+                          PathSample* volOut, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_route_scatter, dim3(path_grid(n)), dim3(kPathThreads), 0, s, recs, incl, n, surfOut, volOut);
 }
 
 // ---- the variant that allocates nothing

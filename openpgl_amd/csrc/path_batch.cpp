@@ -68,6 +68,14 @@ void PathSampleBatch::prepareOwned(const PathVertex* segs, uint32_t numSegments,
   const int b = next_;
   count(segs, numSegments, offsets, numPaths, o, numSurf, numVol);
   next_ = 1 - next_;
+  acquireOwned(b, numSurf, numVol);
+  surf = out_[b][0].as<PathSample>();
+  vol = out_[b][1].as<PathSample>();
+  if (numSurf || numVol) scatter(out_[b][0].as<PathSample>(), out_[b][1].as<PathSample>());
+  lastBuffer_ = b;
+}
+
+void PathSampleBatch::acquireOwned(int b, size_t numSurf, size_t numVol) {
   // the buffer's previous consumers (the gathers of an earlier update) must be done with it
   for (int k = 0; k < 2; k++)
     if (pendingRelease_[b][k]) {
@@ -79,9 +87,34 @@ void PathSampleBatch::prepareOwned(const PathVertex* segs, uint32_t numSegments,
     PGL_HIP_CHECK(hipStreamSynchronize(stream_));  // a reallocation frees memory the wait above protects
   out_[b][0].ensure(ns * sizeof(PathSample));
   out_[b][1].ensure(nv * sizeof(PathSample));
+}
+
+void PathSampleBatch::routeOwned(const PathSample* recs, uint32_t n, const PathSample*& surf, size_t& numSurf,
+                                 const PathSample*& vol, size_t& numVol) {
+  const int b = next_;
+  next_ = 1 - next_;
+  totals_.ensure(24);
+  hTotals_.ensure(24);
+  const size_t V = std::max<size_t>(n, 1);
+  flags_.ensure(V * 8);
+  incl_.ensure(V * 8);
+  const size_t scanBytes = path_scan_temp_bytes((uint32_t)V);
+  scanTemp_.ensure(std::max<size_t>(scanBytes, 16));
+  launch_route_count(recs, n, flags_.as<unsigned long long>(), incl_.as<unsigned long long>(), scanTemp_.p,
+                     scanBytes, totals_.as<unsigned long long>(), stream_);
+  PGL_HIP_CHECK(hipMemcpyAsync(hTotals_.p, totals_.p, 24, hipMemcpyDeviceToHost, stream_));
+  PGL_HIP_CHECK(hipStreamSynchronize(stream_));
+  const unsigned long long* t = hTotals_.as<unsigned long long>();
+  numSurf = (size_t)t[0];
+  numVol = (size_t)t[1];
+  acquireOwned(b, numSurf, numVol);
   surf = out_[b][0].as<PathSample>();
   vol = out_[b][1].as<PathSample>();
-  if (numSurf || numVol) scatter(out_[b][0].as<PathSample>(), out_[b][1].as<PathSample>());
+  if (n) {
+    launch_route_scatter(recs, incl_.as<unsigned long long>(), n, out_[b][0].as<PathSample>(),
+                         out_[b][1].as<PathSample>(), stream_);
+    PGL_HIP_CHECK(hipStreamSynchronize(stream_));
+  }
   lastBuffer_ = b;
 }
 

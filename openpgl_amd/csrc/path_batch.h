@@ -32,7 +32,12 @@ class PathSampleBatch {
   void prepareOwned(const PathVertex* segs, uint32_t numSegments, const uint32_t* offsets, uint32_t numPaths,
                     const PathSampleOptions& o, const PathSample*& surf, size_t& numSurf, const PathSample*& vol,
                     size_t& numVol);
-  // events after which the buffer handed out by the last prepareOwned may be overwritten
+  // Mixed surface / volume records (device) split by their EInsideVolume
+  // flag into the batch's double-buffered output, each part in input order,
+  // as a sample storage routes them (SampleDataStorage.h:72-90).
+  void routeOwned(const PathSample* recs, uint32_t n, const PathSample*& surf, size_t& numSurf,
+                  const PathSample*& vol, size_t& numVol);
+  // events after which the buffer handed out by the last prepareOwned / routeOwned may be overwritten
   void setReleased(hipEvent_t surfaceDone, hipEvent_t volumeDone);
 
   hipStream_t stream() const { return stream_; }
@@ -41,6 +46,8 @@ class PathSampleBatch {
   void count(const PathVertex* segs, uint32_t numSegments, const uint32_t* offsets, uint32_t numPaths,
              const PathSampleOptions& o, size_t& numSurf, size_t& numVol);
   void scatter(PathSample* surfOut, PathSample* volOut);
+  // output buffer b for ns / nv records, once its previous consumers are done
+  void acquireOwned(int b, size_t numSurf, size_t numVol);
 
   hipStream_t stream_ = nullptr;
   DevBuf records_, flags_, pathOf_, tiles_, incl_, scanTemp_, totals_;

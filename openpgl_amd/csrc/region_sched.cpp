@@ -216,6 +216,10 @@ float RegionRunner::kernelMs() const {
 }
 
 void RegionRunner::check() const {
+  if (failNext) {
+    failNext = false;
+    throw std::runtime_error("directional update: failure injected for testing");
+  }
   if (*hErr_ != 0)
     throw std::runtime_error("directional update: a region team barrier timed out (workgroups not co-resident)");
 }

@@ -64,6 +64,8 @@ class RegionRunner {
               const RegionInput* inputs = nullptr, uint32_t numInputs = 0);
   // after the stream has completed: throws if a team barrier timed out
   void check() const;
+  // testing: the next check() reports a failed launch (the kernels run normally)
+  mutable bool failNext = false;
   const RegionLaunchStats& stats() const { return *hStats_; }
   // GPU time of the region kernel(s) of the last launch (HIP events around
   // the kernel launches only), valid after the stream has completed

@@ -2,6 +2,8 @@
 #include "server.h"
 
 #include <algorithm>
+#include <cstdlib>
+#include <set>
 #include <stdexcept>
 
 namespace pglgpu {
@@ -75,18 +77,60 @@ QueryServer::QueryServer(GuidingField* gf, const ServerConfig& cfg) : gf_(gf), c
   refresh();
   dispatcher_ = std::thread([this] { dispatchLoop(); });
   completer_ = std::thread([this] { completionLoop(); });
+  register_live(this);
 }
 
-QueryServer::~QueryServer() {
+// Servers whose owner never released them (a leaked handle) still have
+// running threads when the process exits.  The handler registered with the
+// first server runs before the HIP runtime's static teardown (it is
+// registered later, after the runtime initialised) and stops them.
+namespace {
+std::mutex gLiveMu;
+std::set<QueryServer*>* gLive = nullptr;  // intentionally never destroyed
+
+void stop_live_servers() {
+  std::vector<QueryServer*> v;
+  {
+    std::lock_guard<std::mutex> g(gLiveMu);
+    if (gLive) v.assign(gLive->begin(), gLive->end());
+  }
+  for (QueryServer* q : v) q->stopThreads();
+}
+}  // namespace
+
+void QueryServer::register_live(QueryServer* q) {
+  std::lock_guard<std::mutex> g(gLiveMu);
+  if (!gLive) {
+    gLive = new std::set<QueryServer*>();
+    std::atexit(stop_live_servers);
+  }
+  gLive->insert(q);
+}
+
+void QueryServer::stopThreads() {
   {
     std::unique_lock<std::mutex> lk(mu_);
-    cvDone_.wait(lk, [&] { return pending_.empty() && inFlightBatches_ == 0; });
+    if (joined_) return;
     stop_ = true;
   }
   cvDispatch_.notify_all();
   cvDone_.notify_all();
   if (dispatcher_.joinable()) dispatcher_.join();
   if (completer_.joinable()) completer_.join();
+  std::lock_guard<std::mutex> lk(mu_);
+  joined_ = true;
+}
+
+QueryServer::~QueryServer() {
+  {
+    std::lock_guard<std::mutex> g(gLiveMu);
+    if (gLive) gLive->erase(this);
+  }
+  {
+    std::unique_lock<std::mutex> lk(mu_);
+    if (!joined_) cvDone_.wait(lk, [&] { return pending_.empty() && inFlightBatches_ == 0; });
+  }
+  stopThreads();
   for (auto& s : slots_) {
     (void)hipStreamSynchronize(s.stream);
     (void)hipFree(s.arena);

@@ -79,6 +79,7 @@ class QueryServer {
   void stats(double* out, size_t count);
 
  private:
+  static void register_live(QueryServer* q);
   struct Req {
     uint64_t id;
     QueryRequest q;
@@ -141,6 +142,14 @@ class QueryServer {
   double statDoneReq_ = 0, statDoneQ_ = 0, statBatches_ = 0, statPeakBytes_ = 0, statPeakPending_ = 0,
          statMaxBatch_ = 0;
   std::thread dispatcher_, completer_;
+  bool joined_ = false;  // threads stopped (by the destructor or at process exit)
+
+ public:
+  // Stops and joins the worker threads once the batches in flight are done
+  // (requests not yet dispatched are dropped).  Called for servers still
+  // alive at process exit, from a handler that runs before the HIP
+  // runtime's own teardown, so no thread calls into a finalised runtime.
+  void stopThreads();
 };
 
 }  // namespace pglgpu

@@ -1,6 +1,7 @@
 """Minimal ctypes access to the HIP runtime (device buffers, copies, events)
 for the benchmark and tests; the library itself manages its own memory."""
 import ctypes
+import weakref
 
 import numpy as np
 
@@ -47,6 +48,10 @@ class DeviceArray:
         self.ptr = ctypes.c_void_p()
         check(hip().hipMalloc(ctypes.byref(self.ptr), max(1, nbytes)), "hipMalloc")
         self.nbytes = nbytes
+        # freed when collected or by free(); not at interpreter exit, where a
+        # queued update may still read it (the process exit reclaims it)
+        self._fin = weakref.finalize(self, hip().hipFree, ctypes.c_void_p(self.ptr.value))
+        self._fin.atexit = False
 
     @classmethod
     def from_numpy(cls, a):
@@ -65,15 +70,8 @@ class DeviceArray:
         return self.ptr.value
 
     def free(self):
-        if self.ptr:
-            hip().hipFree(self.ptr)
-            self.ptr = ctypes.c_void_p()
-
-    def __del__(self):
-        try:
-            self.free()
-        except Exception:
-            pass
+        self._fin()
+        self.ptr = ctypes.c_void_p()
 
 
 def synchronize():
@@ -87,13 +85,10 @@ class _PinnedBlock:
         self.ptr = ctypes.c_void_p()
         check(hip().hipHostMalloc(ctypes.byref(self.ptr), max(1, nbytes), 0), "hipHostMalloc")
         self.nbytes = nbytes
-
-    def __del__(self):
-        try:
-            if self.ptr:
-                hip().hipHostFree(self.ptr)
-        except Exception:
-            pass
+        # as DeviceArray: not freed at interpreter exit, where a query server
+        # still draining its requests may write to it
+        self._fin = weakref.finalize(self, hip().hipHostFree, ctypes.c_void_p(self.ptr.value))
+        self._fin.atexit = False
 
 
 def pinned_empty(shape, dtype):
