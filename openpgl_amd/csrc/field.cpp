@@ -633,14 +633,23 @@ void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
   knnPivots_ = (const float4*)(tab + offKnn);
 
   // region-contiguous order: stable sort of (region, sample index)
+  const void* gatherRecs = deviceSamples;  // records the gather (and the deterministic sort) read
+  const uint32_t* gatherIdx = dVals2_.as<uint32_t>();
   if (R <= kSortMaxKeys) {
-    // stable counting sort keyed by the region of each sample's leaf: every
-    // region's first position is known from the kd counts
+    // stable counting sort keyed by the region of each sample's leaf (every
+    // region's first position is known from the kd counts), moving the
+    // records themselves into region order: the passes after it read each
+    // region from one contiguous range instead of gathering 40-byte records
+    // from all over the input
     dCountSortTmp_.ensure(counting_sort_temp_bytes(n, R));
-    counting_sort_leaves(dCountSortTmp_.p, dNodeOf_.as<uint32_t>(), (const KDNode*)tab,
-                         (const uint32_t*)(tab + offBegin), R, n, dKeys2_.as<uint32_t>(), dVals2_.as<uint32_t>(), sp,
-                         deterministic_ ? dWkey_.as<uint32_t>() : nullptr,
-                         deterministic_ ? dWkeySorted_.as<uint32_t>() : nullptr);
+    dRecs_.ensure(size_t(n) * kSampleRecordBytes);
+    counting_sort_leaf_records(dCountSortTmp_.p, dNodeOf_.as<uint32_t>(), (const KDNode*)tab,
+                               (const uint32_t*)(tab + offBegin), R, n, deviceSamples, dRecs_.p,
+                               dKeys2_.as<uint32_t>(), deterministic_ ? dVals2_.as<uint32_t>() : nullptr,
+                               deterministic_ ? dWkey_.as<uint32_t>() : nullptr,
+                               deterministic_ ? dWkeySorted_.as<uint32_t>() : nullptr, sp);
+    gatherRecs = dRecs_.p;
+    gatherIdx = deterministic_ ? dVals2_.as<uint32_t>() : nullptr;
   } else {
     launch_leaf_keys(dNodeOf_.as<uint32_t>(), (const KDNode*)tab, n, dKeys_.as<uint32_t>(), dVals_.as<uint32_t>(),
                      sp);
@@ -656,7 +665,7 @@ void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
     detPin_[P].ensure(det_sort_stage_bytes(n, R));
     // the weight keys travelled with the counting sort (above 4096 regions
     // the tiles read the weights from the records)
-    det_sort_segments(dDetTmp_.p, detPin_[P].p, detPin_[P].bytes, deviceSamples, hBegin_.data(), hCount_.data(), R,
+    det_sort_segments(dDetTmp_.p, detPin_[P].p, detPin_[P].bytes, gatherRecs, hBegin_.data(), hCount_.data(), R,
                       n, dVals2_.as<uint32_t>(), R <= kSortMaxKeys ? dWkeySorted_.as<uint32_t>() : nullptr, sp);
   }
   PGL_HIP_CHECK(hipEventRecord(evSp_[P][2], sp));
@@ -665,7 +674,7 @@ void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
   float* sv[6];
   for (int j = 0; j < 6; j++) sv[j] = dSoa_[P][j].as<float>();
   float* const* soa = sv;
-  launch_gather_reproject(deviceSamples, dKeys2_.as<uint32_t>(), dVals2_.as<uint32_t>(), n,
+  launch_gather_reproject(gatherRecs, dKeys2_.as<uint32_t>(), gatherIdx, n,
                           (const float4*)(tab + offPiv), soa[0], soa[1], soa[2], soa[3], soa[4], soa[5],
                           fc_.noParallax == 0, sp);
   PGL_HIP_CHECK(hipEventRecord(evSp_[P][3], sp));

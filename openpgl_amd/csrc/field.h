@@ -195,6 +195,7 @@ class KindField {
   DevBuf dPx_, dPy_, dPz_, dNodeOf_, dKeys_, dVals_, dKeys2_, dVals2_, dSortTmp_;
   DevBuf dSoa_[2][6];     // region-ordered dx, dy, dz, w, pdf, dist of two consecutive updates
   DevBuf dAos_;           // staging of host records
+  DevBuf dRecs_;          // the update's records in region order (counting-sort path)
   DevBuf dTables_[2];          // per parity: leaf-key nodes | region starts | pivots | KNN pivots
   const float4* knnPivots_ = nullptr;  // KNN pivots of the pending update (in dTables_)
   hipStream_t sp_ = nullptr;  // spatial phase

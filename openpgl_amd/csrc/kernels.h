@@ -65,6 +65,13 @@ void counting_sort_pairs(void* temp, const uint32_t* keys, const uint32_t* keyBe
 void counting_sort_leaves(void* temp, const uint32_t* nodeOf, const KDNode* keyNodes, const uint32_t* keyBegin,
                           uint32_t numKeys, uint32_t n, uint32_t* keysOut, uint32_t* valsOut, hipStream_t s,
                           const uint32_t* carryIn = nullptr, uint32_t* carryOut = nullptr);
+// the same moving the 40-byte records into region order: recsOut[j] =
+// recsIn[i] for the sample i sorted to position j, keysOut[j] its region,
+// posOut[j] = j (optional), carryOut[j] = carryIn[i] (optional).  Later
+// passes then read each region's records from one contiguous range.
+void counting_sort_leaf_records(void* temp, const uint32_t* nodeOf, const KDNode* keyNodes, const uint32_t* keyBegin,
+                                uint32_t numKeys, uint32_t n, const void* recsIn, void* recsOut, uint32_t* keysOut,
+                                uint32_t* posOut, const uint32_t* carryIn, uint32_t* carryOut, hipStream_t s);
 // Deterministic-mode order (kernels_sort.hip): every region's range of a
 // region-contiguous index array put in SampleDataLess order
 // (data/SampleData.h:82-93), as Field::fitRegions/updateRegions sort them
@@ -85,6 +92,8 @@ void det_sort_segments(void* temp, void* pinnedHost, size_t pinnedBytes, const v
 // keys/values of region-contiguous records (begin: numRegions ascending offsets)
 void launch_region_keys(const uint32_t* begin, uint32_t numRegions, uint32_t n, uint32_t* keys, uint32_t* vals,
                         hipStream_t s);
+// SoA output j = record sortedIdx[j] (sortedIdx null: record j), reprojected
+// to the pivot of region sortedKeys[j]
 void launch_gather_reproject(const void* aos, const uint32_t* sortedKeys, const uint32_t* sortedIdx, uint32_t n,
                              const float4* regionPivot, float* dx, float* dy, float* dz, float* w, float* pdf,
                              float* dist, bool reproject, hipStream_t s);

@@ -83,14 +83,41 @@ struct MaxOp {
 // element (key k, index i) at sorted position dst: the (key, index) pair
 // and, with carryIn, carryIn[i]
 struct PairOut {
+  static constexpr bool kRecords = false;
   uint32_t* __restrict__ keysOut;
   uint32_t* __restrict__ valsOut;
   const uint32_t* __restrict__ carryIn;
   uint32_t* __restrict__ carryOut;
-  __device__ __forceinline__ void operator()(uint32_t dst, uint32_t k, uint32_t i) const {
+  const uint2* recsIn = nullptr;
+  __device__ __forceinline__ void operator()(uint32_t dst, uint32_t k, uint32_t i, const uint2*) const {
     keysOut[dst] = k;
     valsOut[dst] = i;
     if (carryIn) carryOut[dst] = carryIn[i];
+  }
+};
+
+// the same with the record itself moved to its sorted position: recsOut[dst]
+// = recsIn[i] (the 40-byte record as five 8-byte words, read from the tile's
+// copy in LDS), posOut[dst] = dst (optional: the identity order the
+// deterministic sort starts from)
+struct RecordOut {
+  static constexpr bool kRecords = true;
+  uint32_t* __restrict__ keysOut;
+  uint32_t* __restrict__ posOut;
+  const uint32_t* __restrict__ carryIn;
+  uint32_t* __restrict__ carryOut;
+  const uint2* __restrict__ recsIn;
+  uint2* __restrict__ recsOut;
+  __device__ __forceinline__ void operator()(uint32_t dst, uint32_t k, uint32_t i, const uint2* rec) const {
+    keysOut[dst] = k;
+    if (posOut) posOut[dst] = dst;
+    if (carryIn) carryOut[dst] = carryIn[i];
+    uint2 w[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) w[q] = rec[q];
+    uint2* d = recsOut + (size_t)dst * 5;
+#pragma unroll
+    for (int q = 0; q < 5; q++) d[q] = w[q];
   }
 };
 
@@ -106,6 +133,8 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_scatter(SortKeys keys, ui
   } tmp;
   __shared__ uint32_t running[kSortMaxKeys];
   __shared__ uint32_t firstKey[kSortThreads], firstVal[kSortThreads], lastKey[kSortThreads];
+  // the tile's records (RecordOut), loaded with coalesced reads
+  __shared__ uint2 recStage[Out::kRecords ? kSortTile * 5 : 1];
   const uint32_t pad = 0xffffffffu;
   for (uint32_t k = threadIdx.x; k < numKeys; k += kSortThreads)
     running[k] = offsets[(size_t)blockIdx.x * numKeys + k];
@@ -115,6 +144,23 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_scatter(SortKeys keys, ui
   for (uint32_t tile = 0; tile < tilesPerBlock; tile++) {
     const uint64_t base = first + (uint64_t)tile * kSortTile;
     if (base >= n) break;  // uniform over the block
+    if constexpr (Out::kRecords) {
+      const uint32_t words = (uint32_t)min<uint64_t>((uint64_t)kSortTile, (uint64_t)n - base) * 5u;
+      const uint2* src = out.recsIn + base * 5;
+      constexpr int kIt = (int)(kSortTile * 5 / kSortThreads);
+      uint2 tmpw[kIt];
+#pragma unroll
+      for (int q = 0; q < kIt; q++) {
+        const uint32_t w = t + (uint32_t)q * kSortThreads;
+        if (w < words) tmpw[q] = src[w];
+      }
+#pragma unroll
+      for (int q = 0; q < kIt; q++) {
+        const uint32_t w = t + (uint32_t)q * kSortThreads;
+        if (w < words) recStage[w] = tmpw[q];
+      }
+      // visible to the scatter below through the barriers of the sort
+    }
     uint32_t k[kSortItems], v[kSortItems];
 #pragma unroll
     for (int j = 0; j < kSortItems; j++) {
@@ -147,7 +193,7 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_scatter(SortKeys keys, ui
       if (v[j] == pad) continue;
       const int pos = (int)(t * kSortItems + j);
       const uint32_t dst = running[k[j]] + (uint32_t)(pos - runStart[j]);
-      if (dst < n) out(dst, k[j], v[j]);
+      if (dst < n) out(dst, k[j], v[j], recStage + (Out::kRecords ? (size_t)(v[j] - base) * 5 : 0));
     }
     __syncthreads();
     // the last real element of each run advances its key's position
@@ -208,6 +254,13 @@ void counting_sort_leaves(void* temp, const uint32_t* nodeOf, const KDNode* keyN
                           const uint32_t* carryIn, uint32_t* carryOut) {
   counting_sort(temp, SortKeys{nullptr, nodeOf, keyNodes}, keyBegin, numKeys, n,
                 PairOut{keysOut, valsOut, carryIn, carryOut}, s);
+}
+
+void counting_sort_leaf_records(void* temp, const uint32_t* nodeOf, const KDNode* keyNodes, const uint32_t* keyBegin,
+                                uint32_t numKeys, uint32_t n, const void* recsIn, void* recsOut, uint32_t* keysOut,
+                                uint32_t* posOut, const uint32_t* carryIn, uint32_t* carryOut, hipStream_t s) {
+  counting_sort(temp, SortKeys{nullptr, nodeOf, keyNodes}, keyBegin, numKeys, n,
+                RecordOut{keysOut, posOut, carryIn, carryOut, (const uint2*)recsIn, (uint2*)recsOut}, s);
 }
 
 // ------------------------------------------------------------ deterministic order

@@ -319,18 +319,28 @@ __global__ void __launch_bounds__(256) k_region_ranges(const uint32_t* __restric
 }
 
 // Gather into region-contiguous SoA, each sample reprojected to its region's
-// pivot (gather_sample).
+// pivot (gather_sample).  Workgroup b runs on XCD b % 8 (round-robin
+// dispatch); it takes range (b % 8) * (G / 8) + b / 8 of G contiguous ranges
+// of `chunk` positions, so each XCD walks one contiguous eighth of the output
+// and the records of a region (a permutation of one contiguous range when
+// the records are in region order) are read into one XCD's L2 only.
 __global__ void __launch_bounds__(256) k_gather_reproject(const SampleAoS* __restrict__ in,
                                                           const uint32_t* __restrict__ sortedKeys,
                                                           const uint32_t* __restrict__ sortedIdx, uint32_t n,
+                                                          uint32_t chunk,
                                                           const float4* __restrict__ regionPivot /* xyz, minDist */,
                                                           float* __restrict__ odx, float* __restrict__ ody,
                                                           float* __restrict__ odz, float* __restrict__ ow,
                                                           float* __restrict__ opdf, float* __restrict__ odist,
                                                           int reproject) {
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
-    gather_sample(in[sortedIdx[j]], regionPivot[sortedKeys[j]], reproject, j, odx, ody, odz, ow, opdf, odist);
+  const uint32_t G = gridDim.x;  // a multiple of 8
+  const uint32_t range = (blockIdx.x & 7u) * (G >> 3) + (blockIdx.x >> 3);
+  const uint64_t begin = (uint64_t)range * chunk;
+  const uint64_t end = min<uint64_t>((uint64_t)n, begin + chunk);
+  for (uint64_t jj = begin + threadIdx.x; jj < end; jj += 256) {
+    const uint32_t j = (uint32_t)jj;
+    gather_sample(in[sortedIdx ? sortedIdx[j] : j], regionPivot[sortedKeys[j]], reproject, j, odx, ody, odz, ow,
+                  opdf, odist);
   }
 }
 
@@ -491,8 +501,16 @@ void launch_region_keys(const uint32_t* begin, uint32_t numRegions, uint32_t n, 
 void launch_gather_reproject(const void* aos, const uint32_t* sortedKeys, const uint32_t* sortedIdx, uint32_t n,
                              const float4* regionPivot, float* dx, float* dy, float* dz, float* w, float* pdf,
                              float* dist, bool reproject, hipStream_t s) {
-  hipLaunchKernelGGL(k_gather_reproject, dim3(grid_for(n, 256)), dim3(256), 0, s, (const SampleAoS*)aos, sortedKeys,
-                     sortedIdx, n, regionPivot, dx, dy, dz, w, pdf, dist, reproject ? 1 : 0);
+  if (n == 0) return;
+  // G ranges of `chunk` positions (a multiple of 256), G a multiple of 8
+  const uint32_t tiles = (n + 255) / 256;
+  const uint32_t G0 = std::min<uint32_t>(tiles, 4096);
+  const uint32_t tilesPerRange = (tiles + G0 - 1) / G0;
+  const uint32_t chunk = tilesPerRange * 256;
+  const uint32_t ranges = (n + chunk - 1) / chunk;
+  const uint32_t G = (ranges + 7) / 8 * 8;
+  hipLaunchKernelGGL(k_gather_reproject, dim3(G), dim3(256), 0, s, (const SampleAoS*)aos, sortedKeys, sortedIdx, n,
+                     chunk, regionPivot, dx, dy, dz, w, pdf, dist, reproject ? 1 : 0);
 }
 
 void launch_knn8(const float4* pivots, uint32_t numRegions, uint32_t* nbIds, float4* nbPts, hipStream_t s) {
