@@ -133,6 +133,7 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_scatter(SortKeys keys, ui
   } tmp;
   __shared__ uint32_t running[kSortMaxKeys];
   __shared__ uint32_t firstKey[kSortThreads], firstVal[kSortThreads], lastKey[kSortThreads];
+  __shared__ uint32_t sDst[kSortTile], sKey[kSortTile], sVal[kSortTile];
   // the tile's records (RecordOut), loaded with coalesced reads
   __shared__ uint2 recStage[Out::kRecords ? kSortTile * 5 : 1];
   const uint32_t pad = 0xffffffffu;
@@ -187,15 +188,27 @@ __global__ void __launch_bounds__(kSortThreads) k_sort_scatter(SortKeys keys, ui
     }
     int runStart[kSortItems];
     Scan(tmp.scan).InclusiveScan(head, runStart, MaxOp());
-    // scatter to the key's running position + rank in its run
+    // destination of every sorted position: the key's running position + rank
+    // in its run (pad for none), written out below in striped order so that
+    // consecutive lanes write consecutive destinations of a run
 #pragma unroll
     for (int j = 0; j < kSortItems; j++) {
-      if (v[j] == pad) continue;
       const int pos = (int)(t * kSortItems + j);
       const uint32_t dst = running[k[j]] + (uint32_t)(pos - runStart[j]);
-      if (dst < n) out(dst, k[j], v[j], recStage + (Out::kRecords ? (size_t)(v[j] - base) * 5 : 0));
+      sDst[pos] = (v[j] == pad || dst >= n) ? pad : dst;
+      sKey[pos] = k[j];
+      sVal[pos] = v[j];
     }
     __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kSortItems; j++) {
+      const int pos = (int)(t + (uint32_t)j * kSortThreads);
+      const uint32_t dst = sDst[pos];
+      if (dst != pad) {
+        const uint32_t vv = sVal[pos];
+        out(dst, sKey[pos], vv, recStage + (Out::kRecords ? (size_t)(vv - base) * 5 : 0));
+      }
+    }
     // the last real element of each run advances its key's position
 #pragma unroll
     for (int j = 0; j < kSortItems; j++) {
