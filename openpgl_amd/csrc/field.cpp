@@ -538,10 +538,45 @@ void KindField::renumberDepthFirst(uint32_t nodesBefore, uint32_t regionsBefore)
 // call returns once the directional phase is queued; the previous update's
 // directional phase is finished (its statistics collected) just before this
 // one is queued, and every read of device state finishes the pending update.
+// Buffers whose size follows the number of regions and nodes are sized for
+// at least kReserveRegions regions (and twice as many nodes) at once, and for
+// twice the current count beyond that.  A buffer that grows is freed, and
+// freeing device or pinned memory waits for the whole device: in the middle
+// of an update that would stall the overlap of this update's spatial phase
+// with the previous update's region kernel.
+static const uint32_t kReserveRegions = 4096;
+
+void KindField::reserve(uint32_t n, uint32_t regions) {
+  const uint32_t Rc = std::max<uint32_t>(kReserveRegions, 2 * regions);
+  const size_t nodes = 2 * size_t(Rc);
+  const size_t acc = 2 * size_t(Rc) * sizeof(IntStatsAcc);
+  dAcc_.ensure(acc);
+  kdDown_.ensure(acc);
+  const size_t up = nodes * sizeof(KDNode) + nodes * sizeof(int32_t) + 6 * size_t(Rc) * 4 + 4 * 16 +
+                    2 * size_t(Rc) * sizeof(FrontierNode);
+  kdUp_.ensure(up);
+  dKdUp_.ensure(up);
+  const size_t tab = nodes * sizeof(KDNode) + size_t(Rc) * 4 + 2 * size_t(Rc) * sizeof(float4) + 64;
+  for (int p = 0; p < 2; p++) {
+    pin_[p][0].ensure(tab);
+    dTables_[p].ensure(tab);
+  }
+  const uint32_t Rs = std::min<uint32_t>(Rc, kSortMaxKeys);
+  dCountSortTmp_.ensure(counting_sort_temp_bytes(n, Rs));
+  if (deterministic_) {
+    dDetTmp_.ensure(det_sort_temp_bytes(n, Rc));
+    for (int p = 0; p < 2; p++) detPin_[p].ensure(det_sort_stage_bytes(n, Rc));
+  }
+  dNbIds_.ensure(size_t(Rc) * 8 * 4);
+  dNbPts_.ensure(size_t(Rc) * 8 * sizeof(float4));
+  dNodes_.ensure(nodes * sizeof(KDNode));
+}
+
 void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
   if (n == 0) return;
   if (failed_) finish();  // throws: a failed field takes no update until reset()
   const auto t0 = std::chrono::steady_clock::now();
+  reserve(n, (uint32_t)regions_.size());
   const int P = parity_;
   UpdateInfo inf;
   inf.samples = n;

@@ -175,6 +175,7 @@ class KindField {
   void ensureRegionCapacity(uint32_t r, hipStream_t s);
   void renumberDepthFirst(uint32_t nodesBefore, uint32_t regionsBefore);
   void updateFromHost(const void* hostSamples, uint32_t n);
+  void reserve(uint32_t n, uint32_t regions);
 
   KdSettings kd_;
   FactoryConfig fc_;
