@@ -28,11 +28,18 @@
 //
 // How each part is pinned (tests/):
 //   * IntegerSampleStatistics / rcp: golden vectors written by the reference's own
-//     SampleStatistics.h compiled in oracle/_ref (tests/test_oracle.py), plus closed forms.
+//     SampleStatistics.h compiled in oracle/_ref (tests/test_oracle.py), plus closed forms;
+//     the rcp of each device width against the reference built for that ISA
+//     (oracle/_ref/libpglref_w4.so, libpglref_w16.so; tests/test_oracle_spatial.py).
 //   * kd build: unbuildable reference (needs oneTBB headers); pinned by invariants
 //     (partition/lookup agreement, leaf capacity, split = mean of argmax variance of
-//     the merged statistics, brute-force recomputation of every leaf's statistics).
-//   * KNN: brute-force 8-NN recomputation and the draw() rule on hand-made cases.
+//     the merged statistics) and, over several frames, by an independent
+//     re-derivation of each Update from the previous tree (parent bounds, strict
+//     threshold, depth limit, history merge) that must match bit for bit and that
+//     each mutated rule breaks (tests/test_oracle_spatial.py).
+//   * KNN: brute-force 8-NN recomputation; the pick (draw, fused or unfused squared
+//     distances, slot ids in the low mantissa bits, rescaled random number) on
+//     hand-made neighbourhoods with known distances (tests/test_oracle_spatial.py).
 //   * directional fit/update, products, sample, pdf: the reference's own directional
 //     headers compiled in oracle/_ref (golden KATs in tests/golden), K=1 closed-form
 //     MLE, FP64 twin agreement, integral of pdf = 1.
