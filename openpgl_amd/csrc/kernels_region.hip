@@ -105,6 +105,9 @@ struct Shared {
   float pre[10];           // merging: weight, direction and mean cosine of the chosen pair before the merge
   float scal[RWAVES][8];
   float stage[5][kChunk];  // sample pass: the current chunk's dx, dy, dz, w and pdf or distance
+  uint32_t stageKey;       // first sample (absolute index) of the chunk held in stage[0..3], ~0: none
+  uint32_t stageCnt;       // its samples
+  const float* stageExtra; // the array whose values stage[4] holds for that chunk (null: none)
   double ll;
   double prevLL, invPrevLL;
   float un[4];
@@ -376,10 +379,21 @@ __device__ void pass_finish(Shared& sh, Team& tm, int llIdx, float fs, float fc,
     team_barrier(sh, tm);
     const unsigned long long tF = wall_clock64();
     const float* p = tm.part + (size_t)buf * tm.nc * kPartStride + v;
-    for (uint32_t c = 0; c < tm.nc; c++) {
-      const float x = __hip_atomic_load(p + (size_t)c * kPartStride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (v == llIdx) fd += (double)x;
-      else kfold(fs, fc, x);
+    // eight loads in flight at a time, folded in chunk order
+    for (uint32_t c0 = 0; c0 < tm.nc; c0 += 8) {
+      float x[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+        x[u] = c0 + u < tm.nc ? __hip_atomic_load(p + (size_t)(c0 + u) * kPartStride, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT)
+                              : 0.f;
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        if (c0 + u < tm.nc) {
+          if (v == llIdx) fd += (double)x[u];
+          else kfold(fs, fc, x[u]);
+        }
+      }
     }
     if (v == 0) sh.S.prof[10] += (float)(wall_clock64() - tF);
   }
@@ -391,32 +405,74 @@ __device__ void pass_finish(Shared& sh, Team& tm, int llIdx, float fs, float fc,
 // Copies samples [first, first + cnt) of the region's stream into the
 // staging arrays (dx, dy, dz, w, and pdf or distance when extra is set):
 // every thread issues all of its coalesced loads at once, so a chunk costs
-// one memory latency instead of one per sample step.  Callers synchronise
-// before reading the stage.
+// one memory latency instead of one per sample step.  A chunk already
+// staged (every pass reads the same samples, and a rank takes the same
+// chunks in every pass) is not loaded again; the extra array is loaded when
+// the stage holds another one.  Callers synchronise before reading the
+// stage and then record what it holds with stage_mark.
 __device__ __forceinline__ void stage_chunk(Shared& sh, const RegionSamples& rs, uint32_t first, uint32_t cnt,
                                             const float* __restrict__ extra) {
   const int tid = threadIdx.x;
+  const bool have = sh.stageKey == first && sh.stageCnt == cnt;
+  const bool loadMain = !have;
+  const bool loadExtra = extra != nullptr && !(have && sh.stageExtra == extra);
+  if (!loadMain && !loadExtra) return;
   float v[kChunk / RT][5];
 #pragma unroll
   for (int m = 0; m < (int)(kChunk / RT); m++) {
     const uint32_t o = tid + m * RT;
     if (o < cnt) {
       const uint32_t i = first + o;
-      v[m][0] = rs.dx[i];
-      v[m][1] = rs.dy[i];
-      v[m][2] = rs.dz[i];
-      v[m][3] = rs.w[i];
-      v[m][4] = extra ? extra[i] : 0.f;
+      if (loadMain) {
+        v[m][0] = rs.dx[i];
+        v[m][1] = rs.dy[i];
+        v[m][2] = rs.dz[i];
+        v[m][3] = rs.w[i];
+      }
+      if (loadExtra) v[m][4] = extra[i];
     }
   }
 #pragma unroll
   for (int m = 0; m < (int)(kChunk / RT); m++) {
     const uint32_t o = tid + m * RT;
     if (o < cnt) {
+      if (loadMain) {
 #pragma unroll
-      for (int a = 0; a < 5; a++) sh.stage[a][o] = v[m][a];
+        for (int a = 0; a < 4; a++) sh.stage[a][o] = v[m][a];
+      }
+      if (loadExtra) sh.stage[4][o] = v[m][4];
     }
   }
+}
+
+// What the stage holds after stage_chunk (thread 0, after the barrier that
+// follows stage_chunk; every thread read the previous values before it).
+__device__ __forceinline__ void stage_mark(Shared& sh, uint32_t first, uint32_t cnt, const float* extra) {
+  if (threadIdx.x != 0) return;
+  const bool have = sh.stageKey == first && sh.stageCnt == cnt;
+  if (!have) sh.stageExtra = extra;
+  else if (extra) sh.stageExtra = extra;
+  sh.stageKey = first;
+  sh.stageCnt = cnt;
+}
+
+// This rank's chunks of a pass: rank, rank + T, ...; with a team of more
+// than one rank they are taken in reverse order when the last of them is
+// still staged from the previous pass (partials are folded in chunk order
+// from the team buffer, so the order does not change any result; a lone
+// rank folds as it goes and keeps the order).
+struct RankChunks {
+  uint32_t count;
+  bool reverse;
+  __device__ __forceinline__ uint32_t chunk(const Team& tm, uint32_t i) const {
+    return tm.rank + (reverse ? count - 1 - i : i) * tm.T;
+  }
+};
+__device__ __forceinline__ RankChunks rank_chunks(const Shared& sh, const Team& tm, uint32_t base) {
+  RankChunks r;
+  r.count = tm.rank < tm.nc ? (tm.nc - tm.rank + tm.T - 1) / tm.T : 0u;
+  r.reverse = tm.T > 1 && r.count > 1 && sh.stageKey == base + (tm.rank + (r.count - 1) * tm.T) * kChunk;
+  return r;
 }
 
 // Per-lane E-step sums of a chunk.
@@ -481,10 +537,13 @@ __device__ void pass_estep(Shared& sh, Team& tm, const RegionSamples& rs, uint32
   load_comps(V, q, c);
   float fs = 0.f, fc = 0.f;
   double fd = 0.0;
-  for (uint32_t ch = tm.rank; ch < tm.nc; ch += tm.T) {
+  const RankChunks rc = rank_chunks(sh, tm, base);
+  for (uint32_t ci = 0; ci < rc.count; ci++) {
+    const uint32_t ch = rc.chunk(tm, ci);
     const uint32_t cb = ch * kChunk, ce = min(n, cb + kChunk);
     stage_chunk(sh, rs, base + cb, ce - cb, nullptr);
     __syncthreads();
+    stage_mark(sh, base + cb, ce - cb, nullptr);
     float bx[8], by[8], bz[8], bw[8];
 #pragma unroll
     for (int j = 0; j < 8; j++) bx[j] = by[j] = bz[j] = bw[j] = 0.f;
@@ -545,10 +604,13 @@ __device__ void pass_distances(Shared& sh, Team& tm, const RegionSamples& rs, ui
   }
   float fs = 0.f, fc = 0.f;
   double fd = 0.0;
-  for (uint32_t ch = tm.rank; ch < tm.nc; ch += tm.T) {
+  const RankChunks rc = rank_chunks(sh, tm, base);
+  for (uint32_t ci = 0; ci < rc.count; ci++) {
+    const uint32_t ch = rc.chunk(tm, ci);
     const uint32_t cb = ch * kChunk, ce = min(n, cb + kChunk);
     stage_chunk(sh, rs, base + cb, ce - cb, rs.dist);
     __syncthreads();
+    stage_mark(sh, base + cb, ce - cb, rs.dist);
     float ad[8], aw[8];
 #pragma unroll
     for (int j = 0; j < 8; j++) ad[j] = aw[j] = 0.f;
@@ -603,10 +665,13 @@ __device__ void pass_split(Shared& sh, Team& tm, const RegionSamples& rs, uint32
   const float inv_mc = 1.f / mcEst;
   float fs = 0.f, fc = 0.f;
   double fd = 0.0;
-  for (uint32_t ch = tm.rank; ch < tm.nc; ch += tm.T) {
+  const RankChunks rc = rank_chunks(sh, tm, base);
+  for (uint32_t ci = 0; ci < rc.count; ci++) {
+    const uint32_t ch = rc.chunk(tm, ci);
     const uint32_t cb = ch * kChunk, ce = min(n, cb + kChunk);
     stage_chunk(sh, rs, base + cb, ce - cb, rs.pdf);
     __syncthreads();
+    stage_mark(sh, base + cb, ce - cb, rs.pdf);
     float achi[8], asa[8], asw[8], acx[8], acy[8], acz[8];
 #pragma unroll
     for (int j = 0; j < 8; j++) achi[j] = asa[j] = asw[j] = acx[j] = acy[j] = acz[j] = 0.f;
@@ -1672,6 +1737,9 @@ PGL_REGION_KERNEL(RegionState* __restrict__ states,
     sh.abort = 0;
     sh.cfg = cfg;
     sh.rs = rs;
+    sh.stageKey = 0xffffffffu;
+    sh.stageCnt = 0;
+    sh.stageExtra = nullptr;
   }
   for (;;) {
     if (tid == 0) {
