@@ -188,6 +188,7 @@ KindField::KindField(const KdSettings& kd, const FactoryConfig& fc, bool determi
   for (int p = 0; p < 2; p++) {
     for (auto& e : evSp_[p]) PGL_HIP_CHECK(hipEventCreate(&e));
     for (auto& e : evDir_[p]) PGL_HIP_CHECK(hipEventCreate(&e));
+    for (auto& e : evKnn_[p]) PGL_HIP_CHECK(hipEventCreate(&e));
   }
   PGL_HIP_CHECK(hipStreamCreate(&sp_));
   std::vector<float> ud;
@@ -202,6 +203,7 @@ KindField::~KindField() {
   for (int p = 0; p < 2; p++) {
     for (auto& e : evSp_[p]) (void)hipEventDestroy(e);
     for (auto& e : evDir_[p]) (void)hipEventDestroy(e);
+    for (auto& e : evKnn_[p]) (void)hipEventDestroy(e);
   }
   (void)hipStreamDestroy(sp_);
 }
@@ -714,6 +716,27 @@ void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
                           fc_.noParallax == 0, sp);
   PGL_HIP_CHECK(hipEventRecord(evSp_[P][3], sp));
   lastGatherParity_ = P;
+  // KNN neighbourhoods of the region pivots (sample-statistics means, uploaded
+  // with the spatial tables) and the tree in its final numbering for the query
+  // kernels: they need only this update's spatial results, so they run on the
+  // spatial stream, overlapping the previous update's region kernel; every
+  // reader of either first finishes the pending update, which waits for them
+  PGL_HIP_CHECK(hipEventRecord(evKnn_[P][0], sp));
+  if (kd_.knnLookup) {
+    dNbIds_.ensure(size_t(R) * 8 * 4);
+    dNbPts_.ensure(size_t(R) * 8 * sizeof(float4));
+    launch_knn8(knnPivots_, R, dNbIds_.as<uint32_t>(), dNbPts_.as<float4>(), sp);
+  }
+  dNodes_.ensure(nodes_.size() * sizeof(KDNode));
+  PGL_HIP_CHECK(hipMemcpyAsync(dNodes_.p, pinned(pin_[P][5], nodes_.data(), nodes_.size()),
+                               nodes_.size() * sizeof(KDNode), hipMemcpyHostToDevice, sp));
+  PGL_HIP_CHECK(hipEventRecord(evKnn_[P][1], sp));
+
+  // the directional launch's host work (jobs, teams, pinned tables) while the
+  // previous update's region kernel still runs
+  RegionRunner& rr = runner();
+  rr.stage(hWork_, RegionSamplesView{soa[0], soa[1], soa[2], soa[3], soa[4], soa[5]}, fc_,
+           dUniformDirs_.as<float>(), nullptr, hInputs_.data(), R);
 
   // ------------------------------------------------ finish the previous update
   finish();
@@ -726,32 +749,16 @@ void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
   }
   ensureRegionCapacity(R, s);
   PGL_HIP_CHECK(hipEventRecord(evDir_[P][0], s));
-  // KNN neighbourhoods of the region pivots (sample-statistics means, uploaded
-  // with the spatial tables)
-  if (kd_.knnLookup) {
-    dNbIds_.ensure(size_t(R) * 8 * 4);
-    dNbPts_.ensure(size_t(R) * 8 * sizeof(float4));
-  }
   PGL_HIP_CHECK(hipStreamWaitEvent(s, evSp_[P][3], 0));
   // the new regions' state copies, the inputs and the launch (one upload, one preparing kernel)
-  RegionRunner& rr = runner();
-  rr.launch(dStates_.as<RegionState>(), hWork_,
-            RegionSamplesView{soa[0], soa[1], soa[2], soa[3], soa[4], soa[5]}, fc_, dUniformDirs_.as<float>(),
-            nullptr, s, hInputs_.data(), R);
+  rr.submit(dStates_.as<RegionState>(), s);
   inf.regionLaunches = rr.lastLaunches;
   inf.regionJobs = rr.lastJobs;
   inf.regionItems = rr.lastItems;
   inf.maxTeam = rr.lastMaxTeam;
   inf.regionGrid = rr.lastGrid;
   PGL_HIP_CHECK(hipEventRecord(evDir_[P][1], s));
-
-  if (kd_.knnLookup) launch_knn8(knnPivots_, R, dNbIds_.as<uint32_t>(), dNbPts_.as<float4>(), s);
-  // the query kernels walk the tree in the final numbering: uploaded behind
-  // the region kernel, off the spatial phase's path; every reader of the
-  // tree first finishes the pending update, which waits for this copy
-  dNodes_.ensure(nodes_.size() * sizeof(KDNode));
-  PGL_HIP_CHECK(hipMemcpyAsync(dNodes_.p, pinned(pin_[P][5], nodes_.data(), nodes_.size()),
-                               nodes_.size() * sizeof(KDNode), hipMemcpyHostToDevice, s));
+  PGL_HIP_CHECK(hipStreamWaitEvent(s, evKnn_[P][1], 0));
   PGL_HIP_CHECK(hipEventRecord(evDir_[P][2], s));
 
   for (auto& r : regions_) { r.splitFlag = false; r.copyFrom = -1; }
@@ -802,7 +809,7 @@ void KindField::finishPending() const {
   PGL_HIP_CHECK(hipEventElapsedTime(&ms, evSp_[P][1], evSp_[P][2])); i.sortMs = ms;
   PGL_HIP_CHECK(hipEventElapsedTime(&ms, evSp_[P][2], evSp_[P][3])); i.gatherMs = ms;
   PGL_HIP_CHECK(hipEventElapsedTime(&ms, evDir_[P][0], evDir_[P][1])); i.directionalMs = ms;
-  PGL_HIP_CHECK(hipEventElapsedTime(&ms, evDir_[P][1], evDir_[P][2])); i.knnMs = ms;
+  PGL_HIP_CHECK(hipEventElapsedTime(&ms, evKnn_[P][0], evKnn_[P][1])); i.knnMs = ms;
   i.regionKernelMs = rr.kernelMs();
   i.regions = pendingRegions_;
   i.wallMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - pendingT0_).count();

@@ -203,6 +203,7 @@ class KindField {
   int parity_ = 0;
   int lastGatherParity_ = -1;
   hipEvent_t evSp_[2][4], evDir_[2][3];
+  hipEvent_t evKnn_[2][2];  // around the KNN build and the tree upload (spatial stream)
   // pinned staging of the per-update uploads, by frame parity: [0] inputs,
   // [1] pivots, [2] copy pairs, [3] KNN pivots, [4] key nodes, [5] nodes,
   // [6] region begins

@@ -48,14 +48,17 @@ RegionRunner::RegionRunner() {
   resident_ = (uint32_t)(perCU * cus);
   defaultGrid_ = (uint32_t)(std::min(perCU, 2) * cus);
   cooperative_ = coop != 0;
-  PGL_SCHED_CHECK(hipMalloc((void**)&dErr_, sizeof(int)));
-  PGL_SCHED_CHECK(hipHostMalloc((void**)&hErr_, sizeof(int), hipHostMallocDefault));
+  PGL_SCHED_CHECK(hipMalloc((void**)&dTot_, sizeof(LaunchTotals)));
+  PGL_SCHED_CHECK(hipHostMalloc((void**)&hTot_, sizeof(LaunchTotals), hipHostMallocDefault));
+  dErr_ = &dTot_->err;
+  hErr_ = &hTot_->err;
+  dStats_ = &dTot_->stats;
+  hStats_ = &hTot_->stats;
   *hErr_ = 0;
-  PGL_SCHED_CHECK(hipMalloc((void**)&dStats_, sizeof(RegionLaunchStats)));
-  PGL_SCHED_CHECK(hipHostMalloc((void**)&hStats_, sizeof(RegionLaunchStats), hipHostMallocDefault));
   *hStats_ = RegionLaunchStats{0, 0, 0, 0};
   PGL_SCHED_CHECK(hipEventCreate(&evBegin_));
   PGL_SCHED_CHECK(hipEventCreate(&evEnd_));
+  PGL_SCHED_CHECK(hipEventCreateWithFlags(&evPrep_, hipEventDisableTiming));
 }
 
 RegionRunner::~RegionRunner() {
@@ -63,29 +66,41 @@ RegionRunner::~RegionRunner() {
   (void)hipHostFree(hStage_);
   (void)hipFree(dQueue_);
   (void)hipFree(dPartial_);
-  (void)hipFree(dErr_);
-  (void)hipHostFree(hErr_);
-  (void)hipFree(dStats_);
-  (void)hipHostFree(hStats_);
+  (void)hipFree(dTot_);
+  (void)hipHostFree(hTot_);
   if (evBegin_) (void)hipEventDestroy(evBegin_);
   if (evEnd_) (void)hipEventDestroy(evEnd_);
+  if (evPrep_) (void)hipEventDestroy(evPrep_);
 }
 
 void RegionRunner::launch(RegionState* states, std::vector<RegionWork> work, const RegionSamplesView& rs,
                           const FactoryConfig& cfg, const float* uniformDirs, float* costHint, hipStream_t s,
                           const RegionInput* inputs, uint32_t numInputs) {
-  lastLaunches = lastJobs = lastItems = lastMaxTeam = lastGrid = 0;
-  *hErr_ = 0;
-  *hStats_ = RegionLaunchStats{0, 0, 0, 0};
-  timed_ = false;
+  stage(std::move(work), rs, cfg, uniformDirs, costHint, inputs, numInputs);
+  submit(states, s);
+}
+
+void RegionRunner::stage(std::vector<RegionWork> work, const RegionSamplesView& rs, const FactoryConfig& cfg,
+                         const float* uniformDirs, float* costHint, const RegionInput* inputs, uint32_t numInputs) {
+  // the pinned staging is read by the previous launch's preparing kernel
+  if (prepQueued_) PGL_SCHED_CHECK(hipEventSynchronize(evPrep_));
+  prepQueued_ = false;
+  Staged& st = staged_;
+  st = Staged();
+  st.valid = true;
+  st.rs = rs;
+  st.cfg = cfg;
+  st.uniformDirs = uniformDirs;
+  st.costHint = costHint;
+  st.numInputs = inputs ? numInputs : 0;
+  stagedMaxTeam_ = 0;
   if (work.empty()) {
+    st.empty = true;
     if (inputs && numInputs) {  // the states still take their inputs
+      if (submitted_ && size_t(numInputs) * sizeof(RegionInput) > hStageBytes_)
+        PGL_SCHED_CHECK(hipEventSynchronize(evEnd_));
       grow_pinned(hStage_, hStageBytes_, size_t(numInputs) * sizeof(RegionInput));
       memcpy(hStage_, inputs, size_t(numInputs) * sizeof(RegionInput));
-      void* dev = nullptr;
-      PGL_SCHED_CHECK(hipHostGetDevicePointer(&dev, hStage_, 0));
-      launch_prepare_regions(states, nullptr, nullptr, 0, (const RegionInput*)dev, numInputs, nullptr, 0, dErr_,
-                             (unsigned long long*)dStats_, s);
     }
     return;
   }
@@ -125,7 +140,7 @@ void RegionRunner::launch(RegionState* states, std::vector<RegionWork> work, con
     jb.pad = 0;
     jb.partialOffset = T > 1 ? partialFloats : 0ull;
     if (T > 1) partialFloats += 2ull * jb.chunks * kPartStride;
-    lastMaxTeam = std::max(lastMaxTeam, T);
+    stagedMaxTeam_ = std::max(stagedMaxTeam_, T);
     if (!perRegion || perRegionTeams)
       for (uint32_t r = 0; r < T; r++) items_.push_back(make_uint2(j, r));
   }
@@ -136,39 +151,79 @@ void RegionRunner::launch(RegionState* states, std::vector<RegionWork> work, con
   const size_t inB = inputs ? size_t(numInputs) * sizeof(RegionInput) : 0;
   const size_t offItems = (jobsB + 15) & ~size_t(15), offIn = (offItems + itemsB + 15) & ~size_t(15);
   const size_t stageB = offIn + inB;
+  // a buffer that grows is freed: first wait until the previous launch,
+  // which may still be running and reading it, has completed
+  const size_t partB = std::max<unsigned long long>(partialFloats, 1) * 4;
+  if (submitted_ && (offIn > stageBytes_ || size_t(J + 1) * 4 > queueBytes_ || partB > partialBytes_ ||
+                     stageB > hStageBytes_))
+    PGL_SCHED_CHECK(hipEventSynchronize(evEnd_));
   grow(dStage_, stageBytes_, offIn);
   grow(dQueue_, queueBytes_, (size_t)(J + 1) * 4);
-  grow(dPartial_, partialBytes_, std::max<unsigned long long>(partialFloats, 1) * 4);
+  grow(dPartial_, partialBytes_, partB);
   grow_pinned(hStage_, hStageBytes_, stageB);
   char* hs = (char*)hStage_;
   memcpy(hs, jobs_.data(), jobsB);
   memcpy(hs + offItems, items_.data(), itemsB);
   if (inB) memcpy(hs + offIn, inputs, inB);
-  char* hdev = nullptr;
-  PGL_SCHED_CHECK(hipHostGetDevicePointer((void**)&hdev, hStage_, 0));
-  char* ds = (char*)dStage_;
-  launch_prepare_regions(states, hdev, ds, offIn, inB ? (const RegionInput*)(hdev + offIn) : nullptr,
-                         inB ? numInputs : 0, (uint32_t*)dQueue_, J + 1, dErr_, (unsigned long long*)dStats_, s);
+  st.J = J;
+  st.G = G;
+  st.offItems = offItems;
+  st.offIn = offIn;
+  st.inB = inB;
+  st.perRegion = perRegion;
+  st.perRegionTeams = perRegionTeams;
+}
 
+void RegionRunner::submit(RegionState* states, hipStream_t s) {
+  if (!staged_.valid) throw std::runtime_error("region runner: nothing staged");
+  Staged& st = staged_;
+  st.valid = false;
+  lastLaunches = lastJobs = lastItems = lastGrid = 0;
+  lastMaxTeam = stagedMaxTeam_;
+  *hErr_ = 0;
+  *hStats_ = RegionLaunchStats{0, 0, 0, 0};
+  timed_ = false;
+  if (hStage_ != hStageMapped_) {
+    PGL_SCHED_CHECK(hipHostGetDevicePointer(&hStageDev_, hStage_, 0));
+    hStageMapped_ = hStage_;
+  }
+  char* hdev = (char*)hStageDev_;
+  if (st.empty) {
+    if (st.numInputs) {
+      launch_prepare_regions(states, nullptr, nullptr, 0, (const RegionInput*)hdev, st.numInputs, nullptr, 0, dErr_,
+                             (unsigned long long*)dStats_, s);
+      PGL_SCHED_CHECK(hipEventRecord(evPrep_, s));
+      prepQueued_ = true;
+    }
+    return;
+  }
+  const uint32_t J = st.J;
+  char* ds = (char*)dStage_;
+  launch_prepare_regions(states, hdev, ds, st.offIn, st.inB ? (const RegionInput*)(hdev + st.offIn) : nullptr,
+                         st.inB ? st.numInputs : 0, (uint32_t*)dQueue_, J + 1, dErr_, (unsigned long long*)dStats_, s);
+  PGL_SCHED_CHECK(hipEventRecord(evPrep_, s));
+  prepQueued_ = true;
+
+  const FactoryConfig& cfg = st.cfg;
   RegionLaunch L;
   L.states = states;
   L.jobs = (const RegionJob*)ds;
   L.numJobs = J;
-  L.items = (const uint2*)(ds + offItems);
+  L.items = (const uint2*)(ds + st.offItems);
   L.numItems = (uint32_t)items_.size();
-  L.grid = std::min<uint32_t>(G, L.numItems);
+  L.grid = std::min<uint32_t>(st.G, L.numItems);
   L.queueAndCounters = (uint32_t*)dQueue_;
   L.partial = (float*)dPartial_;
   L.err = dErr_;
   L.timeoutTicks = timeoutTicks_;
   L.stats = (unsigned long long*)dStats_;
-  L.costHint = costHint;
-  L.dx = rs.dx; L.dy = rs.dy; L.dz = rs.dz; L.w = rs.w; L.pdf = rs.pdf; L.dist = rs.dist;
-  L.uniformDirs = uniformDirs;
+  L.costHint = st.costHint;
+  L.dx = st.rs.dx; L.dy = st.rs.dy; L.dz = st.rs.dz; L.w = st.rs.w; L.pdf = st.rs.pdf; L.dist = st.rs.dist;
+  L.uniformDirs = st.uniformDirs;
   L.cooperative = cooperative_ && schedule.cooperative != 0;
-  L.zeroQueue = perRegion;
+  L.zeroQueue = st.perRegion;
   PGL_SCHED_CHECK(hipEventRecord(evBegin_, s));
-  if (perRegionTeams) {
+  if (st.perRegionTeams) {
     // one launch per job: its consecutive items, the queue and counters zeroed before each
     L.cooperative = false;
     L.zeroQueue = true;
@@ -184,7 +239,7 @@ void RegionRunner::launch(RegionState* states, std::vector<RegionWork> work, con
       lastGrid = std::max(lastGrid, T);
     }
     lastLaunches = J;
-  } else if (perRegion) {
+  } else if (st.perRegion) {
     L.cooperative = false;
     L.numJobs = 1;
     L.numItems = 1;
@@ -201,11 +256,12 @@ void RegionRunner::launch(RegionState* states, std::vector<RegionWork> work, con
     lastGrid = L.grid;
   }
   PGL_SCHED_CHECK(hipEventRecord(evEnd_, s));
+  submitted_ = true;
   timed_ = true;
   lastJobs = J;
   lastItems = (uint32_t)items_.size();
-  PGL_SCHED_CHECK(hipMemcpyAsync(hErr_, dErr_, sizeof(int), hipMemcpyDeviceToHost, s));
-  PGL_SCHED_CHECK(hipMemcpyAsync(hStats_, dStats_, sizeof(RegionLaunchStats), hipMemcpyDeviceToHost, s));
+  // the error flag and the totals are adjacent: one copy back
+  PGL_SCHED_CHECK(hipMemcpyAsync(hTot_, dTot_, sizeof(LaunchTotals), hipMemcpyDeviceToHost, s));
 }
 
 float RegionRunner::kernelMs() const {

@@ -62,6 +62,16 @@ class RegionRunner {
   void launch(RegionState* states, std::vector<RegionWork> work, const RegionSamplesView& rs,
               const FactoryConfig& cfg, const float* uniformDirs, float* costHint, hipStream_t s,
               const RegionInput* inputs = nullptr, uint32_t numInputs = 0);
+  // launch() in two steps: stage() does the host work (jobs, teams, the
+  // pinned tables; it waits only for the previous launch's preparing kernel,
+  // which reads them), so it may run while the previous launch still runs;
+  // submit() queues the preparing kernel and the launch on s once the
+  // previous launch has completed.  The figures of the last completed
+  // launch (check, stats, kernelMs) stay valid until submit().
+  void stage(std::vector<RegionWork> work, const RegionSamplesView& rs, const FactoryConfig& cfg,
+             const float* uniformDirs, float* costHint, const RegionInput* inputs = nullptr,
+             uint32_t numInputs = 0);
+  void submit(RegionState* states, hipStream_t s);
   // after the stream has completed: throws if a team barrier timed out
   void check() const;
   // testing: the next check() reports a failed launch (the kernels run normally)
@@ -87,11 +97,36 @@ class RegionRunner {
   size_t queueBytes_ = 0;
   void* dPartial_ = nullptr;
   size_t partialBytes_ = 0;
+  // the error flag and the totals of a launch, adjacent (one copy back)
+  struct LaunchTotals {
+    int err;
+    int pad;
+    RegionLaunchStats stats;
+  };
+  LaunchTotals* dTot_ = nullptr;
+  LaunchTotals* hTot_ = nullptr;
   int* dErr_ = nullptr;
   int* hErr_ = nullptr;
   RegionLaunchStats* dStats_ = nullptr;
   RegionLaunchStats* hStats_ = nullptr;
   hipEvent_t evBegin_ = nullptr, evEnd_ = nullptr;
+  hipEvent_t evPrep_ = nullptr;  // after the last preparing kernel (which reads the pinned tables)
+  bool prepQueued_ = false;
+  bool submitted_ = false;  // evEnd_ has been recorded
+  // the staged launch: what submit() needs besides the tables
+  struct Staged {
+    bool valid = false, empty = false;
+    RegionSamplesView rs{};
+    FactoryConfig cfg{};
+    const float* uniformDirs = nullptr;
+    float* costHint = nullptr;
+    uint32_t numInputs = 0, J = 0, G = 0;
+    size_t offItems = 0, offIn = 0, inB = 0;
+    bool perRegion = false, perRegionTeams = false;
+  } staged_;
+  uint32_t stagedMaxTeam_ = 0;
+  void* hStageMapped_ = nullptr;  // hStage_ whose device address is hStageDev_
+  void* hStageDev_ = nullptr;
   bool timed_ = false;
   std::vector<RegionJob> jobs_;
   std::vector<uint2> items_;
