@@ -433,7 +433,8 @@ void KindField::kdUpdate(uint32_t n, hipStream_t s) {
     }
     char* const kr = dKdUp_.as<char>();
     launch_kd_round(dPx_.as<float>(), dPy_.as<float>(), dPz_.as<float>(), n, dNodeOf_.as<uint32_t>(),
-                    (const int32_t*)(kr + offS), (const FrontierNode*)kr, F, dStatPartial_.as<IntStatsAcc>(),
+                    (const int32_t*)(kr + offS), (uint32_t)slot.size(), (const FrontierNode*)kr, F,
+                    dStatPartial_.as<IntStatsAcc>(),
                     dAcc_.as<IntStatsAcc>(), s);
     kdDown_.ensure(2 * F * sizeof(IntStatsAcc));
     const IntStatsAcc* acc = kdDown_.as<IntStatsAcc>();

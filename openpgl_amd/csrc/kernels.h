@@ -44,8 +44,8 @@ void launch_descend_stats(const float* px, const float* py, const float* pz, uin
                           uint32_t numNodes, const int32_t* leafSlot, const void* leafParams, uint32_t numLeaves,
                           uint32_t* nodeOf, IntStatsAcc* partial, IntStatsAcc* acc, hipStream_t s);
 void launch_kd_round(const float* px, const float* py, const float* pz, uint32_t n, uint32_t* nodeOf,
-                     const int32_t* frontierSlot, const FrontierNode* frontier, uint32_t numFrontier,
-                     IntStatsAcc* partial, IntStatsAcc* childAcc, hipStream_t s);
+                     const int32_t* frontierSlot, uint32_t numNodes, const FrontierNode* frontier,
+                     uint32_t numFrontier, IntStatsAcc* partial, IntStatsAcc* childAcc, hipStream_t s);
 void launch_leaf_keys(const uint32_t* nodeOf, const KDNode* nodes, uint32_t n, uint32_t* keys, uint32_t* vals,
                       hipStream_t s);
 size_t sort_pairs_temp_bytes(uint32_t n);

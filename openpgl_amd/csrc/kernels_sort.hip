@@ -295,9 +295,10 @@ void counting_sort_leaf_records(void* temp, const uint32_t* nodeOf, const KDNode
 //     weights) are put in order with the full comparator by one thread each,
 //     and a run longer than 32 makes the tile sort by every field, least
 //     significant first (from index order that gives the total order);
-//   * ranges longer than one tile: merge passes in which every element finds
-//     its place in the partner run by binary search, on the weight keys and
-//     the full comparator where the weights are equal.
+//   * ranges longer than one tile: merge passes (merge path: a workgroup
+//     finds where its output range starts and ends in the two runs, stages
+//     those inputs in LDS and merges them there), on the weight keys and the
+//     full comparator where the weights are equal.
 namespace {
 
 // float offset inside the 40-byte record of the comparator's i-th field:
@@ -315,6 +316,12 @@ __device__ __forceinline__ bool record_less_after_weight(const float* __restrict
     if (ka != kb) return ka < kb;
   }
   return a < b;
+}
+
+// the same, out of line: equal weights are rare, and sixteen inlined copies
+// in the merge's unrolled searches make its code outgrow the instruction cache
+__device__ __noinline__ bool record_less_tie(const float* __restrict__ recs, uint32_t a, uint32_t b) {
+  return record_less_after_weight(recs, a, b);
 }
 
 struct DetSegment {
@@ -429,45 +436,142 @@ __global__ void __launch_bounds__(THREADS) k_det_tile_sort(const float* __restri
   }
 }
 
-// one merge pass over the segments of the table (blockIdx.y): sorted runs
-// [k*2w, k*2w + w) and [k*2w + w, (k+1)*2w) of each segment are merged (a
-// segment of at most w elements is copied); every element binary-searches
-// its rank in the partner run: weight keys first, the full comparator where
-// they are equal
+// One merge pass over the segments of the table (blockIdx.y): sorted runs
+// [k*2w, k*2w + w) and [k*2w + w, (k+1)*2w) of each segment become one run (a
+// segment of at most w elements is copied).  Merge path: workgroup x writes
+// outputs [x*kMergeTile, (x+1)*kMergeTile) of its segment.  The number of
+// elements of the first run among the first d outputs of a pair is the first
+// a at which "A[a] before B[d-1-a]" turns false along the diagonal; one wave
+// finds it for the workgroup's first and last output with 64 probes per round
+// (a range shrinks 64-fold per round of loads).  The workgroup then stages
+// its inputs in LDS, every thread finds its own split in LDS and merges
+// kMergeItems outputs sequentially, and the outputs leave through LDS in
+// order.  Order: weight keys, the full comparator where they are equal (no
+// two elements are equal in that total order).
+constexpr int kMergeItems = 8, kMergeTile = 256 * kMergeItems;
+
+__device__ __forceinline__ bool elem_less(const float* __restrict__ recs, uint32_t ka, uint32_t ia, uint32_t kb,
+                                          uint32_t ib) {
+  if (ka != kb) return ka < kb;
+  return record_less_tie(recs, ia, ib);
+}
+
+// first a in [lo, hi) with !(A[a] before B[d-1-a]) (hi if none); uniform over the wave
+__device__ uint32_t wave_merge_split(const float* __restrict__ recs, const uint32_t* __restrict__ kA,
+                                     const uint32_t* __restrict__ iA, const uint32_t* __restrict__ kB,
+                                     const uint32_t* __restrict__ iB, uint32_t d, uint32_t lo, uint32_t hi) {
+  const uint32_t lane = threadIdx.x & 63;
+  while (lo < hi) {
+    const uint32_t step = (hi - lo + 63) / 64;
+    const uint32_t a = lo + lane * step;
+    bool before = false;
+    if (a < hi) {
+      const uint32_t b = d - 1 - a;
+      before = elem_less(recs, kA[a], iA[a], kB[b], iB[b]);
+    }
+    const uint32_t c = (uint32_t)__popcll(__ballot(before));  // probes 0 .. c-1 are before (monotone)
+    if (c == 0) {
+      hi = lo;
+    } else {
+      const uint32_t nlo = lo + (c - 1) * step + 1;
+      hi = min(hi, lo + c * step);
+      lo = nlo;
+    }
+  }
+  return lo;
+}
+
 __global__ void __launch_bounds__(256) k_det_merge(const float* __restrict__ recs, const DetSegment* __restrict__ segs,
                                                    uint32_t w, const uint32_t* __restrict__ idxIn,
                                                    const uint32_t* __restrict__ keyIn, uint32_t* __restrict__ idxOut,
                                                    uint32_t* __restrict__ keyOut) {
+  __shared__ uint32_t sK[kMergeTile], sI[kMergeTile];
+  __shared__ uint32_t sSplit[2];
   const DetSegment sg = segs[blockIdx.y];
   const uint32_t n = sg.count;
+  const uint32_t o0 = blockIdx.x * kMergeTile;
+  if (o0 >= n) return;  // uniform over the workgroup
   const uint32_t* __restrict__ inI = idxIn + sg.begin;
   const uint32_t* __restrict__ inK = keyIn + sg.begin;
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const uint32_t pair = i / (2 * w), start = pair * 2 * w;
-    const uint32_t aEnd = min(start + w, n), bEnd = min(start + 2 * w, n);
-    const uint32_t v = inI[i], k = inK[i];
-    uint32_t lo, hi, rankBase;
-    if (i < aEnd) {  // element of run A: the B elements before it
-      lo = aEnd;
-      hi = bEnd;
-      rankBase = i - start;
-    } else {  // element of run B: the A elements before it
-      lo = start;
-      hi = aEnd;
-      rankBase = i - aEnd;
+  const uint32_t ps = o0 / (2 * w) * (2 * w);  // the pair (kMergeTile divides 2w)
+  const uint32_t aBeg = ps, aEnd = min(ps + w, n), bEnd = min(ps + 2 * w, n);
+  const uint32_t lenA = aEnd - aBeg, lenB = bEnd - aEnd;
+  const uint32_t d0 = o0 - ps, d1 = min(o0 + kMergeTile, bEnd) - ps;
+  const uint32_t wave = threadIdx.x >> 6;
+  if (wave < 2) {
+    const uint32_t d = wave == 0 ? d0 : d1;
+    const uint32_t lo = d > lenB ? d - lenB : 0u, hi = min(d, lenA);
+    const uint32_t a = wave_merge_split(recs, inK + aBeg, inI + aBeg, inK + aEnd, inI + aEnd, d, lo, hi);
+    if ((threadIdx.x & 63) == 0) sSplit[wave] = a;
+  }
+  __syncthreads();
+  const uint32_t a0 = sSplit[0], a1 = sSplit[1];
+  const uint32_t na = a1 - a0, nb = (d1 - a1) - (d0 - a0);
+  // stage A[a0, a1) then B[d0 - a0, d1 - a1): all loads before the LDS writes
+  uint32_t tk[kMergeItems], ti[kMergeItems];
+#pragma unroll
+  for (int u = 0; u < kMergeItems; u++) {
+    const uint32_t j = threadIdx.x + u * 256;
+    tk[u] = ti[u] = 0;
+    if (j < na + nb) {
+      const uint32_t src = j < na ? aBeg + a0 + j : aEnd + (d0 - a0) + (j - na);
+      tk[u] = inK[src];
+      ti[u] = inI[src];
     }
-    const uint32_t first = lo;
+  }
+#pragma unroll
+  for (int u = 0; u < kMergeItems; u++) {
+    const uint32_t j = threadIdx.x + u * 256;
+    if (j < na + nb) {
+      sK[j] = tk[u];
+      sI[j] = ti[u];
+    }
+  }
+  __syncthreads();
+  // this thread's outputs [dl, dl + kMergeItems) of the workgroup's range
+  const uint32_t dl = threadIdx.x * kMergeItems, dn = na + nb;
+  uint32_t outK[kMergeItems], outI[kMergeItems];
+  if (dl < dn) {
+    uint32_t lo = dl > nb ? dl - nb : 0u, hi = min(dl, na);
     while (lo < hi) {
-      const uint32_t mid = (lo + hi) >> 1;
-      const uint32_t km = inK[mid];
-      const bool less = km < k || (km == k && record_less_after_weight(recs, inI[mid], v));
-      if (less) lo = mid + 1;
+      const uint32_t mid = (lo + hi) >> 1, b = na + dl - 1 - mid;
+      if (elem_less(recs, sK[mid], sI[mid], sK[b], sI[b])) lo = mid + 1;
       else hi = mid;
     }
-    const uint32_t o = sg.begin + start + rankBase + (lo - first);
-    idxOut[o] = v;
-    keyOut[o] = k;
+    uint32_t ia = lo, ib = na + (dl - lo);
+    uint32_t hk = ia < na ? sK[ia] : 0u, hi2 = ia < na ? sI[ia] : 0u;
+    uint32_t gk = ib < dn ? sK[ib] : 0u, gi = ib < dn ? sI[ib] : 0u;
+#pragma unroll
+    for (int u = 0; u < kMergeItems; u++) {
+      const bool takeA = ia < na && (ib >= dn || elem_less(recs, hk, hi2, gk, gi));
+      outK[u] = takeA ? hk : gk;
+      outI[u] = takeA ? hi2 : gi;
+      if (takeA) {
+        ia++;
+        if (ia < na) { hk = sK[ia]; hi2 = sI[ia]; }
+      } else {
+        ib++;
+        if (ib < dn) { gk = sK[ib]; gi = sI[ib]; }
+      }
+    }
+  }
+  __syncthreads();
+  if (dl < dn) {
+#pragma unroll
+    for (int u = 0; u < kMergeItems; u++) {
+      sK[dl + u] = outK[u];
+      sI[dl + u] = outI[u];
+    }
+  }
+  __syncthreads();
+  // outputs in order, coalesced
+#pragma unroll
+  for (int u = 0; u < kMergeItems; u++) {
+    const uint32_t j = threadIdx.x + u * 256;
+    if (j < dn) {
+      idxOut[sg.begin + o0 + j] = sI[j];
+      keyOut[sg.begin + o0 + j] = sK[j];
+    }
   }
 }
 
@@ -539,14 +643,17 @@ void det_sort_segments(void* temp, void* pinnedHost, size_t pinnedBytes, const v
     hipLaunchKernelGGL((k_det_tile_sort<256, 8>), dim3(num[1]), dim3(256), 0, s, recs, dev + cap, vals, keysIn, keysA);
   if (num[2] == 0) return;
   // merge passes over all ranges longer than one tile, ping-pong between the buffers
-  const dim3 grid(std::min<uint32_t>((longest + 255) / 256, 64), num[2]);
+  static_assert((2 * kDetTile) % kMergeTile == 0, "a merge workgroup's outputs lie in one pair of runs");
+  const dim3 grid((longest + kMergeTile - 1) / kMergeTile, num[2]);
   uint32_t *ia = vals, *ka = keysA, *ib = idxB, *kb = keysB;
   for (uint32_t w = kDetTile; w < longest; w *= 2) {
     hipLaunchKernelGGL(k_det_merge, grid, dim3(256), 0, s, recs, dev + 2 * cap, w, ia, ka, ib, kb);
     std::swap(ia, ib);
     std::swap(ka, kb);
   }
-  if (ia != vals) hipLaunchKernelGGL(k_det_copy, grid, dim3(256), 0, s, dev + 2 * cap, ia, vals);
+  if (ia != vals)
+    hipLaunchKernelGGL(k_det_copy, dim3(std::min<uint32_t>((longest + 255) / 256, 64), num[2]), dim3(256), 0, s,
+                       dev + 2 * cap, ia, vals);
 }
 
 }  // namespace pglgpu

@@ -80,24 +80,33 @@ __device__ __forceinline__ void int_terms(float p, float c, float inv, long long
   v = (long long)tv;
 }
 
-__device__ __forceinline__ void acc_add(IntStatsAcc* a, const long long m[3], const long long v[3]) {
-  atomicAdd(&a->n, 1ull);
-  for (int j = 0; j < 3; j++) {
-    atomicAdd((unsigned long long*)&a->sum[j], (unsigned long long)m[j]);
-    atomicAdd((unsigned long long*)&a->sq[j], (unsigned long long)v[j]);
-    atomicMin(&a->lo[j], m[j]);
-    atomicMax(&a->hi[j], m[j]);
-  }
-}
+static const int kSlotTile = 448;  // LDS accumulator slots per pass (44.8 KiB)
+static const uint32_t kLdsNodes = 2048;  // trees up to this many nodes are walked from LDS (16 KiB)
+static const uint32_t kLdsSlots = 4096;  // node -> frontier slot tables up to this size are read from LDS (16 KiB)
+static const int kStatsItems = 4;        // samples per thread and iteration of the statistics passes
+// 256 threads with at most 104 VGPRs: one wave per SIMD fits beside two
+// workgroups of the region kernel (the spatial phase overlaps it)
+static const uint32_t kStatsThreads = 256;
 
-__device__ __forceinline__ void acc_merge(IntStatsAcc* dst, const IntStatsAcc& s) {
-  if (s.n == 0) return;
-  atomicAdd(&dst->n, s.n);
+// Block-private statistics table in LDS, one array per field, so that slot s
+// of a field and slot s + 1 sit in neighbouring banks.  (With one 128-byte
+// record per slot, field f of every slot maps to the same two banks and the
+// lanes' atomics on random slots serialise.)
+struct SlotTable {
+  uint32_t n[kSlotTile];
+  long long sum[3][kSlotTile];
+  long long sq[3][kSlotTile];
+  long long lo[3][kSlotTile];
+  long long hi[3][kSlotTile];
+};
+
+__device__ __forceinline__ void slot_init(SlotTable& tb, int c) {
+  tb.n[c] = 0;
   for (int j = 0; j < 3; j++) {
-    atomicAdd((unsigned long long*)&dst->sum[j], (unsigned long long)s.sum[j]);
-    atomicAdd((unsigned long long*)&dst->sq[j], (unsigned long long)s.sq[j]);
-    atomicMin(&dst->lo[j], s.lo[j]);
-    atomicMax(&dst->hi[j], s.hi[j]);
+    tb.sum[j][c] = 0;
+    tb.sq[j][c] = 0;
+    tb.lo[j][c] = 2147483647ll;   //  std::numeric_limits<int>::max()
+    tb.hi[j][c] = -2147483647ll;  // -std::numeric_limits<int>::max()
   }
 }
 
@@ -106,18 +115,17 @@ __device__ __forceinline__ void acc_init(IntStatsAcc* a) {
   for (int j = 0; j < 3; j++) {
     a->sum[j] = 0;
     a->sq[j] = 0;
-    a->lo[j] = 2147483647ll;    //  std::numeric_limits<int>::max()
-    a->hi[j] = -2147483647ll;   // -std::numeric_limits<int>::max()
+    a->lo[j] = 2147483647ll;
+    a->hi[j] = -2147483647ll;
   }
 }
 
-// Accumulation into a block-private LDS table.  When every active lane of
-// the wave targets the same slot (samples arrive grouped by region after the
+// Accumulation into the block's LDS table.  When every active lane of the
+// wave targets the same slot (samples arrive grouped by region after the
 // first frames) the wave reduces with xor-shuffles and one lane issues the
-// LDS atomics; otherwise each lane issues its own LDS atomics (random slots
-// spread over the banks).  Every lane of the wave must call this; inactive
-// lanes pass slot < 0.
-__device__ void wave_accumulate(IntStatsAcc* __restrict__ base, int slot, const long long m[3], const long long v[3]) {
+// LDS atomics; otherwise each lane issues its own LDS atomics.  Every lane of
+// the wave must call this; inactive lanes pass slot < 0.
+__device__ void wave_accumulate(SlotTable& tb, int slot, const long long m[3], const long long v[3]) {
   const int lane = threadIdx.x & 63;
   const unsigned long long active = __ballot(slot >= 0);
   if (!active) return;
@@ -125,13 +133,12 @@ __device__ void wave_accumulate(IntStatsAcc* __restrict__ base, int slot, const 
   const int t = __shfl(slot, leader);
   if (__ballot(slot >= 0 && slot != t) != 0) {
     if (slot >= 0) {
-      IntStatsAcc* a = base + slot;
-      atomicAdd(&a->n, 1ull);
+      atomicAdd(&tb.n[slot], 1u);
       for (int j = 0; j < 3; j++) {
-        atomicAdd((unsigned long long*)&a->sum[j], (unsigned long long)m[j]);
-        atomicAdd((unsigned long long*)&a->sq[j], (unsigned long long)v[j]);
-        atomicMin(&a->lo[j], m[j]);
-        atomicMax(&a->hi[j], m[j]);
+        atomicAdd((unsigned long long*)&tb.sum[j][slot], (unsigned long long)m[j]);
+        atomicAdd((unsigned long long*)&tb.sq[j][slot], (unsigned long long)v[j]);
+        atomicMin(&tb.lo[j][slot], m[j]);
+        atomicMax(&tb.hi[j][slot], m[j]);
       }
     }
     return;
@@ -154,13 +161,12 @@ __device__ void wave_accumulate(IntStatsAcc* __restrict__ base, int slot, const 
     }
   }
   if (lane == leader) {
-    IntStatsAcc* a = base + t;
-    atomicAdd(&a->n, (unsigned long long)__popcll(active));
+    atomicAdd(&tb.n[t], (uint32_t)__popcll(active));
     for (int j = 0; j < 3; j++) {
-      atomicAdd((unsigned long long*)&a->sum[j], (unsigned long long)sm[j]);
-      atomicAdd((unsigned long long*)&a->sq[j], (unsigned long long)sq[j]);
-      atomicMin(&a->lo[j], lo[j]);
-      atomicMax(&a->hi[j], hi[j]);
+      atomicAdd((unsigned long long*)&tb.sum[j][t], (unsigned long long)sm[j]);
+      atomicAdd((unsigned long long*)&tb.sq[j][t], (unsigned long long)sq[j]);
+      atomicMin(&tb.lo[j][t], lo[j]);
+      atomicMax(&tb.hi[j][t], hi[j]);
     }
   }
 }
@@ -171,9 +177,6 @@ struct LeafParams {
   float center[3];
   float invHalf[3];
 };
-
-static const int kSlotTile = 448;  // LDS accumulator slots per pass (56 KiB)
-static const uint32_t kLdsNodes = 2048;  // trees up to this many nodes are walked from LDS (16 KiB)
 
 // Statistics pass over all samples with a block-private accumulator table for
 // slots [slotBase, slotBase + kSlotTile); each block writes its table to
@@ -187,7 +190,7 @@ static const uint32_t kLdsNodes = 2048;  // trees up to this many nodes are walk
 //   else:     one level of the push-down below the frontier nodes: left iff
 //             p[dim] < pivot; child statistics w.r.t. the frontier node's bounds.
 template <bool kDescend>
-__global__ void __launch_bounds__(256) k_spatial_stats(const float* __restrict__ px, const float* __restrict__ py,
+__global__ void __launch_bounds__(kStatsThreads) k_spatial_stats(const float* __restrict__ px, const float* __restrict__ py,
                                                        const float* __restrict__ pz, uint32_t n,
                                                        const KDNode* __restrict__ nodes,
                                                        const int32_t* __restrict__ slotOfNode,
@@ -196,67 +199,229 @@ __global__ void __launch_bounds__(256) k_spatial_stats(const float* __restrict__
                                                        uint32_t* __restrict__ nodeOf, int writeNodes, int slotBase,
                                                        int numSlots, IntStatsAcc* __restrict__ partial,
                                                        uint32_t numNodes) {
-  __shared__ IntStatsAcc acc[kSlotTile];
+  __shared__ SlotTable acc;
+  // descend: the tree, every leaf's index replaced by its slot, and the
+  // tile's leaf parameters; else: the slot of every node and the tile's
+  // frontier nodes.  Each per-sample lookup is then an LDS access instead of
+  // a dependent global load.
   __shared__ KDNode tree[kDescend ? kLdsNodes : 1];
-  const bool inLds = kDescend && numNodes <= kLdsNodes;
-  if (inLds)
-    for (uint32_t c = threadIdx.x; c < numNodes; c += blockDim.x) tree[c] = nodes[c];
-  for (int c = threadIdx.x; c < numSlots; c += blockDim.x) acc_init(&acc[c]);
-  __syncthreads();
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
-    const uint32_t i = base + threadIdx.x;
-    int local = -1;
-    long long m[3] = {0, 0, 0}, v[3] = {0, 0, 0};
-    if (i < n) {
-      const float p[3] = {px[i], py[i], pz[i]};
-      int slot = -1;
-      float c[3], inv[3];
-      if (kDescend) {
-        uint32_t nd = 0;
-        if (inLds) {
-          KDNode k = tree[0];
-          while (!k.isLeaf()) {
-            const uint32_t d = k.dim();
-            const float pd = d == 0 ? p[0] : (d == 1 ? p[1] : p[2]);
-            nd = k.idx() + ((pd < k.pivot) ? 0u : 1u);
-            k = tree[nd];
-          }
-        } else {
-          KDNode k = nodes[0];
-          while (!k.isLeaf()) {
-            const uint32_t d = k.dim();
-            const float pd = d == 0 ? p[0] : (d == 1 ? p[1] : p[2]);
-            nd = k.idx() + ((pd < k.pivot) ? 0u : 1u);
-            k = nodes[nd];
-          }
-        }
-        if (writeNodes) nodeOf[i] = nd;
-        slot = slotOfNode[nd];
-        const LeafParams lp = leafParams[slot];
-        for (int j = 0; j < 3; j++) { c[j] = lp.center[j]; inv[j] = lp.invHalf[j]; }
-      } else {
-        const uint32_t node = nodeOf[i];
-        const int32_t f = slotOfNode[node];
-        if (f >= 0) {
-          const FrontierNode fn = frontier[f];
-          const float pd = fn.dim == 0 ? p[0] : (fn.dim == 1 ? p[1] : p[2]);
-          const uint32_t side = (pd < fn.pivot) ? 0u : 1u;
-          if (writeNodes) nodeOf[i] = fn.left + side;
-          slot = 2 * f + (int)side;
-          for (int j = 0; j < 3; j++) { c[j] = fn.center[j]; inv[j] = fn.invHalf[j]; }
+  __shared__ LeafParams sPar[kDescend ? kSlotTile : 1];
+  __shared__ int32_t sSlot[kDescend ? 1 : kLdsSlots];
+  __shared__ FrontierNode sFr[kDescend ? 1 : kSlotTile / 2];
+  const bool inLds = numNodes <= (kDescend ? kLdsNodes : kLdsSlots);
+  const int f0 = slotBase / 2, nf = (numSlots + 1) / 2;  // frontier nodes of the tile (slotBase is even)
+  // Staging: every load of a table is issued before the first LDS write (a
+  // loop that waits for each load in turn pays the memory latency per
+  // iteration).  Stores to memory come last in every iteration below for the
+  // same reason: the loads that follow a store wait for it.
+  if (kDescend) {
+    if (inLds) {
+      constexpr int kPer = kLdsNodes / kStatsThreads;
+      KDNode tn[kPer];
+      int32_t ts[kPer];
+#pragma unroll
+      for (int u = 0; u < kPer; u++) {
+        const uint32_t c = threadIdx.x + u * kStatsThreads;
+        if (c < numNodes) {
+          tn[u] = nodes[c];
+          ts[u] = slotOfNode[c];
         }
       }
-      if (slot >= slotBase && slot < slotBase + numSlots) {
-        local = slot - slotBase;
-        for (int j = 0; j < 3; j++) int_terms(p[j], c[j], inv[j], m[j], v[j]);
+#pragma unroll
+      for (int u = 0; u < kPer; u++) {
+        const uint32_t c = threadIdx.x + u * kStatsThreads;
+        if (c < numNodes) {
+          KDNode k = tn[u];
+          if (k.isLeaf()) k.bits = (3u << 30) | (uint32_t)ts[u];
+          tree[c] = k;
+        }
       }
     }
-    wave_accumulate(acc, local, m, v);
+    // leaf parameters as words (a struct copy through a register array goes through scratch)
+    constexpr int kW = sizeof(LeafParams) / 4;
+    constexpr int kPerP = (kSlotTile * kW + kStatsThreads - 1) / kStatsThreads;
+    const float* src = (const float*)(leafParams + slotBase);
+    float* dst = (float*)sPar;
+    float tp[kPerP];
+#pragma unroll
+    for (int u = 0; u < kPerP; u++) {
+      const int c = (int)threadIdx.x + u * (int)kStatsThreads;
+      if (c < numSlots * kW) tp[u] = src[c];
+    }
+#pragma unroll
+    for (int u = 0; u < kPerP; u++) {
+      const int c = (int)threadIdx.x + u * (int)kStatsThreads;
+      if (c < numSlots * kW) dst[c] = tp[u];
+    }
+  } else {
+    if (inLds) {
+      constexpr int kPer = kLdsSlots / kStatsThreads;
+      int32_t ts[kPer];
+#pragma unroll
+      for (int u = 0; u < kPer; u++) {
+        const uint32_t c = threadIdx.x + u * kStatsThreads;
+        if (c < numNodes) ts[u] = slotOfNode[c];
+      }
+#pragma unroll
+      for (int u = 0; u < kPer; u++) {
+        const uint32_t c = threadIdx.x + u * kStatsThreads;
+        if (c < numNodes) sSlot[c] = ts[u];
+      }
+    }
+    constexpr int kW = sizeof(FrontierNode) / 4;
+    constexpr int kPerF = (kSlotTile / 2 * kW + kStatsThreads - 1) / kStatsThreads;
+    const uint32_t* src = (const uint32_t*)(frontier + f0);
+    uint32_t* dst = (uint32_t*)sFr;
+    uint32_t tf[kPerF];
+#pragma unroll
+    for (int u = 0; u < kPerF; u++) {
+      const int c = (int)threadIdx.x + u * (int)kStatsThreads;
+      if (c < nf * kW) tf[u] = src[c];
+    }
+#pragma unroll
+    for (int u = 0; u < kPerF; u++) {
+      const int c = (int)threadIdx.x + u * (int)kStatsThreads;
+      if (c < nf * kW) dst[c] = tf[u];
+    }
+  }
+  for (int c = threadIdx.x; c < numSlots; c += blockDim.x) slot_init(acc, c);
+  __syncthreads();
+  // kStatsItems samples per thread and iteration, their loads issued together
+  const uint32_t step = blockDim.x * kStatsItems, stride = gridDim.x * step;
+  for (uint32_t base = blockIdx.x * step; base < n; base += stride) {
+    float p[kStatsItems][3];
+    uint32_t nd[kStatsItems];
+#pragma unroll
+    for (int q = 0; q < kStatsItems; q++) {
+      const uint32_t i = base + q * blockDim.x + threadIdx.x;
+      nd[q] = 0;
+      p[q][0] = p[q][1] = p[q][2] = 0.f;
+      if (i < n) {
+        p[q][0] = px[i];
+        p[q][1] = py[i];
+        p[q][2] = pz[i];
+        if (!kDescend) nd[q] = nodeOf[i];
+      }
+    }
+    // lookups of all samples first (the walks advance in lockstep, so their
+    // dependent loads overlap), then the accumulation, then the stores
+    int slot[kStatsItems];
+    uint32_t node[kStatsItems];
+    float cen[kStatsItems][3], inv[kStatsItems][3];  // centre and 1/half extent of the slot's bounds
+    if (kDescend) {
+      KDNode k[kStatsItems];
+      if (inLds) {
+#pragma unroll
+        for (int q = 0; q < kStatsItems; q++) {
+          node[q] = 0;
+          k[q] = tree[0];
+        }
+        for (;;) {
+          bool more = false;
+#pragma unroll
+          for (int q = 0; q < kStatsItems; q++) {
+            if (k[q].isLeaf()) continue;
+            more = true;
+            const uint32_t d = k[q].dim();
+            const float pd = d == 0 ? p[q][0] : (d == 1 ? p[q][1] : p[q][2]);
+            node[q] = k[q].idx() + ((pd < k[q].pivot) ? 0u : 1u);
+            k[q] = tree[node[q]];
+          }
+          if (!more) break;
+        }
+#pragma unroll
+        for (int q = 0; q < kStatsItems; q++) slot[q] = (int)k[q].idx();
+      } else {
+#pragma unroll
+        for (int q = 0; q < kStatsItems; q++) {
+          node[q] = 0;
+          k[q] = nodes[0];
+        }
+        for (;;) {
+          bool more = false;
+#pragma unroll
+          for (int q = 0; q < kStatsItems; q++) {
+            if (k[q].isLeaf()) continue;
+            more = true;
+            const uint32_t d = k[q].dim();
+            const float pd = d == 0 ? p[q][0] : (d == 1 ? p[q][1] : p[q][2]);
+            node[q] = k[q].idx() + ((pd < k[q].pivot) ? 0u : 1u);
+            k[q] = nodes[node[q]];
+          }
+          if (!more) break;
+        }
+#pragma unroll
+        for (int q = 0; q < kStatsItems; q++) slot[q] = slotOfNode[node[q]];
+      }
+#pragma unroll
+      for (int q = 0; q < kStatsItems; q++) {
+        const uint32_t i = base + q * blockDim.x + threadIdx.x;
+        if (i >= n) slot[q] = -1;
+        for (int j = 0; j < 3; j++) cen[q][j] = inv[q][j] = 0.f;
+        if (slot[q] >= slotBase && slot[q] < slotBase + numSlots) {
+          const LeafParams& lp = sPar[slot[q] - slotBase];
+          for (int j = 0; j < 3; j++) { cen[q][j] = lp.center[j]; inv[q][j] = lp.invHalf[j]; }
+        }
+      }
+    } else {
+      int32_t f[kStatsItems];
+      if (inLds) {
+#pragma unroll
+        for (int q = 0; q < kStatsItems; q++) f[q] = sSlot[nd[q]];
+      } else {
+#pragma unroll
+        for (int q = 0; q < kStatsItems; q++) f[q] = slotOfNode[nd[q]];
+      }
+#pragma unroll
+      for (int q = 0; q < kStatsItems; q++) {
+        const uint32_t i = base + q * blockDim.x + threadIdx.x;
+        slot[q] = -1;
+        node[q] = 0;
+        for (int j = 0; j < 3; j++) cen[q][j] = inv[q][j] = 0.f;
+        if (i < n && f[q] >= 0) {
+          const int fl = f[q] - f0;
+          const FrontierNode fn = (fl >= 0 && fl < nf) ? sFr[fl] : frontier[f[q]];
+          const float pd = fn.dim == 0 ? p[q][0] : (fn.dim == 1 ? p[q][1] : p[q][2]);
+          const uint32_t side = (pd < fn.pivot) ? 0u : 1u;
+          node[q] = fn.left + side;
+          slot[q] = 2 * f[q] + (int)side;
+          for (int j = 0; j < 3; j++) { cen[q][j] = fn.center[j]; inv[q][j] = fn.invHalf[j]; }
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kStatsItems; q++) {
+      int local = -1;
+      long long m[3] = {0, 0, 0}, v[3] = {0, 0, 0};
+      if (slot[q] >= slotBase && slot[q] < slotBase + numSlots) {
+        local = slot[q] - slotBase;
+        for (int j = 0; j < 3; j++) int_terms(p[q][j], cen[q][j], inv[q][j], m[j], v[j]);
+      }
+      wave_accumulate(acc, local, m, v);
+    }
+    if (writeNodes) {
+#pragma unroll
+      for (int q = 0; q < kStatsItems; q++) {
+        const uint32_t i = base + q * blockDim.x + threadIdx.x;
+        // descend: every sample; a round: the samples below the frontier
+        if (i < n && (kDescend || slot[q] >= 0)) nodeOf[i] = node[q];
+      }
+    }
   }
   __syncthreads();
   IntStatsAcc* out = partial + (size_t)blockIdx.x * numSlots;
-  for (int c = threadIdx.x; c < numSlots; c += blockDim.x) out[c] = acc[c];
+  for (int c = threadIdx.x; c < numSlots; c += blockDim.x) {
+    IntStatsAcc a;
+    a.n = acc.n[c];
+    for (int j = 0; j < 3; j++) {
+      a.sum[j] = acc.sum[j][c];
+      a.sq[j] = acc.sq[j][c];
+      a.lo[j] = acc.lo[j][c];
+      a.hi[j] = acc.hi[j][c];
+      a.pad[j] = 0;
+    }
+    out[c] = a;
+  }
 }
 
 // out[slot] = combination of the per-block tables: one workgroup per slot,
@@ -268,14 +433,22 @@ __global__ void __launch_bounds__(256) k_reduce_partials(const IntStatsAcc* __re
   const int c = blockIdx.x;
   IntStatsAcc r;
   acc_init(&r);
-  for (uint32_t b = threadIdx.x; b < blocks; b += blockDim.x) {
-    const IntStatsAcc& p = partial[(size_t)b * numSlots + c];
-    r.n += p.n;
-    for (int j = 0; j < 3; j++) {
-      r.sum[j] += p.sum[j];
-      r.sq[j] += p.sq[j];
-      r.lo[j] = p.lo[j] < r.lo[j] ? p.lo[j] : r.lo[j];
-      r.hi[j] = p.hi[j] > r.hi[j] ? p.hi[j] : r.hi[j];
+  // two rows per step, both loaded before either is combined
+  for (uint32_t b = threadIdx.x; b < blocks; b += 2 * blockDim.x) {
+    const uint32_t b2 = b + blockDim.x;
+    IntStatsAcc p[2];
+    p[0] = partial[(size_t)b * numSlots + c];
+    if (b2 < blocks) p[1] = partial[(size_t)b2 * numSlots + c];
+    else acc_init(&p[1]);
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      r.n += p[h].n;
+      for (int j = 0; j < 3; j++) {
+        r.sum[j] += p[h].sum[j];
+        r.sq[j] += p[h].sq[j];
+        r.lo[j] = p[h].lo[j] < r.lo[j] ? p[h].lo[j] : r.lo[j];
+        r.hi[j] = p[h].hi[j] > r.hi[j] ? p[h].hi[j] : r.hi[j];
+      }
     }
   }
   red[threadIdx.x] = r;
@@ -415,7 +588,8 @@ uint32_t launch_bounds(const float* px, const float* py, const float* pz, uint32
   return g;
 }
 
-static uint32_t stats_blocks(uint32_t n) { return grid_for(n, 256 * 4, 512); }
+// every block's table is one row of the partial tables that k_reduce_partials combines
+static uint32_t stats_blocks(uint32_t n) { return grid_for(n, kStatsThreads * kStatsItems, 512); }
 
 size_t spatial_partial_bytes(uint32_t n) { return (size_t)stats_blocks(n) * kSlotTile * sizeof(IntStatsAcc); }
 
@@ -429,13 +603,13 @@ static void run_stats_passes(bool descend, const float* px, const float* py, con
     // the round kernel reads nodeOf to find its frontier node: write it in the last pass only
     const int writeNodes = base + kSlotTile >= numSlots ? 1 : 0;
     if (descend)
-      hipLaunchKernelGGL(k_spatial_stats<true>, dim3(blocks), dim3(256), 0, s, px, py, pz, n, nodes, slotOfNode,
+      hipLaunchKernelGGL(k_spatial_stats<true>, dim3(blocks), dim3(kStatsThreads), 0, s, px, py, pz, n, nodes, slotOfNode,
                          (const LeafParams*)leafParams, frontier, nodeOf, writeNodes, (int)base, tile, partial,
                          numNodes);
     else
-      hipLaunchKernelGGL(k_spatial_stats<false>, dim3(blocks), dim3(256), 0, s, px, py, pz, n, nodes, slotOfNode,
+      hipLaunchKernelGGL(k_spatial_stats<false>, dim3(blocks), dim3(kStatsThreads), 0, s, px, py, pz, n, nodes, slotOfNode,
                          (const LeafParams*)leafParams, frontier, nodeOf, writeNodes, (int)base, tile, partial,
-                         0u);
+                         numNodes);
     if (tile > 0)
       hipLaunchKernelGGL(k_reduce_partials, dim3(tile), dim3(256), 0, s, partial, blocks, tile, out + base);
     if (numSlots == 0) break;
@@ -450,9 +624,9 @@ void launch_descend_stats(const float* px, const float* py, const float* pz, uin
 }
 
 void launch_kd_round(const float* px, const float* py, const float* pz, uint32_t n, uint32_t* nodeOf,
-                     const int32_t* frontierSlot, const FrontierNode* frontier, uint32_t numFrontier,
-                     IntStatsAcc* partial, IntStatsAcc* childAcc, hipStream_t s) {
-  run_stats_passes(false, px, py, pz, n, nullptr, 0, frontierSlot, nullptr, frontier, nodeOf, 2 * numFrontier,
+                     const int32_t* frontierSlot, uint32_t numNodes, const FrontierNode* frontier,
+                     uint32_t numFrontier, IntStatsAcc* partial, IntStatsAcc* childAcc, hipStream_t s) {
+  run_stats_passes(false, px, py, pz, n, nullptr, numNodes, frontierSlot, nullptr, frontier, nodeOf, 2 * numFrontier,
                    partial, childAcc, s);
 }
 
