@@ -185,11 +185,13 @@ KindField::KindField(const KdSettings& kd, const FactoryConfig& fc, bool determi
                              std::to_string(vectorWidth) +
                              " device (its kd-tree reciprocals follow that device's arithmetic)");
   for (int i = 0; i < 3; i++) { sceneLo_[i] = sceneHi_[i] = treeLo_[i] = treeHi_[i] = 0.f; }
+  kdDown_.coherent = true;
   for (int p = 0; p < 2; p++) {
     for (auto& e : evSp_[p]) PGL_HIP_CHECK(hipEventCreate(&e));
-    for (auto& e : evDir_[p]) PGL_HIP_CHECK(hipEventCreate(&e));
+    PGL_HIP_CHECK(hipEventCreateWithFlags(&evDir_[p], hipEventDisableTiming));
     for (auto& e : evKnn_[p]) PGL_HIP_CHECK(hipEventCreate(&e));
   }
+  PGL_HIP_CHECK(hipEventCreateWithFlags(&evKd_, hipEventDisableTiming));
   PGL_HIP_CHECK(hipStreamCreate(&sp_));
   std::vector<float> ud;
   uniform_directions(ud);
@@ -202,9 +204,10 @@ KindField::~KindField() {
   (void)hipStreamSynchronize(sp_);
   for (int p = 0; p < 2; p++) {
     for (auto& e : evSp_[p]) (void)hipEventDestroy(e);
-    for (auto& e : evDir_[p]) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(evDir_[p]);
     for (auto& e : evKnn_[p]) (void)hipEventDestroy(e);
   }
+  (void)hipEventDestroy(evKd_);
   (void)hipStreamDestroy(sp_);
 }
 
@@ -316,6 +319,18 @@ void KindField::visitLeaf(uint32_t node, uint32_t depth, const float lo[3], cons
   }
 }
 
+// Waits for the work queued on s so far by polling an event recorded behind
+// it: the kd rounds' waits are on the update's critical path, and a blocking
+// stream wait may hand the thread to the OS scheduler and wake it late.
+static void spin_wait(hipEvent_t ev, hipStream_t s) {
+  PGL_HIP_CHECK(hipEventRecord(ev, s));
+  for (;;) {
+    const hipError_t e = hipEventQuery(ev);
+    if (e == hipSuccess) return;
+    if (e != hipErrorNotReady) PGL_HIP_CHECK(e);
+  }
+}
+
 void KindField::kdUpdate(uint32_t n, hipStream_t s) {
   FrontierItemList frontier, next;
   uint32_t rounds = 1;
@@ -369,7 +384,8 @@ void KindField::kdUpdate(uint32_t n, hipStream_t s) {
   std::vector<float> params(6 * L);
   for (uint32_t i = 0; i < L; i++)
     for (int j = 0; j < 3; j++) { params[6 * i + j] = leaves[i].center[j]; params[6 * i + 3 + j] = leaves[i].invHalf[j]; }
-  dAcc_.ensure(std::max<size_t>(L, 2) * sizeof(IntStatsAcc));
+  // the reductions write the statistics straight into coherent pinned memory
+  kdDown_.ensure(std::max<size_t>(L, 2) * sizeof(IntStatsAcc));
   // one upload: nodes | leaf slots | leaf parameters (16-byte aligned parts)
   const size_t nb = nodes_.size() * sizeof(KDNode), sb = leafSlot.size() * sizeof(int32_t), pb = params.size() * 4;
   const size_t offSlot = (nb + 15) & ~size_t(15), offPar = (offSlot + sb + 15) & ~size_t(15);
@@ -387,12 +403,10 @@ void KindField::kdUpdate(uint32_t n, hipStream_t s) {
   dStatPartial_.ensure(spatial_partial_bytes(n));
   launch_descend_stats(dPx_.as<float>(), dPy_.as<float>(), dPz_.as<float>(), n, (const KDNode*)kd,
                        (uint32_t)nodes_.size(), (const int32_t*)(kd + offSlot), kd + offPar, L, dNodeOf_.as<uint32_t>(),
-                       dStatPartial_.as<IntStatsAcc>(), dAcc_.as<IntStatsAcc>(), s);
+                       dStatPartial_.as<IntStatsAcc>(), (IntStatsAcc*)kdDown_.dev, s);
   {
-    kdDown_.ensure(std::max<size_t>(L, 1) * sizeof(IntStatsAcc));
     const IntStatsAcc* acc = kdDown_.as<IntStatsAcc>();
-    PGL_HIP_CHECK(hipMemcpyAsync(kdDown_.p, dAcc_.p, L * sizeof(IntStatsAcc), hipMemcpyDeviceToHost, s));
-    PGL_HIP_CHECK(hipStreamSynchronize(s));
+    spin_wait(evKd_, s);
     for (uint32_t i = 0; i < L; i++) {
       if (acc[i].n == 0) continue;
       const LeafInfo& li = leaves[i];
@@ -419,7 +433,7 @@ void KindField::kdUpdate(uint32_t n, hipStream_t s) {
       bounds_params(it.lo, it.hi, fn[f].center, &halves[3 * f], fn[f].invHalf, vectorWidth_);
       slot[it.node] = (int32_t)f;
     }
-    dAcc_.ensure(2 * F * sizeof(IntStatsAcc));
+    kdDown_.ensure(2 * F * sizeof(IntStatsAcc));
     // one upload: frontier nodes | slots
     const size_t fb = F * sizeof(FrontierNode), rsb = slot.size() * sizeof(int32_t);
     const size_t offS = (fb + 15) & ~size_t(15);
@@ -434,12 +448,9 @@ void KindField::kdUpdate(uint32_t n, hipStream_t s) {
     char* const kr = dKdUp_.as<char>();
     launch_kd_round(dPx_.as<float>(), dPy_.as<float>(), dPz_.as<float>(), n, dNodeOf_.as<uint32_t>(),
                     (const int32_t*)(kr + offS), (uint32_t)slot.size(), (const FrontierNode*)kr, F,
-                    dStatPartial_.as<IntStatsAcc>(),
-                    dAcc_.as<IntStatsAcc>(), s);
-    kdDown_.ensure(2 * F * sizeof(IntStatsAcc));
+                    dStatPartial_.as<IntStatsAcc>(), (IntStatsAcc*)kdDown_.dev, s);
     const IntStatsAcc* acc = kdDown_.as<IntStatsAcc>();
-    PGL_HIP_CHECK(hipMemcpyAsync(kdDown_.p, dAcc_.p, 2 * F * sizeof(IntStatsAcc), hipMemcpyDeviceToHost, s));
-    PGL_HIP_CHECK(hipStreamSynchronize(s));
+    spin_wait(evKd_, s);
     next.items.clear();
     for (uint32_t f = 0; f < F; f++) {
       const FrontierItem& it = frontier.items[f];
@@ -536,11 +547,12 @@ void KindField::renumberDepthFirst(uint32_t nodesBefore, uint32_t regionsBefore)
 // Pipelined over consecutive updates: the spatial phase (ingest, kd update
 // with its host decisions, leaf keys, sort, gather + reprojection) runs on
 // the field's spatial stream and writes one of two region-ordered sample
-// buffers; the directional phase (state copies and inputs, the region kernel,
-// KNN) is queued on the field's stream `s` behind an event of the gather.  The
-// call returns once the directional phase is queued; the previous update's
-// directional phase is finished (its statistics collected) just before this
-// one is queued, and every read of device state finishes the pending update.
+// buffers; the directional phase (state copies and inputs, the region kernel)
+// is queued on the field's stream `s` behind an event of the gather, and
+// behind the previous update's directional phase, which may still be running
+// then.  After queueing it the call finishes the previous update (waits for
+// it, collects its statistics) and returns; every read of device state
+// finishes the pending update.
 // Buffers whose size follows the number of regions and nodes are sized for
 // at least kReserveRegions regions (and twice as many nodes) at once, and for
 // twice the current count beyond that.  A buffer that grows is freed, and
@@ -553,7 +565,6 @@ void KindField::reserve(uint32_t n, uint32_t regions) {
   const uint32_t Rc = std::max<uint32_t>(kReserveRegions, 2 * regions);
   const size_t nodes = 2 * size_t(Rc);
   const size_t acc = 2 * size_t(Rc) * sizeof(IntStatsAcc);
-  dAcc_.ensure(acc);
   kdDown_.ensure(acc);
   const size_t up = nodes * sizeof(KDNode) + nodes * sizeof(int32_t) + 6 * size_t(Rc) * 4 + 4 * 16 +
                     2 * size_t(Rc) * sizeof(FrontierNode);
@@ -717,11 +728,38 @@ void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
                           fc_.noParallax == 0, sp);
   PGL_HIP_CHECK(hipEventRecord(evSp_[P][3], sp));
   lastGatherParity_ = P;
+  // the directional launch's host work (jobs, teams, pinned tables) while the
+  // previous update's region kernel still runs
+  RegionRunner& rr = runner();
+  rr.stage(hWork_, RegionSamplesView{soa[0], soa[1], soa[2], soa[3], soa[4], soa[5]}, fc_,
+           dUniformDirs_.as<float>(), nullptr, hInputs_.data(), R);
+
+  // ---------------------------------------------------------- directional phase
+  // queued on s behind the previous update's directional phase (which may
+  // still run: the stream starts this launch when that one ends, without a
+  // round trip through the host); waits for this update's gather
+  if (first) {
+    ensureRegionCapacity(1, s);
+    PGL_HIP_CHECK(hipMemsetAsync(dStates_.p, 0, sizeof(RegionState), s));
+  }
+  ensureRegionCapacity(R, s);
+  PGL_HIP_CHECK(hipStreamWaitEvent(s, evSp_[P][3], 0));
+  // the new regions' state copies, the inputs and the launch (one upload, one preparing kernel)
+  rr.submit(dStates_.as<RegionState>(), s);
+  const int slot = rr.lastSlot();
+  inf.regionLaunches = rr.lastLaunches;
+  inf.regionJobs = rr.lastJobs;
+  inf.regionItems = rr.lastItems;
+  inf.maxTeam = rr.lastMaxTeam;
+  inf.regionGrid = rr.lastGrid;
+  PGL_HIP_CHECK(hipEventRecord(evDir_[P], s));
+
   // KNN neighbourhoods of the region pivots (sample-statistics means, uploaded
   // with the spatial tables) and the tree in its final numbering for the query
   // kernels: they need only this update's spatial results, so they run on the
-  // spatial stream, overlapping the previous update's region kernel; every
-  // reader of either first finishes the pending update, which waits for them
+  // spatial stream beside the region kernels, queued after the directional
+  // launch; every reader of either first finishes the pending update, which
+  // waits for them
   PGL_HIP_CHECK(hipEventRecord(evKnn_[P][0], sp));
   if (kd_.knnLookup) {
     dNbIds_.ensure(size_t(R) * 8 * 4);
@@ -733,34 +771,17 @@ void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
                                nodes_.size() * sizeof(KDNode), hipMemcpyHostToDevice, sp));
   PGL_HIP_CHECK(hipEventRecord(evKnn_[P][1], sp));
 
-  // the directional launch's host work (jobs, teams, pinned tables) while the
-  // previous update's region kernel still runs
-  RegionRunner& rr = runner();
-  rr.stage(hWork_, RegionSamplesView{soa[0], soa[1], soa[2], soa[3], soa[4], soa[5]}, fc_,
-           dUniformDirs_.as<float>(), nullptr, hInputs_.data(), R);
 
   // ------------------------------------------------ finish the previous update
-  finish();
-
-  // ---------------------------------------------------------- directional phase
-  // (queued on s; waits for this update's gather)
-  if (first) {
-    ensureRegionCapacity(1, s);
-    PGL_HIP_CHECK(hipMemsetAsync(dStates_.p, 0, sizeof(RegionState), s));
+  try {
+    finish();
+  } catch (...) {
+    // the previous update failed: the field is invalid until reset(); this
+    // update's queued work still runs to its end before anything is released
+    (void)hipEventSynchronize(evDir_[P]);
+    (void)hipEventSynchronize(evKnn_[P][1]);
+    throw;
   }
-  ensureRegionCapacity(R, s);
-  PGL_HIP_CHECK(hipEventRecord(evDir_[P][0], s));
-  PGL_HIP_CHECK(hipStreamWaitEvent(s, evSp_[P][3], 0));
-  // the new regions' state copies, the inputs and the launch (one upload, one preparing kernel)
-  rr.submit(dStates_.as<RegionState>(), s);
-  inf.regionLaunches = rr.lastLaunches;
-  inf.regionJobs = rr.lastJobs;
-  inf.regionItems = rr.lastItems;
-  inf.maxTeam = rr.lastMaxTeam;
-  inf.regionGrid = rr.lastGrid;
-  PGL_HIP_CHECK(hipEventRecord(evDir_[P][1], s));
-  PGL_HIP_CHECK(hipStreamWaitEvent(s, evKnn_[P][1], 0));
-  PGL_HIP_CHECK(hipEventRecord(evDir_[P][2], s));
 
   for (auto& r : regions_) { r.splitFlag = false; r.copyFrom = -1; }
   initialized_ = true;
@@ -772,6 +793,7 @@ void KindField::update(const void* deviceSamples, uint32_t n, hipStream_t s) {
   pendingRegions_ = R;
   pendingT0_ = t0;
   pendingParity_ = P;
+  pendingSlot_ = slot;
   pending_ = true;
   parity_ = 1 - P;
 }
@@ -796,22 +818,25 @@ void KindField::drain() const noexcept {
 void KindField::finishPending() const {
   if (!pending_) return;
   const int P = pendingParity_;
-  PGL_HIP_CHECK(hipEventSynchronize(evDir_[P][2]));
+  PGL_HIP_CHECK(hipEventSynchronize(evDir_[P]));
+  PGL_HIP_CHECK(hipEventSynchronize(evKnn_[P][1]));
   pending_ = false;
   RegionRunner& rr = *runner_;
   hintRegions_ = pendingRegions_;
   UpdateInfo& i = pendingInfo_;
-  i.estepEvals = (double)rr.stats().estepEvals;
-  i.auxEvals = (double)rr.stats().distEvals;
-  i.splitEvals = (double)rr.stats().splitEvals;
-  i.emIterations = (double)rr.stats().emIterations;
+  const RegionLaunchStats& ls = rr.stats(pendingSlot_);
+  i.estepEvals = (double)ls.estepEvals;
+  i.auxEvals = (double)ls.distEvals;
+  i.splitEvals = (double)ls.splitEvals;
+  i.emIterations = (double)ls.emIterations;
   float ms = 0.f;
   PGL_HIP_CHECK(hipEventElapsedTime(&ms, evSp_[P][0], evSp_[P][1])); i.spatialMs = ms;
   PGL_HIP_CHECK(hipEventElapsedTime(&ms, evSp_[P][1], evSp_[P][2])); i.sortMs = ms;
   PGL_HIP_CHECK(hipEventElapsedTime(&ms, evSp_[P][2], evSp_[P][3])); i.gatherMs = ms;
-  PGL_HIP_CHECK(hipEventElapsedTime(&ms, evDir_[P][0], evDir_[P][1])); i.directionalMs = ms;
+
   PGL_HIP_CHECK(hipEventElapsedTime(&ms, evKnn_[P][0], evKnn_[P][1])); i.knnMs = ms;
-  i.regionKernelMs = rr.kernelMs();
+  i.regionKernelMs = rr.kernelMs(pendingSlot_);
+  i.directionalMs = i.regionKernelMs;
   i.regions = pendingRegions_;
   i.wallMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - pendingT0_).count();
   info_ = i;
@@ -822,7 +847,7 @@ void KindField::finishPending() const {
   history_.push_back(i);
   if (history_.size() > 256) history_.erase(history_.begin());
   try {
-    rr.check();
+    rr.check(pendingSlot_);
   } catch (const std::exception& e) {
     failed_ = true;
     failMsg_ = std::string("the last update failed (") + e.what() + ")";
@@ -892,13 +917,15 @@ void PinnedBuf::ensure(size_t b) {
   if (b <= bytes) return;
   const size_t nb = std::max(std::max(b, bytes * 2), size_t(65536));
   void* np = nullptr;
-  PGL_HIP_CHECK(hipHostMalloc(&np, nb, hipHostMallocDefault));
+  PGL_HIP_CHECK(hipHostMalloc(&np, nb, coherent ? hipHostMallocCoherent : hipHostMallocDefault));
   if (p) {
     std::memcpy(np, p, bytes);
     (void)hipHostFree(p);
   }
   p = np;
   bytes = nb;
+  dev = nullptr;
+  if (coherent) PGL_HIP_CHECK(hipHostGetDevicePointer(&dev, p, 0));
 }
 
 PinnedBuf::~PinnedBuf() {

@@ -76,10 +76,14 @@ struct DevBuf {
   template <class T> T* as() const { return (T*)p; }
 };
 
-// Growable pinned host allocation (hipHostMalloc).
+// Growable pinned host allocation (hipHostMalloc).  A coherent one is not
+// cached by the GPU: kernels write their results into it directly (through
+// `dev`, its device address), visible to the host once the kernel completes.
 struct PinnedBuf {
   void* p = nullptr;
+  void* dev = nullptr;
   size_t bytes = 0;
+  bool coherent = false;
   void ensure(size_t b);
   ~PinnedBuf();
   template <class T> T* as() const { return (T*)p; }
@@ -202,7 +206,9 @@ class KindField {
   hipStream_t sp_ = nullptr;  // spatial phase
   int parity_ = 0;
   int lastGatherParity_ = -1;
-  hipEvent_t evSp_[2][4], evDir_[2][3];
+  hipEvent_t evSp_[2][4];
+  hipEvent_t evDir_[2];      // end of the directional phase (region kernel and totals copy) on s
+  hipEvent_t evKd_ = nullptr;  // the kd rounds' host waits
   hipEvent_t evKnn_[2][2];  // around the KNN build and the tree upload (spatial stream)
   // pinned staging of the per-update uploads, by frame parity: [0] inputs,
   // [1] pivots, [2] copy pairs, [3] KNN pivots, [4] key nodes, [5] nodes,
@@ -228,9 +234,10 @@ class KindField {
   mutable UpdateInfo pendingInfo_;
   mutable uint32_t pendingRegions_ = 0;
   mutable int pendingParity_ = 0;
+  mutable int pendingSlot_ = 0;  // the region runner's result slot of the pending update
   std::chrono::steady_clock::time_point pendingT0_;
   mutable std::vector<UpdateInfo> history_;
-  DevBuf dNodes_, dKdUp_, dAcc_, dPartial_, dStatPartial_;  // dKdUp_: the kd rounds' uploads
+  DevBuf dNodes_, dKdUp_, dPartial_, dStatPartial_;  // dKdUp_: the kd rounds' uploads
   std::unique_ptr<RegionRunner> runner_;
   mutable uint32_t hintRegions_ = 0;
   DevBuf dStates_, dNbIds_, dNbPts_, dUniformDirs_;

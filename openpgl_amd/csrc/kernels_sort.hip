@@ -289,7 +289,7 @@ void counting_sort_leaf_records(void* temp, const uint32_t* nodeOf, const KDNode
 // Each region's index range is sorted on the device without host round
 // trips, by kernels small enough to share the CUs with the region kernel of
 // the previous update (the spatial phase of an update overlaps it):
-//   * tiles of up to 2048 samples (1024 for short ranges) in one workgroup: a
+//   * tiles of up to 2048 samples in one workgroup: a
 //     stable LDS radix sort by the weight's order key over the bits in which
 //     the tile's keys differ; runs of equal weights (rare for continuous
 //     weights) are put in order with the full comparator by one thread each,
@@ -334,7 +334,8 @@ template <int THREADS, int ITEMS>
 __global__ void __launch_bounds__(THREADS) k_det_tile_sort(const float* __restrict__ recs,
                                                          const DetSegment* __restrict__ segs,
                                                          uint32_t* __restrict__ vals, const uint32_t* __restrict__ keysIn,
-                                                         uint32_t* __restrict__ keysOut) {
+                                                         uint32_t* __restrict__ keysOut, uint32_t* __restrict__ valsAlt,
+                                                         uint32_t* __restrict__ keysAlt) {
   typedef hipcub::BlockRadixSort<uint32_t, THREADS, ITEMS, uint32_t> Sort;
   constexpr int CAP = THREADS * ITEMS;
   __shared__ union {
@@ -344,7 +345,11 @@ __global__ void __launch_bounds__(THREADS) k_det_tile_sort(const float* __restri
   __shared__ uint32_t sKey[CAP];
   __shared__ int longRun;
   __shared__ uint32_t keyAnd, keyOr;
-  const DetSegment sg = segs[blockIdx.x];
+  const DetSegment sg0 = segs[blockIdx.x];
+  // bit 31 of the count: the sorted tile goes to valsAlt / keysAlt (the first
+  // merge pass reads it there)
+  const DetSegment sg{sg0.begin, sg0.count & 0x7fffffffu};
+  const bool alt = (sg0.count >> 31) != 0;
   const uint32_t t = threadIdx.x;
   uint32_t key[ITEMS], idx[ITEMS];
 #pragma unroll
@@ -430,9 +435,11 @@ __global__ void __launch_bounds__(THREADS) k_det_tile_sort(const float* __restri
     for (int j = 0; j < ITEMS; j++) sh.idx[t * ITEMS + j] = idx[j];
     __syncthreads();
   }
+  uint32_t* __restrict__ valsDst = alt ? valsAlt : vals;
+  uint32_t* __restrict__ keysDst = alt ? keysAlt : keysOut;
   for (uint32_t p = t; p < sg.count; p += THREADS) {
-    vals[sg.begin + p] = sh.idx[p];
-    keysOut[sg.begin + p] = sKey[p];
+    valsDst[sg.begin + p] = sh.idx[p];
+    keysDst[sg.begin + p] = sKey[p];
   }
 }
 
@@ -575,18 +582,11 @@ __global__ void __launch_bounds__(256) k_det_merge(const float* __restrict__ rec
   }
 }
 
-__global__ void __launch_bounds__(256) k_det_copy(const DetSegment* __restrict__ segs, const uint32_t* __restrict__ in,
-                                                  uint32_t* __restrict__ out) {
-  const DetSegment sg = segs[blockIdx.y];
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < sg.count; i += stride)
-    out[sg.begin + i] = in[sg.begin + i];
-}
 
 }  // namespace
 
-constexpr uint32_t kDetSmall = 256 * 4, kDetTile = 256 * 8;
-constexpr int kDetTables = 3;
+constexpr uint32_t kDetTile = 256 * 8;
+constexpr int kDetTables = 2;
 
 static size_t det_table_entries(uint32_t n, uint32_t numSegments) { return (size_t)numSegments + n / kDetTile + 3; }
 static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
@@ -605,26 +605,30 @@ void det_sort_segments(void* temp, void* pinnedHost, size_t pinnedBytes, const v
                        uint32_t* vals, uint32_t* weightKeys, hipStream_t s) {
   if (n == 0 || numSegments == 0) return;
   const float* recs = (const float*)records;
-  // tables: 0 ranges <= 1024, 1 tiles of <= 4096 (ranges up to 4096 and the
-  // tiles of longer ranges), 2 ranges longer than one tile
+  // tables: 0 tiles of at most kDetTile (every range up to that and the tiles
+  // of longer ranges), 1 ranges longer than one tile
   const size_t cap = det_table_entries(n, numSegments);
   if (pinnedBytes < det_sort_stage_bytes(n, numSegments))
     throw std::runtime_error("det_sort_segments: staging too small");
-  DetSegment* host = (DetSegment*)pinnedHost;
-  DetSegment* cls[kDetTables] = {host, host + cap, host + 2 * cap};
-  uint32_t num[kDetTables] = {0, 0, 0};
   uint32_t longest = 0;
+  for (uint32_t r = 0; r < numSegments; r++) longest = std::max(longest, count[r]);
+  // merge passes: the sorted tiles of long ranges start in the buffer that
+  // makes the last pass end in vals
+  uint32_t passes = 0;
+  for (uint32_t w = kDetTile; w < longest; w *= 2) passes++;
+  const uint32_t altFlag = (passes & 1) ? 0x80000000u : 0u;
+  DetSegment* host = (DetSegment*)pinnedHost;
+  DetSegment* cls[kDetTables] = {host, host + cap};
+  uint32_t num[kDetTables] = {0, 0};
   for (uint32_t r = 0; r < numSegments; r++) {
     const uint32_t c = count[r];
     if (c < 2) continue;
-    if (c <= kDetSmall) {
+    if (c <= kDetTile) {
       cls[0][num[0]++] = DetSegment{begin[r], c};
     } else {
-      for (uint32_t o = 0; o < c; o += kDetTile) cls[1][num[1]++] = DetSegment{begin[r] + o, std::min(kDetTile, c - o)};
-      if (c > kDetTile) {
-        cls[2][num[2]++] = DetSegment{begin[r], c};
-        longest = std::max(longest, c);
-      }
+      for (uint32_t o = 0; o < c; o += kDetTile)
+        cls[0][num[0]++] = DetSegment{begin[r] + o, std::min(kDetTile, c - o) | altFlag};
+      cls[1][num[1]++] = DetSegment{begin[r], c};
     }
   }
   char* t = (char*)temp;
@@ -635,25 +639,27 @@ void det_sort_segments(void* temp, void* pinnedHost, size_t pinnedBytes, const v
   uint32_t* idxB = (uint32_t*)t;
   t += align256(size_t(n) * 4);
   uint32_t* keysB = (uint32_t*)t;
-  PGL_KERNEL_CHECK(hipMemcpyAsync(dev, host, det_sort_stage_bytes(n, numSegments), hipMemcpyHostToDevice, s));
+  // only the filled parts of the two tables
+  const size_t used = num[1] ? cap + num[1] : num[0];
+  PGL_KERNEL_CHECK(hipMemcpyAsync(dev, host, used * sizeof(DetSegment), hipMemcpyHostToDevice, s));
   const uint32_t* keysIn = weightKeys;  // in place: a tile reads its keys before writing them
   if (num[0])
-    hipLaunchKernelGGL((k_det_tile_sort<256, 4>), dim3(num[0]), dim3(256), 0, s, recs, dev, vals, keysIn, keysA);
-  if (num[1])
-    hipLaunchKernelGGL((k_det_tile_sort<256, 8>), dim3(num[1]), dim3(256), 0, s, recs, dev + cap, vals, keysIn, keysA);
-  if (num[2] == 0) return;
+    hipLaunchKernelGGL((k_det_tile_sort<256, 8>), dim3(num[0]), dim3(256), 0, s, recs, dev, vals, keysIn, keysA, idxB,
+                       keysB);
+  if (num[1] == 0) return;
   // merge passes over all ranges longer than one tile, ping-pong between the buffers
   static_assert((2 * kDetTile) % kMergeTile == 0, "a merge workgroup's outputs lie in one pair of runs");
-  const dim3 grid((longest + kMergeTile - 1) / kMergeTile, num[2]);
+  const dim3 grid((longest + kMergeTile - 1) / kMergeTile, num[1]);
   uint32_t *ia = vals, *ka = keysA, *ib = idxB, *kb = keysB;
-  for (uint32_t w = kDetTile; w < longest; w *= 2) {
-    hipLaunchKernelGGL(k_det_merge, grid, dim3(256), 0, s, recs, dev + 2 * cap, w, ia, ka, ib, kb);
+  if (altFlag) {
     std::swap(ia, ib);
     std::swap(ka, kb);
   }
-  if (ia != vals)
-    hipLaunchKernelGGL(k_det_copy, dim3(std::min<uint32_t>((longest + 255) / 256, 64), num[2]), dim3(256), 0, s,
-                       dev + 2 * cap, ia, vals);
+  for (uint32_t w = kDetTile; w < longest; w *= 2) {
+    hipLaunchKernelGGL(k_det_merge, grid, dim3(256), 0, s, recs, dev + cap, w, ia, ka, ib, kb);
+    std::swap(ia, ib);
+    std::swap(ka, kb);
+  }
 }
 
 }  // namespace pglgpu

@@ -49,15 +49,15 @@ RegionRunner::RegionRunner() {
   defaultGrid_ = (uint32_t)(std::min(perCU, 2) * cus);
   cooperative_ = coop != 0;
   PGL_SCHED_CHECK(hipMalloc((void**)&dTot_, sizeof(LaunchTotals)));
-  PGL_SCHED_CHECK(hipHostMalloc((void**)&hTot_, sizeof(LaunchTotals), hipHostMallocDefault));
+  PGL_SCHED_CHECK(hipHostMalloc((void**)&hTot_, 2 * sizeof(LaunchTotals), hipHostMallocDefault));
   dErr_ = &dTot_->err;
-  hErr_ = &hTot_->err;
   dStats_ = &dTot_->stats;
-  hStats_ = &hTot_->stats;
-  *hErr_ = 0;
-  *hStats_ = RegionLaunchStats{0, 0, 0, 0};
-  PGL_SCHED_CHECK(hipEventCreate(&evBegin_));
-  PGL_SCHED_CHECK(hipEventCreate(&evEnd_));
+  for (int k = 0; k < 2; k++) {
+    hTot_[k].err = 0;
+    hTot_[k].stats = RegionLaunchStats{0, 0, 0, 0};
+    PGL_SCHED_CHECK(hipEventCreate(&evBegin_[k]));
+    PGL_SCHED_CHECK(hipEventCreate(&evEnd_[k]));
+  }
   PGL_SCHED_CHECK(hipEventCreateWithFlags(&evPrep_, hipEventDisableTiming));
 }
 
@@ -68,8 +68,10 @@ RegionRunner::~RegionRunner() {
   (void)hipFree(dPartial_);
   (void)hipFree(dTot_);
   (void)hipHostFree(hTot_);
-  if (evBegin_) (void)hipEventDestroy(evBegin_);
-  if (evEnd_) (void)hipEventDestroy(evEnd_);
+  for (int k = 0; k < 2; k++) {
+    if (evBegin_[k]) (void)hipEventDestroy(evBegin_[k]);
+    if (evEnd_[k]) (void)hipEventDestroy(evEnd_[k]);
+  }
   if (evPrep_) (void)hipEventDestroy(evPrep_);
 }
 
@@ -98,7 +100,7 @@ void RegionRunner::stage(std::vector<RegionWork> work, const RegionSamplesView& 
     st.empty = true;
     if (inputs && numInputs) {  // the states still take their inputs
       if (submitted_ && size_t(numInputs) * sizeof(RegionInput) > hStageBytes_)
-        PGL_SCHED_CHECK(hipEventSynchronize(evEnd_));
+        PGL_SCHED_CHECK(hipEventSynchronize(evEnd_[cur_]));
       grow_pinned(hStage_, hStageBytes_, size_t(numInputs) * sizeof(RegionInput));
       memcpy(hStage_, inputs, size_t(numInputs) * sizeof(RegionInput));
     }
@@ -151,12 +153,13 @@ void RegionRunner::stage(std::vector<RegionWork> work, const RegionSamplesView& 
   const size_t inB = inputs ? size_t(numInputs) * sizeof(RegionInput) : 0;
   const size_t offItems = (jobsB + 15) & ~size_t(15), offIn = (offItems + itemsB + 15) & ~size_t(15);
   const size_t stageB = offIn + inB;
-  // a buffer that grows is freed: first wait until the previous launch,
-  // which may still be running and reading it, has completed
+  // a buffer that grows is freed: first wait until the last launch (and with
+  // it every earlier one on its stream), which may still be queued or
+  // running and reading it, has completed
   const size_t partB = std::max<unsigned long long>(partialFloats, 1) * 4;
   if (submitted_ && (offIn > stageBytes_ || size_t(J + 1) * 4 > queueBytes_ || partB > partialBytes_ ||
                      stageB > hStageBytes_))
-    PGL_SCHED_CHECK(hipEventSynchronize(evEnd_));
+    PGL_SCHED_CHECK(hipEventSynchronize(evEnd_[cur_]));
   grow(dStage_, stageBytes_, offIn);
   grow(dQueue_, queueBytes_, (size_t)(J + 1) * 4);
   grow(dPartial_, partialBytes_, partB);
@@ -180,9 +183,12 @@ void RegionRunner::submit(RegionState* states, hipStream_t s) {
   st.valid = false;
   lastLaunches = lastJobs = lastItems = lastGrid = 0;
   lastMaxTeam = stagedMaxTeam_;
-  *hErr_ = 0;
-  *hStats_ = RegionLaunchStats{0, 0, 0, 0};
-  timed_ = false;
+  // the result slot of this launch (the other one holds the previous launch's)
+  const int k = 1 - cur_;
+  cur_ = k;
+  hTot_[k].err = 0;
+  hTot_[k].stats = RegionLaunchStats{0, 0, 0, 0};
+  timed_[k] = false;
   if (hStage_ != hStageMapped_) {
     PGL_SCHED_CHECK(hipHostGetDevicePointer(&hStageDev_, hStage_, 0));
     hStageMapped_ = hStage_;
@@ -195,6 +201,9 @@ void RegionRunner::submit(RegionState* states, hipStream_t s) {
       PGL_SCHED_CHECK(hipEventRecord(evPrep_, s));
       prepQueued_ = true;
     }
+    // marks the end of everything queued so far (the buffers' last use)
+    PGL_SCHED_CHECK(hipEventRecord(evEnd_[k], s));
+    submitted_ = true;
     return;
   }
   const uint32_t J = st.J;
@@ -222,7 +231,7 @@ void RegionRunner::submit(RegionState* states, hipStream_t s) {
   L.uniformDirs = st.uniformDirs;
   L.cooperative = cooperative_ && schedule.cooperative != 0;
   L.zeroQueue = st.perRegion;
-  PGL_SCHED_CHECK(hipEventRecord(evBegin_, s));
+  PGL_SCHED_CHECK(hipEventRecord(evBegin_[k], s));
   if (st.perRegionTeams) {
     // one launch per job: its consecutive items, the queue and counters zeroed before each
     L.cooperative = false;
@@ -255,28 +264,28 @@ void RegionRunner::submit(RegionState* states, hipStream_t s) {
     lastLaunches = 1;
     lastGrid = L.grid;
   }
-  PGL_SCHED_CHECK(hipEventRecord(evEnd_, s));
+  PGL_SCHED_CHECK(hipEventRecord(evEnd_[k], s));
   submitted_ = true;
-  timed_ = true;
+  timed_[k] = true;
   lastJobs = J;
   lastItems = (uint32_t)items_.size();
   // the error flag and the totals are adjacent: one copy back
-  PGL_SCHED_CHECK(hipMemcpyAsync(hTot_, dTot_, sizeof(LaunchTotals), hipMemcpyDeviceToHost, s));
+  PGL_SCHED_CHECK(hipMemcpyAsync(&hTot_[k], dTot_, sizeof(LaunchTotals), hipMemcpyDeviceToHost, s));
 }
 
-float RegionRunner::kernelMs() const {
-  if (!timed_) return 0.f;
+float RegionRunner::kernelMs(int slot) const {
+  if (!timed_[slot]) return 0.f;
   float ms = 0.f;
-  PGL_SCHED_CHECK(hipEventElapsedTime(&ms, evBegin_, evEnd_));
+  PGL_SCHED_CHECK(hipEventElapsedTime(&ms, evBegin_[slot], evEnd_[slot]));
   return ms;
 }
 
-void RegionRunner::check() const {
+void RegionRunner::check(int slot) const {
   if (failNext) {
     failNext = false;
     throw std::runtime_error("directional update: failure injected for testing");
   }
-  if (*hErr_ != 0)
+  if (hTot_[slot].err != 0)
     throw std::runtime_error("directional update: a region team barrier timed out (workgroups not co-resident)");
 }
 

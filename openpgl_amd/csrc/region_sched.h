@@ -65,21 +65,27 @@ class RegionRunner {
   // launch() in two steps: stage() does the host work (jobs, teams, the
   // pinned tables; it waits only for the previous launch's preparing kernel,
   // which reads them), so it may run while the previous launch still runs;
-  // submit() queues the preparing kernel and the launch on s once the
-  // previous launch has completed.  The figures of the last completed
-  // launch (check, stats, kernelMs) stay valid until submit().
+  // submit() queues the preparing kernel and the launch on s, behind the
+  // previous launch if that is still queued or running.  Each launch keeps
+  // its figures (check, stats, kernelMs) in one of two result slots:
+  // lastSlot() after submit() names it, and they stay valid until the
+  // second submit() after it.
   void stage(std::vector<RegionWork> work, const RegionSamplesView& rs, const FactoryConfig& cfg,
              const float* uniformDirs, float* costHint, const RegionInput* inputs = nullptr,
              uint32_t numInputs = 0);
   void submit(RegionState* states, hipStream_t s);
-  // after the stream has completed: throws if a team barrier timed out
-  void check() const;
+  int lastSlot() const { return cur_; }
+  // after the launch has completed: throws if a team barrier timed out
+  void check(int slot) const;
+  void check() const { check(cur_); }
   // testing: the next check() reports a failed launch (the kernels run normally)
   mutable bool failNext = false;
-  const RegionLaunchStats& stats() const { return *hStats_; }
-  // GPU time of the region kernel(s) of the last launch (HIP events around
-  // the kernel launches only), valid after the stream has completed
-  float kernelMs() const;
+  const RegionLaunchStats& stats(int slot) const { return hTot_[slot].stats; }
+  const RegionLaunchStats& stats() const { return stats(cur_); }
+  // GPU time of the region kernel(s) of a launch (HIP events around the
+  // kernel launches only), valid after the launch has completed
+  float kernelMs(int slot) const;
+  float kernelMs() const { return kernelMs(cur_); }
 
   RegionSchedule schedule;
   uint32_t residentGrid() const { return resident_; }
@@ -97,22 +103,23 @@ class RegionRunner {
   size_t queueBytes_ = 0;
   void* dPartial_ = nullptr;
   size_t partialBytes_ = 0;
-  // the error flag and the totals of a launch, adjacent (one copy back)
+  // the error flag and the totals of a launch, adjacent (one copy back into
+  // the launch's result slot)
   struct LaunchTotals {
     int err;
     int pad;
     RegionLaunchStats stats;
   };
-  LaunchTotals* dTot_ = nullptr;
-  LaunchTotals* hTot_ = nullptr;
+  LaunchTotals* dTot_ = nullptr;  // zeroed by each launch's preparing kernel
+  LaunchTotals* hTot_ = nullptr;  // [2] result slots (pinned)
   int* dErr_ = nullptr;
-  int* hErr_ = nullptr;
   RegionLaunchStats* dStats_ = nullptr;
-  RegionLaunchStats* hStats_ = nullptr;
-  hipEvent_t evBegin_ = nullptr, evEnd_ = nullptr;
+  hipEvent_t evBegin_[2] = {nullptr, nullptr}, evEnd_[2] = {nullptr, nullptr};
+  bool timed_[2] = {false, false};
+  int cur_ = 1;  // slot of the last submit()
   hipEvent_t evPrep_ = nullptr;  // after the last preparing kernel (which reads the pinned tables)
   bool prepQueued_ = false;
-  bool submitted_ = false;  // evEnd_ has been recorded
+  bool submitted_ = false;  // evEnd_[cur_] has been recorded
   // the staged launch: what submit() needs besides the tables
   struct Staged {
     bool valid = false, empty = false;
@@ -127,7 +134,6 @@ class RegionRunner {
   uint32_t stagedMaxTeam_ = 0;
   void* hStageMapped_ = nullptr;  // hStage_ whose device address is hStageDev_
   void* hStageDev_ = nullptr;
-  bool timed_ = false;
   std::vector<RegionJob> jobs_;
   std::vector<uint2> items_;
   // pinned staging of the upload (reused once the previous launch has completed)
