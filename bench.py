@@ -256,12 +256,13 @@ def _device_mem_used():
     return total.value - free.value
 
 
-def mixed_workload(frames=3, surf=16_000_000, vol=4_000_000, offered_qps=1.0e8, server_budget=1 << 30):
+def mixed_workload(frames=9, rounds=1, surf=16_000_000, vol=4_000_000, offered_qps=1.0e8, server_budget=1 << 30):
     """Large mixed workload: 16M surface + 4M volume records per frame (64 blobs),
-    frame 0 builds the field; frames 1.. are timed (a) alone and (b) while the
-    query server runs an open Poisson load of offered_qps on the previous
-    frame's field (refreshed after every update).  The region kernel keeps one
-    workgroup per CU in (b) so that query waves fit beside it."""
+    frame 0 builds the field; frames 1.. (`rounds` times over, so that the
+    serving window holds several hundred requests) are timed (a) alone and
+    (b) while the query server runs an open Poisson load of offered_qps on the
+    previous update's field (refreshed after every update).  The region kernel
+    keeps one workgroup per CU in (b) so that query waves fit beside it."""
     import threading
     import openpgl_amd as pgl
     from openpgl_amd.hip_runtime import DeviceArray, pinned_copy, pinned_empty, synchronize
@@ -278,17 +279,18 @@ def mixed_workload(frames=3, surf=16_000_000, vol=4_000_000, offered_qps=1.0e8, 
     del fs, fv
     synchronize()
     mem_inputs = _device_mem_used() - mem0
-    out = {"workload": "%d surface + %d volume records/frame, 64 blobs, %d frames (frame 0 builds)"
-                       % (surf, vol, frames)}
+    out = {"workload": "%d surface + %d volume records/frame, 64 blobs, %d frames (frame 0 builds; frames 1-%d "
+                       "timed%s)" % (surf, vol, frames, frames - 1, "" if rounds == 1 else " %d times over" % rounds)}
     # (a) alone
     fa = pgl.Field(dev, max_samples_per_leaf=32000, deterministic=True)
     fa.update_surface_device(ds[0].value, surf)
     fa.update_volume_device(dv[0].value, vol)
     synchronize()
     t = time.perf_counter()
-    for f in range(1, frames):
-        fa.update_surface_device(ds[f].value, surf)
-        fa.update_volume_device(dv[f].value, vol)
+    for _ in range(rounds):
+        for f in range(1, frames):
+            fa.update_surface_device(ds[f].value, surf)
+            fa.update_volume_device(dv[f].value, vol)
     synchronize()
     alone_s = time.perf_counter() - t
     mem_field = _device_mem_used() - mem0 - mem_inputs
@@ -346,10 +348,11 @@ def mixed_workload(frames=3, surf=16_000_000, vol=4_000_000, offered_qps=1.0e8, 
     th.start()
     time.sleep(0.05)
     t = time.perf_counter()
-    for f in range(1, frames):
-        fb.update_surface_device(ds[f].value, surf)
-        fb.update_volume_device(dv[f].value, vol)
-        server.refresh()
+    for _ in range(rounds):
+        for f in range(1, frames):
+            fb.update_surface_device(ds[f].value, surf)
+            fb.update_volume_device(dv[f].value, vol)
+            server.refresh()
     synchronize()
     mixed_s = time.perf_counter() - t
     stop.set()
@@ -360,12 +363,15 @@ def mixed_workload(frames=3, surf=16_000_000, vol=4_000_000, offered_qps=1.0e8, 
     lat = tm[:, 2] - tm[:, 0] + np.array([lg for _, _, lg in ids])  # from the scheduled arrival
     nq = sum(n for _, n, _ in ids)
     span = (tm[:, 2].max() - tm[:, 0].min()) * 1e-3
+    # the load the Poisson generator actually produced (queries over the submission window)
+    offered_span = (tm[:, 0].max() - tm[:, 0].min()) * 1e-3
     server.close()
-    samples = (frames - 1) * (surf + vol)
+    samples = rounds * (frames - 1) * (surf + vol)
     out.update({"regions_surface_volume": regions,
                 "update_alone_samples_per_s": samples / alone_s,
                 "update_while_serving_samples_per_s": samples / mixed_s,
                 "serving_during_updates": {"offered_queries_per_s": offered_qps, "requests": len(ids),
+                                           "generated_queries_per_s": nq / offered_span,
                                            "achieved_guided_samples_per_s": nq / span,
                                            "p50_ms": float(np.percentile(lat, 50)),
                                            "p99_ms": float(np.percentile(lat, 99)), "max_ms": float(lat.max()),
