@@ -73,6 +73,27 @@ def test_kdtree_bit_exact_large_tree():
     assert np.array_equal(gpu.leaf_ids(s["pos"]), orc.leaf_ids(s["pos"]))
 
 
+def test_kdtree_bit_exact_beyond_lds_tables():
+    """More than 4096 nodes: the push-down rounds read the node -> frontier
+    table from global memory instead of LDS (and the descending pass walks the
+    tree from global memory, the statistics run in several slot tiles); region
+    statistics and leaf ids bit-exact over two frames."""
+    dev, gpu, orc = _fields(64, 250000, frames=2, seed=5, knn=False)
+    gp, _, _ = gpu.nodes()
+    print("nodes", len(gp), flush=True)
+    assert len(gp) > 4096
+    _tree_equal(gpu, orc)
+    R = gpu.num_regions()
+    assert R == orc.num_regions()
+    for i in range(0, R, 7):
+        a, b = gpu.region(i), orc.region(i)
+        for key in ["ssMean", "ssVariance", "ssLower", "ssUpper", "regionLower", "regionUpper"]:
+            assert np.array_equal(a[key].view(np.uint32), b[key].view(np.uint32)), (i, key, a[key], b[key])
+        assert a["numSamplesThisUpdate"] == b["numSamplesThisUpdate"]
+    s = generate(5000, seed=13, frame=2)
+    assert np.array_equal(gpu.leaf_ids(s["pos"]), orc.leaf_ids(s["pos"]))
+
+
 @pytest.mark.parametrize("frames", [1, 3])
 def test_mixture_parity(frames):
     import openpgl_amd as pgl
