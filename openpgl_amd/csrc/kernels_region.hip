@@ -1378,13 +1378,14 @@ __device__ void merge_step_split(Shared& sh, SplitStatsState& SS, int ci, int cj
 // pairs of the merged slot ci and of slot cj (which receives the last
 // component) change: every other pair has unchanged inputs, so the table
 // equals a full recomputation bit for bit.  The merge loop is sequential by
-// definition.  A step: the arg-min over the table on wave 0 (least cost below
-// the threshold with both components holding split samples, ties to the first
-// pair in (i, j) order -- the serial scan of ThresholdedMergeNext), which
-// also stashes the pair's values; the merge on waves 0 and 1 at once; the
-// three cost terms of the changed pairs on waves 0..2, one pair per lane; the
-// new costs written by wave 0 before its next arg-min.  Each component's
-// product with itself is cached and moves with the component.
+// definition.  A step: wave 0 takes the arg-min over the pairs whose cost did
+// not change (least cost below the threshold with both components holding
+// split samples, ties to the first pair in (i, j) order -- the serial scan of
+// ThresholdedMergeNext) while waves 1..3 compute the three cost terms of the
+// changed pairs, one pair per lane; wave 0 then writes the changed costs,
+// takes them into the arg-min and stashes the chosen pair's values; the
+// merge runs on waves 0 and 1 at once.  Each component's product with itself
+// is cached and moves with the component.
 __device__ int perform_merging(Shared& sh, SuffStats& ST, SplitStatsState& SS, const FactoryConfig& cfg) {
   const unsigned long long tMerge = wall_clock64();
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1405,15 +1406,27 @@ __device__ int perform_merging(Shared& sh, SuffStats& ST, SplitStatsState& SS, c
       }
     }
     __syncthreads();
+    // ci, cj: the pair merged in the previous step (-1 before the first);
+    // pairs with either index changed, every other pair kept its cost
+    int ci = -1, cj = -1;
     for (;;) {
+      const int K = V.K;
+      // the changed pair of this lane (lanes 0..31: (ci, q), lanes 32..63:
+      // (cj, q - 32) unless that pair is listed already)
+      int pi = -1, pj = -1;
+      if (ci >= 0) {
+        const int a0 = lane < 32 ? ci : cj, b0 = lane & 31;
+        if (a0 < K && b0 < K && b0 != a0 && !(lane >= 32 && b0 == ci)) { pi = min(a0, b0); pj = max(a0, b0); }
+      }
+      float best = 3.402823466e38f;
+      int bestP = 0x7fffffff;
+      uint32_t hasSplit = 0;
       if (wave == 0) {
-        const int K = V.K;
-        const uint32_t hasSplit = (uint32_t)__ballot(lane < 32 && SS.sn[lane & 31] > 0.f);
+        // arg-min over the unchanged pairs while waves 1..3 compute the
+        // changed pairs' terms; lane l scans pairs l, l + 64, ... in
+        // increasing order (rows 2r and 2r + 1), reading four at a time
+        hasSplit = (uint32_t)__ballot(lane < 32 && SS.sn[lane & 31] > 0.f);
         const float thr = cfg.mergingThreshold;
-        float best = 3.402823466e38f;
-        int bestP = 0x7fffffff;
-        // lane l scans pairs l, l + 64, ... in increasing order (rows 2r and
-        // 2r + 1), reading four at a time
         for (int r0 = 0; r0 < 16 && 2 * r0 < K; r0 += 4) {
           float cs[4];
 #pragma unroll
@@ -1422,46 +1435,48 @@ __device__ int perform_merging(Shared& sh, SuffStats& ST, SplitStatsState& SS, c
           for (int u = 0; u < 4; u++) {
             const int p = lane + 64 * (r0 + u);
             const int i = p >> 5, j = p & 31;
+            const bool changed = i == ci || i == cj || j == ci || j == cj;
             const float cst = cs[u];
-            if (i < j && j < K && ((hasSplit >> i) & (hasSplit >> j) & 1u) && cst < thr && cst < best) {
+            if (i < j && j < K && !changed && ((hasSplit >> i) & (hasSplit >> j) & 1u) && cst < thr && cst < best) {
               best = cst;
               bestP = p;
             }
           }
         }
+      } else if (pi >= 0) {
+        sh.term[lane][wave - 1] = merge_term(V, sh.selfp, pi, pj, wave - 1, merged_lobe(V, pi, pj));
+      }
+      __syncthreads();
+      if (wave == 0) {
+        // the changed pairs' costs into the table and into the arg-min
+        if (pi >= 0) {
+          const float cst = merge_chi(sh.term[lane][0], sh.term[lane][1], sh.term[lane][2], V.w[pi] + V.w[pj]);
+          sh.cost[pi * 32 + pj] = cst;
+          if (((hasSplit >> pi) & (hasSplit >> pj) & 1u) && cst < cfg.mergingThreshold)
+            argmin_take(best, bestP, cst, pi * 32 + pj);
+        }
         const int pm = wave_argmin(best, bestP);
         if (lane == 0) {
           sh.found = pm != 0x7fffffff;
           if (sh.found) {
-            const int ci = pm >> 5, cj = pm & 31;
-            sh.ci = ci;
-            sh.cj = cj;
-            sh.pre[0] = V.w[ci]; sh.pre[1] = V.mux[ci]; sh.pre[2] = V.muy[ci]; sh.pre[3] = V.muz[ci];
-            sh.pre[4] = V.w[cj]; sh.pre[5] = V.mux[cj]; sh.pre[6] = V.muy[cj]; sh.pre[7] = V.muz[cj];
-            sh.pre[8] = V.mc[ci]; sh.pre[9] = V.mc[cj];
+            const int mi = pm >> 5, mj = pm & 31;
+            sh.ci = mi;
+            sh.cj = mj;
+            sh.pre[0] = V.w[mi]; sh.pre[1] = V.mux[mi]; sh.pre[2] = V.muy[mi]; sh.pre[3] = V.muz[mi];
+            sh.pre[4] = V.w[mj]; sh.pre[5] = V.mux[mj]; sh.pre[6] = V.muy[mj]; sh.pre[7] = V.muz[mj];
+            sh.pre[8] = V.mc[mi]; sh.pre[9] = V.mc[mj];
           }
         }
       }
       __syncthreads();
       if (!sh.found) break;
-      {
-        const int ci = sh.ci, cj = sh.cj;
-        if (wave == 0) merge_step_mixture(sh, ST, ci, cj);
-        else if (wave == 1) merge_step_split(sh, SS, ci, cj);
-      }
+      ci = sh.ci;
+      cj = sh.cj;
+      if (wave == 0) merge_step_mixture(sh, ST, ci, cj);
+      else if (wave == 1) merge_step_split(sh, SS, ci, cj);
       __syncthreads();
       total++;
       if (V.K <= sh.W) break;
-      // pair q of the changed rows: lanes 0..31 (ci, q), lanes 32..63 (cj, q - 32)
-      // unless that pair was already listed; wave t computes term t
-      const int ci = sh.ci, cj = sh.cj, Kn = V.K;
-      const int a0 = lane < 32 ? ci : cj, b0 = lane & 31;
-      int i = -1, j = -1;
-      if (a0 < Kn && b0 < Kn && b0 != a0 && !(lane >= 32 && b0 == ci)) { i = min(a0, b0); j = max(a0, b0); }
-      if (wave < 3 && i >= 0) sh.term[lane][wave] = merge_term(V, sh.selfp, i, j, wave, merged_lobe(V, i, j));
-      __syncthreads();
-      if (wave == 0 && i >= 0) sh.cost[i * 32 + j] = merge_chi(sh.term[lane][0], sh.term[lane][1], sh.term[lane][2], V.w[i] + V.w[j]);
-      // the next arg-min runs on wave 0, after these writes of its own
     }
   }
   if (tid == 0) sh.S.prof[5] += (float)(wall_clock64() - tMerge);
