@@ -10,6 +10,7 @@
 // Scalar Init/Sample/PDF calls are served from a host mirror of the device
 // field (refreshed lazily after each update); batched queries run on the GPU.
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <iostream>
@@ -145,7 +146,36 @@ static float dist_pdf_li(const HostDist* d, const float dir[3]) {
   return p;
 }
 
-static void field_update_host(PGLFieldImpl* F, KindField& kf, const std::vector<PGLSampleData>& samples) {
+void* storage_alloc(size_t bytes) {
+  // one header ahead of the samples records how the block was allocated
+  constexpr size_t kHdr = 64;
+  static const bool pinned = [] {
+    int n = 0;
+    const bool ok = hipGetDeviceCount(&n) == hipSuccess && n > 0;
+    (void)hipGetLastError();
+    return ok;
+  }();
+  char* raw = nullptr;
+  uint32_t kind = 0;
+  if (pinned && hipHostMalloc((void**)&raw, bytes + kHdr, hipHostMallocDefault) == hipSuccess) {
+    kind = 1;
+  } else {
+    (void)hipGetLastError();
+    raw = static_cast<char*>(std::malloc(bytes + kHdr));
+    if (!raw) throw std::bad_alloc();
+  }
+  *reinterpret_cast<uint32_t*>(raw) = kind;
+  return raw + kHdr;
+}
+
+void storage_free(void* p) noexcept {
+  if (!p) return;
+  char* raw = static_cast<char*>(p) - 64;
+  if (*reinterpret_cast<uint32_t*>(raw)) (void)hipHostFree(raw);
+  else std::free(raw);
+}
+
+static void field_update_host(PGLFieldImpl* F, KindField& kf, const SampleVec& samples) {
   if (samples.empty()) return;
   if (samples.size() > kMaxUpdateSamples) throw std::runtime_error("openpgl_amd: at most 2^30 samples per update");
   kf.updateHost(samples.data(), (uint32_t)samples.size(), F->gf->stream);
@@ -604,6 +634,10 @@ void pglSampleStorageAddSample(PGLSampleStorage s, PGLSampleData& sample) {  // 
 
 void pglSampleStorageAddSamples(PGLSampleStorage s, const PGLSampleData* samples, size_t n) {  // :82-90
   std::lock_guard<std::mutex> l(s->mutex);
+  size_t nv = 0;
+  for (size_t i = 0; i < n; i++) nv += (samples[i].flags & PGL_SAMPLE_FLAG_INSIDE_VOLUME) ? 1 : 0;
+  storage_grow(s->surface, n - nv);
+  storage_grow(s->volume, nv);
   for (size_t i = 0; i < n; i++) {
     if (samples[i].flags & PGL_SAMPLE_FLAG_INSIDE_VOLUME) s->volume.push_back(samples[i]);
     else s->surface.push_back(samples[i]);

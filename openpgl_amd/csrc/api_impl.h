@@ -1,5 +1,6 @@
 // Handle types and error macros shared by the C ABI translation units.
 #pragma once
+#include <algorithm>
 #include <iostream>
 #include <mutex>
 #include <stdexcept>
@@ -58,8 +59,35 @@ struct PGLFieldImpl {
   pglgpu::PathSampleBatch* paths = nullptr;  // path-segment updates (created on first use)
 };
 
+// Host memory of a sample storage: page-locked when a HIP device is present,
+// so that an update copies it to the device at the full PCIe rate (a
+// pageable source goes through the runtime's staging copies at about half of
+// it); ordinary memory without a device.
+void* storage_alloc(size_t bytes);
+void storage_free(void* p) noexcept;
+template <class T>
+struct StorageAllocator {
+  using value_type = T;
+  StorageAllocator() noexcept = default;
+  template <class U>
+  StorageAllocator(const StorageAllocator<U>&) noexcept {}
+  T* allocate(size_t n) { return static_cast<T*>(storage_alloc(n * sizeof(T))); }
+  void deallocate(T* p, size_t) noexcept { storage_free(p); }
+  template <class U>
+  bool operator==(const StorageAllocator<U>&) const noexcept { return true; }
+  template <class U>
+  bool operator!=(const StorageAllocator<U>&) const noexcept { return false; }
+};
+using SampleVec = std::vector<PGLSampleData, StorageAllocator<PGLSampleData>>;
+
+// Room for n more samples: grows geometrically (a page-locked allocation is
+// expensive, so batches of any size append without reallocating each time).
+inline void storage_grow(SampleVec& v, size_t n) {
+  if (v.size() + n > v.capacity()) v.reserve(std::max(v.size() + n, 2 * v.capacity()));
+}
+
 struct PGLSampleStorageImpl {
-  std::vector<PGLSampleData> surface, volume;
+  SampleVec surface, volume;
   std::mutex mutex;
 };
 
