@@ -941,10 +941,6 @@ __device__ __forceinline__ void log_map(float x, float y, float z, float& ox, fl
   oy = alpha > 0.f ? y * invSinc : 0.f;
 }
 
-__device__ void vmm_normalize_all(VMMParams& V) {
-  for (int k = 0; k < 32; k++) calc_normalization_lane(V, k);
-}
-
 __device__ bool split_component(VMMParams& V, SplitStatsState& SS, SuffStats& ST, int idx, const FactoryConfig& cfg) {
   const float numAssigned = SS.ssa[idx];
   const float invSw = 1.f / SS.ssw[idx];
@@ -989,7 +985,10 @@ __device__ bool split_component(VMMParams& V, SplitStatsState& SS, SuffStats& ST
   V.w[K] = nw; V.mc[K] = nc1; V.kappa[K] = mc_to_kappa(nc1);
   V.mux[K] = m1x; V.muy[K] = m1y; V.muz[K] = m1z;
   V.K = K + 1;
-  vmm_normalize_all(V);
+  // PAVMM:165-190 renormalises every component here; a component's
+  // normalisation depends only on its own kappa and nothing reads it before
+  // the split loop ends, so the caller renormalises all of them once after
+  // the loop (the same values)
   // SufficientStatistics::splitComponentsStats (WEM:520-550)
   float s = ST.sw[idx];
   s /= 2.f;
@@ -1018,6 +1017,27 @@ __device__ void sort_candidates(const SplitStatsState& SS, int* order) {
     while (j >= 0 && SS.chi[order[j]] < c) { order[j + 1] = order[j]; j--; }
     order[j + 1] = v;
   }
+}
+
+// The same order on wave 0: lane k's place is the number of candidates with
+// a larger value plus the earlier ones with an equal value (thread 0 runs the
+// insertion sort instead when a value is NaN).  Callers synchronise after.
+__device__ void rank_candidates(Shared& sh, const SplitStatsState& SS) {
+  const int tid = threadIdx.x;
+  if (tid >= 64) return;
+  const int K = SS.K;
+  const bool mine = tid < K;
+  const float c = mine ? SS.chi[tid] : 0.f;
+  const bool anyNaN = __ballot(mine && c != c) != 0;
+  if (!anyNaN && mine) {
+    int r = 0;
+    for (int j = 0; j < K; j++) {
+      const float cj = SS.chi[j];
+      r += (cj > c || (cj == c && j < tid)) ? 1 : 0;
+    }
+    sh.order[r] = tid;
+  }
+  if (anyNaN && tid == 0) sort_candidates(SS, sh.order);
 }
 
 // maskedReplace (WEM:448-468)
@@ -1079,9 +1099,10 @@ __device__ int recursive_splitting(Shared& sh, Team& tm, const RegionSamples& rs
     clear_split(sh, sh.lsplit, 32);
     clear_split(sh, sh.lsplit, V.K);
     update_split_stats(sh, tm, sh.lsplit, rs, base, n, mcEst);
+    const unsigned long long tS = wall_clock64();
+    rank_candidates(sh, sh.lsplit);
+    __syncthreads();
     if (tid == 0) {
-      const unsigned long long tS = wall_clock64();
-      sort_candidates(sh.lsplit, sh.order);
       sh.mask = 0u;
       int ns = 0;
       const int numComp = V.K;
@@ -1095,9 +1116,13 @@ __device__ int recursive_splitting(Shared& sh, Team& tm, const RegionSamples& rs
         }
       }
       sh.numSplits = ns;
-      sh.S.prof[6] += (float)(wall_clock64() - tS);
     }
     __syncthreads();
+    if (sh.numSplits > 0) {
+      if (tid < 32) calc_normalization_lane(V, tid);
+      __syncthreads();
+    }
+    if (tid == 0) sh.S.prof[6] += (float)(wall_clock64() - tS);
     if (sh.numSplits > 0) {
       clear_suff(sh, sh.tmp, V.K);
       pits += em(sh, tm, sh.tmp, rs, base, n, cfg, true);
@@ -1583,9 +1608,11 @@ __device__ void region_update_body(Shared& sh, Team& tm, const RegionSamples& rs
     unsigned long long afterMerge = get64(S.afterMergeLo, S.afterMergeHi) + n;
     update_split_stats(sh, tm, S.split, rs, base, n, mcEst);
     if (afterSplit >= (unsigned long long)(cfg.minSamplesForSplitting < 0 ? 0 : cfg.minSamplesForSplitting)) {
+      const unsigned long long tS = wall_clock64();
+      const int kBefore = V.K;
+      rank_candidates(sh, S.split);
+      __syncthreads();
       if (tid == 0) {
-        const unsigned long long tS = wall_clock64();
-        sort_candidates(S.split, sh.order);
         sh.mask = 0u;
         int ns = 0;
         const int numCand = S.split.K;
@@ -1599,9 +1626,15 @@ __device__ void region_update_body(Shared& sh, Team& tm, const RegionSamples& rs
           }
         }
         sh.numSplits = ns;
-        S.prof[6] += (float)(wall_clock64() - tS);
       }
       __syncthreads();
+      // the renormalisation of every component that each split does
+      // (PAVMM:165-190), once after the loop: nothing reads it before
+      if (V.K > kBefore) {
+        if (tid < 32) calc_normalization_lane(V, tid);
+        __syncthreads();
+      }
+      if (tid == 0) S.prof[6] += (float)(wall_clock64() - tS);
       splits = sh.numSplits;
       if (splits > 0 && cfg.partialReFit && n >= (uint32_t)(cfg.minSamplesForPartialRefitting < 0 ? 0 : cfg.minSamplesForPartialRefitting)) {
         copy_suff(sh, sh.tmp, S.suff);
