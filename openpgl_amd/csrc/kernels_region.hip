@@ -490,12 +490,11 @@ struct EStepAcc {
 
 // One sample of the E-step, without branches: a sample the mixture does not
 // cover (pdf <= 1e-16) adds zero to the component sums and its weight to the
-// unassigned sums (lane q = 0 of the quad, like the log-likelihood).
+// unassigned sums (lane q = 0 of the quad, like the log-likelihood).  e and
+// pdf come from soft_eval_t.
 template <int JT>
-__device__ __forceinline__ void estep_sample(const CompRegs& c, int q, float dx, float dy, float dz, float w,
-                                             EStepAcc& a) {
-  float e[8];
-  const float pdf = soft_eval_t<JT>(c, dx, dy, dz, e);
+__device__ __forceinline__ void estep_accumulate(int q, float dx, float dy, float dz, float w, const float e[8],
+                                                 float pdf, EStepAcc& a) {
   const bool ok = pdf > 1e-16f;
   const float wu = (!ok && q == 0) ? w : 0.f;
   a.u0 += wu;
@@ -515,13 +514,30 @@ __device__ __forceinline__ void estep_sample(const CompRegs& c, int q, float dx,
   }
 }
 
-// The staged chunk's samples s = quad, quad + 64, ... in that order, two per
-// step so that the evaluation of one overlaps the sums of the other.
+// The staged chunk's samples s = quad, quad + 64, ... in that order.  Two
+// samples per step: both are evaluated before either is accumulated, so the
+// two exp/dot chains overlap; the sums still take the samples one after the
+// other, in sample order.
 template <int JT>
 __device__ __forceinline__ void estep_chunk(const Shared& sh, const CompRegs& c, int q, int quad, uint32_t cnt,
                                             EStepAcc& a) {
-  for (uint32_t s = quad; s < cnt; s += RT / 4)
-    estep_sample<JT>(c, q, sh.stage[0][s], sh.stage[1][s], sh.stage[2][s], sh.stage[3][s], a);
+  uint32_t s = quad;
+  for (; s + RT / 4 < cnt; s += RT / 2) {
+    const uint32_t t = s + RT / 4;
+    const float ax = sh.stage[0][s], ay = sh.stage[1][s], az = sh.stage[2][s], aw = sh.stage[3][s];
+    const float bx = sh.stage[0][t], by = sh.stage[1][t], bz = sh.stage[2][t], bw = sh.stage[3][t];
+    float ea[8], eb[8];
+    const float pa = soft_eval_t<JT>(c, ax, ay, az, ea);
+    const float pb = soft_eval_t<JT>(c, bx, by, bz, eb);
+    estep_accumulate<JT>(q, ax, ay, az, aw, ea, pa, a);
+    estep_accumulate<JT>(q, bx, by, bz, bw, eb, pb, a);
+  }
+  if (s < cnt) {
+    const float ax = sh.stage[0][s], ay = sh.stage[1][s], az = sh.stage[2][s], aw = sh.stage[3][s];
+    float ea[8];
+    const float pa = soft_eval_t<JT>(c, ax, ay, az, ea);
+    estep_accumulate<JT>(q, ax, ay, az, aw, ea, pa, a);
+  }
 }
 
 // E-step (WEM:847-892).  Result: fin[f*32+k] = sum over samples of
