@@ -98,7 +98,7 @@ struct Shared {
   SuffStats tmp;
   SplitStatsState lsplit;
   float red[RWAVES][32][8];
-  float fxx[32], fxy[32], fxz[32], fyx[32], fyy[32], fyz[32];
+  alignas(16) float frame[32][8];  // split statistics: tangent frame of each component (x axis, y axis, 2 unused)
   float cost[32 * 32];
   float selfp[32][6];      // merging: product of each component with itself (direction, kappa, norm, scale)
   float term[64][3];       // merging: the three cost terms of up to 64 pairs
@@ -670,6 +670,57 @@ __device__ void pass_distances(Shared& sh, Team& tm, const RegionSamples& rs, ui
 // zero-mean tangent-plane covariance in each component's frame.  Result:
 // fin[f*32+k] for f = chi, assigned, weight, cov xx, cov yy, cov xy;
 // fin[kScal] = number of samples with pdf > 0.
+// Per-lane split statistics sums of a chunk.
+struct SplitAcc {
+  float* chi;
+  float* sa;
+  float* sw;
+  float* cx;
+  float* cy;
+  float* cz;
+  float& cnt;
+};
+
+// The staged chunk's samples s = quad, quad + 64, ... in that order, for the
+// first JT component groups with no branch between the components (groups in
+// [J, JT) hold zero-weight components, whose sums are not read).  The frame
+// of component k is one 16 B and one 8 B read of sh.frame[k].
+template <int JT>
+__device__ __forceinline__ void split_chunk(const Shared& sh, const CompRegs& c, int q, int quad, uint32_t cnt,
+                                            float inv_mc, SplitAcc& a) {
+  for (uint32_t s = quad; s < cnt; s += RT / 4) {
+    const float dx = sh.stage[0][s], dy = sh.stage[1][s], dz = sh.stage[2][s], w = sh.stage[3][s];
+    const float spdf = sh.stage[4][s];
+    float e[8];
+    const float pdf = soft_eval_t<JT>(c, dx, dy, dz, e);
+    if (pdf > 1e-16f) {
+      a.cnt += 1.f;
+      const float ip = __builtin_amdgcn_rcpf(pdf);
+      const float value = w * spdf;
+      const float av = value * inv_mc * ip;  // value / (mc pdf)
+      // (value^2 vmf / (mc pdf)^2 - 2 vmf value / (mc pdf) + vmf) / spdf = vmf cf
+      const float cf = (av * av - 2.f * av + 1.f) * __builtin_amdgcn_rcpf(spdf);
+      const float wip = w * ip;
+#pragma unroll
+      for (int j = 0; j < JT; j++) {
+        const int k = 4 * j + q;
+        const float4 f0 = *reinterpret_cast<const float4*>(&sh.frame[k][0]);
+        const float2 f1 = *reinterpret_cast<const float2*>(&sh.frame[k][4]);
+        const float vmf = e[j];
+        a.chi[j] += vmf > 0.f ? vmf * cf : 0.f;
+        a.sa[j] = fmaf(vmf, ip, a.sa[j]);
+        const float lx = dot3f(f0.x, f0.y, f0.z, dx, dy, dz);
+        const float ly = dot3f(f0.w, f1.x, f1.y, dx, dy, dz);
+        const float aw = vmf * wip;
+        a.sw[j] += aw;
+        a.cx[j] = fmaf(aw, lx * lx, a.cx[j]);
+        a.cy[j] = fmaf(aw, ly * ly, a.cy[j]);
+        a.cz[j] = fmaf(aw, lx * ly, a.cz[j]);
+      }
+    }
+  }
+}
+
 __device__ void pass_split(Shared& sh, Team& tm, const RegionSamples& rs, uint32_t base, uint32_t n, float mcEst) {
   const unsigned long long tPass = wall_clock64();
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, q = tid & 3, quad = tid >> 2;
@@ -692,37 +743,8 @@ __device__ void pass_split(Shared& sh, Team& tm, const RegionSamples& rs, uint32
 #pragma unroll
     for (int j = 0; j < 8; j++) achi[j] = asa[j] = asw[j] = acx[j] = acy[j] = acz[j] = 0.f;
     float cnt = 0.f;
-    for (uint32_t s = quad; s < ce - cb; s += RT / 4) {
-      const float dx = sh.stage[0][s], dy = sh.stage[1][s], dz = sh.stage[2][s], w = sh.stage[3][s];
-      const float spdf = sh.stage[4][s];
-      float e[8];
-      const float pdf = soft_eval(c, J, dx, dy, dz, e);
-      if (pdf > 1e-16f) {
-        cnt += 1.f;
-        const float ip = __builtin_amdgcn_rcpf(pdf);
-        const float value = w * spdf;
-        const float a = value * inv_mc * ip;  // value / (mc pdf)
-        // (value^2 vmf / (mc pdf)^2 - 2 vmf value / (mc pdf) + vmf) / spdf = vmf cf
-        const float cf = (a * a - 2.f * a + 1.f) * __builtin_amdgcn_rcpf(spdf);
-        const float wip = w * ip;
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-          if (j < J) {
-            const int k = 4 * j + q;
-            const float vmf = e[j];
-            achi[j] += vmf > 0.f ? vmf * cf : 0.f;
-            asa[j] = fmaf(vmf, ip, asa[j]);
-            const float lx = dot3f(sh.fxx[k], sh.fxy[k], sh.fxz[k], dx, dy, dz);
-            const float ly = dot3f(sh.fyx[k], sh.fyy[k], sh.fyz[k], dx, dy, dz);
-            const float aw = vmf * wip;
-            asw[j] += aw;
-            acx[j] = fmaf(aw, lx * lx, acx[j]);
-            acy[j] = fmaf(aw, ly * ly, acy[j]);
-            acz[j] = fmaf(aw, lx * ly, acz[j]);
-          }
-        }
-      }
-    }
+    SplitAcc a{achi, asa, asw, acx, acy, acz, cnt};
+    with_groups(J, [&](auto jt) { split_chunk<decltype(jt)::value>(sh, c, q, quad, ce - cb, inv_mc, a); });
 #pragma unroll
     for (int j = 0; j < 8; j++) {
       const float v0 = quad_group_sum(achi[j]);
@@ -764,8 +786,11 @@ __device__ PGL_REGION_CALL void update_split_stats(Shared& sh, Team& tm, SplitSt
     const int k = tid;
     float xx, xy, xz, yx, yy, yz;
     frame_of(V.mux[k], V.muy[k], V.muz[k], xx, xy, xz, yx, yy, yz);
-    sh.fxx[k] = xx; sh.fxy[k] = xy; sh.fxz[k] = xz;
-    sh.fyx[k] = yx; sh.fyy[k] = yy; sh.fyz[k] = yz;
+    // components k >= K have zero weight: a zero frame keeps their unread sums finite
+    const bool a = k < V.K;
+    float* f = sh.frame[k];
+    f[0] = a ? xx : 0.f; f[1] = a ? xy : 0.f; f[2] = a ? xz : 0.f;
+    f[3] = a ? yx : 0.f; f[4] = a ? yy : 0.f; f[5] = a ? yz : 0.f;
   }
   __syncthreads();
   pass_split(sh, tm, rs, base, n, mcEst);
