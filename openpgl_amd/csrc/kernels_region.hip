@@ -681,41 +681,63 @@ struct SplitAcc {
   float& cnt;
 };
 
+// Split statistics of one sample against component group j (frame f0, f1):
+// vmf = e[j], ip = 1 / pdf, cf = chi-square factor, wip = w / pdf.
+__device__ __forceinline__ void split_add(SplitAcc& a, int j, float4 f0, float2 f1, float dx, float dy, float dz,
+                                          float vmf, float ip, float cf, float wip) {
+  a.chi[j] += vmf > 0.f ? vmf * cf : 0.f;
+  a.sa[j] = fmaf(vmf, ip, a.sa[j]);
+  const float lx = dot3f(f0.x, f0.y, f0.z, dx, dy, dz);
+  const float ly = dot3f(f0.w, f1.x, f1.y, dx, dy, dz);
+  const float aw = vmf * wip;
+  a.sw[j] += aw;
+  a.cx[j] = fmaf(aw, lx * lx, a.cx[j]);
+  a.cy[j] = fmaf(aw, ly * ly, a.cy[j]);
+  a.cz[j] = fmaf(aw, lx * ly, a.cz[j]);
+}
+
+// Per-sample factors of the split statistics.
+struct SplitFactors {
+  float ip, cf, wip;
+};
+__device__ __forceinline__ SplitFactors split_factors(float pdf, float w, float spdf, float inv_mc) {
+  const float ip = __builtin_amdgcn_rcpf(pdf);
+  const float value = w * spdf;
+  const float av = value * inv_mc * ip;  // value / (mc pdf)
+  // (value^2 vmf / (mc pdf)^2 - 2 vmf value / (mc pdf) + vmf) / spdf = vmf cf
+  const float cf = (av * av - 2.f * av + 1.f) * __builtin_amdgcn_rcpf(spdf);
+  const float wip = w * ip;
+  return {ip, cf, wip};
+}
+
 // The staged chunk's samples s = quad, quad + 64, ... in that order, for the
 // first JT component groups with no branch between the components (groups in
 // [J, JT) hold zero-weight components, whose sums are not read).  The frame
-// of component k is one 16 B and one 8 B read of sh.frame[k].
+// of component k is one 16 B and one 8 B read of sh.frame[k]; the next
+// group's frame is read while the current group's sums are formed.
 template <int JT>
 __device__ __forceinline__ void split_chunk(const Shared& sh, const CompRegs& c, int q, int quad, uint32_t cnt,
                                             float inv_mc, SplitAcc& a) {
   for (uint32_t s = quad; s < cnt; s += RT / 4) {
-    const float dx = sh.stage[0][s], dy = sh.stage[1][s], dz = sh.stage[2][s], w = sh.stage[3][s];
-    const float spdf = sh.stage[4][s];
+    const float dx = sh.stage[0][s], dy = sh.stage[1][s], dz = sh.stage[2][s];
     float e[8];
     const float pdf = soft_eval_t<JT>(c, dx, dy, dz, e);
     if (pdf > 1e-16f) {
+      const SplitFactors F = split_factors(pdf, sh.stage[3][s], sh.stage[4][s], inv_mc);
       a.cnt += 1.f;
-      const float ip = __builtin_amdgcn_rcpf(pdf);
-      const float value = w * spdf;
-      const float av = value * inv_mc * ip;  // value / (mc pdf)
-      // (value^2 vmf / (mc pdf)^2 - 2 vmf value / (mc pdf) + vmf) / spdf = vmf cf
-      const float cf = (av * av - 2.f * av + 1.f) * __builtin_amdgcn_rcpf(spdf);
-      const float wip = w * ip;
+      float4 f0 = *reinterpret_cast<const float4*>(&sh.frame[q][0]);
+      float2 f1 = *reinterpret_cast<const float2*>(&sh.frame[q][4]);
 #pragma unroll
       for (int j = 0; j < JT; j++) {
-        const int k = 4 * j + q;
-        const float4 f0 = *reinterpret_cast<const float4*>(&sh.frame[k][0]);
-        const float2 f1 = *reinterpret_cast<const float2*>(&sh.frame[k][4]);
-        const float vmf = e[j];
-        a.chi[j] += vmf > 0.f ? vmf * cf : 0.f;
-        a.sa[j] = fmaf(vmf, ip, a.sa[j]);
-        const float lx = dot3f(f0.x, f0.y, f0.z, dx, dy, dz);
-        const float ly = dot3f(f0.w, f1.x, f1.y, dx, dy, dz);
-        const float aw = vmf * wip;
-        a.sw[j] += aw;
-        a.cx[j] = fmaf(aw, lx * lx, a.cx[j]);
-        a.cy[j] = fmaf(aw, ly * ly, a.cy[j]);
-        a.cz[j] = fmaf(aw, lx * ly, a.cz[j]);
+        float4 n0 = f0;
+        float2 n1 = f1;
+        if (j + 1 < JT) {
+          n0 = *reinterpret_cast<const float4*>(&sh.frame[4 * (j + 1) + q][0]);
+          n1 = *reinterpret_cast<const float2*>(&sh.frame[4 * (j + 1) + q][4]);
+        }
+        split_add(a, j, f0, f1, dx, dy, dz, e[j], F.ip, F.cf, F.wip);
+        f0 = n0;
+        f1 = n1;
       }
     }
   }
