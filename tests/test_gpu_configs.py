@@ -1,6 +1,6 @@
 """Parity of the HIP path with the oracle at each benchmark workload's own
 settings (default field arguments: maxSamplesPerLeaf 32000, K up to 32,
-split-and-merge, deterministic; SURVEY workload generator):
+split-and-merge, deterministic; the blob generator of openpgl_amd/synth.py):
 
   c1  200k surface samples, one Update;
   c2  8 successive incremental Updates of 1M samples, blob centres drifting
