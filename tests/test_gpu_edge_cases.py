@@ -8,7 +8,9 @@
     single sample keeps every region invalid in the oracle, and so must the
     GPU;
   * identical directions: every sample of a blob shares one direction, so
-    the fitted mean cosines reach their clamp.
+    the fitted mean cosines reach their clamp;
+  * zero weights: 30 % of the samples (the fit holds), or all of them (the
+    oracle's mixtures become NaN and every region invalid: so must the GPU's).
 
 Bars as in test_gpu_field.py: kd tree, region sample statistics and leaf ids
 bit-exact; mixtures three-way against the FP32 and FP64 oracles
@@ -100,3 +102,15 @@ def test_identical_directions():
     dev, gpu, o32, o64 = _train(frames, 2000)
     R, V = _check(gpu, o32, o64, generate(2000, seed=41, frame=2)["pos"], "identical directions")
     assert V > 0
+
+
+@pytest.mark.parametrize("frac,valid", [(0.3, True), (1.0, False)])
+def test_zero_weights(frac, valid):
+    frames = []
+    for f in range(2):
+        s = generate(30000, seed=50, frame=f)
+        s["weight"][: int(len(s) * frac)] = 0.0
+        frames.append(s)
+    dev, gpu, o32, o64 = _train(frames, 2000)
+    R, V = _check(gpu, o32, o64, generate(2000, seed=51, frame=2)["pos"], "zero weights %.1f" % frac)
+    assert (V > 0) == valid
