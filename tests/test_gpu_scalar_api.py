@@ -16,7 +16,9 @@ file, so both sides hold the same mixtures, and compared call by call:
     scalar PDF of the batch's sampled direction equals the batch's pdf
     within 1e-3, on the loaded field and on a GPU-trained field;
   * two cosine products in a row compound, as the reference's in-place
-    product does.
+    product does;
+  * edge cases: positions on and just outside the scene bounds (Init false
+    outside), u < 0 (no KNN draw) and u just below 1, HG g at -0.9, 0, 0.9.
 
 Sampling (PAVMM::sample, ParallaxAwareVonMisesFisherMixture.h:737-806):
 400k GPU samples of one query's distribution are binned on the sphere
@@ -114,6 +116,65 @@ def test_scalar_api_matches_oracle_and_batch(trained, knn, volume):
     print("knn", knn, "volume", volume, "valid", valid, "worst rel vs oracle", worst_orc, "vs batch", worst_batch,
           flush=True)
     assert valid > 0.8 * n
+
+
+@pytest.mark.parametrize("volume", [0, 1])
+def test_query_edge_cases(trained, volume):
+    """Init on and outside the scene bounds (false outside, Field.h:109-145),
+    u < 0 (KNN draw disabled), u just below 1, HG g at -0.9, 0 and 0.9:
+    Init's result, the KNN-updated u and the region id exact against the
+    oracle and between the scalar and batched paths; pdfs within 1e-3."""
+    served, orc_s, orc_v, _ = trained[True]
+    orc = orc_v if volume else orc_s
+    s2c = canonical_ids(orc)
+    lo, hi = (np.float32(v) for v in served.scene_bounds())
+    lo, hi = np.array(lo, np.float32), np.array(hi, np.float32)
+    mid = (lo + hi) * np.float32(0.5)
+    pos = [mid, lo, hi, lo + (hi - lo) * np.float32(0.999)]
+    for a in range(3):
+        for side in (lo, hi):
+            p = mid.copy()
+            p[a] = side[a] + (np.float32(-1.0) if side is lo else np.float32(1.0))
+            pos.append(p)
+            p = mid.copy()
+            p[a] = np.nextafter(side[a], np.float32(-np.inf) if side is lo else np.float32(np.inf))
+            pos.append(p)
+    qpos, _, _, _, _ = queries(16, seed=130 + volume)
+    pos = np.concatenate([np.array(pos, np.float32), qpos])
+    us = np.array([-1.0, 0.0, 0.5, np.nextafter(np.float32(1.0), np.float32(0.0))], np.float32)
+    gs = np.array([-0.9, 0.0, 0.9] if volume else [0.0], np.float32)
+    rng = np.random.default_rng(140 + volume)
+    cases = [(p, u, g) for p in pos for u in us for g in gs]
+    n = len(cases)
+    P = np.array([c[0] for c in cases], np.float32)
+    U1 = np.array([c[1] for c in cases], np.float32)
+    G = np.array([c[2] for c in cases], np.float32)
+    N = rng.normal(size=(n, 3))
+    N = (N / np.linalg.norm(N, axis=1, keepdims=True)).astype(np.float32)
+    U2 = rng.random((n, 2)).astype(np.float32)
+    ev = np.array([0.0, 0.6, 0.8], np.float32)
+    bd, bpdf, bids, _ = served.sample_batch(P, N, U2, u1=U1, g=G, is_volume=np.full(n, volume, np.uint8))
+    dist = served.new_volume_distribution() if volume else served.new_surface_distribution()
+    outside = valid = 0
+    for i in range(n):
+        ok, u_after = dist.init(P[i], float(U1[i]))
+        o_ok, o_u, o_id, _, _, o_pdf_ev, _, _ = orc.scalar_sequence(volume, P[i], float(U1[i]), N[i], float(G[i]), 1,
+                                                                    U2[i], ev)
+        assert ok == o_ok and (bpdf[i] > 0) == ok, (i, P[i], U1[i], ok, o_ok, bpdf[i])
+        if not ok:
+            outside += 1
+            continue
+        valid += 1
+        assert np.float32(u_after) == np.float32(o_u), (i, u_after, o_u)
+        assert dist.id() == s2c[o_id] and bids[i] == dist.id(), (i, dist.id(), bids[i], o_id)
+        if volume:
+            dist.apply_hg_product(N[i], float(G[i]))
+        else:
+            dist.apply_cosine_product(N[i])
+        assert _rel(dist.pdf(ev), o_pdf_ev) <= 1e-3, i
+        assert _rel(dist.pdf(bd[i]), float(bpdf[i])) <= 1e-3, i
+    print("volume", volume, "cases", n, "valid", valid, "outside", outside, flush=True)
+    assert outside >= 6 * len(us) * len(gs) and valid > n // 2
 
 
 def test_repeated_cosine_product_compounds(trained):
