@@ -8,6 +8,8 @@
   than the budget is rejected.
 * The server serves its snapshot while the field is updated, and the updated
   field after a refresh.
+* Requests of 1 to 65 queries (less than one wave up to just over it) equal
+  the batch API; an empty request is rejected.
 """
 import threading
 
@@ -141,4 +143,24 @@ def test_server_snapshot_and_refresh():
     fresh = _run_alone(server, reqs)
     for a, b in zip(fresh, after):
         assert _same(a, b)
+    server.close()
+
+
+def test_server_small_and_empty_requests():
+    import openpgl_amd as pgl
+    dev = pgl.Device()
+    f = _field(dev)
+    server = pgl.Server(f, slots=2)
+    reqs = []
+    for i, n in enumerate([1, 2, 3, 63, 64, 65]):
+        pos, nrm, u2, is_vol, g = queries(n, seed=200 + i)
+        u1 = np.random.default_rng(i).random(n).astype(np.float32) if i % 2 else None
+        reqs.append((pos, nrm, u2, u1, g, is_vol.astype(np.uint8)))
+    outs = _run_alone(server, reqs) + _run_mixed(server, reqs)
+    for i, r in enumerate(reqs):
+        ref = f.sample_batch(r[0], r[1], r[2], u1=r[3], g=r[4], is_volume=r[5])
+        assert _same(outs[i], ref) and _same(outs[len(reqs) + i], ref), i
+    with pytest.raises(RuntimeError):
+        e3 = np.zeros((0, 3), np.float32)
+        server.submit(e3, e3, np.zeros((0, 2), np.float32))
     server.close()
