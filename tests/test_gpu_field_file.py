@@ -11,7 +11,10 @@
     batched queries bit-identically, and training continued from the loaded
     field gives bit-identical results to training continued in memory;
   * a field file written from the oracle's CPU-trained state loads on the
-    GPU and serves queries that match the oracle's (ids exact, pdf 1e-3).
+    GPU and serves queries that match the oracle's (ids exact, pdf 1e-3);
+  * a field whose every region is invalid (NaN mixtures after a first update
+    of a single sample) stores, loads and stores again byte-identically and
+    answers queries as before.
 """
 import numpy as np
 import pytest
@@ -220,3 +223,25 @@ def test_oracle_trained_field_served_on_gpu(tmp_path):
     rel = np.abs(gp[ok] - po) / po
     print("oracle field served: median rel", np.median(rel), "max", rel.max(), flush=True)
     assert np.all(rel <= 1e-3), np.percentile(rel, [50, 95, 99, 100])
+
+
+def test_invalid_regions_round_trip(tmp_path):
+    import openpgl_amd as pgl
+    dev = pgl.Device()
+    f = pgl.Field(dev, max_samples_per_leaf=100, deterministic=True)
+    for i, n in enumerate([1, 2, 7, 2000, 1]):
+        f.update(pgl.SampleStorage(generate(n, seed=30 + n, frame=i)))
+    R = f.num_regions()
+    assert R > 1 and not any(f.region(i)["valid"] for i in range(R))
+    p1 = str(tmp_path / "invalid.pgl")
+    assert f.store(p1)
+    g = pgl.Field.from_file(dev, p1)
+    p2 = str(tmp_path / "invalid2.pgl")
+    assert g.store(p2)
+    assert open(p1, "rb").read() == open(p2, "rb").read()
+    assert g.num_regions() == R and not any(g.region(i)["valid"] for i in range(R))
+    pos, nrm, u2, is_vol, gg = queries(5000, seed=9)
+    ra = f.sample_batch(pos, nrm, u2, g=gg, is_volume=is_vol)
+    rb = g.sample_batch(pos, nrm, u2, g=gg, is_volume=is_vol)
+    for x, y in zip(ra, rb):
+        assert np.array_equal(np.asarray(x).view(np.uint32), np.asarray(y).view(np.uint32))
