@@ -67,23 +67,41 @@ __device__ __forceinline__ float qlane(float v) {
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), L * 0x55, 0xF, 0xF, false));
 }
 
-// Products of this lane's components with the lobe (weight, n, kn, nn):
-// scale 0 for k >= K.
+// This lane's components (k = 4j + q) after the parallax shift to the query
+// position: their directions, formed once per query and shared by every
+// product below.
+struct LaneShift {
+  float mx[8], my[8], mz[8];
+};
+__device__ __forceinline__ void lane_shift(const VMMParams& V, int q, bool par, float sx, float sy, float sz,
+                                           LaneShift& h) {
+  const int K = V.K;
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    const int k = 4 * j + q;
+    h.mx[j] = 0.f; h.my[j] = 0.f; h.mz[j] = 1.f;
+    if (k < K) {
+      float d;
+      shift_comp(V, k, sx, sy, sz, h.mx[j], h.my[j], h.mz[j], d, par);
+    }
+  }
+}
+
+// Products of this lane's shifted components with the lobe (weight, n, kn,
+// nn): scale 0 for k >= K.
 struct LaneComps {
   float s[8], kap[8], mx[8], my[8], mz[8], nrm[8], e2k[8];
 };
-__device__ __forceinline__ void lane_products(const VMMParams& V, int q, bool par, float sx, float sy, float sz,
-                                              float weight, float nx, float ny, float nz, float kn, float nn,
-                                              LaneComps& c) {
+__device__ __forceinline__ void lane_products(const VMMParams& V, int q, const LaneShift& h, float weight, float nx,
+                                              float ny, float nz, float kn, float nn, LaneComps& c) {
   const int K = V.K;
 #pragma unroll
   for (int j = 0; j < 8; j++) {
     const int k = 4 * j + q;
     c.s[j] = 0.f; c.kap[j] = 0.f; c.mx[j] = 0.f; c.my[j] = 0.f; c.mz[j] = 1.f; c.nrm[j] = 0.f; c.e2k[j] = 1.f;
     if (k < K) {
-      float mx, my, mz, d;
-      shift_comp(V, k, sx, sy, sz, mx, my, mz, d, par);
-      const ProdComp p = product_comp(V.w[k], V.kappa[k], mx, my, mz, V.norm[k], weight, nx, ny, nz, kn, nn);
+      const ProdComp p =
+          product_comp(V.w[k], V.kappa[k], h.mx[j], h.my[j], h.mz[j], V.norm[k], weight, nx, ny, nz, kn, nn);
       c.s[j] = p.scale; c.kap[j] = p.kappa; c.mx[j] = p.mx; c.my[j] = p.my; c.mz[j] = p.mz; c.nrm[j] = p.norm;
       c.e2k[j] = p.em2k;
     }
@@ -110,17 +128,19 @@ __device__ __forceinline__ float quad_pdf(const VMMParams& V, int q, const LaneC
 }
 
 // pdf at d of the product with a lobe, the products formed on the fly (quad sum)
-__device__ __forceinline__ float quad_pdf_lobe(const VMMParams& V, int q, bool par, float sx, float sy, float sz,
-                                               float weight, float nx, float ny, float nz, float kn, float nn,
-                                               float integral, float dx, float dy, float dz) {
+__device__ __forceinline__ float quad_pdf_lobe(const VMMParams& V, int q, const LaneShift& h, float weight, float nx,
+                                               float ny, float nz, float kn, float nn, float integral, float dx,
+                                               float dy, float dz) {
   const float inv = 1.f / integral;
   float part = 0.f;
-  for (int j = 0; 4 * j + q < V.K; j++) {
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
     const int k = 4 * j + q;
-    float mx, my, mz, d;
-    shift_comp(V, k, sx, sy, sz, mx, my, mz, d, par);
-    const ProdComp p = product_comp(V.w[k], V.kappa[k], mx, my, mz, V.norm[k], weight, nx, ny, nz, kn, nn);
-    part += eval_comp(p.scale * inv, p.norm, p.kappa, p.mx, p.my, p.mz, dx, dy, dz);
+    if (k < V.K) {
+      const ProdComp p =
+          product_comp(V.w[k], V.kappa[k], h.mx[j], h.my[j], h.mz[j], V.norm[k], weight, nx, ny, nz, kn, nn);
+      part += eval_comp(p.scale * inv, p.norm, p.kappa, p.mx, p.my, p.mz, dx, dy, dz);
+    }
   }
   return qsum4(part);
 }
@@ -204,10 +224,12 @@ __device__ __forceinline__ void query_quad(const QueryField& surf, const QueryFi
   const VMMParams& V = f.regions[r].vmm;
   const bool par = f.parallax != 0;
   const float sx = V.pivot[0] - x, sy = V.pivot[1] - y, sz = V.pivot[2] - z;
+  LaneShift h;
+  lane_shift(V, q, par, sx, sy, sz, h);
   LaneComps c;
   if (!volume) {
     const float kn = kCosineKappa, nn = cosine_normalization();
-    lane_products(V, q, par, sx, sy, sz, 1.f, nx, ny, nz, kn, nn, c);
+    lane_products(V, q, h, 1.f, nx, ny, nz, kn, nn, c);
     integral = quad_integral(c);
     quad_sample(V, q, lane, c, integral, ux, uy, out);
     pdf = quad_pdf(V, q, c, integral, out[0], out[1], out[2]);
@@ -218,8 +240,7 @@ __device__ __forceinline__ void query_quad(const QueryField& surf, const QueryFi
     float sum = 0.f;
     for (int m = 0; m < KP; m++) {
       sgn[m] = (g * rep.mc[m] > 0.f) ? 1.f : -1.f;
-      lane_products(V, q, par, sx, sy, sz, rep.w[m], nx * sgn[m], ny * sgn[m], nz * sgn[m], rep.kappa[m],
-                    rep.norm[m], c);
+      lane_products(V, q, h, rep.w[m], nx * sgn[m], ny * sgn[m], nz * sgn[m], rep.kappa[m], rep.norm[m], c);
       integ[m] = quad_integral(c);
       sum += integ[m];
     }
@@ -232,13 +253,12 @@ __device__ __forceinline__ void query_quad(const QueryField& surf, const QueryFi
       if (nw > ux) break;
       acc = nw;
     }
-    lane_products(V, q, par, sx, sy, sz, rep.w[m], nx * sgn[m], ny * sgn[m], nz * sgn[m], rep.kappa[m], rep.norm[m],
-                  c);
+    lane_products(V, q, h, rep.w[m], nx * sgn[m], ny * sgn[m], nz * sgn[m], rep.kappa[m], rep.norm[m], c);
     quad_sample(V, q, lane, c, integ[m], (ux - acc) / mw[m], uy, out);
     for (int j = 0; j < KP; j++)
       pdf += mw[j] * (j == m ? quad_pdf(V, q, c, integ[j], out[0], out[1], out[2])
-                             : quad_pdf_lobe(V, q, par, sx, sy, sz, rep.w[j], nx * sgn[j], ny * sgn[j], nz * sgn[j],
-                                             rep.kappa[j], rep.norm[j], integ[j], out[0], out[1], out[2]));
+                             : quad_pdf_lobe(V, q, h, rep.w[j], nx * sgn[j], ny * sgn[j], nz * sgn[j], rep.kappa[j],
+                                             rep.norm[j], integ[j], out[0], out[1], out[2]));
     integral = sum;
   }
 }
